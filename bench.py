@@ -1,0 +1,246 @@
+#!/usr/bin/env python3
+"""Benchmark: PSBLAS device SpMV on one MI355X.
+
+Workload (the SpMV format sweep): y = A*x with x ~ U[-1,1) (seed 1) on
+  A: 3D 7-point convection-diffusion, L = 200 (n = 8,000,000, nnz = 55,760,000)
+  B: power-law rows (Zipf tail, mean 16, max 4096), n = 4,000,000
+in CSR / ELL / HLL, FP64 and FP32.  One "step" is one FP64 HLL product on A
+(the headline); ``value`` is its algorithmic bandwidth, bytes = 20 n + 12 nnz
+(test/cudakern/dpdegenmv.F90:934-935), inputs resident in HBM, timed with HIP
+events on the library stream.  The other formats are reported under
+"sweep".  ``cpu_baseline`` times the oracle's OpenMP HLL product on the host
+cores on a bounded sample.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spmv_bytes(n, nnz, single=False):
+    # x read once, y written once, one int per row, value + index per nonzero
+    return (12 * n + 8 * nnz) if single else (20 * n + 12 * nnz)
+
+
+def time_spmv(P, a, xv, yv, steps, warmup):
+    timer = P.DeviceTimer()
+    for _ in range(warmup):
+        P.psb_spmm(1.0, a, xv, 0.0, yv)
+    P.psb_hip_DeviceSync()
+    timer.start()
+    for _ in range(steps):
+        P.psb_spmm(1.0, a, xv, 0.0, yv)
+    ms = timer.stop_ms()
+    return ms / steps
+
+
+def stream_bandwidth(N, nbytes=2 << 30, reps=20):
+    import ctypes as C
+    src, dst = C.c_void_p(), C.c_void_p()
+    N.call("psb_hip_alloc_buffer", C.byref(src), nbytes)
+    N.call("psb_hip_alloc_buffer", C.byref(dst), nbytes)
+    N.call("psb_hip_stream_copy", dst, src, nbytes)
+    N.call("psb_hip_timer_start")
+    for _ in range(reps):
+        N.call("psb_hip_stream_copy", dst, src, nbytes)
+    ms = N.fn("psb_hip_timer_stop")() / reps
+    N.call("psb_hip_free_buffer", src)
+    N.call("psb_hip_free_buffer", dst)
+    return 2 * nbytes / (ms * 1e-3) / 1e9
+
+
+def gen_coo(P, N, kind, size):
+    import ctypes as C
+    import numpy as np
+    if kind == "stencil":
+        a = np.ones(3)
+        b = np.full(3, 1.0 / np.sqrt(3.0))
+        fn = N.fn("psb_hip_gen_pde3d")
+        nz = fn(size, a.ctypes.data, b.ctypes.data, 0.0, 1, 0, None, None, None, None)
+        ia = np.empty(nz, np.int32); ja = np.empty(nz, np.int32); val = np.empty(nz)
+        fn(size, a.ctypes.data, b.ctypes.data, 0.0, 1, 0, ia.ctypes.data, ja.ctypes.data,
+           val.ctypes.data, None)
+        n = size ** 3
+    else:
+        fn = N.fn("psb_hip_gen_powerlaw")
+        nz = fn(size, 4095, 1, None, None, None)
+        ia = np.empty(nz, np.int32); ja = np.empty(nz, np.int32); val = np.empty(nz)
+        fn(size, 4095, 1, ia.ctypes.data, ja.ctypes.data, val.ctypes.data)
+        n = size
+    coo = P.psb_d_coo_sparse_mat(n, n, ia, ja, val)
+    coo.fix()
+    del C
+    return coo
+
+
+def cpu_baseline(seconds=12.0, L=128):
+    """Oracle HLL (hksz 32) FP64 product, OpenMP over the host cores."""
+    from oracle import oracle as O
+    import numpy as np
+    t0 = time.perf_counter()
+    coo = O.fix_coo(O.gen_convdiff(L))
+    h = O.hll_from_coo(coo, 32)
+    x = O.uniform_pm1(1, coo.nc)
+    log(f"cpu_baseline: oracle setup {time.perf_counter() - t0:.1f}s, {O.num_threads()} threads")
+    y = O.hll_spmv(h, x)
+    reps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        y = O.hll_spmv(h, x)
+        reps += 1
+    dt = (time.perf_counter() - t0) / reps
+    gbs = spmv_bytes(coo.nr, coo.nz) / dt / 1e9
+    del y, np
+    return {"value": round(gbs, 3), "unit": "GB/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"oracle HLL(hksz=32) FP64 SpMV, 7-point stencil L={L} "
+                      f"(n={coo.nr}, nnz={coo.nz}), {reps} products in {reps * dt:.1f}s, "
+                      f"ms/product={dt * 1e3:.2f}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--L", type=int, default=200)
+    ap.add_argument("--powerlaw-n", type=int, default=4_000_000)
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(the multi-GPU sharding is described in DESIGN.md)"}))
+        sys.exit(2)
+
+    import numpy as np
+    import psblas3_amd as P
+    from psblas3_amd import _native as N
+
+    P.psb_init(0)
+    dev = P.psb_hip_DeviceName()
+    log(f"bench: device {dev}, {P.psb_hip_MultiProcessors()} CUs")
+    stream = stream_bandwidth(N)
+    log(f"bench: stream copy {stream:.0f} GB/s")
+
+    t0 = time.perf_counter()
+    coo_a = gen_coo(P, N, "stencil", args.L)
+    n, nnz = coo_a.nr, coo_a.get_nzeros()
+    log(f"bench: stencil L={args.L} n={n} nnz={nnz} assembled in {time.perf_counter() - t0:.1f}s")
+
+    x = P.psb_d_hip_vect().all(n)
+    N.call("psb_hip_mv_fill_uniform", x.dev, 1, 0)
+    x.set_dev()
+    y = P.psb_d_hip_vect().all(n)
+
+    # headline: FP64 HLL, hack size 64 (one wavefront per hack)
+    head = P.psb_dspmat_type()
+    head.m = head.n = n
+    head.a = P.psb_d_hip_hlg_sparse_mat(hksz=64).cp_from_coo(coo_a)
+    ms = time_spmv(P, head, x, y, args.steps, args.warmup)
+    nbytes = spmv_bytes(n, nnz)
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    gflops = 2 * nnz / (ms * 1e-3) / 1e9
+    log(f"bench: HLL64 FP64 {ms * 1e3:.1f} us/product  {gbs:.0f} GB/s  {gflops:.1f} GFLOP/s")
+
+    sweep = {}
+    if not args.no_sweep:
+        xs = P.psb_s_hip_vect().all(n)
+        xs.bld(x.get_vect().astype(np.float32))
+        ys = P.psb_s_hip_vect().all(n)
+        variants = [("A", "HLL64", "f64", lambda: P.psb_d_hip_hlg_sparse_mat(64)),
+                    ("A", "HLL32", "f64", lambda: P.psb_d_hip_hlg_sparse_mat(32)),
+                    ("A", "ELL", "f64", lambda: P.psb_d_hip_elg_sparse_mat()),
+                    ("A", "CSR", "f64", lambda: P.psb_d_hip_csrg_sparse_mat()),
+                    ("A", "HLL64", "f32", lambda: P.psb_s_hip_hlg_sparse_mat(64)),
+                    ("A", "ELL", "f32", lambda: P.psb_s_hip_elg_sparse_mat()),
+                    ("A", "CSR", "f32", lambda: P.psb_s_hip_csrg_sparse_mat())]
+        for mat, fmt, dt, mk in variants:
+            if (fmt, dt) == ("HLL64", "f64"):
+                sweep[f"{mat}_{fmt}_{dt}"] = {"us": round(ms * 1e3, 2), "GBs": round(gbs, 1),
+                                              "GFLOPs": round(gflops, 2)}
+                continue
+            m = P.psb_dspmat_type()
+            m.m = m.n = n
+            m.a = mk().cp_from_coo(coo_a)
+            single = dt == "f32"
+            t = time_spmv(P, m, xs if single else x, ys if single else y, max(args.steps // 2, 10), 3)
+            b = spmv_bytes(n, nnz, single)
+            sweep[f"{mat}_{fmt}_{dt}"] = {"us": round(t * 1e3, 2), "GBs": round(b / (t * 1e-3) / 1e9, 1),
+                                          "GFLOPs": round(2 * nnz / (t * 1e-3) / 1e9, 2)}
+            log(f"bench: {mat} {fmt} {dt} {t * 1e3:.1f} us  {sweep[f'{mat}_{fmt}_{dt}']['GBs']} GB/s")
+            m.free()
+        del xs, ys
+        # matrix B: power-law rows
+        t0 = time.perf_counter()
+        coo_b = gen_coo(P, N, "powerlaw", args.powerlaw_n)
+        nb, nnzb = coo_b.nr, coo_b.get_nzeros()
+        log(f"bench: powerlaw n={nb} nnz={nnzb} assembled in {time.perf_counter() - t0:.1f}s")
+        xb = P.psb_d_hip_vect().all(nb)
+        N.call("psb_hip_mv_fill_uniform", xb.dev, 1, 0)
+        xb.set_dev()
+        yb = P.psb_d_hip_vect().all(nb)
+        xbs = P.psb_s_hip_vect().bld(xb.get_vect().astype(np.float32))
+        ybs = P.psb_s_hip_vect().all(nb)
+        for fmt, dt, mk in [("CSR", "f64", lambda: P.psb_d_hip_csrg_sparse_mat()),
+                            ("CSR", "f32", lambda: P.psb_s_hip_csrg_sparse_mat()),
+                            ("HLL64", "f64", lambda: P.psb_d_hip_hlg_sparse_mat(64)),
+                            ("HLL64", "f32", lambda: P.psb_s_hip_hlg_sparse_mat(64))]:
+            m = P.psb_dspmat_type()
+            m.m = m.n = nb
+            m.a = mk().cp_from_coo(coo_b)
+            single = dt == "f32"
+            t = time_spmv(P, m, xbs if single else xb, ybs if single else yb, max(args.steps // 4, 5), 2)
+            b = spmv_bytes(nb, nnzb, single)
+            ent = {"us": round(t * 1e3, 2), "GBs": round(b / (t * 1e-3) / 1e9, 1),
+                   "GFLOPs": round(2 * nnzb / (t * 1e-3) / 1e9, 2)}
+            if fmt.startswith("HLL"):
+                ent["padding"] = round(m.a.isz / nnzb, 2)
+            sweep[f"B_{fmt}_{dt}"] = ent
+            log(f"bench: B {fmt} {dt} {t * 1e3:.1f} us  {ent['GBs']} GB/s")
+            m.free()
+        sweep["B_ELL"] = "not representable (n x max_row_len slots)"
+        sweep["B_n"], sweep["B_nnz"] = nb, nnzb
+
+    cpu = None if args.no_cpu else cpu_baseline()
+    if cpu:
+        log(f"bench: cpu baseline {cpu['value']} GB/s on {cpu['cores']} threads")
+
+    line = {
+        "metric": "SpMV GB/s & %HBM peak; solves/s + p99 latency at 1e-10 resid, mixed vs FP64",
+        "value": round(gbs, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (generated stencil and power-law matrices, x ~ U[-1,1) seed 1)",
+        "config": {"workload": f"HLL FP64 SpMV, 3D 7-point convection-diffusion L={args.L}",
+                   "n": n, "nnz": nnz, "hksz": 64, "bytes_per_product": nbytes,
+                   "gflops": round(gflops, 2), "device": dev},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "stream_copy_GBs": round(stream, 1),
+                     "frac_of_stream": round(gbs / stream, 4)},
+        "cpu_baseline": cpu,
+        "sweep": sweep,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,317 @@
+/*
+ * psb_hip.h -- C ABI of libpsb_hip, the MI355X (gfx950) device layer behind
+ * PSBLAS's psb_[sd]_hip_{hlg,elg,csrg}_mat and psb_[sd]_hip_vect plug-in
+ * types.  Plain C: pointers, sizes and opaque handles; no C++ or torch types.
+ *
+ * Every entry point names the reference interface it replaces.  Semantics
+ * follow the reference CPU path where the CPU and the reference CUDA layer
+ * disagree (HLL padding, xyzw, axybz with beta = 0).
+ *
+ * Conventions (as the reference's device layer):
+ *   - indices are int32; matrix index arrays are 1-based (firstIndex = 1,
+ *     cuda/impl/psb_d_cuda_hlg_to_gpu.F90:56);
+ *   - every call is asynchronous on the library stream unless it returns a
+ *     host scalar or copies to host memory (those synchronise);
+ *   - return value 0 = success, else one of PSB_HIP_* below
+ *     (cuda/spgpu/core.h:69-72 codes 0/1/3 keep their meaning).
+ *   - device handles (void*) are owned by the caller and released with the
+ *     matching free function (cuda/psb_d_cuda_hlg_mat_mod.F90:288-301).
+ */
+#ifndef PSB_HIP_H
+#define PSB_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes (cuda/spgpu/core.h:69-72) */
+#define PSB_HIP_SUCCESS      0
+#define PSB_HIP_UNSUPPORTED  1
+#define PSB_HIP_INVALID      2
+#define PSB_HIP_OUTOFMEM     3
+#define PSB_HIP_RUNTIME      4
+#define PSB_HIP_NOT_INIT     5
+
+/* element types (cuda/spgpu/core.h:77-79) */
+#define PSB_HIP_TYPE_INT     0
+#define PSB_HIP_TYPE_FLOAT   1
+#define PSB_HIP_TYPE_DOUBLE  2
+
+/* geins duplicate handling (cuda/vectordev.h:55-56) */
+#define PSB_HIP_INS_OVERWRITE 0
+#define PSB_HIP_INS_ADD       1
+
+/* ===================================================================== */
+/* Environment -- cuda/psb_cuda_env_mod.F90:43-174, cuda/cuda_util.h:54-69 */
+/* ===================================================================== */
+
+/* gpuInit(dev) + psb_cudaCreateHandle: select the device, create the library
+   stream and the reduction workspace.  Idempotent. */
+int psb_hip_init(int dev);
+/* gpuClose + psb_cudaDestroyHandle */
+int psb_hip_exit(void);
+/* getDeviceCount / getDevice / setDevice */
+int psb_hip_get_device_count(void);
+int psb_hip_get_device(void);
+int psb_hip_set_device(int dev);
+/* getGPUMultiProcessors, getGPUMemoryBusWidth, getGPUMemoryClockRate (kHz),
+   getGPUWarpSize, getGPUMaxThreadsPerMP, getGPUMaxRegistersPerBlock,
+   DeviceHasUVA */
+int psb_hip_get_multiprocessors(void);
+int psb_hip_get_memory_bus_width(void);
+int psb_hip_get_memory_clock_rate(void);
+int psb_hip_get_warp_size(void);
+int psb_hip_get_max_threads_per_mp(void);
+int psb_hip_get_max_registers_per_block(void);
+int psb_hip_device_has_uva(void);
+/* cpyGPUNameString */
+int psb_hip_get_device_name(char *buf, int len);
+/* psb_cuda_MemoryPeakBandwidth (cuda/psb_cuda_env_mod.F90:294-300):
+   2 * memClock(kHz) * busWidth(bit) / 8 / 1000, in MB/s */
+double psb_hip_memory_peak_bandwidth(void);
+/* cudaSync (cuda/cuda_util.h:69): wait for all work on the library stream */
+int psb_hip_device_sync(void);
+/* psb_cudaGetStream: the library stream as an opaque hipStream_t */
+void *psb_hip_get_stream(void);
+/* Last error text (for the host mirror's error messages). */
+const char *psb_hip_last_error(void);
+
+/* Event timing on the library stream (no reference counterpart; the
+   reference times with MPI_Wtime around psb_cuda_DeviceSync). */
+int psb_hip_timer_start(void);
+/* milliseconds since psb_hip_timer_start, measured with HIP events on the
+   library stream (synchronises) */
+double psb_hip_timer_stop(void);
+
+/* Device-to-device copy of nbytes (multiple of 16) with a streaming kernel
+   on the library stream; used to measure achievable HBM bandwidth for the
+   roofline denominator (no reference counterpart: the reference only
+   reports the spec figure of psb_cuda_MemoryPeakBandwidth). */
+int psb_hip_stream_copy(void *dst, const void *src, int64_t nbytes);
+
+/* ===================================================================== */
+/* Raw device buffers -- cuda/psb_d_vectordev_mod.F90:85-160               */
+/* (allocateDouble/writeDouble/readDouble/freeDouble and Float/Int)        */
+/* ===================================================================== */
+int psb_hip_alloc_buffer(void **dptr, int64_t nbytes);
+int psb_hip_free_buffer(void *dptr);
+int psb_hip_write_buffer(void *dptr, const void *host, int64_t nbytes);
+int psb_hip_read_buffer(void *host, const void *dptr, int64_t nbytes);
+
+/* ===================================================================== */
+/* Multivectors -- cuda/vectordev.h:76-87, cuda/dvectordev.c:45-336        */
+/* ===================================================================== */
+
+/* FallocMultiVecDevice (cuda/vectordev.c:83-145): count columns of size
+   entries; pitch = size if count == 1, else rounded up to 256 bytes. */
+int psb_hip_mv_alloc(void **mv, int count, int size, int elem_type);
+/* freeMultiVecDevice */
+void psb_hip_mv_free(void *mv);
+/* getMultiVecDeviceSize / Count / Pitch */
+int psb_hip_mv_size(void *mv);
+int psb_hip_mv_count(void *mv);
+int psb_hip_mv_pitch(void *mv);
+/* raw device pointer of column 0 (for interop and tests) */
+void *psb_hip_mv_data(void *mv);
+/* writeMultiVecDevice{Double,Float,Int} / readMultiVecDevice*: whole
+   first column (count == 1) or count columns of size entries, host
+   leading dimension = size. */
+int psb_hip_mv_write(void *mv, const void *host);
+int psb_hip_mv_read(void *mv, void *host);
+/* writeMultiVecDeviceDoubleR2 / readMultiVecDeviceDoubleR2: host leading
+   dimension ld */
+int psb_hip_mv_write_r2(void *mv, const void *host, int ld);
+int psb_hip_mv_read_r2(void *mv, void *host, int ld);
+/* writeDoubleFirst / readDoubleFirst: n entries starting at 1-based
+   position first of column 0 */
+int psb_hip_mv_write_first(void *mv, int first, int n, const void *host);
+int psb_hip_mv_read_first(void *mv, int first, int n, void *host);
+
+/* setscalMultiVecDevice{Double,Float,Int} (cuda/dvectordev.c:85-96):
+   x(first:last) = val, first/last relative to index_base */
+int psb_hip_mv_setscal_d(double val, int first, int last, int index_base, void *x);
+int psb_hip_mv_setscal_s(float val, int first, int last, int index_base, void *x);
+int psb_hip_mv_setscal_i(int val, int first, int last, int index_base, void *x);
+
+/* geinsMultiVecDeviceDouble (cuda/dvectordev.c:99-125): x(irl(i)) =
+   val(i) (dupl = OVERWRITE) or x(irl(i)) += val(i) (dupl = ADD), n entries,
+   irl an int multivector, val a multivector of x's type. */
+int psb_hip_mv_geins(int n, void *irl, void *val, int dupl, int index_base, void *x);
+
+/* igathMultiVecDeviceDouble / iscatMultiVecDeviceDouble (dvectordev.c:128-
+   178): gather x(idx(first:first+n-1)) of column vector_id into the device
+   buffer hv(hfirst:...), or scatter hv into x as x = beta*x + hv.  idx is a
+   device int array (the VecIdx variants pass an int multivector handle). */
+int psb_hip_mv_igath(void *x, int vector_id, int n, int first, const int *didx,
+                     int hfirst, void *dhv, int index_base);
+int psb_hip_mv_iscat(void *x, int vector_id, int n, int first, const int *didx,
+                     int hfirst, const void *dhv, int index_base, double beta);
+
+/* BLAS-1 over the first n entries of every column (d = double, s = float).
+   References: dvectordev.c:180-336 (device), psi_d_serial_impl.F90 and
+   psb_d_base_vect_mod.F90 (CPU semantics, which are followed). */
+/* scalMultiVecDevice: x = alpha*x over the whole pitch*count */
+int psb_hip_mv_scal(double alpha, void *x);
+/* dotMultiVecDevice: res(c) = x(:,c).y(:,c); res is a host array of count
+   entries; synchronises */
+int psb_hip_mv_dot(double *res, int n, void *x, void *y);
+/* nrm2MultiVecDevice / amaxMultiVecDevice / asumMultiVecDevice */
+int psb_hip_mv_nrm2(double *res, int n, void *x);
+int psb_hip_mv_amax(double *res, int n, void *x);
+int psb_hip_mv_asum(double *res, int n, void *x);
+/* axpbyMultiVecDevice (psi_daxpbyv semantics): y = alpha*x + beta*y */
+int psb_hip_mv_axpby(int n, double alpha, void *x, double beta, void *y);
+/* axpbyz (psi_daxpbyzv): z = alpha*x + beta*y */
+int psb_hip_mv_axpbyz(int n, double alpha, void *x, double beta, void *y, void *z);
+/* upd_xyzMultiVecDevice (psi_d_upd_xyz): y = alpha*x + beta*y;
+   z = gamma*y + delta*z */
+int psb_hip_mv_upd_xyz(int n, double alpha, double beta, double gamma, double delta,
+                       void *x, void *y, void *z);
+/* xyzwMultiVecDevice (psi_dxyzw; the CUDA kernel's a+x at dxyzw.cu:47 is
+   not reproduced): y = a*x+b*y; z = c*y+d*z; w = e*z+f*w */
+int psb_hip_mv_xyzw(int n, double a, double b, double c, double d, double e, double f,
+                    void *x, void *y, void *z, void *w);
+/* axyMultiVecDevice: y = alpha*x.*y */
+int psb_hip_mv_axy(int n, double alpha, void *x, void *y);
+/* axybzMultiVecDevice (d_base_mlt_a_2 semantics): z = alpha*y.*x + beta*z,
+   z not read when beta == 0 */
+int psb_hip_mv_axybz(int n, double alpha, void *x, void *y, double beta, void *z);
+/* Jacobi build step of psb_d_diag_precbld (psb_d_diagprec_impl.f90:261-
+   270): d(i) = 1/d(i), a zero replaced by 1, for i <= n; entries n+1..size
+   set to 1 */
+int psb_hip_mv_diag_inv(int n, void *d);
+/* absMultiVecDevice2 / absMultiVecDevice: y = alpha*|x| */
+int psb_hip_mv_abs(int n, double alpha, void *x, void *y);
+
+/* ===================================================================== */
+/* Host-side assembly helpers (no device work)                              */
+/* ===================================================================== */
+
+/* psb_d_fix_coo (base/serial/impl/psb_d_coo_impl.F90:4094-4896), row major:
+   sorts the 1-based triplets by (row, column), stable, and merges
+   duplicates (dupl: 0 overwrite, 1 add, 2 error -> returns
+   PSB_HIP_INVALID).  In place; *nzout receives the new count. */
+int psb_hip_fix_coo_d(int nr, int nc, int64_t nz, int *ia, int *ja, double *val, int dupl,
+                      int64_t *nzout);
+/* psi_compute_hckoff_from_coo (cuda/impl/psb_d_cuda_cp_hlg_from_coo.F90:117-
+   188): from a row-sorted COO, irn(nr) row counts, idisp(nr+1) row starts
+   (0-based), hkoffs(noffs+1) hack offsets; returns isz in *isz. */
+int psb_hip_hll_offsets(int nr, int64_t nz, const int *ia, int hksz, int *irn, int *idisp,
+                        int *hkoffs, int64_t *isz);
+
+/* Generators of the benchmark operators (host, COO, 1-based, entries in the
+   insertion order of test/pargen/psb_d_pde3d.F90:478-560).  Call with
+   NULL arrays to get the entry count.  a[3], b[3]: diffusion and
+   convection coefficients; hscale = 0 drops h = 1/(L+1).  rhs (may be
+   NULL) receives the boundary term of the sample g when gflag = 1. */
+int64_t psb_hip_gen_pde3d(int L, const double *a, const double *b, double c, int hscale,
+                          int gflag, int *ia, int *ja, double *val, double *rhs);
+int64_t psb_hip_gen_aniso2d(int L, double eps, int *ia, int *ja, double *val);
+int64_t psb_hip_gen_powerlaw(int n, int cap, uint64_t seed, int *ia, int *ja, double *val);
+/* Counter-based uniform [-1,1) fill (host), the same stream the generators
+   use: x(i) = u(seed, stream, i). */
+void psb_hip_fill_uniform_host(uint64_t seed, uint32_t stream, int64_t n, double *x);
+/* The same values generated on the device into a double multivector column. */
+int psb_hip_mv_fill_uniform(void *x, uint64_t seed, uint32_t stream);
+
+/* ===================================================================== */
+/* HLL (HLG) matrices -- cuda/hlldev.h:96-153, cuda/hlldev_mod.F90          */
+/* ===================================================================== */
+
+/* FallocHllDevice (cuda/hlldev.c:145-157). elem_type FLOAT or DOUBLE;
+   allocsize = hkoffs(noffs+1) slots; first_index must be 1. */
+int psb_hip_hll_alloc(void **dmat, int hksz, int rows, int nzeros, int allocsize,
+                      int elem_type, int first_index);
+/* freeHllDevice (also releases the hack offsets, which the reference
+   leaks, cuda/hlldev.c:131-142) */
+void psb_hip_hll_free(void *dmat);
+/* writeHllDevice{Double,Float}: val/ja (allocsize), hkoffs (noffs+1),
+   irn (rows), idiag (rows). val has the matrix element type. */
+int psb_hip_hll_write(void *dmat, const void *val, const int *ja, const int *hkoffs,
+                      const int *irn, const int *idiag);
+/* readHllDevice{Double,Float} */
+int psb_hip_hll_read(void *dmat, void *val, int *ja, int *hkoffs, int *irn, int *idiag);
+/* getHllDeviceParams */
+int psb_hip_hll_get_params(void *dmat, int *hksz, int *rows, int *nzeros, int *allocsize,
+                           int *hack_offs_len, int *first_index, int *elem_type);
+/* psiCopyCooToHlg{Double,Float} (cuda/hlldev.c:420-462): fill the device
+   HLL from a row-sorted COO (ja, val of nza entries, 1-based) with the host
+   irn/hoffs/idisp of psb_hip_hll_offsets.  Padding follows the CPU rule
+   (ext/impl/psi_d_xtr_ell_from_coo.f90:56-60; last hack pre-filled with
+   ja = nr, val = 0, psi_d_convert_hll_from_coo.f90:93-96).  idiag = 1-based
+   slot of the diagonal, 0 if the row has none.  val has the element type
+   of dmat. */
+int psb_hip_hll_copy_coo(int nr, int nc, int nza, int hksz, int noffs, int isz,
+                         const int *irn, const int *hoffs, const int *idisp, const int *ja,
+                         const void *val, void *dmat);
+/* spmvHllDevice{Double,Float} (cuda/hlldev.c:185-207): y = alpha*A*x +
+   beta*y for every column of x/y (the reference computes column 1 only),
+   y not read when beta == 0.  x, y multivectors of the matrix type. */
+int psb_hip_hll_spmv(void *dmat, double alpha, void *x, double beta, void *y);
+/* Attach a single-precision copy of the values to a double HLL matrix so
+   the mixed-precision solvers can run FP32 SpMVs on the same index arrays
+   (no reference counterpart). */
+int psb_hip_hll_make_single(void *dmat);
+/* y = alpha*A*x + beta*y in single precision using that copy. */
+int psb_hip_hll_spmv_single(void *dmat, double alpha, void *x, double beta, void *y);
+/* Diagonal of the matrix into a vector of its type (0 where a row has no
+   diagonal): the device side of get_diag used by psb_d_diag_precbld
+   (prec/impl/psb_d_diagprec_impl.f90:254). */
+int psb_hip_hll_get_diag(void *dmat, void *d);
+/* Kernel variant selection for the HLL SpMV (autotuner hook): 0 = auto,
+   1 = one row per lane, 2 = two rows per lane.  Returns the previous value. */
+int psb_hip_hll_set_variant(int variant);
+
+/* ===================================================================== */
+/* ELL (ELG) matrices -- cuda/elldev.h, cuda/elldev_mod.F90:37-160          */
+/* ===================================================================== */
+
+/* FallocEllDevice: pitch = ceil32(rows) (cuda/spgpu/ell.c:33-37);
+   maxRowSize = ELL width nzm. */
+int psb_hip_ell_alloc(void **dmat, int rows, int max_row_size, int nnzeros, int elem_type,
+                      int first_index);
+void psb_hip_ell_free(void *dmat);
+/* getEllDevicePitch / getEllDeviceMaxRowSize */
+int psb_hip_ell_pitch(void *dmat);
+int psb_hip_ell_max_row_size(void *dmat);
+/* writeEllDevice{Double,Float}: val/ja with host leading dimension ldv
+   (>= rows) and nzm columns, irn (rows), idiag (rows) */
+int psb_hip_ell_write(void *dmat, const void *val, const int *ja, int ldv, const int *irn,
+                      const int *idiag);
+int psb_hip_ell_read(void *dmat, void *val, int *ja, int ldv, int *irn, int *idiag);
+/* psiCopyCooToElg{Double,Float}: fill from a row-sorted COO, CPU padding
+   rule (empty row -> first column of the whole matrix); rows beyond nr in
+   the pitch are ja = 1, val = 0. idisp(nr+1) row starts (0-based). */
+int psb_hip_ell_copy_coo(int nr, int nc, int nza, const int *irn, const int *idisp,
+                         const int *ja, const void *val, void *dmat);
+int psb_hip_ell_get_diag(void *dmat, void *d);
+/* spmvEllDevice{Double,Float} (cuda/elldev.c:195-215) */
+int psb_hip_ell_spmv(void *dmat, double alpha, void *x, double beta, void *y);
+
+/* ===================================================================== */
+/* CSR (CSRG) matrices -- cuda/psb_d_cuda_csrg_mat_mod.F90, the cuSPARSE    */
+/* wrapper cuda/fcusparse_fct.h:31,108-124 replaced by hand-written kernels */
+/* ===================================================================== */
+
+/* allocate for rows, nnz; elem_type FLOAT or DOUBLE; 1-based */
+int psb_hip_csr_alloc(void **dmat, int rows, int cols, int nnz, int elem_type,
+                      int first_index);
+void psb_hip_csr_free(void *dmat);
+/* write irp (rows+1, 1-based), ja, val (nnz); builds the row-length
+   analysis (lanes per row, list of long rows) used by the SpMV. */
+int psb_hip_csr_write(void *dmat, const int *irp, const int *ja, const void *val);
+int psb_hip_csr_read(void *dmat, int *irp, int *ja, void *val);
+/* spmvCSRGDevice: y = alpha*A*x + beta*y */
+int psb_hip_csr_spmv(void *dmat, double alpha, void *x, double beta, void *y);
+int psb_hip_csr_get_diag(void *dmat, void *d);
+/* the lanes-per-row and long-row threshold chosen by the analysis */
+int psb_hip_csr_get_analysis(void *dmat, int *lanes_per_row, int *long_rows,
+                             int *long_threshold);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSB_HIP_H */

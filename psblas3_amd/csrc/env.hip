@@ -1,0 +1,228 @@
+// Environment: device selection, library stream, reduction workspace,
+// device properties, event timing and raw buffers.
+// Replaces cuda/cuda_util.c, cuda/spgpu/core.c and the bind(c) set of
+// cuda/psb_cuda_env_mod.F90:43-174.
+#include <string.h>
+
+#include <mutex>
+#include <string>
+
+#include "psb_hip_internal.hpp"
+
+namespace psbhip {
+
+static Context g_ctx;
+static std::string g_last_error;
+static std::mutex g_init_mutex;
+static hipEvent_t g_ev_start = nullptr, g_ev_stop = nullptr;
+
+Context& ctx() { return g_ctx; }
+
+int set_error(int code, const char* where, const char* what) {
+  g_last_error = std::string(where) + ": " + (what ? what : "");
+  return code;
+}
+
+static int init_device(int dev) {
+  std::lock_guard<std::mutex> lock(g_init_mutex);
+  Context& c = g_ctx;
+  if (c.initialized) {
+    if (c.device == dev) return PSB_HIP_SUCCESS;
+    return set_error(PSB_HIP_INVALID, "psb_hip_init", "already initialised on another device");
+  }
+  int ndev = 0;
+  PSB_HIP_CHECK(hipGetDeviceCount(&ndev));
+  if (dev < 0 || dev >= ndev) return set_error(PSB_HIP_INVALID, "psb_hip_init", "no such device");
+  PSB_HIP_CHECK(hipSetDevice(dev));
+  hipDeviceProp_t prop;
+  PSB_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+  c.num_cus = prop.multiProcessorCount;
+  // default priority stream: the library stream every kernel is launched on
+  PSB_HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  PSB_HIP_CHECK(hipMalloc(&c.red_partials, sizeof(double) * kRedBlocks * 8));
+  PSB_HIP_CHECK(hipMalloc(&c.red_counter, sizeof(unsigned int) * 64));
+  PSB_HIP_CHECK(hipMalloc(&c.red_result, sizeof(double) * 64));
+  PSB_HIP_CHECK(hipMemset(c.red_counter, 0, sizeof(unsigned int) * 64));
+  PSB_HIP_CHECK(hipHostMalloc(&c.host_result, sizeof(double) * 64, hipHostMallocDefault));
+  PSB_HIP_CHECK(hipEventCreate(&g_ev_start));
+  PSB_HIP_CHECK(hipEventCreate(&g_ev_stop));
+  PSB_HIP_CHECK(hipDeviceSynchronize());
+  c.device = dev;
+  c.initialized = true;
+  return PSB_HIP_SUCCESS;
+}
+
+int ensure_init() {
+  if (g_ctx.initialized) return PSB_HIP_SUCCESS;
+  return init_device(0);
+}
+
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+__global__ __launch_bounds__(kBlock) void k_stream_copy(const v2d* __restrict__ src,
+                                                       v2d* __restrict__ dst, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(&src[i]), &dst[i]);
+}
+
+}  // namespace psbhip
+
+using namespace psbhip;
+
+extern "C" {
+
+int psb_hip_init(int dev) { return init_device(dev); }
+
+int psb_hip_exit(void) {
+  std::lock_guard<std::mutex> lock(g_init_mutex);
+  Context& c = g_ctx;
+  if (!c.initialized) return PSB_HIP_SUCCESS;
+  PSB_HIP_CHECK(hipStreamSynchronize(c.stream));
+  hipFree(c.red_partials);
+  hipFree(c.red_counter);
+  hipFree(c.red_result);
+  hipHostFree(c.host_result);
+  hipEventDestroy(g_ev_start);
+  hipEventDestroy(g_ev_stop);
+  hipStreamDestroy(c.stream);
+  c = Context();
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_get_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int psb_hip_get_device(void) {
+  int d = -1;
+  if (hipGetDevice(&d) != hipSuccess) return -1;
+  return d;
+}
+
+int psb_hip_set_device(int dev) {
+  PSB_HIP_CHECK(hipSetDevice(dev));
+  return PSB_HIP_SUCCESS;
+}
+
+static int prop_int(hipDeviceAttribute_t a) {
+  if (ensure_init() != PSB_HIP_SUCCESS) return -1;
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, a, g_ctx.device) != hipSuccess) return -1;
+  return v;
+}
+
+int psb_hip_get_multiprocessors(void) { return prop_int(hipDeviceAttributeMultiprocessorCount); }
+int psb_hip_get_memory_bus_width(void) { return prop_int(hipDeviceAttributeMemoryBusWidth); }
+int psb_hip_get_memory_clock_rate(void) { return prop_int(hipDeviceAttributeMemoryClockRate); }
+int psb_hip_get_warp_size(void) { return prop_int(hipDeviceAttributeWarpSize); }
+int psb_hip_get_max_threads_per_mp(void) {
+  return prop_int(hipDeviceAttributeMaxThreadsPerMultiProcessor);
+}
+int psb_hip_get_max_registers_per_block(void) {
+  return prop_int(hipDeviceAttributeMaxRegistersPerBlock);
+}
+int psb_hip_device_has_uva(void) { return prop_int(hipDeviceAttributeUnifiedAddressing); }
+
+int psb_hip_get_device_name(char* buf, int len) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  hipDeviceProp_t prop;
+  PSB_HIP_CHECK(hipGetDeviceProperties(&prop, g_ctx.device));
+  if (len > 0) {
+    strncpy(buf, prop.name, (size_t)len - 1);
+    buf[len - 1] = 0;
+  }
+  return PSB_HIP_SUCCESS;
+}
+
+double psb_hip_memory_peak_bandwidth(void) {
+  // 2 * clock(kHz) * busWidth(bit) / 8 -> kB/s; / 1000 -> MB/s
+  double clk = (double)psb_hip_get_memory_clock_rate();
+  double bw = (double)psb_hip_get_memory_bus_width();
+  return 2.0 * clk * bw / 8.0 / 1000.0;
+}
+
+int psb_hip_device_sync(void) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  PSB_HIP_CHECK(hipStreamSynchronize(g_ctx.stream));
+  return PSB_HIP_SUCCESS;
+}
+
+void* psb_hip_get_stream(void) {
+  if (ensure_init() != PSB_HIP_SUCCESS) return nullptr;
+  return (void*)g_ctx.stream;
+}
+
+const char* psb_hip_last_error(void) { return g_last_error.c_str(); }
+
+int psb_hip_timer_start(void) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  PSB_HIP_CHECK(hipEventRecord(g_ev_start, g_ctx.stream));
+  return PSB_HIP_SUCCESS;
+}
+
+double psb_hip_timer_stop(void) {
+  if (ensure_init() != PSB_HIP_SUCCESS) return -1.0;
+  if (hipEventRecord(g_ev_stop, g_ctx.stream) != hipSuccess) return -1.0;
+  if (hipEventSynchronize(g_ev_stop) != hipSuccess) return -1.0;
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, g_ev_start, g_ev_stop) != hipSuccess) return -1.0;
+  return (double)ms;
+}
+
+int psb_hip_stream_copy(void* dst, const void* src, int64_t nbytes) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (nbytes % 16 != 0) return set_error(PSB_HIP_INVALID, __func__, "nbytes % 16");
+  int64_t n = nbytes / 16;
+  if (n == 0) return PSB_HIP_SUCCESS;
+  int64_t g = (n + kBlock * 8 - 1) / (kBlock * 8);
+  int64_t cap = (int64_t)(g_ctx.num_cus > 0 ? g_ctx.num_cus : 256) * 16;
+  if (g > cap) g = cap;
+  hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)g), dim3(kBlock), 0, g_ctx.stream,
+                     (const v2d*)src, (v2d*)dst, n);
+  PSB_HIP_LAUNCH_CHECK();
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_alloc_buffer(void** dptr, int64_t nbytes) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  *dptr = nullptr;
+  if (nbytes <= 0) nbytes = 8;
+  PSB_HIP_CHECK(hipMalloc(dptr, (size_t)nbytes));
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_free_buffer(void* dptr) {
+  if (!dptr) return PSB_HIP_SUCCESS;
+  // buffers may still be in use by queued kernels
+  if (g_ctx.initialized) PSB_HIP_CHECK(hipStreamSynchronize(g_ctx.stream));
+  PSB_HIP_CHECK(hipFree(dptr));
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_write_buffer(void* dptr, const void* host, int64_t nbytes) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (nbytes <= 0) return PSB_HIP_SUCCESS;
+  PSB_HIP_CHECK(hipMemcpyAsync(dptr, host, (size_t)nbytes, hipMemcpyHostToDevice, g_ctx.stream));
+  PSB_HIP_CHECK(hipStreamSynchronize(g_ctx.stream));
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_read_buffer(void* host, const void* dptr, int64_t nbytes) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (nbytes <= 0) return PSB_HIP_SUCCESS;
+  PSB_HIP_CHECK(hipMemcpyAsync(host, dptr, (size_t)nbytes, hipMemcpyDeviceToHost, g_ctx.stream));
+  PSB_HIP_CHECK(hipStreamSynchronize(g_ctx.stream));
+  return PSB_HIP_SUCCESS;
+}
+
+}  // extern "C"

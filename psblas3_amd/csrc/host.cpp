@@ -1,0 +1,332 @@
+// Host-side assembly: COO fix (sort + duplicate merge), HLL hack offsets,
+// and the generators of the benchmark operators.
+//
+// psb_hip_fix_coo_d restates psb_d_fix_coo_inner_rowmajor
+// (base/serial/impl/psb_d_coo_impl.F90:4252-4896) with a counting sort by
+// row and a per-row stable column sort spread over host threads; the
+// result is the reference's: rows ascending, columns ascending within a
+// row, duplicates merged left to right in input order.
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <thread>
+#include <vector>
+
+#include "psb_hip.h"
+
+namespace {
+
+int host_threads() {
+  unsigned n = std::thread::hardware_concurrency();
+  if (n == 0) n = 1;
+  if (n > 16) n = 16;
+  return (int)n;
+}
+
+template <typename F>
+void parallel_for(int64_t n, F f) {
+  int nt = host_threads();
+  if (n < 65536 || nt == 1) {
+    f(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  int64_t chunk = (n + nt - 1) / nt;
+  for (int t = 0; t < nt; t++) {
+    int64_t lo = t * chunk, hi = std::min<int64_t>(n, lo + chunk);
+    if (lo >= hi) break;
+    th.emplace_back([=] { f(lo, hi); });
+  }
+  for (auto& t : th) t.join();
+}
+
+inline uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+struct Uniform {
+  uint64_t key;
+  Uniform(uint64_t seed, uint32_t stream)
+      : key(mix64(seed * 0xD1B54A32D192ED03ULL ^ ((uint64_t)stream << 48))) {}
+  double u01(uint64_t i) const { return (double)(mix64(key ^ i) >> 11) * 0x1.0p-53; }
+  double pm1(uint64_t i) const { return 2.0 * u01(i) - 1.0; }
+};
+
+}  // namespace
+
+extern "C" {
+
+int psb_hip_fix_coo_d(int nr, int nc, int64_t nz, int* ia, int* ja, double* val, int dupl,
+                      int64_t* nzout) {
+  (void)nc;
+  if (nz < 2) {
+    *nzout = nz;
+    return PSB_HIP_SUCCESS;
+  }
+  if (dupl < 0 || dupl > 2) return PSB_HIP_INVALID;
+  // 1. counts per row
+  std::vector<int64_t> start((size_t)nr + 2, 0);
+  for (int64_t k = 0; k < nz; k++) {
+    int r = ia[k];
+    if (r < 1 || r > nr) return PSB_HIP_INVALID;
+    start[(size_t)r]++;
+  }
+  // 2. exclusive scan: rowptr[r] = first slot of row r+1
+  std::vector<int64_t> rowptr((size_t)nr + 1);
+  int64_t acc = 0;
+  for (int r = 0; r < nr; r++) {
+    rowptr[r] = acc;
+    acc += start[(size_t)r + 1];
+  }
+  rowptr[nr] = acc;
+  // 3. stable scatter by row
+  std::vector<int> jas((size_t)nz);
+  std::vector<double> vs((size_t)nz);
+  {
+    std::vector<int64_t> fill(rowptr.begin(), rowptr.end() - 1);
+    for (int64_t k = 0; k < nz; k++) {
+      int64_t p = fill[(size_t)ia[k] - 1]++;
+      jas[(size_t)p] = ja[k];
+      vs[(size_t)p] = val[k];
+    }
+  }
+  // 4. per-row stable sort by column
+  parallel_for(nr, [&](int64_t lo, int64_t hi) {
+    std::vector<std::pair<int, int64_t>> tmp;
+    std::vector<double> vtmp;
+    for (int64_t r = lo; r < hi; r++) {
+      int64_t b = rowptr[r], e = rowptr[r + 1];
+      int64_t len = e - b;
+      if (len < 2) continue;
+      if (len <= 32) {  // insertion sort, stable
+        for (int64_t i = b + 1; i < e; i++) {
+          int c = jas[i];
+          double v = vs[i];
+          int64_t j = i - 1;
+          while (j >= b && jas[j] > c) {
+            jas[j + 1] = jas[j];
+            vs[j + 1] = vs[j];
+            j--;
+          }
+          jas[j + 1] = c;
+          vs[j + 1] = v;
+        }
+      } else {
+        tmp.resize(len);
+        for (int64_t i = 0; i < len; i++) tmp[i] = {jas[b + i], i};
+        std::stable_sort(tmp.begin(), tmp.end(),
+                         [](const std::pair<int, int64_t>& x, const std::pair<int, int64_t>& y) {
+                           return x.first < y.first;
+                         });
+        vtmp.assign(vs.begin() + b, vs.begin() + e);
+        for (int64_t i = 0; i < len; i++) {
+          jas[b + i] = tmp[i].first;
+          vs[b + i] = vtmp[tmp[i].second];
+        }
+      }
+    }
+  });
+  // 5. merge duplicates, left to right
+  int64_t k = -1;
+  for (int r = 0; r < nr; r++) {
+    for (int64_t p = rowptr[r]; p < rowptr[r + 1]; p++) {
+      if (k >= 0 && ia[k] == r + 1 && ja[k] == jas[p]) {
+        if (dupl == 1) val[k] = val[k] + vs[p];
+        else if (dupl == 0) val[k] = vs[p];
+        else return PSB_HIP_INVALID;
+      } else {
+        k++;
+        ia[k] = r + 1;
+        ja[k] = jas[p];
+        val[k] = vs[p];
+      }
+    }
+  }
+  *nzout = k + 1;
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_hll_offsets(int nr, int64_t nz, const int* ia, int hksz, int* irn, int* idisp,
+                        int* hkoffs, int64_t* isz) {
+  if (hksz < 1 || nr < 0) return PSB_HIP_INVALID;
+  memset(irn, 0, sizeof(int) * (size_t)nr);
+  for (int64_t k = 0; k < nz; k++) {
+    int r = ia[k];
+    if (r < 1 || r > nr) return PSB_HIP_INVALID;
+    irn[r - 1]++;
+  }
+  idisp[0] = 0;
+  int64_t acc = 0;
+  for (int i = 0; i < nr; i++) {
+    acc += irn[i];
+    if (acc > 0x7fffffffLL) return PSB_HIP_UNSUPPORTED;
+    idisp[i + 1] = (int)acc;
+  }
+  int noffs = (nr + hksz - 1) / hksz;
+  int64_t off = 0;
+  hkoffs[0] = 0;
+  for (int h = 0; h < noffs; h++) {
+    int i0 = h * hksz;
+    int i1 = std::min(nr, i0 + hksz);
+    int mx = 0;
+    for (int i = i0; i < i1; i++) mx = std::max(mx, irn[i]);
+    off += (int64_t)mx * hksz;
+    if (off > 0x7fffffffLL) return PSB_HIP_UNSUPPORTED;
+    hkoffs[h + 1] = (int)off;
+  }
+  *isz = off;
+  return PSB_HIP_SUCCESS;
+}
+
+// ------------------------------------------------------------------ generators
+
+static double g_sample(double x, double y, double z) {
+  // boundary function of the pde3d sample, test/pargen/psb_d_pde3d.F90:160-170
+  if (x == 1.0) return 1.0;
+  if (x == 0.0) return exp(y * y - z * z);
+  return 0.0;
+}
+
+int64_t psb_hip_gen_pde3d(int L, const double* a, const double* b, double c, int hscale,
+                          int gflag, int* ia, int* ja, double* val, double* rhs) {
+  const int64_t n = (int64_t)L * L * L;
+  if (!ia) return 7 * n - 6 * (int64_t)L * L;
+  const double h = hscale ? 1.0 / (L + 1) : 1.0;
+  const double h2 = h * h, two_h = 2.0 * h;
+  const double vxm = -a[0] / h2 - b[0] / two_h, vxp = -a[0] / h2 + b[0] / two_h;
+  const double vym = -a[1] / h2 - b[1] / two_h, vyp = -a[1] / h2 + b[1] / two_h;
+  const double vzm = -a[2] / h2 - b[2] / two_h, vzp = -a[2] / h2 + b[2] / two_h;
+  const double vd = 2.0 * (a[0] + a[1] + a[2]) / h2 + c;
+  const double gh = 1.0 / (L + 1);
+  const int64_t plane = (int64_t)L * L;
+  // one z-plane per task; entries before plane iz: interior planes have
+  // 7L^2 - 4L entries (x and y boundaries), the two end planes L^2 fewer
+  const int64_t per_plane = 7 * plane - 4 * (int64_t)L;
+  parallel_for(L, [&](int64_t z0, int64_t z1) {
+    for (int64_t izm = z0; izm < z1; izm++) {
+      int iz = (int)izm + 1;
+      int64_t k = izm * per_plane - (izm > 0 ? plane : 0);
+      for (int iy = 1; iy <= L; iy++)
+        for (int ix = 1; ix <= L; ix++) {
+          int row = (int)(((int64_t)(iz - 1) * L + (iy - 1)) * L + ix);
+          double x = (ix - 1) * gh, y = (iy - 1) * gh, z = (iz - 1) * gh;
+          double zt = 0.0;
+          if (ix == 1) {
+            if (gflag) zt = g_sample(0.0, y, z) * (-vxm) + zt;
+          } else {
+            ia[k] = row; ja[k] = row - 1; val[k] = vxm; k++;
+          }
+          if (iy == 1) {
+            if (gflag) zt = g_sample(x, 0.0, z) * (-vym) + zt;
+          } else {
+            ia[k] = row; ja[k] = row - L; val[k] = vym; k++;
+          }
+          if (iz == 1) {
+            if (gflag) zt = g_sample(x, y, 0.0) * (-vzm) + zt;
+          } else {
+            ia[k] = row; ja[k] = (int)(row - plane); val[k] = vzm; k++;
+          }
+          ia[k] = row; ja[k] = row; val[k] = vd; k++;
+          if (iz == L) {
+            if (gflag) zt = g_sample(x, y, 1.0) * (-vzp) + zt;
+          } else {
+            ia[k] = row; ja[k] = (int)(row + plane); val[k] = vzp; k++;
+          }
+          if (iy == L) {
+            if (gflag) zt = g_sample(x, 1.0, z) * (-vyp) + zt;
+          } else {
+            ia[k] = row; ja[k] = row + L; val[k] = vyp; k++;
+          }
+          if (ix == L) {
+            if (gflag) zt = g_sample(1.0, y, z) * (-vxp) + zt;
+          } else {
+            ia[k] = row; ja[k] = row + 1; val[k] = vxp; k++;
+          }
+          if (rhs) rhs[row - 1] = zt;
+        }
+    }
+  });
+  return 7 * n - 6 * plane;
+}
+
+int64_t psb_hip_gen_aniso2d(int L, double eps, int* ia, int* ja, double* val) {
+  const int64_t n = (int64_t)L * L;
+  const int64_t total = 5 * n - 4 * (int64_t)L;
+  if (!ia) return total;
+  const double vd = 2.0 * eps + 2.0;
+  int64_t k = 0;
+  for (int iy = 1; iy <= L; iy++)
+    for (int ix = 1; ix <= L; ix++) {
+      int row = (iy - 1) * L + ix;
+      if (ix > 1) { ia[k] = row; ja[k] = row - 1; val[k] = -eps; k++; }
+      if (iy > 1) { ia[k] = row; ja[k] = row - L; val[k] = -1.0; k++; }
+      ia[k] = row; ja[k] = row; val[k] = vd; k++;
+      if (iy < L) { ia[k] = row; ja[k] = row + L; val[k] = -1.0; k++; }
+      if (ix < L) { ia[k] = row; ja[k] = row + 1; val[k] = -eps; k++; }
+    }
+  return k;
+}
+
+// Power-law rows: length 1 + min(cap, n-1, floor(1.775 / v)), v in (0,1];
+// columns uniform without repetition, diagonal first; values U[-1,1).
+static int powerlaw_len(const Uniform& ul, int n, int cap, int i1) {
+  double v = 1.0 - ul.u01((uint64_t)(i1 - 1));
+  double kd = floor(1.775 / v);
+  int kmax = std::min(cap, n - 1);
+  int k = kd > (double)kmax ? kmax : (int)kd;
+  return k + 1;
+}
+
+int64_t psb_hip_gen_powerlaw(int n, int cap, uint64_t seed, int* ia, int* ja, double* val) {
+  Uniform ul(seed, 1), uc(seed, 2), uv(seed, 3);
+  std::vector<int64_t> off((size_t)n + 1, 0);
+  for (int i = 1; i <= n; i++) off[i] = off[i - 1] + powerlaw_len(ul, n, cap, i);
+  if (!ia) return off[n];
+  parallel_for(n, [&](int64_t lo, int64_t hi) {
+    std::vector<int> seen;
+    for (int64_t i0 = lo; i0 < hi; i0++) {
+      int i = (int)i0 + 1;
+      int len = (int)(off[i] - off[i - 1]);
+      int64_t k = off[i - 1];
+      int hs = 16;
+      while (hs < 2 * len) hs <<= 1;
+      seen.assign((size_t)hs, 0);
+      uint64_t base = (uint64_t)(i - 1) << 20;
+      ia[k] = i; ja[k] = i; val[k] = uv.pm1(base); k++;
+      int got = 1;
+      uint64_t attempt = 0;
+      while (got < len) {
+        double u = uc.u01(base + attempt);
+        attempt++;
+        int col = 1 + (int)(u * (double)n);
+        if (col > n) col = n;
+        if (col == i) continue;
+        unsigned h = ((unsigned)col * 2654435761u) & (unsigned)(hs - 1);
+        bool dup = false;
+        while (seen[h]) {
+          if (seen[h] == col) { dup = true; break; }
+          h = (h + 1) & (unsigned)(hs - 1);
+        }
+        if (dup) continue;
+        seen[h] = col;
+        ia[k] = i; ja[k] = col; val[k] = uv.pm1(base + (uint64_t)got); k++;
+        got++;
+      }
+    }
+  });
+  return off[n];
+}
+
+void psb_hip_fill_uniform_host(uint64_t seed, uint32_t stream, int64_t n, double* x) {
+  Uniform u(seed, stream);
+  parallel_for(n, [&](int64_t lo, int64_t hi) {
+    for (int64_t i = lo; i < hi; i++) x[i] = u.pm1((uint64_t)i);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,267 @@
+// Internal declarations shared by the libpsb_hip translation units.
+// Device code targets gfx950 (CDNA4): 64-lane wavefronts, DPP cross-lane
+// moves, 160 KB LDS per CU, 8 XCDs each with its own L2.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "psb_hip.h"
+
+namespace psbhip {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;             // threads per block for streaming kernels
+constexpr int kXcds = 8;                // MI355X: 8 XCDs, blocks dispatched round-robin
+constexpr int kRedBlocks = 1024;        // max blocks of a grid reduction
+
+// ------------------------------------------------------------------ errors
+int set_error(int code, const char* where, const char* what);
+#define PSB_HIP_CHECK(expr)                                                    \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return ::psbhip::set_error(e_ == hipErrorOutOfMemory ? PSB_HIP_OUTOFMEM  \
+                                                           : PSB_HIP_RUNTIME,  \
+                                 __func__, hipGetErrorString(e_));             \
+  } while (0)
+#define PSB_HIP_LAUNCH_CHECK() PSB_HIP_CHECK(hipGetLastError())
+
+// ------------------------------------------------------------------ context
+// One context per process (PSBLAS runs one process per device): the device,
+// the stream every kernel is launched on, and the reduction workspace.
+struct Context {
+  int device = -1;
+  bool initialized = false;
+  hipStream_t stream = nullptr;
+  int num_cus = 0;
+  // grid-reduction workspace: partials (double), completion counters and
+  // result slots, all device memory; plus pinned host slots for read-back.
+  double* red_partials = nullptr;       // kRedBlocks * 4 doubles
+  unsigned int* red_counter = nullptr;  // 64 counters, kept at zero between uses
+  double* red_result = nullptr;         // 64 result slots
+  double* host_result = nullptr;        // pinned, 64 slots
+};
+Context& ctx();
+int ensure_init();                      // lazily initialises device 0
+
+// ------------------------------------------------------------------ vectors
+// Device multivector, the counterpart of cuda/vectordev.h:39-52.
+struct MultiVec {
+  int count;       // number of columns
+  int size;        // rows per column
+  int pitch;       // elements between columns
+  int elem_type;   // PSB_HIP_TYPE_*
+  void* v;         // device memory, pitch*count elements
+};
+
+inline size_t elem_size(int t) {
+  switch (t) {
+    case PSB_HIP_TYPE_INT: return 4;
+    case PSB_HIP_TYPE_FLOAT: return 4;
+    case PSB_HIP_TYPE_DOUBLE: return 8;
+    default: return 0;
+  }
+}
+
+// ------------------------------------------------------------------ launch helpers
+inline int grid_for(int64_t n, int per_block) {
+  int64_t g = (n + per_block - 1) / per_block;
+  return (int)(g < 1 ? 1 : g);
+}
+
+// Grid of a streaming (grid-stride) kernel: enough waves to fill 256 CUs
+// several times over without exceeding the work.
+inline int stream_grid(int64_t n) {
+  int64_t g = (n + kBlock * 4 - 1) / (kBlock * 4);
+  int64_t cap = (int64_t)(ctx().num_cus > 0 ? ctx().num_cus : 256) * 8;
+  if (g > cap) g = cap;
+  return (int)(g < 1 ? 1 : g);
+}
+
+inline int red_grid(int64_t n) {
+  int64_t g = (n + kBlock * 8 - 1) / (kBlock * 8);
+  if (g > kRedBlocks) g = kRedBlocks;
+  return (int)(g < 1 ? 1 : g);
+}
+
+// ------------------------------------------------------------------ device helpers
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T* p) { return __builtin_nontemporal_load(p); }
+template <typename T>
+__device__ __forceinline__ void st_stream(T* p, T v) { __builtin_nontemporal_store(v, p); }
+
+// DPP row rotate (within each 16-lane row) of a 32-bit value.  CTRL is the
+// GFX9 dpp_ctrl encoding: row_ror:N = 0x120 + N.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v) {
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  uint64_t u = __builtin_bit_cast(uint64_t, v);
+  int lo = dpp_i32<CTRL>((int)(uint32_t)u);
+  int hi = dpp_i32<CTRL>((int)(uint32_t)(u >> 32));
+  return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __builtin_bit_cast(float, dpp_i32<CTRL>(__builtin_bit_cast(int, v)));
+}
+template <int CTRL, typename T>
+__device__ __forceinline__ T dpp(T v) {
+  if constexpr (sizeof(T) == 8) return dpp_f64<CTRL>(v);
+  else return dpp_f32<CTRL>(v);
+}
+
+template <typename T>
+__device__ __forceinline__ T readlane(T v, int lane) {
+  if constexpr (sizeof(T) == 8) {
+    uint64_t u = __builtin_bit_cast(uint64_t, v);
+    uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+  }
+}
+
+// Sum over a full wavefront (all 64 lanes must be active).  Four DPP
+// row_ror steps leave every lane of a 16-lane row holding the row sum; the
+// four row sums are combined through readlane in a fixed order, so the
+// result is wave-uniform and deterministic.
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+  v += dpp<0x128>(v);  // row_ror:8
+  v += dpp<0x124>(v);  // row_ror:4
+  v += dpp<0x122>(v);  // row_ror:2
+  v += dpp<0x121>(v);  // row_ror:1
+  return (readlane(v, 0) + readlane(v, 16)) + (readlane(v, 32) + readlane(v, 48));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+  v = fmax(v, dpp<0x128>(v));
+  v = fmax(v, dpp<0x124>(v));
+  v = fmax(v, dpp<0x122>(v));
+  v = fmax(v, dpp<0x121>(v));
+  return fmax(fmax(readlane(v, 0), readlane(v, 16)), fmax(readlane(v, 32), readlane(v, 48)));
+}
+
+// Sum over aligned groups of G lanes (G = 2..16) inside a 16-lane row,
+// result valid in every lane of the group.  Used by the vector-CSR kernel.
+template <int G, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+  // xor butterfly inside the group (ds_swizzle / ds_bpermute lane moves)
+  #pragma unroll
+  for (int m = G / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, G);
+  return v;
+}
+
+// Block-wide sum (blockDim.x == kBlock); every thread gets the result.
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T* lds /* kBlock/kWave */) {
+  T w = wave_sum(v);
+  int wid = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) lds[wid] = w;
+  __syncthreads();
+  T r = 0;
+  #pragma unroll
+  for (int k = 0; k < kBlock / kWave; k++) r += lds[k];
+  __syncthreads();
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ T block_max(T v, T* lds) {
+  T w = wave_max(v);
+  int wid = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) lds[wid] = w;
+  __syncthreads();
+  T r = lds[0];
+  #pragma unroll
+  for (int k = 1; k < kBlock / kWave; k++) r = fmax(r, lds[k]);
+  __syncthreads();
+  return r;
+}
+
+// Last-block-done grid reduction: every block stores its partial, the last
+// block to arrive (device-scope counter) sums the partials in index order,
+// stores the result and re-arms the counter.  Deterministic for a fixed
+// grid size.  Returns true in the threads of the last block.
+// NPART partial values per block (e.g. 2 for fused dot pairs).
+template <int NPART>
+__device__ __forceinline__ bool grid_reduce_sum(double (&v)[NPART], double* partials,
+                                                unsigned int* counter, double* result) {
+  __shared__ double lds[kBlock / kWave];
+  __shared__ bool am_last;
+  #pragma unroll
+  for (int p = 0; p < NPART; p++) {
+    double s = block_sum(v[p], lds);
+    if (threadIdx.x == 0) partials[(size_t)p * kRedBlocks + blockIdx.x] = s;
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned int prev = atomicAdd(counter, 1u);
+    am_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!am_last) return false;
+  __threadfence();
+  #pragma unroll
+  for (int p = 0; p < NPART; p++) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock)
+      s += __builtin_nontemporal_load(&partials[(size_t)p * kRedBlocks + b]);
+    s = block_sum(s, lds);
+    v[p] = s;
+    if (threadIdx.x == 0) result[p] = s;
+  }
+  if (threadIdx.x == 0) *counter = 0u;
+  return true;
+}
+
+template <int NPART>
+__device__ __forceinline__ bool grid_reduce_max(double (&v)[NPART], double* partials,
+                                                unsigned int* counter, double* result) {
+  __shared__ double lds[kBlock / kWave];
+  __shared__ bool am_last;
+  #pragma unroll
+  for (int p = 0; p < NPART; p++) {
+    double s = block_max(v[p], lds);
+    if (threadIdx.x == 0) partials[(size_t)p * kRedBlocks + blockIdx.x] = s;
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned int prev = atomicAdd(counter, 1u);
+    am_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!am_last) return false;
+  __threadfence();
+  #pragma unroll
+  for (int p = 0; p < NPART; p++) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock)
+      s = fmax(s, __builtin_nontemporal_load(&partials[(size_t)p * kRedBlocks + b]));
+    s = block_max(s, lds);
+    v[p] = s;
+    if (threadIdx.x == 0) result[p] = s;
+  }
+  if (threadIdx.x == 0) *counter = 0u;
+  return true;
+}
+
+// XCD-aware block index: blocks are dispatched round-robin over the 8 XCDs;
+// remap so that each XCD works on a contiguous range of the logical grid,
+// keeping neighbouring rows (and the x entries they gather) in one L2.
+// nlog = logical block count; the launched grid is ceil(nlog/8)*8.
+__device__ __forceinline__ int xcd_block(int nlog) {
+  int b = blockIdx.x;
+  int per = (nlog + kXcds - 1) / kXcds;
+  return (b % kXcds) * per + (b / kXcds);
+}
+inline int xcd_grid(int nlog) { return ((nlog + kXcds - 1) / kXcds) * kXcds; }
+
+}  // namespace psbhip

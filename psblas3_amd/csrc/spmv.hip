@@ -1,0 +1,1144 @@
+// Sparse matrix-vector products on gfx950: HLL (hacked ELLPACK), ELL and
+// CSR device matrices, FP64 and FP32, plus the COO -> HLL/ELL device fills.
+//
+// Replaces cuda/hlldev.c, cuda/elldev.c, the SPGPU kernels
+// cuda/spgpu/kernels/hell_spmv_base_template.cuh and ell_spmv_base_template.cuh,
+// the device fills cuda/CUDA/psi_cuda_CopyCooToHlg.cuh / CopyCooToElg.cuh and
+// the cuSPARSE CSR product wrapped by cuda/fcusparse_fct.h:108-124.
+//
+// HLL layout in HBM (the reference's, ext/psb_d_hll_mat_mod.f90:38-104):
+// rows are grouped in hacks of hksz consecutive rows; hack h occupies
+// val/ja[hkoffs[h] .. hkoffs[h+1]) stored column-major inside the hack
+// (slot (r, j) at hkoffs[h] + j*hksz + r), so lane r of a wavefront reads
+// consecutive addresses for every j: one 512-byte (FP64) / 256-byte (FP32 or
+// index) coalesced load per column of the hack.  Kernel: one row per lane,
+// 64 rows per wavefront (one hack at hksz = 64, two at hksz = 32); val and
+// ja are streamed with non-temporal loads (they are read once), x is
+// gathered through the XCD's L2 and the 256 MB Infinity Cache, y written
+// once.  Blocks are remapped so each XCD walks a contiguous range of rows,
+// which keeps the stencil's x neighbours (+-1, +-L, +-L^2) in one L2.
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "psb_hip_internal.hpp"
+
+namespace psbhip {
+
+struct HllDev {
+  int elem_type;
+  int hksz, rows, nzeros, allocsize, noffs, first_index;
+  int max_col;          // largest column index stored (x must be this long)
+  void* val;            // allocsize entries of elem_type
+  int* ja;              // allocsize
+  int* hkoffs;          // noffs + 1
+  int* irn;             // rows
+  int* idiag;           // rows
+  float* val32;         // optional FP32 copy of a double matrix
+};
+
+struct EllDev {
+  int elem_type;
+  int rows, pitch, nzm, nnz, first_index;
+  int max_col;
+  void* val;            // pitch * nzm
+  int* ja;
+  int* irn;
+  int* idiag;
+};
+
+struct CsrDev {
+  int elem_type;
+  int rows, cols, nnz, first_index;
+  int max_col;
+  int* irp;             // rows + 1, 1-based
+  int* ja;
+  void* val;
+  int lanes;            // lanes per row of the vector kernel
+  int long_thr;         // rows longer than this go to the wave-per-row kernel
+  int nlong;
+  int* long_rows;       // 0-based row ids
+};
+
+static int g_hll_variant = 0;
+
+// ------------------------------------------------------------------ inner products
+// acc += sum_{j<w} val[j*stride] * x[ja[j*stride]-1], eight columns at a time:
+// all index/value loads of a group are issued before the dependent gathers.
+template <typename T, typename TV>
+__device__ __forceinline__ T slot_dot(const int* __restrict__ pj, const TV* __restrict__ pv,
+                                      int stride, int w, const T* __restrict__ x) {
+  T acc = T(0);
+  int j = 0;
+  for (; j + 8 <= w; j += 8) {
+    int c[8];
+    TV v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      c[k] = ld_stream(pj + (size_t)(j + k) * stride);
+      v[k] = ld_stream(pv + (size_t)(j + k) * stride);
+    }
+    T xv[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) xv[k] = x[c[k] - 1];
+#pragma unroll
+    for (int k = 0; k < 8; k++) acc = fma((T)v[k], xv[k], acc);
+  }
+  if (j < w) {
+    int c[8];
+    TV v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      bool ok = j + k < w;
+      c[k] = ok ? ld_stream(pj + (size_t)(j + k) * stride) : 1;
+      v[k] = ok ? ld_stream(pv + (size_t)(j + k) * stride) : TV(0);
+    }
+    T xv[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) xv[k] = (j + k < w) ? x[c[k] - 1] : T(0);
+#pragma unroll
+    for (int k = 0; k < 8; k++) acc = fma((T)v[k], xv[k], acc);
+  }
+  return acc;
+}
+
+template <typename T>
+__device__ __forceinline__ void store_y(T* __restrict__ y, int row, T alpha, T acc, T beta) {
+  T r = alpha * acc;
+  if (beta != T(0)) r = fma(beta, y[row], r);
+  y[row] = r;
+}
+
+// ------------------------------------------------------------------ HLL SpMV
+// HK > 0: compile-time hack size (32 or 64); HK == 0: any hksz.
+template <typename T, typename TV, int HK>
+__global__ __launch_bounds__(kBlock) void k_hll_spmv(int nr, int hksz, int nlog,
+                                                     const int* __restrict__ hkoffs,
+                                                     const int* __restrict__ ja,
+                                                     const TV* __restrict__ val,
+                                                     const T* __restrict__ x, T alpha, T beta,
+                                                     T* __restrict__ y) {
+  const int lb = xcd_block(nlog);
+  if (lb >= nlog) return;
+  const int row = lb * kBlock + threadIdx.x;
+  const int r = row < nr ? row : nr - 1;
+  const int stride = HK > 0 ? HK : hksz;
+  const int hk = r / stride;
+  const int lane = r - hk * stride;
+  const int off0 = hkoffs[hk];
+  const int w = (hkoffs[hk + 1] - off0) / stride;
+  T acc = slot_dot<T, TV>(ja + off0 + lane, val + off0 + lane, stride, w, x);
+  if (row < nr) store_y(y, row, alpha, acc, beta);
+}
+
+// Two hacks' rows per lane (rows r and r + 32*... ): a lane handles row
+// base+lane and base+64+lane, doubling the loads in flight per wave.
+template <typename T, typename TV, int HK>
+__global__ __launch_bounds__(kBlock) void k_hll_spmv2(int nr, int hksz, int nlog,
+                                                      const int* __restrict__ hkoffs,
+                                                      const int* __restrict__ ja,
+                                                      const TV* __restrict__ val,
+                                                      const T* __restrict__ x, T alpha, T beta,
+                                                      T* __restrict__ y) {
+  const int lb = xcd_block(nlog);
+  if (lb >= nlog) return;
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int row0 = lb * (2 * kBlock) + wave * 128 + l;
+  const int row1 = row0 + 64;
+  const int stride = HK > 0 ? HK : hksz;
+  const int r0 = row0 < nr ? row0 : nr - 1;
+  const int r1 = row1 < nr ? row1 : nr - 1;
+  const int h0 = r0 / stride, h1 = r1 / stride;
+  const int o0 = hkoffs[h0], o1 = hkoffs[h1];
+  const int w0 = (hkoffs[h0 + 1] - o0) / stride, w1 = (hkoffs[h1 + 1] - o1) / stride;
+  const int* pj0 = ja + o0 + (r0 - h0 * stride);
+  const int* pj1 = ja + o1 + (r1 - h1 * stride);
+  const TV* pv0 = val + o0 + (r0 - h0 * stride);
+  const TV* pv1 = val + o1 + (r1 - h1 * stride);
+  T a0 = T(0), a1 = T(0);
+  const int w = w0 > w1 ? w0 : w1;
+  for (int j = 0; j < w; j += 4) {
+    int c0[4], c1[4];
+    TV v0[4], v1[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      bool ok0 = j + k < w0, ok1 = j + k < w1;
+      c0[k] = ok0 ? ld_stream(pj0 + (size_t)(j + k) * stride) : 1;
+      v0[k] = ok0 ? ld_stream(pv0 + (size_t)(j + k) * stride) : TV(0);
+      c1[k] = ok1 ? ld_stream(pj1 + (size_t)(j + k) * stride) : 1;
+      v1[k] = ok1 ? ld_stream(pv1 + (size_t)(j + k) * stride) : TV(0);
+    }
+    T x0[4], x1[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      x0[k] = (j + k < w0) ? x[c0[k] - 1] : T(0);
+      x1[k] = (j + k < w1) ? x[c1[k] - 1] : T(0);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      a0 = fma((T)v0[k], x0[k], a0);
+      a1 = fma((T)v1[k], x1[k], a1);
+    }
+  }
+  if (row0 < nr) store_y(y, row0, alpha, a0, beta);
+  if (row1 < nr) store_y(y, row1, alpha, a1, beta);
+}
+
+// Multi-RHS HLL product (psb_d_hll_csmm, ext/impl/psb_d_hll_csmm.f90): each
+// lane owns one row and NC columns of x/y (column-major, pitch elements
+// apart); a slot's (ja, val) is loaded once and reused for all NC columns.
+template <typename T, int NC>
+__global__ __launch_bounds__(kBlock) void k_hll_spmm(int nr, int hksz, int nlog,
+                                                     const int* __restrict__ hkoffs,
+                                                     const int* __restrict__ ja,
+                                                     const T* __restrict__ val,
+                                                     const T* __restrict__ x, int ldx,
+                                                     T alpha, T beta, T* __restrict__ y,
+                                                     int ldy, int ncols) {
+  const int lb = xcd_block(nlog);
+  if (lb >= nlog) return;
+  const int row = lb * kBlock + threadIdx.x;
+  const int r = row < nr ? row : nr - 1;
+  const int hk = r / hksz;
+  const int lane = r - hk * hksz;
+  const int off0 = hkoffs[hk];
+  const int w = (hkoffs[hk + 1] - off0) / hksz;
+  const int* pj = ja + off0 + lane;
+  const T* pv = val + off0 + lane;
+  T acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) acc[c] = T(0);
+  for (int j = 0; j < w; j++) {
+    int col = ld_stream(pj + (size_t)j * hksz) - 1;
+    T v = ld_stream(pv + (size_t)j * hksz);
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+      if (c < ncols) acc[c] = fma(v, x[(size_t)c * ldx + col], acc[c]);
+  }
+  if (row < nr) {
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+      if (c < ncols) store_y(y + (size_t)c * ldy, row, alpha, acc[c], beta);
+  }
+}
+
+// ------------------------------------------------------------------ ELL SpMV
+template <typename T, typename TV>
+__global__ __launch_bounds__(kBlock) void k_ell_spmv(int nr, int pitch, int nzm, int nlog,
+                                                     const int* __restrict__ ja,
+                                                     const TV* __restrict__ val,
+                                                     const T* __restrict__ x, T alpha, T beta,
+                                                     T* __restrict__ y) {
+  const int lb = xcd_block(nlog);
+  if (lb >= nlog) return;
+  const int row = lb * kBlock + threadIdx.x;
+  const int r = row < nr ? row : nr - 1;
+  T acc = slot_dot<T, TV>(ja + r, val + r, pitch, nzm, x);
+  if (row < nr) store_y(y, row, alpha, acc, beta);
+}
+
+// ------------------------------------------------------------------ CSR SpMV
+// Vector CSR: G lanes per row (G = 1..64), the lanes stride the row and
+// combine through a lane butterfly.  Rows longer than long_thr are skipped
+// here and computed by k_csr_long (one wavefront per row).
+template <typename T, int G>
+__global__ __launch_bounds__(kBlock) void k_csr_vector(int nr, int nlog,
+                                                       const int* __restrict__ irp,
+                                                       const int* __restrict__ ja,
+                                                       const T* __restrict__ val,
+                                                       const T* __restrict__ x, T alpha, T beta,
+                                                       T* __restrict__ y, int long_thr) {
+  const int lb = xcd_block(nlog);
+  if (lb >= nlog) return;
+  const int gid = lb * (kBlock / G) + (int)threadIdx.x / G;
+  const int lane = threadIdx.x % G;
+  const int r = gid < nr ? gid : nr - 1;
+  int b = irp[r] - 1, e = irp[r + 1] - 1;
+  const bool skip = (e - b) > long_thr;
+  if (skip) e = b;
+  T acc = T(0);
+  int p = b + lane;
+  for (; p + 3 * G < e; p += 4 * G) {
+    int c0 = ld_stream(ja + p), c1 = ld_stream(ja + p + G);
+    int c2 = ld_stream(ja + p + 2 * G), c3 = ld_stream(ja + p + 3 * G);
+    T v0 = ld_stream(val + p), v1 = ld_stream(val + p + G);
+    T v2 = ld_stream(val + p + 2 * G), v3 = ld_stream(val + p + 3 * G);
+    T x0 = x[c0 - 1], x1 = x[c1 - 1], x2 = x[c2 - 1], x3 = x[c3 - 1];
+    acc = fma(v0, x0, acc);
+    acc = fma(v1, x1, acc);
+    acc = fma(v2, x2, acc);
+    acc = fma(v3, x3, acc);
+  }
+  for (; p < e; p += G) acc = fma(ld_stream(val + p), x[ld_stream(ja + p) - 1], acc);
+  if (G > 1) acc = group_sum<G>(acc);
+  if (gid < nr && lane == 0 && !skip) store_y(y, gid, alpha, acc, beta);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_csr_long(int nlong, const int* __restrict__ rows,
+                                                     const int* __restrict__ irp,
+                                                     const int* __restrict__ ja,
+                                                     const T* __restrict__ val,
+                                                     const T* __restrict__ x, T alpha, T beta,
+                                                     T* __restrict__ y) {
+  const int wid = blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int wr = wid < nlong ? wid : nlong - 1;
+  const int row = rows[wr];
+  const int b = irp[row] - 1, e = irp[row + 1] - 1;
+  T acc = T(0);
+  int p = b + lane;
+  for (; p + 3 * kWave < e; p += 4 * kWave) {
+    int c0 = ld_stream(ja + p), c1 = ld_stream(ja + p + kWave);
+    int c2 = ld_stream(ja + p + 2 * kWave), c3 = ld_stream(ja + p + 3 * kWave);
+    T v0 = ld_stream(val + p), v1 = ld_stream(val + p + kWave);
+    T v2 = ld_stream(val + p + 2 * kWave), v3 = ld_stream(val + p + 3 * kWave);
+    T x0 = x[c0 - 1], x1 = x[c1 - 1], x2 = x[c2 - 1], x3 = x[c3 - 1];
+    acc = fma(v0, x0, acc);
+    acc = fma(v1, x1, acc);
+    acc = fma(v2, x2, acc);
+    acc = fma(v3, x3, acc);
+  }
+  for (; p < e; p += kWave) acc = fma(ld_stream(val + p), x[ld_stream(ja + p) - 1], acc);
+  acc = wave_sum(acc);
+  if (wid < nlong && lane == 0) store_y(y, row, alpha, acc, beta);
+}
+
+// ------------------------------------------------------------------ fills
+// psi_d_convert_hll_from_coo + psi_d_xtr_ell_from_coo, CPU padding rule:
+// one thread per slot row; rows past nr in the last hack get ja = nr, val 0;
+// an empty row is padded with the first column of its hack's entries.
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_hll_fill(int nr, int hksz, int noffs,
+                                                     const int* __restrict__ irn,
+                                                     const int* __restrict__ hoffs,
+                                                     const int* __restrict__ idisp,
+                                                     const int* __restrict__ cja,
+                                                     const TV* __restrict__ cval,
+                                                     int* __restrict__ ja, TV* __restrict__ val,
+                                                     int* __restrict__ idiag) {
+  const int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (t >= (int64_t)noffs * hksz) return;
+  const int hk = (int)(t / hksz);
+  const int lane = (int)(t - (int64_t)hk * hksz);
+  const int i = hk * hksz + lane;  // 0-based row
+  const int off = hoffs[hk];
+  const int w = (hoffs[hk + 1] - off) / hksz;
+  if (i >= nr) {
+    for (int j = 0; j < w; j++) {
+      ja[off + (size_t)j * hksz + lane] = nr;
+      val[off + (size_t)j * hksz + lane] = TV(0);
+    }
+    return;
+  }
+  const int rs = irn[i];
+  const int p0 = idisp[i];
+  int dg = 0;
+  int ic = 0;
+  for (int j = 0; j < rs; j++) {
+    ic = cja[p0 + j];
+    if (ic == i + 1) dg = j + 1;
+    ja[off + (size_t)j * hksz + lane] = ic;
+    val[off + (size_t)j * hksz + lane] = cval[p0 + j];
+  }
+  if (rs == 0 && w > 0) ic = cja[idisp[hk * hksz]];
+  for (int j = rs; j < w; j++) {
+    ja[off + (size_t)j * hksz + lane] = ic;
+    val[off + (size_t)j * hksz + lane] = TV(0);
+  }
+  idiag[i] = dg;
+}
+
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_ell_fill(int nr, int pitch, int nzm,
+                                                     const int* __restrict__ irn,
+                                                     const int* __restrict__ idisp,
+                                                     const int* __restrict__ cja,
+                                                     const TV* __restrict__ cval,
+                                                     int* __restrict__ ja, TV* __restrict__ val,
+                                                     int* __restrict__ idiag) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= pitch) return;
+  if (i >= nr) {
+    for (int j = 0; j < nzm; j++) {
+      ja[(size_t)j * pitch + i] = 1;
+      val[(size_t)j * pitch + i] = TV(0);
+    }
+    return;
+  }
+  const int rs = irn[i];
+  const int p0 = idisp[i];
+  int dg = 0, ic = 0;
+  for (int j = 0; j < rs; j++) {
+    ic = cja[p0 + j];
+    if (ic == i + 1) dg = j + 1;
+    ja[(size_t)j * pitch + i] = ic;
+    val[(size_t)j * pitch + i] = cval[p0 + j];
+  }
+  if (rs == 0) ic = cja[0];
+  for (int j = rs; j < nzm; j++) {
+    ja[(size_t)j * pitch + i] = ic;
+    val[(size_t)j * pitch + i] = TV(0);
+  }
+  idiag[i] = dg;
+}
+
+// diagonal extraction: d[i] = val(slot idiag[i]) or 0 when the row has none
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_hll_diag(int nr, int hksz,
+                                                     const int* __restrict__ hkoffs,
+                                                     const int* __restrict__ idiag,
+                                                     const TV* __restrict__ val, TV* __restrict__ d) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nr) return;
+  const int hk = i / hksz, lane = i - hk * hksz;
+  const int s = idiag[i];
+  d[i] = s > 0 ? val[hkoffs[hk] + (size_t)(s - 1) * hksz + lane] : TV(0);
+}
+
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_ell_diag(int nr, int pitch, const int* __restrict__ idiag,
+                                                     const TV* __restrict__ val, TV* __restrict__ d) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nr) return;
+  const int s = idiag[i];
+  d[i] = s > 0 ? val[(size_t)(s - 1) * pitch + i] : TV(0);
+}
+
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_csr_diag(int nr, const int* __restrict__ irp,
+                                                     const int* __restrict__ ja,
+                                                     const TV* __restrict__ val, TV* __restrict__ d) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nr) return;
+  TV a = TV(0);
+  for (int p = irp[i] - 1; p < irp[i + 1] - 1; p++)
+    if (ja[p] == i + 1) a = val[p];
+  d[i] = a;
+}
+
+__global__ __launch_bounds__(kBlock) void k_d2s(int64_t n, const double* __restrict__ a,
+                                                float* __restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    b[i] = (float)a[i];
+}
+
+// ------------------------------------------------------------------ helpers
+static int max_index(const int* ja, int64_t n) {
+  int m = 0;
+  for (int64_t k = 0; k < n; k++) m = ja[k] > m ? ja[k] : m;
+  return m;
+}
+
+static int min_index(const int* ja, int64_t n) {
+  int m = 0x7fffffff;
+  for (int64_t k = 0; k < n; k++) m = ja[k] < m ? ja[k] : m;
+  return n ? m : 1;
+}
+
+static int check_vectors(const MultiVec* x, const MultiVec* y, int elem_type, int need_x,
+                         int need_y) {
+  if (!x || !y) return PSB_HIP_INVALID;
+  if (x->elem_type != elem_type || y->elem_type != elem_type) return PSB_HIP_INVALID;
+  if (x->size < need_x || y->size < need_y) return PSB_HIP_UNSUPPORTED;
+  if (x->count != y->count) return PSB_HIP_UNSUPPORTED;
+  if (x->v == y->v) return PSB_HIP_UNSUPPORTED;  // the product is not in place
+  return PSB_HIP_SUCCESS;
+}
+
+template <typename T, typename TV>
+static int hll_launch(const HllDev* m, const TV* val, const T* x, T alpha, T beta, T* y) {
+  hipStream_t s = ctx().stream;
+  int variant = g_hll_variant;
+  if (variant == 0) variant = 1;
+  if (variant == 2) {
+    int nlog = grid_for(m->rows, 2 * kBlock);
+    dim3 grid(xcd_grid(nlog));
+    if (m->hksz == 64)
+      hipLaunchKernelGGL((k_hll_spmv2<T, TV, 64>), grid, dim3(kBlock), 0, s, m->rows, m->hksz,
+                         nlog, m->hkoffs, m->ja, val, x, alpha, beta, y);
+    else if (m->hksz == 32)
+      hipLaunchKernelGGL((k_hll_spmv2<T, TV, 32>), grid, dim3(kBlock), 0, s, m->rows, m->hksz,
+                         nlog, m->hkoffs, m->ja, val, x, alpha, beta, y);
+    else
+      hipLaunchKernelGGL((k_hll_spmv2<T, TV, 0>), grid, dim3(kBlock), 0, s, m->rows, m->hksz,
+                         nlog, m->hkoffs, m->ja, val, x, alpha, beta, y);
+  } else {
+    int nlog = grid_for(m->rows, kBlock);
+    dim3 grid(xcd_grid(nlog));
+    if (m->hksz == 64)
+      hipLaunchKernelGGL((k_hll_spmv<T, TV, 64>), grid, dim3(kBlock), 0, s, m->rows, m->hksz,
+                         nlog, m->hkoffs, m->ja, val, x, alpha, beta, y);
+    else if (m->hksz == 32)
+      hipLaunchKernelGGL((k_hll_spmv<T, TV, 32>), grid, dim3(kBlock), 0, s, m->rows, m->hksz,
+                         nlog, m->hkoffs, m->ja, val, x, alpha, beta, y);
+    else
+      hipLaunchKernelGGL((k_hll_spmv<T, TV, 0>), grid, dim3(kBlock), 0, s, m->rows, m->hksz,
+                         nlog, m->hkoffs, m->ja, val, x, alpha, beta, y);
+  }
+  PSB_HIP_LAUNCH_CHECK();
+  return PSB_HIP_SUCCESS;
+}
+
+template <typename T>
+static int hll_spmm_launch(const HllDev* m, const T* val, const T* x, int ldx, T alpha, T beta,
+                           T* y, int ldy, int ncols) {
+  hipStream_t s = ctx().stream;
+  int nlog = grid_for(m->rows, kBlock);
+  dim3 grid(xcd_grid(nlog));
+  for (int c0 = 0; c0 < ncols; c0 += 16) {
+    int nc = ncols - c0 < 16 ? ncols - c0 : 16;
+    const T* xc = x + (size_t)c0 * ldx;
+    T* yc = y + (size_t)c0 * ldy;
+    if (nc > 8)
+      hipLaunchKernelGGL((k_hll_spmm<T, 16>), grid, dim3(kBlock), 0, s, m->rows, m->hksz, nlog,
+                         m->hkoffs, m->ja, val, xc, ldx, alpha, beta, yc, ldy, nc);
+    else if (nc > 4)
+      hipLaunchKernelGGL((k_hll_spmm<T, 8>), grid, dim3(kBlock), 0, s, m->rows, m->hksz, nlog,
+                         m->hkoffs, m->ja, val, xc, ldx, alpha, beta, yc, ldy, nc);
+    else if (nc > 1)
+      hipLaunchKernelGGL((k_hll_spmm<T, 4>), grid, dim3(kBlock), 0, s, m->rows, m->hksz, nlog,
+                         m->hkoffs, m->ja, val, xc, ldx, alpha, beta, yc, ldy, nc);
+    else
+      hipLaunchKernelGGL((k_hll_spmm<T, 1>), grid, dim3(kBlock), 0, s, m->rows, m->hksz, nlog,
+                         m->hkoffs, m->ja, val, xc, ldx, alpha, beta, yc, ldy, nc);
+    PSB_HIP_LAUNCH_CHECK();
+  }
+  return PSB_HIP_SUCCESS;
+}
+
+template <typename T>
+static int csr_launch(CsrDev* m, const T* x, T alpha, T beta, T* y) {
+  hipStream_t s = ctx().stream;
+  const T* val = (const T*)m->val;
+  int g = m->lanes;
+  int nlog = grid_for((int64_t)m->rows * g, kBlock);
+  dim3 grid(xcd_grid(nlog));
+#define PSB_CSR_CASE(G)                                                                         \
+  case G:                                                                                       \
+    hipLaunchKernelGGL((k_csr_vector<T, G>), grid, dim3(kBlock), 0, s, m->rows, nlog, m->irp,  \
+                       m->ja, val, x, alpha, beta, y, m->long_thr);                             \
+    break;
+  switch (g) {
+    PSB_CSR_CASE(1)
+    PSB_CSR_CASE(2)
+    PSB_CSR_CASE(4)
+    PSB_CSR_CASE(8)
+    PSB_CSR_CASE(16)
+    PSB_CSR_CASE(32)
+    PSB_CSR_CASE(64)
+    default: return PSB_HIP_INVALID;
+  }
+#undef PSB_CSR_CASE
+  PSB_HIP_LAUNCH_CHECK();
+  if (m->nlong > 0) {
+    int nb = grid_for(m->nlong, kBlock / kWave);
+    hipLaunchKernelGGL((k_csr_long<T>), dim3(nb), dim3(kBlock), 0, s, m->nlong, m->long_rows,
+                       m->irp, m->ja, val, x, alpha, beta, y);
+    PSB_HIP_LAUNCH_CHECK();
+  }
+  return PSB_HIP_SUCCESS;
+}
+
+}  // namespace psbhip
+
+using namespace psbhip;
+
+#define PSB_ENSURE_INIT()          \
+  do {                             \
+    int rc_ = ensure_init();       \
+    if (rc_) return rc_;           \
+  } while (0)
+
+extern "C" {
+
+// ================================================================== HLL
+int psb_hip_hll_alloc(void** dmat, int hksz, int rows, int nzeros, int allocsize, int elem_type,
+                      int first_index) {
+  PSB_ENSURE_INIT();
+  *dmat = nullptr;
+  if (elem_type != PSB_HIP_TYPE_DOUBLE && elem_type != PSB_HIP_TYPE_FLOAT)
+    return set_error(PSB_HIP_UNSUPPORTED, __func__, "element type");
+  if (first_index != 1) return set_error(PSB_HIP_UNSUPPORTED, __func__, "first_index must be 1");
+  if (hksz < 1 || rows < 0 || allocsize < 0) return set_error(PSB_HIP_INVALID, __func__, "sizes");
+  HllDev* m = new HllDev();
+  m->elem_type = elem_type;
+  m->hksz = hksz;
+  m->rows = rows;
+  m->nzeros = nzeros;
+  m->allocsize = allocsize;
+  m->noffs = (rows + hksz - 1) / hksz;
+  m->first_index = first_index;
+  m->max_col = 0;
+  size_t es = elem_size(elem_type);
+  hipError_t e = hipSuccess;
+  size_t asz = (size_t)(allocsize > 0 ? allocsize : 1);
+  if (e == hipSuccess) e = hipMalloc(&m->val, asz * es);
+  if (e == hipSuccess) e = hipMalloc(&m->ja, asz * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&m->hkoffs, sizeof(int) * (m->noffs + 1));
+  if (e == hipSuccess) e = hipMalloc(&m->irn, sizeof(int) * (rows > 0 ? rows : 1));
+  if (e == hipSuccess) e = hipMalloc(&m->idiag, sizeof(int) * (rows > 0 ? rows : 1));
+  if (e != hipSuccess) {
+    psb_hip_hll_free(m);
+    return set_error(PSB_HIP_OUTOFMEM, __func__, hipGetErrorString(e));
+  }
+  *dmat = m;
+  return PSB_HIP_SUCCESS;
+}
+
+void psb_hip_hll_free(void* dmat) {
+  if (!dmat) return;
+  HllDev* m = (HllDev*)dmat;
+  if (ctx().initialized) hipStreamSynchronize(ctx().stream);
+  hipFree(m->val);
+  hipFree(m->ja);
+  hipFree(m->hkoffs);
+  hipFree(m->irn);
+  hipFree(m->idiag);
+  hipFree(m->val32);
+  delete m;
+}
+
+int psb_hip_hll_write(void* dmat, const void* val, const int* ja, const int* hkoffs,
+                      const int* irn, const int* idiag) {
+  PSB_ENSURE_INIT();
+  HllDev* m = (HllDev*)dmat;
+  hipStream_t s = ctx().stream;
+  // the hack offsets must describe the allocation exactly
+  if (hkoffs[0] != 0 || hkoffs[m->noffs] != m->allocsize)
+    return set_error(PSB_HIP_INVALID, __func__, "hkoffs inconsistent with allocsize");
+  for (int h = 0; h < m->noffs; h++)
+    if (hkoffs[h + 1] < hkoffs[h] || (hkoffs[h + 1] - hkoffs[h]) % m->hksz != 0)
+      return set_error(PSB_HIP_INVALID, __func__, "hkoffs not multiples of hksz");
+  if (m->allocsize > 0 && min_index(ja, m->allocsize) < 1)
+    return set_error(PSB_HIP_INVALID, __func__, "ja < 1");
+  m->max_col = max_index(ja, m->allocsize);
+  size_t es = elem_size(m->elem_type);
+  PSB_HIP_CHECK(hipMemcpyAsync(m->val, val, (size_t)m->allocsize * es, hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(m->ja, ja, (size_t)m->allocsize * 4, hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(m->hkoffs, hkoffs, (size_t)(m->noffs + 1) * 4,
+                               hipMemcpyHostToDevice, s));
+  if (m->rows > 0) {
+    PSB_HIP_CHECK(hipMemcpyAsync(m->irn, irn, (size_t)m->rows * 4, hipMemcpyHostToDevice, s));
+    PSB_HIP_CHECK(hipMemcpyAsync(m->idiag, idiag, (size_t)m->rows * 4, hipMemcpyHostToDevice, s));
+  }
+  PSB_HIP_CHECK(hipStreamSynchronize(s));
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_hll_read(void* dmat, void* val, int* ja, int* hkoffs, int* irn, int* idiag) {
+  PSB_ENSURE_INIT();
+  HllDev* m = (HllDev*)dmat;
+  hipStream_t s = ctx().stream;
+  size_t es = elem_size(m->elem_type);
+  if (val)
+    PSB_HIP_CHECK(hipMemcpyAsync(val, m->val, (size_t)m->allocsize * es, hipMemcpyDeviceToHost, s));
+  if (ja)
+    PSB_HIP_CHECK(hipMemcpyAsync(ja, m->ja, (size_t)m->allocsize * 4, hipMemcpyDeviceToHost, s));
+  if (hkoffs)
+    PSB_HIP_CHECK(hipMemcpyAsync(hkoffs, m->hkoffs, (size_t)(m->noffs + 1) * 4,
+                                 hipMemcpyDeviceToHost, s));
+  if (irn && m->rows > 0)
+    PSB_HIP_CHECK(hipMemcpyAsync(irn, m->irn, (size_t)m->rows * 4, hipMemcpyDeviceToHost, s));
+  if (idiag && m->rows > 0)
+    PSB_HIP_CHECK(hipMemcpyAsync(idiag, m->idiag, (size_t)m->rows * 4, hipMemcpyDeviceToHost, s));
+  PSB_HIP_CHECK(hipStreamSynchronize(s));
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_hll_get_params(void* dmat, int* hksz, int* rows, int* nzeros, int* allocsize,
+                           int* hack_offs_len, int* first_index, int* elem_type) {
+  HllDev* m = (HllDev*)dmat;
+  if (!m) return PSB_HIP_INVALID;
+  if (hksz) *hksz = m->hksz;
+  if (rows) *rows = m->rows;
+  if (nzeros) *nzeros = m->nzeros;
+  if (allocsize) *allocsize = m->allocsize;
+  if (hack_offs_len) *hack_offs_len = m->noffs + 1;
+  if (first_index) *first_index = m->first_index;
+  if (elem_type) *elem_type = m->elem_type;
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_hll_copy_coo(int nr, int nc, int nza, int hksz, int noffs, int isz, const int* irn,
+                         const int* hoffs, const int* idisp, const int* ja, const void* val,
+                         void* dmat) {
+  PSB_ENSURE_INIT();
+  HllDev* m = (HllDev*)dmat;
+  if (m->rows != nr || m->hksz != hksz || m->noffs != noffs || m->allocsize != isz)
+    return set_error(PSB_HIP_INVALID, __func__, "matrix shape mismatch");
+  if (hoffs[0] != 0 || hoffs[noffs] != isz || idisp[0] != 0 || idisp[nr] != nza)
+    return set_error(PSB_HIP_INVALID, __func__, "offsets inconsistent");
+  for (int h = 0; h < noffs; h++) {
+    int i0 = h * hksz, i1 = std::min(nr, i0 + hksz), mx = 0;
+    for (int i = i0; i < i1; i++) mx = std::max(mx, irn[i]);
+    if (hoffs[h + 1] - hoffs[h] != mx * hksz)
+      return set_error(PSB_HIP_INVALID, __func__, "hack offsets do not match row counts");
+  }
+  for (int i = 0; i < nr; i++)
+    if (idisp[i + 1] - idisp[i] != irn[i])
+      return set_error(PSB_HIP_INVALID, __func__, "idisp does not match irn");
+  int mn = min_index(ja, nza), mx = max_index(ja, nza);
+  if (nza > 0 && (mn < 1 || mx > nc)) return set_error(PSB_HIP_INVALID, __func__, "ja out of range");
+  m->max_col = std::max(mx, nr);  // padding of the last hack uses ja = nr
+  hipStream_t s = ctx().stream;
+  size_t es = elem_size(m->elem_type);
+  int *d_irn = nullptr, *d_hoffs = nullptr, *d_idisp = nullptr, *d_ja = nullptr;
+  void* d_val = nullptr;
+  hipError_t e = hipSuccess;
+  size_t nzb = (size_t)(nza > 0 ? nza : 1);
+  if (e == hipSuccess) e = hipMalloc(&d_irn, sizeof(int) * (nr > 0 ? nr : 1));
+  if (e == hipSuccess) e = hipMalloc(&d_hoffs, sizeof(int) * (noffs + 1));
+  if (e == hipSuccess) e = hipMalloc(&d_idisp, sizeof(int) * (nr + 1));
+  if (e == hipSuccess) e = hipMalloc(&d_ja, sizeof(int) * nzb);
+  if (e == hipSuccess) e = hipMalloc(&d_val, es * nzb);
+  if (e != hipSuccess) {
+    hipFree(d_irn); hipFree(d_hoffs); hipFree(d_idisp); hipFree(d_ja); hipFree(d_val);
+    return set_error(PSB_HIP_OUTOFMEM, __func__, hipGetErrorString(e));
+  }
+  PSB_HIP_CHECK(hipMemcpyAsync(d_irn, irn, sizeof(int) * nr, hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(d_hoffs, hoffs, sizeof(int) * (noffs + 1), hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(d_idisp, idisp, sizeof(int) * (nr + 1), hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(d_ja, ja, sizeof(int) * nza, hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(d_val, val, es * nza, hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(m->hkoffs, hoffs, sizeof(int) * (noffs + 1), hipMemcpyHostToDevice, s));
+  if (nr > 0)
+    PSB_HIP_CHECK(hipMemcpyAsync(m->irn, irn, sizeof(int) * nr, hipMemcpyHostToDevice, s));
+  int64_t nthreads = (int64_t)noffs * hksz;
+  if (nthreads > 0) {
+    dim3 grid((unsigned)((nthreads + kBlock - 1) / kBlock));
+    if (m->elem_type == PSB_HIP_TYPE_DOUBLE)
+      hipLaunchKernelGGL((k_hll_fill<double>), grid, dim3(kBlock), 0, s, nr, hksz, noffs, d_irn,
+                         d_hoffs, d_idisp, d_ja, (const double*)d_val, m->ja, (double*)m->val,
+                         m->idiag);
+    else
+      hipLaunchKernelGGL((k_hll_fill<float>), grid, dim3(kBlock), 0, s, nr, hksz, noffs, d_irn,
+                         d_hoffs, d_idisp, d_ja, (const float*)d_val, m->ja, (float*)m->val,
+                         m->idiag);
+    PSB_HIP_LAUNCH_CHECK();
+  }
+  PSB_HIP_CHECK(hipStreamSynchronize(s));
+  hipFree(d_irn); hipFree(d_hoffs); hipFree(d_idisp); hipFree(d_ja); hipFree(d_val);
+  m->nzeros = nza;
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_hll_spmv(void* dmat, double alpha, void* xv, double beta, void* yv) {
+  PSB_ENSURE_INIT();
+  HllDev* m = (HllDev*)dmat;
+  MultiVec *x = (MultiVec*)xv, *y = (MultiVec*)yv;
+  int rc = check_vectors(x, y, m->elem_type, m->max_col, m->rows);
+  if (rc) return set_error(rc, __func__, "x/y type, size or aliasing");
+  if (m->rows == 0) return PSB_HIP_SUCCESS;
+  if (x->count == 1) {
+    if (m->elem_type == PSB_HIP_TYPE_DOUBLE)
+      return hll_launch<double, double>(m, (const double*)m->val, (const double*)x->v, alpha,
+                                        beta, (double*)y->v);
+    return hll_launch<float, float>(m, (const float*)m->val, (const float*)x->v, (float)alpha,
+                                    (float)beta, (float*)y->v);
+  }
+  if (m->elem_type == PSB_HIP_TYPE_DOUBLE)
+    return hll_spmm_launch<double>(m, (const double*)m->val, (const double*)x->v, x->pitch, alpha,
+                                   beta, (double*)y->v, y->pitch, x->count);
+  return hll_spmm_launch<float>(m, (const float*)m->val, (const float*)x->v, x->pitch,
+                                (float)alpha, (float)beta, (float*)y->v, y->pitch, x->count);
+}
+
+int psb_hip_hll_make_single(void* dmat) {
+  PSB_ENSURE_INIT();
+  HllDev* m = (HllDev*)dmat;
+  if (m->elem_type != PSB_HIP_TYPE_DOUBLE) return PSB_HIP_UNSUPPORTED;
+  if (!m->val32) {
+    PSB_HIP_CHECK(hipMalloc(&m->val32, sizeof(float) * (size_t)(m->allocsize > 0 ? m->allocsize : 1)));
+  }
+  if (m->allocsize > 0) {
+    hipLaunchKernelGGL(k_d2s, dim3(stream_grid(m->allocsize)), dim3(kBlock), 0, ctx().stream,
+                       (int64_t)m->allocsize, (const double*)m->val, m->val32);
+    PSB_HIP_LAUNCH_CHECK();
+  }
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_hll_spmv_single(void* dmat, double alpha, void* xv, double beta, void* yv) {
+  PSB_ENSURE_INIT();
+  HllDev* m = (HllDev*)dmat;
+  if (!m->val32) return set_error(PSB_HIP_INVALID, __func__, "no single-precision copy");
+  MultiVec *x = (MultiVec*)xv, *y = (MultiVec*)yv;
+  int rc = check_vectors(x, y, PSB_HIP_TYPE_FLOAT, m->max_col, m->rows);
+  if (rc) return set_error(rc, __func__, "x/y type, size or aliasing");
+  if (m->rows == 0) return PSB_HIP_SUCCESS;
+  if (x->count != 1) return PSB_HIP_UNSUPPORTED;
+  return hll_launch<float, float>(m, m->val32, (const float*)x->v, (float)alpha, (float)beta,
+                                  (float*)y->v);
+}
+
+int psb_hip_hll_get_diag(void* dmat, void* dv) {
+  PSB_ENSURE_INIT();
+  HllDev* m = (HllDev*)dmat;
+  MultiVec* d = (MultiVec*)dv;
+  if (d->elem_type != m->elem_type || d->size < m->rows) return PSB_HIP_INVALID;
+  if (m->rows == 0) return PSB_HIP_SUCCESS;
+  dim3 grid(grid_for(m->rows, kBlock));
+  if (m->elem_type == PSB_HIP_TYPE_DOUBLE)
+    hipLaunchKernelGGL((k_hll_diag<double>), grid, dim3(kBlock), 0, ctx().stream, m->rows,
+                       m->hksz, m->hkoffs, m->idiag, (const double*)m->val, (double*)d->v);
+  else
+    hipLaunchKernelGGL((k_hll_diag<float>), grid, dim3(kBlock), 0, ctx().stream, m->rows,
+                       m->hksz, m->hkoffs, m->idiag, (const float*)m->val, (float*)d->v);
+  PSB_HIP_LAUNCH_CHECK();
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_hll_set_variant(int variant) {
+  int old = g_hll_variant;
+  if (variant >= 0 && variant <= 2) g_hll_variant = variant;
+  return old;
+}
+
+// ================================================================== ELL
+int psb_hip_ell_alloc(void** dmat, int rows, int max_row_size, int nnzeros, int elem_type,
+                      int first_index) {
+  PSB_ENSURE_INIT();
+  *dmat = nullptr;
+  if (elem_type != PSB_HIP_TYPE_DOUBLE && elem_type != PSB_HIP_TYPE_FLOAT)
+    return set_error(PSB_HIP_UNSUPPORTED, __func__, "element type");
+  if (first_index != 1) return set_error(PSB_HIP_UNSUPPORTED, __func__, "first_index must be 1");
+  if (rows < 0 || max_row_size < 0) return set_error(PSB_HIP_INVALID, __func__, "sizes");
+  EllDev* m = new EllDev();
+  m->elem_type = elem_type;
+  m->rows = rows;
+  m->pitch = ((rows + 31) / 32) * 32;  // cuda/spgpu/ell.c:33-37
+  m->nzm = max_row_size;
+  m->nnz = nnzeros;
+  m->first_index = first_index;
+  size_t slots = (size_t)m->pitch * m->nzm;
+  if (slots == 0) slots = 1;
+  size_t es = elem_size(elem_type);
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = hipMalloc(&m->val, slots * es);
+  if (e == hipSuccess) e = hipMalloc(&m->ja, slots * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&m->irn, sizeof(int) * (rows > 0 ? rows : 1));
+  if (e == hipSuccess) e = hipMalloc(&m->idiag, sizeof(int) * (rows > 0 ? rows : 1));
+  if (e != hipSuccess) {
+    psb_hip_ell_free(m);
+    return set_error(PSB_HIP_OUTOFMEM, __func__, hipGetErrorString(e));
+  }
+  *dmat = m;
+  return PSB_HIP_SUCCESS;
+}
+
+void psb_hip_ell_free(void* dmat) {
+  if (!dmat) return;
+  EllDev* m = (EllDev*)dmat;
+  if (ctx().initialized) hipStreamSynchronize(ctx().stream);
+  hipFree(m->val);
+  hipFree(m->ja);
+  hipFree(m->irn);
+  hipFree(m->idiag);
+  delete m;
+}
+
+int psb_hip_ell_pitch(void* dmat) { return dmat ? ((EllDev*)dmat)->pitch : -1; }
+int psb_hip_ell_max_row_size(void* dmat) { return dmat ? ((EllDev*)dmat)->nzm : -1; }
+
+int psb_hip_ell_write(void* dmat, const void* val, const int* ja, int ldv, const int* irn,
+                      const int* idiag) {
+  PSB_ENSURE_INIT();
+  EllDev* m = (EllDev*)dmat;
+  if (ldv < m->rows) return set_error(PSB_HIP_INVALID, __func__, "ldv < rows");
+  hipStream_t s = ctx().stream;
+  size_t es = elem_size(m->elem_type);
+  // validate the stored column indices of the real rows
+  int mx = 0;
+  for (int j = 0; j < m->nzm; j++)
+    for (int i = 0; i < m->rows; i++) {
+      int c = ja[(size_t)j * ldv + i];
+      if (c < 1) return set_error(PSB_HIP_INVALID, __func__, "ja < 1");
+      mx = c > mx ? c : mx;
+    }
+  m->max_col = mx;
+  if (m->rows > 0 && m->nzm > 0) {
+    // pad rows rows..pitch-1 with ja = 1, val = 0 through a zeroed staging copy
+    std::vector<char> hv((size_t)m->pitch * m->nzm * es, 0);
+    std::vector<int> hj((size_t)m->pitch * m->nzm, 1);
+    for (int j = 0; j < m->nzm; j++) {
+      memcpy(&hv[(size_t)j * m->pitch * es], (const char*)val + (size_t)j * ldv * es,
+             (size_t)m->rows * es);
+      memcpy(&hj[(size_t)j * m->pitch], ja + (size_t)j * ldv, (size_t)m->rows * 4);
+    }
+    PSB_HIP_CHECK(hipMemcpyAsync(m->val, hv.data(), hv.size(), hipMemcpyHostToDevice, s));
+    PSB_HIP_CHECK(hipMemcpyAsync(m->ja, hj.data(), hj.size() * 4, hipMemcpyHostToDevice, s));
+    PSB_HIP_CHECK(hipStreamSynchronize(s));
+  }
+  if (m->rows > 0) {
+    PSB_HIP_CHECK(hipMemcpyAsync(m->irn, irn, (size_t)m->rows * 4, hipMemcpyHostToDevice, s));
+    PSB_HIP_CHECK(hipMemcpyAsync(m->idiag, idiag, (size_t)m->rows * 4, hipMemcpyHostToDevice, s));
+  }
+  PSB_HIP_CHECK(hipStreamSynchronize(s));
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_ell_read(void* dmat, void* val, int* ja, int ldv, int* irn, int* idiag) {
+  PSB_ENSURE_INIT();
+  EllDev* m = (EllDev*)dmat;
+  if (ldv < m->rows) return PSB_HIP_INVALID;
+  hipStream_t s = ctx().stream;
+  size_t es = elem_size(m->elem_type);
+  if (m->nzm > 0 && m->rows > 0) {
+    if (val)
+      PSB_HIP_CHECK(hipMemcpy2DAsync(val, (size_t)ldv * es, m->val, (size_t)m->pitch * es,
+                                     (size_t)m->rows * es, m->nzm, hipMemcpyDeviceToHost, s));
+    if (ja)
+      PSB_HIP_CHECK(hipMemcpy2DAsync(ja, (size_t)ldv * 4, m->ja, (size_t)m->pitch * 4,
+                                     (size_t)m->rows * 4, m->nzm, hipMemcpyDeviceToHost, s));
+  }
+  if (irn && m->rows > 0)
+    PSB_HIP_CHECK(hipMemcpyAsync(irn, m->irn, (size_t)m->rows * 4, hipMemcpyDeviceToHost, s));
+  if (idiag && m->rows > 0)
+    PSB_HIP_CHECK(hipMemcpyAsync(idiag, m->idiag, (size_t)m->rows * 4, hipMemcpyDeviceToHost, s));
+  PSB_HIP_CHECK(hipStreamSynchronize(s));
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_ell_copy_coo(int nr, int nc, int nza, const int* irn, const int* idisp,
+                         const int* ja, const void* val, void* dmat) {
+  PSB_ENSURE_INIT();
+  EllDev* m = (EllDev*)dmat;
+  if (m->rows != nr) return set_error(PSB_HIP_INVALID, __func__, "rows mismatch");
+  int mxr = 0;
+  for (int i = 0; i < nr; i++) {
+    mxr = std::max(mxr, irn[i]);
+    if (idisp[i + 1] - idisp[i] != irn[i])
+      return set_error(PSB_HIP_INVALID, __func__, "idisp does not match irn");
+  }
+  if (idisp[0] != 0 || idisp[nr] != nza) return set_error(PSB_HIP_INVALID, __func__, "idisp");
+  if (mxr != m->nzm) return set_error(PSB_HIP_INVALID, __func__, "max row size mismatch");
+  int mn = min_index(ja, nza), mx = max_index(ja, nza);
+  if (nza > 0 && (mn < 1 || mx > nc)) return set_error(PSB_HIP_INVALID, __func__, "ja out of range");
+  m->max_col = std::max(mx, 1);
+  hipStream_t s = ctx().stream;
+  size_t es = elem_size(m->elem_type);
+  int *d_irn = nullptr, *d_idisp = nullptr, *d_ja = nullptr;
+  void* d_val = nullptr;
+  size_t nzb = (size_t)(nza > 0 ? nza : 1);
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = hipMalloc(&d_irn, sizeof(int) * (nr > 0 ? nr : 1));
+  if (e == hipSuccess) e = hipMalloc(&d_idisp, sizeof(int) * (nr + 1));
+  if (e == hipSuccess) e = hipMalloc(&d_ja, sizeof(int) * nzb);
+  if (e == hipSuccess) e = hipMalloc(&d_val, es * nzb);
+  if (e != hipSuccess) {
+    hipFree(d_irn); hipFree(d_idisp); hipFree(d_ja); hipFree(d_val);
+    return set_error(PSB_HIP_OUTOFMEM, __func__, hipGetErrorString(e));
+  }
+  PSB_HIP_CHECK(hipMemcpyAsync(d_irn, irn, sizeof(int) * nr, hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(d_idisp, idisp, sizeof(int) * (nr + 1), hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(d_ja, ja, sizeof(int) * nza, hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(d_val, val, es * nza, hipMemcpyHostToDevice, s));
+  if (nr > 0) PSB_HIP_CHECK(hipMemcpyAsync(m->irn, irn, sizeof(int) * nr, hipMemcpyHostToDevice, s));
+  if (m->pitch > 0 && m->nzm > 0) {
+    dim3 grid(grid_for(m->pitch, kBlock));
+    if (m->elem_type == PSB_HIP_TYPE_DOUBLE)
+      hipLaunchKernelGGL((k_ell_fill<double>), grid, dim3(kBlock), 0, s, nr, m->pitch, m->nzm,
+                         d_irn, d_idisp, d_ja, (const double*)d_val, m->ja, (double*)m->val,
+                         m->idiag);
+    else
+      hipLaunchKernelGGL((k_ell_fill<float>), grid, dim3(kBlock), 0, s, nr, m->pitch, m->nzm,
+                         d_irn, d_idisp, d_ja, (const float*)d_val, m->ja, (float*)m->val,
+                         m->idiag);
+    PSB_HIP_LAUNCH_CHECK();
+  } else if (nr > 0) {
+    PSB_HIP_CHECK(hipMemsetAsync(m->idiag, 0, sizeof(int) * nr, s));
+  }
+  PSB_HIP_CHECK(hipStreamSynchronize(s));
+  hipFree(d_irn); hipFree(d_idisp); hipFree(d_ja); hipFree(d_val);
+  m->nnz = nza;
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_ell_get_diag(void* dmat, void* dv) {
+  PSB_ENSURE_INIT();
+  EllDev* m = (EllDev*)dmat;
+  MultiVec* d = (MultiVec*)dv;
+  if (d->elem_type != m->elem_type || d->size < m->rows) return PSB_HIP_INVALID;
+  if (m->rows == 0) return PSB_HIP_SUCCESS;
+  if (m->nzm == 0) {
+    PSB_HIP_CHECK(hipMemsetAsync(d->v, 0, elem_size(m->elem_type) * m->rows, ctx().stream));
+    return PSB_HIP_SUCCESS;
+  }
+  dim3 grid(grid_for(m->rows, kBlock));
+  if (m->elem_type == PSB_HIP_TYPE_DOUBLE)
+    hipLaunchKernelGGL((k_ell_diag<double>), grid, dim3(kBlock), 0, ctx().stream, m->rows,
+                       m->pitch, m->idiag, (const double*)m->val, (double*)d->v);
+  else
+    hipLaunchKernelGGL((k_ell_diag<float>), grid, dim3(kBlock), 0, ctx().stream, m->rows,
+                       m->pitch, m->idiag, (const float*)m->val, (float*)d->v);
+  PSB_HIP_LAUNCH_CHECK();
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_ell_spmv(void* dmat, double alpha, void* xv, double beta, void* yv) {
+  PSB_ENSURE_INIT();
+  EllDev* m = (EllDev*)dmat;
+  MultiVec *x = (MultiVec*)xv, *y = (MultiVec*)yv;
+  int rc = check_vectors(x, y, m->elem_type, m->max_col, m->rows);
+  if (rc) return set_error(rc, __func__, "x/y type, size or aliasing");
+  if (m->rows == 0) return PSB_HIP_SUCCESS;
+  hipStream_t s = ctx().stream;
+  int nlog = grid_for(m->rows, kBlock);
+  dim3 grid(xcd_grid(nlog));
+  // every column of x/y (cuda/elldev.c:195-215 loops count_)
+  for (int c = 0; c < x->count; c++) {
+    if (m->elem_type == PSB_HIP_TYPE_DOUBLE)
+      hipLaunchKernelGGL((k_ell_spmv<double, double>), grid, dim3(kBlock), 0, s, m->rows, m->pitch,
+                         m->nzm, nlog, m->ja, (const double*)m->val,
+                         (const double*)x->v + (size_t)c * x->pitch, alpha, beta,
+                         (double*)y->v + (size_t)c * y->pitch);
+    else
+      hipLaunchKernelGGL((k_ell_spmv<float, float>), grid, dim3(kBlock), 0, s, m->rows, m->pitch,
+                         m->nzm, nlog, m->ja, (const float*)m->val,
+                         (const float*)x->v + (size_t)c * x->pitch, (float)alpha, (float)beta,
+                         (float*)y->v + (size_t)c * y->pitch);
+    PSB_HIP_LAUNCH_CHECK();
+  }
+  return PSB_HIP_SUCCESS;
+}
+
+// ================================================================== CSR
+int psb_hip_csr_alloc(void** dmat, int rows, int cols, int nnz, int elem_type, int first_index) {
+  PSB_ENSURE_INIT();
+  *dmat = nullptr;
+  if (elem_type != PSB_HIP_TYPE_DOUBLE && elem_type != PSB_HIP_TYPE_FLOAT)
+    return set_error(PSB_HIP_UNSUPPORTED, __func__, "element type");
+  if (first_index != 1) return set_error(PSB_HIP_UNSUPPORTED, __func__, "first_index must be 1");
+  if (rows < 0 || cols < 0 || nnz < 0) return set_error(PSB_HIP_INVALID, __func__, "sizes");
+  CsrDev* m = new CsrDev();
+  m->elem_type = elem_type;
+  m->rows = rows;
+  m->cols = cols;
+  m->nnz = nnz;
+  m->first_index = first_index;
+  m->lanes = 8;
+  m->long_thr = 0x7fffffff;
+  size_t es = elem_size(elem_type);
+  size_t nzb = (size_t)(nnz > 0 ? nnz : 1);
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = hipMalloc(&m->irp, sizeof(int) * (rows + 1));
+  if (e == hipSuccess) e = hipMalloc(&m->ja, sizeof(int) * nzb);
+  if (e == hipSuccess) e = hipMalloc(&m->val, es * nzb);
+  if (e != hipSuccess) {
+    psb_hip_csr_free(m);
+    return set_error(PSB_HIP_OUTOFMEM, __func__, hipGetErrorString(e));
+  }
+  *dmat = m;
+  return PSB_HIP_SUCCESS;
+}
+
+void psb_hip_csr_free(void* dmat) {
+  if (!dmat) return;
+  CsrDev* m = (CsrDev*)dmat;
+  if (ctx().initialized) hipStreamSynchronize(ctx().stream);
+  hipFree(m->irp);
+  hipFree(m->ja);
+  hipFree(m->val);
+  hipFree(m->long_rows);
+  delete m;
+}
+
+int psb_hip_csr_write(void* dmat, const int* irp, const int* ja, const void* val) {
+  PSB_ENSURE_INIT();
+  CsrDev* m = (CsrDev*)dmat;
+  if (irp[0] != 1 || irp[m->rows] - 1 != m->nnz)
+    return set_error(PSB_HIP_INVALID, __func__, "irp inconsistent with nnz");
+  for (int i = 0; i < m->rows; i++)
+    if (irp[i + 1] < irp[i]) return set_error(PSB_HIP_INVALID, __func__, "irp not monotone");
+  int mn = min_index(ja, m->nnz), mx = max_index(ja, m->nnz);
+  if (m->nnz > 0 && (mn < 1 || mx > m->cols))
+    return set_error(PSB_HIP_INVALID, __func__, "ja out of range");
+  m->max_col = mx;
+  // analysis: lanes per row from the mean row length, long rows listed
+  double mean = m->rows > 0 ? (double)m->nnz / m->rows : 0.0;
+  int g = 1;
+  while (g < 64 && g < mean) g <<= 1;
+  m->lanes = g;
+  m->long_thr = std::max(256, 8 * g);
+  std::vector<int> longs;
+  for (int i = 0; i < m->rows; i++)
+    if (irp[i + 1] - irp[i] > m->long_thr) longs.push_back(i);
+  m->nlong = (int)longs.size();
+  hipStream_t s = ctx().stream;
+  hipFree(m->long_rows);
+  m->long_rows = nullptr;
+  if (m->nlong > 0) {
+    PSB_HIP_CHECK(hipMalloc(&m->long_rows, sizeof(int) * m->nlong));
+    PSB_HIP_CHECK(hipMemcpyAsync(m->long_rows, longs.data(), sizeof(int) * m->nlong,
+                                 hipMemcpyHostToDevice, s));
+  }
+  size_t es = elem_size(m->elem_type);
+  PSB_HIP_CHECK(hipMemcpyAsync(m->irp, irp, sizeof(int) * (m->rows + 1), hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(m->ja, ja, sizeof(int) * (size_t)m->nnz, hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(m->val, val, es * (size_t)m->nnz, hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipStreamSynchronize(s));
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_csr_read(void* dmat, int* irp, int* ja, void* val) {
+  PSB_ENSURE_INIT();
+  CsrDev* m = (CsrDev*)dmat;
+  hipStream_t s = ctx().stream;
+  size_t es = elem_size(m->elem_type);
+  if (irp)
+    PSB_HIP_CHECK(hipMemcpyAsync(irp, m->irp, sizeof(int) * (m->rows + 1), hipMemcpyDeviceToHost, s));
+  if (ja) PSB_HIP_CHECK(hipMemcpyAsync(ja, m->ja, sizeof(int) * (size_t)m->nnz, hipMemcpyDeviceToHost, s));
+  if (val) PSB_HIP_CHECK(hipMemcpyAsync(val, m->val, es * (size_t)m->nnz, hipMemcpyDeviceToHost, s));
+  PSB_HIP_CHECK(hipStreamSynchronize(s));
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_csr_get_diag(void* dmat, void* dv) {
+  PSB_ENSURE_INIT();
+  CsrDev* m = (CsrDev*)dmat;
+  MultiVec* d = (MultiVec*)dv;
+  if (d->elem_type != m->elem_type || d->size < m->rows) return PSB_HIP_INVALID;
+  if (m->rows == 0) return PSB_HIP_SUCCESS;
+  dim3 grid(grid_for(m->rows, kBlock));
+  if (m->elem_type == PSB_HIP_TYPE_DOUBLE)
+    hipLaunchKernelGGL((k_csr_diag<double>), grid, dim3(kBlock), 0, ctx().stream, m->rows, m->irp,
+                       m->ja, (const double*)m->val, (double*)d->v);
+  else
+    hipLaunchKernelGGL((k_csr_diag<float>), grid, dim3(kBlock), 0, ctx().stream, m->rows, m->irp,
+                       m->ja, (const float*)m->val, (float*)d->v);
+  PSB_HIP_LAUNCH_CHECK();
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_csr_get_analysis(void* dmat, int* lanes_per_row, int* long_rows, int* long_threshold) {
+  CsrDev* m = (CsrDev*)dmat;
+  if (!m) return PSB_HIP_INVALID;
+  if (lanes_per_row) *lanes_per_row = m->lanes;
+  if (long_rows) *long_rows = m->nlong;
+  if (long_threshold) *long_threshold = m->long_thr;
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_csr_spmv(void* dmat, double alpha, void* xv, double beta, void* yv) {
+  PSB_ENSURE_INIT();
+  CsrDev* m = (CsrDev*)dmat;
+  MultiVec *x = (MultiVec*)xv, *y = (MultiVec*)yv;
+  int rc = check_vectors(x, y, m->elem_type, m->max_col, m->rows);
+  if (rc) return set_error(rc, __func__, "x/y type, size or aliasing");
+  if (m->rows == 0) return PSB_HIP_SUCCESS;
+  for (int c = 0; c < x->count; c++) {
+    if (m->elem_type == PSB_HIP_TYPE_DOUBLE)
+      rc = csr_launch<double>(m, (const double*)x->v + (size_t)c * x->pitch, alpha, beta,
+                              (double*)y->v + (size_t)c * y->pitch);
+    else
+      rc = csr_launch<float>(m, (const float*)x->v + (size_t)c * x->pitch, (float)alpha,
+                             (float)beta, (float*)y->v + (size_t)c * y->pitch);
+    if (rc) return rc;
+  }
+  return PSB_HIP_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,230 @@
+"""Krylov solvers over the device plug-ins: psb_krylov (krylov/psb_dkrylov.f90:
+81-227), CG (krylov/psb_dcg.F90:98-333) and BiCGSTAB (krylov/psb_dcgstab.f90:
+96-393) with the stopping tests of krylov/psb_d_krylov_conv_mod.f90:205-356.
+
+These routines issue exactly the reference's sequence of psb_spmm /
+psb_geaxpby / psb_gedot / psb_genrm2 / prec%apply calls, each running on the
+device; every scalar (dot, nrm2) returns to the host as in the reference.
+The fused device-resident solvers (one synchronisation per several
+iterations) live in psblas3_amd.solvers.
+"""
+from dataclasses import dataclass, field
+from typing import List
+
+from . import _native as N
+from .psblas import (psb_geall, psb_geamax, psb_geaxpby, psb_gedot, psb_genrm2, psb_spmm)
+from .vect import psb_d_vect_type
+
+
+@dataclass
+class psb_itconv_type:
+    """stopdat of psb_d_krylov_conv_mod."""
+    istop: int = 2
+    itmax: int = 1000
+    itrace: int = 0
+    eps: float = 0.0
+    ani: float = 0.0
+    bni: float = 0.0
+    bn2: float = 0.0
+    r0n2: float = 0.0
+    errnum: float = 0.0
+    errden: float = 1.0
+    history: List[float] = field(default_factory=list)
+
+
+def psb_spnrmi(a, desc_a=None):
+    """||A||_inf, needed by istop = 1 (read back once)."""
+    import numpy as np
+    a_ = a.a
+    if hasattr(a_, "to_host") and a_.get_fmt() == "CSRG":
+        irp, ja, val = a_.to_host()
+        rows = np.repeat(np.arange(a_.nr), np.diff(irp))
+        return float(np.bincount(rows, weights=np.abs(val), minlength=a_.nr).max())
+    if a_.get_fmt() == "HLG":
+        hk, irn, ja, val, idiag = a_.to_host()
+        hksz = a_.hksz
+        s = np.zeros(a_.nr)
+        for h in range(len(hk) - 1):
+            w = (hk[h + 1] - hk[h]) // hksz
+            blk = np.abs(val[hk[h]:hk[h + 1]]).reshape(w, hksz)
+            r0 = h * hksz
+            r1 = min(a_.nr, r0 + hksz)
+            s[r0:r1] = blk.sum(axis=0)[:r1 - r0]
+        return float(s.max())
+    irn, ja, val, idiag = a_.to_host()
+    v = np.abs(val).reshape(a_.nzm, a_.nr)
+    return float(v.sum(axis=0).max())
+
+
+def psb_init_conv(methdname, istop, itrace, itmax, a, x, b, eps, desc_a):
+    """psb_d_init_conv_vect (psb_d_krylov_conv_mod.f90:205-277)."""
+    s = psb_itconv_type(istop=istop, itmax=itmax, itrace=itrace, eps=eps)
+    if istop == 1:
+        s.ani = psb_spnrmi(a, desc_a)
+        s.bni = psb_geamax(b, desc_a)
+    elif istop == 2:
+        s.bn2 = psb_genrm2(b, desc_a)
+    elif istop == 3:
+        r = psb_geall(desc_a, mold=x.v)
+        psb_geaxpby(1.0, b, 0.0, r, desc_a)
+        psb_spmm(-1.0, a, x, 1.0, r, desc_a)
+        s.r0n2 = psb_genrm2(r, desc_a)
+        r.free()
+    else:
+        raise N.PsbError(N.INVALID, "psb_init_conv", f"invalid istop {istop}")
+    s.errnum, s.errden = 0.0, 1.0
+    if itrace > 0:
+        print(f"{methdname + ':':18s} Iteration   Error Estimate       Tolerance")
+    return s
+
+
+def psb_check_conv(methdname, it, x, r, desc_a, s):
+    """psb_d_check_conv_vect (psb_d_krylov_conv_mod.f90:279-356)."""
+    if s.istop == 1:
+        rni = psb_geamax(r, desc_a)
+        xni = psb_geamax(x, desc_a)
+        s.errnum, s.errden = rni, s.ani * xni + s.bni
+    elif s.istop == 2:
+        s.errnum, s.errden = psb_genrm2(r, desc_a), s.bn2
+    else:
+        s.errnum, s.errden = psb_genrm2(r, desc_a), s.r0n2
+    if s.errden == 0.0:
+        res = s.errnum <= s.eps
+    else:
+        res = s.errnum <= s.eps * s.errden
+    s.history.append(s.errnum / s.errden if s.errden > 0 else s.errnum)
+    res = res or (s.itmax <= it)
+    if s.itrace > 0 and (it % s.itrace == 0 or res):
+        e = s.errnum / s.errden if s.errden > 0 else s.errnum
+        print(f"{methdname + ':':18s} {it:4d}  {e:15.9e}  {s.eps:15.9e}")
+    return res
+
+
+def psb_end_conv(s, itx):
+    """psb_d_end_conv: err = errnum/errden (or errnum), iter = itx."""
+    err = s.errnum / s.errden if s.errden > 0 else s.errnum
+    return itx, err
+
+
+def psb_dcg_vect(a, prec, b, x, eps, desc_a, itmax=1000, itrace=0, istop=2):
+    """psb_dcg_vect (krylov/psb_dcg.F90:98-333)."""
+    methd = "CG"
+    p, q, r, z = (psb_geall(desc_a, mold=x.v) for _ in range(4))
+    itx = 0
+    s = None
+    done = False
+    try:
+        while not done:  # restart
+            if itx >= itmax:
+                break
+            it = 0
+            psb_geaxpby(1.0, b, 0.0, r, desc_a)
+            psb_spmm(-1.0, a, x, 1.0, r, desc_a)
+            rho = 0.0
+            s = psb_init_conv(methd, istop, itrace, itmax, a, x, b, eps, desc_a)
+            while True:  # iteration
+                it += 1
+                itx += 1
+                prec.apply(r, z, desc_a)
+                rho_old = rho
+                rho = psb_gedot(r, z, desc_a)
+                if it == 1:
+                    psb_geaxpby(1.0, z, 0.0, p, desc_a)
+                else:
+                    if rho_old == 0.0:
+                        break
+                    beta = rho / rho_old
+                    psb_geaxpby(1.0, z, beta, p, desc_a)
+                psb_spmm(1.0, a, p, 0.0, q, desc_a)
+                sigma = psb_gedot(p, q, desc_a)
+                if sigma == 0.0:
+                    break
+                alpha = rho / sigma
+                psb_geaxpby(alpha, p, 1.0, x, desc_a)
+                psb_geaxpby(-alpha, q, 1.0, r, desc_a)
+                if psb_check_conv(methd, itx, x, r, desc_a, s):
+                    done = True
+                    break
+    finally:
+        for v in (p, q, r, z):
+            v.free()
+    if s is None:
+        return itx, 0.0, []
+    it_, err = psb_end_conv(s, itx)
+    return it_, err, s.history
+
+
+def psb_dcgstab_vect(a, prec, b, x, eps, desc_a, itmax=1000, itrace=0, istop=2):
+    """psb_dcgstab_vect (krylov/psb_dcgstab.f90:96-393)."""
+    methd = "BiCGSTAB"
+    q, r, p, v, sv, t, z, f = (psb_geall(desc_a, mold=x.v) for _ in range(8))
+    itx = 0
+    s = psb_init_conv(methd, istop, itrace, itmax, a, x, b, eps, desc_a)
+    alpha = omega = rho = 0.0
+    try:
+        done = False
+        while not done:  # restart
+            if itx >= itmax:
+                break
+            it = 0
+            psb_geaxpby(1.0, b, 0.0, r, desc_a)
+            psb_spmm(-1.0, a, x, 1.0, r, desc_a)
+            psb_geaxpby(1.0, r, 0.0, q, desc_a)
+            if psb_check_conv(methd, itx, x, r, desc_a, s):
+                break
+            rho = 0.0
+            while True:
+                it += 1
+                itx += 1
+                rho_old = rho
+                rho = psb_gedot(q, r, desc_a)
+                if rho == 0.0:
+                    break
+                if it == 1:
+                    psb_geaxpby(1.0, r, 0.0, p, desc_a)
+                else:
+                    beta = (rho / rho_old) * (alpha / omega)
+                    psb_geaxpby(-omega, v, 1.0, p, desc_a)
+                    psb_geaxpby(1.0, r, beta, p, desc_a)
+                prec.apply(p, f, desc_a)
+                psb_spmm(1.0, a, f, 0.0, v, desc_a)
+                sigma = psb_gedot(q, v, desc_a)
+                if sigma == 0.0:
+                    break
+                alpha = rho / sigma
+                psb_geaxpby(1.0, r, 0.0, sv, desc_a)
+                psb_geaxpby(-alpha, v, 1.0, sv, desc_a)
+                prec.apply(sv, z, desc_a)
+                psb_spmm(1.0, a, z, 0.0, t, desc_a)
+                sigma = psb_gedot(t, t, desc_a)
+                if sigma == 0.0:
+                    break
+                tau = psb_gedot(t, sv, desc_a)
+                omega = tau / sigma
+                if omega == 0.0:
+                    break
+                psb_geaxpby(alpha, f, 1.0, x, desc_a)
+                psb_geaxpby(omega, z, 1.0, x, desc_a)
+                psb_geaxpby(1.0, sv, 0.0, r, desc_a)
+                psb_geaxpby(-omega, t, 1.0, r, desc_a)
+                if psb_check_conv(methd, itx, x, r, desc_a, s):
+                    done = True
+                    break
+    finally:
+        for w in (q, r, p, v, sv, t, z, f):
+            w.free()
+    it_, err = psb_end_conv(s, itx)
+    return it_, err, s.history
+
+
+def psb_krylov(method, a, prec, b, x, eps, desc_a, itmax=1000, itrace=0, istop=2,
+               irst=None):
+    """psb_krylov dispatcher: 'CG' or 'BICGSTAB' (unknown names fall back to
+    BiCGSTAB as the reference's default branch does).  Returns
+    (iter, err, history); x is updated in place."""
+    if not isinstance(x, psb_d_vect_type) or not isinstance(b, psb_d_vect_type):
+        raise N.PsbError(N.INVALID, "psb_krylov", "b and x must be psb_d_vect_type")
+    m = method.upper()
+    if m == "CG":
+        return psb_dcg_vect(a, prec, b, x, eps, desc_a, itmax, itrace, istop)
+    return psb_dcgstab_vect(a, prec, b, x, eps, desc_a, itmax, itrace, istop)

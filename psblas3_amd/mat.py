@@ -1,0 +1,323 @@
+"""Device sparse matrices: psb_[ds]_hip_{hlg,elg,csrg}_sparse_mat, the plug-ins
+that take the place of psb_[ds]_cuda_{hlg,elg,csrg}_sparse_mat
+(cuda/psb_d_cuda_hlg_mat_mod.F90:44-84, psb_d_cuda_elg_mat_mod.F90:44-60,
+psb_d_cuda_csrg_mat_mod.F90:43-81), and the psb_dspmat_type wrapper whose
+``a`` component holds one of them (base/modules/serial/psb_d_mat_mod.F90).
+
+Assembly follows psb_spall / psb_spins / psb_spasb(mold=) (base/tools/
+psb_dspasb.f90:47,144): triplets are collected host side, fixed by
+psb_hip_fix_coo_d (sort + duplicate merge, psb_d_fix_coo) and handed to the
+plug-in's cp_from_coo, which for HLG computes the hack offsets on the host
+(psi_compute_hckoff_from_coo, cuda/impl/psb_d_cuda_cp_hlg_from_coo.F90:117-188)
+and fills the device arrays with a kernel, as the reference GPU type does.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+
+DUPL_OVWRT, DUPL_ADD, DUPL_ERR = 0, 1, 2
+_DUPL = {"ovwrt": DUPL_OVWRT, "add": DUPL_ADD, "err": DUPL_ERR}
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None and a.size else None
+
+
+class psb_d_coo_sparse_mat:
+    """Host COO in build state (1-based triplets), psb_d_coo_sparse_mat."""
+
+    def __init__(self, nr, nc, ia=None, ja=None, val=None, dtype=np.float64):
+        self.nr, self.nc = int(nr), int(nc)
+        self.ia = np.zeros(0, np.int32) if ia is None else np.ascontiguousarray(ia, np.int32)
+        self.ja = np.zeros(0, np.int32) if ja is None else np.ascontiguousarray(ja, np.int32)
+        self.val = np.zeros(0) if val is None else np.ascontiguousarray(val, np.float64)
+        self.sorted = False
+
+    def get_nzeros(self):
+        return int(self.val.size)
+
+    def fix(self, dupl=DUPL_ADD):
+        """psb_d_fix_coo (row major): in place; raises on dupl=err duplicates."""
+        ia = self.ia.copy()
+        ja = self.ja.copy()
+        val = self.val.copy()
+        nzout = C.c_int64(0)
+        N.call("psb_hip_fix_coo_d", self.nr, self.nc, int(val.size), _ptr(ia), _ptr(ja),
+               _ptr(val), int(dupl), C.byref(nzout))
+        k = nzout.value
+        self.ia, self.ja, self.val = ia[:k].copy(), ja[:k].copy(), val[:k].copy()
+        self.sorted = True
+        return self
+
+
+class _hip_base_mat:
+    fmt = None
+    _etype = None
+    _dtype = None
+
+    def __init__(self):
+        self.dev = None
+        self.nr = self.nc = self.nnz = 0
+
+    def get_nrows(self):
+        return self.nr
+
+    def get_ncols(self):
+        return self.nc
+
+    def get_nzeros(self):
+        return self.nnz
+
+    def get_fmt(self):
+        return self.fmt
+
+    def is_dev(self):
+        return self.dev is not None
+
+    def mold(self):
+        return type(self)()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def _check_vects(self, x, y):
+        from .vect import psb_hip_base_vect
+        for v in (x, y):
+            if not isinstance(v, psb_hip_base_vect):
+                raise N.PsbError(N.INVALID, f"{self.fmt}_vect_mv",
+                                 "x and y must be device vectors (no host fallback)")
+            if v._dtype != self._dtype:
+                raise N.PsbError(N.INVALID, f"{self.fmt}_vect_mv", "precision mismatch")
+        x.ensure_dev()
+        if y.is_host():
+            y.sync()
+
+    def get_diag(self, d):
+        """get_diag on the device into the vector d (0 where no diagonal)."""
+        d.ensure_dev()
+        N.call(self._diag_fn, self.dev, d.dev)
+        d.set_dev()
+
+
+class psb_d_hip_hlg_sparse_mat(_hip_base_mat):
+    """HLL on the device (psb_d_cuda_hlg_sparse_mat).  hksz defaults to the
+    wavefront size, 64 on gfx950, as the reference GPU type uses WarpSize
+    (cuda/impl/psb_d_cuda_cp_hlg_from_coo.F90:60)."""
+    fmt = "HLG"
+    _etype = N.TYPE_DOUBLE
+    _dtype = np.float64
+    _diag_fn = "psb_hip_hll_get_diag"
+
+    def __init__(self, hksz=64):
+        super().__init__()
+        self.hksz = int(hksz)
+        self.irn = self.hkoffs = None
+        self.isz = 0
+
+    def cp_from_coo(self, coo):
+        """psb_d_cuda_cp_hlg_from_coo: offsets on the host, fill on the device."""
+        if not coo.sorted:
+            coo = psb_d_coo_sparse_mat(coo.nr, coo.nc, coo.ia, coo.ja, coo.val).fix()
+        self.free()
+        nr, nz = coo.nr, coo.get_nzeros()
+        noffs = (nr + self.hksz - 1) // self.hksz
+        irn = np.zeros(max(nr, 1), np.int32)
+        idisp = np.zeros(nr + 1, np.int32)
+        hkoffs = np.zeros(noffs + 1, np.int32)
+        isz = C.c_int64(0)
+        N.call("psb_hip_hll_offsets", nr, nz, _ptr(coo.ia), self.hksz, irn.ctypes.data,
+               idisp.ctypes.data, hkoffs.ctypes.data, C.byref(isz))
+        h = C.c_void_p()
+        N.call("psb_hip_hll_alloc", C.byref(h), self.hksz, nr, nz, int(isz.value), self._etype, 1)
+        self.dev = h
+        val = np.ascontiguousarray(coo.val, dtype=self._dtype)
+        N.call("psb_hip_hll_copy_coo", nr, coo.nc, nz, self.hksz, noffs, int(isz.value),
+               irn.ctypes.data, hkoffs.ctypes.data, idisp.ctypes.data, _ptr(coo.ja), _ptr(val),
+               self.dev)
+        self.nr, self.nc, self.nnz = nr, coo.nc, nz
+        self.irn, self.hkoffs, self.isz = irn[:nr], hkoffs, int(isz.value)
+        return self
+
+    def to_host(self):
+        """Read the device arrays back: (hkoffs, irn, ja, val, idiag)."""
+        val = np.empty(max(self.isz, 1), self._dtype)
+        ja = np.empty(max(self.isz, 1), np.int32)
+        hk = np.empty((self.nr + self.hksz - 1) // self.hksz + 1, np.int32)
+        irn = np.empty(max(self.nr, 1), np.int32)
+        idiag = np.empty(max(self.nr, 1), np.int32)
+        N.call("psb_hip_hll_read", self.dev, val.ctypes.data, ja.ctypes.data, hk.ctypes.data,
+               irn.ctypes.data, idiag.ctypes.data)
+        return hk, irn[:self.nr], ja[:self.isz], val[:self.isz], idiag[:self.nr]
+
+    def vect_mv(self, alpha, x, beta, y):
+        """psb_d_cuda_hlg_vect_mv: y = alpha*A*x + beta*y on the device."""
+        self._check_vects(x, y)
+        N.call("psb_hip_hll_spmv", self.dev, float(alpha), x.dev, float(beta), y.dev)
+        y.set_dev()
+
+    def sizeof(self):
+        es = np.dtype(self._dtype).itemsize
+        return self.isz * (es + 4) + 4 * (2 * self.nr + len(self.hkoffs))
+
+    def free(self):
+        if self.dev is not None:
+            N.fn("psb_hip_hll_free")(self.dev)
+            self.dev = None
+
+
+class psb_s_hip_hlg_sparse_mat(psb_d_hip_hlg_sparse_mat):
+    """Single-precision HLL on the device (psb_s_cuda_hlg_sparse_mat)."""
+    _etype = N.TYPE_FLOAT
+    _dtype = np.float32
+
+
+class psb_d_hip_elg_sparse_mat(_hip_base_mat):
+    """ELL on the device (psb_d_cuda_elg_sparse_mat), pitch ceil32(rows)."""
+    fmt = "ELG"
+    _etype = N.TYPE_DOUBLE
+    _dtype = np.float64
+    _diag_fn = "psb_hip_ell_get_diag"
+
+    def __init__(self):
+        super().__init__()
+        self.nzm = 0
+
+    def cp_from_coo(self, coo):
+        if not coo.sorted:
+            coo = psb_d_coo_sparse_mat(coo.nr, coo.nc, coo.ia, coo.ja, coo.val).fix()
+        self.free()
+        nr, nz = coo.nr, coo.get_nzeros()
+        irn = np.bincount(coo.ia - 1, minlength=nr).astype(np.int32) if nz else np.zeros(nr, np.int32)
+        idisp = np.zeros(nr + 1, np.int32)
+        np.cumsum(irn, out=idisp[1:])
+        nzm = int(irn.max()) if nr else 0
+        slots = ((nr + 31) // 32) * 32 * nzm
+        es = np.dtype(self._dtype).itemsize
+        if slots * (es + 4) > 200 * 2**30:
+            raise N.PsbError(N.OUTOFMEM, "ELG cp_from_coo",
+                             f"ELL needs {nr} x {nzm} slots: not representable")
+        h = C.c_void_p()
+        N.call("psb_hip_ell_alloc", C.byref(h), nr, nzm, nz, self._etype, 1)
+        self.dev = h
+        val = np.ascontiguousarray(coo.val, dtype=self._dtype)
+        N.call("psb_hip_ell_copy_coo", nr, coo.nc, nz, _ptr(irn), _ptr(idisp), _ptr(coo.ja),
+               _ptr(val), self.dev)
+        self.nr, self.nc, self.nnz, self.nzm = nr, coo.nc, nz, nzm
+        self.irn = irn
+        return self
+
+    def to_host(self):
+        """(irn, ja, val, idiag) with ja/val of shape (nr, nzm) column-major."""
+        ja = np.empty(max(self.nr * self.nzm, 1), np.int32)
+        val = np.empty(max(self.nr * self.nzm, 1), self._dtype)
+        irn = np.empty(max(self.nr, 1), np.int32)
+        idiag = np.empty(max(self.nr, 1), np.int32)
+        N.call("psb_hip_ell_read", self.dev, val.ctypes.data, ja.ctypes.data, max(self.nr, 1),
+               irn.ctypes.data, idiag.ctypes.data)
+        n = self.nr * self.nzm
+        return irn[:self.nr], ja[:n], val[:n], idiag[:self.nr]
+
+    def pitch(self):
+        return N.fn("psb_hip_ell_pitch")(self.dev)
+
+    def vect_mv(self, alpha, x, beta, y):
+        self._check_vects(x, y)
+        N.call("psb_hip_ell_spmv", self.dev, float(alpha), x.dev, float(beta), y.dev)
+        y.set_dev()
+
+    def free(self):
+        if self.dev is not None:
+            N.fn("psb_hip_ell_free")(self.dev)
+            self.dev = None
+
+
+class psb_s_hip_elg_sparse_mat(psb_d_hip_elg_sparse_mat):
+    _etype = N.TYPE_FLOAT
+    _dtype = np.float32
+
+
+class psb_d_hip_csrg_sparse_mat(_hip_base_mat):
+    """CSR on the device (psb_d_cuda_csrg_sparse_mat), with hand-written
+    vector-CSR and wave-per-row kernels in place of cuSPARSE."""
+    fmt = "CSRG"
+    _etype = N.TYPE_DOUBLE
+    _dtype = np.float64
+    _diag_fn = "psb_hip_csr_get_diag"
+
+    def cp_from_coo(self, coo):
+        """psb_d_cp_csr_from_coo (base/serial/impl/psb_d_csr_impl.F90:3154)."""
+        if not coo.sorted:
+            coo = psb_d_coo_sparse_mat(coo.nr, coo.nc, coo.ia, coo.ja, coo.val).fix()
+        self.free()
+        nr, nz = coo.nr, coo.get_nzeros()
+        cnt = np.bincount(coo.ia - 1, minlength=nr) if nz else np.zeros(nr, np.int64)
+        irp = np.ones(nr + 1, np.int32)
+        np.cumsum(cnt, out=irp[1:])
+        irp[1:] += 1
+        h = C.c_void_p()
+        N.call("psb_hip_csr_alloc", C.byref(h), nr, coo.nc, nz, self._etype, 1)
+        self.dev = h
+        val = np.ascontiguousarray(coo.val, dtype=self._dtype)
+        N.call("psb_hip_csr_write", self.dev, irp.ctypes.data, _ptr(coo.ja), _ptr(val))
+        self.nr, self.nc, self.nnz = nr, coo.nc, nz
+        return self
+
+    def to_host(self):
+        irp = np.empty(self.nr + 1, np.int32)
+        ja = np.empty(max(self.nnz, 1), np.int32)
+        val = np.empty(max(self.nnz, 1), self._dtype)
+        N.call("psb_hip_csr_read", self.dev, irp.ctypes.data, ja.ctypes.data, val.ctypes.data)
+        return irp, ja[:self.nnz], val[:self.nnz]
+
+    def analysis(self):
+        lanes, nlong, thr = C.c_int(), C.c_int(), C.c_int()
+        N.call("psb_hip_csr_get_analysis", self.dev, C.byref(lanes), C.byref(nlong), C.byref(thr))
+        return lanes.value, nlong.value, thr.value
+
+    def vect_mv(self, alpha, x, beta, y):
+        self._check_vects(x, y)
+        N.call("psb_hip_csr_spmv", self.dev, float(alpha), x.dev, float(beta), y.dev)
+        y.set_dev()
+
+    def free(self):
+        if self.dev is not None:
+            N.fn("psb_hip_csr_free")(self.dev)
+            self.dev = None
+
+
+class psb_s_hip_csrg_sparse_mat(psb_d_hip_csrg_sparse_mat):
+    _etype = N.TYPE_FLOAT
+    _dtype = np.float32
+
+
+class psb_dspmat_type:
+    """psb_dspmat_type: ``a`` holds the storage plug-in.  Build state is a
+    host COO; psb_spasb converts it to the mold's format."""
+
+    def __init__(self):
+        self.a = None
+        self.coo = None
+        self.m = self.n = 0
+
+    def get_nrows(self):
+        return self.m
+
+    def get_ncols(self):
+        return self.n
+
+    def get_nzeros(self):
+        return self.a.get_nzeros() if self.a is not None else self.coo.get_nzeros()
+
+    def get_fmt(self):
+        return self.a.get_fmt() if self.a is not None else "COO"
+
+    def free(self):
+        if self.a is not None:
+            self.a.free()
+        self.a = None
+        self.coo = None

@@ -1,0 +1,64 @@
+"""Extract the SpMV known answers of the reference's torture tests as data.
+
+Reads test/torture/psb_{d,s}_mvsv_tester.f90 from the reference checkout and
+writes tests/golden/torture_usmv.json: for every usmv case with trans='n',
+the 1-based triplets (IA, JA, VA), x, alpha, beta, y before (bcy) and the
+expected y after (cy).  Only numbers are kept; no reference source text.
+
+usage: python tests/golden/make_torture_fixture.py [/root/reference]
+"""
+import json
+import os
+import re
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _nums(s):
+    return [float(t) for t in re.findall(r"[-+]?\d+(?:\.\d*)?(?:[eEdD][-+]?\d+)?", s)]
+
+
+def parse(path, kind):
+    text = open(path).read()
+    cases = []
+    for m in re.finditer(r"subroutine (\w_usmv_\w+)\(res,afmt,ctxt\)(.*?)end subroutine", text, re.S):
+        name, body = m.group(1), m.group(2)
+        tr = re.search(r"character::transa='(\w)'", body).group(1)
+        if tr != "n":
+            continue
+
+        def arr(var):
+            mm = re.search(r"::\s*" + var + r"\(\d+\)=\(/([^/]*)/\)", body)
+            return _nums(mm.group(1))
+
+        def scal(var):
+            mm = re.search(r"::\s*" + var + r"\s*=\s*([-+]?\d+(?:\.\d*)?)", body)
+            return float(mm.group(1))
+
+        m_ = int(scal("m"))
+        k_ = int(scal("k"))
+        cases.append({
+            "name": name, "kind": kind, "m": m_, "k": k_,
+            "alpha": scal("alpha"), "beta": scal("beta"),
+            "ia": [int(v) for v in arr("IA")], "ja": [int(v) for v in arr("JA")],
+            "va": arr("VA"), "x": arr("x"), "bcy": arr("bcy"), "cy": arr("cy"),
+        })
+    return cases
+
+
+def main():
+    ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    out = []
+    for kind in ("d", "s"):
+        p = os.path.join(ref, "test", "torture", f"psb_{kind}_mvsv_tester.f90")
+        out += parse(p, kind)
+    dst = os.path.join(HERE, "torture_usmv.json")
+    with open(dst, "w") as f:
+        json.dump({"source": "test/torture/psb_{d,s}_mvsv_tester.f90 (usmv, trans=n)",
+                   "cases": out}, f, indent=1)
+    print(f"{len(out)} cases -> {dst}")
+
+
+if __name__ == "__main__":
+    main()

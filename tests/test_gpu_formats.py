@@ -1,0 +1,205 @@
+"""GPU parity: device COO -> HLL / ELL / CSR conversions (bit-exact against
+the oracle) and SpMV in FP64 / FP32 (per-row bound 4*nnz_row*eps*(|A||x|)_i).
+All device work goes through the C ABI (libpsb_hip.so)."""
+import numpy as np
+import pytest
+
+import psblas3_amd as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _coo_cases(orc):
+    ia = np.array([1, 1, 3, 4, 4, 4, 5, 4], np.int32)
+    ja = np.array([1, 3, 2, 4, 1, 5, 5, 4], np.int32)      # (4,4) duplicated
+    val = np.array([1, 2, 3, 4, 5, 6, 7, 0.5])
+    return {
+        "hand5": orc.COO(5, 5, ia, ja, val),
+        "convdiff_L13": orc.gen_convdiff(13),
+        "convdiff_L40": orc.gen_convdiff(40),
+        "powerlaw_3000": orc.gen_powerlaw(3000, cap=300, seed=1),
+        "aniso_L33": orc.gen_aniso2d(33, 0.01),
+    }
+
+
+def _mat(coo, mold):
+    return P.matrix_from_coo(coo.nr, coo.nc, coo.ia, coo.ja, coo.val, mold=mold)
+
+
+@pytest.mark.parametrize("name", ["hand5", "convdiff_L13", "convdiff_L40", "powerlaw_3000",
+                                  "aniso_L33"])
+@pytest.mark.parametrize("hksz", [32, 64, 7])
+def test_hll_conversion_bitexact(gpu, orc, name, hksz):
+    _say("case", name, "hksz", hksz)
+    coo = _coo_cases(orc)[name]
+    a = _mat(coo, P.psb_d_hip_hlg_sparse_mat(hksz=hksz))
+    hk, irn, ja, val, idiag = a.a.to_host()
+    h = orc.hll_from_coo(orc.fix_coo(coo), hksz)
+    assert np.array_equal(hk, h.hkoffs)
+    assert np.array_equal(irn, h.irn)
+    assert np.array_equal(ja, h.ja)
+    assert np.array_equal(val.view(np.int64), h.val.view(np.int64))
+    assert np.array_equal(idiag, h.idiag)
+    a.free()
+
+
+@pytest.mark.parametrize("name", ["hand5", "convdiff_L13", "powerlaw_3000"])
+def test_ell_conversion_bitexact(gpu, orc, name):
+    _say("case", name)
+    coo = _coo_cases(orc)[name]
+    a = _mat(coo, P.psb_d_hip_elg_sparse_mat())
+    irn, ja, val, idiag = a.a.to_host()
+    e = orc.ell_from_coo(orc.fix_coo(coo))
+    assert a.a.pitch() == ((coo.nr + 31) // 32) * 32
+    assert np.array_equal(irn, e.irn)
+    assert np.array_equal(ja, e.ja)
+    assert np.array_equal(val.view(np.int64), e.val.view(np.int64))
+    assert np.array_equal(idiag, e.idiag)
+    a.free()
+
+
+@pytest.mark.parametrize("name", ["hand5", "convdiff_L13", "powerlaw_3000"])
+def test_csr_conversion_bitexact(gpu, orc, name):
+    _say("case", name)
+    coo = _coo_cases(orc)[name]
+    a = _mat(coo, P.psb_d_hip_csrg_sparse_mat())
+    irp, ja, val = a.a.to_host()
+    oirp, oja, oval = orc.csr_from_coo(orc.fix_coo(coo))
+    assert np.array_equal(irp, oirp) and np.array_equal(ja, oja)
+    assert np.array_equal(val.view(np.int64), oval.view(np.int64))
+    a.free()
+
+
+def _row_bound(coo, x, alpha, beta, y0, eps):
+    nr = coo.nr
+    absax = np.zeros(nr)
+    np.add.at(absax, coo.ia - 1, np.abs(coo.val) * np.abs(x[coo.ja - 1]))
+    nnz_row = np.bincount(coo.ia - 1, minlength=nr)
+    return 4.0 * (nnz_row + 1) * eps * (abs(alpha) * absax + abs(beta) * np.abs(y0)) + 1e-300
+
+
+MOLDS = {
+    "hll32": lambda: P.psb_d_hip_hlg_sparse_mat(hksz=32),
+    "hll64": lambda: P.psb_d_hip_hlg_sparse_mat(hksz=64),
+    "ell": lambda: P.psb_d_hip_elg_sparse_mat(),
+    "csr": lambda: P.psb_d_hip_csrg_sparse_mat(),
+}
+SMOLDS = {
+    "hll32": lambda: P.psb_s_hip_hlg_sparse_mat(hksz=32),
+    "hll64": lambda: P.psb_s_hip_hlg_sparse_mat(hksz=64),
+    "ell": lambda: P.psb_s_hip_elg_sparse_mat(),
+    "csr": lambda: P.psb_s_hip_csrg_sparse_mat(),
+}
+
+
+@pytest.mark.parametrize("fmt", list(MOLDS))
+@pytest.mark.parametrize("name", ["hand5", "convdiff_L40", "powerlaw_3000", "aniso_L33"])
+@pytest.mark.parametrize("ab", [(1.0, 0.0), (-2.5, 0.75), (0.0, 2.0), (1.0, 1.0)])
+def test_spmv_fp64(gpu, orc, fmt, name, ab):
+    alpha, beta = ab
+    _say("case", fmt, name, alpha, beta)
+    coo = _coo_cases(orc)[name]
+    f = orc.fix_coo(coo)
+    a = _mat(coo, MOLDS[fmt]())
+    x = orc.uniform_pm1(1, coo.nc)
+    y0 = orc.uniform_pm1(9, coo.nr)
+    xv = P.vect_from(x)
+    yv = P.vect_from(y0)
+    P.psb_spmm(alpha, a, xv, beta, yv)
+    y = yv.get_vect()
+    irp, ja, val = orc.csr_from_coo(f)
+    yref = orc.csr_spmv(irp, ja, val, x, alpha, beta, y0)
+    bound = _row_bound(f, x, alpha, beta, y0, np.finfo(np.float64).eps)
+    assert np.all(np.abs(y - yref) <= bound), np.max(np.abs(y - yref) - bound)
+    a.free()
+
+
+@pytest.mark.parametrize("fmt", list(SMOLDS))
+@pytest.mark.parametrize("name", ["convdiff_L40", "powerlaw_3000"])
+def test_spmv_fp32(gpu, orc, fmt, name):
+    _say("case", fmt, name)
+    coo = _coo_cases(orc)[name]
+    f = orc.fix_coo(coo)
+    a = _mat(coo, SMOLDS[fmt]())
+    x = orc.uniform_pm1(1, coo.nc).astype(np.float32)
+    y0 = orc.uniform_pm1(2, coo.nr).astype(np.float32)
+    xv = P.vect_from(x, mold=P.psb_s_hip_vect())
+    yv = P.vect_from(y0, mold=P.psb_s_hip_vect())
+    P.psb_spmm(1.5, a, xv, -0.5, yv)
+    y = yv.get_vect().astype(np.float64)
+    irp, ja, val = orc.csr_from_coo(f)
+    yref = orc.csr_spmv(irp, ja, val.astype(np.float32), x, 1.5, -0.5, y0).astype(np.float64)
+    # FP32 oracle itself carries ~nnz_row*eps32 error: allow twice the bound
+    bound = 2 * _row_bound(f, x.astype(np.float64), 1.5, -0.5, y0.astype(np.float64),
+                           np.finfo(np.float32).eps)
+    assert np.all(np.abs(y - yref) <= bound)
+    a.free()
+
+
+def test_spmv_poisson_exact(gpu, orc):
+    """Integer entries, x = ones: every row sum is exact in any order."""
+    _say("case poisson L=30 exact")
+    coo = orc.gen_poisson3d(30)
+    for mold in (P.psb_d_hip_hlg_sparse_mat(32), P.psb_d_hip_elg_sparse_mat(),
+                 P.psb_d_hip_csrg_sparse_mat()):
+        a = _mat(coo, mold)
+        xv = P.vect_from(np.ones(coo.nc))
+        yv = P.vect_from(np.zeros(coo.nr))
+        P.psb_spmm(1.0, a, xv, 0.0, yv)
+        irn = np.bincount(coo.ia - 1, minlength=coo.nr)
+        assert np.array_equal(yv.get_vect(), 7.0 - irn)
+        a.free()
+
+
+def test_spmv_multicolumn(gpu, orc):
+    _say("case multicolumn hll spmm k=5")
+    coo = orc.gen_convdiff(17)
+    f = orc.fix_coo(coo)
+    h = orc.hll_from_coo(f, 32)
+    a = _mat(coo, P.psb_d_hip_hlg_sparse_mat(32))
+    X = np.stack([orc.uniform_pm1(s, coo.nc) for s in range(5)], axis=1)
+    Y0 = np.stack([orc.uniform_pm1(20 + s, coo.nr) for s in range(5)], axis=1)
+    xv = P.vect_from(X)
+    yv = P.vect_from(Y0)
+    P.psb_spmm(2.0, a, xv, -1.0, yv)
+    Y = yv.get_vect()
+    Yref = orc.hll_spmm(h, X, 2.0, -1.0, Y0)
+    for k in range(5):
+        bound = _row_bound(f, X[:, k], 2.0, -1.0, Y0[:, k], np.finfo(float).eps)
+        assert np.all(np.abs(Y[:, k] - Yref[:, k]) <= bound)
+    a.free()
+
+
+def test_hll_variants_agree(gpu, orc):
+    _say("case hll variants")
+    from psblas3_amd import _native as N
+    coo = orc.gen_powerlaw(5000, cap=200, seed=4)
+    a = _mat(coo, P.psb_d_hip_hlg_sparse_mat(64))
+    x = P.vect_from(orc.uniform_pm1(3, coo.nc))
+    outs = []
+    for var in (1, 2):
+        old = N.fn("psb_hip_hll_set_variant")(var)
+        y = P.vect_from(np.zeros(coo.nr))
+        P.psb_spmm(1.0, a, x, 0.0, y)
+        outs.append(y.get_vect())
+        N.fn("psb_hip_hll_set_variant")(old)
+    assert np.array_equal(outs[0], outs[1])   # same per-row order -> same bits
+    a.free()
+
+
+def test_spmv_errors(gpu, orc):
+    _say("case error paths")
+    coo = orc.gen_convdiff(5)
+    a = _mat(coo, P.psb_d_hip_hlg_sparse_mat(32))
+    xs = P.vect_from(np.zeros(10))               # too short
+    y = P.vect_from(np.zeros(coo.nr))
+    with pytest.raises(P.PsbError):
+        P.psb_spmm(1.0, a, xs, 0.0, y)
+    xf = P.vect_from(np.zeros(coo.nc, np.float32), mold=P.psb_s_hip_vect())
+    with pytest.raises(P.PsbError):
+        P.psb_spmm(1.0, a, xf, 0.0, y)            # precision mismatch
+    a.free()
