@@ -310,6 +310,38 @@ int psb_hip_csr_get_diag(void *dmat, void *d);
 int psb_hip_csr_get_analysis(void *dmat, int *lanes_per_row, int *long_rows,
                              int *long_threshold);
 
+/* ===================================================================== */
+/* Device-resident Krylov solvers (no reference counterpart: the reference */
+/* psb_krylov, krylov/psb_dkrylov.f90:81-227, returns every dot and nrm2    */
+/* to the host).  Same recurrences as psb_dcg_vect / psb_dcgstab_vect with  */
+/* the DIAG preconditioner, fused into 3 (CG) or 5 (BiCGSTAB) kernels per   */
+/* iteration; scalars stay on the device and the host polls the stop flag   */
+/* every check_every iterations.                                            */
+/* ===================================================================== */
+
+#define PSB_HIP_FMT_HLL 0
+#define PSB_HIP_FMT_ELL 1
+#define PSB_HIP_FMT_CSR 2
+
+/* method 0 = CG, 1 = BiCGSTAB; fmt = PSB_HIP_FMT_*; diag = Jacobi inverse
+   diagonal (psb_hip_mv_diag_inv) or NULL for no preconditioner; b, x
+   vectors of the matrix type (x holds the initial guess); istop 2
+   (||r||/||b||) or 3 (||r||/||r0||); *iter and *err as psb_krylov's iter
+   and err; hist (may be NULL) receives err per iteration (itmax entries). */
+int psb_hip_krylov_fused(int method, int fmt, void *dmat, void *diag, void *b, void *x,
+                         double eps, int itmax, int istop, int check_every, int *iter,
+                         double *err, double *hist);
+
+/* FP32-inner / FP64-outer iterative refinement on a double HLL matrix with a
+   single copy (psb_hip_hll_make_single): r = b - A x in FP64, inner solve of
+   A d = r/||r|| in FP32 (method 0 CG, 1 BiCGSTAB; Jacobi in FP32) to
+   inner_eps (loosened on the last cycle to what the outer target needs),
+   x += ||r|| d in FP64, until ||r||/||b|| <= eps.  *outer_its refinement
+   cycles, *inner_its total inner iterations, *err final ||r||/||b||. */
+int psb_hip_krylov_mixed(int method, void *dmat, void *diag, void *b, void *x, double eps,
+                         int outer_itmax, double inner_eps, int inner_itmax, int check_every,
+                         int *outer_its, int *inner_its, double *err);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,11 +59,24 @@ int ensure_init() {
 
 typedef double v2d __attribute__((ext_vector_type(2)));
 
+// Each block copies contiguous 16 KB tiles (4 x 16 B per thread, loads
+// issued before stores), grid-stride over the tiles.
 __global__ __launch_bounds__(kBlock) void k_stream_copy(const v2d* __restrict__ src,
                                                        v2d* __restrict__ dst, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+  const int64_t tile = 4 * kBlock;
+  const int64_t ntiles = n / tile;
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const v2d* s = src + t * tile + threadIdx.x;
+    v2d* d = dst + t * tile + threadIdx.x;
+    v2d a0 = s[0], a1 = s[kBlock], a2 = s[2 * kBlock], a3 = s[3 * kBlock];
+    __builtin_nontemporal_store(a0, d);
+    __builtin_nontemporal_store(a1, d + kBlock);
+    __builtin_nontemporal_store(a2, d + 2 * kBlock);
+    __builtin_nontemporal_store(a3, d + 3 * kBlock);
+  }
+  for (int64_t i = ntiles * tile + blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kBlock)
-    __builtin_nontemporal_store(__builtin_nontemporal_load(&src[i]), &dst[i]);
+    dst[i] = src[i];
 }
 
 }  // namespace psbhip
@@ -181,8 +194,8 @@ int psb_hip_stream_copy(void* dst, const void* src, int64_t nbytes) {
   if (nbytes % 16 != 0) return set_error(PSB_HIP_INVALID, __func__, "nbytes % 16");
   int64_t n = nbytes / 16;
   if (n == 0) return PSB_HIP_SUCCESS;
-  int64_t g = (n + kBlock * 8 - 1) / (kBlock * 8);
-  int64_t cap = (int64_t)(g_ctx.num_cus > 0 ? g_ctx.num_cus : 256) * 16;
+  int64_t g = (n + kBlock * 4 - 1) / (kBlock * 4);
+  int64_t cap = (int64_t)(g_ctx.num_cus > 0 ? g_ctx.num_cus : 256) * 8;
   if (g > cap) g = cap;
   hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)g), dim3(kBlock), 0, g_ctx.stream,
                      (const v2d*)src, (v2d*)dst, n);

@@ -26,89 +26,7 @@
 
 namespace psbhip {
 
-struct HllDev {
-  int elem_type;
-  int hksz, rows, nzeros, allocsize, noffs, first_index;
-  int max_col;          // largest column index stored (x must be this long)
-  void* val;            // allocsize entries of elem_type
-  int* ja;              // allocsize
-  int* hkoffs;          // noffs + 1
-  int* irn;             // rows
-  int* idiag;           // rows
-  float* val32;         // optional FP32 copy of a double matrix
-};
-
-struct EllDev {
-  int elem_type;
-  int rows, pitch, nzm, nnz, first_index;
-  int max_col;
-  void* val;            // pitch * nzm
-  int* ja;
-  int* irn;
-  int* idiag;
-};
-
-struct CsrDev {
-  int elem_type;
-  int rows, cols, nnz, first_index;
-  int max_col;
-  int* irp;             // rows + 1, 1-based
-  int* ja;
-  void* val;
-  int lanes;            // lanes per row of the vector kernel
-  int long_thr;         // rows longer than this go to the wave-per-row kernel
-  int nlong;
-  int* long_rows;       // 0-based row ids
-};
-
 static int g_hll_variant = 0;
-
-// ------------------------------------------------------------------ inner products
-// acc += sum_{j<w} val[j*stride] * x[ja[j*stride]-1], eight columns at a time:
-// all index/value loads of a group are issued before the dependent gathers.
-template <typename T, typename TV>
-__device__ __forceinline__ T slot_dot(const int* __restrict__ pj, const TV* __restrict__ pv,
-                                      int stride, int w, const T* __restrict__ x) {
-  T acc = T(0);
-  int j = 0;
-  for (; j + 8 <= w; j += 8) {
-    int c[8];
-    TV v[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      c[k] = ld_stream(pj + (size_t)(j + k) * stride);
-      v[k] = ld_stream(pv + (size_t)(j + k) * stride);
-    }
-    T xv[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) xv[k] = x[c[k] - 1];
-#pragma unroll
-    for (int k = 0; k < 8; k++) acc = fma((T)v[k], xv[k], acc);
-  }
-  if (j < w) {
-    int c[8];
-    TV v[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      bool ok = j + k < w;
-      c[k] = ok ? ld_stream(pj + (size_t)(j + k) * stride) : 1;
-      v[k] = ok ? ld_stream(pv + (size_t)(j + k) * stride) : TV(0);
-    }
-    T xv[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) xv[k] = (j + k < w) ? x[c[k] - 1] : T(0);
-#pragma unroll
-    for (int k = 0; k < 8; k++) acc = fma((T)v[k], xv[k], acc);
-  }
-  return acc;
-}
-
-template <typename T>
-__device__ __forceinline__ void store_y(T* __restrict__ y, int row, T alpha, T acc, T beta) {
-  T r = alpha * acc;
-  if (beta != T(0)) r = fma(beta, y[row], r);
-  y[row] = r;
-}
 
 // ------------------------------------------------------------------ HLL SpMV
 // HK > 0: compile-time hack size (32 or 64); HK == 0: any hksz.

@@ -5,11 +5,13 @@
 These routines issue exactly the reference's sequence of psb_spmm /
 psb_geaxpby / psb_gedot / psb_genrm2 / prec%apply calls, each running on the
 device; every scalar (dot, nrm2) returns to the host as in the reference.
-The fused device-resident solvers (one synchronisation per several
-iterations) live in psblas3_amd.solvers.
+psb_krylov_fused and psb_krylov_mixed below call the device-resident
+solvers of libpsb_hip (one synchronisation per several iterations).
 """
 from dataclasses import dataclass, field
 from typing import List
+
+import numpy as np
 
 from . import _native as N
 from .psblas import (psb_geall, psb_geamax, psb_geaxpby, psb_gedot, psb_genrm2, psb_spmm)
@@ -78,7 +80,7 @@ def psb_init_conv(methdname, istop, itrace, itmax, a, x, b, eps, desc_a):
     return s
 
 
-def psb_check_conv(methdname, it, x, r, desc_a, s):
+def psb_check_conv(methdname, it, x, r, desc_a, s, record=True):
     """psb_d_check_conv_vect (psb_d_krylov_conv_mod.f90:279-356)."""
     if s.istop == 1:
         rni = psb_geamax(r, desc_a)
@@ -92,7 +94,8 @@ def psb_check_conv(methdname, it, x, r, desc_a, s):
         res = s.errnum <= s.eps
     else:
         res = s.errnum <= s.eps * s.errden
-    s.history.append(s.errnum / s.errden if s.errden > 0 else s.errnum)
+    if record:
+        s.history.append(s.errnum / s.errden if s.errden > 0 else s.errnum)
     res = res or (s.itmax <= it)
     if s.itrace > 0 and (it % s.itrace == 0 or res):
         e = s.errnum / s.errden if s.errden > 0 else s.errnum
@@ -170,7 +173,7 @@ def psb_dcgstab_vect(a, prec, b, x, eps, desc_a, itmax=1000, itrace=0, istop=2):
             psb_geaxpby(1.0, b, 0.0, r, desc_a)
             psb_spmm(-1.0, a, x, 1.0, r, desc_a)
             psb_geaxpby(1.0, r, 0.0, q, desc_a)
-            if psb_check_conv(methd, itx, x, r, desc_a, s):
+            if psb_check_conv(methd, itx, x, r, desc_a, s, record=False):
                 break
             rho = 0.0
             while True:
@@ -228,3 +231,53 @@ def psb_krylov(method, a, prec, b, x, eps, desc_a, itmax=1000, itrace=0, istop=2
     if m == "CG":
         return psb_dcg_vect(a, prec, b, x, eps, desc_a, itmax, itrace, istop)
     return psb_dcgstab_vect(a, prec, b, x, eps, desc_a, itmax, itrace, istop)
+
+
+_FMT = {"HLG": 0, "ELG": 1, "CSRG": 2}
+
+
+def psb_krylov_fused(method, a, prec, b, x, eps, desc_a=None, itmax=1000, istop=2,
+                     check_every=8, history=False):
+    """Device-resident CG / BiCGSTAB (same recurrences as psb_krylov, scalars
+    kept on the device, one host poll every check_every iterations).
+    Returns (iter, err, history)."""
+    import ctypes as C
+    import numpy as np
+    if not isinstance(x, psb_d_vect_type) or not isinstance(b, psb_d_vect_type):
+        raise N.PsbError(N.INVALID, "psb_krylov_fused", "b and x must be psb_d_vect_type")
+    b.v.ensure_dev()
+    x.v.ensure_dev()
+    diag = None
+    if prec is not None and prec.ptype == "DIAG":
+        diag = prec.dv.dev
+    it, err = C.c_int(0), C.c_double(0.0)
+    hist = np.zeros(max(itmax, 1)) if history else None
+    meth = 0 if method.upper() == "CG" else 1
+    N.call("psb_hip_krylov_fused", meth, _FMT[a.a.get_fmt()], a.a.dev, diag, b.v.dev, x.v.dev,
+           float(eps), int(itmax), int(istop), int(check_every), C.byref(it), C.byref(err),
+           hist.ctypes.data if history else None)
+    x.v.set_dev()
+    h = list(hist[:it.value]) if history else []
+    return it.value, err.value, h
+
+
+def psb_krylov_mixed(method, a, prec, b, x, eps, desc_a=None, outer_itmax=50, inner_eps=1e-4,
+                     inner_itmax=1000, check_every=8):
+    """FP32-inner / FP64-outer iterative refinement (HLG matrices).  Builds the
+    FP32 value copy on first use.  Returns (outer_its, inner_its, err)."""
+    import ctypes as C
+    if a.a.get_fmt() != "HLG" or a.a._dtype != np.float64:
+        raise N.PsbError(N.UNSUPPORTED, "psb_krylov_mixed", "needs a double HLG matrix")
+    if not getattr(a.a, "_has_single", False):
+        N.call("psb_hip_hll_make_single", a.a.dev)
+        a.a._has_single = True
+    b.v.ensure_dev()
+    x.v.ensure_dev()
+    diag = prec.dv.dev if (prec is not None and prec.ptype == "DIAG") else None
+    oi, ii, err = C.c_int(0), C.c_int(0), C.c_double(0.0)
+    meth = 0 if method.upper() == "CG" else 1
+    N.call("psb_hip_krylov_mixed", meth, a.a.dev, diag, b.v.dev, x.v.dev, float(eps),
+           int(outer_itmax), float(inner_eps), int(inner_itmax), int(check_every),
+           C.byref(oi), C.byref(ii), C.byref(err))
+    x.v.set_dev()
+    return oi.value, ii.value, err.value

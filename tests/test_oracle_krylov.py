@@ -9,6 +9,8 @@ import pytest
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
+from helpers import bicgstab_spread
+
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 
@@ -85,3 +87,17 @@ def test_istop_variants(orc):
         ani = float(abs(A).sum(axis=1).max())
         x, it, err, _ = orc.krylov("BICGSTAB", csr, b, eps=1e-10, istop=istop, ani=ani)
         assert err <= 1e-10 and it > 0
+
+
+def test_bicgstab_rounding_spread_pde3d(orc):
+    """BiCGSTAB on the pde3d sample is sensitive to rounding: perturbing b by a
+    few ulps moves the iteration count by more than +-2, and the compiled
+    reference's 80 lies inside that spread.  GPU BiCGSTAB parity is therefore
+    checked against this spread, CG against +-2."""
+    ref = json.load(open(os.path.join(GOLDEN, "reference_runs.json")))["pde3d_L40_bicgstab_diag"]
+    coo, rhs = orc.gen_pde3d(40, a=ref["coefficients"]["a"], b=(0, 0, 0), c=0.0, hscale=True,
+                             gflag=True)
+    csr = orc.csr_from_coo(orc.fix_coo(coo))
+    lo, hi = bicgstab_spread(orc, csr, rhs, ref["eps"])
+    assert lo <= ref["iterations"] <= hi
+    assert hi - lo > 4
