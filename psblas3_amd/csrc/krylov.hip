@@ -182,7 +182,9 @@ __global__ __launch_bounds__(kBlock) void k_restart(M m, const T* __restrict__ b
     st->breakdown = 0;
     st->it = 0;
     st->errnum = sqrt(v[0]);
-    if (st->istop == 3) st->errden = st->errnum;  // ||r0||
+    // ||r0||: CG re-initialises at every restart (psb_dcg.F90:231), BiCGSTAB
+    // once before its restart loop (psb_dcgstab.f90:220)
+    if (st->istop == 3 && (!bicg || st->itx == 0)) st->errden = st->errnum;
     if (bicg) {
       // BiCGSTAB checks convergence before the first iteration
       bool res = st->errden == 0.0 ? st->errnum <= st->eps : st->errnum <= st->eps * st->errden;
