@@ -46,19 +46,26 @@ def time_spmv(P, a, xv, yv, steps, warmup):
     return ms / steps
 
 
-def stream_bandwidth(N, nbytes=2 << 30, reps=20):
+def stream_bandwidth(N, nbytes=1 << 30, reps=20):
+    """Measured HBM bandwidth: copy (2 arrays) and STREAM triad (3 arrays)."""
     import ctypes as C
-    src, dst = C.c_void_p(), C.c_void_p()
-    N.call("psb_hip_alloc_buffer", C.byref(src), nbytes)
-    N.call("psb_hip_alloc_buffer", C.byref(dst), nbytes)
-    N.call("psb_hip_stream_copy", dst, src, nbytes)
-    N.call("psb_hip_timer_start")
-    for _ in range(reps):
-        N.call("psb_hip_stream_copy", dst, src, nbytes)
-    ms = N.fn("psb_hip_timer_stop")() / reps
-    N.call("psb_hip_free_buffer", src)
-    N.call("psb_hip_free_buffer", dst)
-    return 2 * nbytes / (ms * 1e-3) / 1e9
+    bufs = [C.c_void_p() for _ in range(3)]
+    for b in bufs:
+        N.call("psb_hip_alloc_buffer", C.byref(b), nbytes)
+    out = {}
+    for name, narr, launch in (
+            ("copy", 2, lambda: N.call("psb_hip_stream_copy", bufs[0], bufs[1], nbytes)),
+            ("triad", 3, lambda: N.call("psb_hip_stream_triad", bufs[0], bufs[1], bufs[2], 0.5,
+                                        nbytes))):
+        launch()
+        N.call("psb_hip_timer_start")
+        for _ in range(reps):
+            launch()
+        ms = N.fn("psb_hip_timer_stop")() / reps
+        out[name] = narr * nbytes / (ms * 1e-3) / 1e9
+    for b in bufs:
+        N.call("psb_hip_free_buffer", b)
+    return out
 
 
 def gen_coo(P, N, kind, size):
@@ -83,6 +90,78 @@ def gen_coo(P, N, kind, size):
     coo.fix()
     del C
     return coo
+
+
+def solve_bench(P, N, L=256, eps=1e-10):
+    """FP64 fused CG vs FP32-inner / FP64-outer refinement, SPD Poisson (6,-1)
+    on L^3 (HLL, hksz 64, DIAG), b = A*x_true, x_true ~ U[-1,1) seed 2."""
+    import numpy as np
+    t0 = time.perf_counter()
+    a3 = np.ones(3)
+    b3 = np.zeros(3)
+    fn = N.fn("psb_hip_gen_pde3d")
+    nz = fn(L, a3.ctypes.data, b3.ctypes.data, 0.0, 0, 0, None, None, None, None)
+    ia = np.empty(nz, np.int32); ja = np.empty(nz, np.int32); val = np.empty(nz)
+    fn(L, a3.ctypes.data, b3.ctypes.data, 0.0, 0, 0, ia.ctypes.data, ja.ctypes.data,
+       val.ctypes.data, None)
+    n = L ** 3
+    coo = P.psb_d_coo_sparse_mat(n, n, ia, ja, val)
+    coo.fix()
+    del ia, ja, val
+    a = P.psb_dspmat_type()
+    a.m = a.n = n
+    a.a = P.psb_d_hip_hlg_sparse_mat(hksz=64).cp_from_coo(coo)
+    del coo
+    desc = P.psb_cdall(nl=n)
+    prec = P.psb_dprec_type("DIAG").build(a, desc)
+    xt = P.psb_d_hip_vect().all(n)
+    N.call("psb_hip_mv_fill_uniform", xt.dev, 2, 0)
+    xt.set_dev()
+    b = P.psb_geall(desc)
+    P.psb_spmm(1.0, a, P.psb_d_vect_type(xt), 0.0, b)
+    log(f"solve: Poisson L={L} n={n} nnz={nz} set up in {time.perf_counter() - t0:.1f}s")
+    xtrue = xt.get_vect()
+    out = {"matrix": f"SPD Poisson 7-point L={L}", "n": n, "nnz": int(nz), "eps": eps}
+    # warm both paths (kernel loads, first allocations)
+    x = P.psb_geall(desc)
+    P.psb_krylov_fused("CG", a, prec, b, x, eps, desc, itmax=3)
+    P.psb_krylov_mixed("CG", a, prec, b, x, eps, desc, outer_itmax=1, inner_itmax=3)
+    res = {}
+    for name in ("fp64", "mixed"):
+        x = P.psb_geall(desc)
+        P.psb_hip_DeviceSync()
+        t0 = time.perf_counter()
+        if name == "fp64":
+            it, err, _ = P.psb_krylov_fused("CG", a, prec, b, x, eps, desc, itmax=5000,
+                                            check_every=16)
+            info = {"iterations": it}
+        else:
+            oi, ii, err = P.psb_krylov_mixed("CG", a, prec, b, x, eps, desc, outer_itmax=30,
+                                             inner_eps=1e-4, inner_itmax=5000, check_every=16)
+            info = {"outer": oi, "inner_iterations": ii}
+        P.psb_hip_DeviceSync()
+        dt = time.perf_counter() - t0
+        xs = x.get_vect()
+        res[name] = xs
+        e = float(np.linalg.norm(xs - xtrue) / np.linalg.norm(xtrue))
+        out[name] = dict(info, seconds=round(dt, 4), final_rel_resid=err, err_vs_xtrue=e)
+        log(f"solve: {name} {dt * 1e3:.1f} ms {info} resid {err:.2e} |x-xt|/|xt| {e:.2e}")
+    out["mixed_vs_fp64_solution"] = float(np.linalg.norm(res["mixed"] - res["fp64"])
+                                          / np.linalg.norm(res["fp64"]))
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["seconds"] / out["mixed"]["seconds"], 3)
+    a.free()
+    return out
+
+
+def profiled_traffic(L):
+    """HBM bytes per launch of the headline kernel, from the committed
+    rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled)."""
+    import re
+    path = os.path.join(ROOT, "profiles", "spmv_hll_fp64_stencil_L200.md")
+    if L != 200 or not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch = (\d+)", open(path).read())
+    return int(m.group(1)) if m else None
 
 
 def cpu_baseline(seconds=12.0, L=128):
@@ -117,6 +196,8 @@ def main():
     ap.add_argument("--powerlaw-n", type=int, default=4_000_000)
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--solve-L", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -130,8 +211,9 @@ def main():
     P.psb_init(0)
     dev = P.psb_hip_DeviceName()
     log(f"bench: device {dev}, {P.psb_hip_MultiProcessors()} CUs")
-    stream = stream_bandwidth(N)
-    log(f"bench: stream copy {stream:.0f} GB/s")
+    streams = stream_bandwidth(N)
+    stream = max(streams.values())
+    log(f"bench: stream copy {streams['copy']:.0f} GB/s, triad {streams['triad']:.0f} GB/s")
 
     t0 = time.perf_counter()
     coo_a = gen_coo(P, N, "stencil", args.L)
@@ -212,6 +294,7 @@ def main():
         sweep["B_ELL"] = "not representable (n x max_row_len slots)"
         sweep["B_n"], sweep["B_nnz"] = nb, nnzb
 
+    solve = None if args.no_solve else solve_bench(P, N, args.solve_L)
     cpu = None if args.no_cpu else cpu_baseline()
     if cpu:
         log(f"bench: cpu baseline {cpu['value']} GB/s on {cpu['cores']} threads")
@@ -233,11 +316,13 @@ def main():
                    "n": n, "nnz": nnz, "hksz": 64, "bytes_per_product": nbytes,
                    "gflops": round(gflops, 2), "device": dev},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "stream_copy_GBs": round(stream, 1),
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(args.L),
+                     "stream_copy_GBs": round(streams["copy"], 1),
+                     "stream_triad_GBs": round(streams["triad"], 1),
                      "frac_of_stream": round(gbs / stream, 4)},
         "cpu_baseline": cpu,
         "sweep": sweep,
+        "solve": solve,
     }
     print(json.dumps(line), flush=True)
 

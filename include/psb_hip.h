@@ -90,6 +90,8 @@ double psb_hip_timer_stop(void);
    roofline denominator (no reference counterpart: the reference only
    reports the spec figure of psb_cuda_MemoryPeakBandwidth). */
 int psb_hip_stream_copy(void *dst, const void *src, int64_t nbytes);
+/* STREAM triad a = b + s*c over nbytes per array (multiple of 16). */
+int psb_hip_stream_triad(void *a, const void *b, const void *c, double s, int64_t nbytes);
 
 /* ===================================================================== */
 /* Raw device buffers -- cuda/psb_d_vectordev_mod.F90:85-160               */

@@ -79,6 +79,27 @@ __global__ __launch_bounds__(kBlock) void k_stream_copy(const v2d* __restrict__ 
     dst[i] = src[i];
 }
 
+// STREAM triad a = b + s*c on 16-byte vectors, 4 per thread per tile.
+__global__ __launch_bounds__(kBlock) void k_stream_triad(v2d* __restrict__ a,
+                                                        const v2d* __restrict__ b,
+                                                        const v2d* __restrict__ c, double s,
+                                                        int64_t n) {
+  const int64_t tile = 4 * kBlock;
+  const int64_t ntiles = n / tile;
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int64_t o = t * tile + threadIdx.x;
+    v2d b0 = b[o], b1 = b[o + kBlock], b2 = b[o + 2 * kBlock], b3 = b[o + 3 * kBlock];
+    v2d c0 = c[o], c1 = c[o + kBlock], c2 = c[o + 2 * kBlock], c3 = c[o + 3 * kBlock];
+    __builtin_nontemporal_store(b0 + s * c0, a + o);
+    __builtin_nontemporal_store(b1 + s * c1, a + o + kBlock);
+    __builtin_nontemporal_store(b2 + s * c2, a + o + 2 * kBlock);
+    __builtin_nontemporal_store(b3 + s * c3, a + o + 3 * kBlock);
+  }
+  for (int64_t i = ntiles * tile + blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    a[i] = b[i] + s * c[i];
+}
+
 }  // namespace psbhip
 
 using namespace psbhip;
@@ -199,6 +220,21 @@ int psb_hip_stream_copy(void* dst, const void* src, int64_t nbytes) {
   if (g > cap) g = cap;
   hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)g), dim3(kBlock), 0, g_ctx.stream,
                      (const v2d*)src, (v2d*)dst, n);
+  PSB_HIP_LAUNCH_CHECK();
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_stream_triad(void* a, const void* b, const void* c, double s, int64_t nbytes) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (nbytes % 16 != 0) return set_error(PSB_HIP_INVALID, __func__, "nbytes % 16");
+  int64_t n = nbytes / 16;
+  if (n == 0) return PSB_HIP_SUCCESS;
+  int64_t g = (n + kBlock * 4 - 1) / (kBlock * 4);
+  int64_t cap = (int64_t)(g_ctx.num_cus > 0 ? g_ctx.num_cus : 256) * 8;
+  if (g > cap) g = cap;
+  hipLaunchKernelGGL(k_stream_triad, dim3((unsigned)g), dim3(kBlock), 0, g_ctx.stream, (v2d*)a,
+                     (const v2d*)b, (const v2d*)c, s, n);
   PSB_HIP_LAUNCH_CHECK();
   return PSB_HIP_SUCCESS;
 }

@@ -1,0 +1,185 @@
+! psb_hip_cbind_mod: iso_c_binding interfaces to libpsb_hip (include/psb_hip.h).
+!
+! This is the module a PSBLAS maintainer adds in place of the bind(c) sets of
+! cuda/hlldev_mod.F90, cuda/elldev_mod.F90, cuda/psb_d_vectordev_mod.F90 and
+! cuda/psb_cuda_env_mod.F90: every interface below binds one entry point of the
+! C ABI with the same argument meaning as the CUDA bridge function it replaces.
+module psb_hip_cbind_mod
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: psb_hip_success_     = 0
+  integer(c_int), parameter :: psb_hip_unsupported_ = 1
+  integer(c_int), parameter :: psb_hip_invalid_     = 2
+  integer(c_int), parameter :: psb_hip_outofmem_    = 3
+  integer(c_int), parameter :: psb_hip_type_int_    = 0
+  integer(c_int), parameter :: psb_hip_type_float_  = 1
+  integer(c_int), parameter :: psb_hip_type_double_ = 2
+  integer(c_int), parameter :: psb_hip_fmt_hll_     = 0
+  integer(c_int), parameter :: psb_hip_fmt_ell_     = 1
+  integer(c_int), parameter :: psb_hip_fmt_csr_     = 2
+
+  interface
+    ! ---------------------------------------------------------------- environment
+    function psb_hip_init(dev) result(res) bind(c, name='psb_hip_init')
+      import :: c_int
+      integer(c_int), value :: dev
+      integer(c_int) :: res
+    end function psb_hip_init
+    function psb_hip_exit() result(res) bind(c, name='psb_hip_exit')
+      import :: c_int
+      integer(c_int) :: res
+    end function psb_hip_exit
+    function psb_hip_device_sync() result(res) bind(c, name='psb_hip_device_sync')
+      import :: c_int
+      integer(c_int) :: res
+    end function psb_hip_device_sync
+    function psb_hip_get_multiprocessors() result(res) bind(c, name='psb_hip_get_multiprocessors')
+      import :: c_int
+      integer(c_int) :: res
+    end function psb_hip_get_multiprocessors
+    function psb_hip_memory_peak_bandwidth() result(res) &
+         & bind(c, name='psb_hip_memory_peak_bandwidth')
+      import :: c_double
+      real(c_double) :: res
+    end function psb_hip_memory_peak_bandwidth
+
+    ! ---------------------------------------------------------------- multivectors
+    function psb_hip_mv_alloc(mv, count, size, elem_type) result(res) &
+         & bind(c, name='psb_hip_mv_alloc')
+      import :: c_int, c_ptr
+      type(c_ptr) :: mv
+      integer(c_int), value :: count, size, elem_type
+      integer(c_int) :: res
+    end function psb_hip_mv_alloc
+    subroutine psb_hip_mv_free(mv) bind(c, name='psb_hip_mv_free')
+      import :: c_ptr
+      type(c_ptr), value :: mv
+    end subroutine psb_hip_mv_free
+    function psb_hip_mv_write(mv, host) result(res) bind(c, name='psb_hip_mv_write')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: mv, host
+      integer(c_int) :: res
+    end function psb_hip_mv_write
+    function psb_hip_mv_read(mv, host) result(res) bind(c, name='psb_hip_mv_read')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: mv, host
+      integer(c_int) :: res
+    end function psb_hip_mv_read
+    function psb_hip_mv_dot(res_, n, x, y) result(res) bind(c, name='psb_hip_mv_dot')
+      import :: c_int, c_ptr, c_double
+      real(c_double) :: res_(*)
+      integer(c_int), value :: n
+      type(c_ptr), value :: x, y
+      integer(c_int) :: res
+    end function psb_hip_mv_dot
+    function psb_hip_mv_nrm2(res_, n, x) result(res) bind(c, name='psb_hip_mv_nrm2')
+      import :: c_int, c_ptr, c_double
+      real(c_double) :: res_(*)
+      integer(c_int), value :: n
+      type(c_ptr), value :: x
+      integer(c_int) :: res
+    end function psb_hip_mv_nrm2
+    function psb_hip_mv_axpby(n, alpha, x, beta, y) result(res) bind(c, name='psb_hip_mv_axpby')
+      import :: c_int, c_ptr, c_double
+      integer(c_int), value :: n
+      real(c_double), value :: alpha, beta
+      type(c_ptr), value :: x, y
+      integer(c_int) :: res
+    end function psb_hip_mv_axpby
+    function psb_hip_mv_axybz(n, alpha, x, y, beta, z) result(res) bind(c, name='psb_hip_mv_axybz')
+      import :: c_int, c_ptr, c_double
+      integer(c_int), value :: n
+      real(c_double), value :: alpha, beta
+      type(c_ptr), value :: x, y, z
+      integer(c_int) :: res
+    end function psb_hip_mv_axybz
+    function psb_hip_mv_diag_inv(n, d) result(res) bind(c, name='psb_hip_mv_diag_inv')
+      import :: c_int, c_ptr
+      integer(c_int), value :: n
+      type(c_ptr), value :: d
+      integer(c_int) :: res
+    end function psb_hip_mv_diag_inv
+
+    ! ---------------------------------------------------------------- host assembly
+    function psb_hip_fix_coo_d(nr, nc, nz, ia, ja, val, dupl, nzout) result(res) &
+         & bind(c, name='psb_hip_fix_coo_d')
+      import :: c_int, c_int64_t, c_double
+      integer(c_int), value :: nr, nc, dupl
+      integer(c_int64_t), value :: nz
+      integer(c_int) :: ia(*), ja(*)
+      real(c_double) :: val(*)
+      integer(c_int64_t) :: nzout
+      integer(c_int) :: res
+    end function psb_hip_fix_coo_d
+    function psb_hip_hll_offsets(nr, nz, ia, hksz, irn, idisp, hkoffs, isz) result(res) &
+         & bind(c, name='psb_hip_hll_offsets')
+      import :: c_int, c_int64_t
+      integer(c_int), value :: nr, hksz
+      integer(c_int64_t), value :: nz
+      integer(c_int) :: ia(*), irn(*), idisp(*), hkoffs(*)
+      integer(c_int64_t) :: isz
+      integer(c_int) :: res
+    end function psb_hip_hll_offsets
+
+    ! ---------------------------------------------------------------- HLL (HLG)
+    function psb_hip_hll_alloc(dmat, hksz, rows, nzeros, allocsize, elem_type, first_index) &
+         & result(res) bind(c, name='psb_hip_hll_alloc')
+      import :: c_int, c_ptr
+      type(c_ptr) :: dmat
+      integer(c_int), value :: hksz, rows, nzeros, allocsize, elem_type, first_index
+      integer(c_int) :: res
+    end function psb_hip_hll_alloc
+    subroutine psb_hip_hll_free(dmat) bind(c, name='psb_hip_hll_free')
+      import :: c_ptr
+      type(c_ptr), value :: dmat
+    end subroutine psb_hip_hll_free
+    function psb_hip_hll_copy_coo(nr, nc, nza, hksz, noffs, isz, irn, hoffs, idisp, ja, val, &
+         & dmat) result(res) bind(c, name='psb_hip_hll_copy_coo')
+      import :: c_int, c_ptr, c_double
+      integer(c_int), value :: nr, nc, nza, hksz, noffs, isz
+      integer(c_int) :: irn(*), hoffs(*), idisp(*), ja(*)
+      real(c_double) :: val(*)
+      type(c_ptr), value :: dmat
+      integer(c_int) :: res
+    end function psb_hip_hll_copy_coo
+    function psb_hip_hll_spmv(dmat, alpha, x, beta, y) result(res) bind(c, name='psb_hip_hll_spmv')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: dmat, x, y
+      real(c_double), value :: alpha, beta
+      integer(c_int) :: res
+    end function psb_hip_hll_spmv
+    function psb_hip_hll_get_diag(dmat, d) result(res) bind(c, name='psb_hip_hll_get_diag')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: dmat, d
+      integer(c_int) :: res
+    end function psb_hip_hll_get_diag
+
+    ! ---------------------------------------------------------------- ELL / CSR
+    function psb_hip_ell_spmv(dmat, alpha, x, beta, y) result(res) bind(c, name='psb_hip_ell_spmv')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: dmat, x, y
+      real(c_double), value :: alpha, beta
+      integer(c_int) :: res
+    end function psb_hip_ell_spmv
+    function psb_hip_csr_spmv(dmat, alpha, x, beta, y) result(res) bind(c, name='psb_hip_csr_spmv')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: dmat, x, y
+      real(c_double), value :: alpha, beta
+      integer(c_int) :: res
+    end function psb_hip_csr_spmv
+
+    ! ---------------------------------------------------------------- solvers
+    function psb_hip_krylov_fused(method, fmt, dmat, diag, b, x, eps, itmax, istop, &
+         & check_every, iter, err, hist) result(res) bind(c, name='psb_hip_krylov_fused')
+      import :: c_int, c_ptr, c_double
+      integer(c_int), value :: method, fmt, itmax, istop, check_every
+      type(c_ptr), value :: dmat, diag, b, x, hist
+      real(c_double), value :: eps
+      integer(c_int) :: iter
+      real(c_double) :: err
+      integer(c_int) :: res
+    end function psb_hip_krylov_fused
+  end interface
+
+end module psb_hip_cbind_mod

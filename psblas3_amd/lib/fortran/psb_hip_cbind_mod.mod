@@ -1,0 +1,320 @@
+﻿!mod$ v1 sum:c1da9781114e48aa
+!need$ 0bde2ac47243ead2 i iso_c_binding
+module psb_hip_cbind_mod
+use,intrinsic::iso_c_binding,only:c_associated
+use,intrinsic::iso_c_binding,only:c_funloc
+use,intrinsic::iso_c_binding,only:c_funptr
+use,intrinsic::iso_c_binding,only:c_f_pointer
+use,intrinsic::iso_c_binding,only:c_loc
+use,intrinsic::iso_c_binding,only:c_null_funptr
+use,intrinsic::iso_c_binding,only:c_null_ptr
+use,intrinsic::iso_c_binding,only:c_ptr
+use,intrinsic::iso_c_binding,only:c_sizeof
+use,intrinsic::iso_c_binding,only:operator(==)
+use,intrinsic::iso_c_binding,only:operator(/=)
+use,intrinsic::iso_c_binding,only:c_int8_t
+use,intrinsic::iso_c_binding,only:c_int16_t
+use,intrinsic::iso_c_binding,only:c_int32_t
+use,intrinsic::iso_c_binding,only:c_int64_t
+use,intrinsic::iso_c_binding,only:c_int128_t
+use,intrinsic::iso_c_binding,only:c_int
+use,intrinsic::iso_c_binding,only:c_short
+use,intrinsic::iso_c_binding,only:c_long
+use,intrinsic::iso_c_binding,only:c_long_long
+use,intrinsic::iso_c_binding,only:c_signed_char
+use,intrinsic::iso_c_binding,only:c_size_t
+use,intrinsic::iso_c_binding,only:c_intmax_t
+use,intrinsic::iso_c_binding,only:c_intptr_t
+use,intrinsic::iso_c_binding,only:c_ptrdiff_t
+use,intrinsic::iso_c_binding,only:c_int_least8_t
+use,intrinsic::iso_c_binding,only:c_int_fast8_t
+use,intrinsic::iso_c_binding,only:c_int_least16_t
+use,intrinsic::iso_c_binding,only:c_int_fast16_t
+use,intrinsic::iso_c_binding,only:c_int_least32_t
+use,intrinsic::iso_c_binding,only:c_int_fast32_t
+use,intrinsic::iso_c_binding,only:c_int_least64_t
+use,intrinsic::iso_c_binding,only:c_int_fast64_t
+use,intrinsic::iso_c_binding,only:c_int_least128_t
+use,intrinsic::iso_c_binding,only:c_int_fast128_t
+use,intrinsic::iso_c_binding,only:c_float
+use,intrinsic::iso_c_binding,only:c_double
+use,intrinsic::iso_c_binding,only:c_long_double
+use,intrinsic::iso_c_binding,only:c_float_complex
+use,intrinsic::iso_c_binding,only:c_double_complex
+use,intrinsic::iso_c_binding,only:c_long_double_complex
+use,intrinsic::iso_c_binding,only:c_bool
+use,intrinsic::iso_c_binding,only:c_char
+use,intrinsic::iso_c_binding,only:c_null_char
+use,intrinsic::iso_c_binding,only:c_alert
+use,intrinsic::iso_c_binding,only:c_backspace
+use,intrinsic::iso_c_binding,only:c_form_feed
+use,intrinsic::iso_c_binding,only:c_new_line
+use,intrinsic::iso_c_binding,only:c_carriage_return
+use,intrinsic::iso_c_binding,only:c_horizontal_tab
+use,intrinsic::iso_c_binding,only:c_vertical_tab
+use,intrinsic::iso_c_binding,only:c_float128
+use,intrinsic::iso_c_binding,only:c_float128_complex
+use,intrinsic::iso_c_binding,only:c_uint8_t
+use,intrinsic::iso_c_binding,only:c_uint16_t
+use,intrinsic::iso_c_binding,only:c_uint32_t
+use,intrinsic::iso_c_binding,only:c_uint64_t
+use,intrinsic::iso_c_binding,only:c_uint128_t
+use,intrinsic::iso_c_binding,only:c_unsigned_char
+use,intrinsic::iso_c_binding,only:c_unsigned_short
+use,intrinsic::iso_c_binding,only:c_unsigned
+use,intrinsic::iso_c_binding,only:c_unsigned_long
+use,intrinsic::iso_c_binding,only:c_unsigned_long_long
+use,intrinsic::iso_c_binding,only:c_uintmax_t
+use,intrinsic::iso_c_binding,only:c_uint_fast8_t
+use,intrinsic::iso_c_binding,only:c_uint_fast16_t
+use,intrinsic::iso_c_binding,only:c_uint_fast32_t
+use,intrinsic::iso_c_binding,only:c_uint_fast64_t
+use,intrinsic::iso_c_binding,only:c_uint_fast128_t
+use,intrinsic::iso_c_binding,only:c_uint_least8_t
+use,intrinsic::iso_c_binding,only:c_uint_least16_t
+use,intrinsic::iso_c_binding,only:c_uint_least32_t
+use,intrinsic::iso_c_binding,only:c_uint_least64_t
+use,intrinsic::iso_c_binding,only:c_uint_least128_t
+use,intrinsic::iso_c_binding,only:c_f_procpointer
+integer(4),parameter::psb_hip_success_=0_4
+integer(4),parameter::psb_hip_unsupported_=1_4
+integer(4),parameter::psb_hip_invalid_=2_4
+integer(4),parameter::psb_hip_outofmem_=3_4
+integer(4),parameter::psb_hip_type_int_=0_4
+integer(4),parameter::psb_hip_type_float_=1_4
+integer(4),parameter::psb_hip_type_double_=2_4
+integer(4),parameter::psb_hip_fmt_hll_=0_4
+integer(4),parameter::psb_hip_fmt_ell_=1_4
+integer(4),parameter::psb_hip_fmt_csr_=2_4
+interface
+function psb_hip_init(dev) bind(c,name="psb_hip_init") result(res)
+integer(4),value::dev
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_exit() bind(c,name="psb_hip_exit") result(res)
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_device_sync() bind(c,name="psb_hip_device_sync") result(res)
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_get_multiprocessors() bind(c,name="psb_hip_get_multiprocessors") result(res)
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_memory_peak_bandwidth() bind(c,name="psb_hip_memory_peak_bandwidth") result(res)
+real(8)::res
+end
+end interface
+interface
+function psb_hip_mv_alloc(mv,count,size,elem_type) bind(c,name="psb_hip_mv_alloc") result(res)
+import::c_ptr
+type(c_ptr)::mv
+integer(4),value::count
+integer(4),value::size
+integer(4),value::elem_type
+integer(4)::res
+end
+end interface
+interface
+subroutine psb_hip_mv_free(mv) bind(c,name="psb_hip_mv_free")
+import::c_ptr
+type(c_ptr),value::mv
+end
+end interface
+interface
+function psb_hip_mv_write(mv,host) bind(c,name="psb_hip_mv_write") result(res)
+import::c_ptr
+type(c_ptr),value::mv
+type(c_ptr),value::host
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_mv_read(mv,host) bind(c,name="psb_hip_mv_read") result(res)
+import::c_ptr
+type(c_ptr),value::mv
+type(c_ptr),value::host
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_mv_dot(res_,n,x,y) bind(c,name="psb_hip_mv_dot") result(res)
+import::c_ptr
+real(8)::res_(1_8:*)
+integer(4),value::n
+type(c_ptr),value::x
+type(c_ptr),value::y
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_mv_nrm2(res_,n,x) bind(c,name="psb_hip_mv_nrm2") result(res)
+import::c_ptr
+real(8)::res_(1_8:*)
+integer(4),value::n
+type(c_ptr),value::x
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_mv_axpby(n,alpha,x,beta,y) bind(c,name="psb_hip_mv_axpby") result(res)
+import::c_ptr
+integer(4),value::n
+real(8),value::alpha
+type(c_ptr),value::x
+real(8),value::beta
+type(c_ptr),value::y
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_mv_axybz(n,alpha,x,y,beta,z) bind(c,name="psb_hip_mv_axybz") result(res)
+import::c_ptr
+integer(4),value::n
+real(8),value::alpha
+type(c_ptr),value::x
+type(c_ptr),value::y
+real(8),value::beta
+type(c_ptr),value::z
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_mv_diag_inv(n,d) bind(c,name="psb_hip_mv_diag_inv") result(res)
+import::c_ptr
+integer(4),value::n
+type(c_ptr),value::d
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_fix_coo_d(nr,nc,nz,ia,ja,val,dupl,nzout) bind(c,name="psb_hip_fix_coo_d") result(res)
+integer(4),value::nr
+integer(4),value::nc
+integer(8),value::nz
+integer(4)::ia(1_8:*)
+integer(4)::ja(1_8:*)
+real(8)::val(1_8:*)
+integer(4),value::dupl
+integer(8)::nzout
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_hll_offsets(nr,nz,ia,hksz,irn,idisp,hkoffs,isz) bind(c,name="psb_hip_hll_offsets") result(res)
+integer(4),value::nr
+integer(8),value::nz
+integer(4)::ia(1_8:*)
+integer(4),value::hksz
+integer(4)::irn(1_8:*)
+integer(4)::idisp(1_8:*)
+integer(4)::hkoffs(1_8:*)
+integer(8)::isz
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_hll_alloc(dmat,hksz,rows,nzeros,allocsize,elem_type,first_index) bind(c,name="psb_hip_hll_alloc") result(res)
+import::c_ptr
+type(c_ptr)::dmat
+integer(4),value::hksz
+integer(4),value::rows
+integer(4),value::nzeros
+integer(4),value::allocsize
+integer(4),value::elem_type
+integer(4),value::first_index
+integer(4)::res
+end
+end interface
+interface
+subroutine psb_hip_hll_free(dmat) bind(c,name="psb_hip_hll_free")
+import::c_ptr
+type(c_ptr),value::dmat
+end
+end interface
+interface
+function psb_hip_hll_copy_coo(nr,nc,nza,hksz,noffs,isz,irn,hoffs,idisp,ja,val,dmat) bind(c,name="psb_hip_hll_copy_coo") result(res)
+import::c_ptr
+integer(4),value::nr
+integer(4),value::nc
+integer(4),value::nza
+integer(4),value::hksz
+integer(4),value::noffs
+integer(4),value::isz
+integer(4)::irn(1_8:*)
+integer(4)::hoffs(1_8:*)
+integer(4)::idisp(1_8:*)
+integer(4)::ja(1_8:*)
+real(8)::val(1_8:*)
+type(c_ptr),value::dmat
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_hll_spmv(dmat,alpha,x,beta,y) bind(c,name="psb_hip_hll_spmv") result(res)
+import::c_ptr
+type(c_ptr),value::dmat
+real(8),value::alpha
+type(c_ptr),value::x
+real(8),value::beta
+type(c_ptr),value::y
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_hll_get_diag(dmat,d) bind(c,name="psb_hip_hll_get_diag") result(res)
+import::c_ptr
+type(c_ptr),value::dmat
+type(c_ptr),value::d
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_ell_spmv(dmat,alpha,x,beta,y) bind(c,name="psb_hip_ell_spmv") result(res)
+import::c_ptr
+type(c_ptr),value::dmat
+real(8),value::alpha
+type(c_ptr),value::x
+real(8),value::beta
+type(c_ptr),value::y
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_csr_spmv(dmat,alpha,x,beta,y) bind(c,name="psb_hip_csr_spmv") result(res)
+import::c_ptr
+type(c_ptr),value::dmat
+real(8),value::alpha
+type(c_ptr),value::x
+real(8),value::beta
+type(c_ptr),value::y
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_krylov_fused(method,fmt,dmat,diag,b,x,eps,itmax,istop,check_every,iter,err,hist) bind(c,name="psb_hip_krylov_fused") result(res)
+import::c_ptr
+integer(4),value::method
+integer(4),value::fmt
+type(c_ptr),value::dmat
+type(c_ptr),value::diag
+type(c_ptr),value::b
+type(c_ptr),value::x
+real(8),value::eps
+integer(4),value::itmax
+integer(4),value::istop
+integer(4),value::check_every
+integer(4)::iter
+real(8)::err
+type(c_ptr),value::hist
+integer(4)::res
+end
+end interface
+end

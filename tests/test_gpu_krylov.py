@@ -6,7 +6,7 @@ Tolerances: CG iteration counts within +-2 of the oracle.  BiCGSTAB is
 chaotic in rounding on these problems (tests/test_oracle_krylov.py shows a
 spread of >4 iterations from ulp-level perturbations of b, with the compiled
 reference's own count inside it), so its count must fall inside the oracle's
-rounding spread widened by 2.  Every solve must reach the same stopping
+sampled rounding spread, widened by a quarter of its width plus 2.  Every solve must reach the same stopping
 tolerance and agree with the oracle's solution.
 """
 import json
@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 import psblas3_amd as P
-from helpers import bicgstab_spread
+from helpers import bicgstab_spread, in_spread
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
@@ -50,8 +50,8 @@ def test_reference_pde3d_run(gpu, orc, fmt, fused):
     solver = P.psb_krylov_fused if fused else P.psb_krylov
     it, err, hist = solver("BICGSTAB", a, prec, b, x, ref["eps"], desc, itmax=1000, istop=2)
     csr = orc.csr_from_coo(orc.fix_coo(coo))
-    lo, hi = bicgstab_spread(orc, csr, rhs, ref["eps"])
-    assert lo - 2 <= it <= hi + 2, (it, lo, hi)
+    spread = bicgstab_spread(orc, csr, rhs, ref["eps"])
+    assert in_spread(it, spread), (it, spread)
     assert err <= ref["eps"]
     xo, ito, erro, _ = orc.krylov("BICGSTAB", csr, rhs, eps=ref["eps"], itmax=1000)
     xs = x.get_vect()
@@ -91,12 +91,12 @@ def test_convdiff_L40_bicgstab(gpu, orc):
     f = orc.fix_coo(coo)
     csr = orc.csr_from_coo(f)
     bh = orc.csr_spmv(*csr, np.ones(f.nc))
-    lo, hi = bicgstab_spread(orc, csr, bh, 1e-9)
+    spread = bicgstab_spread(orc, csr, bh, 1e-9)
     a, desc, prec = _setup(coo, P.psb_d_hip_hlg_sparse_mat(32))
     for solver in (P.psb_krylov, P.psb_krylov_fused):
         x = P.vect_from(np.zeros(f.nr))
         it, err, hist = solver("BICGSTAB", a, prec, P.vect_from(bh), x, 1e-9, desc, itmax=1000)
-        assert lo - 2 <= it <= hi + 2, (solver.__name__, it, lo, hi)
+        assert in_spread(it, spread), (solver.__name__, it, spread)
         assert err <= 1e-9
         assert np.allclose(x.get_vect(), 1.0, atol=1e-6)
     a.free()
