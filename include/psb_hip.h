@@ -335,14 +335,20 @@ int psb_hip_krylov_fused(int method, int fmt, void *dmat, void *diag, void *b, v
                          double *err, double *hist);
 
 /* FP32-inner / FP64-outer iterative refinement on a double HLL matrix with a
-   single copy (psb_hip_hll_make_single): r = b - A x in FP64, inner solve of
-   A d = r/||r|| in FP32 (method 0 CG, 1 BiCGSTAB; Jacobi in FP32) to
-   inner_eps (loosened on the last cycle to what the outer target needs),
-   x += ||r|| d in FP64, until ||r||/||b|| <= eps.  *outer_its refinement
-   cycles, *inner_its total inner iterations, *err final ||r||/||b||. */
+   single copy (psb_hip_hll_make_single).  FP64: r = b - A x, x += d and the
+   convergence norms; FP32: the matrix values, the inner Krylov vectors and
+   the Jacobi diagonal.  variant 0 (restarted refinement): each cycle solves
+   A d = r/||r|| with FP32 CG (method 0) or BiCGSTAB (1) to inner_eps
+   (loosened on the last cycle to what the target needs), then x += ||r|| d.
+   variant 1 (reliable updates, CG only): one FP32 CG recurrence; whenever
+   its residual has dropped by inner_eps since the last replacement, the
+   FP32 iterate is added to x in FP64 and r is replaced by the FP64 b - A x,
+   keeping the search direction.  Stops when ||b - A x||/||b|| <= eps
+   (computed in FP64).  *outer_its: cycles / replacements, *inner_its: FP32
+   iterations, *err: final FP64 relative residual. */
 int psb_hip_krylov_mixed(int method, void *dmat, void *diag, void *b, void *x, double eps,
                          int outer_itmax, double inner_eps, int inner_itmax, int check_every,
-                         int *outer_its, int *inner_its, double *err);
+                         int variant, int *outer_its, int *inner_its, double *err);
 
 #ifdef __cplusplus
 }

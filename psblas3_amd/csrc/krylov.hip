@@ -42,6 +42,8 @@ struct KState {
   double rho, rho_old, alpha, beta, omega, sigma;
   double errnum, errden, eps;
   double bnorm;
+  double rref;    // reliable updates: residual norm at the last replacement
+  double delta;   // reliable updates: replacement factor
   int it;         // iterations since the last restart
   int itx;        // total iterations
   int itmax;
@@ -505,6 +507,94 @@ __global__ __launch_bounds__(kBlock) void k_zero(int n, T* __restrict__ a) {
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) a[i] = T(0);
 }
 
+// --------------------------------------------------------------- reliable updates
+// FP32 CG step with an FP64 residual-replacement trigger: as k_cg_x, but when
+// the recursive residual has dropped by `delta` since the last replacement
+// (or reached the absolute target, or the budget is spent) the iteration
+// halts with breakdown = 2 so the host can replace r by the FP64 residual
+// without restarting the recurrence (p and beta are kept).
+__global__ __launch_bounds__(kBlock) void k_ru_x(int n, float* __restrict__ x, float* __restrict__ r,
+                                                 const float* __restrict__ p,
+                                                 const float* __restrict__ q,
+                                                 const float* __restrict__ d,
+                                                 float* __restrict__ z, KState* st,
+                                                 double* partials) {
+  if (halted(st)) return;
+  const Chunk c = block_rows(n);
+  const float alpha = (float)st->alpha;
+  double rr = 0.0, rz = 0.0;
+  for (int i = c.first + threadIdx.x; i < c.last; i += kBlock) {
+    x[i] = x[i] + alpha * p[i];
+    float ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    float zi = d ? d[i] * ri : ri;
+    if (z != r) z[i] = zi;
+    rr += (double)ri * (double)ri;
+    rz += (double)ri * (double)zi;
+  }
+  double v[2] = {rr, rz};
+  if (reduce_last<2>(v, partials, &st->counter[3]) && threadIdx.x == 0) {
+    st->itx += 1;
+    st->it += 1;
+    const double rn = sqrt(v[0]);
+    st->errnum = rn;
+    // rref: norm at the last replacement; eps*errden: the absolute target
+    if (rn <= st->delta * st->rref || rn <= st->eps * st->errden || st->itx >= st->itmax) {
+      st->rho_old = st->rho;
+      st->breakdown = 2;
+      return;
+    }
+    st->rho_old = st->rho;
+    st->rho = v[1];
+    if (st->rho_old == 0.0) {
+      st->itx += 1;
+      st->breakdown = 1;
+    } else {
+      st->beta = st->rho / st->rho_old;
+    }
+  }
+}
+
+// x64 += x32; x32 = 0
+__global__ __launch_bounds__(kBlock) void k_ru_accum(int n, double* __restrict__ x,
+                                                     float* __restrict__ x32) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    x[i] = x[i] + (double)x32[i];
+    x32[i] = 0.0f;
+  }
+}
+
+// r32 = (float)r64; z = d.*r32; rho = r32.z; beta = rho/rho_old (first: it = 0)
+__global__ __launch_bounds__(kBlock) void k_ru_reset(int n, const double* __restrict__ r64,
+                                                     const double* __restrict__ rnorm,
+                                                     float* __restrict__ r,
+                                                     const float* __restrict__ d,
+                                                     float* __restrict__ z, KState* st,
+                                                     double* partials, int first) {
+  const Chunk c = block_rows(n);
+  double rz = 0.0;
+  for (int i = c.first + threadIdx.x; i < c.last; i += kBlock) {
+    float ri = (float)r64[i];
+    r[i] = ri;
+    float zi = d ? d[i] * ri : ri;
+    if (z != r) z[i] = zi;
+    rz += (double)ri * (double)zi;
+  }
+  double v[1] = {rz};
+  if (reduce_last<1>(v, partials, &st->counter[7]) && threadIdx.x == 0) {
+    st->rref = *rnorm;
+    st->breakdown = 0;
+    st->rho = v[0];
+    if (first) {
+      st->it = 0;
+    } else if (st->rho_old == 0.0) {
+      st->breakdown = 1;
+    } else {
+      st->beta = st->rho / st->rho_old;
+    }
+  }
+}
+
 // --------------------------------------------------------------- host driver
 enum { FMT_HLL = 0, FMT_ELL = 1, FMT_CSR = 2 };
 
@@ -776,6 +866,108 @@ static int mv_check(const MultiVec* v, int etype, int n) {
   return PSB_HIP_SUCCESS;
 }
 
+
+// FP32 CG with FP64 reliable updates (the Krylov recurrence is not restarted
+// at a residual replacement).  x (double) holds the initial guess.
+static int mixed_reliable(HllDev* h, const double* b, double* x, const double* d64, double eps,
+                          int outer_itmax, double delta, int inner_itmax, int check_every,
+                          int* outer_its, int* inner_its, double* err) {
+  const int n = h->rows;
+  Solver outer;
+  int rc = setup(outer, 0, FMT_HLL, h, 1);
+  if (rc) return rc;
+  Solver inner;
+  rc = setup(inner, 0, FMT_HLL, h, inner_itmax);
+  if (rc) return rc;
+  inner.use_single = true;
+  inner.etype = PSB_HIP_TYPE_FLOAT;
+  double* r64 = nullptr;
+  float *x32 = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *d = nullptr;
+  size_t fb = sizeof(float) * (size_t)n;
+  if ((rc = outer.alloc((void**)&r64, sizeof(double) * (size_t)n))) return rc;
+  if ((rc = outer.alloc((void**)&x32, fb))) return rc;
+  if ((rc = outer.alloc((void**)&r, fb))) return rc;
+  if ((rc = outer.alloc((void**)&p, fb))) return rc;
+  if ((rc = outer.alloc((void**)&q, fb))) return rc;
+  if (d64) {
+    if ((rc = outer.alloc((void**)&z, fb))) return rc;
+    if ((rc = outer.alloc((void**)&d, fb))) return rc;
+  } else {
+    z = r;
+  }
+  hipStream_t stream = outer.stream;
+  dim3 g(fused_grid(n)), blk(kBlock), gs(stream_grid(n));
+  if (d64) hipLaunchKernelGGL(k_d2s_vec, gs, blk, 0, stream, n, d64, d);
+  hipLaunchKernelGGL((k_zero<float>), gs, blk, 0, stream, n, x32);
+  // outer state: ||b||, and the FP64 residuals
+  *outer.hst = KState{};
+  outer.hst->itmax = 1 << 30;
+  outer.hst->istop = 2;
+  outer.hst->errden = 1.0;
+  PSB_HIP_CHECK(hipMemcpyAsync(outer.st, outer.hst, sizeof(KState), hipMemcpyHostToDevice, stream));
+  hipLaunchKernelGGL((k_bnorm<double>), g, blk, 0, stream, n, b, outer.st, outer.partials);
+  Mats<double> m64 = make_views<double>(outer);
+  Mats<float> m32 = make_views<float>(inner);
+  auto fp64_residual = [&]() -> int {
+    hipLaunchKernelGGL((k_restart<double, HllView<double>>), g, blk, 0, stream, m64.hll, b,
+                       (const double*)x, r64, (double*)nullptr, (const double*)nullptr, r64,
+                       outer.st, outer.partials, 0);
+    PSB_HIP_LAUNCH_CHECK();
+    return read_state(outer);
+  };
+  if ((rc = fp64_residual())) return rc;
+  const double bnorm = outer.hst->bnorm;
+  // inner state: eps*errden is the absolute target, omega the replacement factor
+  *inner.hst = KState{};
+  inner.hst->eps = eps;
+  inner.hst->errden = bnorm;
+  inner.hst->delta = delta;
+  inner.hst->itmax = inner_itmax;
+  inner.hst->istop = 2;
+  PSB_HIP_CHECK(hipMemcpyAsync(inner.st, inner.hst, sizeof(KState), hipMemcpyHostToDevice, stream));
+  hipLaunchKernelGGL(k_ru_reset, g, blk, 0, stream, n, (const double*)r64,
+                     (const double*)&outer.st->errnum, r, (const float*)d, z, inner.st,
+                     inner.partials, 1);
+  PSB_HIP_LAUNCH_CHECK();
+  int updates = 0;
+  double rnorm = outer.hst->errnum;
+  bool conv = bnorm == 0.0 ? rnorm <= eps : rnorm <= eps * bnorm;
+  if (check_every < 1) check_every = 8;
+  while (!conv) {
+    // queue FP32 iterations until a replacement (or breakdown) is requested
+    bool stop = false;
+    while (!stop) {
+      for (int k = 0; k < check_every; k++) {
+        hipLaunchKernelGGL((k_cg_p<float>), g, blk, 0, stream, n, (const float*)z, p, inner.st);
+        hipLaunchKernelGGL((k_cg_q<float, HllView<float>>), g, blk, 0, stream, m32.hll,
+                           (const float*)p, q, inner.st, inner.partials);
+        hipLaunchKernelGGL(k_ru_x, g, blk, 0, stream, n, x32, r, (const float*)p,
+                           (const float*)q, (const float*)d, z, inner.st, inner.partials);
+      }
+      PSB_HIP_LAUNCH_CHECK();
+      if ((rc = read_state(inner))) return rc;
+      stop = inner.hst->breakdown != 0;
+    }
+    // replacement: x += x32 in FP64, r = b - A x in FP64
+    hipLaunchKernelGGL(k_ru_accum, gs, blk, 0, stream, n, x, x32);
+    PSB_HIP_LAUNCH_CHECK();
+    if ((rc = fp64_residual())) return rc;
+    updates++;
+    rnorm = outer.hst->errnum;
+    conv = bnorm == 0.0 ? rnorm <= eps : rnorm <= eps * bnorm;
+    if (conv || updates >= outer_itmax || inner.hst->itx >= inner_itmax) break;
+    // a true breakdown (rho_old == 0) restarts the recurrence from the new residual
+    int first = inner.hst->breakdown == 1 ? 1 : 0;
+    hipLaunchKernelGGL(k_ru_reset, g, blk, 0, stream, n, (const double*)r64,
+                       (const double*)&outer.st->errnum, r, (const float*)d, z, inner.st,
+                       inner.partials, first);
+    PSB_HIP_LAUNCH_CHECK();
+  }
+  *outer_its = updates;
+  *inner_its = inner.hst->itx;
+  *err = bnorm > 0.0 ? rnorm / bnorm : rnorm;
+  return PSB_HIP_SUCCESS;
+}
 }  // namespace psbhip
 
 using namespace psbhip;
@@ -818,7 +1010,7 @@ int psb_hip_krylov_fused(int method, int fmt, void* dmat, void* diag, void* bv, 
 
 int psb_hip_krylov_mixed(int method, void* dmat, void* diag, void* bv, void* xv, double eps,
                          int outer_itmax, double inner_eps, int inner_itmax, int check_every,
-                         int* outer_its, int* inner_its, double* err) {
+                         int variant, int* outer_its, int* inner_its, double* err) {
   int rc = ensure_init();
   if (rc) return rc;
   HllDev* h = (HllDev*)dmat;
@@ -829,6 +1021,18 @@ int psb_hip_krylov_mixed(int method, void* dmat, void* diag, void* bv, void* xv,
   if (mv_check(b, PSB_HIP_TYPE_DOUBLE, n) || mv_check(x, PSB_HIP_TYPE_DOUBLE, n) ||
       (d && mv_check(d, PSB_HIP_TYPE_DOUBLE, n)))
     return set_error(PSB_HIP_INVALID, __func__, "b/x/diag type or size");
+  if (variant == 1) {
+    if (method != 0)
+      return set_error(PSB_HIP_UNSUPPORTED, __func__, "reliable updates are implemented for CG");
+    if (n == 0) {
+      *outer_its = *inner_its = 0;
+      *err = 0.0;
+      return PSB_HIP_SUCCESS;
+    }
+    return mixed_reliable(h, (const double*)b->v, (double*)x->v,
+                          d ? (const double*)d->v : nullptr, eps, outer_itmax, inner_eps,
+                          inner_itmax, check_every, outer_its, inner_its, err);
+  }
   hipStream_t stream = ctx().stream;
   // outer workspace (double): r, and the scalar state of the outer residual
   Solver outer;

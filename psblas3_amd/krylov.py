@@ -262,9 +262,12 @@ def psb_krylov_fused(method, a, prec, b, x, eps, desc_a=None, itmax=1000, istop=
 
 
 def psb_krylov_mixed(method, a, prec, b, x, eps, desc_a=None, outer_itmax=50, inner_eps=1e-4,
-                     inner_itmax=1000, check_every=8):
+                     inner_itmax=5000, check_every=8, variant="restart"):
     """FP32-inner / FP64-outer iterative refinement (HLG matrices).  Builds the
-    FP32 value copy on first use.  Returns (outer_its, inner_its, err)."""
+    FP32 value copy on first use.  variant "restart": each refinement cycle
+    restarts the inner FP32 solve on r/||r||; "reliable" (CG): one FP32
+    recurrence with FP64 residual replacement whenever the residual drops by
+    inner_eps.  Returns (outer_its, inner_its, err)."""
     import ctypes as C
     if a.a.get_fmt() != "HLG" or a.a._dtype != np.float64:
         raise N.PsbError(N.UNSUPPORTED, "psb_krylov_mixed", "needs a double HLG matrix")
@@ -276,8 +279,9 @@ def psb_krylov_mixed(method, a, prec, b, x, eps, desc_a=None, outer_itmax=50, in
     diag = prec.dv.dev if (prec is not None and prec.ptype == "DIAG") else None
     oi, ii, err = C.c_int(0), C.c_int(0), C.c_double(0.0)
     meth = 0 if method.upper() == "CG" else 1
+    var = {"restart": 0, "reliable": 1}[variant]
     N.call("psb_hip_krylov_mixed", meth, a.a.dev, diag, b.v.dev, x.v.dev, float(eps),
-           int(outer_itmax), float(inner_eps), int(inner_itmax), int(check_every),
+           int(outer_itmax), float(inner_eps), int(inner_itmax), int(check_every), var,
            C.byref(oi), C.byref(ii), C.byref(err))
     x.v.set_dev()
     return oi.value, ii.value, err.value

@@ -186,12 +186,13 @@ def test_fused_single_precision(gpu, orc):
     a.free()
 
 
-@pytest.mark.parametrize("method", ["CG", "BICGSTAB"])
-def test_mixed_precision_refinement(gpu, orc, method):
+@pytest.mark.parametrize("method,variant", [("CG", "restart"), ("BICGSTAB", "restart"),
+                                            ("CG", "reliable")])
+def test_mixed_precision_refinement(gpu, orc, method, variant):
     """FP32 inner (1e-4) / FP64 outer to 1e-10 on the SPD Poisson:
     the same final relative residual as FP64, and the error against the
     oracle's FP64 solution of b = A x_true stated."""
-    print("case mixed", method, flush=True)
+    print("case mixed", method, variant, flush=True)
     L = 48
     coo = orc.gen_poisson3d(L)
     f = orc.fix_coo(coo)
@@ -200,7 +201,8 @@ def test_mixed_precision_refinement(gpu, orc, method):
     bh = orc.csr_spmv(*csr, xt)
     a, desc, prec = _setup(coo, P.psb_d_hip_hlg_sparse_mat())
     x = P.vect_from(np.zeros(f.nr))
-    oi, ii, err = P.psb_krylov_mixed(method, a, prec, P.vect_from(bh), x, 1e-10, desc)
+    oi, ii, err = P.psb_krylov_mixed(method, a, prec, P.vect_from(bh), x, 1e-10, desc,
+                                     variant=variant)
     assert err <= 1e-10 and oi >= 2
     xo, ito, erro, _ = orc.krylov("CG", csr, bh, eps=1e-10, itmax=2000)
     xs = x.get_vect()
@@ -209,7 +211,10 @@ def test_mixed_precision_refinement(gpu, orc, method):
     assert np.linalg.norm(xs - xo) <= 1e-7 * np.linalg.norm(xo)
     res = orc.csr_spmv(*csr, xs, 1.0, 0.0) - bh
     assert np.linalg.norm(res) <= 1.01e-10 * np.linalg.norm(bh)
-    print(f"mixed {method}: outer {oi}, inner {ii}, err {err:.2e}, "
+    if variant == "reliable":
+        # the recurrence is not restarted: close to the FP64 CG iteration count
+        assert ii <= 1.25 * ito, (ii, ito)
+    print(f"mixed {method} {variant}: outer {oi}, inner {ii}, err {err:.2e}, "
           f"|x-x_true|/|x_true| = {np.linalg.norm(xs - xt) / np.linalg.norm(xt):.2e}", flush=True)
     a.free()
 
