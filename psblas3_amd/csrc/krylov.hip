@@ -68,6 +68,15 @@ struct HllView {
     const int w = (hkoffs[hk + 1] - off0) / hksz;
     return slot_dot<T, T>(ja + off0 + lane, val + off0 + lane, hksz, w, x);
   }
+  __device__ __forceinline__ void row2(int r0, int r1, const T* __restrict__ x, T& a0,
+                                       T& a1) const {
+    const int h0 = r0 / hksz, h1 = r1 / hksz;
+    const int o0 = hkoffs[h0], o1 = hkoffs[h1];
+    const int w0 = (hkoffs[h0 + 1] - o0) / hksz, w1 = (hkoffs[h1 + 1] - o1) / hksz;
+    const int l0 = r0 - h0 * hksz, l1 = r1 - h1 * hksz;
+    slot_dot2<T, T>(ja + o0 + l0, val + o0 + l0, w0, ja + o1 + l1, val + o1 + l1, w1, hksz, x,
+                    a0, a1);
+  }
 };
 
 template <typename T>
@@ -78,7 +87,25 @@ struct EllView {
   __device__ __forceinline__ T row(int r, const T* __restrict__ x) const {
     return slot_dot<T, T>(ja + r, val + r, pitch, nzm, x);
   }
+  __device__ __forceinline__ void row2(int r0, int r1, const T* __restrict__ x, T& a0,
+                                       T& a1) const {
+    slot_dot2<T, T>(ja + r0, val + r0, nzm, ja + r1, val + r1, nzm, pitch, x, a0, a1);
+  }
 };
+
+// Tile walk of the fused products: two 256-row tiles per step (rows row0 and
+// row0 + 256 of this thread), clamped to the chunk so that every lane
+// computes (the values of clamped rows are discarded).
+#define PSB_FUSED_ROWS(M, X, BODY)                                              \
+  for (int base = c.first; base < c.last; base += 2 * kBlock) {                 \
+    const int row0 = base + threadIdx.x, row1 = row0 + kBlock;                  \
+    const int rc0 = row0 < c.last ? row0 : c.last - 1;                          \
+    const int rc1 = row1 < c.last ? row1 : c.last - 1;                          \
+    T acc0, acc1;                                                               \
+    (M).row2(rc0, rc1, (X), acc0, acc1);                                        \
+    { const int row = row0; const T ax = acc0; if (row < c.last) { BODY } }     \
+    { const int row = row1; const T ax = acc1; if (row < c.last) { BODY } }     \
+  }
 
 // Rows [first, last) owned by this block: the grid is bounded, each block
 // walks a contiguous chunk of 256-row tiles (XCD-aware block order).
@@ -162,23 +189,18 @@ __global__ __launch_bounds__(kBlock) void k_restart(M m, const T* __restrict__ b
   if (st->done) return;
   const Chunk c = block_rows(m.nr);
   double rr = 0.0, rz = 0.0;
-  for (int base = c.first; base < c.last; base += kBlock) {
-    const int row = base + threadIdx.x;
-    const int rc = row < c.last ? row : c.last - 1;
-    T ax = m.row(rc, x);
-    if (row < c.last) {
-      T ri = b[row] - ax;
-      r[row] = ri;
-      rr += (double)ri * (double)ri;
-      if (bicg) {
-        q[row] = ri;
-      } else {
-        T zi = d ? d[row] * ri : ri;
-        if (z != r) z[row] = zi;
-        rz += (double)ri * (double)zi;
-      }
+  PSB_FUSED_ROWS(m, x, {
+    T ri = b[row] - ax;
+    r[row] = ri;
+    rr += (double)ri * (double)ri;
+    if (bicg) {
+      q[row] = ri;
+    } else {
+      T zi = d ? d[row] * ri : ri;
+      if (z != r) z[row] = zi;
+      rz += (double)ri * (double)zi;
     }
-  }
+  })
   double v[2] = {rr, rz};
   if (reduce_last<2>(v, partials, &st->counter[0]) && threadIdx.x == 0) {
     st->breakdown = 0;
@@ -226,7 +248,22 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(int n, const T* __restrict__ z,
   const Chunk c = block_rows(n);
   const bool first = st->it == 0;
   const T beta = (T)st->beta;
-  for (int i = c.first + threadIdx.x; i < c.last; i += kBlock)
+  constexpr int V = 16 / sizeof(T);
+  const int nv = (c.last - c.first) / V;
+  const T* zb = z + c.first;
+  T* pb = p + c.first;
+  for (int k = threadIdx.x; k < nv; k += 2 * kBlock) {
+    const int k1 = k + kBlock;
+    v16<T> z0 = ldv(zb + V * k), p0 = ldv(pb + V * k);
+    v16<T> z1 = z0, p1 = p0;
+    if (k1 < nv) {
+      z1 = ldv(zb + V * k1);
+      p1 = ldv(pb + V * k1);
+    }
+    stv(pb + V * k, first ? z0 : z0 + beta * p0);
+    if (k1 < nv) stv(pb + V * k1, first ? z1 : z1 + beta * p1);
+  }
+  for (int i = c.first + nv * V + threadIdx.x; i < c.last; i += kBlock)
     p[i] = first ? z[i] : z[i] + beta * p[i];
 }
 
@@ -236,15 +273,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_q(M m, const T* __restrict__ p, T
   if (halted(st)) return;
   const Chunk c = block_rows(m.nr);
   double pq = 0.0;
-  for (int base = c.first; base < c.last; base += kBlock) {
-    const int row = base + threadIdx.x;
-    const int rc = row < c.last ? row : c.last - 1;
-    T qi = m.row(rc, p);
-    if (row < c.last) {
-      q[row] = qi;
-      pq += (double)p[row] * (double)qi;
-    }
-  }
+  PSB_FUSED_ROWS(m, p, {
+    q[row] = ax;
+    pq += (double)p[row] * (double)ax;
+  })
   double v[1] = {pq};
   if (reduce_last<1>(v, partials, &st->counter[2]) && threadIdx.x == 0) {
     st->sigma = v[0];
@@ -287,7 +319,39 @@ __global__ __launch_bounds__(kBlock) void k_cg_x(int n, T* __restrict__ x, T* __
   const Chunk c = block_rows(n);
   const T alpha = (T)st->alpha;
   double rr = 0.0, rz = 0.0;
-  for (int i = c.first + threadIdx.x; i < c.last; i += kBlock) {
+  constexpr int V = 16 / sizeof(T);
+  const int nv = (c.last - c.first) / V;
+  const int o = c.first;
+  for (int k = threadIdx.x; k < nv; k += 2 * kBlock) {
+    const int k1 = k + kBlock;
+    const bool two = k1 < nv;
+    const int i0 = o + V * k, i1 = o + V * (two ? k1 : k);
+    v16<T> x0 = ldv(x + i0), p0 = ldv(p + i0), r0 = ldv(r + i0), q0 = ldv(q + i0);
+    v16<T> x1 = ldv(x + i1), p1 = ldv(p + i1), r1 = ldv(r + i1), q1 = ldv(q + i1);
+    v16<T> d0 = r0, d1 = r1;
+    if (d) {
+      d0 = ldv(d + i0);
+      d1 = ldv(d + i1);
+    }
+    x0 = x0 + alpha * p0;
+    r0 = r0 - alpha * q0;
+    x1 = x1 + alpha * p1;
+    r1 = r1 - alpha * q1;
+    v16<T> z0 = d ? d0 * r0 : r0, z1 = d ? d1 * r1 : r1;
+    stv(x + i0, x0);
+    stv(r + i0, r0);
+    if (z != r) stv(z + i0, z0);
+    rr += vdot<T>(r0, r0);
+    rz += vdot<T>(r0, z0);
+    if (two) {
+      stv(x + i1, x1);
+      stv(r + i1, r1);
+      if (z != r) stv(z + i1, z1);
+      rr += vdot<T>(r1, r1);
+      rz += vdot<T>(r1, z1);
+    }
+  }
+  for (int i = o + nv * V + threadIdx.x; i < c.last; i += kBlock) {
     x[i] = x[i] + alpha * p[i];
     T ri = r[i] - alpha * q[i];
     r[i] = ri;
@@ -323,7 +387,33 @@ __global__ __launch_bounds__(kBlock) void k_bi_a(int n, const T* __restrict__ r,
   const Chunk c = block_rows(n);
   const bool first = st->it == 0;
   const T beta = (T)st->beta, momega = (T)(-st->omega);
-  for (int i = c.first + threadIdx.x; i < c.last; i += kBlock) {
+  constexpr int V = 16 / sizeof(T);
+  const int nv = (c.last - c.first) / V;
+  const int o = c.first;
+  for (int k = threadIdx.x; k < nv; k += 2 * kBlock) {
+    const int k1 = k + kBlock;
+    const bool two = k1 < nv;
+    const int i0 = o + V * k, i1 = o + V * (two ? k1 : k);
+    v16<T> r0 = ldv(r + i0), r1 = ldv(r + i1);
+    v16<T> p0 = r0, p1 = r1;
+    if (!first) {
+      v16<T> q0 = ldv(p + i0), q1 = ldv(p + i1), v0 = ldv(v + i0), v1 = ldv(v + i1);
+      p0 = r0 + beta * (momega * v0 + q0);
+      p1 = r1 + beta * (momega * v1 + q1);
+    }
+    v16<T> f0 = p0, f1 = p1;
+    if (d) {
+      f0 = ldv(d + i0) * p0;
+      f1 = ldv(d + i1) * p1;
+    }
+    stv(p + i0, p0);
+    if (f != p) stv(f + i0, f0);
+    if (two) {
+      stv(p + i1, p1);
+      if (f != p) stv(f + i1, f1);
+    }
+  }
+  for (int i = o + nv * V + threadIdx.x; i < c.last; i += kBlock) {
     T pi = first ? r[i] : r[i] + beta * (momega * v[i] + p[i]);
     p[i] = pi;
     if (f != p) f[i] = d ? d[i] * pi : pi;
@@ -337,15 +427,10 @@ __global__ __launch_bounds__(kBlock) void k_bi_v(M m, const T* __restrict__ f, T
   if (halted(st)) return;
   const Chunk c = block_rows(m.nr);
   double qv = 0.0;
-  for (int base = c.first; base < c.last; base += kBlock) {
-    const int row = base + threadIdx.x;
-    const int rc = row < c.last ? row : c.last - 1;
-    T vi = m.row(rc, f);
-    if (row < c.last) {
-      v[row] = vi;
-      qv += (double)q[row] * (double)vi;
-    }
-  }
+  PSB_FUSED_ROWS(m, f, {
+    v[row] = ax;
+    qv += (double)q[row] * (double)ax;
+  })
   double s[1] = {qv};
   if (reduce_last<1>(s, partials, &st->counter[4]) && threadIdx.x == 0) {
     st->sigma = s[0];
@@ -386,7 +471,28 @@ __global__ __launch_bounds__(kBlock) void k_bi_s(int n, const T* __restrict__ r,
   if (halted(st)) return;
   const Chunk c = block_rows(n);
   const T malpha = (T)(-st->alpha);
-  for (int i = c.first + threadIdx.x; i < c.last; i += kBlock) {
+  constexpr int V = 16 / sizeof(T);
+  const int nv = (c.last - c.first) / V;
+  const int o = c.first;
+  for (int k = threadIdx.x; k < nv; k += 2 * kBlock) {
+    const int k1 = k + kBlock;
+    const bool two = k1 < nv;
+    const int i0 = o + V * k, i1 = o + V * (two ? k1 : k);
+    v16<T> s0 = malpha * ldv(v + i0) + ldv(r + i0);
+    v16<T> s1 = malpha * ldv(v + i1) + ldv(r + i1);
+    v16<T> z0 = s0, z1 = s1;
+    if (d) {
+      z0 = ldv(d + i0) * s0;
+      z1 = ldv(d + i1) * s1;
+    }
+    stv(s + i0, s0);
+    if (z != s) stv(z + i0, z0);
+    if (two) {
+      stv(s + i1, s1);
+      if (z != s) stv(z + i1, z1);
+    }
+  }
+  for (int i = o + nv * V + threadIdx.x; i < c.last; i += kBlock) {
     T si = malpha * v[i] + r[i];
     s[i] = si;
     if (z != s) z[i] = d ? d[i] * si : si;
@@ -413,16 +519,11 @@ __global__ __launch_bounds__(kBlock) void k_bi_t(M m, const T* __restrict__ z, T
   if (halted(st)) return;
   const Chunk c = block_rows(m.nr);
   double tt = 0.0, ts = 0.0;
-  for (int base = c.first; base < c.last; base += kBlock) {
-    const int row = base + threadIdx.x;
-    const int rc = row < c.last ? row : c.last - 1;
-    T ti = m.row(rc, z);
-    if (row < c.last) {
-      t[row] = ti;
-      tt += (double)ti * (double)ti;
-      ts += (double)ti * (double)s[row];
-    }
-  }
+  PSB_FUSED_ROWS(m, z, {
+    t[row] = ax;
+    tt += (double)ax * (double)ax;
+    ts += (double)ax * (double)s[row];
+  })
   double v[2] = {tt, ts};
   if (reduce_last<2>(v, partials, &st->counter[5]) && threadIdx.x == 0) omega_final(st, v[0], v[1]);
 }
@@ -452,7 +553,31 @@ __global__ __launch_bounds__(kBlock) void k_bi_x(int n, T* __restrict__ x, T* __
   const Chunk c = block_rows(n);
   const T alpha = (T)st->alpha, omega = (T)st->omega;
   double rr = 0.0, qr = 0.0;
-  for (int i = c.first + threadIdx.x; i < c.last; i += kBlock) {
+  constexpr int V = 16 / sizeof(T);
+  const int nv = (c.last - c.first) / V;
+  const int o = c.first;
+  for (int k = threadIdx.x; k < nv; k += 2 * kBlock) {
+    const int k1 = k + kBlock;
+    const bool two = k1 < nv;
+    const int i0 = o + V * k, i1 = o + V * (two ? k1 : k);
+    v16<T> x0 = ldv(x + i0), x1 = ldv(x + i1), f0 = ldv(f + i0), f1 = ldv(f + i1);
+    v16<T> z0 = ldv(z + i0), z1 = ldv(z + i1), s0 = ldv(s + i0), s1 = ldv(s + i1);
+    v16<T> t0 = ldv(t + i0), t1 = ldv(t + i1), q0 = ldv(q + i0), q1 = ldv(q + i1);
+    x0 = omega * z0 + (alpha * f0 + x0);
+    x1 = omega * z1 + (alpha * f1 + x1);
+    v16<T> r0 = -omega * t0 + s0, r1 = -omega * t1 + s1;
+    stv(x + i0, x0);
+    stv(r + i0, r0);
+    rr += vdot<T>(r0, r0);
+    qr += vdot<T>(q0, r0);
+    if (two) {
+      stv(x + i1, x1);
+      stv(r + i1, r1);
+      rr += vdot<T>(r1, r1);
+      qr += vdot<T>(q1, r1);
+    }
+  }
+  for (int i = o + nv * V + threadIdx.x; i < c.last; i += kBlock) {
     T xi = alpha * f[i] + x[i];
     x[i] = omega * z[i] + xi;
     T ri = -omega * t[i] + s[i];
@@ -523,7 +648,39 @@ __global__ __launch_bounds__(kBlock) void k_ru_x(int n, float* __restrict__ x, f
   const Chunk c = block_rows(n);
   const float alpha = (float)st->alpha;
   double rr = 0.0, rz = 0.0;
-  for (int i = c.first + threadIdx.x; i < c.last; i += kBlock) {
+  constexpr int V = 4;
+  const int nv = (c.last - c.first) / V;
+  const int o = c.first;
+  for (int k = threadIdx.x; k < nv; k += 2 * kBlock) {
+    const int k1 = k + kBlock;
+    const bool two = k1 < nv;
+    const int i0 = o + V * k, i1 = o + V * (two ? k1 : k);
+    v16<float> x0 = ldv(x + i0), p0 = ldv(p + i0), r0 = ldv(r + i0), q0 = ldv(q + i0);
+    v16<float> x1 = ldv(x + i1), p1 = ldv(p + i1), r1 = ldv(r + i1), q1 = ldv(q + i1);
+    v16<float> d0 = r0, d1 = r1;
+    if (d) {
+      d0 = ldv(d + i0);
+      d1 = ldv(d + i1);
+    }
+    x0 = x0 + alpha * p0;
+    r0 = r0 - alpha * q0;
+    x1 = x1 + alpha * p1;
+    r1 = r1 - alpha * q1;
+    v16<float> z0 = d ? d0 * r0 : r0, z1 = d ? d1 * r1 : r1;
+    stv(x + i0, x0);
+    stv(r + i0, r0);
+    if (z != r) stv(z + i0, z0);
+    rr += vdot<float>(r0, r0);
+    rz += vdot<float>(r0, z0);
+    if (two) {
+      stv(x + i1, x1);
+      stv(r + i1, r1);
+      if (z != r) stv(z + i1, z1);
+      rr += vdot<float>(r1, r1);
+      rz += vdot<float>(r1, z1);
+    }
+  }
+  for (int i = o + nv * V + threadIdx.x; i < c.last; i += kBlock) {
     x[i] = x[i] + alpha * p[i];
     float ri = r[i] - alpha * q[i];
     r[i] = ri;

@@ -340,6 +340,56 @@ __device__ __forceinline__ T slot_dot(const int* __restrict__ pj, const TV* __re
   return acc;
 }
 
+// Two independent rows at once: the index/value loads of both are issued
+// before their gathers, doubling the loads in flight per lane.
+template <typename T, typename TV>
+__device__ __forceinline__ void slot_dot2(const int* __restrict__ pj0, const TV* __restrict__ pv0,
+                                          int w0, const int* __restrict__ pj1,
+                                          const TV* __restrict__ pv1, int w1, int stride,
+                                          const T* __restrict__ x, T& a0, T& a1) {
+  a0 = T(0);
+  a1 = T(0);
+  const int w = w0 > w1 ? w0 : w1;
+  for (int j = 0; j < w; j += 4) {
+    int c0[4], c1[4];
+    TV v0[4], v1[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const bool ok0 = j + k < w0, ok1 = j + k < w1;
+      c0[k] = ok0 ? ld_stream(pj0 + (size_t)(j + k) * stride) : 1;
+      v0[k] = ok0 ? ld_stream(pv0 + (size_t)(j + k) * stride) : TV(0);
+      c1[k] = ok1 ? ld_stream(pj1 + (size_t)(j + k) * stride) : 1;
+      v1[k] = ok1 ? ld_stream(pv1 + (size_t)(j + k) * stride) : TV(0);
+    }
+    T x0[4], x1[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      x0[k] = (j + k < w0) ? x[c0[k] - 1] : T(0);
+      x1[k] = (j + k < w1) ? x[c1[k] - 1] : T(0);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      a0 = fma((T)v0[k], x0[k], a0);
+      a1 = fma((T)v1[k], x1[k], a1);
+    }
+  }
+}
+
+// 16-byte vectors of T for the streaming vector kernels
+template <typename T>
+using v16 = T __attribute__((ext_vector_type(16 / sizeof(T))));
+template <typename T>
+__device__ __forceinline__ v16<T> ldv(const T* p) { return *reinterpret_cast<const v16<T>*>(p); }
+template <typename T>
+__device__ __forceinline__ void stv(T* p, v16<T> v) { *reinterpret_cast<v16<T>*>(p) = v; }
+template <typename T>
+__device__ __forceinline__ double vdot(v16<T> a, v16<T> b) {
+  double s = 0.0;
+#pragma unroll
+  for (int e = 0; e < (int)(16 / sizeof(T)); e++) s += (double)a[e] * (double)b[e];
+  return s;
+}
+
 template <typename T>
 __device__ __forceinline__ void store_y(T* __restrict__ y, int row, T alpha, T acc, T beta) {
   T r = alpha * acc;
