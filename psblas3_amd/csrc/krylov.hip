@@ -5,27 +5,28 @@
 // krylov/psb_dcgstab.f90:248-364, convergence krylov/psb_d_krylov_conv_mod.f90:
 // 279-356), regrouped into a few fused kernels per iteration.  Every scalar
 // (rho, sigma, alpha, beta, omega, the residual norm, the stop flag) lives
-// in device memory and is produced by the last block of the reduction that
-// defines it, so iterations are queued without host round trips; the host
-// reads the stop flag once every `check_every` iterations.  Kernels queued
-// after convergence see the flag and return at once, so the iterate and the
-// iteration count are exactly those of the stopping iteration.
+// in device memory, so iterations are queued without host round trips; the
+// host reads the stop flag once every `check_every` iterations.  Kernels
+// queued after convergence see the flag and return at once, so the iterate
+// and the iteration count are exactly those of the stopping iteration.
 //
-// CG per iteration (3 kernels):
-//   K_p   p = z (first) | p = z + beta*p
-//   K_q   q = A*p with the epilogue sum p.q -> sigma, alpha = rho/sigma
-//   K_x   x += alpha*p; r -= alpha*q; z = d.*r; sums r.r, r.z ->
-//         ||r||, stop test, rho_old = rho, rho = r.z, beta = rho/rho_old
-// BiCGSTAB per iteration (5 kernels):
-//   K_a   p = r (first) | p = r + beta*(p - omega*v); f = d.*p
-//   K_v   v = A*f, sum q.v -> alpha = rho/(q.v)
-//   K_s   s = r - alpha*v; z = d.*s
-//   K_t   t = A*z, sums t.t, t.s -> omega
-//   K_x   x = (x + alpha*f) + omega*z; r = s - omega*t; sums r.r, q.r ->
-//         ||r||, stop test, rho, beta
-// Reduction partials are per block of a bounded grid (<= kMaxGrid blocks, a
-// fixed split of the rows), summed by the last block in index order:
-// results are deterministic run to run.
+// Reductions: each block of a fused kernel stores its partial sums; a
+// one-block finalisation kernel then adds the partials in index order and
+// updates the scalars (deterministic run to run).  The kernel boundary makes
+// the partials of every XCD visible; a device-scope fence per block would
+// write back the XCD's whole L2 after the fused kernel's stores, which
+// measured 1.4x slower on MI355X.
+//
+// CG per iteration:        K_p   p = z | z + beta*p
+//                          K_q   q = A*p, partial p.q        F_q  sigma, alpha
+//                          K_x   x += alpha p, r -= alpha q, z = d.*r, partial r.r, r.z
+//                                                            F_x  ||r||, stop test, rho, beta
+// BiCGSTAB per iteration:  K_a   p = r | r + beta*(p - omega v), f = d.*p
+//                          K_v   v = A*f, partial q.v        F_v  alpha
+//                          K_s   s = r - alpha v, z = d.*s
+//                          K_t   t = A*z, partial t.t, t.s   F_t  omega
+//                          K_x   x = (x + alpha f) + omega z, r = s - omega t,
+//                                partial r.r, q.r            F_x  ||r||, stop test, rho, beta
 #include <math.h>
 #include <string.h>
 
@@ -35,7 +36,7 @@
 
 namespace psbhip {
 
-constexpr int kMaxGrid = 2048;   // blocks of every fused kernel (multiple of 8)
+constexpr int kMaxGrid = 4096;   // blocks of a fused kernel (partials per reduction)
 
 struct KState {
   // scalars of the recurrence
@@ -48,10 +49,8 @@ struct KState {
   int itx;        // total iterations
   int itmax;
   int done;       // stop test satisfied (or itmax reached)
-  int breakdown;  // restart requested
+  int breakdown;  // 1: restart requested; 2: residual replacement requested
   int istop;
-  int pad[2];
-  unsigned int counter[8];   // grid-reduction arrival counters (kept at 0)
 };
 
 // --------------------------------------------------------------- views
@@ -68,15 +67,6 @@ struct HllView {
     const int w = (hkoffs[hk + 1] - off0) / hksz;
     return slot_dot<T, T>(ja + off0 + lane, val + off0 + lane, hksz, w, x);
   }
-  __device__ __forceinline__ void row2(int r0, int r1, const T* __restrict__ x, T& a0,
-                                       T& a1) const {
-    const int h0 = r0 / hksz, h1 = r1 / hksz;
-    const int o0 = hkoffs[h0], o1 = hkoffs[h1];
-    const int w0 = (hkoffs[h0 + 1] - o0) / hksz, w1 = (hkoffs[h1 + 1] - o1) / hksz;
-    const int l0 = r0 - h0 * hksz, l1 = r1 - h1 * hksz;
-    slot_dot2<T, T>(ja + o0 + l0, val + o0 + l0, w0, ja + o1 + l1, val + o1 + l1, w1, hksz, x,
-                    a0, a1);
-  }
 };
 
 template <typename T>
@@ -87,85 +77,65 @@ struct EllView {
   __device__ __forceinline__ T row(int r, const T* __restrict__ x) const {
     return slot_dot<T, T>(ja + r, val + r, pitch, nzm, x);
   }
-  __device__ __forceinline__ void row2(int r0, int r1, const T* __restrict__ x, T& a0,
-                                       T& a1) const {
-    slot_dot2<T, T>(ja + r0, val + r0, nzm, ja + r1, val + r1, nzm, pitch, x, a0, a1);
-  }
 };
 
-// Tile walk of the fused products: two 256-row tiles per step (rows row0 and
-// row0 + 256 of this thread), clamped to the chunk so that every lane
-// computes (the values of clamped rows are discarded).
-#define PSB_FUSED_ROWS(M, X, BODY)                                              \
-  for (int base = c.first; base < c.last; base += 2 * kBlock) {                 \
-    const int row0 = base + threadIdx.x, row1 = row0 + kBlock;                  \
-    const int rc0 = row0 < c.last ? row0 : c.last - 1;                          \
-    const int rc1 = row1 < c.last ? row1 : c.last - 1;                          \
-    T acc0, acc1;                                                               \
-    (M).row2(rc0, rc1, (X), acc0, acc1);                                        \
-    { const int row = row0; const T ax = acc0; if (row < c.last) { BODY } }     \
-    { const int row = row1; const T ax = acc1; if (row < c.last) { BODY } }     \
-  }
-
-// Rows [first, last) owned by this block: the grid is bounded, each block
-// walks a contiguous chunk of 256-row tiles (XCD-aware block order).
-struct Chunk {
-  int first, last;
-};
-__device__ __forceinline__ Chunk block_rows(int n) {
-  const int ntiles = (n + kBlock - 1) / kBlock;
-  const int nb = gridDim.x;
-  const int lb = xcd_block(nb);
-  const int per = (ntiles + nb - 1) / nb;
-  int t0 = lb * per, t1 = t0 + per;
-  if (t1 > ntiles) t1 = ntiles;
-  if (t0 > ntiles) t0 = ntiles;
-  Chunk c;
-  c.first = t0 * kBlock;
-  c.last = t1 * kBlock < n ? t1 * kBlock : n;
-  return c;
-}
-
-inline int fused_grid(int n) {
+// Grids.  Products walk 256-row tiles grid-stride (all blocks sweep a
+// compact window, which keeps the x gathers of neighbouring rows in cache);
+// vector kernels walk 16-byte vectors grid-stride.
+inline int spmv_grid(int n) {
   int tiles = (n + kBlock - 1) / kBlock;
-  int g = tiles < kMaxGrid ? tiles : kMaxGrid;
-  return xcd_grid(g < 1 ? 1 : g);
+  return tiles < 1 ? 1 : (tiles < kMaxGrid ? tiles : kMaxGrid);
+}
+template <typename T>
+inline int vec_grid(int n) {
+  const int V = 16 / (int)sizeof(T);
+  int g = (n / V + kBlock - 1) / kBlock;
+  if (g > 2048) g = 2048;
+  return g < 1 ? 1 : g;
 }
 
-// Block reduction + last-block finalisation of NP sums.  partials holds
-// NP * kMaxGrid doubles.  Returns true in the last block (all threads), with
-// v[] holding the grid totals.
+#define PSB_TILE_ROWS(M, X, BODY)                                               \
+  for (int t = blockIdx.x; t * kBlock < (M).nr; t += gridDim.x) {               \
+    const int row = t * kBlock + threadIdx.x;                                   \
+    const int rc = row < (M).nr ? row : (M).nr - 1;                             \
+    const T ax = (M).row(rc, (X));                                              \
+    if (row < (M).nr) { BODY }                                                  \
+  }
+
+// vector loop over n entries: BODY_V sees i (first of 16/sizeof(T) entries),
+// BODY_S handles the scalar tail (block 0)
+#define PSB_VEC_LOOP(N, BODY_V, BODY_S)                                         \
+  {                                                                             \
+    constexpr int V = 16 / sizeof(T);                                           \
+    const int nv = (N) / V;                                                     \
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < nv; k += gridDim.x * kBlock) { \
+      const int i = V * k;                                                      \
+      BODY_V                                                                    \
+    }                                                                           \
+    if (blockIdx.x == 0)                                                        \
+      for (int i = nv * V + threadIdx.x; i < (N); i += kBlock) { BODY_S }       \
+  }
+
+// Each block stores its NP partial sums (partials[p*kMaxGrid + block]).
 template <int NP>
-__device__ __forceinline__ bool reduce_last(double (&v)[NP], double* partials, unsigned int* counter) {
+__device__ __forceinline__ void store_partials(double (&v)[NP], double* partials) {
   __shared__ double lds[kBlock / kWave];
-  __shared__ bool am_last;
 #pragma unroll
   for (int p = 0; p < NP; p++) {
     double s = block_sum(v[p], lds);
     if (threadIdx.x == 0) partials[p * kMaxGrid + blockIdx.x] = s;
   }
-  if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned int prev = atomicAdd(counter, 1u);
-    am_last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!am_last) return false;
-  __threadfence();
-#pragma unroll
-  for (int p = 0; p < NP; p++) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock)
-      s += __builtin_nontemporal_load(&partials[p * kMaxGrid + b]);
-    v[p] = block_sum(s, lds);
-  }
-  if (threadIdx.x == 0) *counter = 0u;
-  return true;
 }
 
 __device__ __forceinline__ bool halted(const KState* st) {
   return st->done != 0 || st->breakdown != 0;
 }
+
+// --------------------------------------------------------------- finalisation
+enum FinOp {
+  FIN_RESTART_CG, FIN_RESTART_BI, FIN_BNORM, FIN_CG_Q, FIN_CG_X, FIN_BI_V, FIN_BI_T, FIN_BI_X,
+  FIN_RU_X, FIN_RU_RESET, FIN_RU_RESET_FIRST
+};
 
 // stop test of psb_d_check_conv_vect for istop 2 and 3 (errden = ||b|| or ||r0||)
 __device__ __forceinline__ void stop_test(KState* st, double rr, double* hist) {
@@ -177,32 +147,29 @@ __device__ __forceinline__ void stop_test(KState* st, double rr, double* hist) {
   if (res || st->itmax <= st->itx) st->done = 1;
 }
 
-// --------------------------------------------------------------- restart kernels
-// r = b - A*x, optional copy q = r (BiCGSTAB shadow), z = d.*r; sums
-// r.r and r.z (CG) or q.r (BiCGSTAB, = r.r).
-template <typename T, class M>
-__global__ __launch_bounds__(kBlock) void k_restart(M m, const T* __restrict__ b,
-                                                    const T* __restrict__ x, T* __restrict__ r,
-                                                    T* __restrict__ q, const T* __restrict__ d,
-                                                    T* __restrict__ z, KState* st,
-                                                    double* partials, int bicg) {
-  if (st->done) return;
-  const Chunk c = block_rows(m.nr);
-  double rr = 0.0, rz = 0.0;
-  PSB_FUSED_ROWS(m, x, {
-    T ri = b[row] - ax;
-    r[row] = ri;
-    rr += (double)ri * (double)ri;
-    if (bicg) {
-      q[row] = ri;
-    } else {
-      T zi = d ? d[row] * ri : ri;
-      if (z != r) z[row] = zi;
-      rz += (double)ri * (double)zi;
-    }
-  })
-  double v[2] = {rr, rz};
-  if (reduce_last<2>(v, partials, &st->counter[0]) && threadIdx.x == 0) {
+template <int OP, int NP>
+__global__ __launch_bounds__(kBlock) void k_fin(KState* st, const double* __restrict__ partials,
+                                                int nb, double* hist,
+                                                const double* __restrict__ aux) {
+  if (OP == FIN_RESTART_CG || OP == FIN_RESTART_BI) {
+    if (st->done) return;
+  } else if (OP != FIN_BNORM && OP != FIN_RU_RESET && OP != FIN_RU_RESET_FIRST) {
+    if (halted(st)) return;
+  }
+  __shared__ double lds[kBlock / kWave];
+  double v[NP];
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) s += partials[p * kMaxGrid + b];
+    v[p] = block_sum(s, lds);
+  }
+  if (threadIdx.x != 0) return;
+  if (OP == FIN_BNORM) {
+    st->bnorm = sqrt(v[0]);
+    if (st->istop == 2) st->errden = st->bnorm;
+  } else if (OP == FIN_RESTART_CG || OP == FIN_RESTART_BI) {
+    const bool bicg = OP == FIN_RESTART_BI;
     st->breakdown = 0;
     st->it = 0;
     st->errnum = sqrt(v[0]);
@@ -218,150 +185,30 @@ __global__ __launch_bounds__(kBlock) void k_restart(M m, const T* __restrict__ b
       st->rho = v[1];
     }
     st->rho_old = 0.0;
-    // the first iteration's breakdown test (rho == 0 in BiCGSTAB)
-    if (!st->done && bicg && st->rho == 0.0) {
+    if (!st->done && bicg && st->rho == 0.0) {  // first-iteration breakdown test
       st->itx += 1;
       st->breakdown = 1;
     }
-  }
-}
-
-// ||b|| for istop = 2
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bnorm(int n, const T* __restrict__ b, KState* st,
-                                                  double* partials) {
-  const Chunk c = block_rows(n);
-  double s = 0.0;
-  for (int i = c.first + threadIdx.x; i < c.last; i += kBlock) s += (double)b[i] * (double)b[i];
-  double v[1] = {s};
-  if (reduce_last<1>(v, partials, &st->counter[1]) && threadIdx.x == 0) {
-    st->bnorm = sqrt(v[0]);
-    if (st->istop == 2) st->errden = st->bnorm;
-  }
-}
-
-// --------------------------------------------------------------- CG kernels
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_p(int n, const T* __restrict__ z, T* __restrict__ p,
-                                                 KState* st) {
-  if (halted(st)) return;
-  const Chunk c = block_rows(n);
-  const bool first = st->it == 0;
-  const T beta = (T)st->beta;
-  constexpr int V = 16 / sizeof(T);
-  const int nv = (c.last - c.first) / V;
-  const T* zb = z + c.first;
-  T* pb = p + c.first;
-  for (int k = threadIdx.x; k < nv; k += 2 * kBlock) {
-    const int k1 = k + kBlock;
-    v16<T> z0 = ldv(zb + V * k), p0 = ldv(pb + V * k);
-    v16<T> z1 = z0, p1 = p0;
-    if (k1 < nv) {
-      z1 = ldv(zb + V * k1);
-      p1 = ldv(pb + V * k1);
-    }
-    stv(pb + V * k, first ? z0 : z0 + beta * p0);
-    if (k1 < nv) stv(pb + V * k1, first ? z1 : z1 + beta * p1);
-  }
-  for (int i = c.first + nv * V + threadIdx.x; i < c.last; i += kBlock)
-    p[i] = first ? z[i] : z[i] + beta * p[i];
-}
-
-template <typename T, class M>
-__global__ __launch_bounds__(kBlock) void k_cg_q(M m, const T* __restrict__ p, T* __restrict__ q,
-                                                 KState* st, double* partials) {
-  if (halted(st)) return;
-  const Chunk c = block_rows(m.nr);
-  double pq = 0.0;
-  PSB_FUSED_ROWS(m, p, {
-    q[row] = ax;
-    pq += (double)p[row] * (double)ax;
-  })
-  double v[1] = {pq};
-  if (reduce_last<1>(v, partials, &st->counter[2]) && threadIdx.x == 0) {
+  } else if (OP == FIN_CG_Q || OP == FIN_BI_V) {
     st->sigma = v[0];
     if (v[0] == 0.0) {
-      st->itx += 1;   // the reference counted this iteration before breaking
+      st->itx += 1;  // the reference counted this iteration before breaking
       st->breakdown = 1;
     } else {
       st->alpha = st->rho / v[0];
     }
-  }
-}
-
-// q = A*p computed by a separate product (CSR): sigma from a dot kernel
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_sigma(int n, const T* __restrict__ p,
-                                                     const T* __restrict__ q, KState* st,
-                                                     double* partials) {
-  if (halted(st)) return;
-  const Chunk c = block_rows(n);
-  double pq = 0.0;
-  for (int i = c.first + threadIdx.x; i < c.last; i += kBlock) pq += (double)p[i] * (double)q[i];
-  double v[1] = {pq};
-  if (reduce_last<1>(v, partials, &st->counter[2]) && threadIdx.x == 0) {
-    st->sigma = v[0];
+  } else if (OP == FIN_BI_T) {
     if (v[0] == 0.0) {
       st->itx += 1;
       st->breakdown = 1;
-    } else {
-      st->alpha = st->rho / v[0];
+      return;
     }
-  }
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_x(int n, T* __restrict__ x, T* __restrict__ r,
-                                                 const T* __restrict__ p, const T* __restrict__ q,
-                                                 const T* __restrict__ d, T* __restrict__ z,
-                                                 KState* st, double* partials, double* hist) {
-  if (halted(st)) return;
-  const Chunk c = block_rows(n);
-  const T alpha = (T)st->alpha;
-  double rr = 0.0, rz = 0.0;
-  constexpr int V = 16 / sizeof(T);
-  const int nv = (c.last - c.first) / V;
-  const int o = c.first;
-  for (int k = threadIdx.x; k < nv; k += 2 * kBlock) {
-    const int k1 = k + kBlock;
-    const bool two = k1 < nv;
-    const int i0 = o + V * k, i1 = o + V * (two ? k1 : k);
-    v16<T> x0 = ldv(x + i0), p0 = ldv(p + i0), r0 = ldv(r + i0), q0 = ldv(q + i0);
-    v16<T> x1 = ldv(x + i1), p1 = ldv(p + i1), r1 = ldv(r + i1), q1 = ldv(q + i1);
-    v16<T> d0 = r0, d1 = r1;
-    if (d) {
-      d0 = ldv(d + i0);
-      d1 = ldv(d + i1);
+    st->omega = v[1] / v[0];
+    if (st->omega == 0.0) {
+      st->itx += 1;
+      st->breakdown = 1;
     }
-    x0 = x0 + alpha * p0;
-    r0 = r0 - alpha * q0;
-    x1 = x1 + alpha * p1;
-    r1 = r1 - alpha * q1;
-    v16<T> z0 = d ? d0 * r0 : r0, z1 = d ? d1 * r1 : r1;
-    stv(x + i0, x0);
-    stv(r + i0, r0);
-    if (z != r) stv(z + i0, z0);
-    rr += vdot<T>(r0, r0);
-    rz += vdot<T>(r0, z0);
-    if (two) {
-      stv(x + i1, x1);
-      stv(r + i1, r1);
-      if (z != r) stv(z + i1, z1);
-      rr += vdot<T>(r1, r1);
-      rz += vdot<T>(r1, z1);
-    }
-  }
-  for (int i = o + nv * V + threadIdx.x; i < c.last; i += kBlock) {
-    x[i] = x[i] + alpha * p[i];
-    T ri = r[i] - alpha * q[i];
-    r[i] = ri;
-    T zi = d ? d[i] * ri : ri;
-    if (z != r) z[i] = zi;
-    rr += (double)ri * (double)ri;
-    rz += (double)ri * (double)zi;
-  }
-  double v[2] = {rr, rz};
-  if (reduce_last<2>(v, partials, &st->counter[3]) && threadIdx.x == 0) {
+  } else if (OP == FIN_CG_X) {
     st->itx += 1;
     st->it += 1;
     stop_test(st, v[0], hist);
@@ -375,218 +222,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_x(int n, T* __restrict__ x, T* __
         st->beta = st->rho / st->rho_old;
       }
     }
-  }
-}
-
-// --------------------------------------------------------------- BiCGSTAB kernels
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bi_a(int n, const T* __restrict__ r, T* __restrict__ p,
-                                                 const T* __restrict__ v, const T* __restrict__ d,
-                                                 T* __restrict__ f, KState* st) {
-  if (halted(st)) return;
-  const Chunk c = block_rows(n);
-  const bool first = st->it == 0;
-  const T beta = (T)st->beta, momega = (T)(-st->omega);
-  constexpr int V = 16 / sizeof(T);
-  const int nv = (c.last - c.first) / V;
-  const int o = c.first;
-  for (int k = threadIdx.x; k < nv; k += 2 * kBlock) {
-    const int k1 = k + kBlock;
-    const bool two = k1 < nv;
-    const int i0 = o + V * k, i1 = o + V * (two ? k1 : k);
-    v16<T> r0 = ldv(r + i0), r1 = ldv(r + i1);
-    v16<T> p0 = r0, p1 = r1;
-    if (!first) {
-      v16<T> q0 = ldv(p + i0), q1 = ldv(p + i1), v0 = ldv(v + i0), v1 = ldv(v + i1);
-      p0 = r0 + beta * (momega * v0 + q0);
-      p1 = r1 + beta * (momega * v1 + q1);
-    }
-    v16<T> f0 = p0, f1 = p1;
-    if (d) {
-      f0 = ldv(d + i0) * p0;
-      f1 = ldv(d + i1) * p1;
-    }
-    stv(p + i0, p0);
-    if (f != p) stv(f + i0, f0);
-    if (two) {
-      stv(p + i1, p1);
-      if (f != p) stv(f + i1, f1);
-    }
-  }
-  for (int i = o + nv * V + threadIdx.x; i < c.last; i += kBlock) {
-    T pi = first ? r[i] : r[i] + beta * (momega * v[i] + p[i]);
-    p[i] = pi;
-    if (f != p) f[i] = d ? d[i] * pi : pi;
-  }
-}
-
-template <typename T, class M>
-__global__ __launch_bounds__(kBlock) void k_bi_v(M m, const T* __restrict__ f, T* __restrict__ v,
-                                                 const T* __restrict__ q, KState* st,
-                                                 double* partials) {
-  if (halted(st)) return;
-  const Chunk c = block_rows(m.nr);
-  double qv = 0.0;
-  PSB_FUSED_ROWS(m, f, {
-    v[row] = ax;
-    qv += (double)q[row] * (double)ax;
-  })
-  double s[1] = {qv};
-  if (reduce_last<1>(s, partials, &st->counter[4]) && threadIdx.x == 0) {
-    st->sigma = s[0];
-    if (s[0] == 0.0) {
-      st->itx += 1;
-      st->breakdown = 1;
-    } else {
-      st->alpha = st->rho / s[0];
-    }
-  }
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bi_dot1(int n, const T* __restrict__ a,
-                                                    const T* __restrict__ b, KState* st,
-                                                    double* partials) {
-  // sigma = a.b after a separate product (CSR path): same finalisation as k_bi_v
-  if (halted(st)) return;
-  const Chunk c = block_rows(n);
-  double s0 = 0.0;
-  for (int i = c.first + threadIdx.x; i < c.last; i += kBlock) s0 += (double)a[i] * (double)b[i];
-  double s[1] = {s0};
-  if (reduce_last<1>(s, partials, &st->counter[4]) && threadIdx.x == 0) {
-    st->sigma = s[0];
-    if (s[0] == 0.0) {
-      st->itx += 1;
-      st->breakdown = 1;
-    } else {
-      st->alpha = st->rho / s[0];
-    }
-  }
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bi_s(int n, const T* __restrict__ r,
-                                                 const T* __restrict__ v, const T* __restrict__ d,
-                                                 T* __restrict__ s, T* __restrict__ z, KState* st) {
-  if (halted(st)) return;
-  const Chunk c = block_rows(n);
-  const T malpha = (T)(-st->alpha);
-  constexpr int V = 16 / sizeof(T);
-  const int nv = (c.last - c.first) / V;
-  const int o = c.first;
-  for (int k = threadIdx.x; k < nv; k += 2 * kBlock) {
-    const int k1 = k + kBlock;
-    const bool two = k1 < nv;
-    const int i0 = o + V * k, i1 = o + V * (two ? k1 : k);
-    v16<T> s0 = malpha * ldv(v + i0) + ldv(r + i0);
-    v16<T> s1 = malpha * ldv(v + i1) + ldv(r + i1);
-    v16<T> z0 = s0, z1 = s1;
-    if (d) {
-      z0 = ldv(d + i0) * s0;
-      z1 = ldv(d + i1) * s1;
-    }
-    stv(s + i0, s0);
-    if (z != s) stv(z + i0, z0);
-    if (two) {
-      stv(s + i1, s1);
-      if (z != s) stv(z + i1, z1);
-    }
-  }
-  for (int i = o + nv * V + threadIdx.x; i < c.last; i += kBlock) {
-    T si = malpha * v[i] + r[i];
-    s[i] = si;
-    if (z != s) z[i] = d ? d[i] * si : si;
-  }
-}
-
-__device__ __forceinline__ void omega_final(KState* st, double tt, double ts) {
-  if (tt == 0.0) {
-    st->itx += 1;
-    st->breakdown = 1;
-    return;
-  }
-  st->omega = ts / tt;
-  if (st->omega == 0.0) {
-    st->itx += 1;
-    st->breakdown = 1;
-  }
-}
-
-template <typename T, class M>
-__global__ __launch_bounds__(kBlock) void k_bi_t(M m, const T* __restrict__ z, T* __restrict__ t,
-                                                 const T* __restrict__ s, KState* st,
-                                                 double* partials) {
-  if (halted(st)) return;
-  const Chunk c = block_rows(m.nr);
-  double tt = 0.0, ts = 0.0;
-  PSB_FUSED_ROWS(m, z, {
-    t[row] = ax;
-    tt += (double)ax * (double)ax;
-    ts += (double)ax * (double)s[row];
-  })
-  double v[2] = {tt, ts};
-  if (reduce_last<2>(v, partials, &st->counter[5]) && threadIdx.x == 0) omega_final(st, v[0], v[1]);
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bi_dot2(int n, const T* __restrict__ t,
-                                                    const T* __restrict__ s, KState* st,
-                                                    double* partials) {
-  if (halted(st)) return;
-  const Chunk c = block_rows(n);
-  double tt = 0.0, ts = 0.0;
-  for (int i = c.first + threadIdx.x; i < c.last; i += kBlock) {
-    tt += (double)t[i] * (double)t[i];
-    ts += (double)t[i] * (double)s[i];
-  }
-  double v[2] = {tt, ts};
-  if (reduce_last<2>(v, partials, &st->counter[5]) && threadIdx.x == 0) omega_final(st, v[0], v[1]);
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bi_x(int n, T* __restrict__ x, T* __restrict__ r,
-                                                 const T* __restrict__ f, const T* __restrict__ z,
-                                                 const T* __restrict__ s, const T* __restrict__ t,
-                                                 const T* __restrict__ q, KState* st,
-                                                 double* partials, double* hist) {
-  if (halted(st)) return;
-  const Chunk c = block_rows(n);
-  const T alpha = (T)st->alpha, omega = (T)st->omega;
-  double rr = 0.0, qr = 0.0;
-  constexpr int V = 16 / sizeof(T);
-  const int nv = (c.last - c.first) / V;
-  const int o = c.first;
-  for (int k = threadIdx.x; k < nv; k += 2 * kBlock) {
-    const int k1 = k + kBlock;
-    const bool two = k1 < nv;
-    const int i0 = o + V * k, i1 = o + V * (two ? k1 : k);
-    v16<T> x0 = ldv(x + i0), x1 = ldv(x + i1), f0 = ldv(f + i0), f1 = ldv(f + i1);
-    v16<T> z0 = ldv(z + i0), z1 = ldv(z + i1), s0 = ldv(s + i0), s1 = ldv(s + i1);
-    v16<T> t0 = ldv(t + i0), t1 = ldv(t + i1), q0 = ldv(q + i0), q1 = ldv(q + i1);
-    x0 = omega * z0 + (alpha * f0 + x0);
-    x1 = omega * z1 + (alpha * f1 + x1);
-    v16<T> r0 = -omega * t0 + s0, r1 = -omega * t1 + s1;
-    stv(x + i0, x0);
-    stv(r + i0, r0);
-    rr += vdot<T>(r0, r0);
-    qr += vdot<T>(q0, r0);
-    if (two) {
-      stv(x + i1, x1);
-      stv(r + i1, r1);
-      rr += vdot<T>(r1, r1);
-      qr += vdot<T>(q1, r1);
-    }
-  }
-  for (int i = o + nv * V + threadIdx.x; i < c.last; i += kBlock) {
-    T xi = alpha * f[i] + x[i];
-    x[i] = omega * z[i] + xi;
-    T ri = -omega * t[i] + s[i];
-    r[i] = ri;
-    rr += (double)ri * (double)ri;
-    qr += (double)q[i] * (double)ri;
-  }
-  double v[2] = {rr, qr};
-  if (reduce_last<2>(v, partials, &st->counter[6]) && threadIdx.x == 0) {
+  } else if (OP == FIN_BI_X) {
     st->itx += 1;
     st->it += 1;
     stop_test(st, v[0], hist);
@@ -600,7 +236,259 @@ __global__ __launch_bounds__(kBlock) void k_bi_x(int n, T* __restrict__ x, T* __
         st->beta = (st->rho / st->rho_old) * (st->alpha / st->omega);
       }
     }
+  } else if (OP == FIN_RU_X) {
+    st->itx += 1;
+    st->it += 1;
+    const double rn = sqrt(v[0]);
+    st->errnum = rn;
+    // rref: norm at the last replacement; eps*errden: the absolute target
+    if (rn <= st->delta * st->rref || rn <= st->eps * st->errden || st->itx >= st->itmax) {
+      st->rho_old = st->rho;
+      st->breakdown = 2;
+      return;
+    }
+    st->rho_old = st->rho;
+    st->rho = v[1];
+    if (st->rho_old == 0.0) {
+      st->itx += 1;
+      st->breakdown = 1;
+    } else {
+      st->beta = st->rho / st->rho_old;
+    }
+  } else if (OP == FIN_RU_RESET || OP == FIN_RU_RESET_FIRST) {
+    st->rref = *aux;
+    st->breakdown = 0;
+    st->rho = v[0];
+    if (OP == FIN_RU_RESET_FIRST) {
+      st->it = 0;
+    } else if (st->rho_old == 0.0) {
+      st->breakdown = 1;
+    } else {
+      st->beta = st->rho / st->rho_old;
+    }
   }
+}
+
+// --------------------------------------------------------------- restart / norms
+// r = b - A*x, then (CG) z = d.*r with partials r.r, r.z or (BiCGSTAB) q = r
+// with partial r.r.
+template <typename T, class M>
+__global__ __launch_bounds__(kBlock) void k_restart(M m, const T* b,
+                                                    const T* __restrict__ x, T* r, T* q,
+                                                    const T* __restrict__ d, T* z, KState* st,
+                                                    double* partials, int bicg) {
+  if (st->done) return;
+  double v[2] = {0.0, 0.0};
+  PSB_TILE_ROWS(m, x, {
+    T ri = b[row] - ax;
+    r[row] = ri;
+    v[0] += (double)ri * (double)ri;
+    if (bicg) {
+      q[row] = ri;
+    } else {
+      T zi = d ? d[row] * ri : ri;
+      if (z != r) z[row] = zi;
+      v[1] += (double)ri * (double)zi;
+    }
+  })
+  store_partials<2>(v, partials);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sumsq(int n, const T* __restrict__ b,
+                                                  double* partials) {
+  double v[1] = {0.0};
+  PSB_VEC_LOOP(n, { v[0] += vdot<T>(ldv(b + i), ldv(b + i)); },
+               { v[0] += (double)b[i] * (double)b[i]; })
+  store_partials<1>(v, partials);
+}
+
+// --------------------------------------------------------------- CG kernels
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_p(int n, const T* __restrict__ z, T* __restrict__ p,
+                                                 const KState* st) {
+  if (halted(st)) return;
+  const bool first = st->it == 0;
+  const T beta = (T)st->beta;
+  PSB_VEC_LOOP(n,
+               {
+                 v16<T> zi = ldv(z + i);
+                 stv(p + i, first ? zi : zi + beta * ldv(p + i));
+               },
+               { p[i] = first ? z[i] : z[i] + beta * p[i]; })
+}
+
+template <typename T, class M>
+__global__ __launch_bounds__(kBlock) void k_cg_q(M m, const T* __restrict__ p, T* __restrict__ q,
+                                                 const KState* st, double* partials) {
+  if (halted(st)) return;
+  double v[1] = {0.0};
+  PSB_TILE_ROWS(m, p, {
+    q[row] = ax;
+    v[0] += (double)p[row] * (double)ax;
+  })
+  store_partials<1>(v, partials);
+}
+
+// a.b partial (the CSR path: the product runs through the CSR kernels)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dot(int n, const T* __restrict__ a,
+                                                const T* __restrict__ b, const KState* st,
+                                                double* partials) {
+  if (halted(st)) return;
+  double v[1] = {0.0};
+  PSB_VEC_LOOP(n, { v[0] += vdot<T>(ldv(a + i), ldv(b + i)); },
+               { v[0] += (double)a[i] * (double)b[i]; })
+  store_partials<1>(v, partials);
+}
+
+// t.t and t.s partials (the CSR path of BiCGSTAB)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dot2(int n, const T* __restrict__ t,
+                                                 const T* __restrict__ s, const KState* st,
+                                                 double* partials) {
+  if (halted(st)) return;
+  double v[2] = {0.0, 0.0};
+  PSB_VEC_LOOP(n,
+               {
+                 v16<T> ti = ldv(t + i);
+                 v[0] += vdot<T>(ti, ti);
+                 v[1] += vdot<T>(ti, ldv(s + i));
+               },
+               {
+                 v[0] += (double)t[i] * (double)t[i];
+                 v[1] += (double)t[i] * (double)s[i];
+               })
+  store_partials<2>(v, partials);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_x(int n, T* __restrict__ x, T* r,
+                                                 const T* __restrict__ p, const T* __restrict__ q,
+                                                 const T* __restrict__ d, T* z, const KState* st,
+                                                 double* partials) {
+  if (halted(st)) return;
+  const T alpha = (T)st->alpha;
+  double v[2] = {0.0, 0.0};
+  PSB_VEC_LOOP(n,
+               {
+                 v16<T> ri = ldv(r + i) - alpha * ldv(q + i);
+                 stv(x + i, ldv(x + i) + alpha * ldv(p + i));
+                 stv(r + i, ri);
+                 v16<T> zi = ri;
+                 if (d) {
+                   zi = ldv(d + i) * ri;
+                   stv(z + i, zi);
+                 }
+                 v[0] += vdot<T>(ri, ri);
+                 v[1] += vdot<T>(ri, zi);
+               },
+               {
+                 x[i] = x[i] + alpha * p[i];
+                 T ri = r[i] - alpha * q[i];
+                 r[i] = ri;
+                 T zi = d ? d[i] * ri : ri;
+                 if (d) z[i] = zi;
+                 v[0] += (double)ri * (double)ri;
+                 v[1] += (double)ri * (double)zi;
+               })
+  store_partials<2>(v, partials);
+}
+
+// --------------------------------------------------------------- BiCGSTAB kernels
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bi_a(int n, const T* __restrict__ r, T* p,
+                                                 const T* __restrict__ v, const T* __restrict__ d,
+                                                 T* f, const KState* st) {
+  if (halted(st)) return;
+  const bool first = st->it == 0;
+  const T beta = (T)st->beta, momega = (T)(-st->omega);
+  PSB_VEC_LOOP(n,
+               {
+                 v16<T> pi = ldv(r + i);
+                 if (!first) pi = pi + beta * (momega * ldv(v + i) + ldv(p + i));
+                 stv(p + i, pi);
+                 if (d) stv(f + i, ldv(d + i) * pi);
+               },
+               {
+                 T pi = first ? r[i] : r[i] + beta * (momega * v[i] + p[i]);
+                 p[i] = pi;
+                 if (d) f[i] = d[i] * pi;
+               })
+}
+
+template <typename T, class M>
+__global__ __launch_bounds__(kBlock) void k_bi_v(M m, const T* __restrict__ f, T* __restrict__ v,
+                                                 const T* __restrict__ q, const KState* st,
+                                                 double* partials) {
+  if (halted(st)) return;
+  double s[1] = {0.0};
+  PSB_TILE_ROWS(m, f, {
+    v[row] = ax;
+    s[0] += (double)q[row] * (double)ax;
+  })
+  store_partials<1>(s, partials);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bi_s(int n, const T* __restrict__ r,
+                                                 const T* __restrict__ v, const T* __restrict__ d,
+                                                 T* s, T* z, const KState* st) {
+  if (halted(st)) return;
+  const T malpha = (T)(-st->alpha);
+  PSB_VEC_LOOP(n,
+               {
+                 v16<T> si = malpha * ldv(v + i) + ldv(r + i);
+                 stv(s + i, si);
+                 if (d) stv(z + i, ldv(d + i) * si);
+               },
+               {
+                 T si = malpha * v[i] + r[i];
+                 s[i] = si;
+                 if (d) z[i] = d[i] * si;
+               })
+}
+
+template <typename T, class M>
+__global__ __launch_bounds__(kBlock) void k_bi_t(M m, const T* __restrict__ z, T* __restrict__ t,
+                                                 const T* __restrict__ s, const KState* st,
+                                                 double* partials) {
+  if (halted(st)) return;
+  double v[2] = {0.0, 0.0};
+  PSB_TILE_ROWS(m, z, {
+    t[row] = ax;
+    v[0] += (double)ax * (double)ax;
+    v[1] += (double)ax * (double)s[row];
+  })
+  store_partials<2>(v, partials);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bi_x(int n, T* __restrict__ x, T* __restrict__ r,
+                                                 const T* __restrict__ f, const T* __restrict__ z,
+                                                 const T* __restrict__ s, const T* __restrict__ t,
+                                                 const T* __restrict__ q, const KState* st,
+                                                 double* partials) {
+  if (halted(st)) return;
+  const T alpha = (T)st->alpha, omega = (T)st->omega;
+  double v[2] = {0.0, 0.0};
+  PSB_VEC_LOOP(n,
+               {
+                 stv(x + i, omega * ldv(z + i) + (alpha * ldv(f + i) + ldv(x + i)));
+                 v16<T> ri = -omega * ldv(t + i) + ldv(s + i);
+                 stv(r + i, ri);
+                 v[0] += vdot<T>(ri, ri);
+                 v[1] += vdot<T>(ldv(q + i), ri);
+               },
+               {
+                 T xi = alpha * f[i] + x[i];
+                 x[i] = omega * z[i] + xi;
+                 T ri = -omega * t[i] + s[i];
+                 r[i] = ri;
+                 v[0] += (double)ri * (double)ri;
+                 v[1] += (double)q[i] * (double)ri;
+               })
+  store_partials<2>(v, partials);
 }
 
 // --------------------------------------------------------------- mixed precision helpers
@@ -632,86 +520,9 @@ __global__ __launch_bounds__(kBlock) void k_zero(int n, T* __restrict__ a) {
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) a[i] = T(0);
 }
 
-// --------------------------------------------------------------- reliable updates
-// FP32 CG step with an FP64 residual-replacement trigger: as k_cg_x, but when
-// the recursive residual has dropped by `delta` since the last replacement
-// (or reached the absolute target, or the budget is spent) the iteration
-// halts with breakdown = 2 so the host can replace r by the FP64 residual
-// without restarting the recurrence (p and beta are kept).
-__global__ __launch_bounds__(kBlock) void k_ru_x(int n, float* __restrict__ x, float* __restrict__ r,
-                                                 const float* __restrict__ p,
-                                                 const float* __restrict__ q,
-                                                 const float* __restrict__ d,
-                                                 float* __restrict__ z, KState* st,
-                                                 double* partials) {
-  if (halted(st)) return;
-  const Chunk c = block_rows(n);
-  const float alpha = (float)st->alpha;
-  double rr = 0.0, rz = 0.0;
-  constexpr int V = 4;
-  const int nv = (c.last - c.first) / V;
-  const int o = c.first;
-  for (int k = threadIdx.x; k < nv; k += 2 * kBlock) {
-    const int k1 = k + kBlock;
-    const bool two = k1 < nv;
-    const int i0 = o + V * k, i1 = o + V * (two ? k1 : k);
-    v16<float> x0 = ldv(x + i0), p0 = ldv(p + i0), r0 = ldv(r + i0), q0 = ldv(q + i0);
-    v16<float> x1 = ldv(x + i1), p1 = ldv(p + i1), r1 = ldv(r + i1), q1 = ldv(q + i1);
-    v16<float> d0 = r0, d1 = r1;
-    if (d) {
-      d0 = ldv(d + i0);
-      d1 = ldv(d + i1);
-    }
-    x0 = x0 + alpha * p0;
-    r0 = r0 - alpha * q0;
-    x1 = x1 + alpha * p1;
-    r1 = r1 - alpha * q1;
-    v16<float> z0 = d ? d0 * r0 : r0, z1 = d ? d1 * r1 : r1;
-    stv(x + i0, x0);
-    stv(r + i0, r0);
-    if (z != r) stv(z + i0, z0);
-    rr += vdot<float>(r0, r0);
-    rz += vdot<float>(r0, z0);
-    if (two) {
-      stv(x + i1, x1);
-      stv(r + i1, r1);
-      if (z != r) stv(z + i1, z1);
-      rr += vdot<float>(r1, r1);
-      rz += vdot<float>(r1, z1);
-    }
-  }
-  for (int i = o + nv * V + threadIdx.x; i < c.last; i += kBlock) {
-    x[i] = x[i] + alpha * p[i];
-    float ri = r[i] - alpha * q[i];
-    r[i] = ri;
-    float zi = d ? d[i] * ri : ri;
-    if (z != r) z[i] = zi;
-    rr += (double)ri * (double)ri;
-    rz += (double)ri * (double)zi;
-  }
-  double v[2] = {rr, rz};
-  if (reduce_last<2>(v, partials, &st->counter[3]) && threadIdx.x == 0) {
-    st->itx += 1;
-    st->it += 1;
-    const double rn = sqrt(v[0]);
-    st->errnum = rn;
-    // rref: norm at the last replacement; eps*errden: the absolute target
-    if (rn <= st->delta * st->rref || rn <= st->eps * st->errden || st->itx >= st->itmax) {
-      st->rho_old = st->rho;
-      st->breakdown = 2;
-      return;
-    }
-    st->rho_old = st->rho;
-    st->rho = v[1];
-    if (st->rho_old == 0.0) {
-      st->itx += 1;
-      st->breakdown = 1;
-    } else {
-      st->beta = st->rho / st->rho_old;
-    }
-  }
-}
-
+// Reliable updates: the FP32 CG step is k_cg_x; its finalisation (FIN_RU_X)
+// requests a residual replacement once the recursive residual has dropped
+// by delta since the last one.
 // x64 += x32; x32 = 0
 __global__ __launch_bounds__(kBlock) void k_ru_accum(int n, double* __restrict__ x,
                                                      float* __restrict__ x32) {
@@ -721,35 +532,19 @@ __global__ __launch_bounds__(kBlock) void k_ru_accum(int n, double* __restrict__
   }
 }
 
-// r32 = (float)r64; z = d.*r32; rho = r32.z; beta = rho/rho_old (first: it = 0)
+// r32 = (float)r64; z = d.*r32; partial r32.z
 __global__ __launch_bounds__(kBlock) void k_ru_reset(int n, const double* __restrict__ r64,
-                                                     const double* __restrict__ rnorm,
-                                                     float* __restrict__ r,
-                                                     const float* __restrict__ d,
-                                                     float* __restrict__ z, KState* st,
-                                                     double* partials, int first) {
-  const Chunk c = block_rows(n);
-  double rz = 0.0;
-  for (int i = c.first + threadIdx.x; i < c.last; i += kBlock) {
+                                                     float* r, const float* __restrict__ d,
+                                                     float* z, double* partials) {
+  double v[1] = {0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     float ri = (float)r64[i];
     r[i] = ri;
     float zi = d ? d[i] * ri : ri;
-    if (z != r) z[i] = zi;
-    rz += (double)ri * (double)zi;
+    if (d) z[i] = zi;
+    v[0] += (double)ri * (double)zi;
   }
-  double v[1] = {rz};
-  if (reduce_last<1>(v, partials, &st->counter[7]) && threadIdx.x == 0) {
-    st->rref = *rnorm;
-    st->breakdown = 0;
-    st->rho = v[0];
-    if (first) {
-      st->it = 0;
-    } else if (st->rho_old == 0.0) {
-      st->breakdown = 1;
-    } else {
-      st->beta = st->rho / st->rho_old;
-    }
-  }
+  store_partials<1>(v, partials);
 }
 
 // --------------------------------------------------------------- host driver
@@ -805,6 +600,13 @@ static Mats<T> make_views(Solver& s) {
   return m;
 }
 
+// One-block finalisation after a fused kernel launched on nb blocks.
+template <int OP, int NP>
+static void fin(Solver& s, int nb, const double* aux = nullptr) {
+  hipLaunchKernelGGL((k_fin<OP, NP>), dim3(1), dim3(kBlock), 0, s.stream, s.st, s.partials, nb,
+                     s.hist, aux);
+}
+
 static int csr_product(Solver& s, const void* x, void* y) {
   // y = A*x through the CSR entry point (vector CSR + long rows)
   MultiVec mx{1, s.n, s.n, s.etype, const_cast<void*>(x)};
@@ -812,18 +614,33 @@ static int csr_product(Solver& s, const void* x, void* y) {
   return psb_hip_csr_spmv(s.mat, 1.0, &mx, 0.0, &my);
 }
 
+// r = b - A x and the restart sums; CSR computes the residual through the
+// product entry point first and passes it as b to an empty-matrix sweep.
 template <typename T>
-static int launch_restart(Solver& s, const Mats<T>& m, const T* b, const T* x, T* r, T* q,
-                          const T* d, T* z, int bicg) {
-  dim3 g(fused_grid(s.n)), blk(kBlock);
-  if (s.fmt == FMT_HLL)
-    hipLaunchKernelGGL((k_restart<T, HllView<T>>), g, blk, 0, s.stream, m.hll, b, x, r, q, d, z,
-                       s.st, s.partials, bicg);
-  else if (s.fmt == FMT_ELL)
-    hipLaunchKernelGGL((k_restart<T, EllView<T>>), g, blk, 0, s.stream, m.ell, b, x, r, q, d, z,
-                       s.st, s.partials, bicg);
-  else
-    return PSB_HIP_UNSUPPORTED;
+static int restart(Solver& s, const Mats<T>& m, const T* b, const T* x, T* r, T* q, const T* d,
+                   T* z, int bicg) {
+  const int n = s.n;
+  int g = spmv_grid(n);
+  dim3 blk(kBlock);
+  if (s.fmt == FMT_HLL) {
+    hipLaunchKernelGGL((k_restart<T, HllView<T>>), dim3(g), blk, 0, s.stream, m.hll, b, x, r, q,
+                       d, z, s.st, s.partials, bicg);
+  } else if (s.fmt == FMT_ELL) {
+    hipLaunchKernelGGL((k_restart<T, EllView<T>>), dim3(g), blk, 0, s.stream, m.ell, b, x, r, q,
+                       d, z, s.st, s.partials, bicg);
+  } else {
+    MultiVec mx{1, n, n, s.etype, (void*)x}, mr{1, n, n, s.etype, r}, mb{1, n, n, s.etype, (void*)b};
+    int rc = psb_hip_mv_axpby(n, 1.0, &mb, 0.0, &mr);
+    if (rc) return rc;
+    rc = psb_hip_csr_spmv(s.mat, -1.0, &mx, 1.0, &mr);
+    if (rc) return rc;
+    EllView<T> zero{n, n, 0, nullptr, nullptr};
+    hipLaunchKernelGGL((k_restart<T, EllView<T>>), dim3(g), blk, 0, s.stream, zero, (const T*)r,
+                       x, r, q, d, z, s.st, s.partials, bicg);
+  }
+  PSB_HIP_LAUNCH_CHECK();
+  if (bicg) fin<FIN_RESTART_BI, 2>(s, g);
+  else fin<FIN_RESTART_CG, 2>(s, g);
   PSB_HIP_LAUNCH_CHECK();
   return PSB_HIP_SUCCESS;
 }
@@ -831,22 +648,28 @@ static int launch_restart(Solver& s, const Mats<T>& m, const T* b, const T* x, T
 // One CG iteration (queued, no host synchronisation).
 template <typename T>
 static int cg_iter(Solver& s, const Mats<T>& m, T* x, T* r, T* p, T* q, const T* d, T* z) {
-  dim3 g(fused_grid(s.n)), blk(kBlock);
-  hipLaunchKernelGGL((k_cg_p<T>), g, blk, 0, s.stream, s.n, (const T*)z, p, s.st);
-  if (s.fmt == FMT_HLL)
-    hipLaunchKernelGGL((k_cg_q<T, HllView<T>>), g, blk, 0, s.stream, m.hll, (const T*)p, q, s.st,
-                       s.partials);
-  else if (s.fmt == FMT_ELL)
-    hipLaunchKernelGGL((k_cg_q<T, EllView<T>>), g, blk, 0, s.stream, m.ell, (const T*)p, q, s.st,
-                       s.partials);
-  else {
+  const int n = s.n;
+  dim3 blk(kBlock), gv(vec_grid<T>(n));
+  const int gs = spmv_grid(n);
+  hipLaunchKernelGGL((k_cg_p<T>), gv, blk, 0, s.stream, n, (const T*)z, p, s.st);
+  if (s.fmt == FMT_HLL) {
+    hipLaunchKernelGGL((k_cg_q<T, HllView<T>>), dim3(gs), blk, 0, s.stream, m.hll, (const T*)p,
+                       q, s.st, s.partials);
+    fin<FIN_CG_Q, 1>(s, gs);
+  } else if (s.fmt == FMT_ELL) {
+    hipLaunchKernelGGL((k_cg_q<T, EllView<T>>), dim3(gs), blk, 0, s.stream, m.ell, (const T*)p,
+                       q, s.st, s.partials);
+    fin<FIN_CG_Q, 1>(s, gs);
+  } else {
     int rc = csr_product(s, p, q);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_cg_sigma<T>), g, blk, 0, s.stream, s.n, (const T*)p, (const T*)q, s.st,
+    hipLaunchKernelGGL((k_dot<T>), gv, blk, 0, s.stream, n, (const T*)p, (const T*)q, s.st,
                        s.partials);
+    fin<FIN_CG_Q, 1>(s, gv.x);
   }
-  hipLaunchKernelGGL((k_cg_x<T>), g, blk, 0, s.stream, s.n, x, r, (const T*)p, (const T*)q, d, z,
-                     s.st, s.partials, s.hist);
+  hipLaunchKernelGGL((k_cg_x<T>), gv, blk, 0, s.stream, n, x, r, (const T*)p, (const T*)q, d, z,
+                     s.st, s.partials);
+  fin<FIN_CG_X, 2>(s, gv.x);
   PSB_HIP_LAUNCH_CHECK();
   return PSB_HIP_SUCCESS;
 }
@@ -854,37 +677,46 @@ static int cg_iter(Solver& s, const Mats<T>& m, T* x, T* r, T* p, T* q, const T*
 template <typename T>
 static int bi_iter(Solver& s, const Mats<T>& m, T* x, T* r, T* q, T* p, T* v, T* f, T* sv, T* z,
                    T* t, const T* d) {
-  dim3 g(fused_grid(s.n)), blk(kBlock);
-  hipLaunchKernelGGL((k_bi_a<T>), g, blk, 0, s.stream, s.n, (const T*)r, p, (const T*)v, d, f,
+  const int n = s.n;
+  dim3 blk(kBlock), gv(vec_grid<T>(n));
+  const int gs = spmv_grid(n);
+  hipLaunchKernelGGL((k_bi_a<T>), gv, blk, 0, s.stream, n, (const T*)r, p, (const T*)v, d, f,
                      s.st);
-  if (s.fmt == FMT_HLL)
-    hipLaunchKernelGGL((k_bi_v<T, HllView<T>>), g, blk, 0, s.stream, m.hll, (const T*)f, v,
+  if (s.fmt == FMT_HLL) {
+    hipLaunchKernelGGL((k_bi_v<T, HllView<T>>), dim3(gs), blk, 0, s.stream, m.hll, (const T*)f, v,
                        (const T*)q, s.st, s.partials);
-  else if (s.fmt == FMT_ELL)
-    hipLaunchKernelGGL((k_bi_v<T, EllView<T>>), g, blk, 0, s.stream, m.ell, (const T*)f, v,
+    fin<FIN_BI_V, 1>(s, gs);
+  } else if (s.fmt == FMT_ELL) {
+    hipLaunchKernelGGL((k_bi_v<T, EllView<T>>), dim3(gs), blk, 0, s.stream, m.ell, (const T*)f, v,
                        (const T*)q, s.st, s.partials);
-  else {
+    fin<FIN_BI_V, 1>(s, gs);
+  } else {
     int rc = csr_product(s, f, v);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_bi_dot1<T>), g, blk, 0, s.stream, s.n, (const T*)q, (const T*)v, s.st,
+    hipLaunchKernelGGL((k_dot<T>), gv, blk, 0, s.stream, n, (const T*)q, (const T*)v, s.st,
                        s.partials);
+    fin<FIN_BI_V, 1>(s, gv.x);
   }
-  hipLaunchKernelGGL((k_bi_s<T>), g, blk, 0, s.stream, s.n, (const T*)r, (const T*)v, d, sv, z,
+  hipLaunchKernelGGL((k_bi_s<T>), gv, blk, 0, s.stream, n, (const T*)r, (const T*)v, d, sv, z,
                      s.st);
-  if (s.fmt == FMT_HLL)
-    hipLaunchKernelGGL((k_bi_t<T, HllView<T>>), g, blk, 0, s.stream, m.hll, (const T*)z, t,
+  if (s.fmt == FMT_HLL) {
+    hipLaunchKernelGGL((k_bi_t<T, HllView<T>>), dim3(gs), blk, 0, s.stream, m.hll, (const T*)z, t,
                        (const T*)sv, s.st, s.partials);
-  else if (s.fmt == FMT_ELL)
-    hipLaunchKernelGGL((k_bi_t<T, EllView<T>>), g, blk, 0, s.stream, m.ell, (const T*)z, t,
+    fin<FIN_BI_T, 2>(s, gs);
+  } else if (s.fmt == FMT_ELL) {
+    hipLaunchKernelGGL((k_bi_t<T, EllView<T>>), dim3(gs), blk, 0, s.stream, m.ell, (const T*)z, t,
                        (const T*)sv, s.st, s.partials);
-  else {
+    fin<FIN_BI_T, 2>(s, gs);
+  } else {
     int rc = csr_product(s, z, t);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_bi_dot2<T>), g, blk, 0, s.stream, s.n, (const T*)t, (const T*)sv, s.st,
+    hipLaunchKernelGGL((k_dot2<T>), gv, blk, 0, s.stream, n, (const T*)t, (const T*)sv, s.st,
                        s.partials);
+    fin<FIN_BI_T, 2>(s, gv.x);
   }
-  hipLaunchKernelGGL((k_bi_x<T>), g, blk, 0, s.stream, s.n, x, r, (const T*)f, (const T*)z,
-                     (const T*)sv, (const T*)t, (const T*)q, s.st, s.partials, s.hist);
+  hipLaunchKernelGGL((k_bi_x<T>), gv, blk, 0, s.stream, n, x, r, (const T*)f, (const T*)z,
+                     (const T*)sv, (const T*)t, (const T*)q, s.st, s.partials);
+  fin<FIN_BI_X, 2>(s, gv.x);
   PSB_HIP_LAUNCH_CHECK();
   return PSB_HIP_SUCCESS;
 }
@@ -892,6 +724,16 @@ static int bi_iter(Solver& s, const Mats<T>& m, T* x, T* r, T* q, T* p, T* v, T*
 static int read_state(Solver& s) {
   PSB_HIP_CHECK(hipMemcpyAsync(s.hst, s.st, sizeof(KState), hipMemcpyDeviceToHost, s.stream));
   PSB_HIP_CHECK(hipStreamSynchronize(s.stream));
+  return PSB_HIP_SUCCESS;
+}
+
+// ||b|| into the state (errden for istop 2)
+template <typename T>
+static int bnorm(Solver& s, const T* b) {
+  dim3 gv(vec_grid<T>(s.n));
+  hipLaunchKernelGGL((k_sumsq<T>), gv, dim3(kBlock), 0, s.stream, s.n, b, s.partials);
+  fin<FIN_BNORM, 1>(s, gv.x);
+  PSB_HIP_LAUNCH_CHECK();
   return PSB_HIP_SUCCESS;
 }
 
@@ -906,7 +748,6 @@ static int solve(Solver& s, const T* b, T* x, const T* d, double eps, int itmax,
     int rc = s.alloc((void**)&w[k], sizeof(T) * (size_t)(n > 0 ? n : 1));
     if (rc) return rc;
   }
-  // vectors the preconditioner would only copy are aliased when d == NULL
   // the initial state goes through the pinned mirror (an async copy must not
   // read a stack object that may be gone when the copy runs)
   *s.hst = KState{};
@@ -916,50 +757,19 @@ static int solve(Solver& s, const T* b, T* x, const T* d, double eps, int itmax,
   s.hst->errden = 1.0;
   PSB_HIP_CHECK(hipMemcpyAsync(s.st, s.hst, sizeof(KState), hipMemcpyHostToDevice, s.stream));
   Mats<T> m = make_views<T>(s);
-  dim3 g(fused_grid(n)), blk(kBlock);
-  hipLaunchKernelGGL((k_bnorm<T>), g, blk, 0, s.stream, n, b, s.st, s.partials);
-  PSB_HIP_LAUNCH_CHECK();
+  int rc = bnorm<T>(s, b);
+  if (rc) return rc;
   if (check_every < 1) check_every = 8;
-  int rc;
   for (;;) {
-    // restart: r = b - A x (and the first z / rho)
+    // restart: r = b - A x (and the first z / rho); without a preconditioner
+    // the vectors it would only copy are aliased (z = r, f = p, z = s)
     if (s.method == 0) {
-      T *r = w[0], *z = d ? w[3] : w[0];
-      if (s.fmt == FMT_CSR) {
-        // r = b - A x via the product entry point, then the restart sums
-        MultiVec mx{1, n, n, s.etype, x}, mr{1, n, n, s.etype, r}, mb{1, n, n, s.etype, (void*)b};
-        rc = psb_hip_mv_axpby(n, 1.0, &mb, 0.0, &mr);
-        if (rc) return rc;
-        rc = psb_hip_csr_spmv(s.mat, -1.0, &mx, 1.0, &mr);
-        if (rc) return rc;
-        // identity "matrix" pass computing the sums: reuse k_restart on an
-        // empty ELL view (row() = 0) with b := r
-        EllView<T> zero{n, n, 0, nullptr, nullptr};
-        hipLaunchKernelGGL((k_restart<T, EllView<T>>), g, blk, 0, s.stream, zero, (const T*)r,
-                           (const T*)x, r, (T*)nullptr, d, z, s.st, s.partials, 0);
-        PSB_HIP_LAUNCH_CHECK();
-      } else {
-        rc = launch_restart<T>(s, m, b, x, r, nullptr, d, z, 0);
-        if (rc) return rc;
-      }
+      T* z = d ? w[3] : w[0];
+      rc = restart<T>(s, m, b, x, w[0], nullptr, d, z, 0);
     } else {
-      T *r = w[0], *q = w[1];
-      if (s.fmt == FMT_CSR) {
-        MultiVec mx{1, n, n, s.etype, x}, mr{1, n, n, s.etype, r}, mb{1, n, n, s.etype, (void*)b};
-        rc = psb_hip_mv_axpby(n, 1.0, &mb, 0.0, &mr);
-        if (rc) return rc;
-        rc = psb_hip_csr_spmv(s.mat, -1.0, &mx, 1.0, &mr);
-        if (rc) return rc;
-        EllView<T> zero{n, n, 0, nullptr, nullptr};
-        hipLaunchKernelGGL((k_restart<T, EllView<T>>), g, blk, 0, s.stream, zero, (const T*)r,
-                           (const T*)x, r, q, d, (T*)nullptr, s.st, s.partials, 1);
-        PSB_HIP_LAUNCH_CHECK();
-      } else {
-        rc = launch_restart<T>(s, m, b, x, r, q, d, nullptr, 1);
-        if (rc) return rc;
-      }
+      rc = restart<T>(s, m, b, x, w[0], w[1], d, nullptr, 1);
     }
-    // iterate
+    if (rc) return rc;
     bool stop = false;
     while (!stop) {
       for (int k = 0; k < check_every; k++) {
@@ -1010,7 +820,7 @@ static int setup(Solver& s, int method, int fmt, void* dmat, int itmax) {
   }
   int rc = s.alloc((void**)&s.st, sizeof(KState));
   if (rc) return rc;
-  rc = s.alloc((void**)&s.partials, sizeof(double) * 8 * kMaxGrid);
+  rc = s.alloc((void**)&s.partials, sizeof(double) * 2 * kMaxGrid);
   if (rc) return rc;
   rc = s.alloc((void**)&s.hist, sizeof(double) * ((size_t)itmax + 2));
   if (rc) return rc;
@@ -1023,6 +833,14 @@ static int mv_check(const MultiVec* v, int etype, int n) {
   return PSB_HIP_SUCCESS;
 }
 
+// FP64 residual of the refinement loops: r64 = b - A64 x (||r64|| into the
+// outer state's errnum), read back.
+static int fp64_residual(Solver& outer, const Mats<double>& m64, const double* b, const double* x,
+                         double* r64) {
+  int rc = restart<double>(outer, m64, b, x, r64, nullptr, nullptr, r64, 0);
+  if (rc) return rc;
+  return read_state(outer);
+}
 
 // FP32 CG with FP64 reliable updates (the Krylov recurrence is not restarted
 // at a residual replacement).  x (double) holds the initial guess.
@@ -1053,7 +871,7 @@ static int mixed_reliable(HllDev* h, const double* b, double* x, const double* d
     z = r;
   }
   hipStream_t stream = outer.stream;
-  dim3 g(fused_grid(n)), blk(kBlock), gs(stream_grid(n));
+  dim3 blk(kBlock), gs(stream_grid(n));
   if (d64) hipLaunchKernelGGL(k_d2s_vec, gs, blk, 0, stream, n, d64, d);
   hipLaunchKernelGGL((k_zero<float>), gs, blk, 0, stream, n, x32);
   // outer state: ||b||, and the FP64 residuals
@@ -1062,44 +880,45 @@ static int mixed_reliable(HllDev* h, const double* b, double* x, const double* d
   outer.hst->istop = 2;
   outer.hst->errden = 1.0;
   PSB_HIP_CHECK(hipMemcpyAsync(outer.st, outer.hst, sizeof(KState), hipMemcpyHostToDevice, stream));
-  hipLaunchKernelGGL((k_bnorm<double>), g, blk, 0, stream, n, b, outer.st, outer.partials);
+  if ((rc = bnorm<double>(outer, b))) return rc;
   Mats<double> m64 = make_views<double>(outer);
   Mats<float> m32 = make_views<float>(inner);
-  auto fp64_residual = [&]() -> int {
-    hipLaunchKernelGGL((k_restart<double, HllView<double>>), g, blk, 0, stream, m64.hll, b,
-                       (const double*)x, r64, (double*)nullptr, (const double*)nullptr, r64,
-                       outer.st, outer.partials, 0);
-    PSB_HIP_LAUNCH_CHECK();
-    return read_state(outer);
-  };
-  if ((rc = fp64_residual())) return rc;
-  const double bnorm = outer.hst->bnorm;
-  // inner state: eps*errden is the absolute target, omega the replacement factor
+  if ((rc = fp64_residual(outer, m64, b, x, r64))) return rc;
+  const double bnrm = outer.hst->bnorm;
+  // inner state: eps*errden is the absolute target, delta the replacement factor
   *inner.hst = KState{};
   inner.hst->eps = eps;
-  inner.hst->errden = bnorm;
+  inner.hst->errden = bnrm;
   inner.hst->delta = delta;
   inner.hst->itmax = inner_itmax;
   inner.hst->istop = 2;
   PSB_HIP_CHECK(hipMemcpyAsync(inner.st, inner.hst, sizeof(KState), hipMemcpyHostToDevice, stream));
-  hipLaunchKernelGGL(k_ru_reset, g, blk, 0, stream, n, (const double*)r64,
-                     (const double*)&outer.st->errnum, r, (const float*)d, z, inner.st,
-                     inner.partials, 1);
+  auto reset = [&](bool first) {
+    hipLaunchKernelGGL(k_ru_reset, gs, blk, 0, stream, n, (const double*)r64, r,
+                       (const float*)d, z, inner.partials);
+    if (first) fin<FIN_RU_RESET_FIRST, 1>(inner, gs.x, &outer.st->errnum);
+    else fin<FIN_RU_RESET, 1>(inner, gs.x, &outer.st->errnum);
+  };
+  reset(true);
   PSB_HIP_LAUNCH_CHECK();
   int updates = 0;
   double rnorm = outer.hst->errnum;
-  bool conv = bnorm == 0.0 ? rnorm <= eps : rnorm <= eps * bnorm;
+  bool conv = bnrm == 0.0 ? rnorm <= eps : rnorm <= eps * bnrm;
   if (check_every < 1) check_every = 8;
+  dim3 gv(vec_grid<float>(n));
+  const int gsp = spmv_grid(n);
   while (!conv) {
     // queue FP32 iterations until a replacement (or breakdown) is requested
     bool stop = false;
     while (!stop) {
       for (int k = 0; k < check_every; k++) {
-        hipLaunchKernelGGL((k_cg_p<float>), g, blk, 0, stream, n, (const float*)z, p, inner.st);
-        hipLaunchKernelGGL((k_cg_q<float, HllView<float>>), g, blk, 0, stream, m32.hll,
+        hipLaunchKernelGGL((k_cg_p<float>), gv, blk, 0, stream, n, (const float*)z, p, inner.st);
+        hipLaunchKernelGGL((k_cg_q<float, HllView<float>>), dim3(gsp), blk, 0, stream, m32.hll,
                            (const float*)p, q, inner.st, inner.partials);
-        hipLaunchKernelGGL(k_ru_x, g, blk, 0, stream, n, x32, r, (const float*)p,
+        fin<FIN_CG_Q, 1>(inner, gsp);
+        hipLaunchKernelGGL((k_cg_x<float>), gv, blk, 0, stream, n, x32, r, (const float*)p,
                            (const float*)q, (const float*)d, z, inner.st, inner.partials);
+        fin<FIN_RU_X, 2>(inner, gv.x);
       }
       PSB_HIP_LAUNCH_CHECK();
       if ((rc = read_state(inner))) return rc;
@@ -1108,23 +927,87 @@ static int mixed_reliable(HllDev* h, const double* b, double* x, const double* d
     // replacement: x += x32 in FP64, r = b - A x in FP64
     hipLaunchKernelGGL(k_ru_accum, gs, blk, 0, stream, n, x, x32);
     PSB_HIP_LAUNCH_CHECK();
-    if ((rc = fp64_residual())) return rc;
+    if ((rc = fp64_residual(outer, m64, b, x, r64))) return rc;
     updates++;
     rnorm = outer.hst->errnum;
-    conv = bnorm == 0.0 ? rnorm <= eps : rnorm <= eps * bnorm;
+    conv = bnrm == 0.0 ? rnorm <= eps : rnorm <= eps * bnrm;
     if (conv || updates >= outer_itmax || inner.hst->itx >= inner_itmax) break;
     // a true breakdown (rho_old == 0) restarts the recurrence from the new residual
-    int first = inner.hst->breakdown == 1 ? 1 : 0;
-    hipLaunchKernelGGL(k_ru_reset, g, blk, 0, stream, n, (const double*)r64,
-                       (const double*)&outer.st->errnum, r, (const float*)d, z, inner.st,
-                       inner.partials, first);
+    reset(inner.hst->breakdown == 1);
     PSB_HIP_LAUNCH_CHECK();
   }
   *outer_its = updates;
   *inner_its = inner.hst->itx;
-  *err = bnorm > 0.0 ? rnorm / bnorm : rnorm;
+  *err = bnrm > 0.0 ? rnorm / bnrm : rnorm;
   return PSB_HIP_SUCCESS;
 }
+
+// Restarted refinement: each cycle solves A32 d = r/||r|| from zero.
+static int mixed_restart(int method, HllDev* h, const double* b, double* x, const double* d64,
+                         double eps, int outer_itmax, double inner_eps, int inner_itmax,
+                         int check_every, int* outer_its, int* inner_its, double* err) {
+  const int n = h->rows;
+  hipStream_t stream = ctx().stream;
+  Solver outer;
+  int rc = setup(outer, 0, FMT_HLL, h, 1);
+  if (rc) return rc;
+  double* r = nullptr;
+  float *r32 = nullptr, *d32 = nullptr, *x32 = nullptr;
+  if ((rc = outer.alloc((void**)&r, sizeof(double) * n))) return rc;
+  if ((rc = outer.alloc((void**)&r32, sizeof(float) * n))) return rc;
+  if ((rc = outer.alloc((void**)&x32, sizeof(float) * n))) return rc;
+  if (d64) {
+    if ((rc = outer.alloc((void**)&d32, sizeof(float) * n))) return rc;
+    hipLaunchKernelGGL(k_d2s_vec, dim3(stream_grid(n)), dim3(kBlock), 0, stream, n, d64, d32);
+    PSB_HIP_LAUNCH_CHECK();
+  }
+  Mats<double> m64 = make_views<double>(outer);
+  *outer.hst = KState{};
+  outer.hst->itmax = 1 << 30;
+  outer.hst->istop = 2;
+  outer.hst->errden = 1.0;
+  PSB_HIP_CHECK(hipMemcpyAsync(outer.st, outer.hst, sizeof(KState), hipMemcpyHostToDevice, stream));
+  if ((rc = bnorm<double>(outer, b))) return rc;
+  int total_inner = 0, outer_k = 0;
+  double bnrm = 0.0, rnorm = 0.0;
+  for (;;) {
+    if ((rc = fp64_residual(outer, m64, b, x, r))) return rc;
+    bnrm = outer.hst->bnorm;
+    rnorm = outer.hst->errnum;
+    bool conv = bnrm == 0.0 ? rnorm <= eps : rnorm <= eps * bnrm;
+    if (conv || outer_k >= outer_itmax || rnorm == 0.0) break;
+    outer_k++;
+    // inner tolerance: inner_eps, no tighter than the outer target needs
+    double ie = inner_eps;
+    if (bnrm > 0.0) {
+      double need = 0.5 * eps * bnrm / rnorm;
+      if (need > ie) ie = need < 0.5 ? need : 0.5;
+    }
+    hipLaunchKernelGGL(k_to_single, dim3(stream_grid(n)), dim3(kBlock), 0, stream, n,
+                       (const double*)r, (const double*)&outer.st->errnum, r32);
+    hipLaunchKernelGGL((k_zero<float>), dim3(stream_grid(n)), dim3(kBlock), 0, stream, n, x32);
+    PSB_HIP_LAUNCH_CHECK();
+    Solver inner;
+    if ((rc = setup(inner, method == 0 ? 0 : 1, FMT_HLL, h, inner_itmax))) return rc;
+    inner.use_single = true;
+    inner.etype = PSB_HIP_TYPE_FLOAT;
+    int it = 0;
+    double ierr = 0.0;
+    rc = solve<float>(inner, r32, x32, d64 ? d32 : nullptr, ie, inner_itmax, 2, check_every, &it,
+                      &ierr);
+    if (rc) return rc;
+    total_inner += it;
+    // x += ||r|| * d  (FP64)
+    hipLaunchKernelGGL(k_update_double, dim3(stream_grid(n)), dim3(kBlock), 0, stream, n, x,
+                       (const float*)x32, (const double*)&outer.st->errnum);
+    PSB_HIP_LAUNCH_CHECK();
+  }
+  *outer_its = outer_k;
+  *inner_its = total_inner;
+  *err = bnrm > 0.0 ? rnorm / bnrm : rnorm;
+  return PSB_HIP_SUCCESS;
+}
+
 }  // namespace psbhip
 
 using namespace psbhip;
@@ -1178,98 +1061,20 @@ int psb_hip_krylov_mixed(int method, void* dmat, void* diag, void* bv, void* xv,
   if (mv_check(b, PSB_HIP_TYPE_DOUBLE, n) || mv_check(x, PSB_HIP_TYPE_DOUBLE, n) ||
       (d && mv_check(d, PSB_HIP_TYPE_DOUBLE, n)))
     return set_error(PSB_HIP_INVALID, __func__, "b/x/diag type or size");
+  if (n == 0) {
+    *outer_its = *inner_its = 0;
+    *err = 0.0;
+    return PSB_HIP_SUCCESS;
+  }
+  const double* d64 = d ? (const double*)d->v : nullptr;
   if (variant == 1) {
     if (method != 0)
       return set_error(PSB_HIP_UNSUPPORTED, __func__, "reliable updates are implemented for CG");
-    if (n == 0) {
-      *outer_its = *inner_its = 0;
-      *err = 0.0;
-      return PSB_HIP_SUCCESS;
-    }
-    return mixed_reliable(h, (const double*)b->v, (double*)x->v,
-                          d ? (const double*)d->v : nullptr, eps, outer_itmax, inner_eps,
+    return mixed_reliable(h, (const double*)b->v, (double*)x->v, d64, eps, outer_itmax, inner_eps,
                           inner_itmax, check_every, outer_its, inner_its, err);
   }
-  hipStream_t stream = ctx().stream;
-  // outer workspace (double): r, and the scalar state of the outer residual
-  Solver outer;
-  rc = setup(outer, 0, FMT_HLL, dmat, 1);
-  if (rc) return rc;
-  double* r = nullptr;
-  float *r32 = nullptr, *d32 = nullptr, *x32 = nullptr;
-  rc = outer.alloc((void**)&r, sizeof(double) * n);
-  if (rc) return rc;
-  rc = outer.alloc((void**)&r32, sizeof(float) * n);
-  if (rc) return rc;
-  rc = outer.alloc((void**)&x32, sizeof(float) * n);
-  if (rc) return rc;
-  if (d) {
-    rc = outer.alloc((void**)&d32, sizeof(float) * n);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_d2s_vec, dim3(stream_grid(n)), dim3(kBlock), 0, stream, n,
-                       (const double*)d->v, d32);
-    PSB_HIP_LAUNCH_CHECK();
-  }
-  Mats<double> m64 = make_views<double>(outer);
-  dim3 g(fused_grid(n)), blk(kBlock);
-  *outer.hst = KState{};
-  outer.hst->itmax = 1 << 30;
-  outer.hst->istop = 2;
-  outer.hst->errden = 1.0;
-  PSB_HIP_CHECK(hipMemcpyAsync(outer.st, outer.hst, sizeof(KState), hipMemcpyHostToDevice, stream));
-  hipLaunchKernelGGL((k_bnorm<double>), g, blk, 0, stream, n, (const double*)b->v, outer.st,
-                     outer.partials);
-  PSB_HIP_LAUNCH_CHECK();
-  int total_inner = 0, outer_k = 0;
-  double bnorm = 0.0, rnorm = 0.0;
-  MultiVec mr32{1, n, n, PSB_HIP_TYPE_FLOAT, r32}, mx32{1, n, n, PSB_HIP_TYPE_FLOAT, x32};
-  MultiVec md32{1, n, n, PSB_HIP_TYPE_FLOAT, d32};
-  for (;;) {
-    // r = b - A x in FP64 (CG restart kernel: also ||r||)
-    hipLaunchKernelGGL((k_restart<double, HllView<double>>), g, blk, 0, stream, m64.hll,
-                       (const double*)b->v, (const double*)x->v, r, (double*)nullptr,
-                       (const double*)nullptr, r, outer.st, outer.partials, 0);
-    PSB_HIP_LAUNCH_CHECK();
-    rc = read_state(outer);
-    if (rc) return rc;
-    bnorm = outer.hst->bnorm;
-    rnorm = outer.hst->errnum;
-    bool conv = bnorm == 0.0 ? rnorm <= eps : rnorm <= eps * bnorm;
-    if (conv || outer_k >= outer_itmax || rnorm == 0.0) break;
-    outer_k++;
-    // inner: A32 * d = r / ||r|| to inner_eps (no looser than needed at the end)
-    double ie = inner_eps;
-    if (bnorm > 0.0) {
-      double need = 0.5 * eps * bnorm / rnorm;
-      if (need > ie) ie = need < 0.5 ? need : 0.5;
-    }
-    hipLaunchKernelGGL(k_to_single, dim3(stream_grid(n)), dim3(kBlock), 0, stream, n,
-                       (const double*)r, (const double*)&outer.st->errnum, r32);
-    hipLaunchKernelGGL((k_zero<float>), dim3(stream_grid(n)), dim3(kBlock), 0, stream, n, x32);
-    PSB_HIP_LAUNCH_CHECK();
-    Solver inner;
-    rc = setup(inner, method == 0 ? 0 : 1, FMT_HLL, dmat, inner_itmax);
-    if (rc) return rc;
-    inner.use_single = true;
-    inner.etype = PSB_HIP_TYPE_FLOAT;
-    int it = 0;
-    double ierr = 0.0;
-    rc = solve<float>(inner, r32, x32, d ? d32 : nullptr, ie, inner_itmax, 2, check_every, &it,
-                      &ierr);
-    if (rc) return rc;
-    total_inner += it;
-    // x += ||r|| * d  (FP64)
-    hipLaunchKernelGGL(k_update_double, dim3(stream_grid(n)), dim3(kBlock), 0, stream, n,
-                       (double*)x->v, (const float*)x32, (const double*)&outer.st->errnum);
-    PSB_HIP_LAUNCH_CHECK();
-    (void)mr32;
-    (void)mx32;
-    (void)md32;
-  }
-  *outer_its = outer_k;
-  *inner_its = total_inner;
-  *err = bnorm > 0.0 ? rnorm / bnorm : rnorm;
-  return PSB_HIP_SUCCESS;
+  return mixed_restart(method, h, (const double*)b->v, (double*)x->v, d64, eps, outer_itmax,
+                       inner_eps, inner_itmax, check_every, outer_its, inner_its, err);
 }
 
 }  // extern "C"
