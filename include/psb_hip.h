@@ -262,8 +262,10 @@ int psb_hip_hll_spmv_single(void *dmat, double alpha, void *x, double beta, void
    diagonal): the device side of get_diag used by psb_d_diag_precbld
    (prec/impl/psb_d_diagprec_impl.f90:254). */
 int psb_hip_hll_get_diag(void *dmat, void *d);
-/* Kernel variant selection for the HLL SpMV (autotuner hook): 0 = auto,
-   1 = one row per lane, 2 = two rows per lane.  Returns the previous value. */
+/* Kernel variant selection for the HLL SpMV (autotuner hook): 0 = auto
+   (two rows per lane for double values, one for float), 1 = one row per lane,
+   2 = two adjacent rows per lane (even hksz); + 4 = XCD-contiguous block
+   order instead of launch order.  Returns the previous value. */
 int psb_hip_hll_set_variant(int variant);
 
 /* ===================================================================== */

@@ -36,7 +36,7 @@
 
 namespace psbhip {
 
-constexpr int kMaxGrid = 4096;   // blocks of a fused kernel (partials per reduction)
+constexpr int kMaxGrid = 32768;  // blocks of a fused kernel (partials per reduction)
 
 struct KState {
   // scalars of the recurrence
@@ -54,18 +54,25 @@ struct KState {
 };
 
 // --------------------------------------------------------------- views
+// rows<R>(row0, x, ax): ax[k] = (A x)[row0 + k] for R = 1, or for R = 2 the
+// adjacent pair (row0 even) sharing 8-byte index and 2-element value loads.
+// Rows past nr are clamped to stored rows (the results are not used).
 template <typename T>
 struct HllView {
   int nr, hksz;
   const int* hkoffs;
   const int* ja;
   const T* val;
-  __device__ __forceinline__ T row(int r, const T* __restrict__ x) const {
+  template <int R>
+  __device__ __forceinline__ void rows(int row0, const T* __restrict__ x, T (&ax)[R]) const {
+    const int last = R == 1 ? nr - 1 : ((nr - 1) & ~1);
+    const int r = row0 < last ? row0 : last;
     const int hk = r / hksz;
     const int lane = r - hk * hksz;
     const int off0 = hkoffs[hk];
     const int w = (hkoffs[hk + 1] - off0) / hksz;
-    return slot_dot<T, T>(ja + off0 + lane, val + off0 + lane, hksz, w, x);
+    if constexpr (R == 1) ax[0] = slot_dot<T, T>(ja + off0 + lane, val + off0 + lane, hksz, w, x);
+    else slot_dot_pair<T, T>(ja + off0 + lane, val + off0 + lane, hksz, w, x, ax[0], ax[1]);
   }
 };
 
@@ -74,16 +81,20 @@ struct EllView {
   int nr, pitch, nzm;
   const int* ja;
   const T* val;
-  __device__ __forceinline__ T row(int r, const T* __restrict__ x) const {
-    return slot_dot<T, T>(ja + r, val + r, pitch, nzm, x);
+  template <int R>
+  __device__ __forceinline__ void rows(int row0, const T* __restrict__ x, T (&ax)[R]) const {
+    const int last = R == 1 ? nr - 1 : ((nr - 1) & ~1);
+    const int r = row0 < last ? row0 : last;
+    if constexpr (R == 1) ax[0] = slot_dot<T, T>(ja + r, val + r, pitch, nzm, x);
+    else slot_dot_pair<T, T>(ja + r, val + r, pitch, nzm, x, ax[0], ax[1]);
   }
 };
 
-// Grids.  Products walk 256-row tiles grid-stride (all blocks sweep a
-// compact window, which keeps the x gathers of neighbouring rows in cache);
-// vector kernels walk 16-byte vectors grid-stride.
-inline int spmv_grid(int n) {
-  int tiles = (n + kBlock - 1) / kBlock;
+// Grids.  Products take one tile of kBlock*R rows per block up to kMaxGrid
+// tiles, grid-stride beyond; vector kernels walk 16-byte vectors grid-stride.
+inline int spmv_grid(int n, int rpl) {
+  const int per = kBlock * rpl;
+  int tiles = (n + per - 1) / per;
   return tiles < 1 ? 1 : (tiles < kMaxGrid ? tiles : kMaxGrid);
 }
 template <typename T>
@@ -94,12 +105,17 @@ inline int vec_grid(int n) {
   return g < 1 ? 1 : g;
 }
 
-#define PSB_TILE_ROWS(M, X, BODY)                                               \
-  for (int t = blockIdx.x; t * kBlock < (M).nr; t += gridDim.x) {               \
-    const int row = t * kBlock + threadIdx.x;                                   \
-    const int rc = row < (M).nr ? row : (M).nr - 1;                             \
-    const T ax = (M).row(rc, (X));                                              \
-    if (row < (M).nr) { BODY }                                                  \
+// BODY runs for each row < nr of the thread's tiles with `row` and `ax` = (A X)[row]
+#define PSB_TILE_ROWS(R, M, X, BODY)                                            \
+  for (int tile_ = blockIdx.x; tile_ * (kBlock * R) < (M).nr; tile_ += gridDim.x) { \
+    const int row0_ = tile_ * (kBlock * R) + R * (int)threadIdx.x;              \
+    T axr_[R];                                                                  \
+    (M).template rows<R>(row0_, (X), axr_);                                     \
+    _Pragma("unroll") for (int k_ = 0; k_ < R; k_++) {                          \
+      const int row = row0_ + k_;                                               \
+      const T ax = axr_[k_];                                                    \
+      if (row < (M).nr) { BODY }                                                \
+    }                                                                           \
   }
 
 // vector loop over n entries: BODY_V sees i (first of 16/sizeof(T) entries),
@@ -147,24 +163,47 @@ __device__ __forceinline__ void stop_test(KState* st, double rr, double* hist) {
   if (res || st->itmax <= st->itx) st->done = 1;
 }
 
+constexpr int kFin = 1024;  // threads of the finalisation kernel
+
+// Sum of partials[0..nb) over the kFin threads: eight independent loads in
+// flight per thread, fixed order (deterministic for a given nb).
+__device__ __forceinline__ double fin_sum(const double* __restrict__ partials, int nb,
+                                          double* lds /* kFin / kWave */) {
+  double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int b = threadIdx.x;
+  for (; b + 7 * kFin < nb; b += 8 * kFin) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) a[k] += partials[b + k * kFin];
+  }
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    if (b + k * kFin < nb) a[k] += partials[b + k * kFin];
+  double t = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+  t = wave_sum(t);
+  if ((threadIdx.x & (kWave - 1)) == 0) lds[threadIdx.x / kWave] = t;
+  __syncthreads();
+  double r = 0.0;
+#pragma unroll
+  for (int k = 0; k < kFin / kWave; k++) r += lds[k];
+  __syncthreads();
+  return r;
+}
+
 template <int OP, int NP>
-__global__ __launch_bounds__(kBlock) void k_fin(KState* st, const double* __restrict__ partials,
-                                                int nb, double* hist,
-                                                const double* __restrict__ aux) {
+__global__ __launch_bounds__(kFin) void k_fin(KState* st, const double* __restrict__ partials,
+                                              int nb, double* hist,
+                                              const double* __restrict__ aux) {
   if (OP == FIN_RESTART_CG || OP == FIN_RESTART_BI) {
     if (st->done) return;
   } else if (OP != FIN_BNORM && OP != FIN_RU_RESET && OP != FIN_RU_RESET_FIRST) {
     if (halted(st)) return;
   }
-  __shared__ double lds[kBlock / kWave];
+  __shared__ double lds[kFin / kWave];
   double v[NP];
 #pragma unroll
-  for (int p = 0; p < NP; p++) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nb; b += kBlock) s += partials[p * kMaxGrid + b];
-    v[p] = block_sum(s, lds);
-  }
+  for (int p = 0; p < NP; p++) v[p] = fin_sum(partials + p * kMaxGrid, nb, lds);
   if (threadIdx.x != 0) return;
+  const double v1 = v[NP - 1];  // second partial (NP == 2)
   if (OP == FIN_BNORM) {
     st->bnorm = sqrt(v[0]);
     if (st->istop == 2) st->errden = st->bnorm;
@@ -182,7 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_fin(KState* st, const double* __rest
       if (res || st->itmax <= st->itx) st->done = 1;
       st->rho = v[0];  // q = r: rho = q.r
     } else {
-      st->rho = v[1];
+      st->rho = v1;
     }
     st->rho_old = 0.0;
     if (!st->done && bicg && st->rho == 0.0) {  // first-iteration breakdown test
@@ -203,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_fin(KState* st, const double* __rest
       st->breakdown = 1;
       return;
     }
-    st->omega = v[1] / v[0];
+    st->omega = v1 / v[0];
     if (st->omega == 0.0) {
       st->itx += 1;
       st->breakdown = 1;
@@ -214,7 +253,7 @@ __global__ __launch_bounds__(kBlock) void k_fin(KState* st, const double* __rest
     stop_test(st, v[0], hist);
     if (!st->done) {
       st->rho_old = st->rho;
-      st->rho = v[1];
+      st->rho = v1;
       if (st->rho_old == 0.0) {
         st->itx += 1;  // next iteration counted, then breaks (psb_dcg.F90:253-258)
         st->breakdown = 1;
@@ -228,7 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_fin(KState* st, const double* __rest
     stop_test(st, v[0], hist);
     if (!st->done) {
       st->rho_old = st->rho;
-      st->rho = v[1];
+      st->rho = v1;
       if (st->rho == 0.0) {
         st->itx += 1;  // next iteration counted, then breaks (psb_dcgstab.f90:267-271)
         st->breakdown = 1;
@@ -248,7 +287,7 @@ __global__ __launch_bounds__(kBlock) void k_fin(KState* st, const double* __rest
       return;
     }
     st->rho_old = st->rho;
-    st->rho = v[1];
+    st->rho = v1;
     if (st->rho_old == 0.0) {
       st->itx += 1;
       st->breakdown = 1;
@@ -272,14 +311,14 @@ __global__ __launch_bounds__(kBlock) void k_fin(KState* st, const double* __rest
 // --------------------------------------------------------------- restart / norms
 // r = b - A*x, then (CG) z = d.*r with partials r.r, r.z or (BiCGSTAB) q = r
 // with partial r.r.
-template <typename T, class M>
+template <typename T, class M, int R>
 __global__ __launch_bounds__(kBlock) void k_restart(M m, const T* b,
                                                     const T* __restrict__ x, T* r, T* q,
                                                     const T* __restrict__ d, T* z, KState* st,
                                                     double* partials, int bicg) {
   if (st->done) return;
   double v[2] = {0.0, 0.0};
-  PSB_TILE_ROWS(m, x, {
+  PSB_TILE_ROWS(R, m, x, {
     T ri = b[row] - ax;
     r[row] = ri;
     v[0] += (double)ri * (double)ri;
@@ -318,12 +357,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(int n, const T* __restrict__ z,
                { p[i] = first ? z[i] : z[i] + beta * p[i]; })
 }
 
-template <typename T, class M>
+template <typename T, class M, int R>
 __global__ __launch_bounds__(kBlock) void k_cg_q(M m, const T* __restrict__ p, T* __restrict__ q,
                                                  const KState* st, double* partials) {
   if (halted(st)) return;
   double v[1] = {0.0};
-  PSB_TILE_ROWS(m, p, {
+  PSB_TILE_ROWS(R, m, p, {
     q[row] = ax;
     v[0] += (double)p[row] * (double)ax;
   })
@@ -417,13 +456,13 @@ __global__ __launch_bounds__(kBlock) void k_bi_a(int n, const T* __restrict__ r,
                })
 }
 
-template <typename T, class M>
+template <typename T, class M, int R>
 __global__ __launch_bounds__(kBlock) void k_bi_v(M m, const T* __restrict__ f, T* __restrict__ v,
                                                  const T* __restrict__ q, const KState* st,
                                                  double* partials) {
   if (halted(st)) return;
   double s[1] = {0.0};
-  PSB_TILE_ROWS(m, f, {
+  PSB_TILE_ROWS(R, m, f, {
     v[row] = ax;
     s[0] += (double)q[row] * (double)ax;
   })
@@ -449,13 +488,13 @@ __global__ __launch_bounds__(kBlock) void k_bi_s(int n, const T* __restrict__ r,
                })
 }
 
-template <typename T, class M>
+template <typename T, class M, int R>
 __global__ __launch_bounds__(kBlock) void k_bi_t(M m, const T* __restrict__ z, T* __restrict__ t,
                                                  const T* __restrict__ s, const KState* st,
                                                  double* partials) {
   if (halted(st)) return;
   double v[2] = {0.0, 0.0};
-  PSB_TILE_ROWS(m, z, {
+  PSB_TILE_ROWS(R, m, z, {
     t[row] = ax;
     v[0] += (double)ax * (double)ax;
     v[1] += (double)ax * (double)s[row];
@@ -554,8 +593,12 @@ struct Solver {
   int method = 0, fmt = 0, n = 0, etype = 0;
   void* mat = nullptr;
   bool use_single = false;       // HLL FP32 copy of a double matrix
+  int rpl = 1;                   // rows per lane of the product kernels (1 or 2)
   KState* st = nullptr;          // device
   KState* hst = nullptr;         // pinned host mirror
+  KState* ring = nullptr;        // pinned, two state copies of the pipelined poll
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int posted = 0;                // state copies queued so far
   double* partials = nullptr;
   double* hist = nullptr;
   std::vector<void*> bufs;
@@ -570,6 +613,9 @@ struct Solver {
   ~Solver() {
     for (void* p : bufs) (void)hipFree(p);
     if (hst) (void)hipHostFree(hst);
+    if (ring) (void)hipHostFree(ring);
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
   }
 };
 
@@ -589,6 +635,7 @@ static Mats<T> make_views(Solver& s) {
     m.hll.hkoffs = h->hkoffs;
     m.hll.ja = h->ja;
     m.hll.val = s.use_single ? (const T*)h->val32 : (const T*)h->val;
+    s.rpl = sizeof(T) == 8 && h->hksz % 2 == 0 ? 2 : 1;
   } else if (s.fmt == FMT_ELL) {
     EllDev* e = (EllDev*)s.mat;
     m.ell.nr = e->rows;
@@ -596,14 +643,26 @@ static Mats<T> make_views(Solver& s) {
     m.ell.nzm = e->nzm;
     m.ell.ja = e->ja;
     m.ell.val = (const T*)e->val;
+    s.rpl = sizeof(T) == 8 && e->pitch % 2 == 0 ? 2 : 1;
+  } else {
+    s.rpl = 1;
   }
   return m;
 }
 
+// Product kernel KERN<T, VT, R> with the solver's rows per lane R.
+#define PSB_PRODUCT(KERN, T, VT, s, g, ...)                                                  \
+  do {                                                                                       \
+    if ((s).rpl == 2)                                                                        \
+      hipLaunchKernelGGL((KERN<T, VT, 2>), dim3(g), dim3(kBlock), 0, (s).stream, __VA_ARGS__); \
+    else                                                                                     \
+      hipLaunchKernelGGL((KERN<T, VT, 1>), dim3(g), dim3(kBlock), 0, (s).stream, __VA_ARGS__); \
+  } while (0)
+
 // One-block finalisation after a fused kernel launched on nb blocks.
 template <int OP, int NP>
 static void fin(Solver& s, int nb, const double* aux = nullptr) {
-  hipLaunchKernelGGL((k_fin<OP, NP>), dim3(1), dim3(kBlock), 0, s.stream, s.st, s.partials, nb,
+  hipLaunchKernelGGL((k_fin<OP, NP>), dim3(1), dim3(kFin), 0, s.stream, s.st, s.partials, nb,
                      s.hist, aux);
 }
 
@@ -620,14 +679,12 @@ template <typename T>
 static int restart(Solver& s, const Mats<T>& m, const T* b, const T* x, T* r, T* q, const T* d,
                    T* z, int bicg) {
   const int n = s.n;
-  int g = spmv_grid(n);
+  int g = spmv_grid(n, s.rpl);
   dim3 blk(kBlock);
   if (s.fmt == FMT_HLL) {
-    hipLaunchKernelGGL((k_restart<T, HllView<T>>), dim3(g), blk, 0, s.stream, m.hll, b, x, r, q,
-                       d, z, s.st, s.partials, bicg);
+    PSB_PRODUCT(k_restart, T, HllView<T>, s, g, m.hll, b, x, r, q, d, z, s.st, s.partials, bicg);
   } else if (s.fmt == FMT_ELL) {
-    hipLaunchKernelGGL((k_restart<T, EllView<T>>), dim3(g), blk, 0, s.stream, m.ell, b, x, r, q,
-                       d, z, s.st, s.partials, bicg);
+    PSB_PRODUCT(k_restart, T, EllView<T>, s, g, m.ell, b, x, r, q, d, z, s.st, s.partials, bicg);
   } else {
     MultiVec mx{1, n, n, s.etype, (void*)x}, mr{1, n, n, s.etype, r}, mb{1, n, n, s.etype, (void*)b};
     int rc = psb_hip_mv_axpby(n, 1.0, &mb, 0.0, &mr);
@@ -635,8 +692,8 @@ static int restart(Solver& s, const Mats<T>& m, const T* b, const T* x, T* r, T*
     rc = psb_hip_csr_spmv(s.mat, -1.0, &mx, 1.0, &mr);
     if (rc) return rc;
     EllView<T> zero{n, n, 0, nullptr, nullptr};
-    hipLaunchKernelGGL((k_restart<T, EllView<T>>), dim3(g), blk, 0, s.stream, zero, (const T*)r,
-                       x, r, q, d, z, s.st, s.partials, bicg);
+    hipLaunchKernelGGL((k_restart<T, EllView<T>, 1>), dim3(g), blk, 0, s.stream, zero,
+                       (const T*)r, x, r, q, d, z, s.st, s.partials, bicg);
   }
   PSB_HIP_LAUNCH_CHECK();
   if (bicg) fin<FIN_RESTART_BI, 2>(s, g);
@@ -650,15 +707,13 @@ template <typename T>
 static int cg_iter(Solver& s, const Mats<T>& m, T* x, T* r, T* p, T* q, const T* d, T* z) {
   const int n = s.n;
   dim3 blk(kBlock), gv(vec_grid<T>(n));
-  const int gs = spmv_grid(n);
+  const int gs = spmv_grid(n, s.rpl);
   hipLaunchKernelGGL((k_cg_p<T>), gv, blk, 0, s.stream, n, (const T*)z, p, s.st);
   if (s.fmt == FMT_HLL) {
-    hipLaunchKernelGGL((k_cg_q<T, HllView<T>>), dim3(gs), blk, 0, s.stream, m.hll, (const T*)p,
-                       q, s.st, s.partials);
+    PSB_PRODUCT(k_cg_q, T, HllView<T>, s, gs, m.hll, (const T*)p, q, s.st, s.partials);
     fin<FIN_CG_Q, 1>(s, gs);
   } else if (s.fmt == FMT_ELL) {
-    hipLaunchKernelGGL((k_cg_q<T, EllView<T>>), dim3(gs), blk, 0, s.stream, m.ell, (const T*)p,
-                       q, s.st, s.partials);
+    PSB_PRODUCT(k_cg_q, T, EllView<T>, s, gs, m.ell, (const T*)p, q, s.st, s.partials);
     fin<FIN_CG_Q, 1>(s, gs);
   } else {
     int rc = csr_product(s, p, q);
@@ -679,16 +734,16 @@ static int bi_iter(Solver& s, const Mats<T>& m, T* x, T* r, T* q, T* p, T* v, T*
                    T* t, const T* d) {
   const int n = s.n;
   dim3 blk(kBlock), gv(vec_grid<T>(n));
-  const int gs = spmv_grid(n);
+  const int gs = spmv_grid(n, s.rpl);
   hipLaunchKernelGGL((k_bi_a<T>), gv, blk, 0, s.stream, n, (const T*)r, p, (const T*)v, d, f,
                      s.st);
   if (s.fmt == FMT_HLL) {
-    hipLaunchKernelGGL((k_bi_v<T, HllView<T>>), dim3(gs), blk, 0, s.stream, m.hll, (const T*)f, v,
-                       (const T*)q, s.st, s.partials);
+    PSB_PRODUCT(k_bi_v, T, HllView<T>, s, gs, m.hll, (const T*)f, v, (const T*)q, s.st,
+                s.partials);
     fin<FIN_BI_V, 1>(s, gs);
   } else if (s.fmt == FMT_ELL) {
-    hipLaunchKernelGGL((k_bi_v<T, EllView<T>>), dim3(gs), blk, 0, s.stream, m.ell, (const T*)f, v,
-                       (const T*)q, s.st, s.partials);
+    PSB_PRODUCT(k_bi_v, T, EllView<T>, s, gs, m.ell, (const T*)f, v, (const T*)q, s.st,
+                s.partials);
     fin<FIN_BI_V, 1>(s, gs);
   } else {
     int rc = csr_product(s, f, v);
@@ -700,12 +755,12 @@ static int bi_iter(Solver& s, const Mats<T>& m, T* x, T* r, T* q, T* p, T* v, T*
   hipLaunchKernelGGL((k_bi_s<T>), gv, blk, 0, s.stream, n, (const T*)r, (const T*)v, d, sv, z,
                      s.st);
   if (s.fmt == FMT_HLL) {
-    hipLaunchKernelGGL((k_bi_t<T, HllView<T>>), dim3(gs), blk, 0, s.stream, m.hll, (const T*)z, t,
-                       (const T*)sv, s.st, s.partials);
+    PSB_PRODUCT(k_bi_t, T, HllView<T>, s, gs, m.hll, (const T*)z, t, (const T*)sv, s.st,
+                s.partials);
     fin<FIN_BI_T, 2>(s, gs);
   } else if (s.fmt == FMT_ELL) {
-    hipLaunchKernelGGL((k_bi_t<T, EllView<T>>), dim3(gs), blk, 0, s.stream, m.ell, (const T*)z, t,
-                       (const T*)sv, s.st, s.partials);
+    PSB_PRODUCT(k_bi_t, T, EllView<T>, s, gs, m.ell, (const T*)z, t, (const T*)sv, s.st,
+                s.partials);
     fin<FIN_BI_T, 2>(s, gs);
   } else {
     int rc = csr_product(s, z, t);
@@ -724,6 +779,29 @@ static int bi_iter(Solver& s, const Mats<T>& m, T* x, T* r, T* q, T* p, T* v, T*
 static int read_state(Solver& s) {
   PSB_HIP_CHECK(hipMemcpyAsync(s.hst, s.st, sizeof(KState), hipMemcpyDeviceToHost, s.stream));
   PSB_HIP_CHECK(hipStreamSynchronize(s.stream));
+  return PSB_HIP_SUCCESS;
+}
+
+// Pipelined poll.  After queueing a batch of iterations the host queues a
+// copy of the state (post_state) and then waits only for the copy queued one
+// batch earlier (prev_halted), so the next batch is already queued while the
+// host looks at the previous one.  Batches queued after the stop are no-ops
+// (every kernel returns on the stop flag); read_state then gives the final
+// state.
+static int post_state(Solver& s) {
+  const int k = s.posted & 1;
+  PSB_HIP_CHECK(hipMemcpyAsync(&s.ring[k], s.st, sizeof(KState), hipMemcpyDeviceToHost, s.stream));
+  PSB_HIP_CHECK(hipEventRecord(s.ev[k], s.stream));
+  s.posted++;
+  return PSB_HIP_SUCCESS;
+}
+
+// Waits for the state copy queued before the last one; *halted = its stop or
+// breakdown flag.  Needs two copies posted since the last read_state.
+static int prev_halted(Solver& s, bool* halted) {
+  const int k = (s.posted - 2) & 1;
+  PSB_HIP_CHECK(hipEventSynchronize(s.ev[k]));
+  *halted = s.ring[k].done != 0 || s.ring[k].breakdown != 0;
   return PSB_HIP_SUCCESS;
 }
 
@@ -770,23 +848,29 @@ static int solve(Solver& s, const T* b, T* x, const T* d, double eps, int itmax,
       rc = restart<T>(s, m, b, x, w[0], w[1], d, nullptr, 1);
     }
     if (rc) return rc;
-    bool stop = false;
-    while (!stop) {
+    auto batch = [&]() -> int {
       for (int k = 0; k < check_every; k++) {
+        int rk;
         if (s.method == 0) {
           T* z = d ? w[3] : w[0];
-          rc = cg_iter<T>(s, m, x, w[0], w[1], w[2], d, z);
+          rk = cg_iter<T>(s, m, x, w[0], w[1], w[2], d, z);
         } else {
           T* f = d ? w[4] : w[2];
           T* z = d ? w[6] : w[5];
-          rc = bi_iter<T>(s, m, x, w[0], w[1], w[2], w[3], f, w[5], z, w[7], d);
+          rk = bi_iter<T>(s, m, x, w[0], w[1], w[2], w[3], f, w[5], z, w[7], d);
         }
-        if (rc) return rc;
+        if (rk) return rk;
       }
-      rc = read_state(s);
-      if (rc) return rc;
-      if (s.hst->done || s.hst->breakdown) stop = true;
+      return post_state(s);
+    };
+    s.posted = 0;
+    if ((rc = batch())) return rc;
+    bool stop = false;
+    while (!stop) {
+      if ((rc = batch())) return rc;
+      if ((rc = prev_halted(s, &stop))) return rc;
     }
+    if ((rc = read_state(s))) return rc;
     if (s.hst->done) break;
     // breakdown: restart unless the iteration budget is spent
     if (s.hst->itx >= itmax) break;
@@ -825,6 +909,8 @@ static int setup(Solver& s, int method, int fmt, void* dmat, int itmax) {
   rc = s.alloc((void**)&s.hist, sizeof(double) * ((size_t)itmax + 2));
   if (rc) return rc;
   PSB_HIP_CHECK(hipHostMalloc((void**)&s.hst, sizeof(KState), hipHostMallocDefault));
+  PSB_HIP_CHECK(hipHostMalloc((void**)&s.ring, 2 * sizeof(KState), hipHostMallocDefault));
+  for (hipEvent_t& e : s.ev) PSB_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   return PSB_HIP_SUCCESS;
 }
 
@@ -906,14 +992,13 @@ static int mixed_reliable(HllDev* h, const double* b, double* x, const double* d
   bool conv = bnrm == 0.0 ? rnorm <= eps : rnorm <= eps * bnrm;
   if (check_every < 1) check_every = 8;
   dim3 gv(vec_grid<float>(n));
-  const int gsp = spmv_grid(n);
+  const int gsp = spmv_grid(n, inner.rpl);
   while (!conv) {
     // queue FP32 iterations until a replacement (or breakdown) is requested
-    bool stop = false;
-    while (!stop) {
+    auto batch = [&]() -> int {
       for (int k = 0; k < check_every; k++) {
         hipLaunchKernelGGL((k_cg_p<float>), gv, blk, 0, stream, n, (const float*)z, p, inner.st);
-        hipLaunchKernelGGL((k_cg_q<float, HllView<float>>), dim3(gsp), blk, 0, stream, m32.hll,
+        hipLaunchKernelGGL((k_cg_q<float, HllView<float>, 1>), dim3(gsp), blk, 0, stream, m32.hll,
                            (const float*)p, q, inner.st, inner.partials);
         fin<FIN_CG_Q, 1>(inner, gsp);
         hipLaunchKernelGGL((k_cg_x<float>), gv, blk, 0, stream, n, x32, r, (const float*)p,
@@ -921,9 +1006,16 @@ static int mixed_reliable(HllDev* h, const double* b, double* x, const double* d
         fin<FIN_RU_X, 2>(inner, gv.x);
       }
       PSB_HIP_LAUNCH_CHECK();
-      if ((rc = read_state(inner))) return rc;
-      stop = inner.hst->breakdown != 0;
+      return post_state(inner);
+    };
+    inner.posted = 0;
+    if ((rc = batch())) return rc;
+    bool stop = false;
+    while (!stop) {
+      if ((rc = batch())) return rc;
+      if ((rc = prev_halted(inner, &stop))) return rc;
     }
+    if ((rc = read_state(inner))) return rc;
     // replacement: x += x32 in FP64, r = b - A x in FP64
     hipLaunchKernelGGL(k_ru_accum, gs, blk, 0, stream, n, x, x32);
     PSB_HIP_LAUNCH_CHECK();

@@ -301,78 +301,93 @@ struct CsrDev {
 };
 
 // ------------------------------------------------------------------ inner products
-// acc += sum_{j<w} val[j*stride] * x[ja[j*stride]-1], eight columns at a time:
-// all index/value loads of a group are issued before the dependent gathers.
+// One group of G slots of a row (slot j at j*stride): acc += val[j] * x[ja[j]-1]
+// in slot order.  Slots past the row's width w are read at slot w-1 (an
+// in-bounds address) and their products dropped, so every index/value load of
+// the group is unconditional and all of them issue before the first gather.
+// (Masking each load instead makes the compiler wait for each index load in
+// turn.)  Requires j < w.
+template <int G, typename T, typename TV>
+__device__ __forceinline__ T slot_group(const int* __restrict__ pj, const TV* __restrict__ pv,
+                                        int stride, int j, int w, const T* __restrict__ x,
+                                        T acc) {
+  int c[G];
+  TV v[G];
+#pragma unroll
+  for (int k = 0; k < G; k++) {
+    const int jj = j + k < w ? j + k : w - 1;
+    c[k] = ld_stream(pj + (size_t)jj * stride);
+    v[k] = ld_stream(pv + (size_t)jj * stride);
+  }
+  T xv[G];
+#pragma unroll
+  for (int k = 0; k < G; k++) xv[k] = x[c[k] - 1];
+#pragma unroll
+  for (int k = 0; k < G; k++) {
+    const T t = fma((T)v[k], xv[k], acc);
+    acc = j + k < w ? t : acc;
+  }
+  return acc;
+}
+
+// sum_{j<w} val[j*stride] * x[ja[j*stride]-1], eight slots per group, the
+// remainder in one group of 8 or 4.
 template <typename T, typename TV>
 __device__ __forceinline__ T slot_dot(const int* __restrict__ pj, const TV* __restrict__ pv,
                                       int stride, int w, const T* __restrict__ x) {
   T acc = T(0);
   int j = 0;
-  for (; j + 8 <= w; j += 8) {
-    int c[8];
-    TV v[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      c[k] = ld_stream(pj + (size_t)(j + k) * stride);
-      v[k] = ld_stream(pv + (size_t)(j + k) * stride);
-    }
-    T xv[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) xv[k] = x[c[k] - 1];
-#pragma unroll
-    for (int k = 0; k < 8; k++) acc = fma((T)v[k], xv[k], acc);
-  }
-  if (j < w) {
-    int c[8];
-    TV v[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      bool ok = j + k < w;
-      c[k] = ok ? ld_stream(pj + (size_t)(j + k) * stride) : 1;
-      v[k] = ok ? ld_stream(pv + (size_t)(j + k) * stride) : TV(0);
-    }
-    T xv[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) xv[k] = (j + k < w) ? x[c[k] - 1] : T(0);
-#pragma unroll
-    for (int k = 0; k < 8; k++) acc = fma((T)v[k], xv[k], acc);
-  }
+  for (; j + 8 <= w; j += 8) acc = slot_group<8, T, TV>(pj, pv, stride, j, w, x, acc);
+  const int rem = w - j;
+  if (rem > 4) acc = slot_group<8, T, TV>(pj, pv, stride, j, w, x, acc);
+  else if (rem > 0) acc = slot_group<4, T, TV>(pj, pv, stride, j, w, x, acc);
   return acc;
 }
 
-// Two independent rows at once: the index/value loads of both are issued
-// before their gathers, doubling the loads in flight per lane.
+template <typename T>
+using vec2 = T __attribute__((ext_vector_type(2)));
+
+// Two adjacent rows of one hack (same width w): one 8-byte index load and one
+// 2-element value load per slot feed both rows.  pj/pv address the even row's
+// slot 0 and are aligned to two elements.  Requires j < w.
+template <int G, typename T, typename TV>
+__device__ __forceinline__ void slot_group_pair(const int* __restrict__ pj,
+                                                const TV* __restrict__ pv, int stride, int j,
+                                                int w, const T* __restrict__ x, T& a0, T& a1) {
+  vec2<int> c[G];
+  vec2<TV> v[G];
+#pragma unroll
+  for (int k = 0; k < G; k++) {
+    const int jj = j + k < w ? j + k : w - 1;
+    c[k] = ld_stream(reinterpret_cast<const vec2<int>*>(pj + (size_t)jj * stride));
+    v[k] = ld_stream(reinterpret_cast<const vec2<TV>*>(pv + (size_t)jj * stride));
+  }
+  T x0[G], x1[G];
+#pragma unroll
+  for (int k = 0; k < G; k++) {
+    x0[k] = x[c[k].x - 1];
+    x1[k] = x[c[k].y - 1];
+  }
+#pragma unroll
+  for (int k = 0; k < G; k++) {
+    const T t0 = fma((T)v[k].x, x0[k], a0);
+    const T t1 = fma((T)v[k].y, x1[k], a1);
+    a0 = j + k < w ? t0 : a0;
+    a1 = j + k < w ? t1 : a1;
+  }
+}
+
 template <typename T, typename TV>
-__device__ __forceinline__ void slot_dot2(const int* __restrict__ pj0, const TV* __restrict__ pv0,
-                                          int w0, const int* __restrict__ pj1,
-                                          const TV* __restrict__ pv1, int w1, int stride,
-                                          const T* __restrict__ x, T& a0, T& a1) {
+__device__ __forceinline__ void slot_dot_pair(const int* __restrict__ pj,
+                                              const TV* __restrict__ pv, int stride, int w,
+                                              const T* __restrict__ x, T& a0, T& a1) {
   a0 = T(0);
   a1 = T(0);
-  const int w = w0 > w1 ? w0 : w1;
-  for (int j = 0; j < w; j += 4) {
-    int c0[4], c1[4];
-    TV v0[4], v1[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const bool ok0 = j + k < w0, ok1 = j + k < w1;
-      c0[k] = ok0 ? ld_stream(pj0 + (size_t)(j + k) * stride) : 1;
-      v0[k] = ok0 ? ld_stream(pv0 + (size_t)(j + k) * stride) : TV(0);
-      c1[k] = ok1 ? ld_stream(pj1 + (size_t)(j + k) * stride) : 1;
-      v1[k] = ok1 ? ld_stream(pv1 + (size_t)(j + k) * stride) : TV(0);
-    }
-    T x0[4], x1[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      x0[k] = (j + k < w0) ? x[c0[k] - 1] : T(0);
-      x1[k] = (j + k < w1) ? x[c1[k] - 1] : T(0);
-    }
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      a0 = fma((T)v0[k], x0[k], a0);
-      a1 = fma((T)v1[k], x1[k], a1);
-    }
-  }
+  int j = 0;
+  for (; j + 8 <= w; j += 8) slot_group_pair<8, T, TV>(pj, pv, stride, j, w, x, a0, a1);
+  const int rem = w - j;
+  if (rem > 4) slot_group_pair<8, T, TV>(pj, pv, stride, j, w, x, a0, a1);
+  else if (rem > 0) slot_group_pair<4, T, TV>(pj, pv, stride, j, w, x, a0, a1);
 }
 
 // 16-byte vectors of T for the streaming vector kernels

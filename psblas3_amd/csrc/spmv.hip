@@ -29,15 +29,16 @@ namespace psbhip {
 static int g_hll_variant = 0;
 
 // ------------------------------------------------------------------ HLL SpMV
-// HK > 0: compile-time hack size (32 or 64); HK == 0: any hksz.
-template <typename T, typename TV, int HK>
+// HK > 0: compile-time hack size (32 or 64); HK == 0: any hksz.  XCD: blocks
+// take XCD-contiguous row ranges (xcd_block); otherwise blocks in launch order.
+template <typename T, typename TV, int HK, bool XCD>
 __global__ __launch_bounds__(kBlock) void k_hll_spmv(int nr, int hksz, int nlog,
                                                      const int* __restrict__ hkoffs,
                                                      const int* __restrict__ ja,
                                                      const TV* __restrict__ val,
                                                      const T* __restrict__ x, T alpha, T beta,
                                                      T* __restrict__ y) {
-  const int lb = xcd_block(nlog);
+  const int lb = XCD ? xcd_block(nlog) : (int)blockIdx.x;
   if (lb >= nlog) return;
   const int row = lb * kBlock + threadIdx.x;
   const int r = row < nr ? row : nr - 1;
@@ -50,57 +51,30 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmv(int nr, int hksz, int nlog,
   if (row < nr) store_y(y, row, alpha, acc, beta);
 }
 
-// Two hacks' rows per lane (rows r and r + 32*... ): a lane handles row
-// base+lane and base+64+lane, doubling the loads in flight per wave.
-template <typename T, typename TV, int HK>
-__global__ __launch_bounds__(kBlock) void k_hll_spmv2(int nr, int hksz, int nlog,
-                                                      const int* __restrict__ hkoffs,
-                                                      const int* __restrict__ ja,
-                                                      const TV* __restrict__ val,
-                                                      const T* __restrict__ x, T alpha, T beta,
-                                                      T* __restrict__ y) {
-  const int lb = xcd_block(nlog);
+// Two adjacent rows per lane (rows 2t and 2t+1 of a 2*kBlock-row tile, same
+// hack since hksz is even): 8-byte index and 2-element value loads.  Rows
+// past nr read the last hack's padding rows (the hack stores hksz rows).
+template <typename T, typename TV, int HK, bool XCD>
+__global__ __launch_bounds__(kBlock) void k_hll_spmv_pair(int nr, int hksz, int nlog,
+                                                          const int* __restrict__ hkoffs,
+                                                          const int* __restrict__ ja,
+                                                          const TV* __restrict__ val,
+                                                          const T* __restrict__ x, T alpha,
+                                                          T beta, T* __restrict__ y) {
+  const int lb = XCD ? xcd_block(nlog) : (int)blockIdx.x;
   if (lb >= nlog) return;
-  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int row0 = lb * (2 * kBlock) + wave * 128 + l;
-  const int row1 = row0 + 64;
+  const int row = lb * (2 * kBlock) + 2 * (int)threadIdx.x;
+  const int last = (nr - 1) & ~1;
+  const int r = row < last ? row : last;
   const int stride = HK > 0 ? HK : hksz;
-  const int r0 = row0 < nr ? row0 : nr - 1;
-  const int r1 = row1 < nr ? row1 : nr - 1;
-  const int h0 = r0 / stride, h1 = r1 / stride;
-  const int o0 = hkoffs[h0], o1 = hkoffs[h1];
-  const int w0 = (hkoffs[h0 + 1] - o0) / stride, w1 = (hkoffs[h1 + 1] - o1) / stride;
-  const int* pj0 = ja + o0 + (r0 - h0 * stride);
-  const int* pj1 = ja + o1 + (r1 - h1 * stride);
-  const TV* pv0 = val + o0 + (r0 - h0 * stride);
-  const TV* pv1 = val + o1 + (r1 - h1 * stride);
-  T a0 = T(0), a1 = T(0);
-  const int w = w0 > w1 ? w0 : w1;
-  for (int j = 0; j < w; j += 4) {
-    int c0[4], c1[4];
-    TV v0[4], v1[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      bool ok0 = j + k < w0, ok1 = j + k < w1;
-      c0[k] = ok0 ? ld_stream(pj0 + (size_t)(j + k) * stride) : 1;
-      v0[k] = ok0 ? ld_stream(pv0 + (size_t)(j + k) * stride) : TV(0);
-      c1[k] = ok1 ? ld_stream(pj1 + (size_t)(j + k) * stride) : 1;
-      v1[k] = ok1 ? ld_stream(pv1 + (size_t)(j + k) * stride) : TV(0);
-    }
-    T x0[4], x1[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      x0[k] = (j + k < w0) ? x[c0[k] - 1] : T(0);
-      x1[k] = (j + k < w1) ? x[c1[k] - 1] : T(0);
-    }
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      a0 = fma((T)v0[k], x0[k], a0);
-      a1 = fma((T)v1[k], x1[k], a1);
-    }
-  }
-  if (row0 < nr) store_y(y, row0, alpha, a0, beta);
-  if (row1 < nr) store_y(y, row1, alpha, a1, beta);
+  const int hk = r / stride;
+  const int lane = r - hk * stride;
+  const int off0 = hkoffs[hk];
+  const int w = (hkoffs[hk + 1] - off0) / stride;
+  T a0, a1;
+  slot_dot_pair<T, TV>(ja + off0 + lane, val + off0 + lane, stride, w, x, a0, a1);
+  if (row < nr) store_y(y, row, alpha, a0, beta);
+  if (row + 1 < nr) store_y(y, row + 1, alpha, a1, beta);
 }
 
 // Multi-RHS HLL product (psb_d_hll_csmm, ext/impl/psb_d_hll_csmm.f90): each
@@ -366,36 +340,39 @@ static int check_vectors(const MultiVec* x, const MultiVec* y, int elem_type, in
   return PSB_HIP_SUCCESS;
 }
 
+template <typename T, typename TV, int HK>
+static void hll_launch_hk(const HllDev* m, bool pair, bool xcd, const TV* val, const T* x,
+                          T alpha, T beta, T* y) {
+  hipStream_t s = ctx().stream;
+  const int nlog = grid_for(m->rows, pair ? 2 * kBlock : kBlock);
+  dim3 grid(xcd ? xcd_grid(nlog) : nlog), blk(kBlock);
+  if (pair && xcd)
+    hipLaunchKernelGGL((k_hll_spmv_pair<T, TV, HK, true>), grid, blk, 0, s, m->rows, m->hksz, nlog,
+                       m->hkoffs, m->ja, val, x, alpha, beta, y);
+  else if (pair)
+    hipLaunchKernelGGL((k_hll_spmv_pair<T, TV, HK, false>), grid, blk, 0, s, m->rows, m->hksz,
+                       nlog, m->hkoffs, m->ja, val, x, alpha, beta, y);
+  else if (xcd)
+    hipLaunchKernelGGL((k_hll_spmv<T, TV, HK, true>), grid, blk, 0, s, m->rows, m->hksz, nlog,
+                       m->hkoffs, m->ja, val, x, alpha, beta, y);
+  else
+    hipLaunchKernelGGL((k_hll_spmv<T, TV, HK, false>), grid, blk, 0, s, m->rows, m->hksz, nlog,
+                       m->hkoffs, m->ja, val, x, alpha, beta, y);
+}
+
+// Variant: 1 one row per lane, 2 two adjacent rows per lane (even hksz), 0
+// chooses (pairs for 8-byte values, single rows for 4-byte values); adding 4
+// gives each XCD a contiguous range of blocks (xcd_block) instead of launch
+// order (measured 3-5% slower on the 7-point stencils at L = 200 and 256).
 template <typename T, typename TV>
 static int hll_launch(const HllDev* m, const TV* val, const T* x, T alpha, T beta, T* y) {
-  hipStream_t s = ctx().stream;
-  int variant = g_hll_variant;
-  if (variant == 0) variant = 1;
-  if (variant == 2) {
-    int nlog = grid_for(m->rows, 2 * kBlock);
-    dim3 grid(xcd_grid(nlog));
-    if (m->hksz == 64)
-      hipLaunchKernelGGL((k_hll_spmv2<T, TV, 64>), grid, dim3(kBlock), 0, s, m->rows, m->hksz,
-                         nlog, m->hkoffs, m->ja, val, x, alpha, beta, y);
-    else if (m->hksz == 32)
-      hipLaunchKernelGGL((k_hll_spmv2<T, TV, 32>), grid, dim3(kBlock), 0, s, m->rows, m->hksz,
-                         nlog, m->hkoffs, m->ja, val, x, alpha, beta, y);
-    else
-      hipLaunchKernelGGL((k_hll_spmv2<T, TV, 0>), grid, dim3(kBlock), 0, s, m->rows, m->hksz,
-                         nlog, m->hkoffs, m->ja, val, x, alpha, beta, y);
-  } else {
-    int nlog = grid_for(m->rows, kBlock);
-    dim3 grid(xcd_grid(nlog));
-    if (m->hksz == 64)
-      hipLaunchKernelGGL((k_hll_spmv<T, TV, 64>), grid, dim3(kBlock), 0, s, m->rows, m->hksz,
-                         nlog, m->hkoffs, m->ja, val, x, alpha, beta, y);
-    else if (m->hksz == 32)
-      hipLaunchKernelGGL((k_hll_spmv<T, TV, 32>), grid, dim3(kBlock), 0, s, m->rows, m->hksz,
-                         nlog, m->hkoffs, m->ja, val, x, alpha, beta, y);
-    else
-      hipLaunchKernelGGL((k_hll_spmv<T, TV, 0>), grid, dim3(kBlock), 0, s, m->rows, m->hksz,
-                         nlog, m->hkoffs, m->ja, val, x, alpha, beta, y);
-  }
+  const int variant = g_hll_variant & 3;
+  const bool xcd = (g_hll_variant & 4) != 0;
+  bool pair = variant == 2 || (variant == 0 && sizeof(TV) == 8);
+  if (m->hksz % 2 != 0) pair = false;
+  if (m->hksz == 64) hll_launch_hk<T, TV, 64>(m, pair, xcd, val, x, alpha, beta, y);
+  else if (m->hksz == 32) hll_launch_hk<T, TV, 32>(m, pair, xcd, val, x, alpha, beta, y);
+  else hll_launch_hk<T, TV, 0>(m, pair, xcd, val, x, alpha, beta, y);
   PSB_HIP_LAUNCH_CHECK();
   return PSB_HIP_SUCCESS;
 }
@@ -711,7 +688,7 @@ int psb_hip_hll_get_diag(void* dmat, void* dv) {
 
 int psb_hip_hll_set_variant(int variant) {
   int old = g_hll_variant;
-  if (variant >= 0 && variant <= 2) g_hll_variant = variant;
+  if ((variant & ~4) >= 0 && (variant & ~4) <= 2) g_hll_variant = variant;
   return old;
 }
 
