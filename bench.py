@@ -159,12 +159,12 @@ def solve_bench(P, N, L=256, eps=1e-10):
     return out
 
 
-def profiled_traffic(L):
+def profiled_traffic(L, variant=0):
     """HBM bytes per launch of the headline kernel, from the committed
     rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled)."""
     import re
     path = os.path.join(ROOT, "profiles", "spmv_hll_fp64_stencil_L200.md")
-    if L != 200 or not os.path.exists(path):
+    if L != 200 or variant != 0 or not os.path.exists(path):
         return None
     m = re.search(r"traffic_bytes_per_launch = (\d+)", open(path).read())
     return int(m.group(1)) if m else None
@@ -204,6 +204,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--solve-L", type=int, default=256)
+    ap.add_argument("--hll-variant", type=int, default=0,
+                    help="HLL kernel variant (psb_hip_hll_set_variant): 0 auto, 1 one row per "
+                         "lane, 2 row pairs, +4 XCD-contiguous block order")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -215,6 +218,7 @@ def main():
     from psblas3_amd import _native as N
 
     P.psb_init(0)
+    N.fn("psb_hip_hll_set_variant")(args.hll_variant)
     dev = P.psb_hip_DeviceName()
     log(f"bench: device {dev}, {P.psb_hip_MultiProcessors()} CUs")
     streams = stream_bandwidth(N)
@@ -319,10 +323,11 @@ def main():
         "dtype": "f64",
         "data": "synthetic (generated stencil and power-law matrices, x ~ U[-1,1) seed 1)",
         "config": {"workload": f"HLL FP64 SpMV, 3D 7-point convection-diffusion L={args.L}",
-                   "n": n, "nnz": nnz, "hksz": 64, "bytes_per_product": nbytes,
+                   "n": n, "nnz": nnz, "hksz": 64, "hll_variant": args.hll_variant,
+                   "bytes_per_product": nbytes,
                    "gflops": round(gflops, 2), "device": dev},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(args.L),
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(args.L, args.hll_variant),
                      "stream_copy_GBs": round(streams["copy"], 1),
                      "stream_triad_GBs": round(streams["triad"], 1),
                      "frac_of_stream": round(gbs / stream, 4)},
