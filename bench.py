@@ -273,6 +273,22 @@ def main():
             log(f"bench: {mat} {fmt} {dt} {t * 1e3:.1f} us  {sweep[f'{mat}_{fmt}_{dt}']['GBs']} GB/s")
             m.free()
         del xs, ys
+        # multi-RHS HLL SpMM on the same matrix: k columns, each U[-1,1) (stream = column)
+        for k in (8, 32):
+            xk = P.psb_d_hip_vect().all(n, k)
+            N.call("psb_hip_mv_fill_uniform", xk.dev, 1, 0)
+            xk.set_dev()
+            yk = P.psb_d_hip_vect().all(n, k)
+            t = time_spmv(P, head, xk, yk, max(args.steps // 4, 5), 2)
+            b = 12 * nnz + 4 * n + 16 * n * k
+            ent = {"us": round(t * 1e3, 2), "GBs": round(b / (t * 1e-3) / 1e9, 1),
+                   "GFLOPs": round(2 * nnz * k / (t * 1e-3) / 1e9, 2), "bytes": b,
+                   "us_per_column": round(t * 1e3 / k, 2)}
+            sweep[f"A_HLL64_f64_spmm_k{k}"] = ent
+            log(f"bench: A HLL64 f64 SpMM k={k} {t * 1e3:.1f} us  {ent['GBs']} GB/s "
+                f"({ent['us_per_column']} us per column)")
+            xk.free()
+            yk.free()
         # matrix B: power-law rows
         t0 = time.perf_counter()
         coo_b = gen_coo(P, N, "powerlaw", args.powerlaw_n)

@@ -215,7 +215,8 @@ int64_t psb_hip_gen_powerlaw(int n, int cap, uint64_t seed, int *ia, int *ja, do
 /* Counter-based uniform [-1,1) fill (host), the same stream the generators
    use: x(i) = u(seed, stream, i). */
 void psb_hip_fill_uniform_host(uint64_t seed, uint32_t stream, int64_t n, double *x);
-/* The same values generated on the device into a double multivector column. */
+/* The same values generated on the device into a double multivector (column c
+   of a multi-column vector uses stream + c). */
 int psb_hip_mv_fill_uniform(void *x, uint64_t seed, uint32_t stream);
 
 /* ===================================================================== */

@@ -77,9 +77,12 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmv_pair(int nr, int hksz, int 
   if (row + 1 < nr) store_y(y, row + 1, alpha, a1, beta);
 }
 
-// Multi-RHS HLL product (psb_d_hll_csmm, ext/impl/psb_d_hll_csmm.f90): each
-// lane owns one row and NC columns of x/y (column-major, pitch elements
-// apart); a slot's (ja, val) is loaded once and reused for all NC columns.
+// Multi-RHS HLL product (psb_d_hll_csmm, ext/impl/psb_d_hll_csmm.f90:33-235):
+// one row per lane and up to NC columns of x/y (column-major, ldx/ldy
+// elements apart) per launch.  Slots go in groups of 4 whose (ja, val) loads
+// issue together (slots past the row's width read its last slot, product
+// dropped, as in slot_group); each slot is loaded once and used for every
+// column, each column accumulating in slot order.
 template <typename T, int NC>
 __global__ __launch_bounds__(kBlock) void k_hll_spmm(int nr, int hksz, int nlog,
                                                      const int* __restrict__ hkoffs,
@@ -88,7 +91,7 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmm(int nr, int hksz, int nlog,
                                                      const T* __restrict__ x, int ldx,
                                                      T alpha, T beta, T* __restrict__ y,
                                                      int ldy, int ncols) {
-  const int lb = xcd_block(nlog);
+  const int lb = blockIdx.x;
   if (lb >= nlog) return;
   const int row = lb * kBlock + threadIdx.x;
   const int r = row < nr ? row : nr - 1;
@@ -100,18 +103,35 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmm(int nr, int hksz, int nlog,
   const T* pv = val + off0 + lane;
   T acc[NC];
 #pragma unroll
-  for (int c = 0; c < NC; c++) acc[c] = T(0);
-  for (int j = 0; j < w; j++) {
-    int col = ld_stream(pj + (size_t)j * hksz) - 1;
-    T v = ld_stream(pv + (size_t)j * hksz);
+  for (int q = 0; q < NC; q++) acc[q] = T(0);
+  for (int j = 0; j < w; j += 4) {
+    int c[4];
+    T v[4];
 #pragma unroll
-    for (int c = 0; c < NC; c++)
-      if (c < ncols) acc[c] = fma(v, x[(size_t)c * ldx + col], acc[c]);
+    for (int k = 0; k < 4; k++) {
+      const int jj = j + k < w ? j + k : w - 1;
+      c[k] = ld_stream(pj + (size_t)jj * hksz) - 1;
+      v[k] = ld_stream(pv + (size_t)jj * hksz);
+    }
+#pragma unroll
+    for (int q = 0; q < NC; q++) {
+      if (q < ncols) {
+        const T* xq = x + (size_t)q * ldx;
+        T xv[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) xv[k] = xq[c[k]];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const T t = fma(v[k], xv[k], acc[q]);
+          acc[q] = j + k < w ? t : acc[q];
+        }
+      }
+    }
   }
   if (row < nr) {
 #pragma unroll
-    for (int c = 0; c < NC; c++)
-      if (c < ncols) store_y(y + (size_t)c * ldy, row, alpha, acc[c], beta);
+    for (int q = 0; q < NC; q++)
+      if (q < ncols) store_y(y + (size_t)q * ldy, row, alpha, acc[q], beta);
   }
 }
 
@@ -381,24 +401,22 @@ template <typename T>
 static int hll_spmm_launch(const HllDev* m, const T* val, const T* x, int ldx, T alpha, T beta,
                            T* y, int ldy, int ncols) {
   hipStream_t s = ctx().stream;
-  int nlog = grid_for(m->rows, kBlock);
-  dim3 grid(xcd_grid(nlog));
-  for (int c0 = 0; c0 < ncols; c0 += 16) {
-    int nc = ncols - c0 < 16 ? ncols - c0 : 16;
+  const int nlog = grid_for(m->rows, kBlock);
+  dim3 grid(nlog), blk(kBlock);
+  for (int c0 = 0; c0 < ncols; c0 += 32) {
+    const int nc = ncols - c0 < 32 ? ncols - c0 : 32;
     const T* xc = x + (size_t)c0 * ldx;
     T* yc = y + (size_t)c0 * ldy;
-    if (nc > 8)
-      hipLaunchKernelGGL((k_hll_spmm<T, 16>), grid, dim3(kBlock), 0, s, m->rows, m->hksz, nlog,
-                         m->hkoffs, m->ja, val, xc, ldx, alpha, beta, yc, ldy, nc);
-    else if (nc > 4)
-      hipLaunchKernelGGL((k_hll_spmm<T, 8>), grid, dim3(kBlock), 0, s, m->rows, m->hksz, nlog,
-                         m->hkoffs, m->ja, val, xc, ldx, alpha, beta, yc, ldy, nc);
-    else if (nc > 1)
-      hipLaunchKernelGGL((k_hll_spmm<T, 4>), grid, dim3(kBlock), 0, s, m->rows, m->hksz, nlog,
-                         m->hkoffs, m->ja, val, xc, ldx, alpha, beta, yc, ldy, nc);
-    else
-      hipLaunchKernelGGL((k_hll_spmm<T, 1>), grid, dim3(kBlock), 0, s, m->rows, m->hksz, nlog,
-                         m->hkoffs, m->ja, val, xc, ldx, alpha, beta, yc, ldy, nc);
+#define PSB_SPMM_CASE(NC)                                                                    \
+  hipLaunchKernelGGL((k_hll_spmm<T, NC>), grid, blk, 0, s, m->rows, m->hksz, nlog, m->hkoffs, \
+                     m->ja, val, xc, ldx, alpha, beta, yc, ldy, nc)
+    if (nc > 16) PSB_SPMM_CASE(32);
+    else if (nc > 8) PSB_SPMM_CASE(16);
+    else if (nc > 4) PSB_SPMM_CASE(8);
+    else if (nc > 2) PSB_SPMM_CASE(4);
+    else if (nc > 1) PSB_SPMM_CASE(2);
+    else PSB_SPMM_CASE(1);
+#undef PSB_SPMM_CASE
     PSB_HIP_LAUNCH_CHECK();
   }
   return PSB_HIP_SUCCESS;

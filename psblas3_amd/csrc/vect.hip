@@ -690,13 +690,13 @@ int psb_hip_mv_fill_uniform(void* x, uint64_t seed, uint32_t stream) {
   MultiVec* v = MV(x);
   if (v->elem_type != PSB_HIP_TYPE_DOUBLE) return PSB_HIP_UNSUPPORTED;
   if (v->size <= 0) return PSB_HIP_SUCCESS;
-  // same key derivation as the host fill (gen.cpp)
-  uint64_t z = seed * 0xD1B54A32D192ED03ULL ^ ((uint64_t)stream << 48);
-  z += 0x9E3779B97F4A7C15ULL;
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-  uint64_t key = z ^ (z >> 31);
   for (int c = 0; c < v->count; c++) {
+    // same key derivation as the host fill (host.cpp); column c uses stream + c
+    uint64_t z = seed * 0xD1B54A32D192ED03ULL ^ ((uint64_t)(stream + (uint32_t)c) << 48);
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    const uint64_t key = z ^ (z >> 31);
     hipLaunchKernelGGL(k_fill_uniform, dim3(stream_grid(v->size)), dim3(kBlock), 0, ctx().stream,
                        (double*)v->v + (int64_t)c * v->pitch, (int64_t)v->size, key);
     PSB_HIP_LAUNCH_CHECK();

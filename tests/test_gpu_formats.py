@@ -155,22 +155,25 @@ def test_spmv_poisson_exact(gpu, orc):
         a.free()
 
 
-def test_spmv_multicolumn(gpu, orc):
-    _say("case multicolumn hll spmm k=5")
-    coo = orc.gen_convdiff(17)
+@pytest.mark.parametrize("k,kind", [(2, "stencil"), (5, "stencil"), (8, "powerlaw"),
+                                    (33, "stencil")])
+def test_spmv_multicolumn(gpu, orc, k, kind):
+    _say(f"case multicolumn hll spmm k={k} {kind}")
+    coo = orc.gen_convdiff(17) if kind == "stencil" else orc.gen_powerlaw(3000, cap=150, seed=7)
     f = orc.fix_coo(coo)
-    h = orc.hll_from_coo(f, 32)
-    a = _mat(coo, P.psb_d_hip_hlg_sparse_mat(32))
-    X = np.stack([orc.uniform_pm1(s, coo.nc) for s in range(5)], axis=1)
-    Y0 = np.stack([orc.uniform_pm1(20 + s, coo.nr) for s in range(5)], axis=1)
+    hk = 32 if kind == "stencil" else 64
+    h = orc.hll_from_coo(f, hk)
+    a = _mat(coo, P.psb_d_hip_hlg_sparse_mat(hk))
+    X = np.stack([orc.uniform_pm1(s, coo.nc) for s in range(k)], axis=1)
+    Y0 = np.stack([orc.uniform_pm1(20 + s, coo.nr) for s in range(k)], axis=1)
     xv = P.vect_from(X)
     yv = P.vect_from(Y0)
     P.psb_spmm(2.0, a, xv, -1.0, yv)
     Y = yv.get_vect()
     Yref = orc.hll_spmm(h, X, 2.0, -1.0, Y0)
-    for k in range(5):
-        bound = _row_bound(f, X[:, k], 2.0, -1.0, Y0[:, k], np.finfo(float).eps)
-        assert np.all(np.abs(Y[:, k] - Yref[:, k]) <= bound)
+    for col in range(k):
+        bound = _row_bound(f, X[:, col], 2.0, -1.0, Y0[:, col], np.finfo(float).eps)
+        assert np.all(np.abs(Y[:, col] - Yref[:, col]) <= bound)
     a.free()
 
 
@@ -181,13 +184,14 @@ def test_hll_variants_agree(gpu, orc):
     a = _mat(coo, P.psb_d_hip_hlg_sparse_mat(64))
     x = P.vect_from(orc.uniform_pm1(3, coo.nc))
     outs = []
-    for var in (1, 2):
+    for var in (1, 2, 5, 6):
         old = N.fn("psb_hip_hll_set_variant")(var)
         y = P.vect_from(np.zeros(coo.nr))
         P.psb_spmm(1.0, a, x, 0.0, y)
         outs.append(y.get_vect())
         N.fn("psb_hip_hll_set_variant")(old)
-    assert np.array_equal(outs[0], outs[1])   # same per-row order -> same bits
+    for o in outs[1:]:
+        assert np.array_equal(outs[0], o)   # same per-row order -> same bits
     a.free()
 
 
