@@ -159,6 +159,116 @@ def solve_bench(P, N, L=256, eps=1e-10):
     return out
 
 
+def aniso_matrix(P, N, L, e):
+    """-e*u_xx - u_yy, 5-point on an L x L grid, as an HLG (hksz 64) matrix."""
+    import numpy as np
+    fn = N.fn("psb_hip_gen_aniso2d")
+    nz = fn(L, e, None, None, None)
+    ia = np.empty(nz, np.int32); ja = np.empty(nz, np.int32); val = np.empty(nz)
+    fn(L, e, ia.ctypes.data, ja.ctypes.data, val.ctypes.data)
+    coo = P.psb_d_coo_sparse_mat(L * L, L * L, ia, ja, val)
+    coo.fix()
+    m = P.psb_dspmat_type()
+    m.m = m.n = L * L
+    m.a = P.psb_d_hip_hlg_sparse_mat(64).cp_from_coo(coo)
+    return m
+
+
+def batch_bench(P, N, nsys=256, eps=1e-8, seed=3):
+    """Batched uneven solves: nsys anisotropic 2-D Poisson systems -e u_xx - u_yy,
+    side L log-uniform in [32, 512] and e log-uniform in [1e-3, 1] (host uniform
+    stream 0 and 1 of the seed), b = A x_true with x_true ~ U[-1,1) (stream
+    1000 + k).  FP32-inner / FP64-refined CG to eps in the batch engine, against
+    the same solves one at a time (FP64 fused CG; single-system mixed CG)."""
+    import numpy as np
+    u = np.empty(nsys)
+    N.fn("psb_hip_fill_uniform_host")(seed, 0, nsys, u.ctypes.data)
+    ue = np.empty(nsys)
+    N.fn("psb_hip_fill_uniform_host")(seed, 1, nsys, ue.ctypes.data)
+    Ls = np.rint(32.0 * 16.0 ** ((u + 1.0) / 2.0)).astype(int)
+    es = 10.0 ** (-3.0 + 3.0 * (ue + 1.0) / 2.0)
+    t0 = time.perf_counter()
+    systems = []
+    for k in range(nsys):
+        L, e = int(Ls[k]), float(es[k])
+        m = aniso_matrix(P, N, L, e)
+        n = L * L
+        desc = P.psb_cdall(nl=n)
+        prec = P.psb_dprec_type("DIAG").build(m, desc)
+        xt = P.psb_d_hip_vect().all(n)
+        N.call("psb_hip_mv_fill_uniform", xt.dev, seed, 1000 + k)
+        xt.set_dev()
+        b = P.psb_d_hip_vect().all(n)
+        P.psb_spmm(1.0, m, xt, 0.0, b)
+        systems.append((m, prec, desc, xt, b, n))
+    rows = int(sum(s[5] for s in systems))
+    nnz = int(sum(s[0].a.nnz for s in systems))
+    log(f"batch: {nsys} systems, L {Ls.min()}..{Ls.max()}, {rows} rows, {nnz} nnz, "
+        f"set up in {time.perf_counter() - t0:.1f}s")
+    out = {"systems": nsys, "rows": rows, "nnz": nnz, "L_min": int(Ls.min()),
+           "L_max": int(Ls.max()), "eps": eps}
+
+    def check(xs):
+        errs = [float(np.linalg.norm(x.get_vect() - s[3].get_vect()) / np.linalg.norm(s[3].get_vect()))
+                for x, s in zip(xs, systems)]
+        return max(errs)
+
+    # batch engine, arena sized for every system at once (3 FP64 + 5 FP32 vectors each)
+    total = 0
+    for s in systems:
+        n = s[5]
+        total += 3 * ((8 * n + 255) // 256 * 256) + 5 * ((4 * n + 255) // 256 * 256)
+    eng = P.psb_hip_batch("CG", max_systems=nsys, arena_bytes=total + (1 << 20), check_every=8)
+    mids = [eng.register(s[0], s[1]) for s in systems]
+    fps = [eng.footprint(m) for m in mids]
+    assert sum(fps) <= total
+    xs = [P.psb_d_hip_vect().all(s[5]) for s in systems]
+    jobs = [(mid, s[4], x) for mid, s, x in zip(mids, systems, xs)]
+    eng.solve(jobs, eps=eps, itmax=20000)           # warm: graph capture, first launches
+    P.psb_hip_DeviceSync()
+    t0 = time.perf_counter()
+    res = eng.solve(jobs, eps=eps, itmax=20000)
+    P.psb_hip_DeviceSync()
+    dt = time.perf_counter() - t0
+    eng.free()
+    out["batch"] = {"seconds": round(dt, 4), "solves_per_s": round(nsys / dt, 1),
+                    "converged": int(sum(r["status"] == 1 for r in res)),
+                    "max_relres": max(r["relres"] for r in res),
+                    "max_err_vs_xtrue": check(xs),
+                    "fp32_iterations_max": int(max(r["iterations"] for r in res)),
+                    "fp32_iterations_total": int(sum(r["iterations"] for r in res)),
+                    "arena_bytes": int(sum(fps))}
+    log(f"batch: engine {dt * 1e3:.1f} ms for {nsys} solves, max relres "
+        f"{out['batch']['max_relres']:.2e}, max its {out['batch']['fp32_iterations_max']}")
+    # one at a time
+    for name in ("fp64_one_at_a_time", "mixed_one_at_a_time"):
+        xs = [P.psb_d_vect_type(P.psb_d_hip_vect().all(s[5])) for s in systems]
+        its = []
+        P.psb_hip_DeviceSync()
+        t0 = time.perf_counter()
+        for s, x in zip(systems, xs):
+            b = P.psb_d_vect_type(s[4])
+            if name.startswith("fp64"):
+                it, err, _ = P.psb_krylov_fused("CG", s[0], s[1], b, x, eps, s[2], itmax=20000,
+                                                check_every=16)
+            else:
+                _, it, err = P.psb_krylov_mixed("CG", s[0], s[1], b, x, eps, s[2], outer_itmax=50,
+                                                inner_eps=1e-4, inner_itmax=20000, check_every=16,
+                                                variant="reliable")
+            its.append(it)
+        P.psb_hip_DeviceSync()
+        t = time.perf_counter() - t0
+        out[name] = {"seconds": round(t, 4), "solves_per_s": round(nsys / t, 1),
+                     "iterations_total": int(sum(its)),
+                     "max_err_vs_xtrue": check([x.v for x in xs])}
+        log(f"batch: {name} {t * 1e3:.1f} ms")
+    out["speedup_vs_fp64_one_at_a_time"] = round(out["fp64_one_at_a_time"]["seconds"] / dt, 3)
+    out["speedup_vs_mixed_one_at_a_time"] = round(out["mixed_one_at_a_time"]["seconds"] / dt, 3)
+    for s in systems:
+        s[0].free()
+    return out
+
+
 def profiled_traffic(L, variant=0):
     """HBM bytes per launch of the headline kernel, from the committed
     rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled)."""
@@ -204,6 +314,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--solve-L", type=int, default=256)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-systems", type=int, default=256)
     ap.add_argument("--hll-variant", type=int, default=0,
                     help="HLL kernel variant (psb_hip_hll_set_variant): 0 auto, 1 one row per "
                          "lane, 2 row pairs, +4 XCD-contiguous block order")
@@ -321,6 +433,7 @@ def main():
         sweep["B_n"], sweep["B_nnz"] = nb, nnzb
 
     solve = None if args.no_solve else solve_bench(P, N, args.solve_L)
+    batch = None if args.no_batch else batch_bench(P, N, args.batch_systems)
     cpu = None if args.no_cpu else cpu_baseline()
     if cpu:
         log(f"bench: cpu baseline {cpu['value']} GB/s on {cpu['cores']} threads")
@@ -350,6 +463,7 @@ def main():
         "cpu_baseline": cpu,
         "sweep": sweep,
         "solve": solve,
+        "batch": batch,
     }
     print(json.dumps(line), flush=True)
 

@@ -33,6 +33,7 @@ extern "C" {
 #define PSB_HIP_OUTOFMEM     3
 #define PSB_HIP_RUNTIME      4
 #define PSB_HIP_NOT_INIT     5
+#define PSB_HIP_BUSY         6   /* batch engine: no free slot or arena room now */
 
 /* element types (cuda/spgpu/core.h:77-79) */
 #define PSB_HIP_TYPE_INT     0
@@ -352,6 +353,47 @@ int psb_hip_krylov_fused(int method, int fmt, void *dmat, void *diag, void *b, v
 int psb_hip_krylov_mixed(int method, void *dmat, void *diag, void *b, void *x, double eps,
                          int outer_itmax, double inner_eps, int inner_itmax, int check_every,
                          int variant, int *outer_its, int *inner_its, double *err);
+
+/* ===================================================================== */
+/* Batched mixed-precision solves -- no counterpart in the reference.       */
+/* Many independent systems (each an HLL double matrix registered once,     */
+/* with its FP32 copy and optional Jacobi diagonal) advance together: a     */
+/* step runs check_every FP32 iterations of every live system and one FP64  */
+/* transition pass (method 0: CG with FP64 residual replacement when the    */
+/* recursive residual drops by delta; method 1: BiCGSTAB restarted          */
+/* refinement with inner tolerance inner_eps), as one captured graph when   */
+/* use_graph != 0.  Systems are admitted (submit) and retired (result)      */
+/* between steps; the vectors of a system live in the engine's arena of     */
+/* arena_bytes, and submit returns PSB_HIP_BUSY when no slot or arena room  */
+/* is free.  A system's results do not depend on the other systems.         */
+/* Per-iteration sequence: krylov/psb_dcg.F90:241-294 and                   */
+/* krylov/psb_dcgstab.f90:248-364; stop test ||b - A x|| <= eps ||b|| in    */
+/* FP64 (krylov/psb_d_krylov_conv_mod.f90:279-356, istop 2).                */
+/* ===================================================================== */
+int psb_hip_batch_create(void **batch, int method, int max_systems, int64_t arena_bytes,
+                         int check_every, double delta, double inner_eps, int use_graph);
+int psb_hip_batch_free(void *batch);
+/* dmat: HLL double matrix (square); diag: double vector of 1/a_ii (as
+   psb_hip_mv_diag_inv leaves it) or NULL.  The engine keeps pointers to
+   both: they must outlive the engine. */
+int psb_hip_batch_register(void *batch, void *dmat, void *diag, int *mat_id);
+/* arena bytes one solve on matrix mat_id occupies (-1 for a bad id) */
+int64_t psb_hip_batch_footprint(void *batch, int mat_id);
+/* b (double vector, copied), x0 (double vector, copied) or NULL for zero;
+   *sys_id receives the slot.  Takes effect at the next step. */
+int psb_hip_batch_submit(void *batch, int mat_id, void *b, void *x0, double eps, int itmax,
+                         int *sys_id);
+/* queue one step (tables first when the live set changed) */
+int psb_hip_batch_step(void *batch);
+/* Systems finished as of the step before the last one queued (wait_last:
+   the last one), up to max_out: slot ids, FP32 iterations, FP64
+   transitions, final ||b - A x||/||b||, status (1 converged, 2 iteration
+   limit).  *nlive: systems still running. */
+int psb_hip_batch_poll(void *batch, int wait_last, int max_out, int *ids, int *iters,
+                       int *outer, double *relres, int *status, int *nout, int *nlive);
+/* copy a finished system's solution into x (double vector, may be NULL) and
+   free its slot */
+int psb_hip_batch_result(void *batch, int sys_id, void *x);
 
 #ifdef __cplusplus
 }

@@ -21,3 +21,4 @@ from .psblas import (matrix_from_coo, psb_cdall, psb_cdasb, psb_cscnv, psb_desc_
                      psb_xyzw, vect_from)
 from .prec import psb_dprec_type, psb_precinit  # noqa: F401
 from .krylov import psb_krylov, psb_krylov_fused, psb_krylov_mixed  # noqa: F401
+from .batch import psb_hip_batch  # noqa: F401

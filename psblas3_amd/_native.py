@@ -12,12 +12,13 @@ LIB_PATH = os.path.join(_HERE, "lib", "libpsb_hip.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "psb_hip.h")
 
 # return codes and element types (include/psb_hip.h)
-SUCCESS, UNSUPPORTED, INVALID, OUTOFMEM, RUNTIME, NOT_INIT = 0, 1, 2, 3, 4, 5
+SUCCESS, UNSUPPORTED, INVALID, OUTOFMEM, RUNTIME, NOT_INIT, BUSY = 0, 1, 2, 3, 4, 5, 6
 TYPE_INT, TYPE_FLOAT, TYPE_DOUBLE = 0, 1, 2
 INS_OVERWRITE, INS_ADD = 0, 1
 
 _ERRNAMES = {UNSUPPORTED: "unsupported", INVALID: "invalid argument", OUTOFMEM: "out of memory",
-             RUNTIME: "HIP runtime error", NOT_INIT: "not initialised"}
+             RUNTIME: "HIP runtime error", NOT_INIT: "not initialised",
+             BUSY: "no free slot or arena room"}
 
 
 class PsbError(RuntimeError):
@@ -141,6 +142,14 @@ _SIGS = {
     "psb_hip_krylov_fused": (i32, [i32, i32, vp, vp, vp, vp, f64, i32, i32, i32, pi32, pf64, vp]),
     "psb_hip_krylov_mixed": (i32, [i32, vp, vp, vp, vp, f64, i32, f64, i32, i32, i32, pi32,
                                    pi32, pf64]),
+    "psb_hip_batch_create": (i32, [pvp, i32, i32, i64, i32, f64, f64, i32]),
+    "psb_hip_batch_free": (i32, [vp]),
+    "psb_hip_batch_register": (i32, [vp, vp, vp, pi32]),
+    "psb_hip_batch_footprint": (i64, [vp, i32]),
+    "psb_hip_batch_submit": (i32, [vp, i32, vp, vp, f64, i32, pi32]),
+    "psb_hip_batch_step": (i32, [vp]),
+    "psb_hip_batch_poll": (i32, [vp, i32, i32, pi32, pi32, pi32, pf64, pi32, pi32, pi32]),
+    "psb_hip_batch_result": (i32, [vp, i32, vp]),
 }
 
 

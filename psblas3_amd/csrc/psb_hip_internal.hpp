@@ -45,6 +45,12 @@ struct Context {
 };
 Context& ctx();
 int ensure_init();                      // lazily initialises device 0
+// early return with the error code when the library cannot initialise
+#define PSB_ENSURE_INIT()          \
+  do {                             \
+    int rc_ = ::psbhip::ensure_init(); \
+    if (rc_) return rc_;           \
+  } while (0)
 
 // ------------------------------------------------------------------ vectors
 // Device multivector, the counterpart of cuda/vectordev.h:39-52.

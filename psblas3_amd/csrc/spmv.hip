@@ -459,12 +459,6 @@ static int csr_launch(CsrDev* m, const T* x, T alpha, T beta, T* y) {
 
 using namespace psbhip;
 
-#define PSB_ENSURE_INIT()          \
-  do {                             \
-    int rc_ = ensure_init();       \
-    if (rc_) return rc_;           \
-  } while (0)
-
 extern "C" {
 
 // ================================================================== HLL

@@ -248,12 +248,6 @@ static int nrm2_col(int n, const T* x, double* res) {
   return PSB_HIP_SUCCESS;
 }
 
-#define PSB_ENSURE_INIT()          \
-  do {                             \
-    int rc_ = ensure_init();       \
-    if (rc_) return rc_;           \
-  } while (0)
-
 template <typename T>
 static int setscal_impl(T val, int first, int last, int index_base, void* x) {
   PSB_ENSURE_INIT();
