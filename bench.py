@@ -269,6 +269,73 @@ def batch_bench(P, N, nsys=256, eps=1e-8, seed=3):
     return out
 
 
+def serve_bench(P, N, nmat=64, seed=4, rates=(200.0, 1000.0, 5000.0), count=160, eps=1e-10):
+    """Serving loop: nmat registered 3-D 7-point convection-diffusion matrices
+    (L uniform in [24, 160], convection magnitude uniform in [0, 10]: host
+    uniform streams 0 and 1 of the seed; HLG hksz 64) with Jacobi
+    preconditioners resident on the device.  For each rate, count requests
+    with Poisson arrivals, each on a uniformly chosen matrix with a U[-1,1)
+    right-hand side, served by mixed-precision BiCGSTAB (continuous batching)
+    and, on the same stream, one at a time by FP64 BiCGSTAB."""
+    import numpy as np
+    u = np.empty(nmat)
+    N.fn("psb_hip_fill_uniform_host")(seed, 0, nmat, u.ctypes.data)
+    w = np.empty(nmat)
+    N.fn("psb_hip_fill_uniform_host")(seed, 1, nmat, w.ctypes.data)
+    Ls = np.minimum(24 + ((u + 1.0) / 2.0 * 137).astype(int), 160)
+    convs = (w + 1.0) / 2.0 * 10.0
+    t0 = time.perf_counter()
+    systems = []
+    fn = N.fn("psb_hip_gen_pde3d")
+    a3 = np.ones(3)
+    for L, cv in zip(Ls, convs):
+        L = int(L)
+        b3 = np.full(3, cv / np.sqrt(3.0))
+        nz = fn(L, a3.ctypes.data, b3.ctypes.data, 0.0, 1, 0, None, None, None, None)
+        ia = np.empty(nz, np.int32); ja = np.empty(nz, np.int32); val = np.empty(nz)
+        fn(L, a3.ctypes.data, b3.ctypes.data, 0.0, 1, 0, ia.ctypes.data, ja.ctypes.data,
+           val.ctypes.data, None)
+        coo = P.psb_d_coo_sparse_mat(L ** 3, L ** 3, ia, ja, val)
+        coo.fix()
+        m = P.psb_dspmat_type()
+        m.m = m.n = L ** 3
+        m.a = P.psb_d_hip_hlg_sparse_mat(64).cp_from_coo(coo)
+        desc = P.psb_cdall(nl=L ** 3)
+        prec = P.psb_dprec_type("DIAG").build(m, desc)
+        systems.append((m, prec, desc))
+        del coo, ia, ja, val
+    rows = int(sum(int(L) ** 3 for L in Ls))
+    log(f"serve: {nmat} matrices, L {Ls.min()}..{Ls.max()}, {rows} rows resident, set up in "
+        f"{time.perf_counter() - t0:.1f}s")
+    out = {"matrices": nmat, "rows_resident": rows, "L_min": int(Ls.min()), "L_max": int(Ls.max()),
+           "eps": eps, "requests_per_rate": count, "method": "BiCGSTAB + Jacobi",
+           "rhs": "U[-1,1) generated on the device per request"}
+    srv = P.psb_hip_server("BICGSTAB", max_systems=256, arena_bytes=6 << 30, check_every=4,
+                           eps=eps)
+    for m, prec, _ in systems:
+        srv.register(m, prec)
+    srv.run([(0.0, k, 999000 + k) for k in range(4)])      # warm: graph capture, first launches
+    rates_out = {}
+    for ri, rate in enumerate(rates):
+        reqs = P.poisson_arrivals(rate, count, nmat, seed=seed, stream=10 * (ri + 1))
+        rec = srv.run(reqs, time_limit=90.0)
+        mixed = P.latency_summary(rec)
+        base = P.serve_one_at_a_time(systems, reqs, eps=eps, seed=seed, time_limit=90.0)
+        fp64 = P.latency_summary(base)
+        rates_out[str(int(rate))] = {"offered_per_s": rate,
+                                     "arrival_span_s": round(reqs[-1][0], 4),
+                                     "mixed_batched": mixed, "fp64_one_at_a_time": fp64}
+        log(f"serve: {rate:.0f} req/s: batched p50 {mixed.get('p50_ms')} ms p99 "
+            f"{mixed.get('p99_ms')} ms {mixed.get('solves_per_s')} solves/s | FP64 one at a time "
+            f"p50 {fp64.get('p50_ms')} ms p99 {fp64.get('p99_ms')} ms {fp64.get('solves_per_s')} "
+            f"solves/s")
+    srv.free()
+    out["rates"] = rates_out
+    for m, _, _ in systems:
+        m.free()
+    return out
+
+
 def profiled_traffic(L, variant=0):
     """HBM bytes per launch of the headline kernel, from the committed
     rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled)."""
@@ -315,6 +382,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--solve-L", type=int, default=256)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--batch-systems", type=int, default=256)
     ap.add_argument("--hll-variant", type=int, default=0,
                     help="HLL kernel variant (psb_hip_hll_set_variant): 0 auto, 1 one row per "
@@ -434,6 +503,7 @@ def main():
 
     solve = None if args.no_solve else solve_bench(P, N, args.solve_L)
     batch = None if args.no_batch else batch_bench(P, N, args.batch_systems)
+    serve = None if args.no_serve else serve_bench(P, N, count=args.serve_requests)
     cpu = None if args.no_cpu else cpu_baseline()
     if cpu:
         log(f"bench: cpu baseline {cpu['value']} GB/s on {cpu['cores']} threads")
@@ -464,6 +534,7 @@ def main():
         "sweep": sweep,
         "solve": solve,
         "batch": batch,
+        "serve": serve,
     }
     print(json.dumps(line), flush=True)
 

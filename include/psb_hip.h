@@ -383,6 +383,9 @@ int64_t psb_hip_batch_footprint(void *batch, int mat_id);
    *sys_id receives the slot.  Takes effect at the next step. */
 int psb_hip_batch_submit(void *batch, int mat_id, void *b, void *x0, double eps, int itmax,
                          int *sys_id);
+/* 1 when a solve on matrix mat_id would be admitted now (a free slot and
+   arena room), else 0 */
+int psb_hip_batch_can_admit(void *batch, int mat_id);
 /* queue one step (tables first when the live set changed) */
 int psb_hip_batch_step(void *batch);
 /* Systems finished as of the step before the last one queued (wait_last:

@@ -22,3 +22,5 @@ from .psblas import (matrix_from_coo, psb_cdall, psb_cdasb, psb_cscnv, psb_desc_
 from .prec import psb_dprec_type, psb_precinit  # noqa: F401
 from .krylov import psb_krylov, psb_krylov_fused, psb_krylov_mixed  # noqa: F401
 from .batch import psb_hip_batch  # noqa: F401
+from .serve import (latency_summary, poisson_arrivals, psb_hip_server,  # noqa: F401
+                    serve_one_at_a_time)

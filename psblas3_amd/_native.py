@@ -147,6 +147,7 @@ _SIGS = {
     "psb_hip_batch_register": (i32, [vp, vp, vp, pi32]),
     "psb_hip_batch_footprint": (i64, [vp, i32]),
     "psb_hip_batch_submit": (i32, [vp, i32, vp, vp, f64, i32, pi32]),
+    "psb_hip_batch_can_admit": (i32, [vp, i32]),
     "psb_hip_batch_step": (i32, [vp]),
     "psb_hip_batch_poll": (i32, [vp, i32, i32, pi32, pi32, pi32, pf64, pi32, pi32, pi32]),
     "psb_hip_batch_result": (i32, [vp, i32, vp]),

@@ -42,7 +42,6 @@ class psb_hip_batch:
         self.check_every = check_every
         self._keep = []          # registered matrices / preconditioners stay alive
         self._n = {}             # mat_id -> rows
-        self._ids = C.c_int * max_systems
 
     # ------------------------------------------------------------ registration
     def register(self, a, prec=None):
@@ -66,6 +65,10 @@ class psb_hip_batch:
         return int(N.fn("psb_hip_batch_footprint")(self.h, int(mat_id)))
 
     # ------------------------------------------------------------ systems
+    def can_admit(self, mat_id):
+        """True when a solve on mat_id would be admitted now."""
+        return bool(N.fn("psb_hip_batch_can_admit")(self.h, int(mat_id)))
+
     def submit(self, mat_id, b, x0=None, eps=1e-8, itmax=10000):
         """Admit one solve (b and x0 are copied).  Returns its slot id, or None
         when no slot or arena room is free now."""
@@ -121,8 +124,8 @@ class psb_hip_batch:
         """Solve jobs = [(mat_id, b, x_out)] with continuous admission: each job
         is submitted as soon as a slot and arena room are free, steps run
         until every job has finished, and each solution is copied into its
-        x_out.  Returns per-job dicts (iterations, outer, relres, status,
-        steps_waited)."""
+        x_out.  Returns per-job dicts (id, iterations, outer, relres, status,
+        admitted_at_step)."""
         pending = list(range(len(jobs)))
         slot_job = {}
         out = [None] * len(jobs)

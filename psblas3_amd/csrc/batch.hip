@@ -795,6 +795,22 @@ int psb_hip_batch_submit(void* batch, int mat_id, void* bv, void* x0v, double ep
   return PSB_HIP_SUCCESS;
 }
 
+int psb_hip_batch_can_admit(void* batch, int mat_id) {
+  Batch* B = (Batch*)batch;
+  if (!B || mat_id < 0 || mat_id >= (int)B->mats.size()) return 0;
+  bool slot = false;
+  for (const BSlot& sl : B->slots)
+    if (!sl.live) {
+      slot = true;
+      break;
+    }
+  if (!slot) return 0;
+  const size_t bytes = sys_bytes(B->method, B->mats[mat_id].h->rows);
+  for (const auto& fr : B->free_ranges)
+    if (fr.second >= bytes) return 1;
+  return 0;
+}
+
 int psb_hip_batch_step(void* batch) {
   Batch* B = (Batch*)batch;
   if (!B) return PSB_HIP_INVALID;
