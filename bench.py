@@ -454,6 +454,12 @@ def main():
             log(f"bench: {mat} {fmt} {dt} {t * 1e3:.1f} us  {sweep[f'{mat}_{fmt}_{dt}']['GBs']} GB/s")
             m.free()
         del xs, ys
+        # autotuner: hack size 32/64 x the four kernel variants on the same matrix
+        tuned, rep = P.psb_hip_hll_autotune(coo_a, reps=20)
+        tuned.free()
+        sweep["A_autotune_f64"] = rep
+        log(f"bench: autotune picked hksz {rep['hksz']} variant {rep['variant']} "
+            f"({rep['us']} us)")
         # multi-RHS HLL SpMM on the same matrix: k columns, each U[-1,1) (stream = column)
         for k in (8, 32):
             xk = P.psb_d_hip_vect().all(n, k)

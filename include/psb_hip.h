@@ -269,6 +269,12 @@ int psb_hip_hll_get_diag(void *dmat, void *d);
    2 = two adjacent rows per lane (even hksz); + 4 = XCD-contiguous block
    order instead of launch order.  Returns the previous value. */
 int psb_hip_hll_set_variant(int variant);
+/* Autotuner (no counterpart in the reference): times y = A*x with each HLL
+   kernel variant (1, 2, 5, 6 above; reps launches each, HIP events), stores
+   the fastest in the matrix (it then overrides the global variant for this
+   matrix) and returns it in *best; us[4] (may be NULL) receives the average
+   microseconds per product of each candidate. */
+int psb_hip_hll_tune(void *dmat, void *x, void *y, int reps, int *best, float *us);
 
 /* ===================================================================== */
 /* ELL (ELG) matrices -- cuda/elldev.h, cuda/elldev_mod.F90:37-160          */

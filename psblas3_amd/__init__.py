@@ -24,3 +24,4 @@ from .krylov import psb_krylov, psb_krylov_fused, psb_krylov_mixed  # noqa: F401
 from .batch import psb_hip_batch  # noqa: F401
 from .serve import (latency_summary, poisson_arrivals, psb_hip_server,  # noqa: F401
                     serve_one_at_a_time)
+from .tune import psb_hip_hll_autotune  # noqa: F401

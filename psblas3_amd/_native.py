@@ -122,6 +122,7 @@ _SIGS = {
     "psb_hip_hll_make_single": (i32, [vp]),
     "psb_hip_hll_spmv_single": (i32, [vp, f64, vp, f64, vp]),
     "psb_hip_hll_set_variant": (i32, [i32]),
+    "psb_hip_hll_tune": (i32, [vp, vp, vp, i32, pi32, C.POINTER(f32)]),
     "psb_hip_hll_get_diag": (i32, [vp, vp]),
     "psb_hip_ell_alloc": (i32, [pvp, i32, i32, i32, i32, i32]),
     "psb_hip_ell_free": (None, [vp]),

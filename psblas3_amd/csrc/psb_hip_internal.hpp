@@ -281,6 +281,7 @@ struct HllDev {
   int* irn;             // rows
   int* idiag;           // rows
   float* val32;         // optional FP32 copy of a double matrix
+  int variant;          // SpMV kernel variant chosen for this matrix (0: the global one)
 };
 
 struct EllDev {

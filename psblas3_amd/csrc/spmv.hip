@@ -386,8 +386,9 @@ static void hll_launch_hk(const HllDev* m, bool pair, bool xcd, const TV* val, c
 // order (measured 3-5% slower on the 7-point stencils at L = 200 and 256).
 template <typename T, typename TV>
 static int hll_launch(const HllDev* m, const TV* val, const T* x, T alpha, T beta, T* y) {
-  const int variant = g_hll_variant & 3;
-  const bool xcd = (g_hll_variant & 4) != 0;
+  const int v = m->variant != 0 ? m->variant : g_hll_variant;
+  const int variant = v & 3;
+  const bool xcd = (v & 4) != 0;
   bool pair = variant == 2 || (variant == 0 && sizeof(TV) == 8);
   if (m->hksz % 2 != 0) pair = false;
   if (m->hksz == 64) hll_launch_hk<T, TV, 64>(m, pair, xcd, val, x, alpha, beta, y);
@@ -696,6 +697,43 @@ int psb_hip_hll_get_diag(void* dmat, void* dv) {
                        m->hksz, m->hkoffs, m->idiag, (const float*)m->val, (float*)d->v);
   PSB_HIP_LAUNCH_CHECK();
   return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_hll_tune(void* dmat, void* xv, void* yv, int reps, int* best, float* us) {
+  PSB_ENSURE_INIT();
+  HllDev* m = (HllDev*)dmat;
+  if (!m || !best || reps < 1) return PSB_HIP_INVALID;
+  static const int cand[4] = {1, 2, 5, 6};
+  hipEvent_t e0, e1;
+  PSB_HIP_CHECK(hipEventCreate(&e0));
+  PSB_HIP_CHECK(hipEventCreate(&e1));
+  const int saved = m->variant;
+  float bt = 0.0f;
+  int rc = PSB_HIP_SUCCESS;
+  *best = 0;
+  for (int c = 0; c < 4 && rc == PSB_HIP_SUCCESS; c++) {
+    m->variant = cand[c];
+    rc = psb_hip_hll_spmv(dmat, 1.0, xv, 0.0, yv);  // warm-up, argument checks
+    if (rc) break;
+    hipError_t e = hipEventRecord(e0, ctx().stream);
+    for (int r = 0; r < reps && rc == PSB_HIP_SUCCESS; r++)
+      rc = psb_hip_hll_spmv(dmat, 1.0, xv, 0.0, yv);
+    if (e == hipSuccess) e = hipEventRecord(e1, ctx().stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess && rc == PSB_HIP_SUCCESS) rc = set_error(PSB_HIP_RUNTIME, __func__, hipGetErrorString(e));
+    const float t = 1e3f * ms / (float)reps;
+    if (us) us[c] = t;
+    if (rc == PSB_HIP_SUCCESS && (*best == 0 || t < bt)) {
+      *best = cand[c];
+      bt = t;
+    }
+  }
+  m->variant = rc == PSB_HIP_SUCCESS ? *best : saved;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
 }
 
 int psb_hip_hll_set_variant(int variant) {

@@ -207,3 +207,28 @@ def test_spmv_errors(gpu, orc):
     with pytest.raises(P.PsbError):
         P.psb_spmm(1.0, a, xf, 0.0, y)            # precision mismatch
     a.free()
+
+
+def test_hll_autotune(gpu, orc):
+    _say("case hll autotune (hksz 32/64 x 4 variants)")
+    coo = orc.gen_convdiff(30)
+    pc = P.psb_d_coo_sparse_mat(coo.nr, coo.nc, coo.ia, coo.ja, coo.val)
+    m, rep = P.psb_hip_hll_autotune(pc, reps=5)
+    assert rep["hksz"] in (32, 64) and rep["variant"] in (1, 2, 5, 6)
+    assert len(rep["table"]) == 8 and all(t > 0 for t in rep["table"].values())
+    assert m.hksz == rep["hksz"]
+    # the tuned matrix gives the same bits as the default kernel on the same hack size
+    x = P.vect_from(orc.uniform_pm1(2, coo.nc))
+    y1 = P.vect_from(np.zeros(coo.nr))
+    a = P.psb_dspmat_type()
+    a.m = a.n = coo.nr
+    a.a = m
+    P.psb_spmm(1.0, a, x, 0.0, y1)
+    ref = P.psb_dspmat_type()
+    ref.m = ref.n = coo.nr
+    ref.a = P.psb_d_hip_hlg_sparse_mat(rep["hksz"]).cp_from_coo(pc)
+    y2 = P.vect_from(np.zeros(coo.nr))
+    P.psb_spmm(1.0, ref, x, 0.0, y2)
+    assert np.array_equal(y1.get_vect(), y2.get_vect())
+    m.free()
+    ref.a.free()
