@@ -180,6 +180,104 @@ module psb_hip_cbind_mod
       real(c_double) :: err
       integer(c_int) :: res
     end function psb_hip_krylov_fused
+
+    function psb_hip_krylov_mixed(method, dmat, diag, b, x, eps, outer_itmax, inner_eps, &
+         & inner_itmax, check_every, variant, outer_its, inner_its, err) result(res) &
+         & bind(c, name='psb_hip_krylov_mixed')
+      import :: c_int, c_ptr, c_double
+      integer(c_int), value :: method, outer_itmax, inner_itmax, check_every, variant
+      type(c_ptr), value :: dmat, diag, b, x
+      real(c_double), value :: eps, inner_eps
+      integer(c_int) :: outer_its, inner_its
+      real(c_double) :: err
+      integer(c_int) :: res
+    end function psb_hip_krylov_mixed
+
+    ! ---------------------------------------------------------------- batched solves
+    function psb_hip_batch_create(batch, method, max_systems, arena_bytes, check_every, &
+         & delta, inner_eps, use_graph) result(res) bind(c, name='psb_hip_batch_create')
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr) :: batch
+      integer(c_int), value :: method, max_systems, check_every, use_graph
+      integer(c_int64_t), value :: arena_bytes
+      real(c_double), value :: delta, inner_eps
+      integer(c_int) :: res
+    end function psb_hip_batch_create
+
+    function psb_hip_batch_free(batch) result(res) bind(c, name='psb_hip_batch_free')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int) :: res
+    end function psb_hip_batch_free
+
+    function psb_hip_batch_register(batch, dmat, diag, mat_id) result(res) &
+         & bind(c, name='psb_hip_batch_register')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch, dmat, diag
+      integer(c_int) :: mat_id
+      integer(c_int) :: res
+    end function psb_hip_batch_register
+
+    function psb_hip_batch_footprint(batch, mat_id) result(res) &
+         & bind(c, name='psb_hip_batch_footprint')
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: batch
+      integer(c_int), value :: mat_id
+      integer(c_int64_t) :: res
+    end function psb_hip_batch_footprint
+
+    function psb_hip_batch_can_admit(batch, mat_id) result(res) &
+         & bind(c, name='psb_hip_batch_can_admit')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int), value :: mat_id
+      integer(c_int) :: res
+    end function psb_hip_batch_can_admit
+
+    function psb_hip_batch_submit(batch, mat_id, b, x0, eps, itmax, sys_id) result(res) &
+         & bind(c, name='psb_hip_batch_submit')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: batch, b, x0
+      integer(c_int), value :: mat_id, itmax
+      real(c_double), value :: eps
+      integer(c_int) :: sys_id
+      integer(c_int) :: res
+    end function psb_hip_batch_submit
+
+    function psb_hip_batch_step(batch) result(res) bind(c, name='psb_hip_batch_step')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int) :: res
+    end function psb_hip_batch_step
+
+    function psb_hip_batch_poll(batch, wait_last, max_out, ids, iters, outer, relres, status, &
+         & nout, nlive) result(res) bind(c, name='psb_hip_batch_poll')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: batch
+      integer(c_int), value :: wait_last, max_out
+      integer(c_int) :: ids(*), iters(*), outer(*), status(*)
+      real(c_double) :: relres(*)
+      integer(c_int) :: nout, nlive
+      integer(c_int) :: res
+    end function psb_hip_batch_poll
+
+    function psb_hip_batch_result(batch, sys_id, x) result(res) &
+         & bind(c, name='psb_hip_batch_result')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch, x
+      integer(c_int), value :: sys_id
+      integer(c_int) :: res
+    end function psb_hip_batch_result
+
+    function psb_hip_hll_tune(dmat, x, y, reps, best, us) result(res) &
+         & bind(c, name='psb_hip_hll_tune')
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: dmat, x, y
+      integer(c_int), value :: reps
+      integer(c_int) :: best
+      real(c_float) :: us(4)
+      integer(c_int) :: res
+    end function psb_hip_hll_tune
   end interface
 
 end module psb_hip_cbind_mod

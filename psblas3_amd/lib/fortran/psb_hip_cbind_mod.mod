@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:c1da9781114e48aa
+﻿!mod$ v1 sum:583f225ed24b9a28
 !need$ 0bde2ac47243ead2 i iso_c_binding
 module psb_hip_cbind_mod
 use,intrinsic::iso_c_binding,only:c_associated
@@ -314,6 +314,130 @@ integer(4),value::check_every
 integer(4)::iter
 real(8)::err
 type(c_ptr),value::hist
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_krylov_mixed(method,dmat,diag,b,x,eps,outer_itmax,inner_eps,inner_itmax,check_every,variant,outer_its,inner_its,err) bind(c,name="psb_hip_krylov_mixed") result(res)
+import::c_ptr
+integer(4),value::method
+type(c_ptr),value::dmat
+type(c_ptr),value::diag
+type(c_ptr),value::b
+type(c_ptr),value::x
+real(8),value::eps
+integer(4),value::outer_itmax
+real(8),value::inner_eps
+integer(4),value::inner_itmax
+integer(4),value::check_every
+integer(4),value::variant
+integer(4)::outer_its
+integer(4)::inner_its
+real(8)::err
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_batch_create(batch,method,max_systems,arena_bytes,check_every,delta,inner_eps,use_graph) bind(c,name="psb_hip_batch_create") result(res)
+import::c_ptr
+type(c_ptr)::batch
+integer(4),value::method
+integer(4),value::max_systems
+integer(8),value::arena_bytes
+integer(4),value::check_every
+real(8),value::delta
+real(8),value::inner_eps
+integer(4),value::use_graph
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_batch_free(batch) bind(c,name="psb_hip_batch_free") result(res)
+import::c_ptr
+type(c_ptr),value::batch
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_batch_register(batch,dmat,diag,mat_id) bind(c,name="psb_hip_batch_register") result(res)
+import::c_ptr
+type(c_ptr),value::batch
+type(c_ptr),value::dmat
+type(c_ptr),value::diag
+integer(4)::mat_id
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_batch_footprint(batch,mat_id) bind(c,name="psb_hip_batch_footprint") result(res)
+import::c_ptr
+type(c_ptr),value::batch
+integer(4),value::mat_id
+integer(8)::res
+end
+end interface
+interface
+function psb_hip_batch_can_admit(batch,mat_id) bind(c,name="psb_hip_batch_can_admit") result(res)
+import::c_ptr
+type(c_ptr),value::batch
+integer(4),value::mat_id
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_batch_submit(batch,mat_id,b,x0,eps,itmax,sys_id) bind(c,name="psb_hip_batch_submit") result(res)
+import::c_ptr
+type(c_ptr),value::batch
+integer(4),value::mat_id
+type(c_ptr),value::b
+type(c_ptr),value::x0
+real(8),value::eps
+integer(4),value::itmax
+integer(4)::sys_id
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_batch_step(batch) bind(c,name="psb_hip_batch_step") result(res)
+import::c_ptr
+type(c_ptr),value::batch
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_batch_poll(batch,wait_last,max_out,ids,iters,outer,relres,status,nout,nlive) bind(c,name="psb_hip_batch_poll") result(res)
+import::c_ptr
+type(c_ptr),value::batch
+integer(4),value::wait_last
+integer(4),value::max_out
+integer(4)::ids(1_8:*)
+integer(4)::iters(1_8:*)
+integer(4)::outer(1_8:*)
+real(8)::relres(1_8:*)
+integer(4)::status(1_8:*)
+integer(4)::nout
+integer(4)::nlive
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_batch_result(batch,sys_id,x) bind(c,name="psb_hip_batch_result") result(res)
+import::c_ptr
+type(c_ptr),value::batch
+integer(4),value::sys_id
+type(c_ptr),value::x
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_hll_tune(dmat,x,y,reps,best,us) bind(c,name="psb_hip_hll_tune") result(res)
+import::c_ptr
+type(c_ptr),value::dmat
+type(c_ptr),value::x
+type(c_ptr),value::y
+integer(4),value::reps
+integer(4)::best
+real(4)::us(1_8:4_8)
 integer(4)::res
 end
 end interface
