@@ -318,9 +318,17 @@ int psb_hip_csr_read(void *dmat, int *irp, int *ja, void *val);
 /* spmvCSRGDevice: y = alpha*A*x + beta*y */
 int psb_hip_csr_spmv(void *dmat, double alpha, void *x, double beta, void *y);
 int psb_hip_csr_get_diag(void *dmat, void *d);
-/* the lanes-per-row and long-row threshold chosen by the analysis */
+/* the lanes-per-row and long-row threshold chosen by the analysis for the
+   uniform kernel */
 int psb_hip_csr_get_analysis(void *dmat, int *lanes_per_row, int *long_rows,
                              int *long_threshold);
+/* CSR kernel choice (no counterpart in the reference, whose CSR product is
+   cuSPARSE): 0 auto (length-binned when row lengths span more than two
+   lane classes or a row exceeds 512 entries), 1 uniform lanes per row
+   (a power of two >= the mean length), 2 length-binned (rows grouped by
+   lanes per row, about four entries per lane; rows over 512 entries one
+   wavefront each).  Returns the previous value. */
+int psb_hip_csr_set_variant(int variant);
 
 /* ===================================================================== */
 /* Device-resident Krylov solvers (no reference counterpart: the reference */

@@ -301,10 +301,17 @@ struct CsrDev {
   int* irp;             // rows + 1, 1-based
   int* ja;
   void* val;
-  int lanes;            // lanes per row of the vector kernel
+  int lanes;            // lanes per row of the uniform vector kernel
   int long_thr;         // rows longer than this go to the wave-per-row kernel
   int nlong;
   int* long_rows;       // 0-based row ids
+  // length-binned kernel: rows grouped by lanes per row (1, 2, ..., 64), rows
+  // longer than kCsrLongRow listed for the wave-per-row kernel
+  bool irregular;       // the binned kernel is the automatic choice
+  int bin_start[8];     // class c: bin_rows[bin_start[c] .. bin_start[c+1])
+  int* bin_rows;        // 0-based row ids, class after class, ascending
+  int nlong_b;
+  int* long_rows_b;
 };
 
 // ------------------------------------------------------------------ inner products

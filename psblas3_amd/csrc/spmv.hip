@@ -27,6 +27,20 @@
 namespace psbhip {
 
 static int g_hll_variant = 0;
+static int g_csr_variant = 0;    // 0 auto, 1 uniform lanes per row, 2 length-binned
+
+constexpr int kCsrEntriesPerLane = 4;
+constexpr int kCsrLongRow = 512;   // binned: longer rows take one wavefront each
+
+// lanes per row of the binned kernel: a power of two near len / 4, 1..64
+inline int csr_class(int len) {
+  int g = 1, c = 0;
+  while (g < 64 && g * kCsrEntriesPerLane < len) {
+    g <<= 1;
+    c++;
+  }
+  return c;
+}
 
 // ------------------------------------------------------------------ HLL SpMV
 // HK > 0: compile-time hack size (32 or 64); HK == 0: any hksz.  XCD: blocks
@@ -185,6 +199,37 @@ __global__ __launch_bounds__(kBlock) void k_csr_vector(int nr, int nlog,
   for (; p < e; p += G) acc = fma(ld_stream(val + p), x[ld_stream(ja + p) - 1], acc);
   if (G > 1) acc = group_sum<G>(acc);
   if (gid < nr && lane == 0 && !skip) store_y(y, gid, alpha, acc, beta);
+}
+
+// Length-binned CSR: the rows of one class (G lanes per row, about four
+// entries per lane) come from a list; same inner loop as k_csr_vector.
+template <typename T, int G>
+__global__ __launch_bounds__(kBlock) void k_csr_list(int count, const int* __restrict__ rows,
+                                                     const int* __restrict__ irp,
+                                                     const int* __restrict__ ja,
+                                                     const T* __restrict__ val,
+                                                     const T* __restrict__ x, T alpha, T beta,
+                                                     T* __restrict__ y) {
+  const int gid = (int)blockIdx.x * (kBlock / G) + (int)threadIdx.x / G;
+  const int lane = threadIdx.x % G;
+  const int row = rows[gid < count ? gid : count - 1];
+  const int b = irp[row] - 1, e = irp[row + 1] - 1;
+  T acc = T(0);
+  int p = b + lane;
+  for (; p + 3 * G < e; p += 4 * G) {
+    int c0 = ld_stream(ja + p), c1 = ld_stream(ja + p + G);
+    int c2 = ld_stream(ja + p + 2 * G), c3 = ld_stream(ja + p + 3 * G);
+    T v0 = ld_stream(val + p), v1 = ld_stream(val + p + G);
+    T v2 = ld_stream(val + p + 2 * G), v3 = ld_stream(val + p + 3 * G);
+    T x0 = x[c0 - 1], x1 = x[c1 - 1], x2 = x[c2 - 1], x3 = x[c3 - 1];
+    acc = fma(v0, x0, acc);
+    acc = fma(v1, x1, acc);
+    acc = fma(v2, x2, acc);
+    acc = fma(v3, x3, acc);
+  }
+  for (; p < e; p += G) acc = fma(ld_stream(val + p), x[ld_stream(ja + p) - 1], acc);
+  if (G > 1) acc = group_sum<G>(acc);
+  if (gid < count && lane == 0) store_y(y, row, alpha, acc, beta);
 }
 
 template <typename T>
@@ -424,7 +469,45 @@ static int hll_spmm_launch(const HllDev* m, const T* val, const T* x, int ldx, T
 }
 
 template <typename T>
+static int csr_launch_binned(CsrDev* m, const T* x, T alpha, T beta, T* y) {
+  hipStream_t s = ctx().stream;
+  const T* val = (const T*)m->val;
+  for (int c = 0; c < 7; c++) {
+    const int count = m->bin_start[c + 1] - m->bin_start[c];
+    if (count == 0) continue;
+    const int* rows = m->bin_rows + m->bin_start[c];
+    const int G = 1 << c;
+    dim3 grid(grid_for((int64_t)count * G, kBlock));
+#define PSB_CSRB_CASE(GG)                                                                     \
+  case GG:                                                                                    \
+    hipLaunchKernelGGL((k_csr_list<T, GG>), grid, dim3(kBlock), 0, s, count, rows, m->irp,   \
+                       m->ja, val, x, alpha, beta, y);                                        \
+    break;
+    switch (G) {
+      PSB_CSRB_CASE(1)
+      PSB_CSRB_CASE(2)
+      PSB_CSRB_CASE(4)
+      PSB_CSRB_CASE(8)
+      PSB_CSRB_CASE(16)
+      PSB_CSRB_CASE(32)
+      PSB_CSRB_CASE(64)
+    }
+#undef PSB_CSRB_CASE
+    PSB_HIP_LAUNCH_CHECK();
+  }
+  if (m->nlong_b > 0) {
+    int nb = grid_for(m->nlong_b, kBlock / kWave);
+    hipLaunchKernelGGL((k_csr_long<T>), dim3(nb), dim3(kBlock), 0, s, m->nlong_b,
+                       m->long_rows_b, m->irp, m->ja, val, x, alpha, beta, y);
+    PSB_HIP_LAUNCH_CHECK();
+  }
+  return PSB_HIP_SUCCESS;
+}
+
+template <typename T>
 static int csr_launch(CsrDev* m, const T* x, T alpha, T beta, T* y) {
+  const bool binned = g_csr_variant == 2 || (g_csr_variant == 0 && m->irregular);
+  if (binned && m->bin_rows) return csr_launch_binned<T>(m, x, alpha, beta, y);
   hipStream_t s = ctx().stream;
   const T* val = (const T*)m->val;
   int g = m->lanes;
@@ -988,6 +1071,8 @@ void psb_hip_csr_free(void* dmat) {
   hipFree(m->ja);
   hipFree(m->val);
   hipFree(m->long_rows);
+  hipFree(m->bin_rows);
+  hipFree(m->long_rows_b);
   delete m;
 }
 
@@ -1012,12 +1097,54 @@ int psb_hip_csr_write(void* dmat, const int* irp, const int* ja, const void* val
   for (int i = 0; i < m->rows; i++)
     if (irp[i + 1] - irp[i] > m->long_thr) longs.push_back(i);
   m->nlong = (int)longs.size();
+  // length-binned analysis
+  int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  std::vector<int> longs_b;
+  int cmin = 7, cmax = -1;
+  for (int i = 0; i < m->rows; i++) {
+    const int len = irp[i + 1] - irp[i];
+    if (len > kCsrLongRow) {
+      longs_b.push_back(i);
+      continue;
+    }
+    const int c = csr_class(len);
+    cnt[c]++;
+    cmin = std::min(cmin, c);
+    cmax = std::max(cmax, c);
+  }
+  m->bin_start[0] = 0;
+  for (int c = 0; c < 7; c++) m->bin_start[c + 1] = m->bin_start[c] + cnt[c];
+  std::vector<int> bins((size_t)(m->bin_start[7] > 0 ? m->bin_start[7] : 1));
+  {
+    int fill[7];
+    for (int c = 0; c < 7; c++) fill[c] = m->bin_start[c];
+    for (int i = 0; i < m->rows; i++) {
+      const int len = irp[i + 1] - irp[i];
+      if (len <= kCsrLongRow) bins[(size_t)fill[csr_class(len)]++] = i;
+    }
+  }
+  m->nlong_b = (int)longs_b.size();
+  m->irregular = m->nlong_b > 0 || (cmax >= 0 && cmax - cmin >= 2);
   hipStream_t s = ctx().stream;
   hipFree(m->long_rows);
   m->long_rows = nullptr;
+  hipFree(m->bin_rows);
+  m->bin_rows = nullptr;
+  hipFree(m->long_rows_b);
+  m->long_rows_b = nullptr;
   if (m->nlong > 0) {
     PSB_HIP_CHECK(hipMalloc(&m->long_rows, sizeof(int) * m->nlong));
     PSB_HIP_CHECK(hipMemcpyAsync(m->long_rows, longs.data(), sizeof(int) * m->nlong,
+                                 hipMemcpyHostToDevice, s));
+  }
+  if (m->rows > 0) {
+    PSB_HIP_CHECK(hipMalloc(&m->bin_rows, sizeof(int) * bins.size()));
+    PSB_HIP_CHECK(hipMemcpyAsync(m->bin_rows, bins.data(), sizeof(int) * bins.size(),
+                                 hipMemcpyHostToDevice, s));
+  }
+  if (m->nlong_b > 0) {
+    PSB_HIP_CHECK(hipMalloc(&m->long_rows_b, sizeof(int) * m->nlong_b));
+    PSB_HIP_CHECK(hipMemcpyAsync(m->long_rows_b, longs_b.data(), sizeof(int) * m->nlong_b,
                                  hipMemcpyHostToDevice, s));
   }
   size_t es = elem_size(m->elem_type);
@@ -1056,6 +1183,12 @@ int psb_hip_csr_get_diag(void* dmat, void* dv) {
                        m->ja, (const float*)m->val, (float*)d->v);
   PSB_HIP_LAUNCH_CHECK();
   return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_csr_set_variant(int variant) {
+  const int old = g_csr_variant;
+  if (variant >= 0 && variant <= 2) g_csr_variant = variant;
+  return old;
 }
 
 int psb_hip_csr_get_analysis(void* dmat, int* lanes_per_row, int* long_rows, int* long_threshold) {

@@ -232,3 +232,37 @@ def test_hll_autotune(gpu, orc):
     assert np.array_equal(y1.get_vect(), y2.get_vect())
     m.free()
     ref.a.free()
+
+
+@pytest.mark.parametrize("kind", ["powerlaw", "stencil", "mixed_long"])
+def test_csr_variants(gpu, orc, kind):
+    _say(f"case csr uniform vs length-binned kernels, {kind}")
+    from psblas3_amd import _native as N
+    if kind == "powerlaw":
+        coo = orc.gen_powerlaw(20000, cap=4095, seed=5)
+    elif kind == "stencil":
+        coo = orc.gen_convdiff(21)
+    else:   # a few rows longer than the binned wave-per-row threshold, empty rows
+        n = 3000
+        rng = np.random.default_rng(3)
+        rows, cols = [], []
+        for r in range(n):
+            k = 0 if r % 97 == 0 else (900 if r % 500 == 1 else 1 + (r % 40))
+            rows += [r + 1] * k
+            cols += list(rng.choice(n, size=k, replace=False) + 1)
+        coo = orc.COO(n, n, np.array(rows, np.int32), np.array(cols, np.int32),
+                      rng.uniform(-1, 1, len(rows)))
+    f = orc.fix_coo(coo)
+    irp, ja, val = orc.csr_from_coo(f)
+    a = _mat(coo, P.psb_d_hip_csrg_sparse_mat())
+    x = orc.uniform_pm1(8, coo.nc)
+    y0 = orc.uniform_pm1(9, coo.nr)
+    ref = orc.csr_spmv(irp, ja, val, x, 1.5, -0.5, y0)
+    bound = _row_bound(f, x, 1.5, -0.5, y0, np.finfo(float).eps)
+    for var in (1, 2, 0):
+        old = N.fn("psb_hip_csr_set_variant")(var)
+        xv, yv = P.vect_from(x), P.vect_from(y0.copy())
+        P.psb_spmm(1.5, a, xv, -0.5, yv)
+        N.fn("psb_hip_csr_set_variant")(old)
+        assert np.all(np.abs(yv.get_vect() - ref) <= bound), var
+    a.free()
