@@ -453,7 +453,7 @@ def main():
                                           "GFLOPs": round(2 * nnz / (t * 1e-3) / 1e9, 2)}
             log(f"bench: {mat} {fmt} {dt} {t * 1e3:.1f} us  {sweep[f'{mat}_{fmt}_{dt}']['GBs']} GB/s")
             m.free()
-        # CSR length-binned kernel on the stencil (auto keeps the uniform one here)
+        # CSR length-binned kernel on the stencil
         m = P.psb_dspmat_type()
         m.m = m.n = n
         m.a = P.psb_d_hip_csrg_sparse_mat().cp_from_coo(coo_a)
@@ -515,7 +515,7 @@ def main():
             sweep[f"B_{fmt}_{dt}"] = ent
             log(f"bench: B {fmt} {dt} {t * 1e3:.1f} us  {ent['GBs']} GB/s")
             m.free()
-        # the two CSR kernels on matrix B (auto picks the length-binned one here)
+        # the two CSR kernels on matrix B
         m = P.psb_dspmat_type()
         m.m = m.n = nb
         m.a = P.psb_d_hip_csrg_sparse_mat().cp_from_coo(coo_b)

@@ -323,8 +323,8 @@ int psb_hip_csr_get_diag(void *dmat, void *d);
 int psb_hip_csr_get_analysis(void *dmat, int *lanes_per_row, int *long_rows,
                              int *long_threshold);
 /* CSR kernel choice (no counterpart in the reference, whose CSR product is
-   cuSPARSE): 0 auto (length-binned when row lengths span more than two
-   lane classes or a row exceeds 512 entries), 1 uniform lanes per row
+   cuSPARSE): 0 auto (the uniform kernel, faster on every matrix measured),
+   1 uniform lanes per row
    (a power of two >= the mean length), 2 length-binned (rows grouped by
    lanes per row, about four entries per lane; rows over 512 entries one
    wavefront each).  Returns the previous value. */

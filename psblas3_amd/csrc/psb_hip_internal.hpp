@@ -307,7 +307,6 @@ struct CsrDev {
   int* long_rows;       // 0-based row ids
   // length-binned kernel: rows grouped by lanes per row (1, 2, ..., 64), rows
   // longer than kCsrLongRow listed for the wave-per-row kernel
-  bool irregular;       // the binned kernel is the automatic choice
   int bin_start[8];     // class c: bin_rows[bin_start[c] .. bin_start[c+1])
   int* bin_rows;        // 0-based row ids, class after class, ascending
   int nlong_b;

@@ -27,7 +27,10 @@
 namespace psbhip {
 
 static int g_hll_variant = 0;
-static int g_csr_variant = 0;    // 0 auto, 1 uniform lanes per row, 2 length-binned
+// 0 auto (= uniform: the length-binned kernel measured slower on both the
+// L=200 stencil, 514 vs 280 us, and the n=4M power-law matrix, 1355 vs
+// 1069 us), 1 uniform lanes per row, 2 length-binned
+static int g_csr_variant = 0;
 
 constexpr int kCsrEntriesPerLane = 4;
 constexpr int kCsrLongRow = 512;   // binned: longer rows take one wavefront each
@@ -506,7 +509,7 @@ static int csr_launch_binned(CsrDev* m, const T* x, T alpha, T beta, T* y) {
 
 template <typename T>
 static int csr_launch(CsrDev* m, const T* x, T alpha, T beta, T* y) {
-  const bool binned = g_csr_variant == 2 || (g_csr_variant == 0 && m->irregular);
+  const bool binned = g_csr_variant == 2;
   if (binned && m->bin_rows) return csr_launch_binned<T>(m, x, alpha, beta, y);
   hipStream_t s = ctx().stream;
   const T* val = (const T*)m->val;
@@ -1100,17 +1103,13 @@ int psb_hip_csr_write(void* dmat, const int* irp, const int* ja, const void* val
   // length-binned analysis
   int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   std::vector<int> longs_b;
-  int cmin = 7, cmax = -1;
   for (int i = 0; i < m->rows; i++) {
     const int len = irp[i + 1] - irp[i];
     if (len > kCsrLongRow) {
       longs_b.push_back(i);
       continue;
     }
-    const int c = csr_class(len);
-    cnt[c]++;
-    cmin = std::min(cmin, c);
-    cmax = std::max(cmax, c);
+    cnt[csr_class(len)]++;
   }
   m->bin_start[0] = 0;
   for (int c = 0; c < 7; c++) m->bin_start[c + 1] = m->bin_start[c] + cnt[c];
@@ -1124,7 +1123,6 @@ int psb_hip_csr_write(void* dmat, const int* irp, const int* ja, const void* val
     }
   }
   m->nlong_b = (int)longs_b.size();
-  m->irregular = m->nlong_b > 0 || (cmax >= 0 && cmax - cmin >= 2);
   hipStream_t s = ctx().stream;
   hipFree(m->long_rows);
   m->long_rows = nullptr;

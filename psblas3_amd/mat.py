@@ -158,6 +158,7 @@ class psb_d_hip_hlg_sparse_mat(_hip_base_mat):
         """psb_d_cuda_hlg_vect_mv: y = alpha*A*x + beta*y on the device."""
         self._check_vects(x, y)
         N.call("psb_hip_hll_spmv", self.dev, float(alpha), x.dev, float(beta), y.dev)
+        y.set_dev()
 
     def tune(self, x, y, reps=20):
         """Time the HLL kernel variants on this matrix with vectors x, y and keep
@@ -166,8 +167,8 @@ class psb_d_hip_hlg_sparse_mat(_hip_base_mat):
         best = C.c_int(0)
         us = (C.c_float * 4)()
         N.call("psb_hip_hll_tune", self.dev, x.dev, y.dev, int(reps), C.byref(best), us)
-        return best.value, {v: float(t) for v, t in zip((1, 2, 5, 6), us)}
         y.set_dev()
+        return best.value, {v: float(t) for v, t in zip((1, 2, 5, 6), us)}
 
     def sizeof(self):
         es = np.dtype(self._dtype).itemsize
