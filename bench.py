@@ -269,7 +269,8 @@ def batch_bench(P, N, nsys=256, eps=1e-8, seed=3):
     return out
 
 
-def serve_bench(P, N, nmat=64, seed=4, rates=(200.0, 1000.0, 5000.0), count=160, eps=1e-10):
+def serve_bench(P, N, nmat=64, seed=4, rates=(200.0, 1000.0, 5000.0), count=160, eps=1e-10,
+                arena_mb=768):
     """Serving loop: nmat registered 3-D 7-point convection-diffusion matrices
     (L uniform in [24, 160], convection magnitude uniform in [0, 10]: host
     uniform streams 0 and 1 of the seed; HLG hksz 64) with Jacobi
@@ -309,9 +310,12 @@ def serve_bench(P, N, nmat=64, seed=4, rates=(200.0, 1000.0, 5000.0), count=160,
         f"{time.perf_counter() - t0:.1f}s")
     out = {"matrices": nmat, "rows_resident": rows, "L_min": int(Ls.min()), "L_max": int(Ls.max()),
            "eps": eps, "requests_per_rate": count, "method": "BiCGSTAB + Jacobi",
+           "arena_MiB": arena_mb,
            "rhs": "U[-1,1) generated on the device per request"}
-    srv = P.psb_hip_server("BICGSTAB", max_systems=256, arena_bytes=6 << 30, check_every=4,
-                           eps=eps)
+    # arena: ~13M rows of BiCGSTAB work vectors (60 bytes per row) in flight, enough to keep
+    # the device busy; requests beyond it wait in the queue (admission by footprint)
+    srv = P.psb_hip_server("BICGSTAB", max_systems=256, arena_bytes=arena_mb << 20,
+                           check_every=4, eps=eps)
     for m, prec, _ in systems:
         srv.register(m, prec)
     srv.run([(0.0, k, 999000 + k) for k in range(4)])      # warm: graph capture, first launches
@@ -384,6 +388,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=160)
+    ap.add_argument("--serve-arena-mb", type=int, default=768)
     ap.add_argument("--batch-systems", type=int, default=256)
     ap.add_argument("--hll-variant", type=int, default=0,
                     help="HLL kernel variant (psb_hip_hll_set_variant): 0 auto, 1 one row per "
@@ -533,7 +538,8 @@ def main():
 
     solve = None if args.no_solve else solve_bench(P, N, args.solve_L)
     batch = None if args.no_batch else batch_bench(P, N, args.batch_systems)
-    serve = None if args.no_serve else serve_bench(P, N, count=args.serve_requests)
+    serve = None if args.no_serve else serve_bench(P, N, count=args.serve_requests,
+                                                   arena_mb=args.serve_arena_mb)
     cpu = None if args.no_cpu else cpu_baseline()
     if cpu:
         log(f"bench: cpu baseline {cpu['value']} GB/s on {cpu['cores']} threads")

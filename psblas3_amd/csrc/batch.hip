@@ -3,7 +3,7 @@
 //
 // Each live system owns a slot: a descriptor (its HLL matrix with the FP32
 // value copy, its FP32 Jacobi diagonal, its vectors in the engine's arena)
-// and a scalar state.  The rows of all live systems are cut into 256-row
+// and a scalar state.  The rows of all live systems are cut into 1024-row
 // tiles; the tile table lists (slot, first row) system after system.  Every
 // streaming kernel walks the tile table grid-stride and skips the tiles of
 // systems that are not in the phase it serves; every reduction stores one
@@ -40,7 +40,8 @@
 
 namespace psbhip {
 
-constexpr int kBT = kBlock;      // rows per tile = threads per block
+constexpr int kBT = kBlock;      // threads per block
+constexpr int kTileRows = 4 * kBT;  // rows per tile
 constexpr int kBGrid = 2048;     // blocks of the tile kernels
 
 enum BPhase { BP_RUN = 0, BP_BREAK = 1, BP_TRANS = 2, BP_INIT = 3 };
@@ -104,8 +105,11 @@ __device__ __forceinline__ T b_row(const BSys& sy, const T* a, int r, const T* _
   return slot_dot<T, T>(sy.ja + off0 + lane, a + off0 + lane, sy.hksz, w, x);
 }
 
-// The body runs for every tile of a system satisfying COND (st, sy, i = row,
-// in = i < n, t_ = tile index in scope).  COND is uniform over the block.
+// The body runs for every tile of a system satisfying COND (st, sy, r0 =
+// the tile's first row, t_ = tile index in scope).  COND is uniform over the
+// block.  A tile is kTileRows rows: elementwise kernels give each thread 4
+// consecutive rows (16-byte accesses), products make 4 passes of 256 rows
+// (one row per lane).
 #define PSB_BATCH_TILES(TB, COND, ...)                                   \
   {                                                                      \
     const int nt_ = *(TB).ntiles;                                        \
@@ -114,8 +118,7 @@ __device__ __forceinline__ T b_row(const BSys& sy, const T* a, int r, const T* _
       BState* st = (TB).st + tl_.x;                                      \
       if (!(COND)) continue;                                             \
       const BSys& sy = (TB).sys[tl_.x];                                  \
-      const int i = tl_.y + (int)threadIdx.x;                            \
-      const bool in = i < sy.n;                                          \
+      const int r0 = tl_.y;                                              \
       __VA_ARGS__                                                        \
     }                                                                    \
   }
@@ -130,23 +133,86 @@ __device__ __forceinline__ void tile_partials(double a, double b, double* partia
   }
 }
 
+// 4 consecutive entries p[0..3] (16-byte aligned); nv < 4: the first nv only
+__device__ __forceinline__ void ld4(const float* p, int nv, float (&a)[4]) {
+  if (nv == 4) {
+    const v16<float> t = ldv(p);
+    a[0] = t[0]; a[1] = t[1]; a[2] = t[2]; a[3] = t[3];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; k++) a[k] = k < nv ? p[k] : 0.0f;
+  }
+}
+__device__ __forceinline__ void st4(float* p, int nv, const float (&a)[4]) {
+  if (nv == 4) {
+    v16<float> t;
+    t[0] = a[0]; t[1] = a[1]; t[2] = a[2]; t[3] = a[3];
+    stv(p, t);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (k < nv) p[k] = a[k];
+  }
+}
+__device__ __forceinline__ void ld4(const double* p, int nv, double (&a)[4]) {
+  if (nv == 4) {
+    const v16<double> t0 = ldv(p), t1 = ldv(p + 2);
+    a[0] = t0[0]; a[1] = t0[1]; a[2] = t1[0]; a[3] = t1[1];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; k++) a[k] = k < nv ? p[k] : 0.0;
+  }
+}
+__device__ __forceinline__ void st4(double* p, int nv, const double (&a)[4]) {
+  if (nv == 4) {
+    v16<double> t0, t1;
+    t0[0] = a[0]; t0[1] = a[1]; t1[0] = a[2]; t1[1] = a[3];
+    stv(p, t0);
+    stv(p + 2, t1);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (k < nv) p[k] = a[k];
+  }
+}
+
+// first row and count (0..4) of this thread's 4 rows in an elementwise tile
+#define PSB_QUAD(I, NV)                                                  \
+  const int I = r0 + 4 * (int)threadIdx.x;                               \
+  const int NV = sy.n - I < 4 ? (sy.n - I > 0 ? sy.n - I : 0) : 4;
+
 // ------------------------------------------------------------------ CG iteration
 __global__ __launch_bounds__(kBT) void kb_cg_p(BTables tb) {
   PSB_BATCH_TILES(tb, b_iterating(st), {
-    if (in) {
-      const float zi = sy.z[i];
-      sy.p[i] = st->it == 0 ? zi : zi + (float)st->beta * sy.p[i];
+    PSB_QUAD(i, nv)
+    if (nv > 0) {
+      float z[4], p[4];
+      ld4(sy.z + i, nv, z);
+      if (st->it == 0) {
+        st4(sy.p + i, nv, z);
+      } else {
+        ld4(sy.p + i, nv, p);
+        const float beta = (float)st->beta;
+_Pragma("unroll")
+        for (int k = 0; k < 4; k++) p[k] = z[k] + beta * p[k];
+        st4(sy.p + i, nv, p);
+      }
     }
   })
 }
 
 __global__ __launch_bounds__(kBT) void kb_cg_q(BTables tb) {
   PSB_BATCH_TILES(tb, b_iterating(st), {
-    const float ax = b_row<float>(sy, sy.a32, in ? i : sy.n - 1, sy.p);
     double pq = 0.0;
-    if (in) {
-      sy.q[i] = ax;
-      pq = (double)sy.p[i] * (double)ax;
+    for (int s = 0; s < kTileRows / kBT; s++) {
+      const int i = r0 + s * kBT + (int)threadIdx.x;
+      if (s * kBT >= sy.n - r0) break;
+      const bool in = i < sy.n;
+      const float ax = b_row<float>(sy, sy.a32, in ? i : sy.n - 1, sy.p);
+      if (in) {
+        sy.q[i] = ax;
+        pq += (double)sy.p[i] * (double)ax;
+      }
     }
     tile_partials(pq, 0.0, tb.partials, t_);
   })
@@ -154,16 +220,27 @@ __global__ __launch_bounds__(kBT) void kb_cg_q(BTables tb) {
 
 __global__ __launch_bounds__(kBT) void kb_cg_x(BTables tb) {
   PSB_BATCH_TILES(tb, b_iterating(st), {
+    PSB_QUAD(i, nv)
     double rr = 0.0, rz = 0.0;
-    if (in) {
+    if (nv > 0) {
       const float alpha = (float)st->alpha;
-      sy.x32[i] = sy.x32[i] + alpha * sy.p[i];
-      const float ri = sy.r[i] - alpha * sy.q[i];
-      sy.r[i] = ri;
-      const float zi = sy.d32 ? sy.d32[i] * ri : ri;
-      sy.z[i] = zi;
-      rr = (double)ri * (double)ri;
-      rz = (double)ri * (double)zi;
+      float x[4], p[4], r[4], q[4], z[4];
+      ld4(sy.x32 + i, nv, x);
+      ld4(sy.p + i, nv, p);
+      ld4(sy.r + i, nv, r);
+      ld4(sy.q + i, nv, q);
+      if (sy.d32) ld4(sy.d32 + i, nv, z);
+_Pragma("unroll")
+      for (int k = 0; k < 4; k++) {
+        x[k] = x[k] + alpha * p[k];
+        r[k] = r[k] - alpha * q[k];
+        z[k] = sy.d32 ? z[k] * r[k] : r[k];
+        rr += (double)r[k] * (double)r[k];
+        rz += (double)r[k] * (double)z[k];
+      }
+      st4(sy.x32 + i, nv, x);
+      st4(sy.r + i, nv, r);
+      st4(sy.z + i, nv, z);
     }
     tile_partials(rr, rz, tb.partials, t_);
   })
@@ -172,23 +249,45 @@ __global__ __launch_bounds__(kBT) void kb_cg_x(BTables tb) {
 // ------------------------------------------------------------------ BiCGSTAB iteration
 __global__ __launch_bounds__(kBT) void kb_bi_a(BTables tb) {
   PSB_BATCH_TILES(tb, b_iterating(st), {
-    if (in) {
-      float pi = sy.r[i];
-      if (st->it != 0)
-        pi = pi + (float)st->beta * ((float)(-st->omega) * sy.v[i] + sy.p[i]);
-      sy.p[i] = pi;
-      sy.f[i] = sy.d32 ? sy.d32[i] * pi : pi;
+    PSB_QUAD(i, nv)
+    if (nv > 0) {
+      float r[4], p[4], v[4], f[4];
+      ld4(sy.r + i, nv, r);
+      if (st->it != 0) {
+        ld4(sy.p + i, nv, p);
+        ld4(sy.v + i, nv, v);
+        const float beta = (float)st->beta, momega = (float)(-st->omega);
+_Pragma("unroll")
+        for (int k = 0; k < 4; k++) p[k] = r[k] + beta * (momega * v[k] + p[k]);
+      } else {
+_Pragma("unroll")
+        for (int k = 0; k < 4; k++) p[k] = r[k];
+      }
+      st4(sy.p + i, nv, p);
+      if (sy.d32) {
+        ld4(sy.d32 + i, nv, f);
+_Pragma("unroll")
+        for (int k = 0; k < 4; k++) f[k] = f[k] * p[k];
+        st4(sy.f + i, nv, f);
+      } else {
+        st4(sy.f + i, nv, p);
+      }
     }
   })
 }
 
 __global__ __launch_bounds__(kBT) void kb_bi_v(BTables tb) {
   PSB_BATCH_TILES(tb, b_iterating(st), {
-    const float ax = b_row<float>(sy, sy.a32, in ? i : sy.n - 1, sy.f);
     double qv = 0.0;
-    if (in) {
-      sy.v[i] = ax;
-      qv = (double)sy.q[i] * (double)ax;
+    for (int s = 0; s < kTileRows / kBT; s++) {
+      const int i = r0 + s * kBT + (int)threadIdx.x;
+      if (s * kBT >= sy.n - r0) break;
+      const bool in = i < sy.n;
+      const float ax = b_row<float>(sy, sy.a32, in ? i : sy.n - 1, sy.f);
+      if (in) {
+        sy.v[i] = ax;
+        qv += (double)sy.q[i] * (double)ax;
+      }
     }
     tile_partials(qv, 0.0, tb.partials, t_);
   })
@@ -196,22 +295,40 @@ __global__ __launch_bounds__(kBT) void kb_bi_v(BTables tb) {
 
 __global__ __launch_bounds__(kBT) void kb_bi_s(BTables tb) {
   PSB_BATCH_TILES(tb, b_iterating(st), {
-    if (in) {
-      const float si = (float)(-st->alpha) * sy.v[i] + sy.r[i];
-      sy.s[i] = si;
-      sy.z[i] = sy.d32 ? sy.d32[i] * si : si;
+    PSB_QUAD(i, nv)
+    if (nv > 0) {
+      float v[4], r[4], z[4];
+      ld4(sy.v + i, nv, v);
+      ld4(sy.r + i, nv, r);
+      const float malpha = (float)(-st->alpha);
+_Pragma("unroll")
+      for (int k = 0; k < 4; k++) r[k] = malpha * v[k] + r[k];   // s
+      st4(sy.s + i, nv, r);
+      if (sy.d32) {
+        ld4(sy.d32 + i, nv, z);
+_Pragma("unroll")
+        for (int k = 0; k < 4; k++) z[k] = z[k] * r[k];
+        st4(sy.z + i, nv, z);
+      } else {
+        st4(sy.z + i, nv, r);
+      }
     }
   })
 }
 
 __global__ __launch_bounds__(kBT) void kb_bi_t(BTables tb) {
   PSB_BATCH_TILES(tb, b_iterating(st), {
-    const float ax = b_row<float>(sy, sy.a32, in ? i : sy.n - 1, sy.z);
     double tt = 0.0, ts = 0.0;
-    if (in) {
-      sy.t[i] = ax;
-      tt = (double)ax * (double)ax;
-      ts = (double)ax * (double)sy.s[i];
+    for (int s = 0; s < kTileRows / kBT; s++) {
+      const int i = r0 + s * kBT + (int)threadIdx.x;
+      if (s * kBT >= sy.n - r0) break;
+      const bool in = i < sy.n;
+      const float ax = b_row<float>(sy, sy.a32, in ? i : sy.n - 1, sy.z);
+      if (in) {
+        sy.t[i] = ax;
+        tt += (double)ax * (double)ax;
+        ts += (double)ax * (double)sy.s[i];
+      }
     }
     tile_partials(tt, ts, tb.partials, t_);
   })
@@ -219,14 +336,26 @@ __global__ __launch_bounds__(kBT) void kb_bi_t(BTables tb) {
 
 __global__ __launch_bounds__(kBT) void kb_bi_x(BTables tb) {
   PSB_BATCH_TILES(tb, b_iterating(st), {
+    PSB_QUAD(i, nv)
     double rr = 0.0, qr = 0.0;
-    if (in) {
+    if (nv > 0) {
       const float alpha = (float)st->alpha, omega = (float)st->omega;
-      sy.x32[i] = omega * sy.z[i] + (alpha * sy.f[i] + sy.x32[i]);
-      const float ri = -omega * sy.t[i] + sy.s[i];
-      sy.r[i] = ri;
-      rr = (double)ri * (double)ri;
-      qr = (double)sy.q[i] * (double)ri;
+      float x[4], z[4], f[4], t[4], s4[4], q[4];
+      ld4(sy.x32 + i, nv, x);
+      ld4(sy.z + i, nv, z);
+      ld4(sy.f + i, nv, f);
+      ld4(sy.t + i, nv, t);
+      ld4(sy.s + i, nv, s4);
+      ld4(sy.q + i, nv, q);
+_Pragma("unroll")
+      for (int k = 0; k < 4; k++) {
+        x[k] = omega * z[k] + (alpha * f[k] + x[k]);
+        s4[k] = -omega * t[k] + s4[k];   // r
+        rr += (double)s4[k] * (double)s4[k];
+        qr += (double)q[k] * (double)s4[k];
+      }
+      st4(sy.x32 + i, nv, x);
+      st4(sy.r + i, nv, s4);
     }
     tile_partials(rr, qr, tb.partials, t_);
   })
@@ -237,12 +366,21 @@ __global__ __launch_bounds__(kBT) void kb_bi_x(BTables tb) {
 // x32 = 0.  A newly admitted system only clears x32.
 __global__ __launch_bounds__(kBT) void kb_accum(BTables tb, int cg) {
   PSB_BATCH_TILES(tb, b_transiting(st), {
-    if (in) {
+    PSB_QUAD(i, nv)
+    if (nv > 0) {
+      float x32[4];
       if (st->phase != BP_INIT) {
         const double scale = cg ? 1.0 : st->ref;
-        sy.x[i] = sy.x[i] + scale * (double)sy.x32[i];
+        double x[4];
+        ld4(sy.x + i, nv, x);
+        ld4(sy.x32 + i, nv, x32);
+_Pragma("unroll")
+        for (int k = 0; k < 4; k++) x[k] = x[k] + scale * (double)x32[k];
+        st4(sy.x + i, nv, x);
       }
-      sy.x32[i] = 0.0f;
+_Pragma("unroll")
+      for (int k = 0; k < 4; k++) x32[k] = 0.0f;
+      st4(sy.x32 + i, nv, x32);
     }
   })
 }
@@ -250,14 +388,19 @@ __global__ __launch_bounds__(kBT) void kb_accum(BTables tb, int cg) {
 // r64 = b - A64 x64 with partials r64.r64 and b.b (FP64)
 __global__ __launch_bounds__(kBT) void kb_res(BTables tb) {
   PSB_BATCH_TILES(tb, b_transiting(st), {
-    const double ax = b_row<double>(sy, sy.a64, in ? i : sy.n - 1, sy.x);
     double rr = 0.0, bb = 0.0;
-    if (in) {
-      const double bi = sy.b[i];
-      const double ri = bi - ax;
-      sy.r64[i] = ri;
-      rr = ri * ri;
-      bb = bi * bi;
+    for (int s = 0; s < kTileRows / kBT; s++) {
+      const int i = r0 + s * kBT + (int)threadIdx.x;
+      if (s * kBT >= sy.n - r0) break;
+      const bool in = i < sy.n;
+      const double ax = b_row<double>(sy, sy.a64, in ? i : sy.n - 1, sy.x);
+      if (in) {
+        const double bi = sy.b[i];
+        const double ri = bi - ax;
+        sy.r64[i] = ri;
+        rr += ri * ri;
+        bb += bi * bi;
+      }
     }
     tile_partials(rr, bb, tb.partials, t_);
   })
@@ -266,13 +409,21 @@ __global__ __launch_bounds__(kBT) void kb_res(BTables tb) {
 // CG: r = (float) r64, z = d.*r, partial r.z
 __global__ __launch_bounds__(kBT) void kb_cg_reset(BTables tb) {
   PSB_BATCH_TILES(tb, b_transiting(st), {
+    PSB_QUAD(i, nv)
     double rz = 0.0;
-    if (in) {
-      const float ri = (float)sy.r64[i];
-      sy.r[i] = ri;
-      const float zi = sy.d32 ? sy.d32[i] * ri : ri;
-      sy.z[i] = zi;
-      rz = (double)ri * (double)zi;
+    if (nv > 0) {
+      double r64[4];
+      float r[4], z[4];
+      ld4(sy.r64 + i, nv, r64);
+      if (sy.d32) ld4(sy.d32 + i, nv, z);
+_Pragma("unroll")
+      for (int k = 0; k < 4; k++) {
+        r[k] = (float)r64[k];
+        z[k] = sy.d32 ? z[k] * r[k] : r[k];
+        rz += (double)r[k] * (double)z[k];
+      }
+      st4(sy.r + i, nv, r);
+      st4(sy.z + i, nv, z);
     }
     tile_partials(rz, 0.0, tb.partials, t_);
   })
@@ -281,12 +432,20 @@ __global__ __launch_bounds__(kBT) void kb_cg_reset(BTables tb) {
 // BiCGSTAB: r = q = (float)(r64 / ||r64||), partial r.r
 __global__ __launch_bounds__(kBT) void kb_bi_restart(BTables tb) {
   PSB_BATCH_TILES(tb, b_transiting(st), {
+    PSB_QUAD(i, nv)
     double rr = 0.0;
-    if (in) {
-      const float ri = (float)(sy.r64[i] * (1.0 / st->rnorm));
-      sy.r[i] = ri;
-      sy.q[i] = ri;
-      rr = (double)ri * (double)ri;
+    if (nv > 0) {
+      const double inv = 1.0 / st->rnorm;
+      double r64[4];
+      float r[4];
+      ld4(sy.r64 + i, nv, r64);
+_Pragma("unroll")
+      for (int k = 0; k < 4; k++) {
+        r[k] = (float)(r64[k] * inv);
+        rr += (double)r[k] * (double)r[k];
+      }
+      st4(sy.r + i, nv, r);
+      st4(sy.q + i, nv, r);
     }
     tile_partials(rr, 0.0, tb.partials, t_);
   })
@@ -548,10 +707,10 @@ static int upload(Batch& B) {
     int nt = 0;
     for (int k = 0; k < B.nslots; k++) {
       const BSlot& sl = B.slots[k];
-      const int nk = sl.live ? (B.sys_h[k].n + kBT - 1) / kBT : 0;
+      const int nk = sl.live ? (B.sys_h[k].n + kTileRows - 1) / kTileRows : 0;
       if (nt + nk > B.tmax) return set_error(PSB_HIP_RUNTIME, "batch", "tile table overflow");
       S.range[k] = make_int2(nt, nk);
-      for (int j = 0; j < nk; j++) S.tiles[nt + j] = make_int2(k, j * kBT);
+      for (int j = 0; j < nk; j++) S.tiles[nt + j] = make_int2(k, j * kTileRows);
       nt += nk;
     }
     *S.ntiles = nt;
@@ -628,8 +787,8 @@ int psb_hip_batch_create(void** batch, int method, int max_systems, int64_t aren
   B->stream = ctx().stream;
   B->arena_bytes = align256((size_t)arena_bytes);
   // tiles: every arena row could be a tile row, plus one partial tile per slot
-  const size_t row_bytes = sys_bytes(method, kBT) / kBT;
-  B->tmax = (int)std::min<size_t>(B->arena_bytes / (row_bytes * kBT) + (size_t)max_systems,
+  const size_t row_bytes = sys_bytes(method, kTileRows) / kTileRows;
+  B->tmax = (int)std::min<size_t>(B->arena_bytes / (row_bytes * kTileRows) + (size_t)max_systems,
                                   (size_t)1 << 30);
   B->slots.resize(max_systems);
   B->sys_h.assign(max_systems, BSys{});
