@@ -329,6 +329,16 @@ def serve_bench(P, N, nmat=64, seed=4, rates=(200.0, 1000.0, 5000.0), count=160,
         rates_out[str(int(rate))] = {"offered_per_s": rate,
                                      "arrival_span_s": round(reqs[-1][0], 4),
                                      "mixed_batched": mixed, "fp64_one_at_a_time": fp64}
+        if rate == 1000.0:
+            # the restarted-refinement BiCGSTAB on the same stream, for comparison
+            srv_r = P.psb_hip_server("BICGSTAB", max_systems=256, arena_bytes=arena_mb << 20,
+                                     check_every=4, eps=eps, refine="restart")
+            for m, prec, _ in systems:
+                srv_r.register(m, prec)
+            srv_r.run([(0.0, k, 999000 + k) for k in range(4)])
+            rates_out["1000"]["mixed_batched_restart"] = P.latency_summary(
+                srv_r.run(reqs, time_limit=90.0))
+            srv_r.free()
         log(f"serve: {rate:.0f} req/s: batched p50 {mixed.get('p50_ms')} ms p99 "
             f"{mixed.get('p99_ms')} ms {mixed.get('solves_per_s')} solves/s | FP64 one at a time "
             f"p50 {fp64.get('p50_ms')} ms p99 {fp64.get('p99_ms')} ms {fp64.get('solves_per_s')} "

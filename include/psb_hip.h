@@ -375,7 +375,8 @@ int psb_hip_krylov_mixed(int method, void *dmat, void *diag, void *b, void *x, d
 /* step runs check_every FP32 iterations of every live system and one FP64  */
 /* transition pass (method 0: CG with FP64 residual replacement when the    */
 /* recursive residual drops by delta; method 1: BiCGSTAB restarted          */
-/* refinement with inner tolerance inner_eps), as one captured graph when   */
+/* refinement with inner tolerance inner_eps; method 2: BiCGSTAB with FP64  */
+/* residual replacement as method 0), as one captured graph when            */
 /* use_graph != 0.  Systems are admitted (submit) and retired (result)      */
 /* between steps; the vectors of a system live in the engine's arena of     */
 /* arena_bytes, and submit returns PSB_HIP_BUSY when no slot or arena room  */

@@ -24,17 +24,20 @@ def _vdev(v):
 
 
 class psb_hip_batch:
-    """Engine handle.  method "CG" (FP32 CG with FP64 residual replacement
-    whenever the recursive residual drops by ``delta``) or "BICGSTAB" (FP32
-    BiCGSTAB restarted refinement with inner tolerance ``inner_eps``)."""
+    """Engine handle.  method "CG" or "BICGSTAB"; refine "reliable" (one FP32
+    recurrence with FP64 residual replacement whenever the recursive residual
+    drops by ``delta``) or, for BiCGSTAB, "restart" (restarted refinement:
+    each cycle solves A d = r/||r|| in FP32 to ``inner_eps``)."""
 
     def __init__(self, method="CG", max_systems=256, arena_bytes=1 << 30, check_every=8,
-                 delta=1e-4, inner_eps=1e-4, use_graph=True):
+                 delta=1e-4, inner_eps=1e-4, use_graph=True, refine="reliable"):
         self.method = method.upper()
-        if self.method not in ("CG", "BICGSTAB"):
-            raise N.PsbError(N.INVALID, "psb_hip_batch", f"method {method}")
+        if self.method not in ("CG", "BICGSTAB") or refine not in ("reliable", "restart") or \
+                (self.method == "CG" and refine == "restart"):
+            raise N.PsbError(N.INVALID, "psb_hip_batch", f"method {method} / refine {refine}")
+        code = 0 if self.method == "CG" else (1 if refine == "restart" else 2)
         h = C.c_void_p()
-        N.call("psb_hip_batch_create", C.byref(h), 0 if self.method == "CG" else 1,
+        N.call("psb_hip_batch_create", C.byref(h), code,
                int(max_systems), int(arena_bytes), int(check_every), float(delta),
                float(inner_eps), 1 if use_graph else 0)
         self.h = h

@@ -21,6 +21,9 @@
 //   BiCGSTAB (method 1): restarted refinement: FP32 BiCGSTAB solves
 //     A d = r64/||r64|| to the inner tolerance, then x64 += ||r64|| d and
 //     r64 = b - A64 x64 in FP64, and the inner solve restarts.
+//   BiCGSTAB (method 2): reliable updates as for CG: the FP32 recurrence
+//     keeps p, v and the shadow residual; at a replacement x64 += x32 and the
+//     FP32 residual restarts from r64, with rho and beta recomputed from it.
 // The transition pass also initialises newly admitted systems (r64 = b - A x0,
 // ||b||) and decides convergence on the FP64 residual (||r64|| <= eps ||b||).
 // The per-iteration operation sequences are those of the fused single-system
@@ -362,15 +365,16 @@ _Pragma("unroll")
 }
 
 // ------------------------------------------------------------------ transitions
-// x64 += scale * x32 (CG: scale 1; BiCGSTAB: ||r64|| of the finished cycle),
-// x32 = 0.  A newly admitted system only clears x32.
-__global__ __launch_bounds__(kBT) void kb_accum(BTables tb, int cg) {
+// x64 += scale * x32 (residual replacement: scale 1; restarted refinement:
+// ||r64|| of the finished cycle), x32 = 0.  A newly admitted system only
+// clears x32.
+__global__ __launch_bounds__(kBT) void kb_accum(BTables tb, int unit_scale) {
   PSB_BATCH_TILES(tb, b_transiting(st), {
     PSB_QUAD(i, nv)
     if (nv > 0) {
       float x32[4];
       if (st->phase != BP_INIT) {
-        const double scale = cg ? 1.0 : st->ref;
+        const double scale = unit_scale ? 1.0 : st->ref;
         double x[4];
         ld4(sy.x + i, nv, x);
         ld4(sy.x32 + i, nv, x32);
@@ -451,13 +455,42 @@ _Pragma("unroll")
   })
 }
 
+// BiCGSTAB with residual replacement: r = (float) r64; a newly admitted or
+// broken-down system also takes q = r.  Partial q.r.
+__global__ __launch_bounds__(kBT) void kb_biru_reset(BTables tb) {
+  PSB_BATCH_TILES(tb, b_transiting(st), {
+    PSB_QUAD(i, nv)
+    double qr = 0.0;
+    if (nv > 0) {
+      const bool fresh = st->phase != BP_TRANS;
+      double r64[4];
+      float r[4], q[4];
+      ld4(sy.r64 + i, nv, r64);
+      if (!fresh) ld4(sy.q + i, nv, q);
+_Pragma("unroll")
+      for (int k = 0; k < 4; k++) {
+        r[k] = (float)r64[k];
+        if (fresh) q[k] = r[k];
+        qr += (double)q[k] * (double)r[k];
+      }
+      st4(sy.r + i, nv, r);
+      if (fresh) st4(sy.q + i, nv, q);
+    }
+    tile_partials(qr, 0.0, tb.partials, t_);
+  })
+}
+
 // ------------------------------------------------------------------ per-system finalisation
-enum BFin { BF_CG_Q, BF_CG_X, BF_BI_V, BF_BI_T, BF_BI_X, BF_RES, BF_CG_RESET, BF_BI_RESTART };
+enum BFin {
+  BF_CG_Q, BF_CG_X, BF_BI_V, BF_BI_T, BF_BI_X, BF_BIRU_X, BF_RES, BF_CG_RESET, BF_BI_RESTART,
+  BF_BIRU_RESET
+};
 
 template <int OP>
 __global__ __launch_bounds__(kBT) void kb_fin(BTables tb) {
   BState* st = tb.st + blockIdx.x;
-  const bool trans = OP == BF_RES || OP == BF_CG_RESET || OP == BF_BI_RESTART;
+  const bool trans =
+      OP == BF_RES || OP == BF_CG_RESET || OP == BF_BI_RESTART || OP == BF_BIRU_RESET;
   if (trans ? !b_transiting(st) : !b_iterating(st)) return;
   const int2 rg = tb.range[blockIdx.x];
   double a = 0.0, b = 0.0;
@@ -522,6 +555,23 @@ __global__ __launch_bounds__(kBT) void kb_fin(BTables tb) {
     } else {
       st->beta = (st->rho / st->rho_old) * (st->alpha / st->omega);
     }
+  } else if (OP == BF_BIRU_X) {
+    st->itx += 1;
+    st->it += 1;
+    const double rn = sqrt(a);
+    st->rr = rn;
+    st->rho_old = st->rho;
+    if (rn <= st->delta * st->ref || rn <= st->eps * st->bnorm || st->itx >= st->itmax) {
+      st->phase = BP_TRANS;   // rho and beta follow from the replaced residual
+      return;
+    }
+    st->rho = b;
+    if (st->rho == 0.0) {
+      st->itx += 1;
+      st->phase = BP_BREAK;
+    } else {
+      st->beta = (st->rho / st->rho_old) * (st->alpha / st->omega);
+    }
   } else if (OP == BF_RES) {
     st->rnorm = sqrt(a);
     if (st->phase == BP_INIT) st->bnorm = sqrt(b);
@@ -537,6 +587,16 @@ __global__ __launch_bounds__(kBT) void kb_fin(BTables tb) {
       st->beta = rho / st->rho_old;
     } else {
       st->it = 0;
+    }
+    st->rho = rho;
+    st->phase = BP_RUN;
+  } else if (OP == BF_BIRU_RESET) {
+    const double rho = a;   // q.r with the replaced residual
+    st->ref = st->rnorm;
+    if (st->phase == BP_TRANS && st->rho_old != 0.0 && rho != 0.0) {
+      st->beta = (rho / st->rho_old) * (st->alpha / st->omega);
+    } else {
+      st->it = 0;           // (re)start: p = r, q = r was taken by the reset kernel
     }
     st->rho = rho;
     st->phase = BP_RUN;
@@ -630,7 +690,7 @@ static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 // arena bytes one system of n rows needs
 static size_t sys_bytes(int method, int n) {
   const size_t d = align256(sizeof(double) * (size_t)n), f = align256(sizeof(float) * (size_t)n);
-  return 3 * d + (method == 0 ? 5 : 9) * f;
+  return 3 * d + (method == 0 ? 5 : 9) * f;   // BiCGSTAB (1, 2): v, f, s, t too
 }
 
 static bool arena_take(Batch& B, size_t bytes, size_t* off) {
@@ -683,15 +743,19 @@ static void enqueue_step(Batch& B) {
       hipLaunchKernelGGL(kb_bi_t, g, blk, 0, s, tb);
       hipLaunchKernelGGL(kb_fin<BF_BI_T>, gs, blk, 0, s, tb);
       hipLaunchKernelGGL(kb_bi_x, g, blk, 0, s, tb);
-      hipLaunchKernelGGL(kb_fin<BF_BI_X>, gs, blk, 0, s, tb);
+      if (B.method == 1) hipLaunchKernelGGL(kb_fin<BF_BI_X>, gs, blk, 0, s, tb);
+      else hipLaunchKernelGGL(kb_fin<BF_BIRU_X>, gs, blk, 0, s, tb);
     }
   }
-  hipLaunchKernelGGL(kb_accum, g, blk, 0, s, tb, B.method == 0 ? 1 : 0);
+  hipLaunchKernelGGL(kb_accum, g, blk, 0, s, tb, B.method == 1 ? 0 : 1);
   hipLaunchKernelGGL(kb_res, g, blk, 0, s, tb);
   hipLaunchKernelGGL(kb_fin<BF_RES>, gs, blk, 0, s, tb);
   if (B.method == 0) {
     hipLaunchKernelGGL(kb_cg_reset, g, blk, 0, s, tb);
     hipLaunchKernelGGL(kb_fin<BF_CG_RESET>, gs, blk, 0, s, tb);
+  } else if (B.method == 2) {
+    hipLaunchKernelGGL(kb_biru_reset, g, blk, 0, s, tb);
+    hipLaunchKernelGGL(kb_fin<BF_BIRU_RESET>, gs, blk, 0, s, tb);
   } else {
     hipLaunchKernelGGL(kb_bi_restart, g, blk, 0, s, tb);
     hipLaunchKernelGGL(kb_fin<BF_BI_RESTART>, gs, blk, 0, s, tb);
@@ -775,7 +839,7 @@ int psb_hip_batch_create(void** batch, int method, int max_systems, int64_t aren
   PSB_ENSURE_INIT();
   if (!batch) return PSB_HIP_INVALID;
   *batch = nullptr;
-  if ((method != 0 && method != 1) || max_systems < 1 || arena_bytes <= 0 || check_every < 1)
+  if (method < 0 || method > 2 || max_systems < 1 || arena_bytes <= 0 || check_every < 1)
     return set_error(PSB_HIP_INVALID, __func__, "method, max_systems, arena or check_every");
   Batch* B = new Batch();
   B->method = method;
@@ -925,7 +989,7 @@ int psb_hip_batch_submit(void* batch, int mat_id, void* bv, void* x0v, double ep
   sy.p = fv[2];
   sy.q = fv[3];
   sy.z = fv[4];
-  if (B->method == 1) {
+  if (B->method != 0) {
     sy.v = fv[5];
     sy.f = fv[6];
     sy.s = fv[7];

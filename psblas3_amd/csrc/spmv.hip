@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmv_pair(int nr, int hksz, int 
 // issue together (slots past the row's width read its last slot, product
 // dropped, as in slot_group); each slot is loaded once and used for every
 // column, each column accumulating in slot order.
-template <typename T, int NC>
+template <typename T, int NC, bool XCD>
 __global__ __launch_bounds__(kBlock) void k_hll_spmm(int nr, int hksz, int nlog,
                                                      const int* __restrict__ hkoffs,
                                                      const int* __restrict__ ja,
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmm(int nr, int hksz, int nlog,
                                                      const T* __restrict__ x, int ldx,
                                                      T alpha, T beta, T* __restrict__ y,
                                                      int ldy, int ncols) {
-  const int lb = blockIdx.x;
+  const int lb = XCD ? xcd_block(nlog) : (int)blockIdx.x;
   if (lb >= nlog) return;
   const int row = lb * kBlock + threadIdx.x;
   const int r = row < nr ? row : nr - 1;
@@ -450,15 +450,25 @@ template <typename T>
 static int hll_spmm_launch(const HllDev* m, const T* val, const T* x, int ldx, T alpha, T beta,
                            T* y, int ldy, int ncols) {
   hipStream_t s = ctx().stream;
+  // XCD-contiguous block order: measured 2-6% faster than launch order for
+  // k = 8, 16 and 32 columns on the L=200 stencil (each XCD's L2 keeps the
+  // neighbouring planes of all columns of its rows)
+  const bool xcd = true;
   const int nlog = grid_for(m->rows, kBlock);
-  dim3 grid(nlog), blk(kBlock);
+  dim3 grid(xcd ? xcd_grid(nlog) : nlog), blk(kBlock);
   for (int c0 = 0; c0 < ncols; c0 += 32) {
     const int nc = ncols - c0 < 32 ? ncols - c0 : 32;
     const T* xc = x + (size_t)c0 * ldx;
     T* yc = y + (size_t)c0 * ldy;
-#define PSB_SPMM_CASE(NC)                                                                    \
-  hipLaunchKernelGGL((k_hll_spmm<T, NC>), grid, blk, 0, s, m->rows, m->hksz, nlog, m->hkoffs, \
-                     m->ja, val, xc, ldx, alpha, beta, yc, ldy, nc)
+#define PSB_SPMM_CASE(NC)                                                                      \
+  do {                                                                                         \
+    if (xcd)                                                                                   \
+      hipLaunchKernelGGL((k_hll_spmm<T, NC, true>), grid, blk, 0, s, m->rows, m->hksz, nlog,   \
+                         m->hkoffs, m->ja, val, xc, ldx, alpha, beta, yc, ldy, nc);            \
+    else                                                                                       \
+      hipLaunchKernelGGL((k_hll_spmm<T, NC, false>), grid, blk, 0, s, m->rows, m->hksz, nlog,  \
+                         m->hkoffs, m->ja, val, xc, ldx, alpha, beta, yc, ldy, nc);            \
+  } while (0)
     if (nc > 16) PSB_SPMM_CASE(32);
     else if (nc > 8) PSB_SPMM_CASE(16);
     else if (nc > 4) PSB_SPMM_CASE(8);

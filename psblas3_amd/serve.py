@@ -30,9 +30,10 @@ class psb_hip_server:
     """Request loop over one batch engine (method "BICGSTAB" or "CG")."""
 
     def __init__(self, method="BICGSTAB", max_systems=128, arena_bytes=4 << 30, check_every=4,
-                 eps=1e-10, itmax=20000, inner_eps=1e-4, delta=1e-4, seed=4):
+                 eps=1e-10, itmax=20000, inner_eps=1e-4, delta=1e-4, seed=4, refine="reliable"):
         self.eng = psb_hip_batch(method, max_systems=max_systems, arena_bytes=arena_bytes,
-                                 check_every=check_every, delta=delta, inner_eps=inner_eps)
+                                 check_every=check_every, delta=delta, inner_eps=inner_eps,
+                                 refine=refine)
         self.eps, self.itmax, self.seed = eps, itmax, seed
         self.mats = []           # (engine mat id, rows, rhs staging vector, solution vector)
 
@@ -139,6 +140,7 @@ def latency_summary(rec):
             "p99_ms": round(float(np.percentile(lat, 99)), 3),
             "mean_ms": round(float(lat.mean()), 3),
             "solves_per_s": round(len(fin) / span, 2) if span > 0 else None,
+            "iterations_mean": round(float(np.mean([r["iterations"] for r in fin])), 1),
             "max_relres": max(r["relres"] for r in fin)}
 
 

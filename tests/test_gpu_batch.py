@@ -130,15 +130,19 @@ def test_batch_graph_equals_eager(gpu, orc):
 BI_CASES = [(10, 0.0), (16, 3.0), (23, 10.0)]
 
 
-def test_batch_bicgstab_convdiff(gpu, orc):
-    _say("case batch BiCGSTAB convection-diffusion eps 1e-10")
+@pytest.mark.parametrize("refine", ["reliable", "restart"])
+def test_batch_bicgstab_convdiff(gpu, orc, refine):
+    _say(f"case batch BiCGSTAB ({refine}) convection-diffusion eps 1e-10")
     systems = [_system(orc, "convdiff", L, c) for L, c in BI_CASES]
-    eng = P.psb_hip_batch("BICGSTAB", max_systems=8, arena_bytes=64 << 20, check_every=4)
+    eng = P.psb_hip_batch("BICGSTAB", max_systems=8, arena_bytes=64 << 20, check_every=4,
+                          refine=refine)
     xs, out, bs = _run(orc, eng, systems, 1e-10, seeds=[5, 6, 7])
     eng.free()
     for (L, c), sys_, x, b, o in zip(BI_CASES, systems, xs, bs, out):
         rr = _relres(orc, sys_[2], b, x)
-        _say(f"  L={L} conv={c}: {o['iterations']} FP32 its, {o['outer']} cycles, relres {rr:.2e}")
+        _, ito, _, _ = orc.krylov("BICGSTAB", sys_[2], b, eps=1e-10, itmax=5000)
+        _say(f"  L={L} conv={c}: {o['iterations']} FP32 its, {o['outer']} transitions, "
+             f"relres {rr:.2e} (oracle FP64 BiCGSTAB {ito} its)")
         assert o["status"] == 1
         assert rr <= 1.0e-10 * 1.01
 

@@ -25,8 +25,10 @@ def test_poisson_arrivals_deterministic():
 
 
 def test_latency_summary():
-    rec = [{"arrival": 0.0, "done": 0.010, "latency": 0.010, "status": 1, "relres": 1e-11},
-           {"arrival": 0.005, "done": 0.020, "latency": 0.015, "status": 1, "relres": 2e-11},
+    rec = [{"arrival": 0.0, "done": 0.010, "latency": 0.010, "status": 1, "relres": 1e-11,
+            "iterations": 10},
+           {"arrival": 0.005, "done": 0.020, "latency": 0.015, "status": 1, "relres": 2e-11,
+            "iterations": 20},
            {"arrival": 0.006, "status": 0}]
     s = latency_summary(rec)
     assert s["finished"] == 2 and s["requests"] == 3 and s["converged"] == 2
