@@ -216,6 +216,25 @@ int64_t psb_hip_gen_powerlaw(int n, int cap, uint64_t seed, int *ia, int *ja, do
 /* Counter-based uniform [-1,1) fill (host), the same stream the generators
    use: x(i) = u(seed, stream, i). */
 void psb_hip_fill_uniform_host(uint64_t seed, uint32_t stream, int64_t n, double *x);
+/* MatrixMarket I/O (host) -- util/psb_d_mmio_impl.f90.
+   psb_hip_mm_read_coo: mm_mat_read (:315-440): a "matrix coordinate" file,
+   type real or pattern (values 1), general or symmetric (the mirrored
+   off-diagonal entries appended after the stored ones, in file order).
+   With ia == NULL: *nr, *nc and *nz = the entry capacity needed.  Else the
+   entries (1-based, unsorted: fix them with psb_hip_fix_coo_d) into arrays
+   of capacity cap; *nz = the count.  PSB_HIP_RUNTIME: cannot open;
+   PSB_HIP_UNSUPPORTED: other object/format/type/symmetry (reference info
+   909 / 904); PSB_HIP_INVALID: malformed or short file (902 / 905). */
+int psb_hip_mm_read_coo(const char *path, int64_t cap, int *nr, int *nc, int64_t *nz, int *ia,
+                        int *ja, double *val);
+/* mm_dvet_read (:32-102): "matrix array real general", first column; with
+   b == NULL only *n. */
+int psb_hip_mm_read_vec(const char *path, int64_t cap, int64_t *n, double *b);
+/* the COO print (base/serial/impl/psb_d_coo_impl.F90:728-763) and
+   mm_dvet1_write (:226-280): values with 19 significant digits */
+int psb_hip_mm_write_coo(const char *path, const char *title, int nr, int nc, int64_t nz,
+                         const int *ia, const int *ja, const double *val);
+int psb_hip_mm_write_vec(const char *path, const char *title, int64_t n, const double *b);
 /* The same values generated on the device into a double multivector (column c
    of a multi-column vector uses stream + c). */
 int psb_hip_mv_fill_uniform(void *x, uint64_t seed, uint32_t stream);

@@ -25,3 +25,4 @@ from .batch import psb_hip_batch  # noqa: F401
 from .serve import (latency_summary, poisson_arrivals, psb_hip_server,  # noqa: F401
                     serve_one_at_a_time)
 from .tune import psb_hip_hll_autotune  # noqa: F401
+from .mmio import mm_array_read, mm_array_write, mm_mat_read, mm_mat_write  # noqa: F401

@@ -144,6 +144,10 @@ _SIGS = {
     "psb_hip_krylov_fused": (i32, [i32, i32, vp, vp, vp, vp, f64, i32, i32, i32, pi32, pf64, vp]),
     "psb_hip_krylov_mixed": (i32, [i32, vp, vp, vp, vp, f64, i32, f64, i32, i32, i32, pi32,
                                    pi32, pf64]),
+    "psb_hip_mm_read_coo": (i32, [C.c_char_p, i64, pi32, pi32, pi64, vp, vp, vp]),
+    "psb_hip_mm_read_vec": (i32, [C.c_char_p, i64, pi64, vp]),
+    "psb_hip_mm_write_coo": (i32, [C.c_char_p, C.c_char_p, i32, i32, i64, vp, vp, vp]),
+    "psb_hip_mm_write_vec": (i32, [C.c_char_p, C.c_char_p, i64, vp]),
     "psb_hip_batch_create": (i32, [pvp, i32, i32, i64, i32, f64, f64, i32]),
     "psb_hip_batch_free": (i32, [vp]),
     "psb_hip_batch_register": (i32, [vp, vp, vp, pi32]),

@@ -6,8 +6,11 @@
 // row and a per-row stable column sort spread over host threads; the
 // result is the reference's: rows ascending, columns ascending within a
 // row, duplicates merged left to right in input order.
+#include <ctype.h>
 #include <math.h>
 #include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -327,6 +330,165 @@ void psb_hip_fill_uniform_host(uint64_t seed, uint32_t stream, int64_t n, double
   parallel_for(n, [&](int64_t lo, int64_t hi) {
     for (int64_t i = lo; i < hi; i++) x[i] = u.pm1((uint64_t)i);
   });
+}
+
+// ---------------------------------------------------------------- MatrixMarket
+// mm_mat_read / mm_dvet_read / the COO print and mm_dvet1_write
+// (util/psb_d_mmio_impl.f90:32-102,226-280,315-486;
+// base/serial/impl/psb_d_coo_impl.F90:728-763).
+
+struct MMFile {
+  FILE* f = nullptr;
+  char object[32] = {0}, format[32] = {0}, type[32] = {0}, sym[32] = {0};
+  long long nrow = 0, ncol = 0, nnz = 0;
+  ~MMFile() {
+    if (f) fclose(f);
+  }
+};
+
+static void mm_lower(char* s) {
+  for (; *s; s++) *s = (char)tolower((unsigned char)*s);
+}
+
+// Opens the file, reads the banner and the size line (nnz only for
+// coordinate files).  0, or a PSB_HIP_* code.
+static int mm_open(const char* path, bool coordinate, MMFile& m) {
+  m.f = fopen(path, "r");
+  if (!m.f) return PSB_HIP_RUNTIME;
+  char line[1100];
+  if (!fgets(line, sizeof line, m.f)) return PSB_HIP_INVALID;
+  char banner[32];
+  if (sscanf(line, "%31s %31s %31s %31s %31s", banner, m.object, m.format, m.type, m.sym) != 5)
+    return PSB_HIP_INVALID;
+  mm_lower(m.object);
+  mm_lower(m.format);
+  mm_lower(m.type);
+  mm_lower(m.sym);
+  if (strcmp(m.object, "matrix") != 0 || strcmp(m.format, coordinate ? "coordinate" : "array"))
+    return PSB_HIP_UNSUPPORTED;
+  do {
+    if (!fgets(line, sizeof line, m.f)) return PSB_HIP_INVALID;
+  } while (line[0] == '%');
+  const int want = coordinate ? 3 : 2;
+  if (sscanf(line, "%lld %lld %lld", &m.nrow, &m.ncol, &m.nnz) < want) return PSB_HIP_INVALID;
+  if (m.nrow < 0 || m.ncol < 0 || m.nnz < 0 || m.nrow > 0x7fffffff || m.ncol > 0x7fffffff)
+    return PSB_HIP_INVALID;
+  return PSB_HIP_SUCCESS;
+}
+
+// The rest of the file in memory, NUL-terminated.
+static bool mm_slurp(FILE* f, std::vector<char>& buf) {
+  const long pos = ftell(f);
+  if (fseek(f, 0, SEEK_END) != 0) return false;
+  const long end = ftell(f);
+  if (fseek(f, pos, SEEK_SET) != 0 || end < pos) return false;
+  buf.resize((size_t)(end - pos) + 1);
+  const size_t got = fread(buf.data(), 1, (size_t)(end - pos), f);
+  buf[got] = 0;
+  return true;
+}
+
+int psb_hip_mm_read_coo(const char* path, int64_t cap, int* nr, int* nc, int64_t* nz, int* ia,
+                        int* ja, double* val) {
+  MMFile m;
+  int rc = mm_open(path, true, m);
+  if (rc) return rc;
+  const bool pattern = strcmp(m.type, "pattern") == 0;
+  const bool symmetric = strcmp(m.sym, "symmetric") == 0;
+  if ((!pattern && strcmp(m.type, "real") != 0) || (!symmetric && strcmp(m.sym, "general") != 0))
+    return PSB_HIP_UNSUPPORTED;   // the reference reads real and pattern, general and symmetric
+  *nr = (int)m.nrow;
+  *nc = (int)m.ncol;
+  const int64_t need = symmetric ? 2 * m.nnz : m.nnz;
+  if (!ia) {                      // size query
+    *nz = need;
+    return PSB_HIP_SUCCESS;
+  }
+  if (cap < need) {
+    *nz = need;
+    return PSB_HIP_INVALID;
+  }
+  std::vector<char> buf;
+  if (!mm_slurp(m.f, buf)) return PSB_HIP_RUNTIME;
+  char* p = buf.data();
+  for (int64_t k = 0; k < m.nnz; k++) {
+    char* e;
+    const long r = strtol(p, &e, 10);
+    if (e == p) return PSB_HIP_INVALID;   // unexpected end of file or bad entry
+    p = e;
+    const long c = strtol(p, &e, 10);
+    if (e == p) return PSB_HIP_INVALID;
+    p = e;
+    double v = 1.0;
+    if (!pattern) {
+      v = strtod(p, &e);
+      if (e == p) return PSB_HIP_INVALID;
+      p = e;
+    }
+    if (r < 1 || r > m.nrow || c < 1 || c > m.ncol) return PSB_HIP_INVALID;
+    ia[k] = (int)r;
+    ja[k] = (int)c;
+    val[k] = v;
+  }
+  int64_t n = m.nnz;
+  if (symmetric) {                // mirrored off-diagonal entries, appended in read order
+    for (int64_t k = 0; k < m.nnz; k++) {
+      if (ia[k] != ja[k]) {
+        ia[n] = ja[k];
+        ja[n] = ia[k];
+        val[n] = val[k];
+        n++;
+      }
+    }
+  }
+  *nz = n;
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_mm_read_vec(const char* path, int64_t cap, int64_t* n, double* b) {
+  MMFile m;
+  int rc = mm_open(path, false, m);
+  if (rc) return rc;
+  if (strcmp(m.type, "real") != 0 || strcmp(m.sym, "general") != 0) return PSB_HIP_UNSUPPORTED;
+  *n = m.nrow;
+  if (!b) return PSB_HIP_SUCCESS;
+  if (cap < m.nrow) return PSB_HIP_INVALID;
+  std::vector<char> buf;
+  if (!mm_slurp(m.f, buf)) return PSB_HIP_RUNTIME;
+  char* p = buf.data();
+  for (int64_t i = 0; i < m.nrow; i++) {   // the first column
+    char* e;
+    b[i] = strtod(p, &e);
+    if (e == p) return PSB_HIP_INVALID;
+    p = e;
+  }
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_mm_write_coo(const char* path, const char* title, int nr, int nc, int64_t nz,
+                         const int* ia, const int* ja, const double* val) {
+  FILE* f = fopen(path, "w");
+  if (!f) return PSB_HIP_RUNTIME;
+  fprintf(f, "%%%%MatrixMarket matrix coordinate real general\n");
+  if (title) fprintf(f, "%% %s\n", title);
+  fprintf(f, "%%\n%% COO\n");
+  fprintf(f, "%d %d %lld\n", nr, nc, (long long)nz);
+  for (int64_t k = 0; k < nz; k++) fprintf(f, "%d %d %.18e\n", ia[k], ja[k], val[k]);
+  const bool ok = ferror(f) == 0;
+  fclose(f);
+  return ok ? PSB_HIP_SUCCESS : PSB_HIP_RUNTIME;
+}
+
+int psb_hip_mm_write_vec(const char* path, const char* title, int64_t n, const double* b) {
+  FILE* f = fopen(path, "w");
+  if (!f) return PSB_HIP_RUNTIME;
+  fprintf(f, "%%%%MatrixMarket matrix array real general\n");
+  fprintf(f, "%% %s\n%% \n", title ? title : "");
+  fprintf(f, "%lld 1\n", (long long)n);
+  for (int64_t i = 0; i < n; i++) fprintf(f, "%.18e\n", b[i]);
+  const bool ok = ferror(f) == 0;
+  fclose(f);
+  return ok ? PSB_HIP_SUCCESS : PSB_HIP_RUNTIME;
 }
 
 }  // extern "C"
