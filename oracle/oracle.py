@@ -97,6 +97,10 @@ def _declare(L):
     L.orc_d_krylov_hll.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _i32p, _i32p, _f64p,
                                    _f64p, _f64p, _f64p, C.c_double, C.c_int, C.c_int, C.c_double,
                                    C.POINTER(C.c_int), C.POINTER(C.c_double), _f64p]
+    L.orc_d_krylov2_csr.restype = C.c_int
+    L.orc_d_krylov2_csr.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p,
+                                    _f64p, C.c_double, C.c_int, C.c_int, C.c_double,
+                                    C.POINTER(C.c_int), C.POINTER(C.c_double), _f64p]
     L.orc_num_threads.restype = C.c_int
 
 
@@ -326,12 +330,33 @@ def diag_build(irp, ja, val):
     return d
 
 
-def krylov(method, a, b, x0=None, eps=1e-9, itmax=1000, istop=2, d=None, ani=0.0):
+KRYLOV_METHODS = {"CG": 0, "BICGSTAB": 1, "FCG": 2, "GCR": 3, "RGMRES": 4, "GMRES": 4,
+                  "BICGSTABL": 5}
+# default restart length / l when irst is absent (psb_dgcr.f90:207-216,
+# psb_drgmres.f90:201-210, psb_dcgstabl.f90:190-199)
+KRYLOV_IRST = {3: 10, 4: 10, 5: 1}
+
+
+def krylov(method, a, b, x0=None, eps=1e-9, itmax=1000, istop=2, d=None, ani=0.0, irst=None):
     """psb_krylov with the DIAG preconditioner.  a: (irp, ja, val) CSR tuple or
-    an HLL.  method: 'CG' or 'BICGSTAB'.  Returns (x, iter, err, hist)."""
-    meth = 0 if method.upper() == "CG" else 1
+    an HLL (CG / BiCGSTAB only).  method: CG, BICGSTAB, FCG, GCR, RGMRES,
+    BICGSTABL (unknown names: BiCGSTAB, as the reference).  Returns (x,
+    iter, err, hist)."""
+    meth = KRYLOV_METHODS.get(method.upper(), 1)
     n = len(b)
     x = np.zeros(n) if x0 is None else np.ascontiguousarray(x0, np.float64).copy()
+    if meth >= 2:
+        irp, ja, val = a
+        if d is None:
+            d = diag_build(irp, ja, val)
+        nl = KRYLOV_IRST.get(meth, 0) if irst is None else int(irst)
+        hist = np.zeros(itmax + nl + 2)
+        it = C.c_int(0)
+        err = C.c_double(0)
+        lib().orc_d_krylov2_csr(meth, nl, len(irp) - 1, irp, ja, val, d,
+                                np.ascontiguousarray(b, np.float64), x, eps, itmax, istop, ani,
+                                C.byref(it), C.byref(err), hist)
+        return x, it.value, err.value, hist[1:it.value + 1].copy()
     hist = np.zeros(itmax + 2)
     it = C.c_int(0)
     err = C.c_double(0)

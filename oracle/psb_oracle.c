@@ -871,6 +871,365 @@ static int orc_dcgstab(const orc_mat_t *a, const double *d, const double *b, dou
   return 0;
 }
 
+EXPORT int orc_d_krylov_csr(int method, int nr, const int *irp, const int *ja, const double *val,
+                            const double *d, const double *b, double *x, double eps, int itmax,
+                            int istop, double ani, int *iter, double *err, double *hist);
+
+/* psb_d_init_conv_vect (krylov/psb_d_krylov_conv_mod.f90:205-277) */
+static void orc_init_conv(orc_conv_t *s, const orc_mat_t *a, const double *x, const double *b)
+{
+  int n = a->nr;
+  if (s->istop == 1) s->bni = orc_d_amax(n, b);
+  if (s->istop == 2) s->bn2 = orc_d_nrm2(n, b);
+  if (s->istop == 3) {
+    double *r = malloc(sizeof(double) * (n > 0 ? n : 1));
+    memcpy(r, b, sizeof(double) * n);
+    orc_spmm(a, -1.0, x, 1.0, r);
+    s->r0n2 = orc_d_nrm2(n, r);
+    free(r);
+  }
+}
+
+static double **orc_vecs(int k, int n)
+{
+  double **v = malloc(sizeof(double *) * k);
+  for (int i = 0; i < k; i++) v[i] = calloc(n > 0 ? n : 1, sizeof(double));
+  return v;
+}
+
+static void orc_free_vecs(double **v, int k)
+{
+  for (int i = 0; i < k; i++) free(v[i]);
+  free(v);
+}
+
+/* dtrsm('l','u','n','n', m, 1, 1.0, h, ldh, y, ...) as the reference BLAS
+   loop: column-oriented back substitution, H(1:m,1:m) upper, h[i][k] =
+   H(i,k) with 1-based indices. */
+static void orc_trsv_upper(int m, double **h, double *y)
+{
+  for (int k = m; k >= 1; k--) {
+    if (y[k] != 0.0) {
+      y[k] = y[k] / h[k][k];
+      for (int i = 1; i <= k - 1; i++) y[i] = y[i] - y[k] * h[i][k];
+    }
+  }
+}
+
+/* reference BLAS drotg / drot (n = 1) */
+static void orc_drotg(double *sa, double *sb, double *c, double *s)
+{
+  double roe = fabs(*sa) > fabs(*sb) ? *sa : *sb;
+  double scale = fabs(*sa) + fabs(*sb);
+  double r, z;
+  if (scale == 0.0) {
+    *c = 1.0; *s = 0.0; r = 0.0; z = 0.0;
+  } else {
+    double ta = *sa / scale, tb = *sb / scale;
+    r = scale * sqrt(ta * ta + tb * tb);
+    if (roe < 0.0) r = -r;
+    *c = *sa / r;
+    *s = *sb / r;
+    z = 1.0;
+    if (fabs(*sa) > fabs(*sb)) z = *s;
+    if (fabs(*sb) >= fabs(*sa) && *c != 0.0) z = 1.0 / *c;
+  }
+  *sa = r;
+  *sb = z;
+}
+
+static void orc_drot(double *x, double *y, double c, double s)
+{
+  double t = c * *x + s * *y;
+  *y = c * *y - s * *x;
+  *x = t;
+}
+
+/* psb_dfcg_vect (krylov/psb_dfcg.F90:150-336), DIAG preconditioner. */
+static int orc_dfcg(const orc_mat_t *a, const double *dg, const double *b, double *x,
+                    double eps, int itmax, int istop, double ani,
+                    int *iter, double *err, double *hist)
+{
+  int n = a->nr;
+  double **V = orc_vecs(5, n);
+  double *w = V[0], *v = V[1], *d = V[2], *q = V[3], *r = V[4];
+  orc_conv_t s = {istop, eps, 0, 0, ani, 0, itmax};
+  orc_init_conv(&s, a, x, b);
+  double errnum = 0, errden = 1, alpha, beta, gamma, delta, theta;
+  int itx = 0, done = 0;
+  for (;;) { /* restart */
+    if (itx >= itmax) break;
+    orc_d_axpby(n, 1.0, b, 0.0, r);
+    orc_spmm(a, -1.0, x, 1.0, r);
+    if (orc_check_conv(&s, itx, n, x, r, &errnum, &errden)) break;
+    orc_d_mlt(n, 1.0, dg, r, 0.0, v);
+    orc_spmm(a, 1.0, v, 0.0, w);
+    alpha = orc_d_dot(n, r, v);
+    beta = orc_d_dot(n, w, v);
+    if (beta == 0.0) break;
+    orc_d_axpby(n, 1.0, v, 0.0, d);
+    orc_d_axpby(n, 1.0, w, 0.0, q);
+    delta = beta;
+    theta = alpha / delta;
+    orc_d_axpby(n, theta, d, 1.0, x);
+    orc_d_axpby(n, -theta, q, 1.0, r);
+    for (;;) { /* iteration */
+      itx++;
+      int conv = orc_check_conv(&s, itx, n, x, r, &errnum, &errden);
+      if (hist) hist[itx] = errden > 0 ? errnum / errden : errnum;
+      if (conv) { done = 1; break; }
+      orc_d_mlt(n, 1.0, dg, r, 0.0, v);
+      orc_spmm(a, 1.0, v, 0.0, w);
+      alpha = orc_d_dot(n, r, v);
+      beta = orc_d_dot(n, w, v);
+      gamma = orc_d_dot(n, q, v);
+      theta = gamma / delta;
+      orc_d_axpby(n, 1.0, v, -theta, d);
+      orc_d_axpby(n, 1.0, w, -theta, q);
+      delta = beta - (gamma * gamma) / delta;
+      if (delta == 0.0) break;
+      theta = alpha / delta;
+      orc_d_axpby(n, theta, d, 1.0, x);
+      orc_d_axpby(n, -theta, q, 1.0, r);
+    }
+    if (done) break;
+  }
+  *iter = itx;
+  *err = errden > 0 ? errnum / errden : errnum;
+  orc_free_vecs(V, 5);
+  return 0;
+}
+
+/* psb_dgcr_vect (krylov/psb_dgcr.f90:246-345), DIAG preconditioner, restart
+   length nl.  The recurrence (z, c, c_scale, h, alpha, r) and the stop
+   tests are the reference's; x is assembled as x0 + sum of every cycle's
+   correction (the reference sets x = 0 before the first cycle's
+   correction and again before the last, which drops x0 and the earlier
+   cycles whenever there is more than one). */
+static int orc_dgcr(const orc_mat_t *a, const double *dg, const double *b, double *x,
+                    double eps, int itmax, int istop, double ani, int nl,
+                    int *iter, double *err, double *hist)
+{
+  int n = a->nr;
+  double *r = calloc(n > 0 ? n : 1, sizeof(double));
+  double **z = orc_vecs(nl + 2, n), **c = orc_vecs(nl + 2, n), **cs = orc_vecs(nl + 2, n);
+  double **h = orc_vecs(nl + 2, nl + 2);
+  double *alpha = calloc(nl + 2, sizeof(double));
+  orc_conv_t s = {istop, eps, 0, 0, ani, 0, itmax};
+  orc_init_conv(&s, a, x, b);
+  double errnum = 0, errden = 1;
+  int itx = 0, conv = 0;
+  for (;;) { /* restart */
+    if (itx >= itmax) break;
+    for (int i = 0; i < nl + 2; i++) memset(h[i], 0, sizeof(double) * (nl + 2));
+    int it = 0, j = 0;
+    orc_d_axpby(n, 1.0, b, 0.0, r);
+    orc_spmm(a, -1.0, x, 1.0, r);
+    if (orc_check_conv(&s, itx, n, x, r, &errnum, &errden)) break;
+    for (;;) { /* iteration */
+      itx++; it++; j = it;
+      orc_d_mlt(n, 1.0, dg, r, 0.0, z[j]);
+      orc_spmm(a, 1.0, z[j], 0.0, c[1]);
+      for (int i = 1; i <= j - 1; i++) {
+        h[i][j] = orc_d_dot(n, cs[i], c[i]);
+        orc_d_axpby(n, 1.0, c[i], 0.0, c[i + 1]);
+        orc_d_axpby(n, -h[i][j], cs[i], 1.0, c[i + 1]);
+      }
+      h[j][j] = orc_d_nrm2(n, c[j]);
+      orc_d_axpby(n, 1.0 / h[j][j], c[j], 0.0, cs[j]);
+      alpha[j] = orc_d_dot(n, cs[j], r);
+      orc_d_axpby(n, -alpha[j], cs[j], 1.0, r);
+      conv = orc_check_conv(&s, itx, n, x, r, &errnum, &errden);
+      if (hist) hist[itx] = errden > 0 ? errnum / errden : errnum;
+      if (conv || j >= nl) break;
+    }
+    orc_trsv_upper(j, h, alpha);
+    for (int i = 1; i <= j; i++) orc_d_axpby(n, alpha[i], z[i], 1.0, x);
+    if (conv) break;
+  }
+  *iter = itx;
+  *err = errden > 0 ? errnum / errden : errnum;
+  free(r); free(alpha);
+  orc_free_vecs(z, nl + 2); orc_free_vecs(c, nl + 2); orc_free_vecs(cs, nl + 2);
+  orc_free_vecs(h, nl + 2);
+  return 0;
+}
+
+/* psb_drgmres_vect (krylov/psb_drgmres.f90:247-475), right DIAG
+   preconditioning, restart nl, istop 2 or 3 (the reference builds x at
+   convergence only for istop 1 and 2; here always). */
+static int orc_drgmres(const orc_mat_t *a, const double *dg, const double *b, double *x,
+                       double eps, int itmax, int istop, double ani, int nl,
+                       int *iter, double *err, double *hist)
+{
+  (void)ani;
+  int n = a->nr;
+  double **v = orc_vecs(nl + 2, n), **h = orc_vecs(nl + 2, nl + 2);
+  double *w = calloc(n > 0 ? n : 1, sizeof(double)), *w1 = calloc(n > 0 ? n : 1, sizeof(double));
+  double *c = calloc(nl + 2, sizeof(double)), *sn = calloc(nl + 2, sizeof(double));
+  double *rs = calloc(nl + 2, sizeof(double));
+  double bn2 = 0, r0n2 = 0, errnum = 0, errden = 1;
+  if (istop == 2) bn2 = orc_d_nrm2(n, b);
+  if (istop == 3) {
+    orc_d_axpby(n, 1.0, b, 0.0, v[1]);
+    orc_spmm(a, -1.0, x, 1.0, v[1]);
+    r0n2 = orc_d_nrm2(n, v[1]);
+  }
+  int itx = 0, done = 0;
+  for (;;) { /* restart */
+    orc_d_axpby(n, 1.0, b, 0.0, v[1]);
+    orc_spmm(a, -1.0, x, 1.0, v[1]);
+    rs[1] = orc_d_nrm2(n, v[1]);
+    for (int k = 2; k < nl + 2; k++) rs[k] = 0.0;
+    double scal = 1.0 / rs[1];
+    errnum = orc_d_nrm2(n, v[1]);
+    errden = istop == 2 ? bn2 : r0n2;
+    if (errnum <= eps * errden) break;
+    for (int i = 0; i < n; i++) v[1][i] = v[1][i] * scal;
+    if (itx >= itmax) break;
+    int i;
+    for (i = 1; i <= nl; i++) { /* inner */
+      itx++;
+      orc_d_mlt(n, 1.0, dg, v[i], 0.0, w1);
+      orc_spmm(a, 1.0, w1, 0.0, w);
+      for (int k = 1; k <= i; k++) {
+        h[k][i] = orc_d_dot(n, v[k], w);
+        orc_d_axpby(n, -h[k][i], v[k], 1.0, w);
+      }
+      h[i + 1][i] = orc_d_nrm2(n, w);
+      orc_d_axpby(n, 1.0 / h[i + 1][i], w, 0.0, v[i + 1]);
+      for (int k = 2; k <= i; k++) orc_drot(&h[k - 1][i], &h[k][i], c[k - 1], sn[k - 1]);
+      double rti = h[i][i], rti1 = h[i + 1][i];
+      orc_drotg(&rti, &rti1, &c[i], &sn[i]);
+      orc_drot(&h[i][i], &h[i + 1][i], c[i], sn[i]);
+      h[i + 1][i] = 0.0;
+      orc_drot(&rs[i], &rs[i + 1], c[i], sn[i]);
+      errnum = fabs(rs[i + 1]);
+      if (hist) hist[itx] = errden > 0 ? errnum / errden : errnum;
+      if (errnum <= eps * errden) { done = 1; break; }
+    }
+    int m = done ? i : nl;
+    orc_trsv_upper(m, h, rs);
+    memset(w1, 0, sizeof(double) * n);
+    for (int k = 1; k <= m; k++) orc_d_axpby(n, rs[k], v[k], 1.0, w1);
+    orc_d_mlt(n, 1.0, dg, w1, 0.0, w);
+    orc_d_axpby(n, 1.0, w, 1.0, x);
+    if (done) break;
+  }
+  *iter = itx;
+  *err = errden > 0 ? errnum / errden : errnum;
+  orc_free_vecs(v, nl + 2); orc_free_vecs(h, nl + 2);
+  free(w); free(w1); free(c); free(sn); free(rs);
+  return 0;
+}
+
+/* psb_dcgstabl_vect (krylov/psb_dcgstabl.f90:249-395): BiCGSTAB(l), left DIAG
+   preconditioning (the stop test sees the preconditioned residual). */
+static int orc_dcgstabl(const orc_mat_t *a, const double *dg, const double *b, double *x,
+                        double eps, int itmax, int istop, double ani, int nl,
+                        int *iter, double *err, double *hist)
+{
+  int n = a->nr;
+  double *r = calloc(n > 0 ? n : 1, sizeof(double)), *rt0 = calloc(n > 0 ? n : 1, sizeof(double));
+  double **rh = orc_vecs(nl + 1, n), **uh = orc_vecs(nl + 1, n);
+  double **taum = orc_vecs(nl + 1, nl + 1);
+  double *gamma = calloc(nl + 1, sizeof(double)), *gamma1 = calloc(nl + 1, sizeof(double));
+  double *gamma2 = calloc(nl + 1, sizeof(double)), *sigma = calloc(nl + 1, sizeof(double));
+  orc_conv_t s = {istop, eps, 0, 0, ani, 0, itmax};
+  orc_init_conv(&s, a, x, b);
+  double errnum = 0, errden = 1, rho, rho_old, alpha, omega, beta;
+  int itx = 0, done = 0;
+  for (;;) { /* restart */
+    if (itx >= itmax) break;
+    orc_d_axpby(n, 1.0, b, 0.0, r);
+    orc_spmm(a, -1.0, x, 1.0, r);
+    orc_d_mlt(n, 1.0, dg, r, 0.0, r);
+    orc_d_axpby(n, 1.0, r, 0.0, rt0);
+    orc_d_axpby(n, 1.0, r, 0.0, rh[0]);
+    orc_d_axpby(n, 0.0, r, 0.0, uh[0]);
+    rho = 1.0; alpha = 0.0; omega = 1.0;
+    if (orc_check_conv(&s, itx, n, x, r, &errnum, &errden)) break;
+    for (;;) { /* iteration */
+      itx += nl;
+      rho = -omega * rho;
+      int brk = 0;
+      for (int j = 0; j < nl; j++) {
+        rho_old = rho;
+        rho = orc_d_dot(n, rh[j], rt0);
+        if (rho == 0.0) { brk = 1; break; }
+        beta = alpha * rho / rho_old;
+        rho_old = rho;
+        for (int k = 0; k <= j; k++) orc_d_axpby(n, 1.0, rh[k], -beta, uh[k]);
+        orc_spmm(a, 1.0, uh[j], 0.0, uh[j + 1]);
+        orc_d_mlt(n, 1.0, dg, uh[j + 1], 0.0, uh[j + 1]);
+        gamma[j] = orc_d_dot(n, uh[j + 1], rt0);
+        if (gamma[j] == 0.0) { brk = 1; break; }
+        alpha = rho / gamma[j];
+        for (int k = 0; k <= j; k++) orc_d_axpby(n, -alpha, uh[k + 1], 1.0, rh[k]);
+        orc_d_axpby(n, alpha, uh[0], 1.0, x);
+        orc_spmm(a, 1.0, rh[j], 0.0, rh[j + 1]);
+        orc_d_mlt(n, 1.0, dg, rh[j + 1], 0.0, rh[j + 1]);
+      }
+      if (brk) break;
+      for (int j = 1; j <= nl; j++) {
+        for (int i = 1; i <= j - 1; i++) {
+          taum[i][j] = orc_d_dot(n, rh[i], rh[j]);
+          taum[i][j] = taum[i][j] / sigma[i];
+          orc_d_axpby(n, -taum[i][j], rh[i], 1.0, rh[j]);
+        }
+        sigma[j] = orc_d_dot(n, rh[j], rh[j]);
+        gamma1[j] = orc_d_dot(n, rh[0], rh[j]);
+        gamma1[j] = gamma1[j] / sigma[j];
+      }
+      gamma[nl] = gamma1[nl];
+      omega = gamma[nl];
+      for (int j = nl - 1; j >= 1; j--) {
+        gamma[j] = gamma1[j];
+        for (int i = j + 1; i <= nl; i++) gamma[j] = gamma[j] - taum[j][i] * gamma[i];
+      }
+      for (int j = 1; j <= nl - 1; j++) {
+        gamma2[j] = gamma[j + 1];
+        for (int i = j + 1; i <= nl - 1; i++) gamma2[j] = gamma2[j] + taum[j][i] * gamma[i + 1];
+      }
+      orc_d_axpby(n, gamma[1], rh[0], 1.0, x);
+      orc_d_axpby(n, -gamma1[nl], rh[nl], 1.0, rh[0]);
+      orc_d_axpby(n, -gamma[nl], uh[nl], 1.0, uh[0]);
+      for (int j = 1; j <= nl - 1; j++) {
+        orc_d_axpby(n, -gamma[j], uh[j], 1.0, uh[0]);
+        orc_d_axpby(n, gamma2[j], rh[j], 1.0, x);
+        orc_d_axpby(n, -gamma1[j], rh[j], 1.0, rh[0]);
+      }
+      int conv = orc_check_conv(&s, itx, n, x, rh[0], &errnum, &errden);
+      if (hist) hist[itx] = errden > 0 ? errnum / errden : errnum;
+      if (conv) { done = 1; break; }
+    }
+    if (done) break;
+  }
+  *iter = itx;
+  *err = errden > 0 ? errnum / errden : errnum;
+  free(r); free(rt0); free(gamma); free(gamma1); free(gamma2); free(sigma);
+  orc_free_vecs(rh, nl + 1); orc_free_vecs(uh, nl + 1); orc_free_vecs(taum, nl + 1);
+  return 0;
+}
+
+/* psb_dkrylov_vect with the further methods (krylov/psb_dkrylov.f90:178-206):
+   2 FCG, 3 GCR, 4 RGMRES, 5 BiCGSTAB(l); irst = restart length / l. */
+EXPORT int orc_d_krylov2_csr(int method, int irst, int nr, const int *irp, const int *ja,
+                             const double *val, const double *d, const double *b, double *x,
+                             double eps, int itmax, int istop, double ani, int *iter,
+                             double *err, double *hist)
+{
+  orc_mat_t a = {0, nr, irp, ja, val, 0, NULL, NULL, NULL, NULL};
+  switch (method) {
+    case 2: return orc_dfcg(&a, d, b, x, eps, itmax, istop, ani, iter, err, hist);
+    case 3: return orc_dgcr(&a, d, b, x, eps, itmax, istop, ani, irst, iter, err, hist);
+    case 4: return orc_drgmres(&a, d, b, x, eps, itmax, istop, ani, irst, iter, err, hist);
+    case 5: return orc_dcgstabl(&a, d, b, x, eps, itmax, istop, ani, irst, iter, err, hist);
+    default: return orc_d_krylov_csr(method, nr, irp, ja, val, d, b, x, eps, itmax, istop, ani,
+                                     iter, err, hist);
+  }
+}
+
 /* psb_dkrylov_vect dispatcher (krylov/psb_dkrylov.f90:81-227): method 0 =
    CG, anything else = BiCGSTAB (the reference's default branch). */
 EXPORT int orc_d_krylov_csr(int method, int nr, const int *irp, const int *ja, const double *val,

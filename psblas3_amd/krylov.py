@@ -1,6 +1,8 @@
 """Krylov solvers over the device plug-ins: psb_krylov (krylov/psb_dkrylov.f90:
-81-227), CG (krylov/psb_dcg.F90:98-333) and BiCGSTAB (krylov/psb_dcgstab.f90:
-96-393) with the stopping tests of krylov/psb_d_krylov_conv_mod.f90:205-356.
+81-227), CG (krylov/psb_dcg.F90:98-333), BiCGSTAB (krylov/psb_dcgstab.f90:
+96-393), FCG (krylov/psb_dfcg.F90), GCR (krylov/psb_dgcr.f90), RGMRES
+(krylov/psb_drgmres.f90) and BiCGSTAB(l) (krylov/psb_dcgstabl.f90) with the
+stopping tests of krylov/psb_d_krylov_conv_mod.f90:205-356.
 
 These routines issue exactly the reference's sequence of psb_spmm /
 psb_geaxpby / psb_gedot / psb_genrm2 / prec%apply calls, each running on the
@@ -8,13 +10,15 @@ device; every scalar (dot, nrm2) returns to the host as in the reference.
 psb_krylov_fused and psb_krylov_mixed below call the device-resident
 solvers of libpsb_hip (one synchronisation per several iterations).
 """
+import math
 from dataclasses import dataclass, field
 from typing import List
 
 import numpy as np
 
 from . import _native as N
-from .psblas import (psb_geall, psb_geamax, psb_geaxpby, psb_gedot, psb_genrm2, psb_spmm)
+from .psblas import (psb_geall, psb_geamax, psb_geaxpby, psb_gedot, psb_genrm2, psb_gescal,
+                     psb_spmm)
 from .vect import psb_d_vect_type
 
 
@@ -220,16 +224,345 @@ def psb_dcgstab_vect(a, prec, b, x, eps, desc_a, itmax=1000, itrace=0, istop=2):
     return it_, err, s.history
 
 
+# ------------------------------------------------------------------ host scalars
+def _trsv_upper(m, h, y):
+    """dtrsm('l','u','n','n', m, 1, 1.0, H, ., y, .) in the reference BLAS loop
+    order (column-oriented back substitution), 1-based h[i][k], y[k]."""
+    for k in range(m, 0, -1):
+        if y[k] != 0.0:
+            y[k] = y[k] / h[k][k]
+            for i in range(1, k):
+                y[i] = y[i] - y[k] * h[i][k]
+
+
+def _drotg(a, b):
+    """Reference BLAS drotg: returns (c, s)."""
+    roe = a if abs(a) > abs(b) else b
+    scale = abs(a) + abs(b)
+    if scale == 0.0:
+        return 1.0, 0.0
+    r = scale * math.sqrt((a / scale) ** 2 + (b / scale) ** 2)
+    if roe < 0.0:
+        r = -r
+    return a / r, b / r
+
+
+def _drot(x, y, c, s):
+    """Reference BLAS drot on one pair: returns (c x + s y, c y - s x)."""
+    return c * x + s * y, c * y - s * x
+
+
+def _zero(v, desc_a):
+    psb_geaxpby(0.0, v, 0.0, v, desc_a)
+
+
+# ------------------------------------------------------------------ more methods
+def psb_dfcg_vect(a, prec, b, x, eps, desc_a, itmax=1000, itrace=0, istop=2):
+    """psb_dfcg_vect (krylov/psb_dfcg.F90:98-348): flexible CG."""
+    methd = "FCG"
+    w, v, d, q, r = (psb_geall(desc_a, mold=x.v) for _ in range(5))
+    s = psb_init_conv(methd, istop, itrace, itmax, a, x, b, eps, desc_a)
+    itx = 0
+    try:
+        done = False
+        while not done:  # restart
+            if itx >= itmax:
+                break
+            psb_geaxpby(1.0, b, 0.0, r, desc_a)
+            psb_spmm(-1.0, a, x, 1.0, r, desc_a)
+            if psb_check_conv(methd, itx, x, r, desc_a, s, record=False):
+                break
+            prec.apply(r, v, desc_a)
+            psb_spmm(1.0, a, v, 0.0, w, desc_a)
+            alpha = psb_gedot(r, v, desc_a)
+            beta = psb_gedot(w, v, desc_a)
+            if beta == 0.0:
+                break
+            psb_geaxpby(1.0, v, 0.0, d, desc_a)
+            psb_geaxpby(1.0, w, 0.0, q, desc_a)
+            delta = beta
+            theta = alpha / delta
+            psb_geaxpby(theta, d, 1.0, x, desc_a)
+            psb_geaxpby(-theta, q, 1.0, r, desc_a)
+            while True:  # iteration
+                itx += 1
+                if psb_check_conv(methd, itx, x, r, desc_a, s):
+                    done = True
+                    break
+                prec.apply(r, v, desc_a)
+                psb_spmm(1.0, a, v, 0.0, w, desc_a)
+                alpha = psb_gedot(r, v, desc_a)
+                beta = psb_gedot(w, v, desc_a)
+                gamma = psb_gedot(q, v, desc_a)
+                theta = gamma / delta
+                psb_geaxpby(1.0, v, -theta, d, desc_a)
+                psb_geaxpby(1.0, w, -theta, q, desc_a)
+                delta = beta - (gamma * gamma) / delta
+                if delta == 0.0:
+                    break
+                theta = alpha / delta
+                psb_geaxpby(theta, d, 1.0, x, desc_a)
+                psb_geaxpby(-theta, q, 1.0, r, desc_a)
+    finally:
+        for t in (w, v, d, q, r):
+            t.free()
+    it_, err = psb_end_conv(s, itx)
+    return it_, err, s.history
+
+
+def psb_dgcr_vect(a, prec, b, x, eps, desc_a, itmax=1000, itrace=0, istop=2, irst=10):
+    """psb_dgcr_vect (krylov/psb_dgcr.f90:109-370), restart length irst.  The
+    recurrence and stop tests are the reference's; x = x0 + every cycle's
+    correction (the reference zeroes x before the first cycle's correction
+    and again before the final one)."""
+    methd = "GCR"
+    nl = int(irst)
+    if nl <= 0:
+        raise N.PsbError(N.INVALID, "psb_dgcr", f"irst {irst}")
+    r = psb_geall(desc_a, mold=x.v)
+    z = [None] + [psb_geall(desc_a, mold=x.v) for _ in range(nl + 1)]
+    c = [None] + [psb_geall(desc_a, mold=x.v) for _ in range(nl + 1)]
+    cs = [None] + [psb_geall(desc_a, mold=x.v) for _ in range(nl + 1)]
+    s = psb_init_conv(methd, istop, itrace, itmax, a, x, b, eps, desc_a)
+    itx = 0
+    try:
+        while True:  # restart
+            if itx >= itmax:
+                break
+            h = [[0.0] * (nl + 2) for _ in range(nl + 2)]
+            alpha = [0.0] * (nl + 2)
+            it = j = 0
+            conv = False
+            psb_geaxpby(1.0, b, 0.0, r, desc_a)
+            psb_spmm(-1.0, a, x, 1.0, r, desc_a)
+            if psb_check_conv(methd, itx, x, r, desc_a, s, record=False):
+                break
+            while True:  # iteration
+                itx += 1
+                it += 1
+                j = it
+                prec.apply(r, z[j], desc_a)
+                psb_spmm(1.0, a, z[j], 0.0, c[1], desc_a)
+                for i in range(1, j):
+                    h[i][j] = psb_gedot(cs[i], c[i], desc_a)
+                    psb_geaxpby(1.0, c[i], 0.0, c[i + 1], desc_a)
+                    psb_geaxpby(-h[i][j], cs[i], 1.0, c[i + 1], desc_a)
+                h[j][j] = psb_genrm2(c[j], desc_a)
+                psb_geaxpby(1.0 / h[j][j], c[j], 0.0, cs[j], desc_a)
+                alpha[j] = psb_gedot(cs[j], r, desc_a)
+                psb_geaxpby(-alpha[j], cs[j], 1.0, r, desc_a)
+                conv = psb_check_conv(methd, itx, x, r, desc_a, s)
+                if conv or j >= nl:
+                    break
+            _trsv_upper(j, h, alpha)
+            for i in range(1, j + 1):
+                psb_geaxpby(alpha[i], z[i], 1.0, x, desc_a)
+            if conv:
+                break
+    finally:
+        for t in [r] + z[1:] + c[1:] + cs[1:]:
+            t.free()
+    it_, err = psb_end_conv(s, itx)
+    return it_, err, s.history
+
+
+def psb_drgmres_vect(a, prec, b, x, eps, desc_a, itmax=1000, itrace=0, istop=2, irst=10):
+    """psb_drgmres_vect (krylov/psb_drgmres.f90:109-502): restarted GMRES with
+    right preconditioning, Givens rotations on the host; istop 2 or 3 (x is
+    updated at convergence for both; the reference omits it for istop 3)."""
+    methd = "RGMRES"
+    nl = int(irst)
+    if nl <= 0:
+        raise N.PsbError(N.INVALID, "psb_drgmres", f"irst {irst}")
+    if istop not in (2, 3):
+        raise N.PsbError(N.UNSUPPORTED, "psb_drgmres", f"istop {istop}")
+    v = [None] + [psb_geall(desc_a, mold=x.v) for _ in range(nl + 1)]
+    w = psb_geall(desc_a, mold=x.v)
+    w1 = psb_geall(desc_a, mold=x.v)
+    hist = []
+    itx = 0
+    errnum, errden = 0.0, 1.0
+    try:
+        bn2 = r0n2 = 0.0
+        if istop == 2:
+            bn2 = psb_genrm2(b, desc_a)
+        else:
+            psb_geaxpby(1.0, b, 0.0, v[1], desc_a)
+            psb_spmm(-1.0, a, x, 1.0, v[1], desc_a)
+            r0n2 = psb_genrm2(v[1], desc_a)
+        while True:  # restart
+            psb_geaxpby(1.0, b, 0.0, v[1], desc_a)
+            psb_spmm(-1.0, a, x, 1.0, v[1], desc_a)
+            rs = [0.0] * (nl + 2)
+            rs[1] = psb_genrm2(v[1], desc_a)
+            scal = 1.0 / rs[1] if rs[1] != 0.0 else math.inf
+            errnum = rs[1]
+            errden = bn2 if istop == 2 else r0n2
+            if errnum <= eps * errden:
+                break
+            psb_gescal(scal, v[1], desc_a)
+            if itx >= itmax:
+                break
+            h = [[0.0] * (nl + 2) for _ in range(nl + 2)]
+            cc = [0.0] * (nl + 2)
+            sn = [0.0] * (nl + 2)
+            done = False
+            i = 0
+            for i in range(1, nl + 1):  # inner
+                itx += 1
+                prec.apply(v[i], w1, desc_a)
+                psb_spmm(1.0, a, w1, 0.0, w, desc_a)
+                for k in range(1, i + 1):
+                    h[k][i] = psb_gedot(v[k], w, desc_a)
+                    psb_geaxpby(-h[k][i], v[k], 1.0, w, desc_a)
+                h[i + 1][i] = psb_genrm2(w, desc_a)
+                psb_geaxpby(1.0 / h[i + 1][i], w, 0.0, v[i + 1], desc_a)
+                for k in range(2, i + 1):
+                    h[k - 1][i], h[k][i] = _drot(h[k - 1][i], h[k][i], cc[k - 1], sn[k - 1])
+                cc[i], sn[i] = _drotg(h[i][i], h[i + 1][i])
+                h[i][i], h[i + 1][i] = _drot(h[i][i], h[i + 1][i], cc[i], sn[i])
+                h[i + 1][i] = 0.0
+                rs[i], rs[i + 1] = _drot(rs[i], rs[i + 1], cc[i], sn[i])
+                errnum = abs(rs[i + 1])
+                hist.append(errnum / errden if errden > 0 else errnum)
+                if itrace > 0 and itx % itrace == 0:
+                    print(f"{methd + ':':18s} {itx:9d} {errnum / errden:20.6e} {eps:20.6e}")
+                if errnum <= eps * errden:
+                    done = True
+                    break
+            m = i if done else nl
+            _trsv_upper(m, h, rs)
+            _zero(w1, desc_a)
+            for k in range(1, m + 1):
+                psb_geaxpby(rs[k], v[k], 1.0, w1, desc_a)
+            prec.apply(w1, w, desc_a)
+            psb_geaxpby(1.0, w, 1.0, x, desc_a)
+            if done:
+                break
+    finally:
+        for t in v[1:] + [w, w1]:
+            t.free()
+    return itx, (errnum / errden if errden > 0 else errnum), hist
+
+
+def psb_dcgstabl_vect(a, prec, b, x, eps, desc_a, itmax=1000, itrace=0, istop=2, irst=1):
+    """psb_dcgstabl_vect (krylov/psb_dcgstabl.f90:106-421): BiCGSTAB(l), l =
+    irst, left preconditioning (the stop test sees the preconditioned
+    residual, as the reference's)."""
+    methd = "BiCGSTAB(L)"
+    nl = int(irst)
+    if nl <= 0:
+        raise N.PsbError(N.INVALID, "psb_dcgstabl", f"irst {irst}")
+    r = psb_geall(desc_a, mold=x.v)
+    rt0 = psb_geall(desc_a, mold=x.v)
+    rh = [psb_geall(desc_a, mold=x.v) for _ in range(nl + 1)]
+    uh = [psb_geall(desc_a, mold=x.v) for _ in range(nl + 1)]
+    s = psb_init_conv(methd, istop, itrace, itmax, a, x, b, eps, desc_a)
+    itx = 0
+    try:
+        done = False
+        while not done:  # restart
+            if itx >= itmax:
+                break
+            psb_geaxpby(1.0, b, 0.0, r, desc_a)
+            psb_spmm(-1.0, a, x, 1.0, r, desc_a)
+            prec.apply(r, r, desc_a)
+            psb_geaxpby(1.0, r, 0.0, rt0, desc_a)
+            psb_geaxpby(1.0, r, 0.0, rh[0], desc_a)
+            psb_geaxpby(0.0, r, 0.0, uh[0], desc_a)
+            rho, alpha, omega = 1.0, 0.0, 1.0
+            if psb_check_conv(methd, itx, x, r, desc_a, s, record=False):
+                break
+            while True:  # iteration
+                itx += nl
+                rho = -omega * rho
+                brk = False
+                gamma = [0.0] * (nl + 1)
+                for j in range(nl):
+                    rho_old = rho
+                    rho = psb_gedot(rh[j], rt0, desc_a)
+                    if rho == 0.0:
+                        brk = True
+                        break
+                    beta = alpha * rho / rho_old
+                    for k in range(j + 1):
+                        psb_geaxpby(1.0, rh[k], -beta, uh[k], desc_a)
+                    psb_spmm(1.0, a, uh[j], 0.0, uh[j + 1], desc_a)
+                    prec.apply(uh[j + 1], uh[j + 1], desc_a)
+                    gamma[j] = psb_gedot(uh[j + 1], rt0, desc_a)
+                    if gamma[j] == 0.0:
+                        brk = True
+                        break
+                    alpha = rho / gamma[j]
+                    for k in range(j + 1):
+                        psb_geaxpby(-alpha, uh[k + 1], 1.0, rh[k], desc_a)
+                    psb_geaxpby(alpha, uh[0], 1.0, x, desc_a)
+                    psb_spmm(1.0, a, rh[j], 0.0, rh[j + 1], desc_a)
+                    prec.apply(rh[j + 1], rh[j + 1], desc_a)
+                if brk:
+                    break
+                taum = [[0.0] * (nl + 1) for _ in range(nl + 1)]
+                sigma = [0.0] * (nl + 1)
+                gamma1 = [0.0] * (nl + 1)
+                gamma2 = [0.0] * (nl + 1)
+                for j in range(1, nl + 1):
+                    for i in range(1, j):
+                        taum[i][j] = psb_gedot(rh[i], rh[j], desc_a)
+                        taum[i][j] = taum[i][j] / sigma[i]
+                        psb_geaxpby(-taum[i][j], rh[i], 1.0, rh[j], desc_a)
+                    sigma[j] = psb_gedot(rh[j], rh[j], desc_a)
+                    gamma1[j] = psb_gedot(rh[0], rh[j], desc_a)
+                    gamma1[j] = gamma1[j] / sigma[j]
+                gamma[nl] = gamma1[nl]
+                omega = gamma[nl]
+                for j in range(nl - 1, 0, -1):
+                    gamma[j] = gamma1[j]
+                    for i in range(j + 1, nl + 1):
+                        gamma[j] = gamma[j] - taum[j][i] * gamma[i]
+                for j in range(1, nl):
+                    gamma2[j] = gamma[j + 1]
+                    for i in range(j + 1, nl):
+                        gamma2[j] = gamma2[j] + taum[j][i] * gamma[i + 1]
+                psb_geaxpby(gamma[1], rh[0], 1.0, x, desc_a)
+                psb_geaxpby(-gamma1[nl], rh[nl], 1.0, rh[0], desc_a)
+                psb_geaxpby(-gamma[nl], uh[nl], 1.0, uh[0], desc_a)
+                for j in range(1, nl):
+                    psb_geaxpby(-gamma[j], uh[j], 1.0, uh[0], desc_a)
+                    psb_geaxpby(gamma2[j], rh[j], 1.0, x, desc_a)
+                    psb_geaxpby(-gamma1[j], rh[j], 1.0, rh[0], desc_a)
+                if psb_check_conv(methd, itx, x, rh[0], desc_a, s):
+                    done = True
+                    break
+    finally:
+        for t in [r, rt0] + rh + uh:
+            t.free()
+    it_, err = psb_end_conv(s, itx)
+    return it_, err, s.history
+
+
 def psb_krylov(method, a, prec, b, x, eps, desc_a, itmax=1000, itrace=0, istop=2,
                irst=None):
-    """psb_krylov dispatcher: 'CG' or 'BICGSTAB' (unknown names fall back to
-    BiCGSTAB as the reference's default branch does).  Returns
+    """psb_krylov dispatcher (krylov/psb_dkrylov.f90:81-227): CG, FCG, GCR,
+    BICGSTAB, RGMRES (or GMRES), BICGSTABL; unknown names fall back to
+    BiCGSTAB as the reference's default branch does.  irst: restart length
+    (GCR, RGMRES; default 10) or l (BICGSTABL; default 1).  Returns
     (iter, err, history); x is updated in place."""
     if not isinstance(x, psb_d_vect_type) or not isinstance(b, psb_d_vect_type):
         raise N.PsbError(N.INVALID, "psb_krylov", "b and x must be psb_d_vect_type")
     m = method.upper()
     if m == "CG":
         return psb_dcg_vect(a, prec, b, x, eps, desc_a, itmax, itrace, istop)
+    if m == "FCG":
+        return psb_dfcg_vect(a, prec, b, x, eps, desc_a, itmax, itrace, istop)
+    if m == "GCR":
+        return psb_dgcr_vect(a, prec, b, x, eps, desc_a, itmax, itrace, istop,
+                             10 if irst is None else irst)
+    if m in ("RGMRES", "GMRES"):
+        return psb_drgmres_vect(a, prec, b, x, eps, desc_a, itmax, itrace, istop,
+                                10 if irst is None else irst)
+    if m == "BICGSTABL":
+        return psb_dcgstabl_vect(a, prec, b, x, eps, desc_a, itmax, itrace, istop,
+                                 1 if irst is None else irst)
     return psb_dcgstab_vect(a, prec, b, x, eps, desc_a, itmax, itrace, istop)
 
 

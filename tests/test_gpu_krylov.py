@@ -247,3 +247,52 @@ def test_dropin_istop_1_and_3(gpu, orc):
         _, ito, _, _ = orc.krylov("CG", csr, bh, eps=1e-8, istop=istop, ani=ani)
         assert abs(it - ito) <= 2 and err <= 1e-8
     a.free()
+
+
+# FCG / GCR / RGMRES / BiCGSTAB(l) through psb_krylov on the device, against
+# the oracle's restatements.  The minimal-residual methods (GCR, RGMRES) and
+# FCG on an SPD matrix are smooth in rounding: +-2 iterations.  BiCGSTAB(l)
+# counts advance by l; its window is the oracle's count +- max(2 l, 15%).
+NEW_CASES = [("FCG", None, "poisson", 14), ("GCR", 8, "convdiff", 12),
+             ("RGMRES", 8, "convdiff", 12), ("RGMRES", 30, "poisson", 14),
+             ("GCR", 30, "convdiff", 10), ("BICGSTABL", 2, "convdiff", 12),
+             ("BICGSTABL", 4, "poisson", 14)]
+
+
+@pytest.mark.parametrize("method,irst,kind,L", NEW_CASES)
+def test_more_krylov_methods(gpu, orc, method, irst, kind, L):
+    print(f"case psb_krylov {method} irst={irst} {kind} L={L}", flush=True)
+    coo = orc.gen_poisson3d(L) if kind == "poisson" else orc.gen_convdiff(L, conv=4.0 / np.sqrt(3.0))
+    csr = orc.csr_from_coo(orc.fix_coo(coo))
+    xt = orc.uniform_pm1(12, coo.nr)
+    bh = orc.csr_spmv(*csr, xt)
+    a, desc, prec = _setup(coo, P.psb_d_hip_hlg_sparse_mat())
+    b = P.vect_from(bh)
+    x = P.vect_from(np.zeros(coo.nr))
+    it, err, hist = P.psb_krylov(method, a, prec, b, x, 1e-9, desc, itmax=2000, irst=irst)
+    xo, ito, erro, histo = orc.krylov(method, csr, bh, eps=1e-9, itmax=2000, irst=irst)
+    print(f"  device {it} its err {err:.3e} / oracle {ito} its err {erro:.3e}", flush=True)
+    assert err <= 1e-9
+    if method == "BICGSTABL":
+        assert it % irst == 0
+        assert abs(it - ito) <= max(2 * irst, int(0.15 * ito))
+    else:
+        assert abs(it - ito) <= 2
+        k = min(len(hist), len(histo), 10)
+        assert np.allclose(hist[:k], histo[:k], rtol=1e-6)
+    xs = x.get_vect()
+    assert np.linalg.norm(xs - xo) <= 1e-5 * np.linalg.norm(xo)
+    a.free()
+
+
+def test_krylov_unknown_method_falls_back(gpu, orc):
+    print("case psb_krylov unknown method -> BiCGSTAB", flush=True)
+    coo = orc.gen_poisson3d(8)
+    csr = orc.csr_from_coo(orc.fix_coo(coo))
+    bh = orc.csr_spmv(*csr, np.ones(coo.nr))
+    a, desc, prec = _setup(coo, P.psb_d_hip_hlg_sparse_mat())
+    x1, x2 = P.vect_from(np.zeros(coo.nr)), P.vect_from(np.zeros(coo.nr))
+    r1 = P.psb_krylov("NOSUCH", a, prec, P.vect_from(bh), x1, 1e-9, desc)
+    r2 = P.psb_krylov("BICGSTAB", a, prec, P.vect_from(bh), x2, 1e-9, desc)
+    assert r1[0] == r2[0] and np.array_equal(x1.get_vect(), x2.get_vect())
+    a.free()

@@ -101,3 +101,95 @@ def test_bicgstab_rounding_spread_pde3d(orc):
     lo, hi = bicgstab_spread(orc, csr, rhs, ref["eps"])
     assert lo <= ref["iterations"] <= hi
     assert hi - lo > 4
+
+
+# ---------------------------------------------------------------- FCG, GCR, RGMRES, BiCGSTAB(l)
+def _csr_scipy(orc, coo):
+    import scipy.sparse as sp
+    f = orc.fix_coo(coo)
+    irp, ja, val = orc.csr_from_coo(f)
+    return (irp, ja, val), sp.csr_matrix((val, ja - 1, irp - 1), shape=(coo.nr, coo.nc))
+
+
+def test_fcg_equals_cg_on_spd(orc):
+    csr, A = _csr_scipy(orc, orc.gen_poisson3d(10))
+    b = A @ orc.uniform_pm1(4, A.shape[0])
+    x1, it1, e1, h1 = orc.krylov("CG", csr, b, eps=1e-10, itmax=500)
+    x2, it2, e2, h2 = orc.krylov("FCG", csr, b, eps=1e-10, itmax=500)
+    assert abs(it1 - it2) <= 1 and e2 <= 1e-10
+    k = min(len(h1), len(h2)) - 2
+    assert np.allclose(h1[:k], h2[:k], rtol=1e-6)
+    assert np.linalg.norm(x1 - x2) <= 1e-8 * np.linalg.norm(x1)
+
+
+@pytest.mark.parametrize("irst", [3, 10, 40])
+def test_gcr_equals_rgmres(orc, irst):
+    csr, A = _csr_scipy(orc, orc.gen_convdiff(9, conv=6.0 / np.sqrt(3.0)))
+    b = A @ orc.uniform_pm1(5, A.shape[0])
+    xg, itg, eg, hg = orc.krylov("GCR", csr, b, eps=1e-10, itmax=800, irst=irst)
+    xr, itr, er, hr = orc.krylov("RGMRES", csr, b, eps=1e-10, itmax=800, irst=irst)
+    assert itg == itr and eg <= 1e-10 and er <= 1e-10
+    assert np.allclose(hg, hr, rtol=1e-5)
+    for x in (xg, xr):   # the returned x carries every cycle's correction
+        assert np.linalg.norm(b - A @ x) <= 1.01e-10 * np.linalg.norm(b)
+
+
+def test_rgmres_minimal_residual(orc):
+    """Unrestarted GMRES: the k-th residual is the least-squares minimum of
+    ||r0 - A P Q_k y|| over the k-dimensional Krylov space of A P."""
+    csr, A = _csr_scipy(orc, orc.gen_convdiff(6, conv=3.0))
+    n = A.shape[0]
+    b = A @ orc.uniform_pm1(6, n)
+    d = orc.diag_build(*csr)
+    x, it, err, hist = orc.krylov("RGMRES", csr, b, eps=1e-12, itmax=60, irst=60)
+    Ad = A.toarray() @ np.diag(d)
+    r0 = b.copy()
+    Q = np.zeros((n, it + 1))
+    Q[:, 0] = r0 / np.linalg.norm(r0)
+    for k in range(1, it + 1):
+        w = Ad @ Q[:, k - 1]
+        for _ in range(2):           # twice-repeated Gram-Schmidt
+            w -= Q[:, :k] @ (Q[:, :k].T @ w)
+        Q[:, k] = w / np.linalg.norm(w)
+        y, *_ = np.linalg.lstsq(Ad @ Q[:, :k], r0, rcond=None)
+        best = np.linalg.norm(r0 - Ad @ Q[:, :k] @ y) / np.linalg.norm(b)
+        assert abs(hist[k - 1] - best) <= 1e-7 * best + 1e-14, (k, hist[k - 1], best)
+
+
+def test_bicgstabl_1_equals_bicgstab_unpreconditioned(orc):
+    csr, A = _csr_scipy(orc, orc.gen_convdiff(10, conv=4.0 / np.sqrt(3.0)))
+    b = A @ orc.uniform_pm1(7, A.shape[0])
+    ones = np.ones(A.shape[0])
+    _, it1, _, h1 = orc.krylov("BICGSTAB", csr, b, eps=1e-9, itmax=500, d=ones)
+    _, itl, _, hl = orc.krylov("BICGSTABL", csr, b, eps=1e-9, itmax=500, d=ones, irst=1)
+    assert abs(it1 - itl) <= 1
+    k = min(len(h1), len(hl)) - 3
+    assert np.allclose(h1[:k], hl[:k], rtol=1e-4)
+
+
+@pytest.mark.parametrize("l", [2, 4])
+def test_bicgstabl_left_preconditioned_stop(orc, l):
+    csr, A = _csr_scipy(orc, orc.gen_convdiff(11, conv=2.0))
+    n = A.shape[0]
+    xt = orc.uniform_pm1(8, n)
+    b = A @ xt
+    d = orc.diag_build(*csr)
+    x, it, err, hist = orc.krylov("BICGSTABL", csr, b, eps=1e-10, itmax=1000, irst=l)
+    assert it % l == 0 and err <= 1e-10
+    # the reported error is the preconditioned residual over ||b|| (istop 2)
+    pr = np.linalg.norm(d * (b - A @ x)) / np.linalg.norm(b)
+    assert abs(pr - err) <= 0.05 * err + 1e-13
+    # the true residual is ||D^-1|| (about 1/h^2 = 144 x 6) times larger
+    assert np.linalg.norm(x - xt) <= 1e-4 * np.linalg.norm(xt)
+
+
+def test_new_methods_solution(orc):
+    import scipy.sparse.linalg as spl
+    csr, A = _csr_scipy(orc, orc.gen_convdiff(8, conv=5.0 / np.sqrt(3.0)))
+    b = A @ orc.uniform_pm1(9, A.shape[0])
+    xs = spl.spsolve(A.tocsc(), b)
+    # FCG is a CG variant (symmetric positive definite matrices): see above
+    for m, irst, tol in (("GCR", 5, 1e-7), ("RGMRES", 5, 1e-7), ("BICGSTABL", 2, 1e-5)):
+        x, it, err, _ = orc.krylov(m, csr, b, eps=1e-11, itmax=3000, irst=irst)
+        assert err <= 1e-11, m
+        assert np.linalg.norm(x - xs) <= tol * np.linalg.norm(xs), m
