@@ -388,6 +388,27 @@ int psb_hip_krylov_mixed(int method, void *dmat, void *diag, void *b, void *x, d
                          int variant, int *outer_its, int *inner_its, double *err);
 
 /* ===================================================================== */
+/* BJAC / ILU(0) preconditioner -- prec/impl/psb_d_ilu0_fact.f90,           */
+/* prec/impl/psb_d_bjacprec_impl.f90 (the reference's device triangular     */
+/* solve is cuSPARSE SpSV, cuda/fcusparse_fct.h:145-215)                    */
+/* ===================================================================== */
+/* psb_dilu0_fact (ILU_N, upd 'F'): A in CSR (1-based, ascending columns) ->
+   L strict lower (unit diagonal implied), U strict upper scaled by its row's
+   inverse pivot, d inverse pivots; lja/lval and uja/uval need room for nnz.
+   PSB_HIP_INVALID with *bad_row = i when |pivot_i| < 1.1e-16
+   (psb_err_pivot_too_small_).  Host only. */
+int psb_hip_ilu0_fact(int n, const int *irp, const int *ja, const double *val, int *lirp,
+                      int *lja, double *lval, int *uirp, int *uja, double *uval, double *d,
+                      int *bad_row);
+/* the factors on the device, levelled for the triangular solves */
+int psb_hip_ilu_alloc(void **ilu, int n, const int *lirp, const int *lja, const double *lval,
+                      const int *uirp, const int *uja, const double *uval, const double *d);
+int psb_hip_ilu_free(void *ilu);
+int psb_hip_ilu_levels(void *ilu, int *lower_levels, int *upper_levels);
+/* psb_d_bjac_apply_vect (ILU, trans 'N'): y = alpha * U^-1 (d .* L^-1 x) + beta * y */
+int psb_hip_ilu_apply(void *ilu, double alpha, void *x, double beta, void *y);
+
+/* ===================================================================== */
 /* Batched mixed-precision solves -- no counterpart in the reference.       */
 /* Many independent systems (each an HLL double matrix registered once,     */
 /* with its FP32 copy and optional Jacobi diagonal) advance together: a     */

@@ -101,6 +101,19 @@ def _declare(L):
     L.orc_d_krylov2_csr.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p,
                                     _f64p, C.c_double, C.c_int, C.c_int, C.c_double,
                                     C.POINTER(C.c_int), C.POINTER(C.c_double), _f64p]
+    L.orc_dilu0_fact.restype = C.c_int
+    L.orc_dilu0_fact.argtypes = [C.c_int, _i32p, _i32p, _f64p, _i32p, _i32p, _f64p, _i32p, _i32p,
+                                 _f64p, _f64p]
+    L.orc_d_csr_trsv_unit.restype = None
+    L.orc_d_csr_trsv_unit.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
+    L.orc_d_bjac_apply.restype = None
+    L.orc_d_bjac_apply.argtypes = [C.c_int, _i32p, _i32p, _f64p, _i32p, _i32p, _f64p, _f64p, _f64p,
+                                   _f64p]
+    L.orc_d_krylov_ilu_csr.restype = C.c_int
+    L.orc_d_krylov_ilu_csr.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _i32p, _f64p, _i32p,
+                                       _i32p, _f64p, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p,
+                                       C.c_double, C.c_int, C.c_int, C.c_double,
+                                       C.POINTER(C.c_int), C.POINTER(C.c_double), _f64p]
     L.orc_num_threads.restype = C.c_int
 
 
@@ -372,6 +385,67 @@ def krylov(method, a, b, x0=None, eps=1e-9, itmax=1000, istop=2, d=None, ani=0.0
             d = diag_build(irp, ja, val)
         lib().orc_d_krylov_csr(meth, len(irp) - 1, irp, ja, val, d, b, x, eps, itmax, istop, ani,
                                C.byref(it), C.byref(err), hist)
+    return x, it.value, err.value, hist[1:it.value + 1].copy()
+
+
+# ---------------------------------------------------------------- BJAC / ILU(0)
+class ILU0:
+    """ILU(0) factors (psb_dilu0_fact): L strict lower (unit), U strict upper
+    scaled by the row's inverse pivot, d inverse pivots; CSR 1-based."""
+
+    def __init__(self, n, lirp, lja, lval, uirp, uja, uval, d):
+        self.n = n
+        self.lirp, self.lja, self.lval = lirp, lja, lval
+        self.uirp, self.uja, self.uval = uirp, uja, uval
+        self.d = d
+
+
+def ilu0_fact(irp, ja, val):
+    n = len(irp) - 1
+    nz = len(val)
+    lirp = np.zeros(n + 1, np.int32); uirp = np.zeros(n + 1, np.int32)
+    lja = np.zeros(max(nz, 1), np.int32); uja = np.zeros(max(nz, 1), np.int32)
+    lval = np.zeros(max(nz, 1)); uval = np.zeros(max(nz, 1)); d = np.zeros(n)
+    rc = lib().orc_dilu0_fact(n, np.ascontiguousarray(irp, np.int32), np.ascontiguousarray(ja, np.int32),
+                              np.ascontiguousarray(val, np.float64), lirp, lja, lval, uirp, uja, uval, d)
+    if rc < 0:
+        raise ZeroDivisionError(f"ILU(0) pivot of row {-rc} too small")
+    nl, nu = lirp[n] - 1, uirp[n] - 1
+    return ILU0(n, lirp, lja[:nl].copy(), lval[:nl].copy(), uirp, uja[:nu].copy(),
+                uval[:nu].copy(), d)
+
+
+def trsv_unit(irp, ja, val, x, lower=True):
+    y = np.zeros(len(irp) - 1)
+    lib().orc_d_csr_trsv_unit(len(irp) - 1, 1 if lower else 0, irp, ja, val,
+                              np.ascontiguousarray(x, np.float64), y)
+    return y
+
+
+def bjac_apply(f, x):
+    """psb_d_bjac_apply_vect (ILU): y = U^-1 (d .* L^-1 x)."""
+    y = np.zeros(f.n)
+    lib().orc_d_bjac_apply(f.n, f.lirp, f.lja, f.lval, f.uirp, f.uja, f.uval, f.d,
+                           np.ascontiguousarray(x, np.float64), y)
+    return y
+
+
+def krylov_bjac(method, csr, b, x0=None, eps=1e-9, itmax=1000, istop=2, irst=None, f=None):
+    """psb_krylov with the BJAC / ILU(0) preconditioner (factors f, or
+    computed here).  Returns (x, iter, err, hist)."""
+    irp, ja, val = csr
+    if f is None:
+        f = ilu0_fact(irp, ja, val)
+    meth = KRYLOV_METHODS.get(method.upper(), 1)
+    n = len(b)
+    x = np.zeros(n) if x0 is None else np.ascontiguousarray(x0, np.float64).copy()
+    nl = KRYLOV_IRST.get(meth, 0) if irst is None else int(irst)
+    hist = np.zeros(itmax + nl + 2)
+    it = C.c_int(0)
+    err = C.c_double(0)
+    lib().orc_d_krylov_ilu_csr(meth, nl, n, irp, ja, val, f.lirp, f.lja, f.lval, f.uirp, f.uja,
+                               f.uval, f.d, np.ascontiguousarray(b, np.float64), x, eps, itmax,
+                               istop, 0.0, C.byref(it), C.byref(err), hist)
     return x, it.value, err.value, hist[1:it.value + 1].copy()
 
 

@@ -746,12 +746,124 @@ static int orc_check_conv(orc_conv_t *s, int it, int n, const double *x, const d
   return res || (s->itmax <= it);
 }
 
+/* ------------------------------------------------------------------ */
+/* BJAC with ILU(0) (prec/impl/psb_d_ilu0_fact.f90, psb_d_bjacprec_impl.f90) */
+/* ------------------------------------------------------------------ */
+
+/* psb_dilu0_factint (prec/impl/psb_d_ilu0_fact.f90:330-460), ialg = ILU_N,
+   upd = 'F', no shift, on a CSR matrix with ascending columns.  L: strict
+   lower part (unit diagonal implied), U: strict upper part scaled by the
+   inverse pivot of its row, d: inverse pivots.  lirp/uirp (n+1), lja/lval
+   and uja/uval with room for nnz entries.  Returns 0, or -i (1-based row)
+   when the pivot of row i is below 1.1e-16 in magnitude
+   (psb_err_pivot_too_small_). */
+EXPORT int orc_dilu0_fact(int n, const int *irp, const int *ja, const double *val,
+                          int *lirp, int *lja, double *lval, int *uirp, int *uja, double *uval,
+                          double *d)
+{
+  int l1 = 0, l2 = 0;
+  lirp[0] = 1;
+  uirp[0] = 1;
+  for (int i = 1; i <= n; i++) {
+    d[i - 1] = 0.0;
+    /* ilu_copyin: L part (k < i), diagonal, U part (k > i) */
+    for (int j = irp[i - 1]; j < irp[i]; j++) {
+      int k = ja[j - 1];
+      if (k < i && k >= 1) { l1++; lval[l1 - 1] = val[j - 1]; lja[l1 - 1] = k; }
+      else if (k == i) d[i - 1] = val[j - 1];
+      else if (k > i && k <= n) { l2++; uval[l2 - 1] = val[j - 1]; uja[l2 - 1] = k; }
+    }
+    lirp[i] = l1 + 1;
+    uirp[i] = l2 + 1;
+    double dia = d[i - 1];
+    for (int kk = lirp[i - 1]; kk < lirp[i]; kk++) {
+      double temp = lval[kk - 1];
+      int k = lja[kk - 1];
+      lval[kk - 1] = temp * d[k - 1];
+      int low1 = kk + 1, low2 = uirp[i - 1];
+      for (int jj = uirp[k - 1]; jj < uirp[k]; jj++) {
+        int j = uja[jj - 1];
+        if (j < i) {
+          for (int ll = low1; ll < lirp[i]; ll++) {
+            int l = lja[ll - 1];
+            if (l > j) { low1 = ll; break; }
+            if (l == j) { lval[ll - 1] = lval[ll - 1] - temp * uval[jj - 1]; low1 = ll + 1; break; }
+          }
+        } else if (j == i) {
+          dia = dia - temp * uval[jj - 1];
+        } else {
+          for (int ll = low2; ll < uirp[i]; ll++) {
+            int l = uja[ll - 1];
+            if (l > j) { low2 = ll; break; }
+            if (l == j) { uval[ll - 1] = uval[ll - 1] - temp * uval[jj - 1]; low2 = ll + 1; break; }
+          }
+        }
+      }
+    }
+    if (fabs(dia) < 1.1e-16) return -i;
+    dia = 1.0 / dia;
+    d[i - 1] = dia;
+    for (int kk = uirp[i - 1]; kk < uirp[i]; kk++) uval[kk - 1] = uval[kk - 1] * dia;
+  }
+  return 0;
+}
+
+/* inner_csrsv (base/serial/impl/psb_d_csr_impl.F90:890-941), no transpose,
+   unit diagonal: lower (rows ascending) or upper (rows descending),
+   y(i) = x(i) - sum_j val(j) y(ja(j)) accumulated in row order. */
+EXPORT void orc_d_csr_trsv_unit(int n, int lower, const int *irp, const int *ja,
+                                const double *val, const double *x, double *y)
+{
+  for (int t = 0; t < n; t++) {
+    int i = lower ? t : n - 1 - t;
+    double acc = 0.0;
+    for (int j = irp[i] - 1; j < irp[i + 1] - 1; j++) acc = acc + val[j] * y[ja[j] - 1];
+    y[i] = x[i] - acc;
+  }
+}
+
+/* The preconditioner of the oracle solvers: DIAG (y = d .* x,
+   psb_d_diag_apply_vect) or, when lirp is set, BJAC with ILU(0) factors
+   (psb_d_bjac_apply_vect, prec/impl/psb_d_bjacprec_impl.f90:166-173):
+   w = L^-1 x scaled by d (psb_spsm scale 'L'), y = U^-1 w.  x may alias y. */
+typedef struct {
+  const double *d;
+  int n;
+  const int *lirp, *lja;
+  const double *lval;
+  const int *uirp, *uja;
+  const double *uval;
+  const double *dd;
+  double *tmp;
+} orc_prec_t;
+
+static void orc_papply(const orc_prec_t *p, const double *x, double *y)
+{
+  if (!p->lirp) {
+    orc_d_mlt(p->n, 1.0, p->d, x, 0.0, y);
+    return;
+  }
+  orc_d_csr_trsv_unit(p->n, 1, p->lirp, p->lja, p->lval, x, p->tmp);
+  for (int i = 0; i < p->n; i++) p->tmp[i] = p->tmp[i] * p->dd[i];
+  orc_d_csr_trsv_unit(p->n, 0, p->uirp, p->uja, p->uval, p->tmp, y);
+}
+
+EXPORT void orc_d_bjac_apply(int n, const int *lirp, const int *lja, const double *lval,
+                             const int *uirp, const int *uja, const double *uval,
+                             const double *dd, const double *x, double *y)
+{
+  double *tmp = calloc(n > 0 ? n : 1, sizeof(double));
+  orc_prec_t p = {NULL, n, lirp, lja, lval, uirp, uja, uval, dd, tmp};
+  orc_papply(&p, x, y);
+  free(tmp);
+}
+
 /*
  * psb_dcg_vect (krylov/psb_dcg.F90:98-333) with the DIAG preconditioner
  * applied as z = d .* r (psb_d_diag_apply_vect, psb_d_diagprec_impl.f90:116).
  * hist (optional, length itmax+1) receives errnum/errden per iteration.
  */
-static int orc_dcg(const orc_mat_t *a, const double *d, const double *b, double *x,
+static int orc_dcg(const orc_mat_t *a, const orc_prec_t *pc, const double *b, double *x,
                    double eps, int itmax, int istop, double ani,
                    int *iter, double *err, double *hist)
 {
@@ -773,7 +885,7 @@ static int orc_dcg(const orc_mat_t *a, const double *d, const double *b, double 
     if (istop == 3) s.r0n2 = orc_d_nrm2(n, r);
     for (;;) { /* iteration */
       it++; itx++;
-      orc_d_mlt(n, 1.0, d, r, 0.0, z);
+      orc_papply(pc, r, z);
       rho_old = rho;
       rho = orc_d_dot(n, r, z);
       if (it == 1) orc_d_axpby(n, 1.0, z, 0.0, p);
@@ -802,7 +914,7 @@ static int orc_dcg(const orc_mat_t *a, const double *d, const double *b, double 
 }
 
 /* psb_dcgstab_vect (krylov/psb_dcgstab.f90:96-393), DIAG preconditioner. */
-static int orc_dcgstab(const orc_mat_t *a, const double *d, const double *b, double *x,
+static int orc_dcgstab(const orc_mat_t *a, const orc_prec_t *pc, const double *b, double *x,
                        double eps, int itmax, int istop, double ani,
                        int *iter, double *err, double *hist)
 {
@@ -841,14 +953,14 @@ static int orc_dcgstab(const orc_mat_t *a, const double *d, const double *b, dou
         orc_d_axpby(n, -omega, v, 1.0, p);
         orc_d_axpby(n, 1.0, r, beta, p);
       }
-      orc_d_mlt(n, 1.0, d, p, 0.0, f);
+      orc_papply(pc, p, f);
       orc_spmm(a, 1.0, f, 0.0, v);
       sigma = orc_d_dot(n, q, v);
       if (sigma == 0.0) break;
       alpha = rho / sigma;
       orc_d_axpby(n, 1.0, r, 0.0, s);
       orc_d_axpby(n, -alpha, v, 1.0, s);
-      orc_d_mlt(n, 1.0, d, s, 0.0, z);
+      orc_papply(pc, s, z);
       orc_spmm(a, 1.0, z, 0.0, t);
       sigma = orc_d_dot(n, t, t);
       if (sigma == 0.0) break;
@@ -946,7 +1058,7 @@ static void orc_drot(double *x, double *y, double c, double s)
 }
 
 /* psb_dfcg_vect (krylov/psb_dfcg.F90:150-336), DIAG preconditioner. */
-static int orc_dfcg(const orc_mat_t *a, const double *dg, const double *b, double *x,
+static int orc_dfcg(const orc_mat_t *a, const orc_prec_t *pc, const double *b, double *x,
                     double eps, int itmax, int istop, double ani,
                     int *iter, double *err, double *hist)
 {
@@ -962,7 +1074,7 @@ static int orc_dfcg(const orc_mat_t *a, const double *dg, const double *b, doubl
     orc_d_axpby(n, 1.0, b, 0.0, r);
     orc_spmm(a, -1.0, x, 1.0, r);
     if (orc_check_conv(&s, itx, n, x, r, &errnum, &errden)) break;
-    orc_d_mlt(n, 1.0, dg, r, 0.0, v);
+    orc_papply(pc, r, v);
     orc_spmm(a, 1.0, v, 0.0, w);
     alpha = orc_d_dot(n, r, v);
     beta = orc_d_dot(n, w, v);
@@ -978,7 +1090,7 @@ static int orc_dfcg(const orc_mat_t *a, const double *dg, const double *b, doubl
       int conv = orc_check_conv(&s, itx, n, x, r, &errnum, &errden);
       if (hist) hist[itx] = errden > 0 ? errnum / errden : errnum;
       if (conv) { done = 1; break; }
-      orc_d_mlt(n, 1.0, dg, r, 0.0, v);
+      orc_papply(pc, r, v);
       orc_spmm(a, 1.0, v, 0.0, w);
       alpha = orc_d_dot(n, r, v);
       beta = orc_d_dot(n, w, v);
@@ -1006,7 +1118,7 @@ static int orc_dfcg(const orc_mat_t *a, const double *dg, const double *b, doubl
    correction (the reference sets x = 0 before the first cycle's
    correction and again before the last, which drops x0 and the earlier
    cycles whenever there is more than one). */
-static int orc_dgcr(const orc_mat_t *a, const double *dg, const double *b, double *x,
+static int orc_dgcr(const orc_mat_t *a, const orc_prec_t *pc, const double *b, double *x,
                     double eps, int itmax, int istop, double ani, int nl,
                     int *iter, double *err, double *hist)
 {
@@ -1028,7 +1140,7 @@ static int orc_dgcr(const orc_mat_t *a, const double *dg, const double *b, doubl
     if (orc_check_conv(&s, itx, n, x, r, &errnum, &errden)) break;
     for (;;) { /* iteration */
       itx++; it++; j = it;
-      orc_d_mlt(n, 1.0, dg, r, 0.0, z[j]);
+      orc_papply(pc, r, z[j]);
       orc_spmm(a, 1.0, z[j], 0.0, c[1]);
       for (int i = 1; i <= j - 1; i++) {
         h[i][j] = orc_d_dot(n, cs[i], c[i]);
@@ -1058,7 +1170,7 @@ static int orc_dgcr(const orc_mat_t *a, const double *dg, const double *b, doubl
 /* psb_drgmres_vect (krylov/psb_drgmres.f90:247-475), right DIAG
    preconditioning, restart nl, istop 2 or 3 (the reference builds x at
    convergence only for istop 1 and 2; here always). */
-static int orc_drgmres(const orc_mat_t *a, const double *dg, const double *b, double *x,
+static int orc_drgmres(const orc_mat_t *a, const orc_prec_t *pc, const double *b, double *x,
                        double eps, int itmax, int istop, double ani, int nl,
                        int *iter, double *err, double *hist)
 {
@@ -1090,7 +1202,7 @@ static int orc_drgmres(const orc_mat_t *a, const double *dg, const double *b, do
     int i;
     for (i = 1; i <= nl; i++) { /* inner */
       itx++;
-      orc_d_mlt(n, 1.0, dg, v[i], 0.0, w1);
+      orc_papply(pc, v[i], w1);
       orc_spmm(a, 1.0, w1, 0.0, w);
       for (int k = 1; k <= i; k++) {
         h[k][i] = orc_d_dot(n, v[k], w);
@@ -1112,7 +1224,7 @@ static int orc_drgmres(const orc_mat_t *a, const double *dg, const double *b, do
     orc_trsv_upper(m, h, rs);
     memset(w1, 0, sizeof(double) * n);
     for (int k = 1; k <= m; k++) orc_d_axpby(n, rs[k], v[k], 1.0, w1);
-    orc_d_mlt(n, 1.0, dg, w1, 0.0, w);
+    orc_papply(pc, w1, w);
     orc_d_axpby(n, 1.0, w, 1.0, x);
     if (done) break;
   }
@@ -1125,7 +1237,7 @@ static int orc_drgmres(const orc_mat_t *a, const double *dg, const double *b, do
 
 /* psb_dcgstabl_vect (krylov/psb_dcgstabl.f90:249-395): BiCGSTAB(l), left DIAG
    preconditioning (the stop test sees the preconditioned residual). */
-static int orc_dcgstabl(const orc_mat_t *a, const double *dg, const double *b, double *x,
+static int orc_dcgstabl(const orc_mat_t *a, const orc_prec_t *pc, const double *b, double *x,
                         double eps, int itmax, int istop, double ani, int nl,
                         int *iter, double *err, double *hist)
 {
@@ -1143,7 +1255,7 @@ static int orc_dcgstabl(const orc_mat_t *a, const double *dg, const double *b, d
     if (itx >= itmax) break;
     orc_d_axpby(n, 1.0, b, 0.0, r);
     orc_spmm(a, -1.0, x, 1.0, r);
-    orc_d_mlt(n, 1.0, dg, r, 0.0, r);
+    orc_papply(pc, r, r);
     orc_d_axpby(n, 1.0, r, 0.0, rt0);
     orc_d_axpby(n, 1.0, r, 0.0, rh[0]);
     orc_d_axpby(n, 0.0, r, 0.0, uh[0]);
@@ -1161,14 +1273,14 @@ static int orc_dcgstabl(const orc_mat_t *a, const double *dg, const double *b, d
         rho_old = rho;
         for (int k = 0; k <= j; k++) orc_d_axpby(n, 1.0, rh[k], -beta, uh[k]);
         orc_spmm(a, 1.0, uh[j], 0.0, uh[j + 1]);
-        orc_d_mlt(n, 1.0, dg, uh[j + 1], 0.0, uh[j + 1]);
+        orc_papply(pc, uh[j + 1], uh[j + 1]);
         gamma[j] = orc_d_dot(n, uh[j + 1], rt0);
         if (gamma[j] == 0.0) { brk = 1; break; }
         alpha = rho / gamma[j];
         for (int k = 0; k <= j; k++) orc_d_axpby(n, -alpha, uh[k + 1], 1.0, rh[k]);
         orc_d_axpby(n, alpha, uh[0], 1.0, x);
         orc_spmm(a, 1.0, rh[j], 0.0, rh[j + 1]);
-        orc_d_mlt(n, 1.0, dg, rh[j + 1], 0.0, rh[j + 1]);
+        orc_papply(pc, rh[j + 1], rh[j + 1]);
       }
       if (brk) break;
       for (int j = 1; j <= nl; j++) {
@@ -1220,11 +1332,13 @@ EXPORT int orc_d_krylov2_csr(int method, int irst, int nr, const int *irp, const
                              double *err, double *hist)
 {
   orc_mat_t a = {0, nr, irp, ja, val, 0, NULL, NULL, NULL, NULL};
+  orc_prec_t pcv = {d, nr, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL};
+  const orc_prec_t *pc = &pcv;
   switch (method) {
-    case 2: return orc_dfcg(&a, d, b, x, eps, itmax, istop, ani, iter, err, hist);
-    case 3: return orc_dgcr(&a, d, b, x, eps, itmax, istop, ani, irst, iter, err, hist);
-    case 4: return orc_drgmres(&a, d, b, x, eps, itmax, istop, ani, irst, iter, err, hist);
-    case 5: return orc_dcgstabl(&a, d, b, x, eps, itmax, istop, ani, irst, iter, err, hist);
+    case 2: return orc_dfcg(&a, pc, b, x, eps, itmax, istop, ani, iter, err, hist);
+    case 3: return orc_dgcr(&a, pc, b, x, eps, itmax, istop, ani, irst, iter, err, hist);
+    case 4: return orc_drgmres(&a, pc, b, x, eps, itmax, istop, ani, irst, iter, err, hist);
+    case 5: return orc_dcgstabl(&a, pc, b, x, eps, itmax, istop, ani, irst, iter, err, hist);
     default: return orc_d_krylov_csr(method, nr, irp, ja, val, d, b, x, eps, itmax, istop, ani,
                                      iter, err, hist);
   }
@@ -1237,8 +1351,9 @@ EXPORT int orc_d_krylov_csr(int method, int nr, const int *irp, const int *ja, c
                             int istop, double ani, int *iter, double *err, double *hist)
 {
   orc_mat_t a = {0, nr, irp, ja, val, 0, NULL, NULL, NULL, NULL};
-  if (method == 0) return orc_dcg(&a, d, b, x, eps, itmax, istop, ani, iter, err, hist);
-  return orc_dcgstab(&a, d, b, x, eps, itmax, istop, ani, iter, err, hist);
+  orc_prec_t pc = {d, nr, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL};
+  if (method == 0) return orc_dcg(&a, &pc, b, x, eps, itmax, istop, ani, iter, err, hist);
+  return orc_dcgstab(&a, &pc, b, x, eps, itmax, istop, ani, iter, err, hist);
 }
 
 EXPORT int orc_d_krylov_hll(int method, int nr, int hksz, const int *hkoffs, const int *irn,
@@ -1247,8 +1362,35 @@ EXPORT int orc_d_krylov_hll(int method, int nr, int hksz, const int *hkoffs, con
                             int istop, double ani, int *iter, double *err, double *hist)
 {
   orc_mat_t a = {1, nr, NULL, NULL, NULL, hksz, hkoffs, irn, hja, hval};
-  if (method == 0) return orc_dcg(&a, d, b, x, eps, itmax, istop, ani, iter, err, hist);
-  return orc_dcgstab(&a, d, b, x, eps, itmax, istop, ani, iter, err, hist);
+  orc_prec_t pc = {d, nr, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL};
+  if (method == 0) return orc_dcg(&a, &pc, b, x, eps, itmax, istop, ani, iter, err, hist);
+  return orc_dcgstab(&a, &pc, b, x, eps, itmax, istop, ani, iter, err, hist);
+}
+
+/* Any method (0 CG, 1 BiCGSTAB, 2-5 as orc_d_krylov2_csr) with the BJAC /
+   ILU(0) preconditioner of factors from orc_dilu0_fact. */
+EXPORT int orc_d_krylov_ilu_csr(int method, int irst, int nr, const int *irp, const int *ja,
+                                const double *val, const int *lirp, const int *lja,
+                                const double *lval, const int *uirp, const int *uja,
+                                const double *uval, const double *dd, const double *b,
+                                double *x, double eps, int itmax, int istop, double ani,
+                                int *iter, double *err, double *hist)
+{
+  orc_mat_t a = {0, nr, irp, ja, val, 0, NULL, NULL, NULL, NULL};
+  double *tmp = calloc(nr > 0 ? nr : 1, sizeof(double));
+  orc_prec_t pcv = {NULL, nr, lirp, lja, lval, uirp, uja, uval, dd, tmp};
+  const orc_prec_t *pc = &pcv;
+  int rc;
+  switch (method) {
+    case 0: rc = orc_dcg(&a, pc, b, x, eps, itmax, istop, ani, iter, err, hist); break;
+    case 2: rc = orc_dfcg(&a, pc, b, x, eps, itmax, istop, ani, iter, err, hist); break;
+    case 3: rc = orc_dgcr(&a, pc, b, x, eps, itmax, istop, ani, irst, iter, err, hist); break;
+    case 4: rc = orc_drgmres(&a, pc, b, x, eps, itmax, istop, ani, irst, iter, err, hist); break;
+    case 5: rc = orc_dcgstabl(&a, pc, b, x, eps, itmax, istop, ani, irst, iter, err, hist); break;
+    default: rc = orc_dcgstab(&a, pc, b, x, eps, itmax, istop, ani, iter, err, hist); break;
+  }
+  free(tmp);
+  return rc;
 }
 
 EXPORT int orc_num_threads(void)

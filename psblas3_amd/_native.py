@@ -148,6 +148,11 @@ _SIGS = {
     "psb_hip_mm_read_vec": (i32, [C.c_char_p, i64, pi64, vp]),
     "psb_hip_mm_write_coo": (i32, [C.c_char_p, C.c_char_p, i32, i32, i64, vp, vp, vp]),
     "psb_hip_mm_write_vec": (i32, [C.c_char_p, C.c_char_p, i64, vp]),
+    "psb_hip_ilu0_fact": (i32, [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, pi32]),
+    "psb_hip_ilu_alloc": (i32, [pvp, i32, vp, vp, vp, vp, vp, vp, vp]),
+    "psb_hip_ilu_free": (i32, [vp]),
+    "psb_hip_ilu_levels": (i32, [vp, pi32, pi32]),
+    "psb_hip_ilu_apply": (i32, [vp, f64, vp, f64, vp]),
     "psb_hip_batch_create": (i32, [pvp, i32, i32, i64, i32, f64, f64, i32]),
     "psb_hip_batch_free": (i32, [vp]),
     "psb_hip_batch_register": (i32, [vp, vp, vp, pi32]),

@@ -53,6 +53,9 @@ class psb_hip_batch:
         inner = a.a if hasattr(a, "a") else a
         if inner.get_fmt() != "HLG" or inner._dtype != np.float64:
             raise N.PsbError(N.UNSUPPORTED, "psb_hip_batch.register", "needs a double HLG matrix")
+        if prec is not None and prec.ptype not in ("DIAG", "NONE"):
+            raise N.PsbError(N.UNSUPPORTED, "psb_hip_batch.register",
+                             f"preconditioner {prec.ptype} (DIAG or NONE)")
         diag = None
         if prec is not None and prec.ptype == "DIAG":
             diag = prec.dv.dev

@@ -569,6 +569,12 @@ def psb_krylov(method, a, prec, b, x, eps, desc_a, itmax=1000, itrace=0, istop=2
 _FMT = {"HLG": 0, "ELG": 1, "CSRG": 2}
 
 
+def _diag_only(prec, name):
+    """The device-resident solvers apply DIAG (or no) preconditioning."""
+    if prec is not None and prec.ptype not in ("DIAG", "NONE"):
+        raise N.PsbError(N.UNSUPPORTED, name, f"preconditioner {prec.ptype} (DIAG or NONE)")
+
+
 def psb_krylov_fused(method, a, prec, b, x, eps, desc_a=None, itmax=1000, istop=2,
                      check_every=8, history=False):
     """Device-resident CG / BiCGSTAB (same recurrences as psb_krylov, scalars
@@ -580,6 +586,7 @@ def psb_krylov_fused(method, a, prec, b, x, eps, desc_a=None, itmax=1000, istop=
         raise N.PsbError(N.INVALID, "psb_krylov_fused", "b and x must be psb_d_vect_type")
     b.v.ensure_dev()
     x.v.ensure_dev()
+    _diag_only(prec, "psb_krylov_fused")
     diag = None
     if prec is not None and prec.ptype == "DIAG":
         diag = prec.dv.dev
@@ -609,6 +616,7 @@ def psb_krylov_mixed(method, a, prec, b, x, eps, desc_a=None, outer_itmax=50, in
         a.a._has_single = True
     b.v.ensure_dev()
     x.v.ensure_dev()
+    _diag_only(prec, "psb_krylov_mixed")
     diag = prec.dv.dev if (prec is not None and prec.ptype == "DIAG") else None
     oi, ii, err = C.c_int(0), C.c_int(0), C.c_double(0.0)
     meth = 0 if method.upper() == "CG" else 1

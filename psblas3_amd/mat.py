@@ -154,6 +154,18 @@ class psb_d_hip_hlg_sparse_mat(_hip_base_mat):
                irn.ctypes.data, idiag.ctypes.data)
         return hk, irn[:self.nr], ja[:self.isz], val[:self.isz], idiag[:self.nr]
 
+    def to_csr(self):
+        """The stored entries as host CSR (irp, ja, val), 1-based, padding dropped:
+        row r's entries are the first irn(r) slots of its hack column."""
+        hk, irn, ja, val, _ = self.to_host()
+        nr, hs = self.nr, self.hksz
+        rows = np.repeat(np.arange(nr, dtype=np.int64), irn)
+        start = np.concatenate([[0], np.cumsum(irn, dtype=np.int64)[:-1]])
+        j = np.arange(len(rows), dtype=np.int64) - np.repeat(start, irn)
+        pos = hk[rows // hs].astype(np.int64) + j * hs + rows % hs
+        irp = np.concatenate([[1], 1 + np.cumsum(irn, dtype=np.int64)]).astype(np.int32)
+        return irp, ja[pos].astype(np.int32), val[pos].copy()
+
     def vect_mv(self, alpha, x, beta, y):
         """psb_d_cuda_hlg_vect_mv: y = alpha*A*x + beta*y on the device."""
         self._check_vects(x, y)
@@ -232,6 +244,16 @@ class psb_d_hip_elg_sparse_mat(_hip_base_mat):
         n = self.nr * self.nzm
         return irn[:self.nr], ja[:n], val[:n], idiag[:self.nr]
 
+    def to_csr(self):
+        """The stored entries as host CSR (irp, ja, val), padding dropped."""
+        irn, ja, val, _ = self.to_host()
+        rows = np.repeat(np.arange(self.nr, dtype=np.int64), irn)
+        start = np.concatenate([[0], np.cumsum(irn, dtype=np.int64)[:-1]])
+        j = np.arange(len(rows), dtype=np.int64) - np.repeat(start, irn)
+        pos = j * self.nr + rows
+        irp = np.concatenate([[1], 1 + np.cumsum(irn, dtype=np.int64)]).astype(np.int32)
+        return irp, ja[pos].astype(np.int32), val[pos].copy()
+
     def pitch(self):
         return N.fn("psb_hip_ell_pitch")(self.dev)
 
@@ -283,6 +305,9 @@ class psb_d_hip_csrg_sparse_mat(_hip_base_mat):
         val = np.empty(max(self.nnz, 1), self._dtype)
         N.call("psb_hip_csr_read", self.dev, irp.ctypes.data, ja.ctypes.data, val.ctypes.data)
         return irp, ja[:self.nnz], val[:self.nnz]
+
+    def to_csr(self):
+        return self.to_host()
 
     def analysis(self):
         lanes, nlong, thr = C.c_int(), C.c_int(), C.c_int()

@@ -1,12 +1,22 @@
-"""Preconditioners: NONE and DIAG (Jacobi), psb_dprec_type
-(prec/psb_d_prec_type.f90, prec/psb_d_diagprec.f90,
-prec/impl/psb_d_diagprec_impl.f90).
+"""Preconditioners: NONE, DIAG (Jacobi) and BJAC (block Jacobi with ILU(0);
+one block at one process), psb_dprec_type (prec/psb_d_prec_type.f90,
+prec/psb_d_diagprec.f90, prec/psb_d_bjacprec.f90).
 
 DIAG build: d(i) = 1/a_ii with a zero diagonal replaced by 1
 (psb_d_diag_precbld, psb_d_diagprec_impl.f90:229-293), computed on the
 device from the matrix's own diagonal.  Apply: y = alpha*d.*x + beta*y
 through the vector's mlt (:116), i.e. psb_hip_mv_axybz.
+
+BJAC build (psb_d_bjac_precbld, psb_d_bjacprec_impl.f90:490-700, ILU(0),
+the default factorization): the matrix is read back as host CSR and factored
+by psb_hip_ilu0_fact (psb_dilu0_fact); the factors go to the device levelled
+for the triangular solves.  Apply (psb_d_bjac_apply_vect, :166-173):
+y = alpha * U^-1 (d .* L^-1 x) + beta * y on the device (psb_hip_ilu_apply).
 """
+import ctypes as C
+
+import numpy as np
+
 from . import _native as N
 from .vect import psb_d_hip_vect, psb_d_vect_type
 
@@ -14,16 +24,20 @@ from .vect import psb_d_hip_vect, psb_d_vect_type
 class psb_dprec_type:
     def __init__(self, ptype="DIAG"):
         self.ptype = ptype.upper()
-        if self.ptype not in ("DIAG", "NONE", "JACOBI"):
+        if self.ptype not in ("DIAG", "NONE", "JACOBI", "BJAC"):
             raise N.PsbError(N.UNSUPPORTED, "psb_precinit", f"preconditioner {ptype}")
         if self.ptype == "JACOBI":
             self.ptype = "DIAG"
         self.dv = None
+        self.ilu = None
+        self.levels = None
 
     def build(self, a, desc_a=None, vmold=None):
         """psb_precbld(a, desc, prec, info, vmold=)."""
         if self.ptype == "NONE":
             return self
+        if self.ptype == "BJAC":
+            return self._build_ilu0(a)
         mold = vmold or (a.a and _vmold_for(a.a))
         d = mold.mold()
         d.all(a.get_ncols())
@@ -43,15 +57,49 @@ class psb_dprec_type:
         if self.ptype == "NONE":
             # psb_d_null_prec apply: y = alpha*x + beta*y
             N.call("psb_hip_mv_axpby", yv.get_nrows(), float(alpha), xv.dev, float(beta), yv.dev)
+        elif self.ptype == "BJAC":
+            if trans.upper() != "N":
+                raise N.PsbError(N.UNSUPPORTED, "psb_bjac_apply", f"trans {trans}")
+            N.call("psb_hip_ilu_apply", self.ilu, float(alpha), xv.dev, float(beta), yv.dev)
         else:
             N.call("psb_hip_mv_axybz", yv.get_nrows(), float(alpha), self.dv.dev, xv.dev,
                    float(beta), yv.dev)
         yv.set_dev()
 
+    def _build_ilu0(self, a):
+        inner = a.a if hasattr(a, "a") else a
+        if inner._dtype != np.float64:
+            raise N.PsbError(N.UNSUPPORTED, "psb_bjac_precbld", "double matrices")
+        irp, ja, val = inner.to_csr()
+        n, nz = len(irp) - 1, max(len(val), 1)
+        lirp = np.zeros(n + 1, np.int32)
+        uirp = np.zeros(n + 1, np.int32)
+        lja, uja = np.zeros(nz, np.int32), np.zeros(nz, np.int32)
+        lval, uval, d = np.zeros(nz), np.zeros(nz), np.zeros(max(n, 1))
+        bad = C.c_int(0)
+        rc = N.fn("psb_hip_ilu0_fact")(n, irp.ctypes.data, ja.ctypes.data, val.ctypes.data,
+                                       lirp.ctypes.data, lja.ctypes.data, lval.ctypes.data,
+                                       uirp.ctypes.data, uja.ctypes.data, uval.ctypes.data,
+                                       d.ctypes.data, C.byref(bad))
+        if rc != N.SUCCESS:
+            raise N.PsbError(rc, "psb_dilu0_fact", f"pivot of row {bad.value} too small")
+        h = C.c_void_p()
+        N.call("psb_hip_ilu_alloc", C.byref(h), n, lirp.ctypes.data, lja.ctypes.data,
+               lval.ctypes.data, uirp.ctypes.data, uja.ctypes.data, uval.ctypes.data,
+               d.ctypes.data)
+        self.ilu = h
+        lo, up = C.c_int(0), C.c_int(0)
+        N.call("psb_hip_ilu_levels", h, C.byref(lo), C.byref(up))
+        self.levels = (lo.value, up.value)
+        return self
+
     def free(self):
         if self.dv is not None:
             self.dv.free()
             self.dv = None
+        if self.ilu is not None:
+            N.call("psb_hip_ilu_free", self.ilu)
+            self.ilu = None
 
 
 def _vmold_for(a):
