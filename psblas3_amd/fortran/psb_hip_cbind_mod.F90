@@ -269,6 +269,40 @@ module psb_hip_cbind_mod
       integer(c_int) :: res
     end function psb_hip_batch_result
 
+    ! ---------------------------------------------------------------- BJAC / ILU(0)
+    function psb_hip_ilu0_fact(n, irp, ja, val, lirp, lja, lval, uirp, uja, uval, d, &
+         & bad_row) result(res) bind(c, name='psb_hip_ilu0_fact')
+      import :: c_int, c_double
+      integer(c_int), value :: n
+      integer(c_int) :: irp(*), ja(*), lirp(*), lja(*), uirp(*), uja(*), bad_row
+      real(c_double) :: val(*), lval(*), uval(*), d(*)
+      integer(c_int) :: res
+    end function psb_hip_ilu0_fact
+
+    function psb_hip_ilu_alloc(ilu, n, lirp, lja, lval, uirp, uja, uval, d) result(res) &
+         & bind(c, name='psb_hip_ilu_alloc')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr) :: ilu
+      integer(c_int), value :: n
+      integer(c_int) :: lirp(*), lja(*), uirp(*), uja(*)
+      real(c_double) :: lval(*), uval(*), d(*)
+      integer(c_int) :: res
+    end function psb_hip_ilu_alloc
+
+    function psb_hip_ilu_free(ilu) result(res) bind(c, name='psb_hip_ilu_free')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ilu
+      integer(c_int) :: res
+    end function psb_hip_ilu_free
+
+    function psb_hip_ilu_apply(ilu, alpha, x, beta, y) result(res) &
+         & bind(c, name='psb_hip_ilu_apply')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ilu, x, y
+      real(c_double), value :: alpha, beta
+      integer(c_int) :: res
+    end function psb_hip_ilu_apply
+
     function psb_hip_hll_tune(dmat, x, y, reps, best, us) result(res) &
          & bind(c, name='psb_hip_hll_tune')
       import :: c_int, c_ptr, c_float

@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:583f225ed24b9a28
+﻿!mod$ v1 sum:8419c3678168e143
 !need$ 0bde2ac47243ead2 i iso_c_binding
 module psb_hip_cbind_mod
 use,intrinsic::iso_c_binding,only:c_associated
@@ -426,6 +426,56 @@ import::c_ptr
 type(c_ptr),value::batch
 integer(4),value::sys_id
 type(c_ptr),value::x
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_ilu0_fact(n,irp,ja,val,lirp,lja,lval,uirp,uja,uval,d,bad_row) bind(c,name="psb_hip_ilu0_fact") result(res)
+integer(4),value::n
+integer(4)::irp(1_8:*)
+integer(4)::ja(1_8:*)
+real(8)::val(1_8:*)
+integer(4)::lirp(1_8:*)
+integer(4)::lja(1_8:*)
+real(8)::lval(1_8:*)
+integer(4)::uirp(1_8:*)
+integer(4)::uja(1_8:*)
+real(8)::uval(1_8:*)
+real(8)::d(1_8:*)
+integer(4)::bad_row
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_ilu_alloc(ilu,n,lirp,lja,lval,uirp,uja,uval,d) bind(c,name="psb_hip_ilu_alloc") result(res)
+import::c_ptr
+type(c_ptr)::ilu
+integer(4),value::n
+integer(4)::lirp(1_8:*)
+integer(4)::lja(1_8:*)
+real(8)::lval(1_8:*)
+integer(4)::uirp(1_8:*)
+integer(4)::uja(1_8:*)
+real(8)::uval(1_8:*)
+real(8)::d(1_8:*)
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_ilu_free(ilu) bind(c,name="psb_hip_ilu_free") result(res)
+import::c_ptr
+type(c_ptr),value::ilu
+integer(4)::res
+end
+end interface
+interface
+function psb_hip_ilu_apply(ilu,alpha,x,beta,y) bind(c,name="psb_hip_ilu_apply") result(res)
+import::c_ptr
+type(c_ptr),value::ilu
+real(8),value::alpha
+type(c_ptr),value::x
+real(8),value::beta
+type(c_ptr),value::y
 integer(4)::res
 end
 end interface
