@@ -159,6 +159,97 @@ def solve_bench(P, N, L=256, eps=1e-10):
     return out
 
 
+def methods_bench(P, N, L=40, eps=1e-9, itmax=1000):
+    """Every Krylov method behind psb_krylov with DIAG and BJAC/ILU(0): the
+    convection-diffusion stencil at L (BiCGSTAB, BiCGSTAB(2), RGMRES(10),
+    GCR(10)) and the SPD Poisson (6,-1) stencil at L (CG, FCG); b = A*ones,
+    x0 = 0, istop 2.  The device-resident BiCGSTAB / CG with DIAG are timed
+    beside them."""
+    import numpy as np
+    out = {"L": L, "eps": eps}
+    probs = {}
+    for kind in ("convdiff", "poisson"):
+        if kind == "convdiff":
+            coo = gen_coo(P, N, "stencil", L)
+        else:
+            a3, b3 = np.ones(3), np.zeros(3)
+            fn = N.fn("psb_hip_gen_pde3d")
+            nz = fn(L, a3.ctypes.data, b3.ctypes.data, 0.0, 0, 0, None, None, None, None)
+            ia = np.empty(nz, np.int32); ja = np.empty(nz, np.int32); val = np.empty(nz)
+            fn(L, a3.ctypes.data, b3.ctypes.data, 0.0, 0, 0, ia.ctypes.data, ja.ctypes.data,
+               val.ctypes.data, None)
+            coo = P.psb_d_coo_sparse_mat(L ** 3, L ** 3, ia, ja, val)
+            coo.fix()
+        n = coo.nr
+        a = P.psb_dspmat_type()
+        a.m = a.n = n
+        a.a = P.psb_d_hip_hlg_sparse_mat(hksz=64).cp_from_coo(coo)
+        desc = P.psb_cdall(nl=n)
+        b = P.psb_geall(desc)
+        P.psb_spmm(1.0, a, P.vect_from(np.ones(n)), 0.0, b)
+        precs = {}
+        for pt in ("DIAG", "BJAC"):
+            t0 = time.perf_counter()
+            precs[pt] = P.psb_dprec_type(pt).build(a, desc)
+            P.psb_hip_DeviceSync()
+            out[f"{kind}_{pt}_build_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+        probs[kind] = (a, desc, b, precs, n)
+    bj = probs["convdiff"][3]["BJAC"]
+    out["bjac_levels"] = list(bj.levels)
+    # one BJAC apply (levelled triangular solves) against one DIAG apply
+    a, desc, b, precs, n = probs["convdiff"]
+    y = P.psb_geall(desc)
+    for pt in ("DIAG", "BJAC"):
+        for _ in range(3):
+            precs[pt].apply(b, y, desc)
+        P.psb_hip_DeviceSync()
+        timer = P.DeviceTimer()
+        timer.start()
+        for _ in range(50):
+            precs[pt].apply(b, y, desc)
+        out[f"{pt}_apply_us"] = round(timer.stop_ms() / 50 * 1e3, 2)
+    runs = [("BICGSTAB", "convdiff", None), ("BICGSTABL", "convdiff", 2),
+            ("RGMRES", "convdiff", 10), ("GCR", "convdiff", 10),
+            ("CG", "poisson", None), ("FCG", "poisson", None)]
+    res = {}
+    for method, kind, irst in runs:
+        a, desc, b, precs, n = probs[kind]
+        for pt in ("DIAG", "BJAC"):
+            x = P.psb_geall(desc)
+            P.psb_hip_DeviceSync()
+            t0 = time.perf_counter()
+            it, err, _ = P.psb_krylov(method, a, precs[pt], b, x, eps, desc, itmax=itmax,
+                                      irst=irst)
+            P.psb_hip_DeviceSync()
+            dt = time.perf_counter() - t0
+            e = float(np.max(np.abs(x.get_vect() - 1.0)))
+            res[f"{method}_{pt}"] = {"matrix": kind, "iterations": it, "ms": round(dt * 1e3, 2),
+                                     "ms_per_iteration": round(dt * 1e3 / max(it, 1), 3),
+                                     "final_rel_resid": err, "max_err_vs_ones": e}
+            log(f"methods: {method}({irst}) + {pt} on {kind} L={L}: {it} its {dt * 1e3:.1f} ms "
+                f"resid {err:.1e}")
+    for method, kind in (("BICGSTAB", "convdiff"), ("CG", "poisson")):
+        a, desc, b, precs, n = probs[kind]
+        x = P.psb_geall(desc)
+        P.psb_krylov_fused(method, a, precs["DIAG"], b, x, eps, desc, itmax=3)
+        x = P.psb_geall(desc)
+        P.psb_hip_DeviceSync()
+        t0 = time.perf_counter()
+        it, err, _ = P.psb_krylov_fused(method, a, precs["DIAG"], b, x, eps, desc, itmax=itmax)
+        P.psb_hip_DeviceSync()
+        dt = time.perf_counter() - t0
+        res[f"{method}_DIAG_fused"] = {"matrix": kind, "iterations": it, "ms": round(dt * 1e3, 2),
+                                       "ms_per_iteration": round(dt * 1e3 / max(it, 1), 3),
+                                       "final_rel_resid": err}
+        log(f"methods: fused {method} + DIAG on {kind} L={L}: {it} its {dt * 1e3:.1f} ms")
+    out["runs"] = res
+    for a, desc, b, precs, n in probs.values():
+        for p in precs.values():
+            p.free()
+        a.free()
+    return out
+
+
 def aniso_matrix(P, N, L, e):
     """-e*u_xx - u_yy, 5-point on an L x L grid, as an HLG (hksz 64) matrix."""
     import numpy as np
@@ -396,6 +487,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--solve-L", type=int, default=256)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-methods", action="store_true")
+    ap.add_argument("--methods-L", type=int, default=40)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--serve-arena-mb", type=int, default=768)
@@ -547,6 +640,7 @@ def main():
         sweep["B_n"], sweep["B_nnz"] = nb, nnzb
 
     solve = None if args.no_solve else solve_bench(P, N, args.solve_L)
+    methods = None if args.no_methods else methods_bench(P, N, args.methods_L)
     batch = None if args.no_batch else batch_bench(P, N, args.batch_systems)
     serve = None if args.no_serve else serve_bench(P, N, count=args.serve_requests,
                                                    arena_mb=args.serve_arena_mb)
@@ -579,6 +673,7 @@ def main():
         "cpu_baseline": cpu,
         "sweep": sweep,
         "solve": solve,
+        "methods": methods,
         "batch": batch,
         "serve": serve,
     }

@@ -158,3 +158,89 @@ def test_setscal_first_last(gpu):
     N.call("psb_hip_mv_diag_inv", 3, d.dev)
     d.set_dev()
     assert d.get_vect().tolist() == [0.5, 1.0, -0.25, 1.0]   # zero -> 1; beyond n -> 1
+
+
+def test_vector_plugin_methods(gpu, orc):
+    """The type-bound procedures of the device vector (dot_v, axpby_v,
+    upd_xyz, xyzw, mlt_v, mlt_v_2, scal, nrm2/amax/asum, absval, ins_v/ins_a)
+    against the oracle's CPU routines."""
+    print("case vector plug-in methods", flush=True)
+    n = 3000
+    x, y, z, w = (orc.uniform_pm1(s, n) for s in (11, 12, 13, 14))
+    D = lambda a: P.psb_d_hip_vect().bld(a)
+    xv, yv, zv, wv = D(x), D(y), D(z), D(w)
+    assert abs(xv.dot_v(n, yv) - orc.dot(x, y)) <= 2 * n * np.finfo(float).eps * np.abs(x) @ np.abs(y)
+    yv.axpby_v(n, 1.5, xv, -0.5)
+    ye = orc.axpby(1.5, x, -0.5, y.copy())
+    assert _bits_equal(yv.get_vect(), ye)
+    yv2, zv2 = D(y), D(z)
+    zv2.upd_xyz(n, 2.0, 0.5, -1.0, 3.0, xv, yv2)
+    ye2, ze2 = orc.upd_xyz(2.0, 0.5, -1.0, 3.0, x, y.copy(), z.copy())
+    assert _bits_equal(yv2.get_vect(), ye2) and _bits_equal(zv2.get_vect(), ze2)
+    yv3, zv3 = D(y), D(z)
+    wv.xyzw(n, 1.0, 2.0, 3.0, -1.0, 0.25, 4.0, xv, yv3, zv3)
+    e = orc.xyzw(1.0, 2.0, 3.0, -1.0, 0.25, 4.0, x, y.copy(), z.copy(), w.copy())
+    assert all(_bits_equal(g.get_vect(), r) for g, r in zip((yv3, zv3, wv), e))
+    m = D(y)
+    m.mlt_v(xv)
+    assert _bits_equal(m.get_vect(), x * y)
+    m2 = D(z)
+    m2.mlt_v_2(2.0, xv, D(y), 0.5)
+    assert _bits_equal(m2.get_vect(), orc.mlt(2.0, x, y, 0.5, z.copy()))
+    s = D(x)
+    s.scal(-3.0)
+    assert _bits_equal(s.get_vect(), -3.0 * x)
+    assert xv.amax() == np.max(np.abs(x))
+    assert xv.asum() == pytest.approx(np.sum(np.abs(x)), rel=1e-13)
+    assert xv.nrm2() == pytest.approx(np.linalg.norm(x), rel=1e-13)
+    a = D(x)
+    a.absval()
+    assert _bits_equal(a.get_vect(), np.abs(x))
+    b = P.psb_d_hip_vect().all(n)
+    xv.absval(b)
+    assert _bits_equal(b.get_vect(), np.abs(x))
+    # insertion: overwrite, then add with a repeated index
+    t = P.psb_d_hip_vect().all(10)
+    t.ins_a(3, np.array([2, 9, 2], np.int32), np.array([1.0, 2.0, 5.0]), 1)
+    assert t.get_vect().tolist() == [0, 6.0, 0, 0, 0, 0, 0, 0, 2.0, 0]
+    t.ins_v(2, P.psb_i_hip_vect().bld(np.array([9, 1], np.int32)), D(np.array([-1.0, 7.0])), 0)
+    assert t.get_vect().tolist() == [7.0, 6.0, 0, 0, 0, 0, 0, 0, -1.0, 0]
+    with pytest.raises(P.PsbError):
+        t.ins_a(1, np.array([11], np.int32), np.array([1.0]), 0)
+
+
+def test_halo_buffer_methods(gpu, orc):
+    """gthzv_x / gthzbuf / sctb / sctb_x / sctb_buf: the halo-exchange halves
+    (cuda/psb_d_cuda_vect_mod.F90:222-555), exact against numpy indexing."""
+    print("case halo buffers", flush=True)
+    n = 5000
+    x = orc.uniform_pm1(21, n)
+    xv = P.psb_d_hip_vect().bld(x)
+    idx = np.array([17, 1, 5000, 250, 4999, 3, 1200, 77], np.int32)
+    iv = P.psb_i_hip_vect().bld(idx)
+    assert np.array_equal(xv.gthzv_x(3, 4, iv), x[idx[2:6] - 1])
+    xv.new_buffer(len(idx))
+    xv.gthzbuf(1, 3, iv)
+    xv.gthzbuf(4, 5, iv)
+    assert np.array_equal(xv.combuf, x[idx - 1])
+    # sctb_buf: y(idx(i:i+n-1)) = beta*y + combuf(i:i+n-1)
+    yv = P.psb_d_hip_vect().bld(np.ones(n))
+    yv.new_buffer(len(idx))
+    yv.combuf[:] = np.arange(1.0, 9.0)
+    yv.sctb_buf(2, 6, iv, 0.5)
+    ye = np.ones(n)
+    ye[idx[1:7] - 1] = 0.5 + np.arange(2.0, 8.0)
+    assert np.array_equal(yv.get_vect(), ye)
+    # sctb_x from a host array, beta = 0
+    yv.sctb_x(1, 2, iv, np.array([9.0, -9.0]), 0.0)
+    ye[idx[:2] - 1] = [9.0, -9.0]
+    assert np.array_equal(yv.get_vect(), ye)
+    # sctb on the host copy (the reference's path), repeated index accumulates
+    yv.sctb(3, np.array([4, 4, 10], np.int32), np.array([1.0, 2.0, 3.0]), 1.0)
+    ye[3] += 3.0
+    ye[9] += 3.0
+    assert yv.is_host() and np.array_equal(yv.get_vect(), ye)
+    with pytest.raises(P.PsbError):
+        xv.gthzv_x(6, 4, iv)                    # window beyond idx
+    xv.free()
+    yv.free()
