@@ -208,6 +208,22 @@ def methods_bench(P, N, L=40, eps=1e-9, itmax=1000):
         for _ in range(50):
             precs[pt].apply(b, y, desc)
         out[f"{pt}_apply_us"] = round(timer.stop_ms() / 50 * 1e3, 2)
+    out["bjac_workgroup_runs"] = list(bj.segments)
+    # the same factors solved with one launch per level
+    old = N.fn("psb_hip_ilu_set_variant")(1)
+    pl = P.psb_dprec_type("BJAC").build(a, desc)
+    N.fn("psb_hip_ilu_set_variant")(old)
+    for _ in range(3):
+        pl.apply(b, y, desc)
+    P.psb_hip_DeviceSync()
+    timer = P.DeviceTimer()
+    timer.start()
+    for _ in range(50):
+        pl.apply(b, y, desc)
+    out["BJAC_apply_us_launch_per_level"] = round(timer.stop_ms() / 50 * 1e3, 2)
+    pl.free()
+    log(f"methods: apply DIAG {out['DIAG_apply_us']} us, BJAC {out['BJAC_apply_us']} us "
+        f"(launch per level {out['BJAC_apply_us_launch_per_level']} us), levels {out['bjac_levels']}")
     runs = [("BICGSTAB", "convdiff", None), ("BICGSTABL", "convdiff", 2),
             ("RGMRES", "convdiff", 10), ("GCR", "convdiff", 10),
             ("CG", "poisson", None), ("FCG", "poisson", None)]
@@ -367,8 +383,10 @@ def serve_bench(P, N, nmat=64, seed=4, rates=(200.0, 1000.0, 5000.0), count=160,
     uniform streams 0 and 1 of the seed; HLG hksz 64) with Jacobi
     preconditioners resident on the device.  For each rate, count requests
     with Poisson arrivals, each on a uniformly chosen matrix with a U[-1,1)
-    right-hand side, served by mixed-precision BiCGSTAB (continuous batching)
-    and, on the same stream, one at a time by FP64 BiCGSTAB."""
+    right-hand side, served by mixed-precision BiCGSTAB with restarted
+    refinement (continuous batching) and, on the same stream, one at a time by
+    FP64 BiCGSTAB; at 1000 req/s also by the batched BiCGSTAB with reliable
+    updates."""
     import numpy as np
     u = np.empty(nmat)
     N.fn("psb_hip_fill_uniform_host")(seed, 0, nmat, u.ctypes.data)
@@ -406,7 +424,7 @@ def serve_bench(P, N, nmat=64, seed=4, rates=(200.0, 1000.0, 5000.0), count=160,
     # arena: ~13M rows of BiCGSTAB work vectors (60 bytes per row) in flight, enough to keep
     # the device busy; requests beyond it wait in the queue (admission by footprint)
     srv = P.psb_hip_server("BICGSTAB", max_systems=256, arena_bytes=arena_mb << 20,
-                           check_every=4, eps=eps)
+                           check_every=4, eps=eps, refine="restart")
     for m, prec, _ in systems:
         srv.register(m, prec)
     srv.run([(0.0, k, 999000 + k) for k in range(4)])      # warm: graph capture, first launches
@@ -421,13 +439,13 @@ def serve_bench(P, N, nmat=64, seed=4, rates=(200.0, 1000.0, 5000.0), count=160,
                                      "arrival_span_s": round(reqs[-1][0], 4),
                                      "mixed_batched": mixed, "fp64_one_at_a_time": fp64}
         if rate == 1000.0:
-            # the restarted-refinement BiCGSTAB on the same stream, for comparison
+            # BiCGSTAB with FP64 reliable updates on the same stream, for comparison
             srv_r = P.psb_hip_server("BICGSTAB", max_systems=256, arena_bytes=arena_mb << 20,
-                                     check_every=4, eps=eps, refine="restart")
+                                     check_every=4, eps=eps, refine="reliable")
             for m, prec, _ in systems:
                 srv_r.register(m, prec)
             srv_r.run([(0.0, k, 999000 + k) for k in range(4)])
-            rates_out["1000"]["mixed_batched_restart"] = P.latency_summary(
+            rates_out["1000"]["mixed_batched_reliable"] = P.latency_summary(
                 srv_r.run(reqs, time_limit=90.0))
             srv_r.free()
         log(f"serve: {rate:.0f} req/s: batched p50 {mixed.get('p50_ms')} ms p99 "

@@ -405,6 +405,12 @@ int psb_hip_ilu_alloc(void **ilu, int n, const int *lirp, const int *lja, const 
                       const int *uirp, const int *uja, const double *uval, const double *d);
 int psb_hip_ilu_free(void *ilu);
 int psb_hip_ilu_levels(void *ilu, int *lower_levels, int *upper_levels);
+/* runs of small levels solved by one workgroup (0 when the variant is 1) */
+int psb_hip_ilu_segments(void *ilu, int *lower_segments, int *upper_segments);
+/* triangular-solve kernels for factors allocated afterwards: 0 (default) runs
+   of small levels by one workgroup each, other levels one launch each; 1 one
+   launch per level.  Same results either way.  Returns the previous value. */
+int psb_hip_ilu_set_variant(int variant);
 /* psb_d_bjac_apply_vect (ILU, trans 'N'): y = alpha * U^-1 (d .* L^-1 x) + beta * y */
 int psb_hip_ilu_apply(void *ilu, double alpha, void *x, double beta, void *y);
 

@@ -152,6 +152,8 @@ _SIGS = {
     "psb_hip_ilu_alloc": (i32, [pvp, i32, vp, vp, vp, vp, vp, vp, vp]),
     "psb_hip_ilu_free": (i32, [vp]),
     "psb_hip_ilu_levels": (i32, [vp, pi32, pi32]),
+    "psb_hip_ilu_segments": (i32, [vp, pi32, pi32]),
+    "psb_hip_ilu_set_variant": (i32, [i32]),
     "psb_hip_ilu_apply": (i32, [vp, f64, vp, f64, vp]),
     "psb_hip_batch_create": (i32, [pvp, i32, i32, i64, i32, f64, f64, i32]),
     "psb_hip_batch_free": (i32, [vp]),

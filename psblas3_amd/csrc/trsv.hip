@@ -8,9 +8,25 @@
 // rows of that level, one thread per row.  A row's sum runs in its CSR order,
 // y(i) = x(i) - sum_j val(j) y(ja(j)) with separate multiply and add, as the
 // reference's inner_csrsv (base/serial/impl/psb_d_csr_impl.F90:890-941), so
-// the result equals the sequential CPU solve bit for bit.  The per-level
-// launches of both triangles (a stencil in natural order has about 3L
-// levels) are captured once into a graph that works on internal buffers.
+// the result equals the sequential CPU solve bit for bit.
+//
+// A launch per level costs a few microseconds of dispatch, more than the work
+// of a small level, so runs of consecutive small levels (each at most 2048
+// rows of at most 4 entries, or 1024 rows of at most 8) are solved by ONE
+// workgroup of 1024 threads that walks the levels with an LDS barrier
+// between them.  Everything a run reads of the solution lives in LDS
+// (128 KB): its own rows (first holding x, then overwritten by the solution)
+// and the "halo" of rows solved before the run that it depends on, loaded
+// once at the start.  A level therefore waits only for the LDS traffic of the
+// level before it; the run's solution goes to global memory once, at its end.
+// A run's entries (16-bit LDS slot, value) are laid out per level, column-
+// major and padded to the run's width, the smallest of 2, 3, 4, 8 that fits
+// (entry k of the level's t-th row at off + k*count + t; a padding slot adds
+// value 0 times a zero LDS slot, which leaves the sum's bits unchanged), and
+// are loaded two levels ahead.  One workgroup's load bandwidth bounds a level
+// (about 10 bytes per entry); larger or wider levels get a launch of their
+// own (one thread per row, CSR arrays).  Both triangles are captured once
+// into a graph that works on internal buffers.
 #pragma clang fp contract(off)
 
 #include <string.h>
@@ -22,6 +38,18 @@
 
 namespace psbhip {
 
+// A run of levels solved by one workgroup: levels [lev0, lev1), entry width
+// W, nhalo rows from before the run at halo[hoff ..].
+struct TriSegment {
+  int lev0, lev1, width, hoff, nhalo;
+};
+
+constexpr int kSegThreads = 1024;
+constexpr int kSegRows = 16384;          // LDS slots of a run: its rows, its halo, two spare
+constexpr int kSinkSlot = kSegRows - 2;  // written by lanes beyond their level, never read
+constexpr int kZeroSlot = kSegRows - 1;  // holds 0, read by padding entries
+constexpr int kSegLevels = 1024;         // levels of a run (their descriptors sit in LDS)
+
 struct TriLevels {
   int n = 0, nnz = 0, nlev = 0;
   int* irp = nullptr;
@@ -29,7 +57,21 @@ struct TriLevels {
   double* val = nullptr;
   int* rows = nullptr;          // rows grouped by level
   std::vector<int> lev_ptr;     // host: level l = rows[lev_ptr[l] .. lev_ptr[l+1])
+  // per-level column-major copies for the workgroup-solved runs
+  int4* lev_info = nullptr;     // {count, width, entry offset, row offset} per level
+  uint16_t* eja = nullptr;      // LDS slot of each dependency (kSegRows <= 65536)
+  double* eval = nullptr;
+  int* halo = nullptr;          // rows before each run that the run reads
+  std::vector<TriSegment> segs; // host: the runs, in level order
+  std::vector<int> seg_of;      // host: segment index of a level, -1 if launched alone
 };
+
+// 0: runs of small levels by one workgroup (default); 1: one launch per level.
+static int g_trsv_variant = 0;
+
+// Rows per thread and entry width a workgroup run supports.
+__host__ __device__ constexpr int seg_rpt(int W) { return W <= 4 ? 2 : 1; }
+constexpr int kSegWidths[] = {2, 3, 4, 8};   // the entry widths a run can be built with
 
 struct IluDev {
   int n = 0;
@@ -56,6 +98,139 @@ __global__ __launch_bounds__(kBlock) void k_trsv_level(int count, const int* __r
   y[i] = x[i] - acc;
 }
 
+// Levels [lev0, lev1) by one workgroup.  Row r of thread t in a level is
+// level row t + r*kSegThreads; lanes beyond the level work on the level's last
+// row and write the sink slot.  Loads are unconditional and neither the sum
+// nor the stores are behind a branch, so a level's LDS reads are in flight
+// together.  The sum runs over the row's entries in CSR order, then the
+// padding adds +0 to it (never -0: the sum starts at +0), which leaves it
+// unchanged.
+template <int W>
+struct SegStage {
+  static constexpr int R = seg_rpt(W);
+  int dep[R][W];
+  double v[R][W];
+};
+
+template <int W>
+__device__ __forceinline__ void seg_fetch(SegStage<W>& S, const int4 L,
+                                          const uint16_t* __restrict__ eja,
+                                          const double* __restrict__ eval) {
+#pragma unroll
+  for (int r = 0; r < SegStage<W>::R; r++) {
+    const int tt = min((int)threadIdx.x + r * kSegThreads, L.x - 1);
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+      const int64_t e = (int64_t)L.z + (int64_t)k * L.x + tt;
+      S.dep[r][k] = eja[e];
+      S.v[r][k] = eval[e];
+    }
+  }
+}
+
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <int W>
+__device__ __forceinline__ void seg_solve(const SegStage<W>& S, const int4 L, int base,
+                                          double* ys) {
+  constexpr int R = SegStage<W>::R;
+  lds_barrier();   // the LDS slots of the levels before this one are written
+  double yi[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int tt = min((int)threadIdx.x + r * kSegThreads, L.x - 1);
+    double yk[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) yk[k] = ys[S.dep[r][k]];
+    const double xi = ys[L.w - base + tt];
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < W; k++) acc = acc + S.v[r][k] * yk[k];
+    yi[r] = xi - acc;
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int t = threadIdx.x + r * kSegThreads;
+    ys[t < L.x ? L.w - base + t : kSinkSlot] = yi[r];
+  }
+}
+
+// dst[p] = src[idx[p]] for p < n by the workgroup, kGatherU entries per
+// thread in flight (the indices clamped, the stores guarded).
+constexpr int kGatherU = 8;
+
+__device__ __forceinline__ void wg_gather_to_lds(double* dst, const double* src,
+                                                 const int* __restrict__ idx, int n) {
+  for (int p0 = 0; p0 < n; p0 += kGatherU * kSegThreads) {
+    int j[kGatherU];
+    double v[kGatherU];
+#pragma unroll
+    for (int u = 0; u < kGatherU; u++) j[u] = idx[min(p0 + u * kSegThreads + (int)threadIdx.x, n - 1)];
+#pragma unroll
+    for (int u = 0; u < kGatherU; u++) v[u] = src[j[u]];
+#pragma unroll
+    for (int u = 0; u < kGatherU; u++) {
+      const int p = p0 + u * kSegThreads + threadIdx.x;
+      if (p < n) dst[p] = v[u];
+    }
+  }
+}
+
+template <int W>
+__global__ __launch_bounds__(kSegThreads) void k_trsv_seg(int lev0, int lev1, int base, int nrun,
+                                                          const int* __restrict__ halo, int nhalo,
+                                                          const int4* __restrict__ info,
+                                                          const int* __restrict__ rows,
+                                                          const uint16_t* __restrict__ eja,
+                                                          const double* __restrict__ eval,
+                                                          const double* __restrict__ x,
+                                                          double* y) {
+  __shared__ double ys[kSegRows];
+  __shared__ int4 linfo[kSegLevels];   // the run's level descriptors: no scalar load per level
+  for (int q = threadIdx.x; q < lev1 - lev0; q += kSegThreads) linfo[q] = info[lev0 + q];
+  wg_gather_to_lds(ys, x, rows + base, nrun);
+  if (nhalo > 0) wg_gather_to_lds(ys + nrun, y, halo, nhalo);
+  if (threadIdx.x == 0) ys[kZeroSlot] = 0.0;
+  // entries two levels ahead: the stages rotate, the one solved holds level
+  // l.  The fetches are unconditional (past the run's end they repeat its
+  // last level), so every path leaves the same loads in flight and the
+  // compiler's waits stay exact.
+  lds_barrier();
+  SegStage<W> s0, s1, s2;
+  auto level = [&](int l) { return linfo[min(l, lev1 - 1) - lev0]; };
+  int4 L0 = level(lev0), L1 = level(lev0 + 1), L2 = L0;
+  seg_fetch<W>(s0, L0, eja, eval);
+  seg_fetch<W>(s1, L1, eja, eval);
+  auto step = [&](int l, const SegStage<W>& cur, const int4 Lc, SegStage<W>& ahead, int4& La) {
+    La = level(l + 2);
+    seg_fetch<W>(ahead, La, eja, eval);
+    seg_solve<W>(cur, Lc, base, ys);
+  };
+  for (int l = lev0; l < lev1; l += 3) {
+    step(l, s0, L0, s2, L2);
+    if (l + 1 >= lev1) break;
+    step(l + 1, s1, L1, s0, L0);
+    if (l + 2 >= lev1) break;
+    step(l + 2, s2, L2, s1, L1);
+  }
+  lds_barrier();
+  for (int p0 = 0; p0 < nrun; p0 += kGatherU * kSegThreads) {
+    int j[kGatherU];
+#pragma unroll
+    for (int u = 0; u < kGatherU; u++)
+      j[u] = rows[base + min(p0 + u * kSegThreads + (int)threadIdx.x, nrun - 1)];
+#pragma unroll
+    for (int u = 0; u < kGatherU; u++) {
+      const int p = p0 + u * kSegThreads + threadIdx.x;
+      if (p < nrun) y[j[u]] = ys[p];
+    }
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_scale_by(int n, double* __restrict__ y,
                                                      const double* __restrict__ d) {
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) y[i] = y[i] * d[i];
@@ -64,8 +239,8 @@ __global__ __launch_bounds__(kBlock) void k_scale_by(int n, double* __restrict__
 // Level of every row (longest chain of dependencies through the triangle),
 // rows grouped by level in ascending row order within a level.
 static void analyse(int n, bool lower, const int* irp, const int* ja, std::vector<int>& rows,
-                    std::vector<int>& lev_ptr) {
-  std::vector<int> lev(n, 0);
+                    std::vector<int>& lev_ptr, std::vector<int>& lev) {
+  lev.assign(n, 0);
   int nlev = 0;
   for (int t = 0; t < n; t++) {
     const int i = lower ? t : n - 1 - t;
@@ -82,8 +257,139 @@ static void analyse(int n, bool lower, const int* irp, const int* ja, std::vecto
   for (int i = 0; i < n; i++) rows[fill[lev[i]]++] = i;
 }
 
+// Runs of consecutive levels a workgroup can take: every level of a run has
+// at most seg_rpt(W) * kSegThreads rows of at most W entries, W one of
+// kSegWidths,
+// the run's rows plus its halo fill at most kSegRows - 2 LDS slots, and it has
+// at most kSegLevels levels.
+static void plan_segments(TriLevels& T, const int* irp, const int* ja,
+                          const std::vector<int>& rows, const std::vector<int>& lev) {
+  T.segs.clear();
+  T.seg_of.assign(T.nlev, -1);
+  std::vector<int> width(T.nlev, 0);
+  for (int l = 0; l < T.nlev; l++)
+    for (int p = T.lev_ptr[l]; p < T.lev_ptr[l + 1]; p++)
+      width[l] = std::max(width[l], irp[rows[p] + 1] - irp[rows[p]]);
+  auto fits = [&](int l, int W) {
+    return width[l] <= W && T.lev_ptr[l + 1] - T.lev_ptr[l] <= seg_rpt(W) * kSegThreads;
+  };
+  std::vector<int> stamp(T.n > 0 ? T.n : 1, -1);
+  int cand = 0;
+  int l = 0;
+  auto first_fit = [&](int q, int from) {   // smallest width >= from that level q fits
+    for (int W : kSegWidths)
+      if (W >= from && fits(q, W)) return W;
+    return 0;
+  };
+  while (l < T.nlev) {
+    int W = first_fit(l, 1);
+    if (!W) {
+      l++;
+      continue;
+    }
+    cand++;
+    int nh = 0;
+    auto add_halo = [&](int q) {   // the rows before the run that level q reads
+      for (int p = T.lev_ptr[q]; p < T.lev_ptr[q + 1]; p++)
+        for (int e = irp[rows[p]] - 1; e < irp[rows[p] + 1] - 1; e++) {
+          const int j = ja[e] - 1;
+          if (lev[j] < l && stamp[j] != cand) {
+            stamp[j] = cand;
+            nh++;
+          }
+        }
+    };
+    add_halo(l);
+    int e = l + 1;
+    for (; e < T.nlev; e++) {
+      bool ok = fits(e, W);
+      for (int W2 = first_fit(e, W + 1); !ok && W2; W2 = first_fit(e, W2 + 1)) {
+        bool all = true;   // every level so far must also fit the wider limits
+        for (int q = l; q < e && all; q++) all = fits(q, W2);
+        if (all) {
+          W = W2;
+          ok = true;
+        }
+      }
+      if (!ok) break;
+      add_halo(e);
+      if (T.lev_ptr[e + 1] - T.lev_ptr[l] + nh > kSegRows - 2 || e + 1 - l > kSegLevels) break;
+    }
+    if (e - l >= 2) {   // a lone level is as cheap as its own launch
+      for (int q = l; q < e; q++) T.seg_of[q] = (int)T.segs.size();
+      T.segs.push_back({l, e, W, 0, 0});
+    }
+    l = e;
+  }
+}
+
+static int upload_segments(TriLevels& T, const int* irp, const int* ja, const double* val,
+                           const std::vector<int>& rows, const std::vector<int>& lev) {
+  if (T.segs.empty()) return PSB_HIP_SUCCESS;
+  std::vector<int4> info(T.nlev, make_int4(0, 0, 0, 0));
+  int64_t ent = 0;
+  for (const TriSegment& S : T.segs)
+    for (int l = S.lev0; l < S.lev1; l++) {
+      const int c = T.lev_ptr[l + 1] - T.lev_ptr[l];
+      info[l] = make_int4(c, S.width, (int)std::min<int64_t>(ent, INT32_MAX), T.lev_ptr[l]);
+      ent += (int64_t)c * S.width;
+    }
+  if (ent > INT32_MAX) return set_error(PSB_HIP_UNSUPPORTED, "ilu", "triangle too large");
+  std::vector<int> pos(T.n > 0 ? T.n : 1, 0);   // place of each row in level order
+  for (int p = 0; p < T.n; p++) pos[rows[p]] = p;
+  std::vector<uint16_t> eja(ent, kZeroSlot);   // padding: the zero slot, value 0
+  std::vector<double> eval(ent, 0.0);
+  std::vector<int> halo, slot(T.n > 0 ? T.n : 1, -1);
+  for (TriSegment& S : T.segs) {
+    const int base = T.lev_ptr[S.lev0], nrun = T.lev_ptr[S.lev1] - base;
+    S.hoff = (int)halo.size();
+    for (int l = S.lev0; l < S.lev1; l++) {
+      const int4 L = info[l];
+      for (int t = 0; t < L.x; t++) {
+        const int i = rows[L.w + t];
+        const int b = irp[i] - 1, c = irp[i + 1] - 1 - b;
+        for (int k = 0; k < c; k++) {
+          const int j = ja[b + k] - 1;
+          int sl;
+          if (lev[j] >= S.lev0) {
+            sl = pos[j] - base;   // solved inside the run
+          } else {
+            if (slot[j] < 0) {     // first use of this row of the halo
+              slot[j] = nrun + (int)halo.size() - S.hoff;
+              halo.push_back(j);
+            }
+            sl = slot[j];
+          }
+          eja[(size_t)L.z + (size_t)k * L.x + t] = (uint16_t)sl;
+          eval[(size_t)L.z + (size_t)k * L.x + t] = val[b + k];
+        }
+      }
+    }
+    S.nhalo = (int)halo.size() - S.hoff;
+    for (int h = S.hoff; h < (int)halo.size(); h++) slot[halo[h]] = -1;
+    if (nrun + S.nhalo > kSegRows - 2)
+      return set_error(PSB_HIP_RUNTIME, "ilu", "run larger than its LDS slots");
+  }
+  PSB_HIP_CHECK(hipMalloc(&T.lev_info, sizeof(int4) * (size_t)T.nlev));
+  PSB_HIP_CHECK(hipMalloc(&T.eja, sizeof(uint16_t) * eja.size()));
+  PSB_HIP_CHECK(hipMalloc(&T.eval, sizeof(double) * eval.size()));
+  PSB_HIP_CHECK(hipMalloc(&T.halo, sizeof(int) * std::max<size_t>(halo.size(), 1)));
+  hipStream_t s = ctx().stream;
+  PSB_HIP_CHECK(hipMemcpyAsync(T.lev_info, info.data(), sizeof(int4) * info.size(),
+                               hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(T.eja, eja.data(), sizeof(uint16_t) * eja.size(),
+                               hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(T.eval, eval.data(), sizeof(double) * eval.size(),
+                               hipMemcpyHostToDevice, s));
+  if (!halo.empty())
+    PSB_HIP_CHECK(hipMemcpyAsync(T.halo, halo.data(), sizeof(int) * halo.size(),
+                                 hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipStreamSynchronize(s));   // the host vectors go out of scope
+  return PSB_HIP_SUCCESS;
+}
+
 static int upload_tri(TriLevels& T, int n, bool lower, const int* irp, const int* ja,
-                      const double* val) {
+                      const double* val, bool segments) {
   T.n = n;
   T.nnz = irp[n] - 1;
   for (int i = 0; i < n; i++)
@@ -92,8 +398,8 @@ static int upload_tri(TriLevels& T, int n, bool lower, const int* irp, const int
       if (c < 1 || c > n || (lower ? c > i : c < i + 2))
         return set_error(PSB_HIP_INVALID, "ilu", "factor entry outside its triangle");
     }
-  std::vector<int> rows;
-  analyse(n, lower, irp, ja, rows, T.lev_ptr);
+  std::vector<int> rows, lev;
+  analyse(n, lower, irp, ja, rows, T.lev_ptr, lev);
   T.nlev = (int)T.lev_ptr.size() - 1;
   const size_t nzb = (size_t)(T.nnz > 0 ? T.nnz : 1);
   PSB_HIP_CHECK(hipMalloc(&T.irp, sizeof(int) * (size_t)(n + 1)));
@@ -111,19 +417,43 @@ static int upload_tri(TriLevels& T, int n, bool lower, const int* irp, const int
     PSB_HIP_CHECK(hipMemcpyAsync(T.rows, rows.data(), sizeof(int) * (size_t)n,
                                  hipMemcpyHostToDevice, s));
   PSB_HIP_CHECK(hipStreamSynchronize(s));   // host arrays are the caller's
-  return PSB_HIP_SUCCESS;
+  if (segments) {
+    plan_segments(T, irp, ja, rows, lev);
+  } else {
+    T.segs.clear();
+    T.seg_of.assign(T.nlev, -1);   // every level launched alone
+  }
+  return upload_segments(T, irp, ja, val, rows, lev);
 }
 
 static void enqueue_levels(const TriLevels& T, const double* x, double* y, hipStream_t s) {
-  for (int l = 0; l < T.nlev; l++) {
+  for (int l = 0; l < T.nlev;) {
+    const int g = T.seg_of[l];
+    if (g >= 0) {
+      const TriSegment& S = T.segs[g];
+      const int base = T.lev_ptr[S.lev0], nrun = T.lev_ptr[S.lev1] - base;
+#define PSB_SEG_LAUNCH(W)                                                                   \
+  hipLaunchKernelGGL(k_trsv_seg<W>, dim3(1), dim3(kSegThreads), 0, s, S.lev0, S.lev1, base, nrun, \
+                     T.halo + S.hoff, S.nhalo, T.lev_info, T.rows, T.eja, T.eval, x, y)
+      switch (S.width) {
+        case 2: PSB_SEG_LAUNCH(2); break;
+        case 3: PSB_SEG_LAUNCH(3); break;
+        case 4: PSB_SEG_LAUNCH(4); break;
+        default: PSB_SEG_LAUNCH(8); break;
+      }
+#undef PSB_SEG_LAUNCH
+      l = S.lev1;
+      continue;
+    }
     const int count = T.lev_ptr[l + 1] - T.lev_ptr[l];
     hipLaunchKernelGGL(k_trsv_level, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
                        count, T.rows + T.lev_ptr[l], T.irp, T.ja, T.val, x, y);
+    l++;
   }
 }
 
 static void free_tri(TriLevels& T) {
-  void* p[] = {T.irp, T.ja, T.val, T.rows};
+  void* p[] = {T.irp, T.ja, T.val, T.rows, T.lev_info, T.eja, T.eval, T.halo};
   for (void* q : p)
     if (q) (void)hipFree(q);
   T = TriLevels{};
@@ -155,8 +485,9 @@ int psb_hip_ilu_alloc(void** ilu, int n, const int* lirp, const int* lja, const 
   *ilu = nullptr;
   IluDev* F = new IluDev();
   F->n = n;
-  int rc = upload_tri(F->L, n, true, lirp, lja, lval);
-  if (!rc) rc = upload_tri(F->U, n, false, uirp, uja, uval);
+  const bool seg = g_trsv_variant != 1;
+  int rc = upload_tri(F->L, n, true, lirp, lja, lval, seg);
+  if (!rc) rc = upload_tri(F->U, n, false, uirp, uja, uval, seg);
   if (rc) {
     destroy_ilu(F);
     return rc;
@@ -187,6 +518,20 @@ int psb_hip_ilu_alloc(void** ilu, int n, const int* lirp, const int* lja, const 
     }
   }
   *ilu = F;
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_ilu_set_variant(int variant) {
+  const int old = g_trsv_variant;
+  if (variant == 0 || variant == 1) g_trsv_variant = variant;
+  return old;
+}
+
+int psb_hip_ilu_segments(void* ilu, int* lower_segments, int* upper_segments) {
+  IluDev* F = (IluDev*)ilu;
+  if (!F) return PSB_HIP_INVALID;
+  if (lower_segments) *lower_segments = (int)F->L.segs.size();
+  if (upper_segments) *upper_segments = (int)F->U.segs.size();
   return PSB_HIP_SUCCESS;
 }
 

@@ -31,6 +31,7 @@ class psb_dprec_type:
         self.dv = None
         self.ilu = None
         self.levels = None
+        self.segments = None
 
     def build(self, a, desc_a=None, vmold=None):
         """psb_precbld(a, desc, prec, info, vmold=)."""
@@ -91,6 +92,8 @@ class psb_dprec_type:
         lo, up = C.c_int(0), C.c_int(0)
         N.call("psb_hip_ilu_levels", h, C.byref(lo), C.byref(up))
         self.levels = (lo.value, up.value)
+        N.call("psb_hip_ilu_segments", h, C.byref(lo), C.byref(up))
+        self.segments = (lo.value, up.value)
         return self
 
     def free(self):

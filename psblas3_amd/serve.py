@@ -27,10 +27,15 @@ from .vect import psb_d_hip_vect
 
 
 class psb_hip_server:
-    """Request loop over one batch engine (method "BICGSTAB" or "CG")."""
+    """Request loop over one batch engine (method "BICGSTAB" or "CG").  refine
+    None picks restarted refinement for BiCGSTAB (fewer FP32 iterations than
+    reliable updates on the convection-diffusion requests measured) and
+    reliable updates for CG."""
 
     def __init__(self, method="BICGSTAB", max_systems=128, arena_bytes=4 << 30, check_every=4,
-                 eps=1e-10, itmax=20000, inner_eps=1e-4, delta=1e-4, seed=4, refine="reliable"):
+                 eps=1e-10, itmax=20000, inner_eps=1e-4, delta=1e-4, seed=4, refine=None):
+        if refine is None:
+            refine = "restart" if method.upper() == "BICGSTAB" else "reliable"
         self.eng = psb_hip_batch(method, max_systems=max_systems, arena_bytes=arena_bytes,
                                  check_every=check_every, delta=delta, inner_eps=inner_eps,
                                  refine=refine)

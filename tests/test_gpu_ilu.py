@@ -22,16 +22,79 @@ def _setup(orc, coo, ptype="BJAC"):
     return a, desc, prec, csr
 
 
+def _mixed_levels(orc):
+    """n = 20000: rows < 10000 have no lower entries (a level 0 of 10000 rows,
+    beyond one workgroup), rows >= 10000 depend on i - 10000 and on i - 1 in
+    chains of 50 (levels of about 200 rows), and every 997th row has 12
+    lower entries (levels too wide for a workgroup run); the pattern is
+    symmetric so U mirrors it.  Diagonally dominant."""
+    n = 20000
+    rng = np.random.default_rng(17)
+    pairs = set()
+    for i in range(10000, n):
+        pairs.add((i, i - 10000))
+        if i % 50:
+            pairs.add((i, i - 1))
+    for i in range(10500, n, 997):
+        for j in rng.choice(i - 1, 12, replace=False):
+            pairs.add((i, int(j)))
+    ia, ja, val = [], [], []
+    deg = np.zeros(n)
+    for i, j in pairs:
+        for r, c in ((i, j), (j, i)):
+            v = -0.5 - rng.random()
+            ia.append(r + 1); ja.append(c + 1); val.append(v)
+            deg[r] += abs(v)
+    for i in range(n):
+        ia.append(i + 1); ja.append(i + 1); val.append(deg[i] + 1.0)
+    return orc.COO(n, n, np.array(ia, np.int32), np.array(ja, np.int32), np.array(val))
+
+
+def _band6(orc):
+    """n = 3000, lower and upper offsets 1, 2, 3, 5, 8, 13 (six entries per
+    side: the widest run template), nonsymmetric values, diagonally dominant;
+    every level is one row."""
+    n = 3000
+    rng = np.random.default_rng(23)
+    ia, ja, val = [], [], []
+    for i in range(n):
+        s = 0.0
+        for o in (1, 2, 3, 5, 8, 13):
+            for j in (i - o, i + o):
+                if 0 <= j < n:
+                    v = rng.uniform(-1.0, 1.0)
+                    ia.append(i + 1); ja.append(j + 1); val.append(v)
+                    s += abs(v)
+        ia.append(i + 1); ja.append(i + 1); val.append(s + 0.5)
+    return orc.COO(n, n, np.array(ia, np.int32), np.array(ja, np.int32), np.array(val))
+
+
 CASES = {"convdiff12": lambda o: o.gen_convdiff(12, conv=3.0),
          "aniso33": lambda o: o.gen_aniso2d(33, 0.02),
-         "poisson10": lambda o: o.gen_poisson3d(10)}
+         "poisson10": lambda o: o.gen_poisson3d(10),
+         "poisson40": lambda o: o.gen_poisson3d(40),
+         "mixed_levels": _mixed_levels,
+         "band6": _band6}
+
+
+@pytest.fixture(params=[0, 1], ids=["workgroup-runs", "launch-per-level"])
+def trsv_variant(request):
+    from psblas3_amd import _native as N
+    old = N.fn("psb_hip_ilu_set_variant")(request.param)
+    yield request.param
+    N.fn("psb_hip_ilu_set_variant")(old)
 
 
 @pytest.mark.parametrize("name", list(CASES))
-def test_ilu_apply_bitexact(gpu, orc, name):
-    _say(f"case BJAC/ILU(0) apply {name}")
+def test_ilu_apply_bitexact(gpu, orc, name, trsv_variant):
+    _say(f"case BJAC/ILU(0) apply {name} variant {trsv_variant}")
     coo = CASES[name](orc)
     a, desc, prec, csr = _setup(orc, coo)
+    if trsv_variant == 1:
+        assert prec.segments == (0, 0)
+    else:
+        assert min(prec.segments) >= 1
+    _say(f"  levels {prec.levels} workgroup runs {prec.segments}")
     f = orc.ilu0_fact(*csr)
     x = orc.uniform_pm1(6, coo.nr)
     y = P.vect_from(np.zeros(coo.nr))
