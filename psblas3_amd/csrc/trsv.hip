@@ -4,11 +4,16 @@
 // with scale 'L' and the inverse pivots, then psb_spsm on U).
 //
 // Each triangle is solved by levels: the host analysis gives every row the
-// length of its longest dependency chain, and one kernel per level solves all
-// rows of that level, one thread per row.  A row's sum runs in its CSR order,
+// length of its longest dependency chain.  A row's sum runs in its CSR order,
 // y(i) = x(i) - sum_j val(j) y(ja(j)) with separate multiply and add, as the
 // reference's inner_csrsv (base/serial/impl/psb_d_csr_impl.F90:890-941), so
 // the result equals the sequential CPU solve bit for bit.
+//
+// The solve runs in level order: each triangle is renumbered so that level l
+// holds positions lev_ptr[l] .. lev_ptr[l+1]-1 (rows ascending inside a
+// level), x is gathered into that order by one full-grid kernel and the
+// solution scattered back by another, so every level reads and writes a
+// contiguous range.
 //
 // A launch per level costs a few microseconds of dispatch, more than the work
 // of a small level, so runs of consecutive small levels (each at most 2048
@@ -16,17 +21,17 @@
 // workgroup of 1024 threads that walks the levels with an LDS barrier
 // between them.  Everything a run reads of the solution lives in LDS
 // (128 KB): its own rows (first holding x, then overwritten by the solution)
-// and the "halo" of rows solved before the run that it depends on, loaded
-// once at the start.  A level therefore waits only for the LDS traffic of the
-// level before it; the run's solution goes to global memory once, at its end.
-// A run's entries (16-bit LDS slot, value) are laid out per level, column-
-// major and padded to the run's width, the smallest of 2, 3, 4, 8 that fits
-// (entry k of the level's t-th row at off + k*count + t; a padding slot adds
-// value 0 times a zero LDS slot, which leaves the sum's bits unchanged), and
-// are loaded two levels ahead.  One workgroup's load bandwidth bounds a level
-// (about 10 bytes per entry); larger or wider levels get a launch of their
-// own (one thread per row, CSR arrays).  Both triangles are captured once
-// into a graph that works on internal buffers.
+// and the "halo" of positions solved before the run that it depends on,
+// loaded once at the start.  A level therefore waits only for the LDS traffic
+// of the level before it; the run's solution goes to global memory once, at
+// its end.  A run's entries (16-bit LDS slot, value) are laid out per level,
+// column-major and padded to the run's width, the smallest of 2, 3, 4, 8 that
+// fits (entry k of the level's t-th row at off + k*count + t; a padding slot
+// adds value 0 times a zero LDS slot, which leaves the sum's bits unchanged),
+// and are loaded two levels ahead.  Larger or wider levels get a launch of
+// their own (one thread per row, level-ordered CSR).  The permutations, both
+// triangles and the pivot scaling are captured once into a graph that works
+// on internal buffers.
 #pragma clang fp contract(off)
 
 #include <string.h>
@@ -39,7 +44,7 @@
 namespace psbhip {
 
 // A run of levels solved by one workgroup: levels [lev0, lev1), entry width
-// W, nhalo rows from before the run at halo[hoff ..].
+// W, nhalo positions from before the run at halo[hoff ..].
 struct TriSegment {
   int lev0, lev1, width, hoff, nhalo;
 };
@@ -52,16 +57,16 @@ constexpr int kSegLevels = 1024;         // levels of a run (their descriptors s
 
 struct TriLevels {
   int n = 0, nnz = 0, nlev = 0;
-  int* irp = nullptr;
+  int* perm = nullptr;          // level position -> row
+  int* irp = nullptr;           // level-ordered CSR, 0-based; columns are level positions
   int* ja = nullptr;
   double* val = nullptr;
-  int* rows = nullptr;          // rows grouped by level
-  std::vector<int> lev_ptr;     // host: level l = rows[lev_ptr[l] .. lev_ptr[l+1])
+  std::vector<int> lev_ptr;     // host: level l = positions lev_ptr[l] .. lev_ptr[l+1]-1
   // per-level column-major copies for the workgroup-solved runs
-  int4* lev_info = nullptr;     // {count, width, entry offset, row offset} per level
+  int4* lev_info = nullptr;     // {count, width, entry offset, first position} per level
   uint16_t* eja = nullptr;      // LDS slot of each dependency (kSegRows <= 65536)
   double* eval = nullptr;
-  int* halo = nullptr;          // rows before each run that the run reads
+  int* halo = nullptr;          // positions before each run that the run reads
   std::vector<TriSegment> segs; // host: the runs, in level order
   std::vector<int> seg_of;      // host: segment index of a level, -1 if launched alone
 };
@@ -77,14 +82,37 @@ struct IluDev {
   int n = 0;
   TriLevels L, U;
   double* d = nullptr;          // inverse pivots
-  double* tin = nullptr;        // input copy
-  double* t1 = nullptr;         // d .* L^-1 tin
-  double* t2 = nullptr;         // U^-1 t1
+  double* tin = nullptr;        // input copy (row order)
+  double* xa = nullptr;         // right-hand side of the triangle being solved (level order)
+  double* ya = nullptr;         // its solution (level order)
+  double* t1 = nullptr;         // L^-1 tin (row order)
+  double* t2 = nullptr;         // U^-1 (d .* t1) (row order)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
 
-__global__ __launch_bounds__(kBlock) void k_trsv_level(int count, const int* __restrict__ rows,
+// dst[p] = src[perm[p]], times d[perm[p]] when d is given (the pivot scaling
+// y(i) * d(i) of the reference's 'L' scale)
+__global__ __launch_bounds__(kBlock) void k_perm_gather(int n, const int* __restrict__ perm,
+                                                        const double* __restrict__ src,
+                                                        const double* __restrict__ d,
+                                                        double* __restrict__ dst) {
+  for (int p = blockIdx.x * kBlock + threadIdx.x; p < n; p += gridDim.x * kBlock) {
+    const int i = perm[p];
+    dst[p] = d ? src[i] * d[i] : src[i];
+  }
+}
+
+// dst[perm[p]] = src[p]
+__global__ __launch_bounds__(kBlock) void k_perm_scatter(int n, const int* __restrict__ perm,
+                                                         const double* __restrict__ src,
+                                                         double* __restrict__ dst) {
+  for (int p = blockIdx.x * kBlock + threadIdx.x; p < n; p += gridDim.x * kBlock)
+    dst[perm[p]] = src[p];
+}
+
+// One level: positions p0 .. p0+count-1, one thread each.
+__global__ __launch_bounds__(kBlock) void k_trsv_level(int p0, int count,
                                                        const int* __restrict__ irp,
                                                        const int* __restrict__ ja,
                                                        const double* __restrict__ val,
@@ -92,10 +120,10 @@ __global__ __launch_bounds__(kBlock) void k_trsv_level(int count, const int* __r
                                                        double* y) {
   const int t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= count) return;
-  const int i = rows[t];
+  const int p = p0 + t;
   double acc = 0.0;
-  for (int p = irp[i] - 1; p < irp[i + 1] - 1; p++) acc = acc + val[p] * y[ja[p] - 1];
-  y[i] = x[i] - acc;
+  for (int e = irp[p]; e < irp[p + 1]; e++) acc = acc + val[e] * y[ja[e]];
+  y[p] = x[p] - acc;
 }
 
 // Levels [lev0, lev1) by one workgroup.  Row r of thread t in a level is
@@ -159,32 +187,11 @@ __device__ __forceinline__ void seg_solve(const SegStage<W>& S, const int4 L, in
   }
 }
 
-// dst[p] = src[idx[p]] for p < n by the workgroup, kGatherU entries per
-// thread in flight (the indices clamped, the stores guarded).
-constexpr int kGatherU = 8;
-
-__device__ __forceinline__ void wg_gather_to_lds(double* dst, const double* src,
-                                                 const int* __restrict__ idx, int n) {
-  for (int p0 = 0; p0 < n; p0 += kGatherU * kSegThreads) {
-    int j[kGatherU];
-    double v[kGatherU];
-#pragma unroll
-    for (int u = 0; u < kGatherU; u++) j[u] = idx[min(p0 + u * kSegThreads + (int)threadIdx.x, n - 1)];
-#pragma unroll
-    for (int u = 0; u < kGatherU; u++) v[u] = src[j[u]];
-#pragma unroll
-    for (int u = 0; u < kGatherU; u++) {
-      const int p = p0 + u * kSegThreads + threadIdx.x;
-      if (p < n) dst[p] = v[u];
-    }
-  }
-}
-
+// x and y in level order; the run's positions are base .. base+nrun-1.
 template <int W>
 __global__ __launch_bounds__(kSegThreads) void k_trsv_seg(int lev0, int lev1, int base, int nrun,
                                                           const int* __restrict__ halo, int nhalo,
                                                           const int4* __restrict__ info,
-                                                          const int* __restrict__ rows,
                                                           const uint16_t* __restrict__ eja,
                                                           const double* __restrict__ eval,
                                                           const double* __restrict__ x,
@@ -192,14 +199,14 @@ __global__ __launch_bounds__(kSegThreads) void k_trsv_seg(int lev0, int lev1, in
   __shared__ double ys[kSegRows];
   __shared__ int4 linfo[kSegLevels];   // the run's level descriptors: no scalar load per level
   for (int q = threadIdx.x; q < lev1 - lev0; q += kSegThreads) linfo[q] = info[lev0 + q];
-  wg_gather_to_lds(ys, x, rows + base, nrun);
-  if (nhalo > 0) wg_gather_to_lds(ys + nrun, y, halo, nhalo);
+  for (int p = threadIdx.x; p < nrun; p += kSegThreads) ys[p] = x[base + p];
+  for (int h = threadIdx.x; h < nhalo; h += kSegThreads) ys[nrun + h] = y[halo[h]];
   if (threadIdx.x == 0) ys[kZeroSlot] = 0.0;
-  // entries two levels ahead: the stages rotate, the one solved holds level
-  // l.  The fetches are unconditional (past the run's end they repeat its
-  // last level), so every path leaves the same loads in flight and the
-  // compiler's waits stay exact.
   lds_barrier();
+  // Entries two levels ahead: the stages rotate, the one solved holds level
+  // l.  The loop runs whole rounds of three and the fetches are unconditional
+  // (past the run's end they repeat its last level), so every path leaves the
+  // same loads in flight and the compiler's waits stay exact.
   SegStage<W> s0, s1, s2;
   auto level = [&](int l) { return linfo[min(l, lev1 - 1) - lev0]; };
   int4 L0 = level(lev0), L1 = level(lev0 + 1), L2 = L0;
@@ -208,32 +215,15 @@ __global__ __launch_bounds__(kSegThreads) void k_trsv_seg(int lev0, int lev1, in
   auto step = [&](int l, const SegStage<W>& cur, const int4 Lc, SegStage<W>& ahead, int4& La) {
     La = level(l + 2);
     seg_fetch<W>(ahead, La, eja, eval);
-    seg_solve<W>(cur, Lc, base, ys);
+    if (l < lev1) seg_solve<W>(cur, Lc, base, ys);   // uniform
   };
   for (int l = lev0; l < lev1; l += 3) {
     step(l, s0, L0, s2, L2);
-    if (l + 1 >= lev1) break;
     step(l + 1, s1, L1, s0, L0);
-    if (l + 2 >= lev1) break;
     step(l + 2, s2, L2, s1, L1);
   }
   lds_barrier();
-  for (int p0 = 0; p0 < nrun; p0 += kGatherU * kSegThreads) {
-    int j[kGatherU];
-#pragma unroll
-    for (int u = 0; u < kGatherU; u++)
-      j[u] = rows[base + min(p0 + u * kSegThreads + (int)threadIdx.x, nrun - 1)];
-#pragma unroll
-    for (int u = 0; u < kGatherU; u++) {
-      const int p = p0 + u * kSegThreads + threadIdx.x;
-      if (p < nrun) y[j[u]] = ys[p];
-    }
-  }
-}
-
-__global__ __launch_bounds__(kBlock) void k_scale_by(int n, double* __restrict__ y,
-                                                     const double* __restrict__ d) {
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) y[i] = y[i] * d[i];
+  for (int p = threadIdx.x; p < nrun; p += kSegThreads) y[base + p] = ys[p];
 }
 
 // Level of every row (longest chain of dependencies through the triangle),
@@ -259,9 +249,8 @@ static void analyse(int n, bool lower, const int* irp, const int* ja, std::vecto
 
 // Runs of consecutive levels a workgroup can take: every level of a run has
 // at most seg_rpt(W) * kSegThreads rows of at most W entries, W one of
-// kSegWidths,
-// the run's rows plus its halo fill at most kSegRows - 2 LDS slots, and it has
-// at most kSegLevels levels.
+// kSegWidths; the run's rows plus its halo fill at most kSegRows - 2 LDS
+// slots, and it has at most kSegLevels levels.
 static void plan_segments(TriLevels& T, const int* irp, const int* ja,
                           const std::vector<int>& rows, const std::vector<int>& lev) {
   T.segs.clear();
@@ -273,14 +262,14 @@ static void plan_segments(TriLevels& T, const int* irp, const int* ja,
   auto fits = [&](int l, int W) {
     return width[l] <= W && T.lev_ptr[l + 1] - T.lev_ptr[l] <= seg_rpt(W) * kSegThreads;
   };
-  std::vector<int> stamp(T.n > 0 ? T.n : 1, -1);
-  int cand = 0;
-  int l = 0;
   auto first_fit = [&](int q, int from) {   // smallest width >= from that level q fits
     for (int W : kSegWidths)
       if (W >= from && fits(q, W)) return W;
     return 0;
   };
+  std::vector<int> stamp(T.n > 0 ? T.n : 1, -1);
+  int cand = 0;
+  int l = 0;
   while (l < T.nlev) {
     int W = first_fit(l, 1);
     if (!W) {
@@ -324,7 +313,8 @@ static void plan_segments(TriLevels& T, const int* irp, const int* ja,
 }
 
 static int upload_segments(TriLevels& T, const int* irp, const int* ja, const double* val,
-                           const std::vector<int>& rows, const std::vector<int>& lev) {
+                           const std::vector<int>& rows, const std::vector<int>& pos,
+                           const std::vector<int>& lev) {
   if (T.segs.empty()) return PSB_HIP_SUCCESS;
   std::vector<int4> info(T.nlev, make_int4(0, 0, 0, 0));
   int64_t ent = 0;
@@ -335,8 +325,6 @@ static int upload_segments(TriLevels& T, const int* irp, const int* ja, const do
       ent += (int64_t)c * S.width;
     }
   if (ent > INT32_MAX) return set_error(PSB_HIP_UNSUPPORTED, "ilu", "triangle too large");
-  std::vector<int> pos(T.n > 0 ? T.n : 1, 0);   // place of each row in level order
-  for (int p = 0; p < T.n; p++) pos[rows[p]] = p;
   std::vector<uint16_t> eja(ent, kZeroSlot);   // padding: the zero slot, value 0
   std::vector<double> eval(ent, 0.0);
   std::vector<int> halo, slot(T.n > 0 ? T.n : 1, -1);
@@ -354,9 +342,9 @@ static int upload_segments(TriLevels& T, const int* irp, const int* ja, const do
           if (lev[j] >= S.lev0) {
             sl = pos[j] - base;   // solved inside the run
           } else {
-            if (slot[j] < 0) {     // first use of this row of the halo
+            if (slot[j] < 0) {     // first use of this position of the halo
               slot[j] = nrun + (int)halo.size() - S.hoff;
-              halo.push_back(j);
+              halo.push_back(pos[j]);
             }
             sl = slot[j];
           }
@@ -366,7 +354,7 @@ static int upload_segments(TriLevels& T, const int* irp, const int* ja, const do
       }
     }
     S.nhalo = (int)halo.size() - S.hoff;
-    for (int h = S.hoff; h < (int)halo.size(); h++) slot[halo[h]] = -1;
+    for (int h = S.hoff; h < (int)halo.size(); h++) slot[rows[halo[h]]] = -1;
     if (nrun + S.nhalo > kSegRows - 2)
       return set_error(PSB_HIP_RUNTIME, "ilu", "run larger than its LDS slots");
   }
@@ -401,31 +389,45 @@ static int upload_tri(TriLevels& T, int n, bool lower, const int* irp, const int
   std::vector<int> rows, lev;
   analyse(n, lower, irp, ja, rows, T.lev_ptr, lev);
   T.nlev = (int)T.lev_ptr.size() - 1;
-  const size_t nzb = (size_t)(T.nnz > 0 ? T.nnz : 1);
-  PSB_HIP_CHECK(hipMalloc(&T.irp, sizeof(int) * (size_t)(n + 1)));
-  PSB_HIP_CHECK(hipMalloc(&T.ja, sizeof(int) * nzb));
-  PSB_HIP_CHECK(hipMalloc(&T.val, sizeof(double) * nzb));
-  PSB_HIP_CHECK(hipMalloc(&T.rows, sizeof(int) * (size_t)(n > 0 ? n : 1)));
-  hipStream_t s = ctx().stream;
-  PSB_HIP_CHECK(hipMemcpyAsync(T.irp, irp, sizeof(int) * (size_t)(n + 1), hipMemcpyHostToDevice, s));
-  if (T.nnz > 0) {
-    PSB_HIP_CHECK(hipMemcpyAsync(T.ja, ja, sizeof(int) * (size_t)T.nnz, hipMemcpyHostToDevice, s));
-    PSB_HIP_CHECK(hipMemcpyAsync(T.val, val, sizeof(double) * (size_t)T.nnz, hipMemcpyHostToDevice,
-                                 s));
+  std::vector<int> pos(n > 0 ? n : 1, 0);
+  for (int p = 0; p < n; p++) pos[rows[p]] = p;
+  // the triangle in level order: row rows[p] becomes position p, its entries
+  // keep their CSR order and point at the positions of their columns
+  std::vector<int> pirp(n + 1, 0), pja(T.nnz > 0 ? T.nnz : 1, 0);
+  std::vector<double> pval(T.nnz > 0 ? T.nnz : 1, 0.0);
+  for (int p = 0; p < n; p++) {
+    const int i = rows[p], b = irp[i] - 1, c = irp[i + 1] - 1 - b;
+    for (int k = 0; k < c; k++) {
+      pja[pirp[p] + k] = pos[ja[b + k] - 1];
+      pval[pirp[p] + k] = val[b + k];
+    }
+    pirp[p + 1] = pirp[p] + c;
   }
+  PSB_HIP_CHECK(hipMalloc(&T.irp, sizeof(int) * pirp.size()));
+  PSB_HIP_CHECK(hipMalloc(&T.ja, sizeof(int) * pja.size()));
+  PSB_HIP_CHECK(hipMalloc(&T.val, sizeof(double) * pval.size()));
+  PSB_HIP_CHECK(hipMalloc(&T.perm, sizeof(int) * (size_t)(n > 0 ? n : 1)));
+  hipStream_t s = ctx().stream;
+  PSB_HIP_CHECK(hipMemcpyAsync(T.irp, pirp.data(), sizeof(int) * pirp.size(),
+                               hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(T.ja, pja.data(), sizeof(int) * pja.size(), hipMemcpyHostToDevice,
+                               s));
+  PSB_HIP_CHECK(hipMemcpyAsync(T.val, pval.data(), sizeof(double) * pval.size(),
+                               hipMemcpyHostToDevice, s));
   if (n > 0)
-    PSB_HIP_CHECK(hipMemcpyAsync(T.rows, rows.data(), sizeof(int) * (size_t)n,
+    PSB_HIP_CHECK(hipMemcpyAsync(T.perm, rows.data(), sizeof(int) * (size_t)n,
                                  hipMemcpyHostToDevice, s));
-  PSB_HIP_CHECK(hipStreamSynchronize(s));   // host arrays are the caller's
+  PSB_HIP_CHECK(hipStreamSynchronize(s));   // the host vectors go out of scope
   if (segments) {
     plan_segments(T, irp, ja, rows, lev);
   } else {
     T.segs.clear();
     T.seg_of.assign(T.nlev, -1);   // every level launched alone
   }
-  return upload_segments(T, irp, ja, val, rows, lev);
+  return upload_segments(T, irp, ja, val, rows, pos, lev);
 }
 
+// x and y in level order
 static void enqueue_levels(const TriLevels& T, const double* x, double* y, hipStream_t s) {
   for (int l = 0; l < T.nlev;) {
     const int g = T.seg_of[l];
@@ -434,7 +436,7 @@ static void enqueue_levels(const TriLevels& T, const double* x, double* y, hipSt
       const int base = T.lev_ptr[S.lev0], nrun = T.lev_ptr[S.lev1] - base;
 #define PSB_SEG_LAUNCH(W)                                                                   \
   hipLaunchKernelGGL(k_trsv_seg<W>, dim3(1), dim3(kSegThreads), 0, s, S.lev0, S.lev1, base, nrun, \
-                     T.halo + S.hoff, S.nhalo, T.lev_info, T.rows, T.eja, T.eval, x, y)
+                     T.halo + S.hoff, S.nhalo, T.lev_info, T.eja, T.eval, x, y)
       switch (S.width) {
         case 2: PSB_SEG_LAUNCH(2); break;
         case 3: PSB_SEG_LAUNCH(3); break;
@@ -447,13 +449,26 @@ static void enqueue_levels(const TriLevels& T, const double* x, double* y, hipSt
     }
     const int count = T.lev_ptr[l + 1] - T.lev_ptr[l];
     hipLaunchKernelGGL(k_trsv_level, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                       count, T.rows + T.lev_ptr[l], T.irp, T.ja, T.val, x, y);
+                       T.lev_ptr[l], count, T.irp, T.ja, T.val, x, y);
     l++;
   }
 }
 
+// The application after the input copy: tin -> t2 (captured).
+static void enqueue_apply(IluDev* F, hipStream_t s) {
+  const int n = F->n, g = stream_grid(n);
+  hipLaunchKernelGGL(k_perm_gather, dim3(g), dim3(kBlock), 0, s, n, F->L.perm, F->tin,
+                     (const double*)nullptr, F->xa);
+  enqueue_levels(F->L, F->xa, F->ya, s);
+  hipLaunchKernelGGL(k_perm_scatter, dim3(g), dim3(kBlock), 0, s, n, F->L.perm, F->ya, F->t1);
+  hipLaunchKernelGGL(k_perm_gather, dim3(g), dim3(kBlock), 0, s, n, F->U.perm, F->t1, F->d,
+                     F->xa);
+  enqueue_levels(F->U, F->xa, F->ya, s);
+  hipLaunchKernelGGL(k_perm_scatter, dim3(g), dim3(kBlock), 0, s, n, F->U.perm, F->ya, F->t2);
+}
+
 static void free_tri(TriLevels& T) {
-  void* p[] = {T.irp, T.ja, T.val, T.rows, T.lev_info, T.eja, T.eval, T.halo};
+  void* p[] = {T.perm, T.irp, T.ja, T.val, T.lev_info, T.eja, T.eval, T.halo};
   for (void* q : p)
     if (q) (void)hipFree(q);
   T = TriLevels{};
@@ -466,7 +481,7 @@ static void destroy_ilu(IluDev* F) {
   if (F->graph) (void)hipGraphDestroy(F->graph);
   free_tri(F->L);
   free_tri(F->U);
-  void* p[] = {F->d, F->tin, F->t1, F->t2};
+  void* p[] = {F->d, F->tin, F->xa, F->ya, F->t1, F->t2};
   for (void* q : p)
     if (q) (void)hipFree(q);
   delete F;
@@ -494,28 +509,24 @@ int psb_hip_ilu_alloc(void** ilu, int n, const int* lirp, const int* lja, const 
   }
   const size_t nb = sizeof(double) * (size_t)(n > 0 ? n : 1);
   hipError_t e = hipSuccess;
-  double** bufs[] = {&F->d, &F->tin, &F->t1, &F->t2};
+  double** bufs[] = {&F->d, &F->tin, &F->xa, &F->ya, &F->t1, &F->t2};
   for (double** b : bufs)
     if (e == hipSuccess) e = hipMalloc((void**)b, nb);
   if (e == hipSuccess && n > 0)
     e = hipMemcpy(F->d, d, sizeof(double) * (size_t)n, hipMemcpyHostToDevice);
+  if (e == hipSuccess && n > 0) {
+    hipStream_t s = ctx().stream;
+    e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      enqueue_apply(F, s);
+      e = hipStreamEndCapture(s, &F->graph);
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&F->exec, F->graph, nullptr, nullptr, 0);
+  }
   if (e != hipSuccess) {
     destroy_ilu(F);
     return set_error(e == hipErrorOutOfMemory ? PSB_HIP_OUTOFMEM : PSB_HIP_RUNTIME, __func__,
                      hipGetErrorString(e));
-  }
-  if (n > 0) {
-    hipStream_t s = ctx().stream;
-    PSB_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    enqueue_levels(F->L, F->tin, F->t1, s);
-    hipLaunchKernelGGL(k_scale_by, dim3(stream_grid(n)), dim3(kBlock), 0, s, n, F->t1, F->d);
-    enqueue_levels(F->U, F->t1, F->t2, s);
-    e = hipStreamEndCapture(s, &F->graph);
-    if (e == hipSuccess) e = hipGraphInstantiate(&F->exec, F->graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-      destroy_ilu(F);
-      return set_error(PSB_HIP_RUNTIME, __func__, hipGetErrorString(e));
-    }
   }
   *ilu = F;
   return PSB_HIP_SUCCESS;
