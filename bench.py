@@ -47,16 +47,23 @@ def time_spmv(P, a, xv, yv, steps, warmup):
 
 
 def stream_bandwidth(N, nbytes=1 << 30, reps=20):
-    """Measured HBM bandwidth: copy (2 arrays) and STREAM triad (3 arrays)."""
+    """Measured HBM bandwidth: read-only sum (1 array), copy (2 arrays),
+    STREAM triad (3 arrays) and "mix", the traffic shape of the L=200 HLL
+    product without its gathers (12 coalesced column reads per 16-byte write:
+    768 MiB read, 64 MiB written, like its 740 MB of matrix + x and 64 MB of
+    y)."""
     import ctypes as C
     bufs = [C.c_void_p() for _ in range(3)]
     for b in bufs:
         N.call("psb_hip_alloc_buffer", C.byref(b), nbytes)
     out = {}
     for name, narr, launch in (
+            ("read", 1, lambda: N.call("psb_hip_stream_read", bufs[0], nbytes, bufs[1])),
             ("copy", 2, lambda: N.call("psb_hip_stream_copy", bufs[0], bufs[1], nbytes)),
             ("triad", 3, lambda: N.call("psb_hip_stream_triad", bufs[0], bufs[1], bufs[2], 0.5,
-                                        nbytes))):
+                                        nbytes)),
+            ("mix", 13 / 16, lambda: N.call("psb_hip_stream_mix", bufs[0], 12, nbytes // 16,
+                                            bufs[1]))):
         launch()
         N.call("psb_hip_timer_start")
         for _ in range(reps):
@@ -530,7 +537,8 @@ def main():
     log(f"bench: device {dev}, {P.psb_hip_MultiProcessors()} CUs")
     streams = stream_bandwidth(N)
     stream = max(streams.values())
-    log(f"bench: stream copy {streams['copy']:.0f} GB/s, triad {streams['triad']:.0f} GB/s")
+    log(f"bench: stream read {streams['read']:.0f} GB/s, copy {streams['copy']:.0f} GB/s, "
+        f"triad {streams['triad']:.0f} GB/s, SpMV-shaped mix {streams['mix']:.0f} GB/s")
 
     t0 = time.perf_counter()
     coo_a = gen_coo(P, N, "stencil", args.L)
@@ -685,9 +693,12 @@ def main():
                    "gflops": round(gflops, 2), "device": dev},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(args.L, args.hll_variant),
+                     "stream_read_GBs": round(streams["read"], 1),
+                     "stream_mix_GBs": round(streams["mix"], 1),
                      "stream_copy_GBs": round(streams["copy"], 1),
                      "stream_triad_GBs": round(streams["triad"], 1),
-                     "frac_of_stream": round(gbs / stream, 4)},
+                     "frac_of_stream": round(gbs / stream, 4),
+                     "frac_of_stream_mix": round(gbs / streams["mix"], 4)},
         "cpu_baseline": cpu,
         "sweep": sweep,
         "solve": solve,

@@ -91,6 +91,14 @@ double psb_hip_timer_stop(void);
    roofline denominator (no reference counterpart: the reference only
    reports the spec figure of psb_cuda_MemoryPeakBandwidth). */
 int psb_hip_stream_copy(void *dst, const void *src, int64_t nbytes);
+/* Read-only stream over nbytes of a (multiple of 16): one sum per block into
+   partials (device, at least 8 * 8 * multiprocessors bytes); the bandwidth
+   ceiling of read-dominated kernels such as SpMV. */
+int psb_hip_stream_read(const void *a, int64_t nbytes, void *partials);
+/* The traffic of a column-major sparse product without its gathers:
+   y[i] = sum_{k<ratio} a[k*m + i] over 16-byte elements, m = write_bytes/16;
+   reads ratio * write_bytes from a, writes write_bytes to y. */
+int psb_hip_stream_mix(const void *a, int ratio, int64_t write_bytes, void *y);
 /* STREAM triad a = b + s*c over nbytes per array (multiple of 16). */
 int psb_hip_stream_triad(void *a, const void *b, const void *c, double s, int64_t nbytes);
 

@@ -70,6 +70,8 @@ _SIGS = {
     "psb_hip_timer_stop": (f64, []),
     "psb_hip_stream_copy": (i32, [vp, vp, i64]),
     "psb_hip_stream_triad": (i32, [vp, vp, vp, f64, i64]),
+    "psb_hip_stream_read": (i32, [vp, i64, vp]),
+    "psb_hip_stream_mix": (i32, [vp, i32, i64, vp]),
     "psb_hip_alloc_buffer": (i32, [pvp, i64]),
     "psb_hip_free_buffer": (i32, [vp]),
     "psb_hip_write_buffer": (i32, [vp, vp, i64]),

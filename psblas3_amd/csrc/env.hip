@@ -80,6 +80,40 @@ __global__ __launch_bounds__(kBlock) void k_stream_copy(const v2d* __restrict__ 
 }
 
 // STREAM triad a = b + s*c on 16-byte vectors, 4 per thread per tile.
+// Read-only stream: per-block sums of a (4 x 16 bytes per thread in flight),
+// one partial per block so the loads cannot be dropped.
+__global__ __launch_bounds__(kBlock) void k_stream_read(const v2d* __restrict__ a, int64_t n,
+                                                       double* __restrict__ partials) {
+  __shared__ double lds[kBlock / kWave];
+  const int64_t tile = 4 * kBlock;
+  const int64_t ntiles = n / tile;
+  v2d acc = {0.0, 0.0};
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const v2d* s = a + t * tile + threadIdx.x;
+    const v2d a0 = __builtin_nontemporal_load(s), a1 = __builtin_nontemporal_load(s + kBlock);
+    const v2d a2 = __builtin_nontemporal_load(s + 2 * kBlock);
+    const v2d a3 = __builtin_nontemporal_load(s + 3 * kBlock);
+    acc += (a0 + a1) + (a2 + a3);
+  }
+  for (int64_t i = ntiles * tile + blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    acc += a[i];
+  const double s = block_sum(acc.x + acc.y, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// The traffic shape of a column-major sparse product without its gathers:
+// y[i] = sum_k a[k*m + i] for k < ratio (ratio reads per write, coalesced).
+__global__ __launch_bounds__(kBlock) void k_stream_mix(const v2d* __restrict__ a, int ratio,
+                                                      int64_t m, v2d* __restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * kBlock) {
+    v2d s = {0.0, 0.0};
+    for (int k = 0; k < ratio; k++) s += __builtin_nontemporal_load(a + (int64_t)k * m + i);
+    __builtin_nontemporal_store(s, y + i);
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_stream_triad(v2d* __restrict__ a,
                                                         const v2d* __restrict__ b,
                                                         const v2d* __restrict__ c, double s,
@@ -220,6 +254,36 @@ int psb_hip_stream_copy(void* dst, const void* src, int64_t nbytes) {
   if (g > cap) g = cap;
   hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)g), dim3(kBlock), 0, g_ctx.stream,
                      (const v2d*)src, (v2d*)dst, n);
+  PSB_HIP_LAUNCH_CHECK();
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_stream_read(const void* a, int64_t nbytes, void* partials) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (nbytes % 16 != 0 || !partials) return set_error(PSB_HIP_INVALID, __func__, "nbytes % 16");
+  int64_t n = nbytes / 16;
+  if (n == 0) return PSB_HIP_SUCCESS;
+  int64_t g = (n + kBlock * 4 - 1) / (kBlock * 4);
+  int64_t cap = (int64_t)(g_ctx.num_cus > 0 ? g_ctx.num_cus : 256) * 8;
+  if (g > cap) g = cap;
+  hipLaunchKernelGGL(k_stream_read, dim3((unsigned)g), dim3(kBlock), 0, g_ctx.stream,
+                     (const v2d*)a, n, (double*)partials);
+  PSB_HIP_LAUNCH_CHECK();
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_stream_mix(const void* a, int ratio, int64_t write_bytes, void* y) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (write_bytes % 16 != 0 || ratio < 1) return set_error(PSB_HIP_INVALID, __func__, "sizes");
+  const int64_t m = write_bytes / 16;
+  if (m == 0) return PSB_HIP_SUCCESS;
+  int64_t g = (m + kBlock - 1) / kBlock;
+  int64_t cap = (int64_t)(g_ctx.num_cus > 0 ? g_ctx.num_cus : 256) * 16;
+  if (g > cap) g = cap;
+  hipLaunchKernelGGL(k_stream_mix, dim3((unsigned)g), dim3(kBlock), 0, g_ctx.stream,
+                     (const v2d*)a, ratio, m, (v2d*)y);
   PSB_HIP_LAUNCH_CHECK();
   return PSB_HIP_SUCCESS;
 }
