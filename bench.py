@@ -587,17 +587,18 @@ def main():
                                           "GFLOPs": round(2 * nnz / (t * 1e-3) / 1e9, 2)}
             log(f"bench: {mat} {fmt} {dt} {t * 1e3:.1f} us  {sweep[f'{mat}_{fmt}_{dt}']['GBs']} GB/s")
             m.free()
-        # CSR length-binned kernel on the stencil
+        # the other CSR kernels on the stencil: length-binned, row-block
         m = P.psb_dspmat_type()
         m.m = m.n = n
         m.a = P.psb_d_hip_csrg_sparse_mat().cp_from_coo(coo_a)
-        old = N.fn("psb_hip_csr_set_variant")(2)
-        t = time_spmv(P, m, x, y, max(args.steps // 2, 10), 3)
-        N.fn("psb_hip_csr_set_variant")(old)
+        for var, tag in ((1, "uniform"), (2, "binned"), (3, "rowblock")):
+            old = N.fn("psb_hip_csr_set_variant")(var)
+            t = time_spmv(P, m, x, y, max(args.steps // 2, 10), 3)
+            N.fn("psb_hip_csr_set_variant")(old)
+            sweep[f"A_CSR_f64_{tag}"] = {"us": round(t * 1e3, 2),
+                                         "GBs": round(spmv_bytes(n, nnz) / (t * 1e-3) / 1e9, 1)}
+            log(f"bench: A CSR f64 {tag} {t * 1e3:.1f} us  {sweep[f'A_CSR_f64_{tag}']['GBs']} GB/s")
         m.free()
-        sweep["A_CSR_f64_binned"] = {"us": round(t * 1e3, 2),
-                                     "GBs": round(spmv_bytes(n, nnz) / (t * 1e-3) / 1e9, 1)}
-        log(f"bench: A CSR f64 binned {t * 1e3:.1f} us  {sweep['A_CSR_f64_binned']['GBs']} GB/s")
         del xs, ys
         # autotuner: hack size 32/64 x the four kernel variants on the same matrix
         tuned, rep = P.psb_hip_hll_autotune(coo_a, reps=20)
@@ -653,7 +654,7 @@ def main():
         m = P.psb_dspmat_type()
         m.m = m.n = nb
         m.a = P.psb_d_hip_csrg_sparse_mat().cp_from_coo(coo_b)
-        for var, tag in ((1, "uniform"), (2, "binned")):
+        for var, tag in ((1, "uniform"), (2, "binned"), (3, "rowblock")):
             old = N.fn("psb_hip_csr_set_variant")(var)
             t = time_spmv(P, m, xb, yb, max(args.steps // 4, 5), 2)
             N.fn("psb_hip_csr_set_variant")(old)

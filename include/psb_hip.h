@@ -350,11 +350,14 @@ int psb_hip_csr_get_diag(void *dmat, void *d);
 int psb_hip_csr_get_analysis(void *dmat, int *lanes_per_row, int *long_rows,
                              int *long_threshold);
 /* CSR kernel choice (no counterpart in the reference, whose CSR product is
-   cuSPARSE): 0 auto (the uniform kernel, faster on every matrix measured),
-   1 uniform lanes per row
-   (a power of two >= the mean length), 2 length-binned (rows grouped by
-   lanes per row, about four entries per lane; rows over 512 entries one
-   wavefront each).  Returns the previous value. */
+   cuSPARSE): 0 auto (the row-block kernel, fastest on every matrix
+   measured), 1 uniform lanes per row (a power of two >= the mean length),
+   2 length-binned (rows grouped by lanes per row, about four entries per
+   lane), 3 row-block (runs of consecutive rows whose entries fit in LDS:
+   products streamed into LDS, then one thread per row sums them in CSR
+   order, so those rows equal the CPU loop's bits when beta = 0).  In 2 and 3
+   rows over 512 entries take one wavefront each.  Returns the previous
+   value. */
 int psb_hip_csr_set_variant(int variant);
 
 /* ===================================================================== */

@@ -311,6 +311,10 @@ struct CsrDev {
   int* bin_rows;        // 0-based row ids, class after class, ascending
   int nlong_b;
   int* long_rows_b;
+  // row-block kernel: blocks of consecutive rows (at most kBlock rows, their
+  // entries fit the block's LDS), rows longer than kCsrLongRow excluded
+  int nsblk;
+  int2* sblk;           // (first row, one past the last row), 0-based
 };
 
 // ------------------------------------------------------------------ inner products

@@ -33,7 +33,8 @@ static int g_hll_variant = 0;
 static int g_csr_variant = 0;
 
 constexpr int kCsrEntriesPerLane = 4;
-constexpr int kCsrLongRow = 512;   // binned: longer rows take one wavefront each
+constexpr int kCsrLongRow = 512;   // binned / row-block: longer rows take one wavefront each
+constexpr int kCsrBlockNnz = 2048; // row-block kernel: entries of a block (products in LDS)
 
 // lanes per row of the binned kernel: a power of two near len / 4, 1..64
 inline int csr_class(int len) {
@@ -202,6 +203,52 @@ __global__ __launch_bounds__(kBlock) void k_csr_vector(int nr, int nlog,
   for (; p < e; p += G) acc = fma(ld_stream(val + p), x[ld_stream(ja + p) - 1], acc);
   if (G > 1) acc = group_sum<G>(acc);
   if (gid < nr && lane == 0 && !skip) store_y(y, gid, alpha, acc, beta);
+}
+
+// Row-block CSR (the CSR-stream scheme of Greathouse and Daga, SC14): a block
+// takes a run of consecutive rows whose entries fit in LDS.  All threads
+// stream the run's entries (coalesced index and value loads, eight per thread
+// in flight, the x gathers after them) and store the products val * x in LDS;
+// then one thread per row adds its row's products in CSR order from 0.  The
+// products are separate multiplies and the sum separate adds, the order of
+// the reference's CPU loop (psb_d_csr_csmv, base/serial/impl/
+// psb_d_csr_impl.F90:46-417), so the row sums equal the CPU's bit for bit.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_csr_rowblock(const int2* __restrict__ blk,
+                                                         const int* __restrict__ irp,
+                                                         const int* __restrict__ ja,
+                                                         const T* __restrict__ val,
+                                                         const T* __restrict__ x, T alpha, T beta,
+                                                         T* __restrict__ y) {
+  constexpr int U = kCsrBlockNnz / kBlock;
+  __shared__ T prod[kCsrBlockNnz];
+  const int2 rr = blk[blockIdx.x];
+  const int e0 = irp[rr.x] - 1, e1 = irp[rr.y] - 1;
+  if (e1 > e0) {
+    int c[U];
+    T v[U], xv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int k = min(e0 + (int)threadIdx.x + u * kBlock, e1 - 1);
+      c[u] = ld_stream(ja + k);
+      v[u] = ld_stream(val + k);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) xv[u] = x[c[u] - 1];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int k = (int)threadIdx.x + u * kBlock;
+      if (e0 + k < e1) prod[k] = v[u] * xv[u];   // stored, so never fused into the sum
+    }
+  }
+  __syncthreads();
+  const int r = rr.x + (int)threadIdx.x;
+  if (r < rr.y) {
+    const int b = irp[r] - 1 - e0, e = irp[r + 1] - 1 - e0;
+    T acc = T(0);
+    for (int k = b; k < e; k++) acc = acc + prod[k];
+    store_y(y, r, alpha, acc, beta);
+  }
 }
 
 // Length-binned CSR: the rows of one class (G lanes per row, about four
@@ -518,9 +565,29 @@ static int csr_launch_binned(CsrDev* m, const T* x, T alpha, T beta, T* y) {
 }
 
 template <typename T>
+static int csr_launch_rowblock(CsrDev* m, const T* x, T alpha, T beta, T* y) {
+  hipStream_t s = ctx().stream;
+  const T* val = (const T*)m->val;
+  if (m->nsblk > 0) {
+    hipLaunchKernelGGL((k_csr_rowblock<T>), dim3(m->nsblk), dim3(kBlock), 0, s, m->sblk, m->irp,
+                       m->ja, val, x, alpha, beta, y);
+    PSB_HIP_LAUNCH_CHECK();
+  }
+  if (m->nlong_b > 0) {
+    int nb = grid_for(m->nlong_b, kBlock / kWave);
+    hipLaunchKernelGGL((k_csr_long<T>), dim3(nb), dim3(kBlock), 0, s, m->nlong_b,
+                       m->long_rows_b, m->irp, m->ja, val, x, alpha, beta, y);
+    PSB_HIP_LAUNCH_CHECK();
+  }
+  return PSB_HIP_SUCCESS;
+}
+
+template <typename T>
 static int csr_launch(CsrDev* m, const T* x, T alpha, T beta, T* y) {
   const bool binned = g_csr_variant == 2;
   if (binned && m->bin_rows) return csr_launch_binned<T>(m, x, alpha, beta, y);
+  if ((g_csr_variant == 0 || g_csr_variant == 3) && m->rows > 0)
+    return csr_launch_rowblock<T>(m, x, alpha, beta, y);
   hipStream_t s = ctx().stream;
   const T* val = (const T*)m->val;
   int g = m->lanes;
@@ -1086,6 +1153,7 @@ void psb_hip_csr_free(void* dmat) {
   hipFree(m->long_rows);
   hipFree(m->bin_rows);
   hipFree(m->long_rows_b);
+  hipFree(m->sblk);
   delete m;
 }
 
@@ -1133,6 +1201,25 @@ int psb_hip_csr_write(void* dmat, const int* irp, const int* ja, const void* val
     }
   }
   m->nlong_b = (int)longs_b.size();
+  // row blocks: consecutive rows up to kBlock of them and kCsrBlockNnz
+  // entries; a row longer than kCsrLongRow ends the block and is left out
+  std::vector<int2> sblk;
+  {
+    int r0 = 0;
+    while (r0 < m->rows) {
+      if (irp[r0 + 1] - irp[r0] > kCsrLongRow) {
+        r0++;
+        continue;
+      }
+      int r1 = r0;
+      while (r1 < m->rows && r1 - r0 < kBlock && irp[r1 + 1] - irp[r1] <= kCsrLongRow &&
+             irp[r1 + 1] - irp[r0] <= kCsrBlockNnz)
+        r1++;
+      sblk.push_back(make_int2(r0, r1));
+      r0 = r1;
+    }
+  }
+  m->nsblk = (int)sblk.size();
   hipStream_t s = ctx().stream;
   hipFree(m->long_rows);
   m->long_rows = nullptr;
@@ -1148,6 +1235,13 @@ int psb_hip_csr_write(void* dmat, const int* irp, const int* ja, const void* val
   if (m->rows > 0) {
     PSB_HIP_CHECK(hipMalloc(&m->bin_rows, sizeof(int) * bins.size()));
     PSB_HIP_CHECK(hipMemcpyAsync(m->bin_rows, bins.data(), sizeof(int) * bins.size(),
+                                 hipMemcpyHostToDevice, s));
+  }
+  hipFree(m->sblk);
+  m->sblk = nullptr;
+  if (m->nsblk > 0) {
+    PSB_HIP_CHECK(hipMalloc(&m->sblk, sizeof(int2) * m->nsblk));
+    PSB_HIP_CHECK(hipMemcpyAsync(m->sblk, sblk.data(), sizeof(int2) * m->nsblk,
                                  hipMemcpyHostToDevice, s));
   }
   if (m->nlong_b > 0) {
@@ -1195,7 +1289,7 @@ int psb_hip_csr_get_diag(void* dmat, void* dv) {
 
 int psb_hip_csr_set_variant(int variant) {
   const int old = g_csr_variant;
-  if (variant >= 0 && variant <= 2) g_csr_variant = variant;
+  if (variant >= 0 && variant <= 3) g_csr_variant = variant;
   return old;
 }
 

@@ -259,10 +259,23 @@ def test_csr_variants(gpu, orc, kind):
     y0 = orc.uniform_pm1(9, coo.nr)
     ref = orc.csr_spmv(irp, ja, val, x, 1.5, -0.5, y0)
     bound = _row_bound(f, x, 1.5, -0.5, y0, np.finfo(float).eps)
-    for var in (1, 2, 0):
+    for var in (1, 2, 3, 0):
         old = N.fn("psb_hip_csr_set_variant")(var)
         xv, yv = P.vect_from(x), P.vect_from(y0.copy())
         P.psb_spmm(1.5, a, xv, -0.5, yv)
         N.fn("psb_hip_csr_set_variant")(old)
         assert np.all(np.abs(yv.get_vect() - ref) <= bound), var
+    # the row-block kernel sums separately rounded products in CSR order, as
+    # the CPU loop does: with beta = 0 its rows of at most 512 entries match
+    # the oracle bit for bit (longer rows go to the wave-per-row kernel)
+    old = N.fn("psb_hip_csr_set_variant")(3)
+    yv = P.vect_from(np.zeros(coo.nr))
+    P.psb_spmm(1.5, a, P.vect_from(x), 0.0, yv)
+    N.fn("psb_hip_csr_set_variant")(old)
+    ref0 = orc.csr_spmv(irp, ja, val, x, 1.5, 0.0)
+    short = np.diff(irp) <= 512
+    got = yv.get_vect()
+    assert np.array_equal(got[short].view(np.int64), ref0[short].view(np.int64))
+    assert np.all(np.abs(got - ref0) <= _row_bound(f, x, 1.5, 0.0, np.zeros(coo.nr),
+                                                    np.finfo(float).eps))
     a.free()
