@@ -89,6 +89,7 @@ struct BTables {
   const BSys* __restrict__ sys;
   BState* st;
   double* partials;                // two per tile
+  int tcap;                        // tile table entries (>= kBGrid)
 };
 
 __device__ __forceinline__ bool b_iterating(const BState* s) {
@@ -110,21 +111,50 @@ __device__ __forceinline__ T b_row(const BSys& sy, const T* a, int r, const T* _
 
 // The body runs for every tile of a system satisfying COND (st, sy, r0 =
 // the tile's first row, t_ = tile index in scope).  COND is uniform over the
-// block.  A tile is kTileRows rows: elementwise kernels give each thread 4
-// consecutive rows (16-byte accesses), products make 4 passes of 256 rows
-// (one row per lane).
+// block.  A tile is kTileRows rows: elementwise kernels give each of their
+// kBT threads 4 consecutive rows (16-byte accesses), product kernels run
+// kTileRows threads, one row each.  The descriptor loads are kept short: the
+// block's first tile entry is read together with the tile count (the table
+// has at least gridDim.x entries), a tile's state and system descriptor
+// together, and the next tile's entry before the body runs.
 #define PSB_BATCH_TILES(TB, COND, ...)                                   \
   {                                                                      \
     const int nt_ = *(TB).ntiles;                                        \
-    for (int t_ = blockIdx.x; t_ < nt_; t_ += gridDim.x) {               \
-      const int2 tl_ = (TB).tiles[t_];                                   \
+    int2 tl_ = (TB).tiles[blockIdx.x];                                   \
+    int2 tn_;                                                            \
+    for (int t_ = blockIdx.x; t_ < nt_; t_ += gridDim.x, tl_ = tn_) {    \
       BState* st = (TB).st + tl_.x;                                      \
+      const BSys sy = (TB).sys[tl_.x];                                   \
+      tn_ = (TB).tiles[min(t_ + (int)gridDim.x, (TB).tcap - 1)];         \
       if (!(COND)) continue;                                             \
-      const BSys& sy = (TB).sys[tl_.x];                                  \
       const int r0 = tl_.y;                                              \
       __VA_ARGS__                                                        \
     }                                                                    \
   }
+
+// Product kernels run one row per thread of a tile.
+constexpr int kBTP = kTileRows;
+
+__device__ __forceinline__ void tile_partials_p(double a, double b, double* partials, int t) {
+  __shared__ double lds[2][kBTP / kWave];
+  const double wa = wave_sum(a), wb = wave_sum(b);
+  const int wid = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    lds[0][wid] = wa;
+    lds[1][wid] = wb;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sa = 0.0, sb = 0.0;
+    for (int k = 0; k < kBTP / kWave; k++) {
+      sa += lds[0][k];
+      sb += lds[1][k];
+    }
+    partials[2 * t] = sa;
+    partials[2 * t + 1] = sb;
+  }
+  __syncthreads();   // the next tile reuses lds
+}
 
 __device__ __forceinline__ void tile_partials(double a, double b, double* partials, int t) {
   __shared__ double lds[2][kBT / kWave];
@@ -204,20 +234,17 @@ _Pragma("unroll")
   })
 }
 
-__global__ __launch_bounds__(kBT) void kb_cg_q(BTables tb) {
+__global__ __launch_bounds__(kBTP) void kb_cg_q(BTables tb) {
   PSB_BATCH_TILES(tb, b_iterating(st), {
+    const int i = r0 + (int)threadIdx.x;
+    const bool in = i < sy.n;
+    const float ax = b_row<float>(sy, sy.a32, in ? i : sy.n - 1, sy.p);
     double pq = 0.0;
-    for (int s = 0; s < kTileRows / kBT; s++) {
-      const int i = r0 + s * kBT + (int)threadIdx.x;
-      if (s * kBT >= sy.n - r0) break;
-      const bool in = i < sy.n;
-      const float ax = b_row<float>(sy, sy.a32, in ? i : sy.n - 1, sy.p);
-      if (in) {
-        sy.q[i] = ax;
-        pq += (double)sy.p[i] * (double)ax;
-      }
+    if (in) {
+      sy.q[i] = ax;
+      pq = (double)sy.p[i] * (double)ax;
     }
-    tile_partials(pq, 0.0, tb.partials, t_);
+    tile_partials_p(pq, 0.0, tb.partials, t_);
   })
 }
 
@@ -279,20 +306,17 @@ _Pragma("unroll")
   })
 }
 
-__global__ __launch_bounds__(kBT) void kb_bi_v(BTables tb) {
+__global__ __launch_bounds__(kBTP) void kb_bi_v(BTables tb) {
   PSB_BATCH_TILES(tb, b_iterating(st), {
+    const int i = r0 + (int)threadIdx.x;
+    const bool in = i < sy.n;
+    const float ax = b_row<float>(sy, sy.a32, in ? i : sy.n - 1, sy.f);
     double qv = 0.0;
-    for (int s = 0; s < kTileRows / kBT; s++) {
-      const int i = r0 + s * kBT + (int)threadIdx.x;
-      if (s * kBT >= sy.n - r0) break;
-      const bool in = i < sy.n;
-      const float ax = b_row<float>(sy, sy.a32, in ? i : sy.n - 1, sy.f);
-      if (in) {
-        sy.v[i] = ax;
-        qv += (double)sy.q[i] * (double)ax;
-      }
+    if (in) {
+      sy.v[i] = ax;
+      qv = (double)sy.q[i] * (double)ax;
     }
-    tile_partials(qv, 0.0, tb.partials, t_);
+    tile_partials_p(qv, 0.0, tb.partials, t_);
   })
 }
 
@@ -319,21 +343,18 @@ _Pragma("unroll")
   })
 }
 
-__global__ __launch_bounds__(kBT) void kb_bi_t(BTables tb) {
+__global__ __launch_bounds__(kBTP) void kb_bi_t(BTables tb) {
   PSB_BATCH_TILES(tb, b_iterating(st), {
+    const int i = r0 + (int)threadIdx.x;
+    const bool in = i < sy.n;
+    const float ax = b_row<float>(sy, sy.a32, in ? i : sy.n - 1, sy.z);
     double tt = 0.0, ts = 0.0;
-    for (int s = 0; s < kTileRows / kBT; s++) {
-      const int i = r0 + s * kBT + (int)threadIdx.x;
-      if (s * kBT >= sy.n - r0) break;
-      const bool in = i < sy.n;
-      const float ax = b_row<float>(sy, sy.a32, in ? i : sy.n - 1, sy.z);
-      if (in) {
-        sy.t[i] = ax;
-        tt += (double)ax * (double)ax;
-        ts += (double)ax * (double)sy.s[i];
-      }
+    if (in) {
+      sy.t[i] = ax;
+      tt = (double)ax * (double)ax;
+      ts = (double)ax * (double)sy.s[i];
     }
-    tile_partials(tt, ts, tb.partials, t_);
+    tile_partials_p(tt, ts, tb.partials, t_);
   })
 }
 
@@ -390,23 +411,20 @@ _Pragma("unroll")
 }
 
 // r64 = b - A64 x64 with partials r64.r64 and b.b (FP64)
-__global__ __launch_bounds__(kBT) void kb_res(BTables tb) {
+__global__ __launch_bounds__(kBTP) void kb_res(BTables tb) {
   PSB_BATCH_TILES(tb, b_transiting(st), {
+    const int i = r0 + (int)threadIdx.x;
+    const bool in = i < sy.n;
+    const double ax = b_row<double>(sy, sy.a64, in ? i : sy.n - 1, sy.x);
     double rr = 0.0, bb = 0.0;
-    for (int s = 0; s < kTileRows / kBT; s++) {
-      const int i = r0 + s * kBT + (int)threadIdx.x;
-      if (s * kBT >= sy.n - r0) break;
-      const bool in = i < sy.n;
-      const double ax = b_row<double>(sy, sy.a64, in ? i : sy.n - 1, sy.x);
-      if (in) {
-        const double bi = sy.b[i];
-        const double ri = bi - ax;
-        sy.r64[i] = ri;
-        rr += ri * ri;
-        bb += bi * bi;
-      }
+    if (in) {
+      const double bi = sy.b[i];
+      const double ri = bi - ax;
+      sy.r64[i] = ri;
+      rr = ri * ri;
+      bb = bi * bi;
     }
-    tile_partials(rr, bb, tb.partials, t_);
+    tile_partials_p(rr, bb, tb.partials, t_);
   })
 }
 
@@ -682,6 +700,7 @@ static BTables tables(const Batch& B) {
   t.sys = B.sys;
   t.st = B.st;
   t.partials = B.partials;
+  t.tcap = B.tmax;
   return t;
 }
 
@@ -726,21 +745,21 @@ static void arena_give(Batch& B, size_t off, size_t bytes) {
 // The kernels of one step (check_every iterations + the transition pass).
 static void enqueue_step(Batch& B) {
   const BTables tb = tables(B);
-  dim3 g(kBGrid), blk(kBT), gs(B.nslots);
+  dim3 g(kBGrid), blk(kBT), blkp(kBTP), gs(B.nslots);
   hipStream_t s = B.stream;
   for (int k = 0; k < B.check_every; k++) {
     if (B.method == 0) {
       hipLaunchKernelGGL(kb_cg_p, g, blk, 0, s, tb);
-      hipLaunchKernelGGL(kb_cg_q, g, blk, 0, s, tb);
+      hipLaunchKernelGGL(kb_cg_q, g, blkp, 0, s, tb);
       hipLaunchKernelGGL(kb_fin<BF_CG_Q>, gs, blk, 0, s, tb);
       hipLaunchKernelGGL(kb_cg_x, g, blk, 0, s, tb);
       hipLaunchKernelGGL(kb_fin<BF_CG_X>, gs, blk, 0, s, tb);
     } else {
       hipLaunchKernelGGL(kb_bi_a, g, blk, 0, s, tb);
-      hipLaunchKernelGGL(kb_bi_v, g, blk, 0, s, tb);
+      hipLaunchKernelGGL(kb_bi_v, g, blkp, 0, s, tb);
       hipLaunchKernelGGL(kb_fin<BF_BI_V>, gs, blk, 0, s, tb);
       hipLaunchKernelGGL(kb_bi_s, g, blk, 0, s, tb);
-      hipLaunchKernelGGL(kb_bi_t, g, blk, 0, s, tb);
+      hipLaunchKernelGGL(kb_bi_t, g, blkp, 0, s, tb);
       hipLaunchKernelGGL(kb_fin<BF_BI_T>, gs, blk, 0, s, tb);
       hipLaunchKernelGGL(kb_bi_x, g, blk, 0, s, tb);
       if (B.method == 1) hipLaunchKernelGGL(kb_fin<BF_BI_X>, gs, blk, 0, s, tb);
@@ -748,7 +767,7 @@ static void enqueue_step(Batch& B) {
     }
   }
   hipLaunchKernelGGL(kb_accum, g, blk, 0, s, tb, B.method == 1 ? 0 : 1);
-  hipLaunchKernelGGL(kb_res, g, blk, 0, s, tb);
+  hipLaunchKernelGGL(kb_res, g, blkp, 0, s, tb);
   hipLaunchKernelGGL(kb_fin<BF_RES>, gs, blk, 0, s, tb);
   if (B.method == 0) {
     hipLaunchKernelGGL(kb_cg_reset, g, blk, 0, s, tb);
@@ -854,6 +873,7 @@ int psb_hip_batch_create(void** batch, int method, int max_systems, int64_t aren
   const size_t row_bytes = sys_bytes(method, kTileRows) / kTileRows;
   B->tmax = (int)std::min<size_t>(B->arena_bytes / (row_bytes * kTileRows) + (size_t)max_systems,
                                   (size_t)1 << 30);
+  B->tmax = std::max(B->tmax, kBGrid);   // the tile kernels read entry blockIdx.x unconditionally
   B->slots.resize(max_systems);
   B->sys_h.assign(max_systems, BSys{});
   B->free_ranges[0] = B->arena_bytes;
