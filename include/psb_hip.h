@@ -292,10 +292,14 @@ int psb_hip_hll_spmv_single(void *dmat, double alpha, void *x, double beta, void
    (prec/impl/psb_d_diagprec_impl.f90:254). */
 int psb_hip_hll_get_diag(void *dmat, void *d);
 /* Kernel variant selection for the HLL SpMV (autotuner hook): 0 = auto
-   (two rows per lane for double values, one for float), 1 = one row per lane,
-   2 = two adjacent rows per lane (even hksz); + 4 = XCD-contiguous block
-   order instead of launch order.  Returns the previous value. */
+   (two rows per lane for double values, one for float, XCD block order),
+   1 = one row per lane, 2 = two adjacent rows per lane (even hksz), both in
+   launch order; + 4 = XCD block order.  Returns the previous value. */
 int psb_hip_hll_set_variant(int variant);
+/* XCD block order: logical blocks dealt to the XCDs in chunks of `blocks`
+   (default 16; 0: one contiguous range per XCD).  No counterpart in the
+   reference.  Returns the previous value. */
+int psb_hip_hll_set_xcd_chunk(int blocks);
 /* Autotuner (no counterpart in the reference): times y = A*x with each HLL
    kernel variant (1, 2, 5, 6 above; reps launches each, HIP events), stores
    the fastest in the matrix (it then overrides the global variant for this

@@ -270,6 +270,22 @@ __device__ __forceinline__ int xcd_block(int nlog) {
 }
 inline int xcd_grid(int nlog) { return ((nlog + kXcds - 1) / kXcds) * kXcds; }
 
+// Chunked XCD order: the logical blocks are cut into chunks of `chunk`
+// blocks dealt round-robin to the XCDs (chunk c to XCD c % kXcds), each XCD
+// walking its chunks in order; chunk <= 0 gives the contiguous order above.
+// Launch xcd_chunk_grid(nlog, chunk) blocks; the mapping is a permutation,
+// blocks mapped past nlog return.
+__device__ __forceinline__ int xcd_block_chunk(int nlog, int chunk) {
+  if (chunk <= 0) return xcd_block(nlog);
+  const int b = blockIdx.x, j = b % kXcds, s = b / kXcds;
+  return ((s / chunk) * kXcds + j) * chunk + (s % chunk);
+}
+inline int xcd_chunk_grid(int nlog, int chunk) {
+  if (chunk <= 0) return xcd_grid(nlog);
+  const int nc = (nlog + chunk - 1) / chunk;
+  return ((nc + kXcds - 1) / kXcds) * kXcds * chunk;
+}
+
 // ------------------------------------------------------------------ device matrices
 struct HllDev {
   int elem_type;

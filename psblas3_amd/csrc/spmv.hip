@@ -27,6 +27,11 @@
 namespace psbhip {
 
 static int g_hll_variant = 0;
+// XCD order: chunks of 16 blocks dealt to the XCDs in turn (0: one contiguous
+// range per XCD).  On the 7-point stencils at L = 200 and 256 chunks of 4-64
+// blocks measured 1-2% (FP64 pairs) and 2-5% (FP32) faster than launch order,
+// contiguous ranges 3-5% slower.
+static int g_hll_xchunk = 16;
 // 0 auto (= uniform: the length-binned kernel measured slower on both the
 // L=200 stencil, 514 vs 280 us, and the n=4M power-law matrix, 1355 vs
 // 1069 us), 1 uniform lanes per row, 2 length-binned
@@ -50,13 +55,13 @@ inline int csr_class(int len) {
 // HK > 0: compile-time hack size (32 or 64); HK == 0: any hksz.  XCD: blocks
 // take XCD-contiguous row ranges (xcd_block); otherwise blocks in launch order.
 template <typename T, typename TV, int HK, bool XCD>
-__global__ __launch_bounds__(kBlock) void k_hll_spmv(int nr, int hksz, int nlog,
+__global__ __launch_bounds__(kBlock) void k_hll_spmv(int nr, int hksz, int nlog, int xchunk,
                                                      const int* __restrict__ hkoffs,
                                                      const int* __restrict__ ja,
                                                      const TV* __restrict__ val,
                                                      const T* __restrict__ x, T alpha, T beta,
                                                      T* __restrict__ y) {
-  const int lb = XCD ? xcd_block(nlog) : (int)blockIdx.x;
+  const int lb = XCD ? xcd_block_chunk(nlog, xchunk) : (int)blockIdx.x;
   if (lb >= nlog) return;
   const int row = lb * kBlock + threadIdx.x;
   const int r = row < nr ? row : nr - 1;
@@ -73,13 +78,13 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmv(int nr, int hksz, int nlog,
 // hack since hksz is even): 8-byte index and 2-element value loads.  Rows
 // past nr read the last hack's padding rows (the hack stores hksz rows).
 template <typename T, typename TV, int HK, bool XCD>
-__global__ __launch_bounds__(kBlock) void k_hll_spmv_pair(int nr, int hksz, int nlog,
+__global__ __launch_bounds__(kBlock) void k_hll_spmv_pair(int nr, int hksz, int nlog, int xchunk,
                                                           const int* __restrict__ hkoffs,
                                                           const int* __restrict__ ja,
                                                           const TV* __restrict__ val,
                                                           const T* __restrict__ x, T alpha,
                                                           T beta, T* __restrict__ y) {
-  const int lb = XCD ? xcd_block(nlog) : (int)blockIdx.x;
+  const int lb = XCD ? xcd_block_chunk(nlog, xchunk) : (int)blockIdx.x;
   if (lb >= nlog) return;
   const int row = lb * (2 * kBlock) + 2 * (int)threadIdx.x;
   const int last = (nr - 1) & ~1;
@@ -460,30 +465,31 @@ static void hll_launch_hk(const HllDev* m, bool pair, bool xcd, const TV* val, c
                           T alpha, T beta, T* y) {
   hipStream_t s = ctx().stream;
   const int nlog = grid_for(m->rows, pair ? 2 * kBlock : kBlock);
-  dim3 grid(xcd ? xcd_grid(nlog) : nlog), blk(kBlock);
+  const int xc = g_hll_xchunk;
+  dim3 grid(xcd ? xcd_chunk_grid(nlog, xc) : nlog), blk(kBlock);
   if (pair && xcd)
     hipLaunchKernelGGL((k_hll_spmv_pair<T, TV, HK, true>), grid, blk, 0, s, m->rows, m->hksz, nlog,
-                       m->hkoffs, m->ja, val, x, alpha, beta, y);
+                       xc, m->hkoffs, m->ja, val, x, alpha, beta, y);
   else if (pair)
     hipLaunchKernelGGL((k_hll_spmv_pair<T, TV, HK, false>), grid, blk, 0, s, m->rows, m->hksz,
-                       nlog, m->hkoffs, m->ja, val, x, alpha, beta, y);
+                       nlog, 0, m->hkoffs, m->ja, val, x, alpha, beta, y);
   else if (xcd)
     hipLaunchKernelGGL((k_hll_spmv<T, TV, HK, true>), grid, blk, 0, s, m->rows, m->hksz, nlog,
-                       m->hkoffs, m->ja, val, x, alpha, beta, y);
+                       xc, m->hkoffs, m->ja, val, x, alpha, beta, y);
   else
     hipLaunchKernelGGL((k_hll_spmv<T, TV, HK, false>), grid, blk, 0, s, m->rows, m->hksz, nlog,
-                       m->hkoffs, m->ja, val, x, alpha, beta, y);
+                       0, m->hkoffs, m->ja, val, x, alpha, beta, y);
 }
 
 // Variant: 1 one row per lane, 2 two adjacent rows per lane (even hksz), 0
-// chooses (pairs for 8-byte values, single rows for 4-byte values); adding 4
-// gives each XCD a contiguous range of blocks (xcd_block) instead of launch
-// order (measured 3-5% slower on the 7-point stencils at L = 200 and 256).
+// chooses (pairs for 8-byte values, single rows for 4-byte values, both in
+// XCD order); adding 4 deals the blocks to the XCDs in chunks of
+// g_hll_xchunk blocks (xcd_block_chunk) instead of launch order.
 template <typename T, typename TV>
 static int hll_launch(const HllDev* m, const TV* val, const T* x, T alpha, T beta, T* y) {
   const int v = m->variant != 0 ? m->variant : g_hll_variant;
   const int variant = v & 3;
-  const bool xcd = (v & 4) != 0;
+  const bool xcd = (v & 4) != 0 || v == 0;
   bool pair = variant == 2 || (variant == 0 && sizeof(TV) == 8);
   if (m->hksz % 2 != 0) pair = false;
   if (m->hksz == 64) hll_launch_hk<T, TV, 64>(m, pair, xcd, val, x, alpha, beta, y);
@@ -1285,6 +1291,12 @@ int psb_hip_csr_get_diag(void* dmat, void* dv) {
                        m->ja, (const float*)m->val, (float*)d->v);
   PSB_HIP_LAUNCH_CHECK();
   return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_hll_set_xcd_chunk(int blocks) {
+  const int old = g_hll_xchunk;
+  if (blocks >= 0) g_hll_xchunk = blocks;
+  return old;
 }
 
 int psb_hip_csr_set_variant(int variant) {
