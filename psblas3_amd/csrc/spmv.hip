@@ -113,8 +113,8 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmm(int nr, int hksz, int nlog,
                                                      const T* __restrict__ val,
                                                      const T* __restrict__ x, int ldx,
                                                      T alpha, T beta, T* __restrict__ y,
-                                                     int ldy, int ncols) {
-  const int lb = XCD ? xcd_block(nlog) : (int)blockIdx.x;
+                                                     int ldy, int ncols, int xchunk) {
+  const int lb = XCD ? xcd_block_chunk(nlog, xchunk) : (int)blockIdx.x;
   if (lb >= nlog) return;
   const int row = lb * kBlock + threadIdx.x;
   const int r = row < nr ? row : nr - 1;
@@ -503,12 +503,14 @@ template <typename T>
 static int hll_spmm_launch(const HllDev* m, const T* val, const T* x, int ldx, T alpha, T beta,
                            T* y, int ldy, int ncols) {
   hipStream_t s = ctx().stream;
-  // XCD-contiguous block order: measured 2-6% faster than launch order for
-  // k = 8, 16 and 32 columns on the L=200 stencil (each XCD's L2 keeps the
-  // neighbouring planes of all columns of its rows)
+  // XCD block order, in chunks of g_hll_xchunk blocks: one contiguous range
+  // per XCD measured 2-6% faster than launch order for k = 8, 16 and 32
+  // columns on the L=200 stencil, chunks of 16 blocks another 2-5% (485.7 vs
+  // 496.4-521.8 us for k = 8, 1782.8 vs 1873.4-1892.6 us for k = 32)
   const bool xcd = true;
   const int nlog = grid_for(m->rows, kBlock);
-  dim3 grid(xcd ? xcd_grid(nlog) : nlog), blk(kBlock);
+  const int xch = g_hll_xchunk;
+  dim3 grid(xcd ? xcd_chunk_grid(nlog, xch) : nlog), blk(kBlock);
   for (int c0 = 0; c0 < ncols; c0 += 32) {
     const int nc = ncols - c0 < 32 ? ncols - c0 : 32;
     const T* xc = x + (size_t)c0 * ldx;
@@ -517,10 +519,10 @@ static int hll_spmm_launch(const HllDev* m, const T* val, const T* x, int ldx, T
   do {                                                                                         \
     if (xcd)                                                                                   \
       hipLaunchKernelGGL((k_hll_spmm<T, NC, true>), grid, blk, 0, s, m->rows, m->hksz, nlog,   \
-                         m->hkoffs, m->ja, val, xc, ldx, alpha, beta, yc, ldy, nc);            \
+                         m->hkoffs, m->ja, val, xc, ldx, alpha, beta, yc, ldy, nc, xch);       \
     else                                                                                       \
       hipLaunchKernelGGL((k_hll_spmm<T, NC, false>), grid, blk, 0, s, m->rows, m->hksz, nlog,  \
-                         m->hkoffs, m->ja, val, xc, ldx, alpha, beta, yc, ldy, nc);            \
+                         m->hkoffs, m->ja, val, xc, ldx, alpha, beta, yc, ldy, nc, 0);         \
   } while (0)
     if (nc > 16) PSB_SPMM_CASE(32);
     else if (nc > 8) PSB_SPMM_CASE(16);
