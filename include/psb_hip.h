@@ -296,9 +296,11 @@ int psb_hip_hll_get_diag(void *dmat, void *d);
    1 = one row per lane, 2 = two adjacent rows per lane (even hksz), both in
    launch order; + 4 = XCD block order.  Returns the previous value. */
 int psb_hip_hll_set_variant(int variant);
-/* XCD block order: logical blocks dealt to the XCDs in chunks of `blocks`
-   (default 16; 0: one contiguous range per XCD).  No counterpart in the
-   reference.  Returns the previous value. */
+/* XCD block order of the HLL (variants 0 and + 4), ELL, multi-RHS HLL and
+   row-block CSR products: logical blocks dealt to the XCDs in chunks of
+   `blocks` (default 16; 0: one contiguous range per XCD, for the row-block
+   CSR kernel launch order).  No counterpart in the reference.  Returns the
+   previous value. */
 int psb_hip_hll_set_xcd_chunk(int blocks);
 /* Autotuner (no counterpart in the reference): times y = A*x with each HLL
    kernel variant (1, 2, 5, 6 above; reps launches each, HIP events), stores

@@ -29,8 +29,10 @@ namespace psbhip {
 static int g_hll_variant = 0;
 // XCD order: chunks of 16 blocks dealt to the XCDs in turn (0: one contiguous
 // range per XCD).  On the 7-point stencils at L = 200 and 256 chunks of 4-64
-// blocks measured 1-2% (FP64 pairs) and 2-5% (FP32) faster than launch order,
-// contiguous ranges 3-5% slower.
+// blocks measured 1-2% (FP64 pairs) and 2-5% (FP32) faster than launch order
+// for HLL, contiguous ranges 3-5% slower; for ELL the chunks beat contiguous
+// ranges by 9% (FP64) and 6% (FP32), for the row-block CSR kernel launch
+// order by 1-2%.
 static int g_hll_xchunk = 16;
 // 0 auto (= uniform: the length-binned kernel measured slower on both the
 // L=200 stencil, 514 vs 280 us, and the n=4M power-law matrix, 1355 vs
@@ -161,11 +163,11 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmm(int nr, int hksz, int nlog,
 // ------------------------------------------------------------------ ELL SpMV
 template <typename T, typename TV>
 __global__ __launch_bounds__(kBlock) void k_ell_spmv(int nr, int pitch, int nzm, int nlog,
-                                                     const int* __restrict__ ja,
+                                                     int xchunk, const int* __restrict__ ja,
                                                      const TV* __restrict__ val,
                                                      const T* __restrict__ x, T alpha, T beta,
                                                      T* __restrict__ y) {
-  const int lb = xcd_block(nlog);
+  const int lb = xcd_block_chunk(nlog, xchunk);
   if (lb >= nlog) return;
   const int row = lb * kBlock + threadIdx.x;
   const int r = row < nr ? row : nr - 1;
@@ -219,7 +221,8 @@ __global__ __launch_bounds__(kBlock) void k_csr_vector(int nr, int nlog,
 // the reference's CPU loop (psb_d_csr_csmv, base/serial/impl/
 // psb_d_csr_impl.F90:46-417), so the row sums equal the CPU's bit for bit.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_csr_rowblock(const int2* __restrict__ blk,
+__global__ __launch_bounds__(kBlock) void k_csr_rowblock(int nblk, int xchunk,
+                                                         const int2* __restrict__ blk,
                                                          const int* __restrict__ irp,
                                                          const int* __restrict__ ja,
                                                          const T* __restrict__ val,
@@ -227,7 +230,9 @@ __global__ __launch_bounds__(kBlock) void k_csr_rowblock(const int2* __restrict_
                                                          T* __restrict__ y) {
   constexpr int U = kCsrBlockNnz / kBlock;
   __shared__ T prod[kCsrBlockNnz];
-  const int2 rr = blk[blockIdx.x];
+  const int b = xchunk > 0 ? xcd_block_chunk(nblk, xchunk) : (int)blockIdx.x;
+  if (b >= nblk) return;
+  const int2 rr = blk[b];
   const int e0 = irp[rr.x] - 1, e1 = irp[rr.y] - 1;
   if (e1 > e0) {
     int c[U];
@@ -577,8 +582,10 @@ static int csr_launch_rowblock(CsrDev* m, const T* x, T alpha, T beta, T* y) {
   hipStream_t s = ctx().stream;
   const T* val = (const T*)m->val;
   if (m->nsblk > 0) {
-    hipLaunchKernelGGL((k_csr_rowblock<T>), dim3(m->nsblk), dim3(kBlock), 0, s, m->sblk, m->irp,
-                       m->ja, val, x, alpha, beta, y);
+    const int xc = g_hll_xchunk;
+    const int g = xc > 0 ? xcd_chunk_grid(m->nsblk, xc) : m->nsblk;
+    hipLaunchKernelGGL((k_csr_rowblock<T>), dim3(g), dim3(kBlock), 0, s, m->nsblk, xc, m->sblk,
+                       m->irp, m->ja, val, x, alpha, beta, y);
     PSB_HIP_LAUNCH_CHECK();
   }
   if (m->nlong_b > 0) {
@@ -1103,17 +1110,18 @@ int psb_hip_ell_spmv(void* dmat, double alpha, void* xv, double beta, void* yv) 
   if (m->rows == 0) return PSB_HIP_SUCCESS;
   hipStream_t s = ctx().stream;
   int nlog = grid_for(m->rows, kBlock);
-  dim3 grid(xcd_grid(nlog));
+  const int xc = g_hll_xchunk;
+  dim3 grid(xcd_chunk_grid(nlog, xc));
   // every column of x/y (cuda/elldev.c:195-215 loops count_)
   for (int c = 0; c < x->count; c++) {
     if (m->elem_type == PSB_HIP_TYPE_DOUBLE)
       hipLaunchKernelGGL((k_ell_spmv<double, double>), grid, dim3(kBlock), 0, s, m->rows, m->pitch,
-                         m->nzm, nlog, m->ja, (const double*)m->val,
+                         m->nzm, nlog, xc, m->ja, (const double*)m->val,
                          (const double*)x->v + (size_t)c * x->pitch, alpha, beta,
                          (double*)y->v + (size_t)c * y->pitch);
     else
       hipLaunchKernelGGL((k_ell_spmv<float, float>), grid, dim3(kBlock), 0, s, m->rows, m->pitch,
-                         m->nzm, nlog, m->ja, (const float*)m->val,
+                         m->nzm, nlog, xc, m->ja, (const float*)m->val,
                          (const float*)x->v + (size_t)c * x->pitch, (float)alpha, (float)beta,
                          (float*)y->v + (size_t)c * y->pitch);
     PSB_HIP_LAUNCH_CHECK();
