@@ -244,3 +244,34 @@ def test_halo_buffer_methods(gpu, orc):
         xv.gthzv_x(6, 4, iv)                    # window beyond idx
     xv.free()
     yv.free()
+
+
+def test_stream_kernels(gpu):
+    """The bandwidth probes compute what they claim: the read-only stream's
+    per-block sums add up to the array sum, the SpMV-shaped stream writes
+    y[i] = sum_k a[k*m + i] (16-byte elements)."""
+    print("case stream kernels", flush=True)
+    rng = np.random.default_rng(31)
+    m, ratio = 4096, 12                          # 16-byte elements
+    a = rng.integers(-8, 8, size=(ratio * m * 2)).astype(np.float64)   # exact sums
+    nbytes = a.nbytes
+    da, dy, dp = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    for p, nb in ((da, nbytes), (dy, m * 16), (dp, 1 << 20)):
+        N.call("psb_hip_alloc_buffer", C.byref(p), nb)
+    try:
+        N.call("psb_hip_write_buffer", da, a.ctypes.data, nbytes)
+        N.call("psb_hip_write_buffer", dp, np.zeros(1 << 17).ctypes.data, 1 << 20)
+        N.call("psb_hip_stream_read", da, nbytes, dp)
+        N.call("psb_hip_device_sync")
+        part = np.empty(1 << 17)
+        N.call("psb_hip_read_buffer", part.ctypes.data, dp, 1 << 20)
+        assert part.sum() == a.sum()
+        N.call("psb_hip_stream_mix", da, ratio, m * 16, dy)
+        N.call("psb_hip_device_sync")
+        y = np.empty(2 * m)
+        N.call("psb_hip_read_buffer", y.ctypes.data, dy, m * 16)
+        cols = a[:ratio * m * 2].reshape(ratio, m, 2)      # column k, element i, two doubles
+        assert np.array_equal(y.reshape(m, 2), cols.sum(axis=0))
+    finally:
+        for p in (da, dy, dp):
+            N.call("psb_hip_free_buffer", p)

@@ -279,3 +279,44 @@ def test_csr_variants(gpu, orc, kind):
     assert np.all(np.abs(got - ref0) <= _row_bound(f, x, 1.5, 0.0, np.zeros(coo.nr),
                                                     np.finfo(float).eps))
     a.free()
+
+
+@pytest.mark.parametrize("fmt", ["hll64", "ell", "csr"])
+def test_xcd_block_order_is_a_permutation(gpu, orc, fmt):
+    """Every row is computed by one lane whatever the block order, so the
+    chunked XCD orders (chunks of 1, 4, 16, 64 blocks, or one range per XCD)
+    give the same bits as each other and as launch order; the multi-RHS
+    product too.  Matrix large enough for many chunks per XCD."""
+    from psblas3_amd import _native as N
+    _say(f"case XCD block orders, {fmt}")
+    coo = orc.gen_convdiff(48)                      # 110,592 rows: 216 / 432 blocks
+    a = _mat(coo, MOLDS[fmt]())
+    x = orc.uniform_pm1(4, coo.nc)
+    xk = np.asfortranarray(np.stack([orc.uniform_pm1(5 + q, coo.nc) for q in range(3)], axis=1))
+    outs = []
+    old = N.fn("psb_hip_hll_set_xcd_chunk")(16)
+    try:
+        for chunk in (16, 0, 1, 4, 64):
+            N.fn("psb_hip_hll_set_xcd_chunk")(chunk)
+            yv = P.vect_from(np.zeros(coo.nr))
+            P.psb_spmm(1.0, a, P.vect_from(x), 0.0, yv)
+            y = yv.get_vect()
+            if fmt == "hll64":
+                yk = P.vect_from(np.zeros((coo.nr, 3)))
+                P.psb_spmm(1.0, a, P.vect_from(xk), 0.0, yk)
+                y = np.concatenate([y, np.asarray(yk.get_vect()).ravel(order="F")])
+            outs.append(y)
+        if fmt == "hll64":   # launch order (variant 2) for the single product
+            oldv = N.fn("psb_hip_hll_set_variant")(2)
+            yv = P.vect_from(np.zeros(coo.nr))
+            P.psb_spmm(1.0, a, P.vect_from(x), 0.0, yv)
+            N.fn("psb_hip_hll_set_variant")(oldv)
+            assert np.array_equal(yv.get_vect().view(np.int64), outs[0][:coo.nr].view(np.int64))
+    finally:
+        N.fn("psb_hip_hll_set_xcd_chunk")(old)
+    for y in outs[1:]:
+        assert np.array_equal(y.view(np.int64), outs[0].view(np.int64))
+    irp, ja, val = orc.csr_from_coo(orc.fix_coo(coo))
+    ref = orc.csr_spmv(irp, ja, val, x)
+    assert np.allclose(outs[0][:coo.nr], ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    a.free()
