@@ -293,17 +293,23 @@ class psb_hip_base_vect:
             if keep is not None:
                 self.combuf[:len(keep)] = keep
 
-    def gthzv_x(self, i, n, idx):
-        """d_cuda_gthzv_x (:222-329): returns x(idx(i:i+n-1)) as a host array
-        (gathered on the device, one read back)."""
+    def gthzv_x(self, i, n, idx, y=None):
+        """d_cuda_gthzv_x (:222-329): y(1:n) = x(idx(i:i+n-1)), gathered on
+        the device and read back once; y is a host array (allocated and
+        returned when not given)."""
         didx = self._index_window("gthzv_x", i, n, idx)
+        if y is not None and (len(y) < n or np.asarray(y).dtype != self._dtype):
+            raise N.PsbError(N.INVALID, "gthzv_x", "y shorter than n or of another type")
         out = np.empty(n, dtype=self._dtype)
         if n == 0:
-            return out
+            return out if y is None else y
         self._stage(n)
         N.call("psb_hip_mv_igath", self.dev, 0, n, i, didx, 1, self.dt_buf, 1)
         N.call("psb_hip_read_buffer", out.ctypes.data, self.dt_buf, n * out.itemsize)
-        return out
+        if y is None:
+            return out
+        y[:n] = out
+        return y
 
     def gthzbuf(self, i, n, idx):
         """d_cuda_gthzbuf (:331-382): combuf(i:i+n-1) = x(idx(i:i+n-1))."""

@@ -219,6 +219,9 @@ def test_halo_buffer_methods(gpu, orc):
     idx = np.array([17, 1, 5000, 250, 4999, 3, 1200, 77], np.int32)
     iv = P.psb_i_hip_vect().bld(idx)
     assert np.array_equal(xv.gthzv_x(3, 4, iv), x[idx[2:6] - 1])
+    yh = np.full(6, -1.0)
+    xv.gthzv_x(1, 5, iv, yh)
+    assert np.array_equal(yh[:5], x[idx[:5] - 1]) and yh[5] == -1.0
     xv.new_buffer(len(idx))
     xv.gthzbuf(1, 3, iv)
     xv.gthzbuf(4, 5, iv)
