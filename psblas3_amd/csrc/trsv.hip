@@ -12,8 +12,8 @@
 // The solve runs in level order: each triangle is renumbered so that level l
 // holds positions lev_ptr[l] .. lev_ptr[l+1]-1 (rows ascending inside a
 // level), x is gathered into that order by one full-grid kernel and the
-// solution scattered back by another, so every level reads and writes a
-// contiguous range.
+// solution scattered back by another (the last one also applies alpha and
+// beta), so every level reads and writes a contiguous range.
 //
 // A launch per level costs a few microseconds of dispatch, more than the work
 // of a small level, so runs of consecutive small levels (each at most 2048
@@ -85,11 +85,9 @@ struct IluDev {
   int n = 0;
   TriLevels L, U;
   double* d = nullptr;          // inverse pivots
-  double* tin = nullptr;        // input copy (row order)
   double* xa = nullptr;         // right-hand side of the triangle being solved (level order)
   double* ya = nullptr;         // its solution (level order)
-  double* t1 = nullptr;         // L^-1 tin (row order)
-  double* t2 = nullptr;         // U^-1 (d .* t1) (row order)
+  double* t1 = nullptr;         // L^-1 x (row order)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -112,6 +110,22 @@ __global__ __launch_bounds__(kBlock) void k_perm_scatter(int n, const int* __res
                                                          double* __restrict__ dst) {
   for (int p = blockIdx.x * kBlock + threadIdx.x; p < n; p += gridDim.x * kBlock)
     dst[perm[p]] = src[p];
+}
+
+// y[perm[p]] = alpha * src[p] + beta * y[perm[p]] with the special cases of
+// the library's axpby (alpha or beta 0 or 1), separately rounded
+__global__ __launch_bounds__(kBlock) void k_perm_scatter_axpby(int n, const int* __restrict__ perm,
+                                                               const double* __restrict__ src,
+                                                               double alpha, double beta,
+                                                               double* y) {
+  for (int p = blockIdx.x * kBlock + threadIdx.x; p < n; p += gridDim.x * kBlock) {
+    const int i = perm[p];
+    double r;
+    if (alpha == 0.0) r = (beta == 0.0) ? 0.0 : (beta == 1.0 ? y[i] : beta * y[i]);
+    else if (beta == 0.0) r = (alpha == 1.0) ? src[p] : alpha * src[p];
+    else r = alpha * src[p] + beta * y[i];
+    y[i] = r;
+  }
 }
 
 // One level: positions p0 .. p0+count-1, one thread each.
@@ -461,17 +475,16 @@ static void enqueue_levels(const TriLevels& T, const double* x, double* y, hipSt
   }
 }
 
-// The application after the input copy: tin -> t2 (captured).
+// The captured part of an application: x in L level order (xa) -> U^-1
+// (d .* L^-1 x) in U level order (ya).  The gather of the caller's x and the
+// scatter into the caller's y (with alpha and beta) run around it.
 static void enqueue_apply(IluDev* F, hipStream_t s) {
   const int n = F->n, g = stream_grid(n);
-  hipLaunchKernelGGL(k_perm_gather, dim3(g), dim3(kBlock), 0, s, n, F->L.perm, F->tin,
-                     (const double*)nullptr, F->xa);
   enqueue_levels(F->L, F->xa, F->ya, s);
   hipLaunchKernelGGL(k_perm_scatter, dim3(g), dim3(kBlock), 0, s, n, F->L.perm, F->ya, F->t1);
   hipLaunchKernelGGL(k_perm_gather, dim3(g), dim3(kBlock), 0, s, n, F->U.perm, F->t1, F->d,
                      F->xa);
   enqueue_levels(F->U, F->xa, F->ya, s);
-  hipLaunchKernelGGL(k_perm_scatter, dim3(g), dim3(kBlock), 0, s, n, F->U.perm, F->ya, F->t2);
 }
 
 static void free_tri(TriLevels& T) {
@@ -488,7 +501,7 @@ static void destroy_ilu(IluDev* F) {
   if (F->graph) (void)hipGraphDestroy(F->graph);
   free_tri(F->L);
   free_tri(F->U);
-  void* p[] = {F->d, F->tin, F->xa, F->ya, F->t1, F->t2};
+  void* p[] = {F->d, F->xa, F->ya, F->t1};
   for (void* q : p)
     if (q) (void)hipFree(q);
   delete F;
@@ -516,7 +529,7 @@ int psb_hip_ilu_alloc(void** ilu, int n, const int* lirp, const int* lja, const 
   }
   const size_t nb = sizeof(double) * (size_t)(n > 0 ? n : 1);
   hipError_t e = hipSuccess;
-  double** bufs[] = {&F->d, &F->tin, &F->xa, &F->ya, &F->t1, &F->t2};
+  double** bufs[] = {&F->d, &F->xa, &F->ya, &F->t1};
   for (double** b : bufs)
     if (e == hipSuccess) e = hipMalloc((void**)b, nb);
   if (e == hipSuccess && n > 0)
@@ -575,11 +588,15 @@ int psb_hip_ilu_apply(void* ilu, double alpha, void* xv, double beta, void* yv) 
     return set_error(PSB_HIP_INVALID, __func__, "x/y type or size");
   if (F->n == 0) return PSB_HIP_SUCCESS;
   hipStream_t s = ctx().stream;
-  PSB_HIP_CHECK(hipMemcpyAsync(F->tin, x->v, sizeof(double) * (size_t)F->n,
-                               hipMemcpyDeviceToDevice, s));
+  const int n = F->n, g = stream_grid(n);
+  hipLaunchKernelGGL(k_perm_gather, dim3(g), dim3(kBlock), 0, s, n, F->L.perm,
+                     (const double*)x->v, (const double*)nullptr, F->xa);
+  PSB_HIP_LAUNCH_CHECK();
   PSB_HIP_CHECK(hipGraphLaunch(F->exec, s));
-  MultiVec t2{1, F->n, F->n, PSB_HIP_TYPE_DOUBLE, F->t2};
-  return psb_hip_mv_axpby(F->n, alpha, &t2, beta, y);
+  hipLaunchKernelGGL(k_perm_scatter_axpby, dim3(g), dim3(kBlock), 0, s, n, F->U.perm, F->ya, alpha,
+                     beta, (double*)y->v);
+  PSB_HIP_LAUNCH_CHECK();
+  return PSB_HIP_SUCCESS;
 }
 
 }  // extern "C"
