@@ -21,7 +21,13 @@
  *   - BLAS-1: vecoperation known answers (test/kernel/vecoperation.f90),
  *     math.fsum error bounds;
  *   - Krylov: reference run of psb_d_pde3d L=40 BiCGSTAB+DIAG (80 its,
- *     err 9.197e-7), scipy.sparse.linalg solutions, CG invariants.
+ *     err 9.197e-7), scipy.sparse.linalg solutions, CG invariants;
+ *     FCG / GCR / RGMRES / BiCGSTAB(l): identities with CG and GMRES on
+ *     cases where they coincide, brute-force minimal residual over the
+ *     Krylov space for GCR / RGMRES, direct solves;
+ *   - ILU(0) / BJAC: the defining property (L U equals A on A's pattern),
+ *     exact LU on tridiagonal matrices, dense triangular solves (scipy);
+ *     the library's host factorization equals it bit for bit.
  *   - nrm2: netlib reference-BLAS scaled algorithm; the reference links an
  *     external BLAS (MKL here), whose summation order is not published:
  *     parity unpinned beyond the error bound.
