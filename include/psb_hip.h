@@ -365,6 +365,25 @@ int psb_hip_csr_get_analysis(void *dmat, int *lanes_per_row, int *long_rows,
    rows over 512 entries take one wavefront each.  Returns the previous
    value. */
 int psb_hip_csr_set_variant(int variant);
+/* spsvCSRGDevice (cuda/fcusparse_fct.h:132-200, called by
+   psb_d_cuda_csrg_inner_vect_sv, cuda/impl/psb_d_cuda_csrg_inner_vect_sv.F90):
+   y = alpha * op(T)^-1 x + beta * y for the triangular matrix T held in the
+   CSR matrix (square).  lower / unit are the matrix's triangle flags (the
+   reference reads them from a%is_lower() / a%is_unit()); a non-unit row
+   keeps its diagonal as its last (lower) or first (upper) entry; trans 0
+   'N', 1 'T', 2 'C'.  Results equal psb_d_csr_cssv
+   (base/serial/impl/psb_d_csr_impl.F90:782-1039) bit for bit, for every
+   trans and beta (the reference solves on the device only for 'N' with
+   beta = 0 and copies to the host otherwise).  The level analysis is built
+   from the device copy of the matrix on the first call for a (lower, unit,
+   transposed) combination and kept until psb_hip_csr_write or
+   psb_hip_csr_free; PSB_HIP_INVALID when an entry lies outside the triangle
+   or a non-unit row is empty.  x and y may be the same vector; several
+   columns are solved one after another. */
+int psb_hip_csr_spsv(void *dmat, int lower, int unit, int trans, double alpha, void *x,
+                     double beta, void *y);
+/* levels and workgroup runs of the prepared solve (0, 0 before the first) */
+int psb_hip_csr_spsv_levels(void *dmat, int *levels, int *segments);
 
 /* ===================================================================== */
 /* Device-resident Krylov solvers (no reference counterpart: the reference */

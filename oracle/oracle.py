@@ -106,6 +106,10 @@ def _declare(L):
                                  _f64p, _f64p]
     L.orc_d_csr_trsv_unit.restype = None
     L.orc_d_csr_trsv_unit.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
+    for name, fp, ct in (("orc_d_csr_cssv", _f64p, C.c_double), ("orc_s_csr_cssv", _f32p, C.c_float)):
+        getattr(L, name).restype = None
+        getattr(L, name).argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, ct, _i32p, _i32p, fp, fp,
+                                     ct, fp]
     L.orc_d_bjac_apply.restype = None
     L.orc_d_bjac_apply.argtypes = [C.c_int, _i32p, _i32p, _f64p, _i32p, _i32p, _f64p, _f64p, _f64p,
                                    _f64p]
@@ -420,6 +424,21 @@ def trsv_unit(irp, ja, val, x, lower=True):
     lib().orc_d_csr_trsv_unit(len(irp) - 1, 1 if lower else 0, irp, ja, val,
                               np.ascontiguousarray(x, np.float64), y)
     return y
+
+
+def csr_cssv(irp, ja, val, x, lower=True, unit=False, trans="N", alpha=1.0, beta=0.0, y=None):
+    """psb_[ds]_csr_cssv: alpha * T^-1 x + beta * y for a triangular CSR T
+    (diagonal last in a lower row, first in an upper row, absent if unit);
+    the dtype of val picks the precision."""
+    single = np.asarray(val).dtype == np.float32
+    dt = np.float32 if single else np.float64
+    n = len(irp) - 1
+    out = np.zeros(n, dt) if y is None else np.array(y, dt, copy=True)
+    fn = lib().orc_s_csr_cssv if single else lib().orc_d_csr_cssv
+    fn(n, 1 if lower else 0, 1 if unit else 0, 0 if trans.upper() == "N" else 1, alpha,
+       np.ascontiguousarray(irp, np.int32), np.ascontiguousarray(ja, np.int32),
+       np.ascontiguousarray(val, dt), np.ascontiguousarray(x, dt), beta, out)
+    return out
 
 
 def bjac_apply(f, x):

@@ -828,6 +828,65 @@ EXPORT void orc_d_csr_trsv_unit(int n, int lower, const int *irp, const int *ja,
   }
 }
 
+/* psb_d_csr_cssv (base/serial/impl/psb_d_csr_impl.F90:782-888) with its
+   inner_csrsv (:890-1039): y = alpha * T^-1 x + beta * y for a triangular
+   CSR matrix T.  lower/upper and unit/non-unit as the matrix is flagged; a
+   non-unit row keeps its diagonal as its last (lower) or first (upper)
+   entry and divides by it.  trans 0 is 'N' (row sums accumulated from 0 in
+   CSR order, then x(i) - sum); 1 is 'T' and 'C' (identical for real data:
+   y = x, then each solved y(i) is subtracted, times the row's entries, from
+   the y of their columns, rows in solve order).  alpha == 0 returns beta*y
+   without solving; beta == 0 scales the solution in place (alpha == -1 by
+   negation); otherwise the solution goes through the library axpby. */
+#define ORC_CSR_CSSV(NAME, T, AXPBY)                                            \
+static void NAME##_inner(int tra, int lower, int unit, int n, const int *irp, \
+                         const int *ja, const T *val, const T *x, T *y)       \
+{                                                                              \
+  if (!tra) {                                                                  \
+    for (int t = 0; t < n; t++) {                                              \
+      const int i = lower ? t : n - 1 - t;                                     \
+      int b = irp[i] - 1, e = irp[i + 1] - 1;                                  \
+      if (!unit) { if (lower) e--; else b++; }                                 \
+      T acc = (T)0;                                                            \
+      for (int j = b; j < e; j++) acc = acc + val[j] * y[ja[j] - 1];           \
+      y[i] = unit ? x[i] - acc                                                 \
+                  : (x[i] - acc) / val[lower ? irp[i + 1] - 2 : irp[i] - 1];   \
+    }                                                                          \
+    return;                                                                    \
+  }                                                                            \
+  for (int i = 0; i < n; i++) y[i] = x[i];                                     \
+  for (int t = 0; t < n; t++) {                                                \
+    const int i = lower ? n - 1 - t : t;                                       \
+    int b = irp[i] - 1, e = irp[i + 1] - 1;                                    \
+    if (!unit) {                                                               \
+      if (lower) { y[i] = y[i] / val[e - 1]; e--; }                            \
+      else { y[i] = y[i] / val[b]; b++; }                                      \
+    }                                                                          \
+    const T acc = y[i];                                                        \
+    for (int j = b; j < e; j++) y[ja[j] - 1] = y[ja[j] - 1] - val[j] * acc;    \
+  }                                                                            \
+}                                                                              \
+EXPORT void NAME(int n, int lower, int unit, int trans, T alpha, const int *irp, \
+                 const int *ja, const T *val, const T *x, T beta, T *y)        \
+{                                                                              \
+  if (alpha == (T)0) {                                                         \
+    for (int i = 0; i < n; i++) y[i] = beta == (T)0 ? (T)0 : beta * y[i];      \
+    return;                                                                    \
+  }                                                                            \
+  if (beta == (T)0) {                                                          \
+    NAME##_inner(trans, lower, unit, n, irp, ja, val, x, y);                   \
+    if (alpha == (T)-1) for (int i = 0; i < n; i++) y[i] = -y[i];              \
+    else if (alpha != (T)1) for (int i = 0; i < n; i++) y[i] = alpha * y[i];   \
+    return;                                                                    \
+  }                                                                            \
+  T *tmp = calloc(n > 0 ? n : 1, sizeof(T));                                   \
+  NAME##_inner(trans, lower, unit, n, irp, ja, val, x, tmp);                   \
+  AXPBY(n, alpha, tmp, beta, y);                                               \
+  free(tmp);                                                                   \
+}
+ORC_CSR_CSSV(orc_d_csr_cssv, double, orc_d_axpby)
+ORC_CSR_CSSV(orc_s_csr_cssv, float, orc_s_axpby)
+
 /* The preconditioner of the oracle solvers: DIAG (y = d .* x,
    psb_d_diag_apply_vect) or, when lirp is set, BJAC with ILU(0) factors
    (psb_d_bjac_apply_vect, prec/impl/psb_d_bjacprec_impl.f90:166-173):

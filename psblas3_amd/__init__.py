@@ -17,7 +17,7 @@ from .vect import (psb_d_hip_vect, psb_d_vect_type, psb_i_hip_vect,  # noqa: F40
 from .psblas import (matrix_from_coo, psb_cdall, psb_cdasb, psb_cscnv, psb_desc_type,  # noqa: F401
                      psb_geabs, psb_geall, psb_geamax, psb_geasb, psb_geasum, psb_geaxpby,
                      psb_geaxpbyz, psb_gedot, psb_gefree, psb_gemlt, psb_genrm2, psb_gescal,
-                     psb_spall, psb_spasb, psb_spfree, psb_spins, psb_spmm, psb_upd_xyz,
+                     psb_spall, psb_spasb, psb_spfree, psb_spins, psb_spmm, psb_spsm, psb_upd_xyz,
                      psb_xyzw, vect_from)
 from .prec import psb_dprec_type, psb_precinit  # noqa: F401
 from .krylov import psb_krylov, psb_krylov_fused, psb_krylov_mixed  # noqa: F401

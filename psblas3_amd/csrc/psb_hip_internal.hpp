@@ -331,7 +331,12 @@ struct CsrDev {
   // entries fit the block's LDS), rows longer than kCsrLongRow excluded
   int nsblk;
   int2* sblk;           // (first row, one past the last row), 0-based
+  // triangular solve prepared on first use (level analysis, see trsv.hip)
+  void* sv = nullptr;
 };
+
+// frees the prepared triangular solve of a CSR matrix (trsv.hip)
+void release_csr_sv(CsrDev* m);
 
 // ------------------------------------------------------------------ inner products
 // One group of G slots of a row (slot j at j*stride): acc += val[j] * x[ja[j]-1]

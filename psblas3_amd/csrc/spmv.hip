@@ -1163,6 +1163,7 @@ void psb_hip_csr_free(void* dmat) {
   if (!dmat) return;
   CsrDev* m = (CsrDev*)dmat;
   if (ctx().initialized) hipStreamSynchronize(ctx().stream);
+  release_csr_sv(m);
   hipFree(m->irp);
   hipFree(m->ja);
   hipFree(m->val);
@@ -1176,6 +1177,7 @@ void psb_hip_csr_free(void* dmat) {
 int psb_hip_csr_write(void* dmat, const int* irp, const int* ja, const void* val) {
   PSB_ENSURE_INIT();
   CsrDev* m = (CsrDev*)dmat;
+  release_csr_sv(m);   // a prepared triangular solve describes the old values
   if (irp[0] != 1 || irp[m->rows] - 1 != m->nnz)
     return set_error(PSB_HIP_INVALID, __func__, "irp inconsistent with nnz");
   for (int i = 0; i < m->rows; i++)

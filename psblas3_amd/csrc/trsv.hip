@@ -1,13 +1,26 @@
-// BJAC / ILU(0) preconditioner application on the device:
-// y = alpha * U^-1 (d .* L^-1 x) + beta * y with L unit lower, U unit upper
-// (psb_d_bjac_apply_vect, prec/impl/psb_d_bjacprec_impl.f90:166-173: psb_spsm
-// with scale 'L' and the inverse pivots, then psb_spsm on U).
+// Sparse triangular solves on the device:
+//  - BJAC / ILU(0) preconditioner application:
+//    y = alpha * U^-1 (d .* L^-1 x) + beta * y with L unit lower, U unit upper
+//    (psb_d_bjac_apply_vect, prec/impl/psb_d_bjacprec_impl.f90:166-173:
+//    psb_spsm with scale 'L' and the inverse pivots, then psb_spsm on U);
+//  - the CSR matrix's own solve y = alpha * op(T)^-1 x + beta * y, T lower or
+//    upper, unit or not, op 'N', 'T' or 'C' (psb_d_cuda_csrg_inner_vect_sv,
+//    cuda/impl/psb_d_cuda_csrg_inner_vect_sv.F90; the CPU algorithm it must
+//    equal is psb_d_csr_cssv, base/serial/impl/psb_d_csr_impl.F90:782-1039),
+//    FP64 and FP32.
 //
 // Each triangle is solved by levels: the host analysis gives every row the
 // length of its longest dependency chain.  A row's sum runs in its CSR order,
 // y(i) = x(i) - sum_j val(j) y(ja(j)) with separate multiply and add, as the
 // reference's inner_csrsv (base/serial/impl/psb_d_csr_impl.F90:890-941), so
 // the result equals the sequential CPU solve bit for bit.
+//
+// A non-unit row divides by its diagonal, (x(i) - sum) / a(i,i), as the CPU
+// does.  The transposed solve (op 'T' / 'C') is solved as the triangle of the
+// transpose, whose row c lists the entries (i, c) of T in the order the CPU
+// applies them (rows in solve order, CSR order inside a row); its rows are
+// computed as the CPU updates them, by running subtraction
+// y(c) = ((x(c) - v1 y(i1)) - v2 y(i2)) ..., then the division.
 //
 // The solve runs in level order: each triangle is renumbered so that level l
 // holds positions lev_ptr[l] .. lev_ptr[l+1]-1 (rows ascending inside a
@@ -55,17 +68,18 @@ constexpr int kSinkSlot = kSegRows - 2;  // written by lanes beyond their level,
 constexpr int kZeroSlot = kSegRows - 1;  // holds 0, read by padding entries
 constexpr int kSegLevels = 1024;         // levels of a run (their descriptors sit in LDS)
 
+template <typename T>
 struct TriLevels {
   int n = 0, nnz = 0, nlev = 0;
   int* perm = nullptr;          // level position -> row
   int* irp = nullptr;           // level-ordered CSR, 0-based; columns are level positions
   int* ja = nullptr;
-  double* val = nullptr;
+  T* val = nullptr;
   std::vector<int> lev_ptr;     // host: level l = positions lev_ptr[l] .. lev_ptr[l+1]-1
   // per-level column-major copies for the workgroup-solved runs
   int4* lev_info = nullptr;     // {count, width, entry offset, first position} per level
   uint16_t* eja = nullptr;      // LDS slot of each dependency (kSegRows <= 65536)
-  double* eval = nullptr;
+  T* eval = nullptr;
   int* halo = nullptr;          // positions before each run that the run reads
   std::vector<TriSegment> segs; // host: the runs, in level order
   std::vector<int> seg_of;      // host: segment index of a level, -1 if launched alone
@@ -83,7 +97,7 @@ constexpr int kSegWidths[] = {2, 3, 4, 8};   // the entry widths a run can be bu
 
 struct IluDev {
   int n = 0;
-  TriLevels L, U;
+  TriLevels<double> L, U;
   double* d = nullptr;          // inverse pivots
   double* xa = nullptr;         // right-hand side of the triangle being solved (level order)
   double* ya = nullptr;         // its solution (level order)
@@ -92,12 +106,28 @@ struct IluDev {
   hipGraphExec_t exec = nullptr;
 };
 
+// A CSR matrix's triangle prepared for op(T)^-1 x: the strict part (or, for
+// op 'T' / 'C', the strict part of the transpose) in level order, and the
+// diagonal in the same order when the triangle is not unit.
+template <typename T>
+struct TriSolve {
+  int n = 0, key = -1;          // key: lower | unit << 1 | transposed << 2
+  bool run = false;             // transposed: rows by running subtraction
+  TriLevels<T> tl;
+  T* dv = nullptr;              // diagonal in level order (non-unit), else null
+  T* xa = nullptr;              // right-hand side (level order)
+  T* ya = nullptr;              // solution (level order)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
 // dst[p] = src[perm[p]], times d[perm[p]] when d is given (the pivot scaling
 // y(i) * d(i) of the reference's 'L' scale)
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_perm_gather(int n, const int* __restrict__ perm,
-                                                        const double* __restrict__ src,
-                                                        const double* __restrict__ d,
-                                                        double* __restrict__ dst) {
+                                                        const T* __restrict__ src,
+                                                        const T* __restrict__ d,
+                                                        T* __restrict__ dst) {
   for (int p = blockIdx.x * kBlock + threadIdx.x; p < n; p += gridDim.x * kBlock) {
     const int i = perm[p];
     dst[p] = d ? src[i] * d[i] : src[i];
@@ -105,42 +135,53 @@ __global__ __launch_bounds__(kBlock) void k_perm_gather(int n, const int* __rest
 }
 
 // dst[perm[p]] = src[p]
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_perm_scatter(int n, const int* __restrict__ perm,
-                                                         const double* __restrict__ src,
-                                                         double* __restrict__ dst) {
+                                                         const T* __restrict__ src,
+                                                         T* __restrict__ dst) {
   for (int p = blockIdx.x * kBlock + threadIdx.x; p < n; p += gridDim.x * kBlock)
     dst[perm[p]] = src[p];
 }
 
 // y[perm[p]] = alpha * src[p] + beta * y[perm[p]] with the special cases of
-// the library's axpby (alpha or beta 0 or 1), separately rounded
+// the library's axpby (alpha or beta 0 or 1), separately rounded; beta == 0
+// and alpha == -1 negate, as the CPU solve's scaling does
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_perm_scatter_axpby(int n, const int* __restrict__ perm,
-                                                               const double* __restrict__ src,
-                                                               double alpha, double beta,
-                                                               double* y) {
+                                                               const T* __restrict__ src,
+                                                               T alpha, T beta, T* y) {
   for (int p = blockIdx.x * kBlock + threadIdx.x; p < n; p += gridDim.x * kBlock) {
     const int i = perm[p];
-    double r;
-    if (alpha == 0.0) r = (beta == 0.0) ? 0.0 : (beta == 1.0 ? y[i] : beta * y[i]);
-    else if (beta == 0.0) r = (alpha == 1.0) ? src[p] : alpha * src[p];
+    T r;
+    if (alpha == T(0)) r = (beta == T(0)) ? T(0) : (beta == T(1) ? y[i] : beta * y[i]);
+    else if (beta == T(0)) r = (alpha == T(1)) ? src[p] : alpha * src[p];
     else r = alpha * src[p] + beta * y[i];
     y[i] = r;
   }
 }
 
-// One level: positions p0 .. p0+count-1, one thread each.
+// One level: positions p0 .. p0+count-1, one thread each.  RUN: running
+// subtraction from x (the transposed solve); DIV: divide by the diagonal.
+template <typename T, bool DIV, bool RUN>
 __global__ __launch_bounds__(kBlock) void k_trsv_level(int p0, int count,
                                                        const int* __restrict__ irp,
                                                        const int* __restrict__ ja,
-                                                       const double* __restrict__ val,
-                                                       const double* __restrict__ x,
-                                                       double* y) {
+                                                       const T* __restrict__ val,
+                                                       const T* __restrict__ dv,
+                                                       const T* __restrict__ x, T* y) {
   const int t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= count) return;
   const int p = p0 + t;
-  double acc = 0.0;
-  for (int e = irp[p]; e < irp[p + 1]; e++) acc = acc + val[e] * y[ja[e]];
-  y[p] = x[p] - acc;
+  T r;
+  if (RUN) {
+    r = x[p];
+    for (int e = irp[p]; e < irp[p + 1]; e++) r = r - val[e] * y[ja[e]];
+  } else {
+    T acc = T(0);
+    for (int e = irp[p]; e < irp[p + 1]; e++) acc = acc + val[e] * y[ja[e]];
+    r = x[p] - acc;
+  }
+  y[p] = DIV ? r / dv[p] : r;
 }
 
 // Levels [lev0, lev1) by one workgroup.  Row r of thread t in a level is
@@ -149,20 +190,23 @@ __global__ __launch_bounds__(kBlock) void k_trsv_level(int p0, int count,
 // nor the stores are behind a branch, so a level's LDS reads are in flight
 // together.  The sum runs over the row's entries in CSR order, then the
 // padding adds +0 to it (never -0: the sum starts at +0), which leaves it
-// unchanged.
-template <int W>
+// unchanged; in the running form the padding subtracts +0, which leaves
+// every value, -0 included, unchanged.
+template <typename T, int W, bool DIV>
 struct SegStage {
   static constexpr int R = seg_rpt(W);
   int dep[R][W];
-  double v[R][W];
+  T v[R][W];
+  T dv[DIV ? R : 1];
 };
 
-template <int W>
-__device__ __forceinline__ void seg_fetch(SegStage<W>& S, const int4 L,
+template <typename T, int W, bool DIV>
+__device__ __forceinline__ void seg_fetch(SegStage<T, W, DIV>& S, const int4 L,
                                           const uint16_t* __restrict__ eja,
-                                          const double* __restrict__ eval) {
+                                          const T* __restrict__ eval,
+                                          const T* __restrict__ dv) {
 #pragma unroll
-  for (int r = 0; r < SegStage<W>::R; r++) {
+  for (int r = 0; r < SegStage<T, W, DIV>::R; r++) {
     const int tt = min((int)threadIdx.x + r * kSegThreads, L.x - 1);
 #pragma unroll
     for (int k = 0; k < W; k++) {
@@ -170,6 +214,7 @@ __device__ __forceinline__ void seg_fetch(SegStage<W>& S, const int4 L,
       S.dep[r][k] = eja[e];
       S.v[r][k] = eval[e];
     }
+    if (DIV) S.dv[r] = dv[L.w + tt];
   }
 }
 
@@ -179,23 +224,31 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-template <int W>
-__device__ __forceinline__ void seg_solve(const SegStage<W>& S, const int4 L, int base,
-                                          double* ys) {
-  constexpr int R = SegStage<W>::R;
+template <typename T, int W, bool DIV, bool RUN>
+__device__ __forceinline__ void seg_solve(const SegStage<T, W, DIV>& S, const int4 L, int base,
+                                          T* ys) {
+  constexpr int R = SegStage<T, W, DIV>::R;
   lds_barrier();   // the LDS slots of the levels before this one are written
-  double yi[R];
+  T yi[R];
 #pragma unroll
   for (int r = 0; r < R; r++) {
     const int tt = min((int)threadIdx.x + r * kSegThreads, L.x - 1);
-    double yk[W];
+    T yk[W];
 #pragma unroll
     for (int k = 0; k < W; k++) yk[k] = ys[S.dep[r][k]];
-    const double xi = ys[L.w - base + tt];
-    double acc = 0.0;
+    const T xi = ys[L.w - base + tt];
+    T v;
+    if (RUN) {
+      v = xi;
 #pragma unroll
-    for (int k = 0; k < W; k++) acc = acc + S.v[r][k] * yk[k];
-    yi[r] = xi - acc;
+      for (int k = 0; k < W; k++) v = v - S.v[r][k] * yk[k];
+    } else {
+      T acc = T(0);
+#pragma unroll
+      for (int k = 0; k < W; k++) acc = acc + S.v[r][k] * yk[k];
+      v = xi - acc;
+    }
+    yi[r] = DIV ? v / S.dv[r] : v;
   }
 #pragma unroll
   for (int r = 0; r < R; r++) {
@@ -204,42 +257,43 @@ __device__ __forceinline__ void seg_solve(const SegStage<W>& S, const int4 L, in
   }
 }
 
-// x and y in level order; the run's positions are base .. base+nrun-1.
-template <int W, int NS>
+// x and y in level order; the run's positions are base .. base+nrun-1; dv
+// the diagonal in level order when DIV.
+template <typename T, int W, int NS, bool DIV, bool RUN>
 __global__ __launch_bounds__(kSegThreads) void k_trsv_seg(int lev0, int lev1, int base, int nrun,
                                                           const int* __restrict__ halo, int nhalo,
                                                           const int4* __restrict__ info,
                                                           const uint16_t* __restrict__ eja,
-                                                          const double* __restrict__ eval,
-                                                          const double* __restrict__ x,
-                                                          double* y) {
-  __shared__ double ys[kSegRows];
+                                                          const T* __restrict__ eval,
+                                                          const T* __restrict__ dv,
+                                                          const T* __restrict__ x, T* y) {
+  __shared__ T ys[kSegRows];
   __shared__ int4 linfo[kSegLevels];   // the run's level descriptors: no scalar load per level
   for (int q = threadIdx.x; q < lev1 - lev0; q += kSegThreads) linfo[q] = info[lev0 + q];
   for (int p = threadIdx.x; p < nrun; p += kSegThreads) ys[p] = x[base + p];
   for (int h = threadIdx.x; h < nhalo; h += kSegThreads) ys[nrun + h] = y[halo[h]];
-  if (threadIdx.x == 0) ys[kZeroSlot] = 0.0;
+  if (threadIdx.x == 0) ys[kZeroSlot] = T(0);
   lds_barrier();
   // Entries NS-1 levels ahead: NS register stages rotate, the one solved
   // holds level l.  The loop runs whole rounds of NS (the stage indices are
   // compile-time constants once unrolled) and the fetches are unconditional
   // (past the run's end they repeat its last level), so every path leaves the
   // same loads in flight and the compiler's waits stay exact.
-  SegStage<W> st[NS];
+  SegStage<T, W, DIV> st[NS];
   int4 Ls[NS];
   auto level = [&](int l) { return linfo[min(l, lev1 - 1) - lev0]; };
 #pragma unroll
   for (int q = 0; q < NS - 1; q++) {
     Ls[q] = level(lev0 + q);
-    seg_fetch<W>(st[q], Ls[q], eja, eval);
+    seg_fetch<T, W, DIV>(st[q], Ls[q], eja, eval, dv);
   }
   for (int l = lev0; l < lev1; l += NS) {
 #pragma unroll
     for (int j = 0; j < NS; j++) {
       const int ahead = (j + NS - 1) % NS;
       Ls[ahead] = level(l + j + NS - 1);
-      seg_fetch<W>(st[ahead], Ls[ahead], eja, eval);
-      if (l + j < lev1) seg_solve<W>(st[j], Ls[j], base, ys);   // uniform
+      seg_fetch<T, W, DIV>(st[ahead], Ls[ahead], eja, eval, dv);
+      if (l + j < lev1) seg_solve<T, W, DIV, RUN>(st[j], Ls[j], base, ys);   // uniform
     }
   }
   lds_barrier();
@@ -271,7 +325,8 @@ static void analyse(int n, bool lower, const int* irp, const int* ja, std::vecto
 // at most seg_rpt(W) * kSegThreads rows of at most W entries, W one of
 // kSegWidths; the run's rows plus its halo fill at most kSegRows - 2 LDS
 // slots, and it has at most kSegLevels levels.
-static void plan_segments(TriLevels& T, const int* irp, const int* ja,
+template <typename V>
+static void plan_segments(TriLevels<V>& T, const int* irp, const int* ja,
                           const std::vector<int>& rows, const std::vector<int>& lev) {
   T.segs.clear();
   T.seg_of.assign(T.nlev, -1);
@@ -332,7 +387,8 @@ static void plan_segments(TriLevels& T, const int* irp, const int* ja,
   }
 }
 
-static int upload_segments(TriLevels& T, const int* irp, const int* ja, const double* val,
+template <typename V>
+static int upload_segments(TriLevels<V>& T, const int* irp, const int* ja, const V* val,
                            const std::vector<int>& rows, const std::vector<int>& pos,
                            const std::vector<int>& lev) {
   if (T.segs.empty()) return PSB_HIP_SUCCESS;
@@ -344,9 +400,9 @@ static int upload_segments(TriLevels& T, const int* irp, const int* ja, const do
       info[l] = make_int4(c, S.width, (int)std::min<int64_t>(ent, INT32_MAX), T.lev_ptr[l]);
       ent += (int64_t)c * S.width;
     }
-  if (ent > INT32_MAX) return set_error(PSB_HIP_UNSUPPORTED, "ilu", "triangle too large");
+  if (ent > INT32_MAX) return set_error(PSB_HIP_UNSUPPORTED, "trsv", "triangle too large");
   std::vector<uint16_t> eja(ent, kZeroSlot);   // padding: the zero slot, value 0
-  std::vector<double> eval(ent, 0.0);
+  std::vector<V> eval(ent, V(0));
   std::vector<int> halo, slot(T.n > 0 ? T.n : 1, -1);
   for (TriSegment& S : T.segs) {
     const int base = T.lev_ptr[S.lev0], nrun = T.lev_ptr[S.lev1] - base;
@@ -376,18 +432,18 @@ static int upload_segments(TriLevels& T, const int* irp, const int* ja, const do
     S.nhalo = (int)halo.size() - S.hoff;
     for (int h = S.hoff; h < (int)halo.size(); h++) slot[rows[halo[h]]] = -1;
     if (nrun + S.nhalo > kSegRows - 2)
-      return set_error(PSB_HIP_RUNTIME, "ilu", "run larger than its LDS slots");
+      return set_error(PSB_HIP_RUNTIME, "trsv", "run larger than its LDS slots");
   }
   PSB_HIP_CHECK(hipMalloc(&T.lev_info, sizeof(int4) * (size_t)T.nlev));
   PSB_HIP_CHECK(hipMalloc(&T.eja, sizeof(uint16_t) * eja.size()));
-  PSB_HIP_CHECK(hipMalloc(&T.eval, sizeof(double) * eval.size()));
+  PSB_HIP_CHECK(hipMalloc(&T.eval, sizeof(V) * eval.size()));
   PSB_HIP_CHECK(hipMalloc(&T.halo, sizeof(int) * std::max<size_t>(halo.size(), 1)));
   hipStream_t s = ctx().stream;
   PSB_HIP_CHECK(hipMemcpyAsync(T.lev_info, info.data(), sizeof(int4) * info.size(),
                                hipMemcpyHostToDevice, s));
   PSB_HIP_CHECK(hipMemcpyAsync(T.eja, eja.data(), sizeof(uint16_t) * eja.size(),
                                hipMemcpyHostToDevice, s));
-  PSB_HIP_CHECK(hipMemcpyAsync(T.eval, eval.data(), sizeof(double) * eval.size(),
+  PSB_HIP_CHECK(hipMemcpyAsync(T.eval, eval.data(), sizeof(V) * eval.size(),
                                hipMemcpyHostToDevice, s));
   if (!halo.empty())
     PSB_HIP_CHECK(hipMemcpyAsync(T.halo, halo.data(), sizeof(int) * halo.size(),
@@ -396,15 +452,16 @@ static int upload_segments(TriLevels& T, const int* irp, const int* ja, const do
   return PSB_HIP_SUCCESS;
 }
 
-static int upload_tri(TriLevels& T, int n, bool lower, const int* irp, const int* ja,
-                      const double* val, bool segments) {
+template <typename V>
+static int upload_tri(TriLevels<V>& T, int n, bool lower, const int* irp, const int* ja,
+                      const V* val, bool segments) {
   T.n = n;
   T.nnz = irp[n] - 1;
   for (int i = 0; i < n; i++)
     for (int p = irp[i] - 1; p < irp[i + 1] - 1; p++) {
       const int c = ja[p];
       if (c < 1 || c > n || (lower ? c > i : c < i + 2))
-        return set_error(PSB_HIP_INVALID, "ilu", "factor entry outside its triangle");
+        return set_error(PSB_HIP_INVALID, "trsv", "entry outside its triangle");
     }
   std::vector<int> rows, lev;
   analyse(n, lower, irp, ja, rows, T.lev_ptr, lev);
@@ -414,7 +471,7 @@ static int upload_tri(TriLevels& T, int n, bool lower, const int* irp, const int
   // the triangle in level order: row rows[p] becomes position p, its entries
   // keep their CSR order and point at the positions of their columns
   std::vector<int> pirp(n + 1, 0), pja(T.nnz > 0 ? T.nnz : 1, 0);
-  std::vector<double> pval(T.nnz > 0 ? T.nnz : 1, 0.0);
+  std::vector<V> pval(T.nnz > 0 ? T.nnz : 1, V(0));
   for (int p = 0; p < n; p++) {
     const int i = rows[p], b = irp[i] - 1, c = irp[i + 1] - 1 - b;
     for (int k = 0; k < c; k++) {
@@ -425,14 +482,14 @@ static int upload_tri(TriLevels& T, int n, bool lower, const int* irp, const int
   }
   PSB_HIP_CHECK(hipMalloc(&T.irp, sizeof(int) * pirp.size()));
   PSB_HIP_CHECK(hipMalloc(&T.ja, sizeof(int) * pja.size()));
-  PSB_HIP_CHECK(hipMalloc(&T.val, sizeof(double) * pval.size()));
+  PSB_HIP_CHECK(hipMalloc(&T.val, sizeof(V) * pval.size()));
   PSB_HIP_CHECK(hipMalloc(&T.perm, sizeof(int) * (size_t)(n > 0 ? n : 1)));
   hipStream_t s = ctx().stream;
   PSB_HIP_CHECK(hipMemcpyAsync(T.irp, pirp.data(), sizeof(int) * pirp.size(),
                                hipMemcpyHostToDevice, s));
   PSB_HIP_CHECK(hipMemcpyAsync(T.ja, pja.data(), sizeof(int) * pja.size(), hipMemcpyHostToDevice,
                                s));
-  PSB_HIP_CHECK(hipMemcpyAsync(T.val, pval.data(), sizeof(double) * pval.size(),
+  PSB_HIP_CHECK(hipMemcpyAsync(T.val, pval.data(), sizeof(V) * pval.size(),
                                hipMemcpyHostToDevice, s));
   if (n > 0)
     PSB_HIP_CHECK(hipMemcpyAsync(T.perm, rows.data(), sizeof(int) * (size_t)n,
@@ -447,17 +504,20 @@ static int upload_tri(TriLevels& T, int n, bool lower, const int* irp, const int
   return upload_segments(T, irp, ja, val, rows, pos, lev);
 }
 
-// x and y in level order
-static void enqueue_levels(const TriLevels& T, const double* x, double* y, hipStream_t s) {
+// x and y in level order; dv the diagonal in level order (null: unit);
+// run: rows by running subtraction (the transposed solve)
+template <typename V, bool DIV, bool RUN>
+static void enqueue_levels_t(const TriLevels<V>& T, const V* dv, const V* x, V* y,
+                             hipStream_t s) {
   for (int l = 0; l < T.nlev;) {
     const int g = T.seg_of[l];
     if (g >= 0) {
       const TriSegment& S = T.segs[g];
       const int base = T.lev_ptr[S.lev0], nrun = T.lev_ptr[S.lev1] - base;
-#define PSB_SEG_LAUNCH(W)                                                                   \
-  hipLaunchKernelGGL((k_trsv_seg<W, kSegStages>), dim3(1), dim3(kSegThreads), 0, s, S.lev0,   \
-                     S.lev1, base, nrun, T.halo + S.hoff, S.nhalo, T.lev_info, T.eja, T.eval, x, \
-                     y)
+#define PSB_SEG_LAUNCH(W)                                                                     \
+  hipLaunchKernelGGL((k_trsv_seg<V, W, kSegStages, DIV, RUN>), dim3(1), dim3(kSegThreads), 0, s, \
+                     S.lev0, S.lev1, base, nrun, T.halo + S.hoff, S.nhalo, T.lev_info, T.eja,   \
+                     T.eval, dv, x, y)
       switch (S.width) {
         case 2: PSB_SEG_LAUNCH(2); break;
         case 3: PSB_SEG_LAUNCH(3); break;
@@ -469,10 +529,19 @@ static void enqueue_levels(const TriLevels& T, const double* x, double* y, hipSt
       continue;
     }
     const int count = T.lev_ptr[l + 1] - T.lev_ptr[l];
-    hipLaunchKernelGGL(k_trsv_level, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                       T.lev_ptr[l], count, T.irp, T.ja, T.val, x, y);
+    hipLaunchKernelGGL((k_trsv_level<V, DIV, RUN>), dim3((count + kBlock - 1) / kBlock),
+                       dim3(kBlock), 0, s, T.lev_ptr[l], count, T.irp, T.ja, T.val, dv, x, y);
     l++;
   }
+}
+
+template <typename V>
+static void enqueue_levels(const TriLevels<V>& T, const V* dv, bool run, const V* x, V* y,
+                           hipStream_t s) {
+  if (dv && run) enqueue_levels_t<V, true, true>(T, dv, x, y, s);
+  else if (dv) enqueue_levels_t<V, true, false>(T, dv, x, y, s);
+  else if (run) enqueue_levels_t<V, false, true>(T, dv, x, y, s);
+  else enqueue_levels_t<V, false, false>(T, dv, x, y, s);
 }
 
 // The captured part of an application: x in L level order (xa) -> U^-1
@@ -480,18 +549,20 @@ static void enqueue_levels(const TriLevels& T, const double* x, double* y, hipSt
 // scatter into the caller's y (with alpha and beta) run around it.
 static void enqueue_apply(IluDev* F, hipStream_t s) {
   const int n = F->n, g = stream_grid(n);
-  enqueue_levels(F->L, F->xa, F->ya, s);
-  hipLaunchKernelGGL(k_perm_scatter, dim3(g), dim3(kBlock), 0, s, n, F->L.perm, F->ya, F->t1);
-  hipLaunchKernelGGL(k_perm_gather, dim3(g), dim3(kBlock), 0, s, n, F->U.perm, F->t1, F->d,
-                     F->xa);
-  enqueue_levels(F->U, F->xa, F->ya, s);
+  enqueue_levels<double>(F->L, nullptr, false, F->xa, F->ya, s);
+  hipLaunchKernelGGL(k_perm_scatter<double>, dim3(g), dim3(kBlock), 0, s, n, F->L.perm, F->ya,
+                     F->t1);
+  hipLaunchKernelGGL(k_perm_gather<double>, dim3(g), dim3(kBlock), 0, s, n, F->U.perm, F->t1,
+                     F->d, F->xa);
+  enqueue_levels<double>(F->U, nullptr, false, F->xa, F->ya, s);
 }
 
-static void free_tri(TriLevels& T) {
+template <typename V>
+static void free_tri(TriLevels<V>& T) {
   void* p[] = {T.perm, T.irp, T.ja, T.val, T.lev_info, T.eja, T.eval, T.halo};
   for (void* q : p)
     if (q) (void)hipFree(q);
-  T = TriLevels{};
+  T = TriLevels<V>{};
 }
 
 static void destroy_ilu(IluDev* F) {
@@ -505,6 +576,171 @@ static void destroy_ilu(IluDev* F) {
   for (void* q : p)
     if (q) (void)hipFree(q);
   delete F;
+}
+
+template <typename V>
+static void destroy_sv(TriSolve<V>* S) {
+  if (!S) return;
+  if (ctx().initialized) (void)hipStreamSynchronize(ctx().stream);
+  if (S->exec) (void)hipGraphExecDestroy(S->exec);
+  if (S->graph) (void)hipGraphDestroy(S->graph);
+  free_tri(S->tl);
+  void* p[] = {S->dv, S->xa, S->ya};
+  for (void* q : p)
+    if (q) (void)hipFree(q);
+  delete S;
+}
+
+// Prepares op(T)^-1 for the CSR triangle (irp, ja, val), 1-based host
+// arrays: the diagonal of a non-unit row (its last entry if lower, first if
+// upper, the entries psb_d_csr_cssv divides by) is taken out, the rest must
+// lie strictly inside the triangle.  For op 'T' / 'C' the strict part is
+// transposed with row c's entries in the CPU's update order: the rows i of T
+// in solve order (descending for a lower triangle), CSR order within a row.
+template <typename V>
+static int build_sv(TriSolve<V>* S, int n, const int* irp, const int* ja, const V* val,
+                    bool lower, bool unit, bool trans) {
+  S->n = n;
+  S->run = trans;
+  const int nnz = irp[n] - 1;
+  std::vector<int> sirp(n + 1, 1), sja;
+  std::vector<V> sval, dv;
+  sja.reserve(nnz > 0 ? nnz : 1);
+  sval.reserve(nnz > 0 ? nnz : 1);
+  if (!unit) dv.resize(n > 0 ? n : 1);
+  for (int i = 0; i < n; i++) {
+    int b = irp[i] - 1, e = irp[i + 1] - 1;
+    if (!unit) {
+      if (e <= b) return set_error(PSB_HIP_INVALID, "trsv", "non-unit row without entries");
+      if (lower) dv[i] = val[--e];
+      else dv[i] = val[b++];
+    }
+    for (int k = b; k < e; k++) {
+      sja.push_back(ja[k]);
+      sval.push_back(val[k]);
+    }
+    sirp[i + 1] = (int)sja.size() + 1;
+  }
+  bool tlower = lower;
+  if (trans) {
+    for (int i = 0; i < n; i++)   // the strict part must be a triangle before it is turned
+      for (int k = sirp[i] - 1; k < sirp[i + 1] - 1; k++)
+        if (sja[k] < 1 || sja[k] > n || (lower ? sja[k] > i : sja[k] < i + 2))
+          return set_error(PSB_HIP_INVALID, "trsv", "entry outside its triangle");
+    const int snz = sirp[n] - 1;
+    std::vector<int> cnt(n + 1, 0);
+    for (int k = 0; k < snz; k++) cnt[sja[k]]++;
+    std::vector<int> tirp(n + 1, 1);
+    for (int c = 0; c < n; c++) tirp[c + 1] = tirp[c] + cnt[c + 1];
+    std::vector<int> fill(tirp.begin(), tirp.end() - 1), tja(snz > 0 ? snz : 1);
+    std::vector<V> tval(snz > 0 ? snz : 1);
+    for (int t = 0; t < n; t++) {
+      const int i = lower ? n - 1 - t : t;   // the CPU's order of the rows
+      for (int k = sirp[i] - 1; k < sirp[i + 1] - 1; k++) {
+        const int q = fill[sja[k] - 1]++ - 1;
+        tja[q] = i + 1;
+        tval[q] = sval[k];
+      }
+    }
+    sirp.swap(tirp);
+    sja.swap(tja);
+    sval.swap(tval);
+    tlower = !lower;
+  }
+  int rc = upload_tri(S->tl, n, tlower, sirp.data(), sja.data(), sval.data(),
+                      g_trsv_variant != 1);
+  if (rc) return rc;
+  const size_t nb = sizeof(V) * (size_t)(n > 0 ? n : 1);
+  hipError_t e = hipSuccess;
+  V** bufs[] = {&S->xa, &S->ya};
+  for (V** b : bufs)
+    if (e == hipSuccess) e = hipMalloc((void**)b, nb);
+  hipStream_t s = ctx().stream;
+  if (e == hipSuccess && !unit && n > 0) {   // the diagonal, gathered into level order
+    e = hipMalloc((void**)&S->dv, nb);
+    if (e == hipSuccess) e = hipMemcpy(S->xa, dv.data(), nb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_perm_gather<V>, dim3(stream_grid(n)), dim3(kBlock), 0, s, n,
+                         S->tl.perm, S->xa, (const V*)nullptr, S->dv);
+      e = hipStreamSynchronize(s);
+    }
+  }
+  if (e == hipSuccess && n > 0) {
+    e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      enqueue_levels<V>(S->tl, S->dv, S->run, S->xa, S->ya, s);
+      e = hipStreamEndCapture(s, &S->graph);
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&S->exec, S->graph, nullptr, nullptr, 0);
+  }
+  if (e != hipSuccess)
+    return set_error(e == hipErrorOutOfMemory ? PSB_HIP_OUTOFMEM : PSB_HIP_RUNTIME, "trsv",
+                     hipGetErrorString(e));
+  return PSB_HIP_SUCCESS;
+}
+
+// The CSR matrix's prepared solve for key (lower | unit << 1 | trans << 2),
+// built from the device copy of the matrix on first use and kept until the
+// matrix is rewritten, freed or asked for another triangle or op.
+template <typename V>
+static int csr_sv_get(CsrDev* m, int key, TriSolve<V>** out) {
+  TriSolve<V>* S = (TriSolve<V>*)m->sv;
+  if (S && S->key == key) {
+    *out = S;
+    return PSB_HIP_SUCCESS;
+  }
+  release_csr_sv(m);
+  std::vector<int> irp(m->rows + 1), ja(m->nnz > 0 ? m->nnz : 1);
+  std::vector<V> val(m->nnz > 0 ? m->nnz : 1);
+  hipStream_t s = ctx().stream;
+  PSB_HIP_CHECK(hipMemcpyAsync(irp.data(), m->irp, sizeof(int) * irp.size(),
+                               hipMemcpyDeviceToHost, s));
+  if (m->nnz > 0) {
+    PSB_HIP_CHECK(hipMemcpyAsync(ja.data(), m->ja, sizeof(int) * (size_t)m->nnz,
+                                 hipMemcpyDeviceToHost, s));
+    PSB_HIP_CHECK(hipMemcpyAsync(val.data(), m->val, sizeof(V) * (size_t)m->nnz,
+                                 hipMemcpyDeviceToHost, s));
+  }
+  PSB_HIP_CHECK(hipStreamSynchronize(s));
+  S = new TriSolve<V>();
+  const int rc = build_sv<V>(S, m->rows, irp.data(), ja.data(), val.data(), (key & 1) != 0,
+                             (key & 2) != 0, (key & 4) != 0);
+  if (rc) {
+    destroy_sv(S);
+    return rc;
+  }
+  S->key = key;
+  m->sv = S;
+  *out = S;
+  return PSB_HIP_SUCCESS;
+}
+
+template <typename V>
+static int csr_sv_apply(CsrDev* m, int key, V alpha, const MultiVec* x, V beta, MultiVec* y) {
+  TriSolve<V>* S = nullptr;
+  int rc = csr_sv_get<V>(m, key, &S);
+  if (rc) return rc;
+  hipStream_t s = ctx().stream;
+  const int n = S->n, g = stream_grid(n);
+  for (int c = 0; c < x->count; c++) {
+    const V* xc = (const V*)x->v + (size_t)c * x->pitch;
+    V* yc = (V*)y->v + (size_t)c * y->pitch;
+    hipLaunchKernelGGL(k_perm_gather<V>, dim3(g), dim3(kBlock), 0, s, n, S->tl.perm, xc,
+                       (const V*)nullptr, S->xa);
+    PSB_HIP_LAUNCH_CHECK();
+    PSB_HIP_CHECK(hipGraphLaunch(S->exec, s));
+    hipLaunchKernelGGL(k_perm_scatter_axpby<V>, dim3(g), dim3(kBlock), 0, s, n, S->tl.perm,
+                       S->ya, alpha, beta, yc);
+    PSB_HIP_LAUNCH_CHECK();
+  }
+  return PSB_HIP_SUCCESS;
+}
+
+void release_csr_sv(CsrDev* m) {
+  if (!m || !m->sv) return;
+  if (m->elem_type == PSB_HIP_TYPE_DOUBLE) destroy_sv((TriSolve<double>*)m->sv);
+  else destroy_sv((TriSolve<float>*)m->sv);
+  m->sv = nullptr;
 }
 
 }  // namespace psbhip
@@ -589,13 +825,47 @@ int psb_hip_ilu_apply(void* ilu, double alpha, void* xv, double beta, void* yv) 
   if (F->n == 0) return PSB_HIP_SUCCESS;
   hipStream_t s = ctx().stream;
   const int n = F->n, g = stream_grid(n);
-  hipLaunchKernelGGL(k_perm_gather, dim3(g), dim3(kBlock), 0, s, n, F->L.perm,
+  hipLaunchKernelGGL(k_perm_gather<double>, dim3(g), dim3(kBlock), 0, s, n, F->L.perm,
                      (const double*)x->v, (const double*)nullptr, F->xa);
   PSB_HIP_LAUNCH_CHECK();
   PSB_HIP_CHECK(hipGraphLaunch(F->exec, s));
-  hipLaunchKernelGGL(k_perm_scatter_axpby, dim3(g), dim3(kBlock), 0, s, n, F->U.perm, F->ya, alpha,
-                     beta, (double*)y->v);
+  hipLaunchKernelGGL(k_perm_scatter_axpby<double>, dim3(g), dim3(kBlock), 0, s, n, F->U.perm,
+                     F->ya, alpha, beta, (double*)y->v);
   PSB_HIP_LAUNCH_CHECK();
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_csr_spsv(void* dmat, int lower, int unit, int trans, double alpha, void* xv,
+                     double beta, void* yv) {
+  PSB_ENSURE_INIT();
+  CsrDev* m = (CsrDev*)dmat;
+  MultiVec *x = (MultiVec*)xv, *y = (MultiVec*)yv;
+  if (!m || !x || !y) return set_error(PSB_HIP_INVALID, __func__, "null argument");
+  if (m->rows != m->cols) return set_error(PSB_HIP_INVALID, __func__, "matrix not square");
+  if (trans < 0 || trans > 2) return set_error(PSB_HIP_INVALID, __func__, "trans");
+  if (x->elem_type != m->elem_type || y->elem_type != m->elem_type || x->size < m->rows ||
+      y->size < m->rows || x->count != y->count)
+    return set_error(PSB_HIP_INVALID, __func__, "x/y type, size or count");
+  if (m->rows == 0) return PSB_HIP_SUCCESS;
+  const int key = (lower ? 1 : 0) | (unit ? 2 : 0) | (trans ? 4 : 0);
+  if (m->elem_type == PSB_HIP_TYPE_DOUBLE)
+    return csr_sv_apply<double>(m, key, alpha, x, beta, y);
+  return csr_sv_apply<float>(m, key, (float)alpha, x, (float)beta, y);
+}
+
+int psb_hip_csr_spsv_levels(void* dmat, int* levels, int* segments) {
+  CsrDev* m = (CsrDev*)dmat;
+  if (!m) return PSB_HIP_INVALID;
+  int nl = 0, ns = 0;
+  if (m->sv && m->elem_type == PSB_HIP_TYPE_DOUBLE) {
+    nl = ((TriSolve<double>*)m->sv)->tl.nlev;
+    ns = (int)((TriSolve<double>*)m->sv)->tl.segs.size();
+  } else if (m->sv) {
+    nl = ((TriSolve<float>*)m->sv)->tl.nlev;
+    ns = (int)((TriSolve<float>*)m->sv)->tl.segs.size();
+  }
+  if (levels) *levels = nl;
+  if (segments) *segments = ns;
   return PSB_HIP_SUCCESS;
 }
 

@@ -60,6 +60,44 @@ class _hip_base_mat:
     def __init__(self):
         self.dev = None
         self.nr = self.nc = self.nnz = 0
+        # triangle flags of psb_d_base_sparse_mat (set_triangle, set_lower,
+        # set_unit: base/modules/serial/psb_base_mat_mod.f90); a triangular
+        # solve reads them as the reference's inner_vect_sv does
+        self._triangle = False
+        self._upper = False
+        self._unit = False
+
+    def set_triangle(self, val=True):
+        self._triangle = bool(val)
+
+    def set_lower(self, val=True):
+        self._upper = not val
+
+    def set_upper(self, val=True):
+        self._upper = bool(val)
+
+    def set_unit(self, val=True):
+        self._unit = bool(val)
+
+    def is_triangle(self):
+        return self._triangle
+
+    def is_lower(self):
+        return not self._upper
+
+    def is_upper(self):
+        return self._upper
+
+    def is_unit(self):
+        return self._unit
+
+    def inner_vect_sv(self, alpha, x, beta, y, trans="N"):
+        """Triangular solve y = alpha * op(T)^-1 x + beta * y.  Only the CSR
+        format solves on the device (psb_d_cuda_csrg_inner_vect_sv); the
+        reference's HLG and ELG types copy to the host for it, which this
+        library does not do."""
+        raise N.PsbError(N.UNSUPPORTED, f"{self.fmt}_inner_vect_sv",
+                         "triangular solves run on the CSRG format")
 
     def get_nrows(self):
         return self.nr
@@ -319,6 +357,30 @@ class psb_d_hip_csrg_sparse_mat(_hip_base_mat):
         N.call("psb_hip_csr_spmv", self.dev, float(alpha), x.dev, float(beta), y.dev)
         y.set_dev()
 
+    def inner_vect_sv(self, alpha, x, beta, y, trans="N"):
+        """psb_d_cuda_csrg_inner_vect_sv (cuda/impl/psb_d_cuda_csrg_inner_vect_sv.F90):
+        y = alpha * op(T)^-1 x + beta * y on the device, with the results of
+        psb_d_csr_cssv (base/serial/impl/psb_d_csr_impl.F90:782-1039) for every
+        trans ('N', 'T', 'C') and beta; the matrix must be flagged triangular
+        (psb_err_invalid_mat_state_ otherwise, as the reference)."""
+        if not self.is_triangle():
+            raise N.PsbError(N.INVALID, "csrg_inner_vect_sv", "matrix is not triangular")
+        t = str(trans).upper()
+        if t not in ("N", "T", "C"):
+            raise N.PsbError(N.INVALID, "csrg_inner_vect_sv", f"trans {trans!r}")
+        self._check_vects(x, y)
+        N.call("psb_hip_csr_spsv", self.dev, 1 if self.is_lower() else 0,
+               1 if self.is_unit() else 0, "NTC".index(t), float(alpha), x.dev, float(beta),
+               y.dev)
+        y.set_dev()
+
+    def sv_levels(self):
+        """(levels, workgroup runs) of the prepared triangular solve, (0, 0)
+        before the first solve."""
+        lv, sg = C.c_int(), C.c_int()
+        N.call("psb_hip_csr_spsv_levels", self.dev, C.byref(lv), C.byref(sg))
+        return lv.value, sg.value
+
     def free(self):
         if self.dev is not None:
             N.fn("psb_hip_csr_free")(self.dev)
@@ -338,6 +400,40 @@ class psb_dspmat_type:
         self.a = None
         self.coo = None
         self.m = self.n = 0
+        self._tri = (False, False, False)   # triangle, upper, unit (until assembly)
+
+    # the triangle flags (psb_dspmat_type%set_triangle etc.): kept here in the
+    # build state and carried into the storage at assembly / conversion
+    def _set_flag(self, k, val):
+        t = list(self._tri)
+        t[k] = bool(val)
+        self._tri = tuple(t)
+        if self.a is not None:
+            self.a._triangle, self.a._upper, self.a._unit = self._tri
+
+    def set_triangle(self, val=True):
+        self._set_flag(0, val)
+
+    def set_upper(self, val=True):
+        self._set_flag(1, val)
+
+    def set_lower(self, val=True):
+        self._set_flag(1, not val)
+
+    def set_unit(self, val=True):
+        self._set_flag(2, val)
+
+    def is_triangle(self):
+        return self._tri[0]
+
+    def is_upper(self):
+        return self._tri[1]
+
+    def is_lower(self):
+        return not self._tri[1]
+
+    def is_unit(self):
+        return self._tri[2]
 
     def get_nrows(self):
         return self.m

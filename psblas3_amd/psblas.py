@@ -82,6 +82,7 @@ def psb_spasb(a, desc=None, mold=None, afmt=None, dupl="add", keep_coo=False):
         mold = _fmt_mold(afmt or "HLG")
     a.a = mold.mold() if not hasattr(mold, "hksz") else type(mold)(hksz=mold.hksz)
     a.a.cp_from_coo(a.coo)
+    a.a._triangle, a.a._upper, a.a._unit = a._tri
     if not keep_coo:
         a.coo = None
     return a
@@ -94,6 +95,7 @@ def psb_cscnv(a, mold):
     old = a.a
     a.a = mold.mold() if not hasattr(mold, "hksz") else type(mold)(hksz=mold.hksz)
     a.a.cp_from_coo(a.coo)
+    a.a._triangle, a.a._upper, a.a._unit = a._tri
     if old is not None:
         old.free()
     return a
@@ -172,6 +174,17 @@ def psb_spmm(alpha, a, x, beta, y, desc_a=None, trans="N"):
         raise N.PsbError(N.UNSUPPORTED, "psb_spmm", "transposed products are out of scope")
     xv, yv = _dev(x), _dev(y)
     a.a.vect_mv(alpha, xv, beta, yv)
+
+
+def psb_spsm(alpha, a, x, beta, y, desc_a=None, trans="N", scale="U", choice=None, diag=None):
+    """psb_spsm / psb_dspsv_vect (base/psblas/psb_dspsm.f90): y = alpha *
+    op(T)^-1 x + beta * y with the triangular matrix T (its storage flagged
+    by set_triangle / set_lower / set_upper / set_unit).  The diagonal
+    scalings (scale 'L' / 'R' with diag) are not provided."""
+    if str(scale).upper() != "U" or diag is not None:
+        raise N.PsbError(N.UNSUPPORTED, "psb_spsm", "diagonal scaling of the solve")
+    xv, yv = _dev(x), _dev(y)
+    a.a.inner_vect_sv(alpha, xv, beta, yv, trans=trans)
 
 
 def psb_geaxpby(alpha, x, beta, y, desc_a=None):

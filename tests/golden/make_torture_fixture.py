@@ -1,9 +1,12 @@
-"""Extract the SpMV known answers of the reference's torture tests as data.
+"""Extract the SpMV and triangular-solve known answers of the reference's
+torture tests as data.
 
 Reads test/torture/psb_{d,s}_mvsv_tester.f90 from the reference checkout and
-writes tests/golden/torture_usmv.json: for every usmv case with trans='n',
-the 1-based triplets (IA, JA, VA), x, alpha, beta, y before (bcy) and the
-expected y after (cy).  Only numbers are kept; no reference source text.
+writes tests/golden/torture_usmv.json (every usmv case with trans='n') and
+tests/golden/torture_ussv.json (every ussv case, trans 'n', 't' and 'c', with
+the triangle flags the case sets on its matrix): the 1-based triplets (IA,
+JA, VA), x, alpha, beta, y before (bcy) and the expected y after (cy).  Only
+numbers and flags are kept; no reference source text.
 
 usage: python tests/golden/make_torture_fixture.py [/root/reference]
 """
@@ -19,13 +22,14 @@ def _nums(s):
     return [float(t) for t in re.findall(r"[-+]?\d+(?:\.\d*)?(?:[eEdD][-+]?\d+)?", s)]
 
 
-def parse(path, kind):
+def parse(path, kind, op="usmv"):
     text = open(path).read()
     cases = []
-    for m in re.finditer(r"subroutine (\w_usmv_\w+)\(res,afmt,ctxt\)(.*?)end subroutine", text, re.S):
+    for m in re.finditer(r"subroutine (\w_" + op + r"_\w+)\(res,afmt,ctxt\)(.*?)end subroutine",
+                         text, re.S):
         name, body = m.group(1), m.group(2)
         tr = re.search(r"character::transa='(\w)'", body).group(1)
-        if tr != "n":
+        if op == "usmv" and tr != "n":
             continue
 
         def arr(var):
@@ -38,26 +42,32 @@ def parse(path, kind):
 
         m_ = int(scal("m"))
         k_ = int(scal("k"))
-        cases.append({
+        case = {
             "name": name, "kind": kind, "m": m_, "k": k_,
             "alpha": scal("alpha"), "beta": scal("beta"),
             "ia": [int(v) for v in arr("IA")], "ja": [int(v) for v in arr("JA")],
             "va": arr("VA"), "x": arr("x"), "bcy": arr("bcy"), "cy": arr("cy"),
-        })
+        }
+        if op == "ussv":
+            case["trans"] = tr
+            case["lower"] = "a%set_upper()" not in body
+            case["unit"] = re.search(r"a%set_unit\(\.true\.\)", body) is not None
+        cases.append(case)
     return cases
 
 
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
-    out = []
-    for kind in ("d", "s"):
-        p = os.path.join(ref, "test", "torture", f"psb_{kind}_mvsv_tester.f90")
-        out += parse(p, kind)
-    dst = os.path.join(HERE, "torture_usmv.json")
-    with open(dst, "w") as f:
-        json.dump({"source": "test/torture/psb_{d,s}_mvsv_tester.f90 (usmv, trans=n)",
-                   "cases": out}, f, indent=1)
-    print(f"{len(out)} cases -> {dst}")
+    for op, label in (("usmv", "usmv, trans=n"), ("ussv", "ussv, trans n/t/c")):
+        out = []
+        for kind in ("d", "s"):
+            p = os.path.join(ref, "test", "torture", f"psb_{kind}_mvsv_tester.f90")
+            out += parse(p, kind, op)
+        dst = os.path.join(HERE, f"torture_{op}.json")
+        with open(dst, "w") as f:
+            json.dump({"source": f"test/torture/psb_{{d,s}}_mvsv_tester.f90 ({label})",
+                       "cases": out}, f, indent=1)
+        print(f"{len(out)} cases -> {dst}")
 
 
 if __name__ == "__main__":
