@@ -168,6 +168,16 @@ module psb_hip_cbind_mod
       real(c_double), value :: alpha, beta
       integer(c_int) :: res
     end function psb_hip_csr_spmv
+    ! spsvCSRGDevice: y = alpha * op(T)^-1 x + beta * y, lower / unit from
+    ! a%is_lower() / a%is_unit(), trans 0 'N', 1 'T', 2 'C'
+    function psb_hip_csr_spsv(dmat, lower, unit, trans, alpha, x, beta, y) result(res) &
+         & bind(c, name='psb_hip_csr_spsv')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: dmat, x, y
+      integer(c_int), value :: lower, unit, trans
+      real(c_double), value :: alpha, beta
+      integer(c_int) :: res
+    end function psb_hip_csr_spsv
 
     ! ---------------------------------------------------------------- solvers
     function psb_hip_krylov_fused(method, fmt, dmat, diag, b, x, eps, itmax, istop, &
