@@ -27,7 +27,10 @@
  *     Krylov space for GCR / RGMRES, direct solves;
  *   - ILU(0) / BJAC: the defining property (L U equals A on A's pattern),
  *     exact LU on tridiagonal matrices, dense triangular solves (scipy);
- *     the library's host factorization equals it bit for bit.
+ *     the library's host factorization equals it bit for bit;
+ *   - CSR triangular solve (psb_d_csr_cssv): the torture SpSV known answers
+ *     (test/torture/psb_{d,s}_mvsv_tester.f90), scipy dense triangular
+ *     solves for every triangle / diagonal / op, exact integer solutions.
  *   - nrm2: netlib reference-BLAS scaled algorithm; the reference links an
  *     external BLAS (MKL here), whose summation order is not published:
  *     parity unpinned beyond the error bound.
