@@ -231,6 +231,27 @@ def methods_bench(P, N, L=40, eps=1e-9, itmax=1000):
     pl.free()
     log(f"methods: apply DIAG {out['DIAG_apply_us']} us, BJAC {out['BJAC_apply_us']} us "
         f"(launch per level {out['BJAC_apply_us_launch_per_level']} us), levels {out['bjac_levels']}")
+    # the CSR matrix's own triangular solve (psb_spsm) with the stencil's lower
+    # triangle, diagonal included: op 'N' and the transposed op 'T'
+    coo = gen_coo(P, N, "stencil", L)
+    low = coo.ia >= coo.ja
+    t = P.psb_dspmat_type()
+    t.m = t.n = n
+    t.a = P.psb_d_hip_csrg_sparse_mat().cp_from_coo(
+        P.psb_d_coo_sparse_mat(n, n, coo.ia[low], coo.ja[low], coo.val[low]))
+    t.set_triangle()
+    t.set_lower()
+    for op in ("N", "T"):
+        P.psb_spsm(1.0, t, b, 0.0, y, desc, trans=op)   # builds the level analysis
+        P.psb_hip_DeviceSync()
+        timer = P.DeviceTimer()
+        timer.start()
+        for _ in range(50):
+            P.psb_spsm(1.0, t, b, 0.0, y, desc, trans=op)
+        out[f"csr_spsv_{op}_us"] = round(timer.stop_ms() / 50 * 1e3, 2)
+    t.free()
+    log(f"methods: CSR triangular solve (lower, non-unit) N {out['csr_spsv_N_us']} us, "
+        f"T {out['csr_spsv_T_us']} us")
     runs = [("BICGSTAB", "convdiff", None), ("BICGSTABL", "convdiff", 2),
             ("RGMRES", "convdiff", 10), ("GCR", "convdiff", 10),
             ("CG", "poisson", None), ("FCG", "poisson", None)]
