@@ -291,6 +291,16 @@ int psb_hip_hll_spmv_single(void *dmat, double alpha, void *x, double beta, void
    diagonal): the device side of get_diag used by psb_d_diag_precbld
    (prec/impl/psb_d_diagprec_impl.f90:254). */
 int psb_hip_hll_get_diag(void *dmat, void *d);
+/* psb_d_cuda_hlg_scals / psb_d_cuda_hlg_scal (cuda/impl/psb_d_cuda_hlg_scals.F90,
+   psb_d_cuda_hlg_scal.F90: the CPU psb_d_hll_scals / psb_d_hll_scal, then
+   to_gpu), done on the device copy: scals multiplies every stored value,
+   padding included, by s; scal multiplies each row's entries (its irn
+   entries, padding untouched) by d(row) when left, by d(column) otherwise
+   (ext/impl/psb_d_hll_scal.f90:83-130).  d is a device vector of the
+   matrix's type with rows entries (left) or the largest stored column
+   (right).  The FP32 copy of psb_hip_hll_make_single follows. */
+int psb_hip_hll_scals(void *dmat, double s);
+int psb_hip_hll_scal(void *dmat, void *d, int left);
 /* Kernel variant selection for the HLL SpMV (autotuner hook): 0 = auto
    (two rows per lane for double values, one for float, XCD block order),
    1 = one row per lane, 2 = two adjacent rows per lane (even hksz), both in
@@ -332,6 +342,11 @@ int psb_hip_ell_read(void *dmat, void *val, int *ja, int ldv, int *irn, int *idi
 int psb_hip_ell_copy_coo(int nr, int nc, int nza, const int *irn, const int *idisp,
                          const int *ja, const void *val, void *dmat);
 int psb_hip_ell_get_diag(void *dmat, void *d);
+/* psb_d_cuda_elg_scals / _scal (cuda/impl/psb_d_cuda_elg_scals.F90, _scal.F90):
+   scals the whole value array; scal left the whole row (padding included),
+   right the row's irn entries (ext/impl/psb_d_ell_scal.f90:64-90). */
+int psb_hip_ell_scals(void *dmat, double s);
+int psb_hip_ell_scal(void *dmat, void *d, int left);
 /* spmvEllDevice{Double,Float} (cuda/elldev.c:195-215) */
 int psb_hip_ell_spmv(void *dmat, double alpha, void *x, double beta, void *y);
 
@@ -351,6 +366,12 @@ int psb_hip_csr_read(void *dmat, int *irp, int *ja, void *val);
 /* spmvCSRGDevice: y = alpha*A*x + beta*y */
 int psb_hip_csr_spmv(void *dmat, double alpha, void *x, double beta, void *y);
 int psb_hip_csr_get_diag(void *dmat, void *d);
+/* psb_d_cuda_csrg_scals / _scal (cuda/impl/psb_d_cuda_csrg_scals.F90, _scal.F90):
+   the nnz values times s; scal left / right by d(row) / d(ja)
+   (base/serial/impl/psb_d_csr_impl.F90:1607-1715).  A prepared triangular
+   solve is dropped. */
+int psb_hip_csr_scals(void *dmat, double s);
+int psb_hip_csr_scal(void *dmat, void *d, int left);
 /* the lanes-per-row and long-row threshold chosen by the analysis for the
    uniform kernel */
 int psb_hip_csr_get_analysis(void *dmat, int *lanes_per_row, int *long_rows,

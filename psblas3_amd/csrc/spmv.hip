@@ -322,6 +322,67 @@ __global__ __launch_bounds__(kBlock) void k_csr_long(int nlong, const int* __res
   if (wid < nlong && lane == 0) store_y(y, row, alpha, acc, beta);
 }
 
+// ------------------------------------------------------------------ scaling
+// scals: every stored value times s (psb_d_hll_scals and psb_d_ell_scals
+// scale the whole value array, padding included; psb_d_csr_scals its nnz
+// values).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scale_vals(int64_t n, T s, T* __restrict__ val) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    val[i] = val[i] * s;
+}
+
+// scal with a vector d, one thread per row: left multiplies by d(row), right
+// by d(column).  CSR (psb_d_csr_scal, base/serial/impl/psb_d_csr_impl.F90:
+// 1607-1679): the row's entries.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_csr_scal(int rows, const int* __restrict__ irp,
+                                                     const int* __restrict__ ja,
+                                                     const T* __restrict__ d, int left,
+                                                     T* __restrict__ val) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= rows) return;
+  for (int k = irp[r] - 1; k < irp[r + 1] - 1; k++)
+    val[k] = val[k] * (left ? d[r] : d[ja[k] - 1]);
+}
+
+// HLL (psb_d_hll_scal, ext/impl/psb_d_hll_scal.f90:83-130): the row's irn
+// entries, its padding untouched.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_hll_scal(int rows, int hksz,
+                                                     const int* __restrict__ hkoffs,
+                                                     const int* __restrict__ irn,
+                                                     const int* __restrict__ ja,
+                                                     const T* __restrict__ d, int left,
+                                                     T* __restrict__ val) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= rows) return;
+  const int hk = r / hksz;
+  const size_t off = (size_t)hkoffs[hk] + (r - hk * hksz);
+  for (int j = 0; j < irn[r]; j++) {
+    const size_t p = off + (size_t)j * hksz;
+    val[p] = val[p] * (left ? d[r] : d[ja[p] - 1]);
+  }
+}
+
+// ELL (psb_d_ell_scal, ext/impl/psb_d_ell_scal.f90:64-90): left scales the
+// whole row, padding included; right the row's irn entries.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_ell_scal(int rows, int pitch, int nzm,
+                                                     const int* __restrict__ irn,
+                                                     const int* __restrict__ ja,
+                                                     const T* __restrict__ d, int left,
+                                                     T* __restrict__ val) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= rows) return;
+  const int w = left ? nzm : irn[r];
+  for (int j = 0; j < w; j++) {
+    const size_t p = (size_t)j * pitch + r;
+    val[p] = val[p] * (left ? d[r] : d[ja[p] - 1]);
+  }
+}
+
 // ------------------------------------------------------------------ fills
 // psi_d_convert_hll_from_coo + psi_d_xtr_ell_from_coo, CPU padding rule:
 // one thread per slot row; rows past nr in the last hack get ja = nr, val 0;
@@ -443,6 +504,23 @@ __global__ __launch_bounds__(kBlock) void k_d2s(int64_t n, const double* __restr
 }
 
 // ------------------------------------------------------------------ helpers
+static bool scal_vec_ok(const MultiVec* d, int elem_type, int need) {
+  return d && d->elem_type == elem_type && d->size >= need && d->count >= 1;
+}
+
+static void refresh_single(HllDev* m) {   // the FP32 copy follows the scaled values
+  if (m->val32 && m->allocsize > 0)
+    hipLaunchKernelGGL(k_d2s, dim3(stream_grid(m->allocsize)), dim3(kBlock), 0, ctx().stream,
+                       (int64_t)m->allocsize, (const double*)m->val, m->val32);
+}
+
+template <typename T>
+static void launch_scale(int64_t n, double s, void* val) {
+  if (n > 0)
+    hipLaunchKernelGGL((k_scale_vals<T>), dim3(stream_grid(n)), dim3(kBlock), 0, ctx().stream, n,
+                       (T)s, (T*)val);
+}
+
 static int max_index(const int* ja, int64_t n) {
   int m = 0;
   for (int64_t k = 0; k < n; k++) m = ja[k] > m ? ja[k] : m;
@@ -1323,6 +1401,95 @@ int psb_hip_csr_get_analysis(void* dmat, int* lanes_per_row, int* long_rows, int
   if (lanes_per_row) *lanes_per_row = m->lanes;
   if (long_rows) *long_rows = m->nlong;
   if (long_threshold) *long_threshold = m->long_thr;
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_hll_scals(void* dmat, double s) {
+  PSB_ENSURE_INIT();
+  HllDev* m = (HllDev*)dmat;
+  if (!m) return PSB_HIP_INVALID;
+  if (m->elem_type == PSB_HIP_TYPE_DOUBLE) launch_scale<double>(m->allocsize, s, m->val);
+  else launch_scale<float>(m->allocsize, s, m->val);
+  refresh_single(m);
+  PSB_HIP_LAUNCH_CHECK();
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_ell_scals(void* dmat, double s) {
+  PSB_ENSURE_INIT();
+  EllDev* m = (EllDev*)dmat;
+  if (!m) return PSB_HIP_INVALID;
+  const int64_t n = (int64_t)m->pitch * m->nzm;
+  if (m->elem_type == PSB_HIP_TYPE_DOUBLE) launch_scale<double>(n, s, m->val);
+  else launch_scale<float>(n, s, m->val);
+  PSB_HIP_LAUNCH_CHECK();
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_csr_scals(void* dmat, double s) {
+  PSB_ENSURE_INIT();
+  CsrDev* m = (CsrDev*)dmat;
+  if (!m) return PSB_HIP_INVALID;
+  release_csr_sv(m);
+  if (m->elem_type == PSB_HIP_TYPE_DOUBLE) launch_scale<double>(m->nnz, s, m->val);
+  else launch_scale<float>(m->nnz, s, m->val);
+  PSB_HIP_LAUNCH_CHECK();
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_hll_scal(void* dmat, void* dv, int left) {
+  PSB_ENSURE_INIT();
+  HllDev* m = (HllDev*)dmat;
+  const MultiVec* d = (const MultiVec*)dv;
+  if (!m || !scal_vec_ok(d, m->elem_type, left ? m->rows : m->max_col))
+    return set_error(PSB_HIP_INVALID, __func__, "d type or size");
+  if (m->rows == 0) return PSB_HIP_SUCCESS;
+  dim3 grid(grid_for(m->rows, kBlock));
+  if (m->elem_type == PSB_HIP_TYPE_DOUBLE)
+    hipLaunchKernelGGL((k_hll_scal<double>), grid, dim3(kBlock), 0, ctx().stream, m->rows, m->hksz,
+                       m->hkoffs, m->irn, m->ja, (const double*)d->v, left, (double*)m->val);
+  else
+    hipLaunchKernelGGL((k_hll_scal<float>), grid, dim3(kBlock), 0, ctx().stream, m->rows, m->hksz,
+                       m->hkoffs, m->irn, m->ja, (const float*)d->v, left, (float*)m->val);
+  refresh_single(m);
+  PSB_HIP_LAUNCH_CHECK();
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_ell_scal(void* dmat, void* dv, int left) {
+  PSB_ENSURE_INIT();
+  EllDev* m = (EllDev*)dmat;
+  const MultiVec* d = (const MultiVec*)dv;
+  if (!m || !scal_vec_ok(d, m->elem_type, left ? m->rows : m->max_col))
+    return set_error(PSB_HIP_INVALID, __func__, "d type or size");
+  if (m->rows == 0) return PSB_HIP_SUCCESS;
+  dim3 grid(grid_for(m->rows, kBlock));
+  if (m->elem_type == PSB_HIP_TYPE_DOUBLE)
+    hipLaunchKernelGGL((k_ell_scal<double>), grid, dim3(kBlock), 0, ctx().stream, m->rows, m->pitch,
+                       m->nzm, m->irn, m->ja, (const double*)d->v, left, (double*)m->val);
+  else
+    hipLaunchKernelGGL((k_ell_scal<float>), grid, dim3(kBlock), 0, ctx().stream, m->rows, m->pitch,
+                       m->nzm, m->irn, m->ja, (const float*)d->v, left, (float*)m->val);
+  PSB_HIP_LAUNCH_CHECK();
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_csr_scal(void* dmat, void* dv, int left) {
+  PSB_ENSURE_INIT();
+  CsrDev* m = (CsrDev*)dmat;
+  const MultiVec* d = (const MultiVec*)dv;
+  if (!m || !scal_vec_ok(d, m->elem_type, left ? m->rows : m->max_col))
+    return set_error(PSB_HIP_INVALID, __func__, "d type or size");
+  release_csr_sv(m);
+  if (m->rows == 0) return PSB_HIP_SUCCESS;
+  dim3 grid(grid_for(m->rows, kBlock));
+  if (m->elem_type == PSB_HIP_TYPE_DOUBLE)
+    hipLaunchKernelGGL((k_csr_scal<double>), grid, dim3(kBlock), 0, ctx().stream, m->rows, m->irp,
+                       m->ja, (const double*)d->v, left, (double*)m->val);
+  else
+    hipLaunchKernelGGL((k_csr_scal<float>), grid, dim3(kBlock), 0, ctx().stream, m->rows, m->irp,
+                       m->ja, (const float*)d->v, left, (float*)m->val);
+  PSB_HIP_LAUNCH_CHECK();
   return PSB_HIP_SUCCESS;
 }
 

@@ -141,6 +141,120 @@ class _hip_base_mat:
         N.call(self._diag_fn, self.dev, d.dev)
         d.set_dev()
 
+    # ---- state of the dual-memory plug-in (cuda/psb_d_cuda_hlg_mat_mod.F90:
+    # is_host/is_dev/is_sync, set_*, sync, to_gpu, from_gpu).  The storage
+    # lives on the device only: it is always "dev", sync and to_gpu have
+    # nothing to copy, from_gpu reads the arrays back.
+    def is_host(self):
+        return False
+
+    def is_sync(self):
+        return False
+
+    def set_host(self):
+        pass
+
+    def set_dev(self):
+        pass
+
+    def set_sync(self):
+        pass
+
+    def sync(self):
+        pass
+
+    def to_gpu(self):
+        pass
+
+    def from_gpu(self):
+        return self.to_host()
+
+    def to_coo(self):
+        """The stored entries as a host COO (row-major sorted)."""
+        irp, ja, val = self.to_csr()
+        ia = np.repeat(np.arange(1, self.nr + 1, dtype=np.int32), np.diff(irp))
+        return psb_d_coo_sparse_mat(self.nr, self.nc, ia, ja, val).fix()
+
+    def cp_from_fmt(self, b):
+        """cp_from_fmt: this format built from another storage's entries."""
+        return self.cp_from_coo(b.to_coo())
+
+    def mv_from_coo(self, coo):
+        """mv_from_coo: build, then leave the source empty."""
+        self.cp_from_coo(coo)
+        coo.ia = np.zeros(0, np.int32)
+        coo.ja = np.zeros(0, np.int32)
+        coo.val = np.zeros(0)
+        return self
+
+    def mv_from_fmt(self, b):
+        self.cp_from_fmt(b)
+        b.free()
+        return self
+
+    # ---- host-array products (csmv / csmm: device temporaries, as
+    # cuda/impl/psb_d_cuda_hlg_csmv.F90:62-81 does)
+    def csmv(self, alpha, x, beta, y, trans="N"):
+        """y = alpha*A*x + beta*y with host arrays; returns the new y."""
+        from .vect import psb_d_hip_vect, psb_s_hip_vect
+        if str(trans).upper() != "N":
+            raise N.PsbError(N.UNSUPPORTED, f"{self.fmt}_csmv", "transposed products")
+        mold = psb_s_hip_vect if self._dtype == np.float32 else psb_d_hip_vect
+        x = np.asarray(x, self._dtype)
+        y = np.asarray(y, self._dtype)
+        if x.shape[0] < self.nc or y.shape[0] < self.nr:
+            raise N.PsbError(N.INVALID, f"{self.fmt}_csmv", "x or y too short")
+        xv, yv = mold().bld(x), mold().bld(y)
+        try:
+            self.vect_mv(alpha, xv, beta, yv)
+            return yv.get_vect()
+        finally:
+            xv.free()
+            yv.free()
+
+    csmm = csmv   # columns of 2-D x / y are the multivector's columns
+
+    # ---- scaling (scals / scal): on the device copy
+    def make_nonunit(self):
+        """make_nonunit: a unit triangle gets its unit diagonal stored."""
+        if not self._unit:
+            return
+        coo = self.to_coo()
+        k = min(self.nr, self.nc)
+        d = np.arange(1, k + 1, dtype=np.int32)
+        coo = psb_d_coo_sparse_mat(self.nr, self.nc, np.concatenate([coo.ia, d]),
+                                   np.concatenate([coo.ja, d]), np.concatenate([coo.val, np.ones(k)]))
+        flags = (self._triangle, self._upper)
+        self.cp_from_coo(coo.fix())
+        self._triangle, self._upper, self._unit = flags[0], flags[1], False
+
+    def scals(self, d):
+        """psb_d_cuda_*_scals: every value times the scalar d."""
+        self.make_nonunit()
+        N.call(self._pfx + "_scals", self.dev, float(d))
+
+    def scal(self, d, side="L"):
+        """psb_d_cuda_*_scal: rows (side 'L') or columns ('R') times d."""
+        from .vect import psb_d_hip_vect, psb_hip_base_vect, psb_s_hip_vect
+        left = str(side).upper() == "L"
+        need = self.nr if left else self.nc
+        self.make_nonunit()
+        if isinstance(d, psb_hip_base_vect):
+            if d._dtype != self._dtype or d.get_nrows() < need:
+                raise N.PsbError(N.INVALID, f"{self.fmt}_scal", "d precision or size")
+            d.ensure_dev()
+            N.call(self._pfx + "_scal", self.dev, d.dev, 1 if left else 0)
+            return
+        d = np.asarray(d, self._dtype)
+        if d.shape[0] < need:
+            raise N.PsbError(N.INVALID, f"{self.fmt}_scal", "d too short")
+        mold = psb_s_hip_vect if self._dtype == np.float32 else psb_d_hip_vect
+        dv = mold().bld(d)
+        try:
+            N.call(self._pfx + "_scal", self.dev, dv.dev, 1 if left else 0)
+        finally:
+            dv.free()
+
 
 class psb_d_hip_hlg_sparse_mat(_hip_base_mat):
     """HLL on the device (psb_d_cuda_hlg_sparse_mat).  hksz defaults to the
@@ -150,6 +264,7 @@ class psb_d_hip_hlg_sparse_mat(_hip_base_mat):
     _etype = N.TYPE_DOUBLE
     _dtype = np.float64
     _diag_fn = "psb_hip_hll_get_diag"
+    _pfx = "psb_hip_hll"
 
     def __init__(self, hksz=64):
         super().__init__()
@@ -242,6 +357,7 @@ class psb_d_hip_elg_sparse_mat(_hip_base_mat):
     _etype = N.TYPE_DOUBLE
     _dtype = np.float64
     _diag_fn = "psb_hip_ell_get_diag"
+    _pfx = "psb_hip_ell"
 
     def __init__(self):
         super().__init__()
@@ -318,6 +434,7 @@ class psb_d_hip_csrg_sparse_mat(_hip_base_mat):
     _etype = N.TYPE_DOUBLE
     _dtype = np.float64
     _diag_fn = "psb_hip_csr_get_diag"
+    _pfx = "psb_hip_csr"
 
     def cp_from_coo(self, coo):
         """psb_d_cp_csr_from_coo (base/serial/impl/psb_d_csr_impl.F90:3154)."""
