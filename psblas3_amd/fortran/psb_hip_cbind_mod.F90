@@ -168,6 +168,43 @@ module psb_hip_cbind_mod
       real(c_double), value :: alpha, beta
       integer(c_int) :: res
     end function psb_hip_csr_spmv
+    ! scals / scal of the three formats, on the device copy
+    function psb_hip_hll_scals(dmat, s) result(res) bind(c, name='psb_hip_hll_scals')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: dmat
+      real(c_double), value :: s
+      integer(c_int) :: res
+    end function psb_hip_hll_scals
+    function psb_hip_hll_scal(dmat, d, left) result(res) bind(c, name='psb_hip_hll_scal')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: dmat, d
+      integer(c_int), value :: left
+      integer(c_int) :: res
+    end function psb_hip_hll_scal
+    function psb_hip_ell_scals(dmat, s) result(res) bind(c, name='psb_hip_ell_scals')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: dmat
+      real(c_double), value :: s
+      integer(c_int) :: res
+    end function psb_hip_ell_scals
+    function psb_hip_ell_scal(dmat, d, left) result(res) bind(c, name='psb_hip_ell_scal')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: dmat, d
+      integer(c_int), value :: left
+      integer(c_int) :: res
+    end function psb_hip_ell_scal
+    function psb_hip_csr_scals(dmat, s) result(res) bind(c, name='psb_hip_csr_scals')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: dmat
+      real(c_double), value :: s
+      integer(c_int) :: res
+    end function psb_hip_csr_scals
+    function psb_hip_csr_scal(dmat, d, left) result(res) bind(c, name='psb_hip_csr_scal')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: dmat, d
+      integer(c_int), value :: left
+      integer(c_int) :: res
+    end function psb_hip_csr_scal
     ! spsvCSRGDevice: y = alpha * op(T)^-1 x + beta * y, lower / unit from
     ! a%is_lower() / a%is_unit(), trans 0 'N', 1 'T', 2 'C'
     function psb_hip_csr_spsv(dmat, lower, unit, trans, alpha, x, beta, y) result(res) &
