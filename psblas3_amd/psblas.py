@@ -179,12 +179,42 @@ def psb_spmm(alpha, a, x, beta, y, desc_a=None, trans="N"):
 def psb_spsm(alpha, a, x, beta, y, desc_a=None, trans="N", scale="U", choice=None, diag=None):
     """psb_spsm / psb_dspsv_vect (base/psblas/psb_dspsm.f90): y = alpha *
     op(T)^-1 x + beta * y with the triangular matrix T (its storage flagged
-    by set_triangle / set_lower / set_upper / set_unit).  The diagonal
-    scalings (scale 'L' / 'R' with diag) are not provided."""
-    if str(scale).upper() != "U" or diag is not None:
-        raise N.PsbError(N.UNSUPPORTED, "psb_spsm", "diagonal scaling of the solve")
+    by set_triangle / set_lower / set_upper / set_unit), and with a diagonal
+    d (diag) the scaled forms of psb_d_base_vect_cssv
+    (base/serial/impl/psb_d_base_mat_impl.F90:2015-2130): scale 'R' solves
+    with d .* x, scale 'L' gives alpha * d .* (op(T)^-1 x) + beta * y (the
+    product d .* t, then y = t + beta * y); without diag the scale is
+    ignored, with diag a scale other than 'L' / 'R' is an error (info 31).
+    Every step runs on the device."""
     xv, yv = _dev(x), _dev(y)
-    a.a.inner_vect_sv(alpha, xv, beta, yv, trans=trans)
+    t = str(trans).upper()
+    if diag is None:
+        a.a.inner_vect_sv(alpha, xv, beta, yv, trans=t)
+        return
+    dv = _dev(diag)
+    _kind(xv, yv, dv)
+    sc = "L" if scale is None else str(scale).upper()
+    nar, nac = a.get_nrows(), a.get_ncols()
+    if sc not in ("L", "R"):
+        raise N.PsbError(31, "psb_spsm", f"scale {scale!r} with a diagonal")
+    if dv.get_nrows() < (nac if sc == "R" else nar):
+        raise N.PsbError(N.INVALID, "psb_spsm", "diagonal too short")
+    if sc == "R":                       # tmp = d .* x, then the solve
+        tmp = xv.mold()
+        tmp.all(nac)
+        tmp.mlt_v_2(1.0, dv, xv, 0.0)
+        a.a.inner_vect_sv(alpha, tmp, beta, yv, trans=t)
+        tmp.free()
+    elif beta == 0.0:                   # the solve, then y = y .* d
+        a.a.inner_vect_sv(alpha, xv, 0.0, yv, trans=t)
+        yv.mlt_v(dv)
+    else:                               # tmp = d .* (alpha T^-1 x); y = tmp + beta y
+        tmp = yv.mold()
+        tmp.all(nar)
+        a.a.inner_vect_sv(alpha, xv, 0.0, tmp, trans=t)
+        tmp.mlt_v(dv)
+        yv.axpby_v(nar, 1.0, tmp, beta)
+        tmp.free()
 
 
 def psb_geaxpby(alpha, x, beta, y, desc_a=None):

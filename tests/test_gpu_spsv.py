@@ -135,6 +135,33 @@ def test_spsv_alpha_beta(gpu, orc, alpha, beta, trans):
     a.free()
 
 
+@pytest.mark.parametrize("scale", ["L", "R"])
+@pytest.mark.parametrize("beta", [0.0, 0.5])
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_spsm_diag_scaling(gpu, orc, scale, beta, trans):
+    """psb_spsm with a diagonal: 'R' solves with d .* x; 'L' scales the
+    solution (y = y .* d when beta = 0, else t .* d then t + beta * y), the
+    steps of psb_d_base_vect_cssv, each one rounding like the CPU's."""
+    _say(f"case psb_spsm scale={scale} beta={beta} trans={trans}")
+    n = 400
+    irp, ja, val = tri_csr(orc, n, False, False, seed=61)
+    a, desc = _tri_mat(irp, ja, val, False, False)
+    x = orc.uniform_pm1(62, n)
+    d = orc.uniform_pm1(63, n) + 2.0
+    y0 = orc.uniform_pm1(64, n)
+    y = P.vect_from(y0.copy())
+    P.psb_spsm(2.0, a, P.vect_from(x), beta, y, desc, trans=trans, scale=scale,
+               diag=P.vect_from(d))
+    if scale == "R":
+        ref = orc.csr_cssv(irp, ja, val, d * x, lower=False, trans=trans, alpha=2.0, beta=beta,
+                           y=y0)
+    else:
+        t = orc.csr_cssv(irp, ja, val, x, lower=False, trans=trans, alpha=2.0) * d
+        ref = t if beta == 0.0 else orc.axpby(1.0, t, beta, y0.copy())
+    assert np.array_equal(_bits(y.get_vect()), _bits(ref))
+    a.free()
+
+
 @pytest.mark.parametrize("case", _ussv_cases(), ids=lambda c: c["name"])
 def test_torture_ussv_device(gpu, orc, case):
     single = case["kind"] == "s"
@@ -217,10 +244,13 @@ def test_spsv_errors(gpu, orc):
         P.psb_spsm(1.0, b, x, 0.0, y, desc, trans="T")
     assert b.a.sv_levels() == (0, 0)
     b.free()
-    # a diagonal scaling of the solve, and the HLG format
+    # a diagonal with a scale other than 'L' / 'R' (info 31), a short
+    # diagonal, and the HLG format
     c, desc = _tri_mat(irp, ja, val, True, False)
     with pytest.raises(P.PsbError):
-        P.psb_spsm(1.0, c, x, 0.0, y, desc, scale="L", diag=P.vect_from(np.ones(50)))
+        P.psb_spsm(1.0, c, x, 0.0, y, desc, scale="U", diag=P.vect_from(np.ones(50)))
+    with pytest.raises(P.PsbError):
+        P.psb_spsm(1.0, c, x, 0.0, y, desc, scale="L", diag=P.vect_from(np.ones(49)))
     c.free()
     h = P.matrix_from_coo(50, 50, np.repeat(np.arange(1, 51, dtype=np.int32), np.diff(irp)),
                           ja, val, mold=P.psb_d_hip_hlg_sparse_mat(32))
