@@ -42,6 +42,16 @@ class psb_hip_base_vect:
         self.zero()
         return self
 
+    def asb(self, n):
+        """d_cuda_asb_m (cuda/psb_d_cuda_vect_mod.F90:677-702): make room for
+        at least n entries, keeping the values (new entries are zero)."""
+        if n <= self.n:
+            return self
+        old = self.get_vect()
+        grown = np.zeros((n,) if self.count == 1 else (n, self.count), dtype=self._dtype)
+        grown[:self.n] = old
+        return self.bld(grown)
+
     def bld(self, values):
         """d_cuda_bld_x: build from host values (n,) or (n, count)."""
         a = np.asarray(values, dtype=self._dtype)
@@ -412,6 +422,16 @@ class psb_d_vect_type:
 
     def get_vect(self):
         return self.v.get_vect()
+
+    def cnv(self, mold=None):
+        """d_vect_cnv (base/modules/serial/psb_d_vect_mod.F90:590-612): move
+        the values into storage of the mold's type."""
+        nv = (mold or psb_d_hip_vect()).mold()
+        if self.v is not None:
+            nv.bld(self.v.get_vect().astype(nv._dtype))
+            self.v.free()
+        self.v = nv
+        return self
 
     def set(self, val):
         self.v.set_scal(val)

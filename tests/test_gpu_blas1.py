@@ -207,6 +207,18 @@ def test_vector_plugin_methods(gpu, orc):
     assert t.get_vect().tolist() == [7.0, 6.0, 0, 0, 0, 0, 0, 0, -1.0, 0]
     with pytest.raises(P.PsbError):
         t.ins_a(1, np.array([11], np.int32), np.array([1.0]), 0)
+    # asb grows the vector and keeps its values; cnv moves them to another type
+    t.asb(14)
+    assert t.get_nrows() == 14
+    assert t.get_vect().tolist() == [7.0, 6.0, 0, 0, 0, 0, 0, 0, -1.0, 0, 0, 0, 0, 0]
+    t.asb(5)
+    assert t.get_nrows() == 14
+    vt = P.psb_d_vect_type(D(x))
+    vt.cnv(P.psb_s_hip_vect())
+    assert vt.v._dtype == np.float32 and np.array_equal(vt.get_vect(), x.astype(np.float32))
+    vt.cnv()
+    assert vt.v._dtype == np.float64 and np.array_equal(vt.get_vect(), x.astype(np.float32))
+    vt.free()
 
 
 def test_halo_buffer_methods(gpu, orc):
