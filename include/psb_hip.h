@@ -378,14 +378,30 @@ int psb_hip_csr_get_analysis(void *dmat, int *lanes_per_row, int *long_rows,
                              int *long_threshold);
 /* CSR kernel choice (no counterpart in the reference, whose CSR product is
    cuSPARSE): 0 auto (the row-block kernel, fastest on every matrix
-   measured), 1 uniform lanes per row (a power of two >= the mean length),
+   measured, on the column slices when the matrix has them), 1 uniform lanes per row (a power of two >= the mean length),
    2 length-binned (rows grouped by lanes per row, about four entries per
    lane), 3 row-block (runs of consecutive rows whose entries fit in LDS:
    products streamed into LDS, then one thread per row sums them in CSR
-   order, so those rows equal the CPU loop's bits when beta = 0).  In 2 and 3
-   rows over 512 entries take one wavefront each.  Returns the previous
-   value. */
+   order, so those rows equal the CPU loop's bits when beta = 0; on the
+   whole matrix even when it has column slices), 4 the same as 0.  In 2, 3
+   and the row-block kernel of 0 / 4, rows over 512 entries take one
+   wavefront each.
+   Returns the previous value. */
 int psb_hip_csr_set_variant(int variant);
+/* Column slices of a CSR matrix (no reference counterpart): for a matrix
+   whose x does not fit the L2 caches and whose columns are scattered (more
+   than half of the entries of sampled rows farther than cols/16 from the
+   diagonal, x over 12 MiB), psb_hip_csr_write also stores the entries split
+   by consecutive column ranges of about 6.4 MB of x each (at most 16), and
+   the product (variant 0 or 4) runs the slices one after another,
+   accumulating into y: each slice's random gathers hit an x range that stays
+   in L2.  The row sums then add the slices' partial sums, so they follow the
+   per-row error bound rather than the CPU loop's bits.  Uses twice the
+   matrix's memory.  set_slices: -1 automatic (default), 0 or 1 none, >= 2
+   that many, applied by the next psb_hip_csr_write; returns the previous
+   value.  psb_hip_csr_slices: the matrix's slice count (0: none). */
+int psb_hip_csr_set_slices(int slices);
+int psb_hip_csr_slices(void *dmat);
 /* spsvCSRGDevice (cuda/fcusparse_fct.h:132-200, called by
    psb_d_cuda_csrg_inner_vect_sv, cuda/impl/psb_d_cuda_csrg_inner_vect_sv.F90):
    y = alpha * op(T)^-1 x + beta * y for the triangular matrix T held in the

@@ -333,6 +333,11 @@ struct CsrDev {
   int2* sblk;           // (first row, one past the last row), 0-based
   // triangular solve prepared on first use (level analysis, see trsv.hip)
   void* sv = nullptr;
+  // column slices: the same rows restricted to consecutive column ranges, each
+  // a CSR matrix of its own, used by the product when x is large and the
+  // columns scattered (the slices' x ranges stay in L2); null when unsliced
+  int nslices = 0;
+  CsrDev** slices = nullptr;
 };
 
 // frees the prepared triangular solve of a CSR matrix (trsv.hip)

@@ -34,10 +34,21 @@ static int g_hll_variant = 0;
 // ranges by 9% (FP64) and 6% (FP32), for the row-block CSR kernel launch
 // order by 1-2%.
 static int g_hll_xchunk = 16;
-// 0 auto (= uniform: the length-binned kernel measured slower on both the
-// L=200 stencil, 514 vs 280 us, and the n=4M power-law matrix, 1355 vs
-// 1069 us), 1 uniform lanes per row, 2 length-binned
+// 0 auto (the row-block kernel, on the column slices when the matrix has
+// them), 1 uniform lanes per row, 2 length-binned, 3 row-block on the whole
+// matrix, 4 = 0
 static int g_csr_variant = 0;
+// Column slices built by psb_hip_csr_write: -1 automatic (matrices whose x is
+// over kCsrSliceMinX bytes and whose entries mostly lie more than cols/16
+// from the diagonal, in slices of about kCsrSliceBytes of x), 0 or 1 none,
+// >= 2 that many.  On the n = 4M power-law matrix (uniform random columns, x
+// 32 MB) 2, 3, 4, 5, 6 and 8 slices measured 942, 846, 779, 751, 807 and
+// 916 us against 1041 us unsliced: a slice's x range stays in the XCDs' L2
+// while its rows run, so fewer of the random gathers miss.
+static int g_csr_slices = -1;
+constexpr size_t kCsrSliceMinX = (size_t)12 << 20;
+constexpr size_t kCsrSliceBytes = 6400000;
+constexpr int kCsrMaxSlices = 16;
 
 constexpr int kCsrEntriesPerLane = 4;
 constexpr int kCsrLongRow = 512;   // binned / row-block: longer rows take one wavefront each
@@ -677,9 +688,17 @@ static int csr_launch_rowblock(CsrDev* m, const T* x, T alpha, T beta, T* y) {
 
 template <typename T>
 static int csr_launch(CsrDev* m, const T* x, T alpha, T beta, T* y) {
+  if (m->nslices > 0 && (g_csr_variant == 0 || g_csr_variant == 4)) {
+    // y = alpha A_0 x + beta y, then y = alpha A_s x + y slice after slice
+    for (int s = 0; s < m->nslices; s++) {
+      const int rc = csr_launch_rowblock<T>(m->slices[s], x, alpha, s == 0 ? beta : T(1), y);
+      if (rc) return rc;
+    }
+    return PSB_HIP_SUCCESS;
+  }
   const bool binned = g_csr_variant == 2;
   if (binned && m->bin_rows) return csr_launch_binned<T>(m, x, alpha, beta, y);
-  if ((g_csr_variant == 0 || g_csr_variant == 3) && m->rows > 0)
+  if ((g_csr_variant == 0 || g_csr_variant == 3 || g_csr_variant == 4) && m->rows > 0)
     return csr_launch_rowblock<T>(m, x, alpha, beta, y);
   hipStream_t s = ctx().stream;
   const T* val = (const T*)m->val;
@@ -715,6 +734,180 @@ static int csr_launch(CsrDev* m, const T* x, T alpha, T beta, T* y) {
 }  // namespace psbhip
 
 using namespace psbhip;
+
+namespace psbhip {
+static void free_csr_slices(CsrDev* m);
+
+static int csr_write_impl(CsrDev* m, const int* irp, const int* ja, const void* val) {
+  release_csr_sv(m);   // a prepared triangular solve describes the old values
+  free_csr_slices(m);
+  if (irp[0] != 1 || irp[m->rows] - 1 != m->nnz)
+    return set_error(PSB_HIP_INVALID, __func__, "irp inconsistent with nnz");
+  for (int i = 0; i < m->rows; i++)
+    if (irp[i + 1] < irp[i]) return set_error(PSB_HIP_INVALID, __func__, "irp not monotone");
+  int mn = min_index(ja, m->nnz), mx = max_index(ja, m->nnz);
+  if (m->nnz > 0 && (mn < 1 || mx > m->cols))
+    return set_error(PSB_HIP_INVALID, __func__, "ja out of range");
+  m->max_col = mx;
+  // analysis: lanes per row from the mean row length, long rows listed
+  double mean = m->rows > 0 ? (double)m->nnz / m->rows : 0.0;
+  int g = 1;
+  while (g < 64 && g < mean) g <<= 1;
+  m->lanes = g;
+  m->long_thr = std::max(256, 8 * g);
+  std::vector<int> longs;
+  for (int i = 0; i < m->rows; i++)
+    if (irp[i + 1] - irp[i] > m->long_thr) longs.push_back(i);
+  m->nlong = (int)longs.size();
+  // length-binned analysis
+  int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  std::vector<int> longs_b;
+  for (int i = 0; i < m->rows; i++) {
+    const int len = irp[i + 1] - irp[i];
+    if (len > kCsrLongRow) {
+      longs_b.push_back(i);
+      continue;
+    }
+    cnt[csr_class(len)]++;
+  }
+  m->bin_start[0] = 0;
+  for (int c = 0; c < 7; c++) m->bin_start[c + 1] = m->bin_start[c] + cnt[c];
+  std::vector<int> bins((size_t)(m->bin_start[7] > 0 ? m->bin_start[7] : 1));
+  {
+    int fill[7];
+    for (int c = 0; c < 7; c++) fill[c] = m->bin_start[c];
+    for (int i = 0; i < m->rows; i++) {
+      const int len = irp[i + 1] - irp[i];
+      if (len <= kCsrLongRow) bins[(size_t)fill[csr_class(len)]++] = i;
+    }
+  }
+  m->nlong_b = (int)longs_b.size();
+  // row blocks: consecutive rows up to kBlock of them and kCsrBlockNnz
+  // entries; a row longer than kCsrLongRow ends the block and is left out
+  std::vector<int2> sblk;
+  {
+    int r0 = 0;
+    while (r0 < m->rows) {
+      if (irp[r0 + 1] - irp[r0] > kCsrLongRow) {
+        r0++;
+        continue;
+      }
+      int r1 = r0;
+      while (r1 < m->rows && r1 - r0 < kBlock && irp[r1 + 1] - irp[r1] <= kCsrLongRow &&
+             irp[r1 + 1] - irp[r0] <= kCsrBlockNnz)
+        r1++;
+      sblk.push_back(make_int2(r0, r1));
+      r0 = r1;
+    }
+  }
+  m->nsblk = (int)sblk.size();
+  hipStream_t s = ctx().stream;
+  hipFree(m->long_rows);
+  m->long_rows = nullptr;
+  hipFree(m->bin_rows);
+  m->bin_rows = nullptr;
+  hipFree(m->long_rows_b);
+  m->long_rows_b = nullptr;
+  if (m->nlong > 0) {
+    PSB_HIP_CHECK(hipMalloc(&m->long_rows, sizeof(int) * m->nlong));
+    PSB_HIP_CHECK(hipMemcpyAsync(m->long_rows, longs.data(), sizeof(int) * m->nlong,
+                                 hipMemcpyHostToDevice, s));
+  }
+  if (m->rows > 0) {
+    PSB_HIP_CHECK(hipMalloc(&m->bin_rows, sizeof(int) * bins.size()));
+    PSB_HIP_CHECK(hipMemcpyAsync(m->bin_rows, bins.data(), sizeof(int) * bins.size(),
+                                 hipMemcpyHostToDevice, s));
+  }
+  hipFree(m->sblk);
+  m->sblk = nullptr;
+  if (m->nsblk > 0) {
+    PSB_HIP_CHECK(hipMalloc(&m->sblk, sizeof(int2) * m->nsblk));
+    PSB_HIP_CHECK(hipMemcpyAsync(m->sblk, sblk.data(), sizeof(int2) * m->nsblk,
+                                 hipMemcpyHostToDevice, s));
+  }
+  if (m->nlong_b > 0) {
+    PSB_HIP_CHECK(hipMalloc(&m->long_rows_b, sizeof(int) * m->nlong_b));
+    PSB_HIP_CHECK(hipMemcpyAsync(m->long_rows_b, longs_b.data(), sizeof(int) * m->nlong_b,
+                                 hipMemcpyHostToDevice, s));
+  }
+  size_t es = elem_size(m->elem_type);
+  PSB_HIP_CHECK(hipMemcpyAsync(m->irp, irp, sizeof(int) * (m->rows + 1), hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(m->ja, ja, sizeof(int) * (size_t)m->nnz, hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(m->val, val, es * (size_t)m->nnz, hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipStreamSynchronize(s));
+  return PSB_HIP_SUCCESS;
+}
+
+// Column slices (see g_csr_slices): slice s holds the entries of columns
+// s*w+1 .. (s+1)*w of every row, in CSR order, as a CSR matrix of its own with
+// the row-block analysis; the product runs the slices one after another,
+// accumulating into y.
+static void free_csr_slices(CsrDev* m) {
+  if (!m->slices) return;
+  for (int s = 0; s < m->nslices; s++) psb_hip_csr_free(m->slices[s]);
+  delete[] m->slices;
+  m->slices = nullptr;
+  m->nslices = 0;
+}
+
+static int csr_slice_count(const CsrDev* m, const int* irp, const int* ja) {
+  if (g_csr_slices >= 2) return std::min(g_csr_slices, std::max(m->cols, 1));
+  if (g_csr_slices >= 0 || m->nnz == 0 || m->rows == 0) return 0;
+  const size_t xb = (size_t)m->cols * elem_size(m->elem_type);
+  if (xb <= kCsrSliceMinX) return 0;
+  // scattered columns: most entries of about 65536 sampled rows at least
+  // cols/16 columns away from the diagonal
+  const int step = std::max(1, m->rows / 65536);
+  const int64_t dist = m->cols / 16;
+  int64_t far = 0, tot = 0;
+  for (int i = 0; i < m->rows; i += step)
+    for (int k = irp[i] - 1; k < irp[i + 1] - 1; k++) {
+      const int64_t d = (int64_t)ja[k] - 1 - i;
+      far += (d > dist || d < -dist) ? 1 : 0;
+      tot++;
+    }
+  if (far * 2 <= tot) return 0;
+  return (int)std::min<size_t>(kCsrMaxSlices, (xb + kCsrSliceBytes - 1) / kCsrSliceBytes);
+}
+
+static int csr_build_slices(CsrDev* m, const int* irp, const int* ja, const void* val) {
+  const int S = csr_slice_count(m, irp, ja);
+  if (S < 2) return PSB_HIP_SUCCESS;
+  const size_t es = elem_size(m->elem_type);
+  const int w = (m->cols + S - 1) / S;
+  m->slices = new CsrDev*[S]();
+  m->nslices = S;
+  std::vector<int> sirp(m->rows + 1), sja;
+  std::vector<char> sval;
+  for (int s = 0; s < S; s++) {
+    const int lo = s * w + 1, hi = std::min(m->cols, (s + 1) * w);
+    sja.clear();
+    sval.clear();
+    sirp[0] = 1;
+    for (int i = 0; i < m->rows; i++) {
+      for (int k = irp[i] - 1; k < irp[i + 1] - 1; k++)
+        if (ja[k] >= lo && ja[k] <= hi) {
+          sja.push_back(ja[k]);
+          const char* v = (const char*)val + (size_t)k * es;
+          sval.insert(sval.end(), v, v + es);
+        }
+      sirp[i + 1] = (int)sja.size() + 1;
+    }
+    void* sub = nullptr;
+    int rc = psb_hip_csr_alloc(&sub, m->rows, m->cols, (int)sja.size(), m->elem_type, 1);
+    if (!rc) {
+      m->slices[s] = (CsrDev*)sub;
+      rc = csr_write_impl((CsrDev*)sub, sirp.data(), sja.data(), sval.data());
+    }
+    if (rc) {
+      free_csr_slices(m);
+      return rc;
+    }
+  }
+  return PSB_HIP_SUCCESS;
+}
+
+}  // namespace psbhip
 
 extern "C" {
 
@@ -1242,6 +1435,7 @@ void psb_hip_csr_free(void* dmat) {
   CsrDev* m = (CsrDev*)dmat;
   if (ctx().initialized) hipStreamSynchronize(ctx().stream);
   release_csr_sv(m);
+  free_csr_slices(m);
   hipFree(m->irp);
   hipFree(m->ja);
   hipFree(m->val);
@@ -1255,103 +1449,11 @@ void psb_hip_csr_free(void* dmat) {
 int psb_hip_csr_write(void* dmat, const int* irp, const int* ja, const void* val) {
   PSB_ENSURE_INIT();
   CsrDev* m = (CsrDev*)dmat;
-  release_csr_sv(m);   // a prepared triangular solve describes the old values
-  if (irp[0] != 1 || irp[m->rows] - 1 != m->nnz)
-    return set_error(PSB_HIP_INVALID, __func__, "irp inconsistent with nnz");
-  for (int i = 0; i < m->rows; i++)
-    if (irp[i + 1] < irp[i]) return set_error(PSB_HIP_INVALID, __func__, "irp not monotone");
-  int mn = min_index(ja, m->nnz), mx = max_index(ja, m->nnz);
-  if (m->nnz > 0 && (mn < 1 || mx > m->cols))
-    return set_error(PSB_HIP_INVALID, __func__, "ja out of range");
-  m->max_col = mx;
-  // analysis: lanes per row from the mean row length, long rows listed
-  double mean = m->rows > 0 ? (double)m->nnz / m->rows : 0.0;
-  int g = 1;
-  while (g < 64 && g < mean) g <<= 1;
-  m->lanes = g;
-  m->long_thr = std::max(256, 8 * g);
-  std::vector<int> longs;
-  for (int i = 0; i < m->rows; i++)
-    if (irp[i + 1] - irp[i] > m->long_thr) longs.push_back(i);
-  m->nlong = (int)longs.size();
-  // length-binned analysis
-  int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  std::vector<int> longs_b;
-  for (int i = 0; i < m->rows; i++) {
-    const int len = irp[i + 1] - irp[i];
-    if (len > kCsrLongRow) {
-      longs_b.push_back(i);
-      continue;
-    }
-    cnt[csr_class(len)]++;
-  }
-  m->bin_start[0] = 0;
-  for (int c = 0; c < 7; c++) m->bin_start[c + 1] = m->bin_start[c] + cnt[c];
-  std::vector<int> bins((size_t)(m->bin_start[7] > 0 ? m->bin_start[7] : 1));
-  {
-    int fill[7];
-    for (int c = 0; c < 7; c++) fill[c] = m->bin_start[c];
-    for (int i = 0; i < m->rows; i++) {
-      const int len = irp[i + 1] - irp[i];
-      if (len <= kCsrLongRow) bins[(size_t)fill[csr_class(len)]++] = i;
-    }
-  }
-  m->nlong_b = (int)longs_b.size();
-  // row blocks: consecutive rows up to kBlock of them and kCsrBlockNnz
-  // entries; a row longer than kCsrLongRow ends the block and is left out
-  std::vector<int2> sblk;
-  {
-    int r0 = 0;
-    while (r0 < m->rows) {
-      if (irp[r0 + 1] - irp[r0] > kCsrLongRow) {
-        r0++;
-        continue;
-      }
-      int r1 = r0;
-      while (r1 < m->rows && r1 - r0 < kBlock && irp[r1 + 1] - irp[r1] <= kCsrLongRow &&
-             irp[r1 + 1] - irp[r0] <= kCsrBlockNnz)
-        r1++;
-      sblk.push_back(make_int2(r0, r1));
-      r0 = r1;
-    }
-  }
-  m->nsblk = (int)sblk.size();
-  hipStream_t s = ctx().stream;
-  hipFree(m->long_rows);
-  m->long_rows = nullptr;
-  hipFree(m->bin_rows);
-  m->bin_rows = nullptr;
-  hipFree(m->long_rows_b);
-  m->long_rows_b = nullptr;
-  if (m->nlong > 0) {
-    PSB_HIP_CHECK(hipMalloc(&m->long_rows, sizeof(int) * m->nlong));
-    PSB_HIP_CHECK(hipMemcpyAsync(m->long_rows, longs.data(), sizeof(int) * m->nlong,
-                                 hipMemcpyHostToDevice, s));
-  }
-  if (m->rows > 0) {
-    PSB_HIP_CHECK(hipMalloc(&m->bin_rows, sizeof(int) * bins.size()));
-    PSB_HIP_CHECK(hipMemcpyAsync(m->bin_rows, bins.data(), sizeof(int) * bins.size(),
-                                 hipMemcpyHostToDevice, s));
-  }
-  hipFree(m->sblk);
-  m->sblk = nullptr;
-  if (m->nsblk > 0) {
-    PSB_HIP_CHECK(hipMalloc(&m->sblk, sizeof(int2) * m->nsblk));
-    PSB_HIP_CHECK(hipMemcpyAsync(m->sblk, sblk.data(), sizeof(int2) * m->nsblk,
-                                 hipMemcpyHostToDevice, s));
-  }
-  if (m->nlong_b > 0) {
-    PSB_HIP_CHECK(hipMalloc(&m->long_rows_b, sizeof(int) * m->nlong_b));
-    PSB_HIP_CHECK(hipMemcpyAsync(m->long_rows_b, longs_b.data(), sizeof(int) * m->nlong_b,
-                                 hipMemcpyHostToDevice, s));
-  }
-  size_t es = elem_size(m->elem_type);
-  PSB_HIP_CHECK(hipMemcpyAsync(m->irp, irp, sizeof(int) * (m->rows + 1), hipMemcpyHostToDevice, s));
-  PSB_HIP_CHECK(hipMemcpyAsync(m->ja, ja, sizeof(int) * (size_t)m->nnz, hipMemcpyHostToDevice, s));
-  PSB_HIP_CHECK(hipMemcpyAsync(m->val, val, es * (size_t)m->nnz, hipMemcpyHostToDevice, s));
-  PSB_HIP_CHECK(hipStreamSynchronize(s));
-  return PSB_HIP_SUCCESS;
+  const int rc = csr_write_impl(m, irp, ja, val);
+  if (rc) return rc;
+  return csr_build_slices(m, irp, ja, val);
 }
+
 
 int psb_hip_csr_read(void* dmat, int* irp, int* ja, void* val) {
   PSB_ENSURE_INIT();
@@ -1391,8 +1493,19 @@ int psb_hip_hll_set_xcd_chunk(int blocks) {
 
 int psb_hip_csr_set_variant(int variant) {
   const int old = g_csr_variant;
-  if (variant >= 0 && variant <= 3) g_csr_variant = variant;
+  if (variant >= 0 && variant <= 4) g_csr_variant = variant;
   return old;
+}
+
+int psb_hip_csr_set_slices(int slices) {
+  const int old = g_csr_slices;
+  if (slices >= -1 && slices <= kCsrMaxSlices) g_csr_slices = slices;
+  return old;
+}
+
+int psb_hip_csr_slices(void* dmat) {
+  CsrDev* m = (CsrDev*)dmat;
+  return m ? m->nslices : -1;
 }
 
 int psb_hip_csr_get_analysis(void* dmat, int* lanes_per_row, int* long_rows, int* long_threshold) {
@@ -1431,8 +1544,11 @@ int psb_hip_csr_scals(void* dmat, double s) {
   CsrDev* m = (CsrDev*)dmat;
   if (!m) return PSB_HIP_INVALID;
   release_csr_sv(m);
-  if (m->elem_type == PSB_HIP_TYPE_DOUBLE) launch_scale<double>(m->nnz, s, m->val);
-  else launch_scale<float>(m->nnz, s, m->val);
+  for (int k = -1; k < m->nslices; k++) {   // the matrix, then its column slices
+    CsrDev* c = k < 0 ? m : m->slices[k];
+    if (m->elem_type == PSB_HIP_TYPE_DOUBLE) launch_scale<double>(c->nnz, s, c->val);
+    else launch_scale<float>(c->nnz, s, c->val);
+  }
   PSB_HIP_LAUNCH_CHECK();
   return PSB_HIP_SUCCESS;
 }
@@ -1483,12 +1599,15 @@ int psb_hip_csr_scal(void* dmat, void* dv, int left) {
   release_csr_sv(m);
   if (m->rows == 0) return PSB_HIP_SUCCESS;
   dim3 grid(grid_for(m->rows, kBlock));
-  if (m->elem_type == PSB_HIP_TYPE_DOUBLE)
-    hipLaunchKernelGGL((k_csr_scal<double>), grid, dim3(kBlock), 0, ctx().stream, m->rows, m->irp,
-                       m->ja, (const double*)d->v, left, (double*)m->val);
-  else
-    hipLaunchKernelGGL((k_csr_scal<float>), grid, dim3(kBlock), 0, ctx().stream, m->rows, m->irp,
-                       m->ja, (const float*)d->v, left, (float*)m->val);
+  for (int k = -1; k < m->nslices; k++) {   // the matrix, then its column slices
+    CsrDev* c = k < 0 ? m : m->slices[k];
+    if (m->elem_type == PSB_HIP_TYPE_DOUBLE)
+      hipLaunchKernelGGL((k_csr_scal<double>), grid, dim3(kBlock), 0, ctx().stream, c->rows,
+                         c->irp, c->ja, (const double*)d->v, left, (double*)c->val);
+    else
+      hipLaunchKernelGGL((k_csr_scal<float>), grid, dim3(kBlock), 0, ctx().stream, c->rows,
+                         c->irp, c->ja, (const float*)d->v, left, (float*)c->val);
+  }
   PSB_HIP_LAUNCH_CHECK();
   return PSB_HIP_SUCCESS;
 }

@@ -491,6 +491,11 @@ class psb_d_hip_csrg_sparse_mat(_hip_base_mat):
                y.dev)
         y.set_dev()
 
+    def slices(self):
+        """Column slices the product runs on (0: the whole matrix); see
+        psb_hip_csr_set_slices."""
+        return int(N.fn("psb_hip_csr_slices")(self.dev))
+
     def sv_levels(self):
         """(levels, workgroup runs) of the prepared triangular solve, (0, 0)
         before the first solve."""

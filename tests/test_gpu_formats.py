@@ -281,6 +281,83 @@ def test_csr_variants(gpu, orc, kind):
     a.free()
 
 
+@pytest.mark.parametrize("slices", [2, 5, 16])
+@pytest.mark.parametrize("single", [False, True], ids=["d", "s"])
+def test_csr_column_slices(gpu, orc, slices, single):
+    """The product over column slices (psb_hip_csr_set_slices) adds each row's
+    slice sums: within the per-row bound for alpha / beta, also after the
+    matrix (and so its slices) is scaled."""
+    _say(f"case csr column slices {slices} single={single}")
+    from psblas3_amd import _native as N
+    coo = orc.gen_powerlaw(20000, cap=1500, seed=7)
+    f = orc.fix_coo(coo)
+    irp, ja, val = orc.csr_from_coo(f)
+    dt = np.float32 if single else np.float64
+    eps = np.finfo(dt).eps
+    mold = P.psb_s_hip_csrg_sparse_mat() if single else P.psb_d_hip_csrg_sparse_mat()
+    vm = P.psb_s_hip_vect() if single else None
+    old = N.fn("psb_hip_csr_set_slices")(slices)
+    try:
+        a = _mat(coo, mold)
+    finally:
+        N.fn("psb_hip_csr_set_slices")(old)
+    assert a.a.slices() == slices
+    x = orc.uniform_pm1(10, coo.nc)
+    y0 = orc.uniform_pm1(11, coo.nr)
+    for alpha, beta in ((1.5, -0.5), (1.0, 0.0)):
+        ref = orc.csr_spmv(irp, ja, val, x, alpha, beta, y0)
+        bound = _row_bound(f, x, alpha, beta, y0, eps) * (2 if single else 1)
+        yv = P.vect_from(y0.astype(dt), mold=vm)
+        P.psb_spmm(alpha, a, P.vect_from(x.astype(dt), mold=vm), beta, yv)
+        assert np.all(np.abs(yv.get_vect() - ref) <= bound), (alpha, beta)
+    # scaling reaches the slices: 2 * diag(d) A
+    d = orc.uniform_pm1(12, coo.nr) + 2.0
+    a.a.scals(2.0)
+    a.a.scal(d.astype(dt), side="L")
+    scaled = 2.0 * val * d[np.repeat(np.arange(coo.nr), np.diff(irp))]
+    ref = orc.csr_spmv(irp, ja, scaled, x)
+    fs = orc.COO(coo.nr, coo.nc, f.ia, f.ja, np.abs(scaled))
+    bound = _row_bound(fs, x, 1.0, 0.0, np.zeros(coo.nr), eps) * (8 if single else 2)
+    yv = P.vect_from(np.zeros(coo.nr, dt), mold=vm)
+    P.psb_spmm(1.0, a, P.vect_from(x.astype(dt), mold=vm), 0.0, yv)
+    assert np.all(np.abs(yv.get_vect() - ref) <= bound)
+    # variant 3 runs the row-block kernel on the whole matrix
+    oldv = N.fn("psb_hip_csr_set_variant")(3)
+    y3 = P.vect_from(np.zeros(coo.nr, dt), mold=vm)
+    P.psb_spmm(1.0, a, P.vect_from(x.astype(dt), mold=vm), 0.0, y3)
+    N.fn("psb_hip_csr_set_variant")(oldv)
+    assert np.all(np.abs(y3.get_vect() - ref) <= bound)
+    a.free()
+
+
+def test_csr_slices_chosen_automatically(gpu, orc):
+    """Column slices are built for a scattered matrix whose x exceeds the
+    threshold (13.6 MB of x: 3 slices), not for a banded one of the same
+    size, nor below the threshold."""
+    _say("case csr automatic column slices")
+    from psblas3_amd import _native as N
+    assert N.fn("psb_hip_csr_set_slices")(-1) == -1
+    n = 1_700_000
+    coo = orc.gen_powerlaw(n, cap=1000, seed=9)
+    a = _mat(coo, P.psb_d_hip_csrg_sparse_mat())
+    assert a.a.slices() == 3
+    f = orc.fix_coo(coo)
+    irp, ja, val = orc.csr_from_coo(f)
+    x = orc.uniform_pm1(13, n)
+    yv = P.vect_from(np.zeros(n))
+    P.psb_spmm(1.0, a, P.vect_from(x), 0.0, yv)
+    ref = orc.csr_spmv(irp, ja, val, x)
+    assert np.all(np.abs(yv.get_vect() - ref) <= _row_bound(f, x, 1.0, 0.0, np.zeros(n),
+                                                             np.finfo(float).eps))
+    a.free()
+    s = _mat(orc.gen_convdiff(120), P.psb_d_hip_csrg_sparse_mat())      # 1.728M rows, banded
+    assert s.a.slices() == 0
+    s.free()
+    small = _mat(orc.gen_powerlaw(200000, cap=500, seed=9), P.psb_d_hip_csrg_sparse_mat())
+    assert small.a.slices() == 0
+    small.free()
+
+
 @pytest.mark.parametrize("fmt", ["hll64", "ell", "csr"])
 def test_xcd_block_order_is_a_permutation(gpu, orc, fmt):
     """Every row is computed by one lane whatever the block order, so the
