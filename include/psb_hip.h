@@ -388,6 +388,19 @@ int psb_hip_csr_get_analysis(void *dmat, int *lanes_per_row, int *long_rows,
    wavefront each.
    Returns the previous value. */
 int psb_hip_csr_set_variant(int variant);
+/* Transposed product (the reference computes op 'T' / 'C' on the host:
+   cuda/impl/psb_d_cuda_hlg_vect_mv.F90 and the CSRG / ELG ones call the CPU
+   spmm for trans): tmat_alloc builds the transpose of A (nr x nc, CSR
+   irp/ja/val on the host, 1-based) on the device; tmat_spmv computes
+   y = alpha * A^T x + beta * y with the results of psb_d_csr_csmv trans
+   'T' / 'C' (base/serial/impl/psb_d_csr_impl.F90:129-141,308-370) and of
+   psb_d_hll_csmv (ext/impl/psb_d_hll_csmv.f90:70-110,300-330) bit for bit:
+   each y(c) is scaled by beta, then updated entry by entry in the CPU's
+   order.  x has nr entries, y nc; every column of x / y. */
+int psb_hip_tmat_alloc(void **tmat, int nr, int nc, const int *irp, const int *ja,
+                       const void *val, int elem_type);
+void psb_hip_tmat_free(void *tmat);
+int psb_hip_tmat_spmv(void *tmat, double alpha, void *x, double beta, void *y);
 /* Column slices of a CSR matrix (no reference counterpart): for a matrix
    whose x does not fit the L2 caches and whose columns are scattered (more
    than half of the entries of sampled rows farther than cols/16 from the

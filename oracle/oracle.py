@@ -106,6 +106,9 @@ def _declare(L):
                                  _f64p, _f64p]
     L.orc_d_csr_trsv_unit.restype = None
     L.orc_d_csr_trsv_unit.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
+    for name, fp, ct in (("orc_d_csr_csmv_t", _f64p, C.c_double), ("orc_s_csr_csmv_t", _f32p, C.c_float)):
+        getattr(L, name).restype = None
+        getattr(L, name).argtypes = [C.c_int, C.c_int, _i32p, _i32p, fp, ct, fp, ct, fp]
     for name, fp, ct in (("orc_d_csr_cssv", _f64p, C.c_double), ("orc_s_csr_cssv", _f32p, C.c_float)):
         getattr(L, name).restype = None
         getattr(L, name).argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, ct, _i32p, _i32p, fp, fp,
@@ -348,7 +351,7 @@ def diag_build(irp, ja, val):
 
 
 KRYLOV_METHODS = {"CG": 0, "BICGSTAB": 1, "FCG": 2, "GCR": 3, "RGMRES": 4, "GMRES": 4,
-                  "BICGSTABL": 5}
+                  "BICGSTABL": 5, "CGS": 6, "BICG": 7}
 # default restart length / l when irst is absent (psb_dgcr.f90:207-216,
 # psb_drgmres.f90:201-210, psb_dcgstabl.f90:190-199)
 KRYLOV_IRST = {3: 10, 4: 10, 5: 1}
@@ -357,7 +360,7 @@ KRYLOV_IRST = {3: 10, 4: 10, 5: 1}
 def krylov(method, a, b, x0=None, eps=1e-9, itmax=1000, istop=2, d=None, ani=0.0, irst=None):
     """psb_krylov with the DIAG preconditioner.  a: (irp, ja, val) CSR tuple or
     an HLL (CG / BiCGSTAB only).  method: CG, BICGSTAB, FCG, GCR, RGMRES,
-    BICGSTABL (unknown names: BiCGSTAB, as the reference).  Returns (x,
+    BICGSTABL, CGS, BICG (unknown names: BiCGSTAB, as the reference).  Returns (x,
     iter, err, hist)."""
     meth = KRYLOV_METHODS.get(method.upper(), 1)
     n = len(b)
@@ -424,6 +427,20 @@ def trsv_unit(irp, ja, val, x, lower=True):
     lib().orc_d_csr_trsv_unit(len(irp) - 1, 1 if lower else 0, irp, ja, val,
                               np.ascontiguousarray(x, np.float64), y)
     return y
+
+
+def csr_spmv_t(irp, ja, val, x, alpha=1.0, beta=0.0, y=None, nc=None):
+    """psb_[ds]_csr_csmv with trans 'T': alpha * A^T x + beta * y, A with
+    len(irp) - 1 rows and nc columns (default: the largest column index)."""
+    single = np.asarray(val).dtype == np.float32
+    dt = np.float32 if single else np.float64
+    nr = len(irp) - 1
+    nc = int(np.max(ja)) if nc is None and len(ja) else (nc or 0)
+    out = np.zeros(nc, dt) if y is None else np.array(y, dt, copy=True)
+    fn = lib().orc_s_csr_csmv_t if single else lib().orc_d_csr_csmv_t
+    fn(nr, nc, np.ascontiguousarray(irp, np.int32), np.ascontiguousarray(ja, np.int32),
+       np.ascontiguousarray(val, dt), alpha, np.ascontiguousarray(x, dt), beta, out)
+    return out
 
 
 def csr_cssv(irp, ja, val, x, lower=True, unit=False, trans="N", alpha=1.0, beta=0.0, y=None):

@@ -435,6 +435,37 @@ ORC_CSR_CSMV(orc_d_csr_csmv, double)
 ORC_CSR_CSMV(orc_s_csr_csmv, float)
 
 /*
+ * psb_d_csr_csmv, trans 'T' / 'C' (base/serial/impl/psb_d_csr_impl.F90:129-141,
+ * 308-370; identical for real data): alpha == 0 gives y = beta*y; otherwise
+ * y = beta*y (0 -> 0, 1 -> unchanged, -1 -> -y), then for the rows i in
+ * ascending order and their entries in CSR order y(ja) = y(ja) + val*x(i)
+ * (alpha == 1), y(ja) - val*x(i) (alpha == -1), else y(ja) + alpha*val*x(i).
+ * The CPU HLL (ext/impl/psb_d_hll_csmv.f90:70-110,300-330) applies the same
+ * updates in the same order (rows ascending, entries in stored order).
+ * nr rows of A (x has nr entries), nc columns (y has nc).
+ */
+#define ORC_CSR_CSMV_T(NAME, T)                                                 \
+EXPORT void NAME(int nr, int nc, const int *irp, const int *ja, const T *val,  \
+                 T alpha, const T *x, T beta, T *y)                             \
+{                                                                               \
+  if (alpha == (T)0) {                                                          \
+    for (int c = 0; c < nc; c++) y[c] = beta == (T)0 ? (T)0 : beta * y[c];      \
+    return;                                                                     \
+  }                                                                             \
+  for (int c = 0; c < nc; c++)                                                  \
+    y[c] = beta == (T)0 ? (T)0 : (beta == (T)1 ? y[c] : (beta == (T)-1 ? -y[c] : beta * y[c])); \
+  for (int i = 0; i < nr; i++)                                                  \
+    for (int j = irp[i] - 1; j < irp[i + 1] - 1; j++) {                         \
+      const int c = ja[j] - 1;                                                  \
+      if (alpha == (T)1) y[c] = y[c] + val[j] * x[i];                           \
+      else if (alpha == (T)-1) y[c] = y[c] - val[j] * x[i];                     \
+      else y[c] = y[c] + alpha * val[j] * x[i];                                 \
+    }                                                                           \
+}
+ORC_CSR_CSMV_T(orc_d_csr_csmv_t, double)
+ORC_CSR_CSMV_T(orc_s_csr_csmv_t, float)
+
+/*
  * psb_d_ell_csmv, no-transpose (ext/impl/psb_d_ell_csmv.f90:33-433): the
  * loop runs over the full padded width nzm for every row (padding adds
  * 0*x), then y = alpha*acc + beta*y.
@@ -1392,8 +1423,142 @@ static int orc_dcgstabl(const orc_mat_t *a, const orc_prec_t *pc, const double *
   return 0;
 }
 
+/* psb_dcgs_vect (krylov/psb_dcgs.f90:100-400): CGS, a breakdown (rho or
+   sigma zero) restarts from the true residual. */
+static int orc_dcgs(const orc_mat_t *a, const orc_prec_t *pc, const double *b, double *x,
+                    double eps, int itmax, int istop, double ani,
+                    int *iter, double *err, double *hist)
+{
+  int n = a->nr;
+  double **w = orc_vecs(10, n);
+  double *q = w[0], *qt = w[1], *r = w[2], *rt = w[3], *p = w[4], *v = w[5], *uv = w[6];
+  double *z = w[7], *f = w[8], *s = w[9];
+  orc_conv_t st = {istop, eps, 0, 0, ani, 0, itmax};
+  double errnum = 0, errden = 1, alpha, beta, rho = 0, rho_old, sigma;
+  int itx = 0;
+  if (istop == 2) st.bn2 = orc_d_nrm2(n, b);
+  if (istop == 1) st.bni = orc_d_amax(n, b);
+  if (istop == 3) {
+    memcpy(r, b, sizeof(double) * n);
+    orc_spmm(a, -1.0, x, 1.0, r);
+    st.r0n2 = orc_d_nrm2(n, r);
+  }
+  for (;;) { /* restart */
+    if (itx >= itmax) break;
+    int it = 0, done = 0;
+    orc_d_axpby(n, 1.0, b, 0.0, r);
+    orc_spmm(a, -1.0, x, 1.0, r);
+    orc_d_axpby(n, 1.0, r, 0.0, rt);
+    if (orc_check_conv(&st, itx, n, x, r, &errnum, &errden)) break;
+    rho = 0.0;
+    for (;;) {
+      it++; itx++;
+      rho_old = rho;
+      rho = orc_d_dot(n, rt, r);
+      if (rho == 0.0) break;
+      if (it == 1) {
+        orc_d_axpby(n, 1.0, r, 0.0, uv);
+        orc_d_axpby(n, 1.0, r, 0.0, p);
+      } else {
+        beta = rho / rho_old;
+        orc_d_axpby(n, 1.0, r, 0.0, uv);
+        orc_d_axpby(n, beta, q, 1.0, uv);
+        orc_d_axpby(n, 1.0, q, beta, p);
+        orc_d_axpby(n, 1.0, uv, beta, p);
+      }
+      orc_papply(pc, p, f);
+      orc_spmm(a, 1.0, f, 0.0, v);
+      sigma = orc_d_dot(n, rt, v);
+      if (sigma == 0.0) break;
+      alpha = rho / sigma;
+      orc_d_axpby(n, 1.0, uv, 0.0, q);
+      orc_d_axpby(n, -alpha, v, 1.0, q);
+      orc_d_axpby(n, 1.0, uv, 0.0, s);
+      orc_d_axpby(n, 1.0, q, 1.0, s);
+      orc_papply(pc, s, z);
+      orc_d_axpby(n, alpha, z, 1.0, x);
+      orc_spmm(a, 1.0, z, 0.0, qt);
+      orc_d_axpby(n, -alpha, qt, 1.0, r);
+      int conv = orc_check_conv(&st, itx, n, x, r, &errnum, &errden);
+      if (hist) hist[itx] = errden > 0 ? errnum / errden : errnum;
+      if (conv) { done = 1; break; }
+    }
+    if (done) break;
+  }
+  *iter = itx;
+  *err = errden > 0 ? errnum / errden : errnum;
+  orc_free_vecs(w, 10);
+  return 0;
+}
+
+/* psb_dbicg_vect (krylov/psb_dbicg.f90:100-330): BiCG with the shadow
+   residual rt = r0; the products with A^T are psb_d_csr_csmv's transposed
+   branch (CSR matrices); the preconditioner's transposed apply is DIAG's,
+   the same as its apply. */
+static int orc_dbicg(const orc_mat_t *a, const orc_prec_t *pc, const double *b, double *x,
+                     double eps, int itmax, int istop, double ani,
+                     int *iter, double *err, double *hist)
+{
+  int n = a->nr;
+  double **w = orc_vecs(8, n);
+  double *q = w[0], *qt = w[1], *r = w[2], *rt = w[3], *p = w[4], *pt = w[5], *z = w[6];
+  double *zt = w[7];
+  orc_conv_t st = {istop, eps, 0, 0, ani, 0, itmax};
+  double errnum = 0, errden = 1, alpha, beta, rho = 0, rho_old, sigma;
+  int itx = 0;
+  if (istop == 2) st.bn2 = orc_d_nrm2(n, b);
+  if (istop == 1) st.bni = orc_d_amax(n, b);
+  if (istop == 3) {
+    memcpy(r, b, sizeof(double) * n);
+    orc_spmm(a, -1.0, x, 1.0, r);
+    st.r0n2 = orc_d_nrm2(n, r);
+  }
+  for (;;) { /* restart */
+    if (itx >= itmax) break;
+    int it = 0, done = 0;
+    orc_d_axpby(n, 1.0, b, 0.0, r);
+    orc_spmm(a, -1.0, x, 1.0, r);
+    orc_d_axpby(n, 1.0, r, 0.0, rt);
+    rho = 0.0;
+    if (orc_check_conv(&st, itx, n, x, r, &errnum, &errden)) break;
+    for (;;) {
+      it++; itx++;
+      orc_papply(pc, r, z);
+      orc_papply(pc, rt, zt);
+      rho_old = rho;
+      rho = orc_d_dot(n, rt, z);
+      if (rho == 0.0) break;
+      if (it == 1) {
+        orc_d_axpby(n, 1.0, z, 0.0, p);
+        orc_d_axpby(n, 1.0, zt, 0.0, pt);
+      } else {
+        beta = rho / rho_old;
+        orc_d_axpby(n, 1.0, z, beta, p);
+        orc_d_axpby(n, 1.0, zt, beta, pt);
+      }
+      orc_spmm(a, 1.0, p, 0.0, q);
+      orc_d_csr_csmv_t(n, n, a->irp, a->ja, a->val, 1.0, pt, 0.0, qt);
+      sigma = orc_d_dot(n, pt, q);
+      if (sigma == 0.0) break;
+      alpha = rho / sigma;
+      orc_d_axpby(n, alpha, p, 1.0, x);
+      orc_d_axpby(n, -alpha, q, 1.0, r);
+      orc_d_axpby(n, -alpha, qt, 1.0, rt);
+      int conv = orc_check_conv(&st, itx, n, x, r, &errnum, &errden);
+      if (hist) hist[itx] = errden > 0 ? errnum / errden : errnum;
+      if (conv) { done = 1; break; }
+    }
+    if (done) break;
+  }
+  *iter = itx;
+  *err = errden > 0 ? errnum / errden : errnum;
+  orc_free_vecs(w, 8);
+  return 0;
+}
+
 /* psb_dkrylov_vect with the further methods (krylov/psb_dkrylov.f90:178-206):
-   2 FCG, 3 GCR, 4 RGMRES, 5 BiCGSTAB(l); irst = restart length / l. */
+   2 FCG, 3 GCR, 4 RGMRES, 5 BiCGSTAB(l), 6 CGS, 7 BiCG; irst = restart
+   length / l. */
 EXPORT int orc_d_krylov2_csr(int method, int irst, int nr, const int *irp, const int *ja,
                              const double *val, const double *d, const double *b, double *x,
                              double eps, int itmax, int istop, double ani, int *iter,
@@ -1407,6 +1572,8 @@ EXPORT int orc_d_krylov2_csr(int method, int irst, int nr, const int *irp, const
     case 3: return orc_dgcr(&a, pc, b, x, eps, itmax, istop, ani, irst, iter, err, hist);
     case 4: return orc_drgmres(&a, pc, b, x, eps, itmax, istop, ani, irst, iter, err, hist);
     case 5: return orc_dcgstabl(&a, pc, b, x, eps, itmax, istop, ani, irst, iter, err, hist);
+    case 6: return orc_dcgs(&a, pc, b, x, eps, itmax, istop, ani, iter, err, hist);
+    case 7: return orc_dbicg(&a, pc, b, x, eps, itmax, istop, ani, iter, err, hist);
     default: return orc_d_krylov_csr(method, nr, irp, ja, val, d, b, x, eps, itmax, istop, ani,
                                      iter, err, hist);
   }

@@ -144,6 +144,9 @@ _SIGS = {
     "psb_hip_csr_set_variant": (i32, [i32]),
     "psb_hip_csr_spsv": (i32, [vp, i32, i32, i32, f64, vp, f64, vp]),
     "psb_hip_csr_set_slices": (i32, [i32]),
+    "psb_hip_tmat_alloc": (i32, [pvp, i32, i32, vp, vp, vp, i32]),
+    "psb_hip_tmat_free": (None, [vp]),
+    "psb_hip_tmat_spmv": (i32, [vp, f64, vp, f64, vp]),
     "psb_hip_csr_slices": (i32, [vp]),
     "psb_hip_hll_scals": (i32, [vp, f64]),
     "psb_hip_hll_scal": (i32, [vp, vp, i32]),

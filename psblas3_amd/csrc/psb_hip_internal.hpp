@@ -340,6 +340,16 @@ struct CsrDev {
   CsrDev** slices = nullptr;
 };
 
+// The transpose of a matrix for y = alpha A^T x + beta y (spmv.hip): row c
+// holds the entries (i, c) of A, rows i ascending, 0-based.
+struct TransDev {
+  int elem_type;
+  int rows, cols, nnz;  // rows = columns of A, cols = rows of A
+  int* irp;             // rows + 1 offsets, 0-based
+  int* src;             // the row i of A of each entry
+  void* val;
+};
+
 // frees the prepared triangular solve of a CSR matrix (trsv.hip)
 void release_csr_sv(CsrDev* m);
 

@@ -394,6 +394,40 @@ __global__ __launch_bounds__(kBlock) void k_ell_scal(int rows, int pitch, int nz
   }
 }
 
+// ------------------------------------------------------------------ transposed product
+// y = alpha * A^T x + beta * y over the transpose of A, one thread per row c
+// of A^T (column c of A) whose entries hold the rows i of A in ascending
+// order.  The row is computed as the CPU's column-oriented loop updates y(c)
+// (psb_d_csr_csmv trans 'T', base/serial/impl/psb_d_csr_impl.F90:308-370):
+// y(c) scaled by beta first, then each entry added in that order, as
+// y + v*x(i), y - v*x(i) (alpha = -1) or y + (alpha*v)*x(i).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_tmv_run(int rows, const int* __restrict__ irp,
+                                                    const int* __restrict__ src,
+                                                    const T* __restrict__ val,
+                                                    const T* __restrict__ x, T alpha, T beta,
+                                                    T* __restrict__ y) {
+#pragma clang fp contract(off)   // separate multiply and add, as the CPU loop
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= rows) return;
+  T r;
+  if (alpha == T(0)) {
+    y[c] = beta == T(0) ? T(0) : beta * y[c];
+    return;
+  }
+  if (beta == T(0)) r = T(0);
+  else if (beta == T(1)) r = y[c];
+  else if (beta == T(-1)) r = -y[c];
+  else r = beta * y[c];
+  for (int e = irp[c]; e < irp[c + 1]; e++) {
+    const T v = val[e], xi = x[src[e]];
+    if (alpha == T(1)) r = r + v * xi;
+    else if (alpha == T(-1)) r = r - v * xi;
+    else r = r + alpha * v * xi;
+  }
+  y[c] = r;
+}
+
 // ------------------------------------------------------------------ fills
 // psi_d_convert_hll_from_coo + psi_d_xtr_ell_from_coo, CPU padding rule:
 // one thread per slot row; rows past nr in the last hack get ja = nr, val 0;
@@ -1607,6 +1641,87 @@ int psb_hip_csr_scal(void* dmat, void* dv, int left) {
     else
       hipLaunchKernelGGL((k_csr_scal<float>), grid, dim3(kBlock), 0, ctx().stream, c->rows,
                          c->irp, c->ja, (const float*)d->v, left, (float*)c->val);
+  }
+  PSB_HIP_LAUNCH_CHECK();
+  return PSB_HIP_SUCCESS;
+}
+
+// ------------------------------------------------------------------ transposed product
+int psb_hip_tmat_alloc(void** tmat, int nr, int nc, const int* irp, const int* ja,
+                       const void* val, int elem_type) {
+  PSB_ENSURE_INIT();
+  if (!tmat) return PSB_HIP_INVALID;
+  *tmat = nullptr;
+  if (elem_type != PSB_HIP_TYPE_DOUBLE && elem_type != PSB_HIP_TYPE_FLOAT)
+    return set_error(PSB_HIP_UNSUPPORTED, __func__, "element type");
+  if (nr < 0 || nc < 0 || !irp || irp[0] != 1) return set_error(PSB_HIP_INVALID, __func__, "irp");
+  const int nnz = irp[nr] - 1;
+  for (int i = 0; i < nr; i++)
+    if (irp[i + 1] < irp[i]) return set_error(PSB_HIP_INVALID, __func__, "irp not monotone");
+  for (int k = 0; k < nnz; k++)
+    if (ja[k] < 1 || ja[k] > nc) return set_error(PSB_HIP_INVALID, __func__, "ja out of range");
+  // A^T: row c lists (i, val) for the entries (i, c) of A, rows i ascending
+  // and, within a row of A, in its CSR order
+  const size_t es = elem_size(elem_type);
+  std::vector<int> tirp(nc + 1, 0), src(nnz > 0 ? nnz : 1);
+  std::vector<char> tval((size_t)(nnz > 0 ? nnz : 1) * es);
+  for (int k = 0; k < nnz; k++) tirp[ja[k]]++;
+  for (int c = 0; c < nc; c++) tirp[c + 1] += tirp[c];
+  std::vector<int> fill(tirp.begin(), tirp.end() - 1);
+  for (int i = 0; i < nr; i++)
+    for (int k = irp[i] - 1; k < irp[i + 1] - 1; k++) {
+      const int q = fill[ja[k] - 1]++;
+      src[q] = i;
+      memcpy(tval.data() + (size_t)q * es, (const char*)val + (size_t)k * es, es);
+    }
+  TransDev* t = new TransDev();
+  t->elem_type = elem_type;
+  t->rows = nc;
+  t->cols = nr;
+  t->nnz = nnz;
+  hipError_t e = hipMalloc(&t->irp, sizeof(int) * tirp.size());
+  if (e == hipSuccess) e = hipMalloc(&t->src, sizeof(int) * src.size());
+  if (e == hipSuccess) e = hipMalloc(&t->val, tval.size());
+  if (e == hipSuccess) e = hipMemcpy(t->irp, tirp.data(), sizeof(int) * tirp.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(t->src, src.data(), sizeof(int) * src.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(t->val, tval.data(), tval.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    psb_hip_tmat_free(t);
+    return set_error(e == hipErrorOutOfMemory ? PSB_HIP_OUTOFMEM : PSB_HIP_RUNTIME, __func__,
+                     hipGetErrorString(e));
+  }
+  *tmat = t;
+  return PSB_HIP_SUCCESS;
+}
+
+void psb_hip_tmat_free(void* tmat) {
+  if (!tmat) return;
+  TransDev* t = (TransDev*)tmat;
+  if (ctx().initialized) hipStreamSynchronize(ctx().stream);
+  hipFree(t->irp);
+  hipFree(t->src);
+  hipFree(t->val);
+  delete t;
+}
+
+int psb_hip_tmat_spmv(void* tmat, double alpha, void* xv, double beta, void* yv) {
+  PSB_ENSURE_INIT();
+  TransDev* t = (TransDev*)tmat;
+  MultiVec *x = (MultiVec*)xv, *y = (MultiVec*)yv;
+  if (!t) return PSB_HIP_INVALID;
+  int rc = check_vectors(x, y, t->elem_type, t->cols, t->rows);
+  if (rc) return set_error(rc, __func__, "x/y type, size or aliasing");
+  if (t->rows == 0) return PSB_HIP_SUCCESS;
+  dim3 grid(grid_for(t->rows, kBlock));
+  for (int c = 0; c < x->count; c++) {
+    if (t->elem_type == PSB_HIP_TYPE_DOUBLE)
+      hipLaunchKernelGGL((k_tmv_run<double>), grid, dim3(kBlock), 0, ctx().stream, t->rows, t->irp,
+                         t->src, (const double*)t->val, (const double*)x->v + (size_t)c * x->pitch,
+                         alpha, beta, (double*)y->v + (size_t)c * y->pitch);
+    else
+      hipLaunchKernelGGL((k_tmv_run<float>), grid, dim3(kBlock), 0, ctx().stream, t->rows, t->irp,
+                         t->src, (const float*)t->val, (const float*)x->v + (size_t)c * x->pitch,
+                         (float)alpha, (float)beta, (float*)y->v + (size_t)c * y->pitch);
   }
   PSB_HIP_LAUNCH_CHECK();
   return PSB_HIP_SUCCESS;

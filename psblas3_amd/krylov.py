@@ -1,7 +1,8 @@
 """Krylov solvers over the device plug-ins: psb_krylov (krylov/psb_dkrylov.f90:
 81-227), CG (krylov/psb_dcg.F90:98-333), BiCGSTAB (krylov/psb_dcgstab.f90:
 96-393), FCG (krylov/psb_dfcg.F90), GCR (krylov/psb_dgcr.f90), RGMRES
-(krylov/psb_drgmres.f90) and BiCGSTAB(l) (krylov/psb_dcgstabl.f90) with the
+(krylov/psb_drgmres.f90), BiCGSTAB(l) (krylov/psb_dcgstabl.f90), CGS
+(krylov/psb_dcgs.f90) and BiCG (krylov/psb_dbicg.f90) with the
 stopping tests of krylov/psb_d_krylov_conv_mod.f90:205-356.
 
 These routines issue exactly the reference's sequence of psb_spmm /
@@ -219,6 +220,120 @@ def psb_dcgstab_vect(a, prec, b, x, eps, desc_a, itmax=1000, itrace=0, istop=2):
                     break
     finally:
         for w in (q, r, p, v, sv, t, z, f):
+            w.free()
+    it_, err = psb_end_conv(s, itx)
+    return it_, err, s.history
+
+
+def psb_dcgs_vect(a, prec, b, x, eps, desc_a, itmax=1000, itrace=0, istop=2):
+    """psb_dcgs_vect (krylov/psb_dcgs.f90:100-400): CGS; a zero rho or sigma
+    restarts from the true residual."""
+    methd = "CGS"
+    q, qt, r, rt, p, v, uv, z, f, sv = (psb_geall(desc_a, mold=x.v) for _ in range(10))
+    itx = 0
+    s = psb_init_conv(methd, istop, itrace, itmax, a, x, b, eps, desc_a)
+    try:
+        done = False
+        while not done:  # restart
+            if itx >= itmax:
+                break
+            it = 0
+            psb_geaxpby(1.0, b, 0.0, r, desc_a)
+            psb_spmm(-1.0, a, x, 1.0, r, desc_a)
+            psb_geaxpby(1.0, r, 0.0, rt, desc_a)
+            if psb_check_conv(methd, itx, x, r, desc_a, s, record=False):
+                break
+            rho = 0.0
+            while True:
+                it += 1
+                itx += 1
+                rho_old = rho
+                rho = psb_gedot(rt, r, desc_a)
+                if rho == 0.0:
+                    break
+                if it == 1:
+                    psb_geaxpby(1.0, r, 0.0, uv, desc_a)
+                    psb_geaxpby(1.0, r, 0.0, p, desc_a)
+                else:
+                    beta = rho / rho_old
+                    psb_geaxpby(1.0, r, 0.0, uv, desc_a)
+                    psb_geaxpby(beta, q, 1.0, uv, desc_a)
+                    psb_geaxpby(1.0, q, beta, p, desc_a)
+                    psb_geaxpby(1.0, uv, beta, p, desc_a)
+                prec.apply(p, f, desc_a)
+                psb_spmm(1.0, a, f, 0.0, v, desc_a)
+                sigma = psb_gedot(rt, v, desc_a)
+                if sigma == 0.0:
+                    break
+                alpha = rho / sigma
+                psb_geaxpby(1.0, uv, 0.0, q, desc_a)
+                psb_geaxpby(-alpha, v, 1.0, q, desc_a)
+                psb_geaxpby(1.0, uv, 0.0, sv, desc_a)
+                psb_geaxpby(1.0, q, 1.0, sv, desc_a)
+                prec.apply(sv, z, desc_a)
+                psb_geaxpby(alpha, z, 1.0, x, desc_a)
+                psb_spmm(1.0, a, z, 0.0, qt, desc_a)
+                psb_geaxpby(-alpha, qt, 1.0, r, desc_a)
+                if psb_check_conv(methd, itx, x, r, desc_a, s):
+                    done = True
+                    break
+    finally:
+        for w in (q, qt, r, rt, p, v, uv, z, f, sv):
+            w.free()
+    it_, err = psb_end_conv(s, itx)
+    return it_, err, s.history
+
+
+def psb_dbicg_vect(a, prec, b, x, eps, desc_a, itmax=1000, itrace=0, istop=2):
+    """psb_dbicg_vect (krylov/psb_dbicg.f90:100-330): BiCG with the shadow
+    residual rt = r0; the products with A^T (psb_spmm trans 'C') and the
+    preconditioner's transposed apply run on the device."""
+    methd = "BICG"
+    q, qt, r, rt, p, pt, z, zt = (psb_geall(desc_a, mold=x.v) for _ in range(8))
+    itx = 0
+    s = psb_init_conv(methd, istop, itrace, itmax, a, x, b, eps, desc_a)
+    try:
+        done = False
+        while not done:  # restart
+            if itx >= itmax:
+                break
+            it = 0
+            psb_geaxpby(1.0, b, 0.0, r, desc_a)
+            psb_spmm(-1.0, a, x, 1.0, r, desc_a)
+            psb_geaxpby(1.0, r, 0.0, rt, desc_a)
+            rho = 0.0
+            if psb_check_conv(methd, itx, x, r, desc_a, s, record=False):
+                break
+            while True:
+                it += 1
+                itx += 1
+                prec.apply(r, z, desc_a)
+                prec.apply(rt, zt, desc_a, trans="C")
+                rho_old = rho
+                rho = psb_gedot(rt, z, desc_a)
+                if rho == 0.0:
+                    break
+                if it == 1:
+                    psb_geaxpby(1.0, z, 0.0, p, desc_a)
+                    psb_geaxpby(1.0, zt, 0.0, pt, desc_a)
+                else:
+                    beta = rho / rho_old
+                    psb_geaxpby(1.0, z, beta, p, desc_a)
+                    psb_geaxpby(1.0, zt, beta, pt, desc_a)
+                psb_spmm(1.0, a, p, 0.0, q, desc_a)
+                psb_spmm(1.0, a, pt, 0.0, qt, desc_a, trans="C")
+                sigma = psb_gedot(pt, q, desc_a)
+                if sigma == 0.0:
+                    break
+                alpha = rho / sigma
+                psb_geaxpby(alpha, p, 1.0, x, desc_a)
+                psb_geaxpby(-alpha, q, 1.0, r, desc_a)
+                psb_geaxpby(-alpha, qt, 1.0, rt, desc_a)
+                if psb_check_conv(methd, itx, x, r, desc_a, s):
+                    done = True
+                    break
+    finally:
+        for w in (q, qt, r, rt, p, pt, z, zt):
             w.free()
     it_, err = psb_end_conv(s, itx)
     return it_, err, s.history
@@ -543,7 +658,7 @@ def psb_dcgstabl_vect(a, prec, b, x, eps, desc_a, itmax=1000, itrace=0, istop=2,
 def psb_krylov(method, a, prec, b, x, eps, desc_a, itmax=1000, itrace=0, istop=2,
                irst=None):
     """psb_krylov dispatcher (krylov/psb_dkrylov.f90:81-227): CG, FCG, GCR,
-    BICGSTAB, RGMRES (or GMRES), BICGSTABL; unknown names fall back to
+    CGS, BICG, BICGSTAB, RGMRES (or GMRES), BICGSTABL; unknown names fall back to
     BiCGSTAB as the reference's default branch does.  irst: restart length
     (GCR, RGMRES; default 10) or l (BICGSTABL; default 1).  Returns
     (iter, err, history); x is updated in place."""
@@ -563,6 +678,10 @@ def psb_krylov(method, a, prec, b, x, eps, desc_a, itmax=1000, itrace=0, istop=2
     if m == "BICGSTABL":
         return psb_dcgstabl_vect(a, prec, b, x, eps, desc_a, itmax, itrace, istop,
                                  1 if irst is None else irst)
+    if m == "CGS":
+        return psb_dcgs_vect(a, prec, b, x, eps, desc_a, itmax, itrace, istop)
+    if m == "BICG":
+        return psb_dbicg_vect(a, prec, b, x, eps, desc_a, itmax, itrace, istop)
     return psb_dcgstab_vect(a, prec, b, x, eps, desc_a, itmax, itrace, istop)
 
 

@@ -214,6 +214,29 @@ class _hip_base_mat:
 
     csmm = csmv   # columns of 2-D x / y are the multivector's columns
 
+    # ---- transposed product (the reference runs it on the host): the
+    # transpose is built on the device from the stored entries on first use
+    # and kept until the storage changes
+    def _drop_t(self):
+        t = getattr(self, "_tmat", None)
+        if t is not None:
+            N.fn("psb_hip_tmat_free")(t)
+            self._tmat = None
+
+    def vect_mv_t(self, alpha, x, beta, y):
+        """y = alpha * A^T x + beta * y on the device, equal bit for bit to the
+        CPU's trans 'T' / 'C' product (psb_d_csr_csmv, psb_d_hll_csmv)."""
+        self._check_vects(x, y)
+        if getattr(self, "_tmat", None) is None:
+            irp, ja, val = self.to_csr()
+            h = C.c_void_p()
+            val = np.ascontiguousarray(val, self._dtype)
+            N.call("psb_hip_tmat_alloc", C.byref(h), self.nr, self.nc, irp.ctypes.data,
+                   np.ascontiguousarray(ja, np.int32).ctypes.data, val.ctypes.data, self._etype)
+            self._tmat = h
+        N.call("psb_hip_tmat_spmv", self._tmat, float(alpha), x.dev, float(beta), y.dev)
+        y.set_dev()
+
     # ---- scaling (scals / scal): on the device copy
     def make_nonunit(self):
         """make_nonunit: a unit triangle gets its unit diagonal stored."""
@@ -231,6 +254,7 @@ class _hip_base_mat:
     def scals(self, d):
         """psb_d_cuda_*_scals: every value times the scalar d."""
         self.make_nonunit()
+        self._drop_t()
         N.call(self._pfx + "_scals", self.dev, float(d))
 
     def scal(self, d, side="L"):
@@ -239,6 +263,7 @@ class _hip_base_mat:
         left = str(side).upper() == "L"
         need = self.nr if left else self.nc
         self.make_nonunit()
+        self._drop_t()
         if isinstance(d, psb_hip_base_vect):
             if d._dtype != self._dtype or d.get_nrows() < need:
                 raise N.PsbError(N.INVALID, f"{self.fmt}_scal", "d precision or size")
@@ -340,6 +365,7 @@ class psb_d_hip_hlg_sparse_mat(_hip_base_mat):
         return self.isz * (es + 4) + 4 * (2 * self.nr + len(self.hkoffs))
 
     def free(self):
+        self._drop_t()
         if self.dev is not None:
             N.fn("psb_hip_hll_free")(self.dev)
             self.dev = None
@@ -417,6 +443,7 @@ class psb_d_hip_elg_sparse_mat(_hip_base_mat):
         y.set_dev()
 
     def free(self):
+        self._drop_t()
         if self.dev is not None:
             N.fn("psb_hip_ell_free")(self.dev)
             self.dev = None
@@ -504,6 +531,7 @@ class psb_d_hip_csrg_sparse_mat(_hip_base_mat):
         return lv.value, sg.value
 
     def free(self):
+        self._drop_t()
         if self.dev is not None:
             N.fn("psb_hip_csr_free")(self.dev)
             self.dev = None

@@ -30,6 +30,7 @@ class psb_dprec_type:
             self.ptype = "DIAG"
         self.dv = None
         self.ilu = None
+        self._tri = None
         self.levels = None
         self.segments = None
 
@@ -59,13 +60,41 @@ class psb_dprec_type:
             # psb_d_null_prec apply: y = alpha*x + beta*y
             N.call("psb_hip_mv_axpby", yv.get_nrows(), float(alpha), xv.dev, float(beta), yv.dev)
         elif self.ptype == "BJAC":
-            if trans.upper() != "N":
-                raise N.PsbError(N.UNSUPPORTED, "psb_bjac_apply", f"trans {trans}")
-            N.call("psb_hip_ilu_apply", self.ilu, float(alpha), xv.dev, float(beta), yv.dev)
+            t = str(trans).upper()
+            if t == "N":
+                N.call("psb_hip_ilu_apply", self.ilu, float(alpha), xv.dev, float(beta), yv.dev)
+            elif t in ("T", "C"):
+                self._apply_bjac_t(xv, yv, alpha, beta)
+            else:
+                raise N.PsbError(N.INVALID, "psb_bjac_apply", f"trans {trans!r}")
         else:
             N.call("psb_hip_mv_axybz", yv.get_nrows(), float(alpha), self.dv.dev, xv.dev,
                    float(beta), yv.dev)
         yv.set_dev()
+
+    def _apply_bjac_t(self, xv, yv, alpha, beta):
+        """psb_d_bjac_apply_vect, trans 'T' / 'C' (prec/impl/psb_d_bjacprec_impl.f90):
+        w = d .* (U^-T x) (psb_spsm scale 'L'), then y = alpha L^-T w + beta y,
+        with the factors as unit-triangular CSR matrices solved on the device
+        (built on the first transposed apply)."""
+        from .mat import psb_d_coo_sparse_mat, psb_d_hip_csrg_sparse_mat
+        if self._tri is None:
+            n, lirp, lja, lval, uirp, uja, uval, d = self._factors
+            tri = []
+            for irp, ja, val, lower in ((lirp, lja, lval, True), (uirp, uja, uval, False)):
+                ia = np.repeat(np.arange(1, n + 1, dtype=np.int32), np.diff(irp))
+                m = psb_d_hip_csrg_sparse_mat().cp_from_coo(
+                    psb_d_coo_sparse_mat(n, n, ia, ja, val).fix())
+                m.set_triangle()
+                m.set_lower(lower)
+                m.set_unit()
+                tri.append(m)
+            dv = psb_d_hip_vect().bld(d[:n])
+            self._tri = (tri[0], tri[1], dv, psb_d_hip_vect().all(n))
+        lt, ut, dv, w = self._tri
+        ut.inner_vect_sv(1.0, xv, 0.0, w, trans="T")
+        w.mlt_v(dv)
+        lt.inner_vect_sv(alpha, w, beta, yv, trans="T")
 
     def _build_ilu0(self, a):
         inner = a.a if hasattr(a, "a") else a
@@ -89,6 +118,10 @@ class psb_dprec_type:
                lval.ctypes.data, uirp.ctypes.data, uja.ctypes.data, uval.ctypes.data,
                d.ctypes.data)
         self.ilu = h
+        nl, nu = int(lirp[n]) - 1, int(uirp[n]) - 1
+        self._factors = (n, lirp, lja[:nl].copy(), lval[:nl].copy(), uirp, uja[:nu].copy(),
+                         uval[:nu].copy(), d)
+        self._tri = None
         lo, up = C.c_int(0), C.c_int(0)
         N.call("psb_hip_ilu_levels", h, C.byref(lo), C.byref(up))
         self.levels = (lo.value, up.value)
@@ -103,6 +136,10 @@ class psb_dprec_type:
         if self.ilu is not None:
             N.call("psb_hip_ilu_free", self.ilu)
             self.ilu = None
+        if getattr(self, "_tri", None) is not None:
+            for obj in self._tri:
+                obj.free()
+            self._tri = None
 
 
 def _vmold_for(a):

@@ -169,11 +169,16 @@ def _kind(*vs):
 
 # ------------------------------------------------------------------ BLAS
 def psb_spmm(alpha, a, x, beta, y, desc_a=None, trans="N"):
-    """psb_spmm (base/psblas/psb_dspmm.f90:57-310): y = alpha*A*x + beta*y."""
-    if trans.upper() != "N":
-        raise N.PsbError(N.UNSUPPORTED, "psb_spmm", "transposed products are out of scope")
+    """psb_spmm (base/psblas/psb_dspmm.f90:57-310): y = alpha*op(A)*x + beta*y;
+    op 'T' / 'C' runs on the transpose the storage builds on the device."""
+    t = str(trans).upper()
+    if t not in ("N", "T", "C"):
+        raise N.PsbError(N.INVALID, "psb_spmm", f"trans {trans!r}")
     xv, yv = _dev(x), _dev(y)
-    a.a.vect_mv(alpha, xv, beta, yv)
+    if t == "N":
+        a.a.vect_mv(alpha, xv, beta, yv)
+    else:                       # 'T' and 'C' are the same product for real data
+        a.a.vect_mv_t(alpha, xv, beta, yv)
 
 
 def psb_spsm(alpha, a, x, beta, y, desc_a=None, trans="N", scale="U", choice=None, diag=None):

@@ -397,3 +397,66 @@ def test_xcd_block_order_is_a_permutation(gpu, orc, fmt):
     ref = orc.csr_spmv(irp, ja, val, x)
     assert np.allclose(outs[0][:coo.nr], ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
     a.free()
+
+
+def _torture_t_cases():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "torture_usmv_t.json")) as f:
+        return json.load(f)["cases"]
+
+
+@pytest.mark.parametrize("fmt", ["hll32", "ell", "csr"])
+@pytest.mark.parametrize("single", [False, True], ids=["d", "s"])
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (-1.0, 1.0), (2.5, -0.5), (0.0, 2.0),
+                                        (1.0, -1.0)])
+def test_transposed_product_bitexact(gpu, orc, fmt, single, alpha, beta):
+    """psb_spmm with trans 'T' / 'C': the transpose built on the device from
+    the stored entries, each y(c) updated in the CPU loop's order, equal to
+    psb_d_csr_csmv's transposed branch bit for bit (rectangular matrix, empty
+    rows and columns)."""
+    _say(f"case transposed product {fmt} single={single} alpha={alpha} beta={beta}")
+    base = orc.gen_powerlaw(5000, cap=400, seed=12)
+    keep = (base.ja % 53) != 0                       # some empty columns
+    coo = orc.COO(5000, 5200, base.ia[keep], base.ja[keep], base.val[keep])
+    f = orc.fix_coo(coo)
+    irp, ja, val = orc.csr_from_coo(f)
+    dt = np.float32 if single else np.float64
+    mold = {"hll32": lambda: P.psb_s_hip_hlg_sparse_mat(32) if single else P.psb_d_hip_hlg_sparse_mat(32),
+            "ell": lambda: P.psb_s_hip_elg_sparse_mat() if single else P.psb_d_hip_elg_sparse_mat(),
+            "csr": lambda: P.psb_s_hip_csrg_sparse_mat() if single else P.psb_d_hip_csrg_sparse_mat()}[fmt]
+    a = _mat(coo, mold())
+    vm = P.psb_s_hip_vect() if single else None
+    x = orc.uniform_pm1(13, coo.nr).astype(dt)
+    y0 = orc.uniform_pm1(14, coo.nc).astype(dt)
+    for op in ("T", "C"):
+        yv = P.vect_from(y0.copy(), mold=vm)
+        P.psb_spmm(alpha, a, P.vect_from(x, mold=vm), beta, yv, trans=op)
+        ref = orc.csr_spmv_t(irp, ja, val.astype(dt), x, alpha, beta, y0, nc=coo.nc)
+        got = yv.get_vect()
+        assert got.dtype == dt
+        assert np.array_equal(got.view(np.int32 if single else np.int64),
+                              ref.view(np.int32 if single else np.int64)), op
+    # the transpose follows a rewrite of the storage
+    a.a.scals(2.0)
+    yv = P.vect_from(y0.copy(), mold=vm)
+    P.psb_spmm(alpha, a, P.vect_from(x, mold=vm), beta, yv, trans="T")
+    ref = orc.csr_spmv_t(irp, ja, (2.0 * val).astype(dt) if not single else
+                         (val.astype(dt) * dt(2.0)), x, alpha, beta, y0, nc=coo.nc)
+    assert np.array_equal(yv.get_vect(), ref)
+    a.free()
+
+
+@pytest.mark.parametrize("case", _torture_t_cases(), ids=lambda c: c["name"])
+def test_torture_usmv_transposed_device(gpu, orc, case):
+    single = case["kind"] == "s"
+    dt = np.float32 if single else np.float64
+    vm = P.psb_s_hip_vect() if single else None
+    coo = orc.COO(case["m"], case["k"], np.array(case["ia"], np.int32),
+                  np.array(case["ja"], np.int32), np.array(case["va"]))
+    a = _mat(coo, P.psb_s_hip_csrg_sparse_mat() if single else P.psb_d_hip_csrg_sparse_mat())
+    y = P.vect_from(np.array(case["bcy"], dt), mold=vm)
+    P.psb_spmm(case["alpha"], a, P.vect_from(np.array(case["x"], dt), mold=vm), case["beta"], y,
+               trans=case["trans"])
+    assert y.get_vect().tolist() == case["cy"]
+    a.free()

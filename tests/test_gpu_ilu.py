@@ -148,3 +148,25 @@ def test_device_resident_solvers_refuse_bjac(gpu, orc):
     eng.free()
     prec.free()
     a.free()
+
+
+@pytest.mark.parametrize("name", ["convdiff12", "band6", "poisson40"])
+def test_bjac_transposed_apply_bitexact(gpu, orc, name):
+    """prec%apply with trans 'T' / 'C' (psb_d_bjac_apply_vect): w = d .* U^-T x
+    (psb_spsm scale 'L'), then y = alpha L^-T w + beta y, equal to the same
+    composition of the oracle's CSR solves bit for bit."""
+    _say(f"case BJAC transposed apply {name}")
+    coo = CASES[name](orc)
+    a, desc, prec, csr = _setup(orc, coo)
+    f = orc.ilu0_fact(*csr)
+    x = orc.uniform_pm1(16, coo.nr)
+    y0 = orc.uniform_pm1(17, coo.nr)
+    w = orc.csr_cssv(f.uirp, f.uja, f.uval, x, lower=False, unit=True, trans="T") * f.d
+    for trans, alpha, beta in (("T", 1.0, 0.0), ("C", 2.0, -0.5)):
+        y = P.vect_from(y0.copy())
+        prec.apply(P.vect_from(x), y, desc, trans=trans, alpha=alpha, beta=beta)
+        ref = orc.csr_cssv(f.lirp, f.lja, f.lval, w, lower=True, unit=True, trans="T",
+                           alpha=alpha, beta=beta, y=y0)
+        assert np.array_equal(y.get_vect().view(np.int64), ref.view(np.int64)), trans
+    prec.free()
+    a.free()

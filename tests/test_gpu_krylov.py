@@ -253,10 +253,14 @@ def test_dropin_istop_1_and_3(gpu, orc):
 # the oracle's restatements.  The minimal-residual methods (GCR, RGMRES) and
 # FCG on an SPD matrix are smooth in rounding: +-2 iterations.  BiCGSTAB(l)
 # counts advance by l; its window is the oracle's count +- max(2 l, 15%).
+# BiCG (CG's recurrence with a shadow sequence through A^T) is +-2 as well;
+# CGS squares the BiCG polynomial and amplifies rounding: +- max(3, 20%).
 NEW_CASES = [("FCG", None, "poisson", 14), ("GCR", 8, "convdiff", 12),
              ("RGMRES", 8, "convdiff", 12), ("RGMRES", 30, "poisson", 14),
              ("GCR", 30, "convdiff", 10), ("BICGSTABL", 2, "convdiff", 12),
-             ("BICGSTABL", 4, "poisson", 14)]
+             ("BICGSTABL", 4, "poisson", 14), ("BICG", None, "convdiff", 12),
+             ("BICG", None, "poisson", 14), ("CGS", None, "convdiff", 12),
+             ("CGS", None, "poisson", 10)]
 
 
 @pytest.mark.parametrize("method,irst,kind,L", NEW_CASES)
@@ -276,6 +280,8 @@ def test_more_krylov_methods(gpu, orc, method, irst, kind, L):
     if method == "BICGSTABL":
         assert it % irst == 0
         assert abs(it - ito) <= max(2 * irst, int(0.15 * ito))
+    elif method == "CGS":
+        assert abs(it - ito) <= max(3, int(0.2 * ito))
     else:
         assert abs(it - ito) <= 2
         k = min(len(hist), len(histo), 10)
@@ -295,4 +301,30 @@ def test_krylov_unknown_method_falls_back(gpu, orc):
     r1 = P.psb_krylov("NOSUCH", a, prec, P.vect_from(bh), x1, 1e-9, desc)
     r2 = P.psb_krylov("BICGSTAB", a, prec, P.vect_from(bh), x2, 1e-9, desc)
     assert r1[0] == r2[0] and np.array_equal(x1.get_vect(), x2.get_vect())
+    a.free()
+
+
+def test_bicg_with_bjac(gpu, orc):
+    """BiCG with BJAC / ILU(0): the shadow sequence applies the factors
+    transposed (w = d .* U^-T x, then L^-T w) on the device; it converges to
+    the solution in fewer iterations than with DIAG."""
+    print("case psb_krylov BICG + BJAC", flush=True)
+    coo = orc.gen_convdiff(12, conv=4.0 / np.sqrt(3.0))
+    csr = orc.csr_from_coo(orc.fix_coo(coo))
+    xt = orc.uniform_pm1(21, coo.nr)
+    bh = orc.csr_spmv(*csr, xt)
+    a = P.matrix_from_coo(coo.nr, coo.nc, coo.ia, coo.ja, coo.val,
+                          mold=P.psb_d_hip_hlg_sparse_mat(64))
+    desc = P.psb_cdall(nl=coo.nr)
+    its = {}
+    for pt in ("DIAG", "BJAC"):
+        prec = P.psb_dprec_type(pt).build(a, desc)
+        x = P.vect_from(np.zeros(coo.nr))
+        it, err, _ = P.psb_krylov("BICG", a, prec, P.vect_from(bh), x, 1e-10, desc, itmax=1000)
+        assert err <= 1e-10
+        assert np.linalg.norm(x.get_vect() - xt) <= 1e-6 * np.linalg.norm(xt)
+        its[pt] = it
+        prec.free()
+    print(f"  DIAG {its['DIAG']} its, BJAC {its['BJAC']} its", flush=True)
+    assert its["BJAC"] < its["DIAG"]
     a.free()

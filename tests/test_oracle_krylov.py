@@ -193,3 +193,56 @@ def test_new_methods_solution(orc):
         x, it, err, _ = orc.krylov(m, csr, b, eps=1e-11, itmax=3000, irst=irst)
         assert err <= 1e-11, m
         assert np.linalg.norm(x - xs) <= tol * np.linalg.norm(xs), m
+
+
+# ---------------------------------------------------------------- CGS and BiCG
+def _jacobi_scipy(A, b, fn, tol, maxiter=1000):
+    cnt = [0]
+
+    def cb(xk):
+        cnt[0] += 1
+    x, info = fn(A, b, rtol=tol, M=sp.diags(1.0 / A.diagonal()), callback=cb, maxiter=maxiter)
+    return x, cnt[0], info
+
+
+@pytest.mark.parametrize("method,fn", [("CGS", spla.cgs), ("BICG", spla.bicg)])
+@pytest.mark.parametrize("kind", ["convdiff", "aniso"])
+def test_cgs_bicg_match_scipy(orc, method, fn, kind):
+    """psb_dcgs / psb_dbicg with DIAG against scipy's right-preconditioned
+    CGS / BiCG with the Jacobi matrix: the same recurrences, so the iteration
+    counts agree within one and the solutions within the tolerance."""
+    coo = orc.gen_convdiff(12, conv=3.0) if kind == "convdiff" else orc.gen_aniso2d(30, 0.05)
+    f = orc.fix_coo(coo)
+    irp, ja, val = orc.csr_from_coo(f)
+    A = sp.csr_matrix((val, ja - 1, irp - 1), shape=(coo.nr, coo.nc))
+    xt = orc.uniform_pm1(1, coo.nr)
+    b = A @ xt
+    x, it, err, _ = orc.krylov(method, (irp, ja, val), b, eps=1e-10)
+    xs, its, info = _jacobi_scipy(A, b, fn, 1e-10)
+    assert info == 0 and err <= 1e-10
+    assert abs(it - its) <= 1, (it, its)
+    assert np.linalg.norm(x - xs) <= 1e-7 * np.linalg.norm(xs)
+
+
+def test_bicg_equals_cg_on_spd(orc):
+    """On a symmetric matrix with a symmetric preconditioner BiCG's shadow
+    sequence mirrors the primal one (rt = r, pt = p), so it takes CG's
+    iterates: the same iteration count and solution."""
+    coo = orc.gen_poisson3d(10)
+    irp, ja, val = orc.csr_from_coo(orc.fix_coo(coo))
+    b = orc.csr_spmv(irp, ja, val, orc.uniform_pm1(2, coo.nr))
+    xc, itc, _, _ = orc.krylov("CG", (irp, ja, val), b, eps=1e-10)
+    xb, itb, _, _ = orc.krylov("BICG", (irp, ja, val), b, eps=1e-10)
+    assert abs(itb - itc) <= 1
+    assert np.linalg.norm(xb - xc) <= 1e-8 * np.linalg.norm(xc)
+
+
+def test_cgs_bicg_zero_rhs(orc):
+    """A zero right-hand side: the convergence check before the first
+    iteration (errden = ||b|| = 0, so errnum <= eps decides) stops with zero
+    iterations and x untouched."""
+    coo = orc.gen_convdiff(6, conv=1.0)
+    irp, ja, val = orc.csr_from_coo(orc.fix_coo(coo))
+    for m in ("CGS", "BICG"):
+        x, it, err, _ = orc.krylov(m, (irp, ja, val), np.zeros(coo.nr), eps=1e-10)
+        assert it == 0 and np.all(x == 0.0)

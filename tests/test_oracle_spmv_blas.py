@@ -179,3 +179,46 @@ def test_upd_xyz_xyzw(orc):
     assert np.array_equal(w3, 5.0 * z3 + 6.0 * w)
     y4, z4, w4 = orc.xyzw(1.0, 0.0, 3.0, 4.0, 5.0, 6.0, x, y, z, w)   # a zero -> no-op
     assert np.array_equal(y4, y) and np.array_equal(w4, w)
+
+
+# ---------------------------------------------------------------- transposed product
+def _torture_t_cases():
+    with open(os.path.join(GOLDEN, "torture_usmv_t.json")) as f:
+        return json.load(f)["cases"]
+
+
+@pytest.mark.parametrize("case", _torture_t_cases(), ids=lambda c: c["name"])
+def test_torture_usmv_transposed(orc, case):
+    """The 32 torture usmv cases with trans 't' and 'c' (same arithmetic for
+    real data), through the oracle's transposed CSR product."""
+    coo = orc.fix_coo(orc.COO(case["m"], case["k"], np.array(case["ia"], np.int32),
+                              np.array(case["ja"], np.int32), np.array(case["va"])))
+    dt = np.float32 if case["kind"] == "s" else np.float64
+    irp, ja, val = orc.csr_from_coo(coo)
+    y = orc.csr_spmv_t(irp, ja, val.astype(dt), np.array(case["x"], dt), case["alpha"],
+                       case["beta"], np.array(case["bcy"], dt), nc=case["k"])
+    assert y.tolist() == case["cy"]
+
+
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (-1.0, 1.0), (2.5, -0.5), (0.0, 3.0)])
+def test_csr_transposed_product(orc, alpha, beta):
+    """Against scipy's A^T x within the per-entry bound (the accumulation
+    into y makes the error grow with the column count), and exactly on an
+    integer case."""
+    coo = orc.gen_powerlaw(3000, cap=300, seed=4)
+    f = orc.fix_coo(coo)
+    irp, ja, val = orc.csr_from_coo(f)
+    A = sp.csr_matrix((val, ja - 1, irp - 1), shape=(coo.nr, coo.nc))
+    x = orc.uniform_pm1(5, coo.nr)
+    y0 = orc.uniform_pm1(6, coo.nc)
+    y = orc.csr_spmv_t(irp, ja, val, x, alpha, beta, y0, nc=coo.nc)
+    ref = alpha * (A.T @ x) + beta * y0
+    cnt = np.bincount(ja - 1, minlength=coo.nc)
+    bound = 4 * (cnt + 2) * np.finfo(float).eps * (abs(alpha) * (abs(A).T @ np.abs(x))
+                                                    + abs(beta) * np.abs(y0))
+    assert np.all(np.abs(y - ref) <= bound + 1e-300)
+    # integers: exact
+    vi = np.round(val * 4)
+    xi = np.round(x * 8)
+    yi = orc.csr_spmv_t(irp, ja, vi, xi, 1.0, 0.0, None, nc=coo.nc)
+    assert np.array_equal(yi, sp.csr_matrix((vi, ja - 1, irp - 1), shape=A.shape).T @ xi)
