@@ -24,7 +24,10 @@
  *     err 9.197e-7), scipy.sparse.linalg solutions, CG invariants;
  *     FCG / GCR / RGMRES / BiCGSTAB(l): identities with CG and GMRES on
  *     cases where they coincide, brute-force minimal residual over the
- *     Krylov space for GCR / RGMRES, direct solves;
+ *     Krylov space for GCR / RGMRES, direct solves; CGS / BiCG: scipy's
+ *     Jacobi-preconditioned cgs / bicg iteration counts, BiCG = CG on SPD;
+ *   - transposed CSR product: the torture usmv cases with trans 't' / 'c',
+ *     scipy A^T x, an exact integer case;
  *   - ILU(0) / BJAC: the defining property (L U equals A on A's pattern),
  *     exact LU on tridiagonal matrices, dense triangular solves (scipy);
  *     the library's host factorization equals it bit for bit;
