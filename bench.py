@@ -169,7 +169,7 @@ def solve_bench(P, N, L=256, eps=1e-10):
 def methods_bench(P, N, L=40, eps=1e-9, itmax=1000):
     """Every Krylov method behind psb_krylov with DIAG and BJAC/ILU(0): the
     convection-diffusion stencil at L (BiCGSTAB, BiCGSTAB(2), RGMRES(10),
-    GCR(10)) and the SPD Poisson (6,-1) stencil at L (CG, FCG); b = A*ones,
+    GCR(10), CGS, BiCG) and the SPD Poisson (6,-1) stencil at L (CG, FCG); b = A*ones,
     x0 = 0, istop 2.  The device-resident BiCGSTAB / CG with DIAG are timed
     beside them."""
     import numpy as np
@@ -254,6 +254,7 @@ def methods_bench(P, N, L=40, eps=1e-9, itmax=1000):
         f"T {out['csr_spsv_T_us']} us")
     runs = [("BICGSTAB", "convdiff", None), ("BICGSTABL", "convdiff", 2),
             ("RGMRES", "convdiff", 10), ("GCR", "convdiff", 10),
+            ("CGS", "convdiff", None), ("BICG", "convdiff", None),
             ("CG", "poisson", None), ("FCG", "poisson", None)]
     res = {}
     for method, kind, irst in runs:
