@@ -195,18 +195,23 @@ class _hip_base_mat:
     # ---- host-array products (csmv / csmm: device temporaries, as
     # cuda/impl/psb_d_cuda_hlg_csmv.F90:62-81 does)
     def csmv(self, alpha, x, beta, y, trans="N"):
-        """y = alpha*A*x + beta*y with host arrays; returns the new y."""
+        """y = alpha*op(A)*x + beta*y with host arrays; returns the new y."""
         from .vect import psb_d_hip_vect, psb_s_hip_vect
-        if str(trans).upper() != "N":
-            raise N.PsbError(N.UNSUPPORTED, f"{self.fmt}_csmv", "transposed products")
+        t = str(trans).upper()
+        if t not in ("N", "T", "C"):
+            raise N.PsbError(N.INVALID, f"{self.fmt}_csmv", f"trans {trans!r}")
         mold = psb_s_hip_vect if self._dtype == np.float32 else psb_d_hip_vect
         x = np.asarray(x, self._dtype)
         y = np.asarray(y, self._dtype)
-        if x.shape[0] < self.nc or y.shape[0] < self.nr:
+        nx, ny = (self.nc, self.nr) if t == "N" else (self.nr, self.nc)
+        if x.shape[0] < nx or y.shape[0] < ny:
             raise N.PsbError(N.INVALID, f"{self.fmt}_csmv", "x or y too short")
         xv, yv = mold().bld(x), mold().bld(y)
         try:
-            self.vect_mv(alpha, xv, beta, yv)
+            if t == "N":
+                self.vect_mv(alpha, xv, beta, yv)
+            else:
+                self.vect_mv_t(alpha, xv, beta, yv)
             return yv.get_vect()
         finally:
             xv.free()

@@ -176,8 +176,14 @@ def test_csmv_csmm_host_arrays(gpu, orc, fmt):
         ref = orc.csr_spmv(irp, ja, val, xk[:, q].copy())
         bound = orc.csr_spmv(irp, ja, np.abs(val), np.abs(xk[:, q]))
         assert np.all(np.abs(np.asarray(yk)[:, q] - ref) <= 4 * 25 * 2.2e-16 * bound)
+    # trans 'T': the device transposed product, equal to the CPU loop's bits
+    xt = orc.uniform_pm1(41, coo.nr)
+    yt0 = orc.uniform_pm1(42, coo.nc)
+    yt = m.csmv(2.0, xt, 0.5, yt0, trans="T")
+    irp, ja, val = m.to_csr()
+    assert np.array_equal(yt, orc.csr_spmv_t(irp, ja, val, xt, 2.0, 0.5, yt0, nc=coo.nc))
     with pytest.raises(P.PsbError):
-        m.csmv(1.0, x, 0.0, y0, trans="T")
+        m.csmv(1.0, x, 0.0, y0, trans="X")
     m.free()
 
 
