@@ -644,6 +644,21 @@ def main():
                 f"({ent['us_per_column']} us per column)")
             xk.free()
             yk.free()
+        # the transposed product y = A^T x on the same matrix (HLL storage; the
+        # transpose is built on the device at the first call)
+        P.psb_spmm(1.0, head, x, 0.0, y, trans="T")
+        P.psb_hip_DeviceSync()
+        timer = P.DeviceTimer()
+        timer.start()
+        for _ in range(max(args.steps // 4, 5)):
+            P.psb_spmm(1.0, head, x, 0.0, y, trans="T")
+        t = timer.stop_ms() / max(args.steps // 4, 5)
+        b = spmv_bytes(n, nnz)
+        sweep["A_HLL64_f64_transposed"] = {"us": round(t * 1e3, 2),
+                                           "GBs": round(b / (t * 1e-3) / 1e9, 1)}
+        log(f"bench: A HLL64 f64 transposed product {t * 1e3:.1f} us  "
+            f"{sweep['A_HLL64_f64_transposed']['GBs']} GB/s")
+        head.a._drop_t()
         # matrix B: power-law rows
         t0 = time.perf_counter()
         coo_b = gen_coo(P, N, "powerlaw", args.powerlaw_n)

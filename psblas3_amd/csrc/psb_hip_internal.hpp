@@ -340,14 +340,16 @@ struct CsrDev {
   CsrDev** slices = nullptr;
 };
 
-// The transpose of a matrix for y = alpha A^T x + beta y (spmv.hip): row c
-// holds the entries (i, c) of A, rows i ascending, 0-based.
+// The transpose of a matrix for y = alpha A^T x + beta y (spmv.hip), in hacks
+// of 64 rows stored column-major like HLL: row c holds the entries (i, c) of
+// A, rows i ascending, 0-based.
 struct TransDev {
   int elem_type;
   int rows, cols, nnz;  // rows = columns of A, cols = rows of A
-  int* irp;             // rows + 1 offsets, 0-based
-  int* src;             // the row i of A of each entry
-  void* val;
+  int* hkoffs;          // hack offsets (slots), hacks + 1
+  int* cnt;             // entries of each row
+  int* src;             // the row i of A of each slot (0 in padding)
+  void* val;            // each slot's value (0 in padding)
 };
 
 // frees the prepared triangular solve of a CSR matrix (trsv.hip)

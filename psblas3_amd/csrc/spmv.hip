@@ -395,14 +395,20 @@ __global__ __launch_bounds__(kBlock) void k_ell_scal(int rows, int pitch, int nz
 }
 
 // ------------------------------------------------------------------ transposed product
-// y = alpha * A^T x + beta * y over the transpose of A, one thread per row c
-// of A^T (column c of A) whose entries hold the rows i of A in ascending
-// order.  The row is computed as the CPU's column-oriented loop updates y(c)
-// (psb_d_csr_csmv trans 'T', base/serial/impl/psb_d_csr_impl.F90:308-370):
-// y(c) scaled by beta first, then each entry added in that order, as
-// y + v*x(i), y - v*x(i) (alpha = -1) or y + (alpha*v)*x(i).
+// y = alpha * A^T x + beta * y over the transpose of A stored like HLL: hacks
+// of kTHack rows of A^T (columns of A), slot (c, j) at hkoffs[h] + j*kTHack +
+// lane, one row per lane, so a wavefront's slot loads are coalesced.  Row c
+// holds the rows i of A with an entry in column c, ascending, and is computed
+// as the CPU's column-oriented loop updates y(c) (psb_d_csr_csmv trans 'T',
+// base/serial/impl/psb_d_csr_impl.F90:308-370): y(c) scaled by beta first,
+// then each entry added in that order, as y + v*x(i), y - v*x(i) (alpha = -1)
+// or y + (alpha*v)*x(i), separately rounded.  Slots past the row's count
+// (padding: source 0, value 0) are loaded and their update dropped.
+constexpr int kTHack = 64;
+
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_tmv_run(int rows, const int* __restrict__ irp,
+__global__ __launch_bounds__(kBlock) void k_tmv_run(int rows, const int* __restrict__ hkoffs,
+                                                    const int* __restrict__ cnt,
                                                     const int* __restrict__ src,
                                                     const T* __restrict__ val,
                                                     const T* __restrict__ x, T alpha, T beta,
@@ -410,20 +416,37 @@ __global__ __launch_bounds__(kBlock) void k_tmv_run(int rows, const int* __restr
 #pragma clang fp contract(off)   // separate multiply and add, as the CPU loop
   const int c = blockIdx.x * kBlock + threadIdx.x;
   if (c >= rows) return;
-  T r;
   if (alpha == T(0)) {
     y[c] = beta == T(0) ? T(0) : beta * y[c];
     return;
   }
+  T r;
   if (beta == T(0)) r = T(0);
   else if (beta == T(1)) r = y[c];
   else if (beta == T(-1)) r = -y[c];
   else r = beta * y[c];
-  for (int e = irp[c]; e < irp[c + 1]; e++) {
-    const T v = val[e], xi = x[src[e]];
-    if (alpha == T(1)) r = r + v * xi;
-    else if (alpha == T(-1)) r = r - v * xi;
-    else r = r + alpha * v * xi;
+  const int h = c / kTHack, lane = c - h * kTHack;
+  const int off = hkoffs[h] + lane;
+  const int w = (hkoffs[h + 1] - hkoffs[h]) / kTHack;
+  const int n = cnt[c];
+  for (int j0 = 0; j0 < w; j0 += 4) {
+    int s[4];
+    T v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int j = j0 + k < w ? j0 + k : w - 1;
+      s[k] = ld_stream(src + off + (size_t)j * kTHack);
+      v[k] = ld_stream(val + off + (size_t)j * kTHack);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const T xi = x[s[k]];
+      T t;
+      if (alpha == T(1)) t = r + v[k] * xi;
+      else if (alpha == T(-1)) t = r - v[k] * xi;
+      else t = r + alpha * v[k] * xi;
+      r = j0 + k < n ? t : r;
+    }
   }
   y[c] = r;
 }
@@ -1660,30 +1683,47 @@ int psb_hip_tmat_alloc(void** tmat, int nr, int nc, const int* irp, const int* j
     if (irp[i + 1] < irp[i]) return set_error(PSB_HIP_INVALID, __func__, "irp not monotone");
   for (int k = 0; k < nnz; k++)
     if (ja[k] < 1 || ja[k] > nc) return set_error(PSB_HIP_INVALID, __func__, "ja out of range");
-  // A^T: row c lists (i, val) for the entries (i, c) of A, rows i ascending
-  // and, within a row of A, in its CSR order
+  // A^T row c = column c of A: its count, then hacks of kTHack rows whose
+  // width is their longest row; entries placed rows of A ascending and, within
+  // a row of A, in its CSR order
   const size_t es = elem_size(elem_type);
-  std::vector<int> tirp(nc + 1, 0), src(nnz > 0 ? nnz : 1);
-  std::vector<char> tval((size_t)(nnz > 0 ? nnz : 1) * es);
-  for (int k = 0; k < nnz; k++) tirp[ja[k]]++;
-  for (int c = 0; c < nc; c++) tirp[c + 1] += tirp[c];
-  std::vector<int> fill(tirp.begin(), tirp.end() - 1);
+  const int nh = (nc + kTHack - 1) / kTHack;
+  std::vector<int> cnt(nc > 0 ? nc : 1, 0), hk(nh + 1, 0);
+  for (int k = 0; k < nnz; k++) cnt[ja[k] - 1]++;
+  int64_t total = 0;
+  for (int h = 0; h < nh; h++) {
+    int w = 0;
+    for (int c = h * kTHack; c < std::min(nc, (h + 1) * kTHack); c++) w = std::max(w, cnt[c]);
+    total += (int64_t)w * kTHack;
+    if (total > INT32_MAX)   // a hack is as wide as its fullest column of A
+      return set_error(PSB_HIP_UNSUPPORTED, __func__, "transpose too large (padded slots)");
+    hk[h + 1] = (int)total;
+  }
+  const size_t slots = (size_t)hk[nh];
+  std::vector<int> src(slots > 0 ? slots : 1, 0), fill(nc > 0 ? nc : 1, 0);
+  std::vector<char> tval((slots > 0 ? slots : 1) * es, 0);   // padding: source 0, value 0
   for (int i = 0; i < nr; i++)
     for (int k = irp[i] - 1; k < irp[i + 1] - 1; k++) {
-      const int q = fill[ja[k] - 1]++;
+      const int c = ja[k] - 1, h = c / kTHack;
+      const size_t q = (size_t)hk[h] + (size_t)fill[c]++ * kTHack + (c - h * kTHack);
       src[q] = i;
-      memcpy(tval.data() + (size_t)q * es, (const char*)val + (size_t)k * es, es);
+      memcpy(tval.data() + q * es, (const char*)val + (size_t)k * es, es);
     }
   TransDev* t = new TransDev();
   t->elem_type = elem_type;
   t->rows = nc;
   t->cols = nr;
   t->nnz = nnz;
-  hipError_t e = hipMalloc(&t->irp, sizeof(int) * tirp.size());
+  hipError_t e = hipMalloc(&t->hkoffs, sizeof(int) * hk.size());
+  if (e == hipSuccess) e = hipMalloc(&t->cnt, sizeof(int) * cnt.size());
   if (e == hipSuccess) e = hipMalloc(&t->src, sizeof(int) * src.size());
   if (e == hipSuccess) e = hipMalloc(&t->val, tval.size());
-  if (e == hipSuccess) e = hipMemcpy(t->irp, tirp.data(), sizeof(int) * tirp.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(t->src, src.data(), sizeof(int) * src.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(t->hkoffs, hk.data(), sizeof(int) * hk.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(t->cnt, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(t->src, src.data(), sizeof(int) * src.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(t->val, tval.data(), tval.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     psb_hip_tmat_free(t);
@@ -1698,7 +1738,8 @@ void psb_hip_tmat_free(void* tmat) {
   if (!tmat) return;
   TransDev* t = (TransDev*)tmat;
   if (ctx().initialized) hipStreamSynchronize(ctx().stream);
-  hipFree(t->irp);
+  hipFree(t->hkoffs);
+  hipFree(t->cnt);
   hipFree(t->src);
   hipFree(t->val);
   delete t;
@@ -1715,13 +1756,15 @@ int psb_hip_tmat_spmv(void* tmat, double alpha, void* xv, double beta, void* yv)
   dim3 grid(grid_for(t->rows, kBlock));
   for (int c = 0; c < x->count; c++) {
     if (t->elem_type == PSB_HIP_TYPE_DOUBLE)
-      hipLaunchKernelGGL((k_tmv_run<double>), grid, dim3(kBlock), 0, ctx().stream, t->rows, t->irp,
-                         t->src, (const double*)t->val, (const double*)x->v + (size_t)c * x->pitch,
-                         alpha, beta, (double*)y->v + (size_t)c * y->pitch);
+      hipLaunchKernelGGL((k_tmv_run<double>), grid, dim3(kBlock), 0, ctx().stream, t->rows,
+                         t->hkoffs, t->cnt, t->src, (const double*)t->val,
+                         (const double*)x->v + (size_t)c * x->pitch, alpha, beta,
+                         (double*)y->v + (size_t)c * y->pitch);
     else
-      hipLaunchKernelGGL((k_tmv_run<float>), grid, dim3(kBlock), 0, ctx().stream, t->rows, t->irp,
-                         t->src, (const float*)t->val, (const float*)x->v + (size_t)c * x->pitch,
-                         (float)alpha, (float)beta, (float*)y->v + (size_t)c * y->pitch);
+      hipLaunchKernelGGL((k_tmv_run<float>), grid, dim3(kBlock), 0, ctx().stream, t->rows,
+                         t->hkoffs, t->cnt, t->src, (const float*)t->val,
+                         (const float*)x->v + (size_t)c * x->pitch, (float)alpha, (float)beta,
+                         (float*)y->v + (size_t)c * y->pitch);
   }
   PSB_HIP_LAUNCH_CHECK();
   return PSB_HIP_SUCCESS;
