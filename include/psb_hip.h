@@ -486,6 +486,11 @@ int psb_hip_krylov_mixed(int method, void *dmat, void *diag, void *b, void *x, d
 int psb_hip_ilu0_fact(int n, const int *irp, const int *ja, const double *val, int *lirp,
                       int *lja, double *lval, int *uirp, int *uja, double *uval, double *d,
                       int *bad_row);
+/* psb_silu0_fact (prec/impl/psb_s_ilu0_fact.f90): the same in single
+   precision, pivot threshold 5e-8 (s_epstol). */
+int psb_hip_ilu0_fact_s(int n, const int *irp, const int *ja, const float *val, int *lirp,
+                        int *lja, float *lval, int *uirp, int *uja, float *uval, float *d,
+                        int *bad_row);
 /* the factors on the device, levelled for the triangular solves */
 int psb_hip_ilu_alloc(void **ilu, int n, const int *lirp, const int *lja, const double *lval,
                       const int *uirp, const int *uja, const double *uval, const double *d);

@@ -104,6 +104,9 @@ def _declare(L):
     L.orc_dilu0_fact.restype = C.c_int
     L.orc_dilu0_fact.argtypes = [C.c_int, _i32p, _i32p, _f64p, _i32p, _i32p, _f64p, _i32p, _i32p,
                                  _f64p, _f64p]
+    L.orc_silu0_fact.restype = C.c_int
+    L.orc_silu0_fact.argtypes = [C.c_int, _i32p, _i32p, _f32p, _i32p, _i32p, _f32p, _i32p, _i32p,
+                                 _f32p, _f32p]
     L.orc_d_csr_trsv_unit.restype = None
     L.orc_d_csr_trsv_unit.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
     for name, fp, ct in (("orc_d_csr_csmv_t", _f64p, C.c_double), ("orc_s_csr_csmv_t", _f32p, C.c_float)):
@@ -408,13 +411,16 @@ class ILU0:
 
 
 def ilu0_fact(irp, ja, val):
+    """psb_[ds]ilu0_fact; the dtype of val picks the precision."""
     n = len(irp) - 1
     nz = len(val)
+    dt = np.float32 if np.asarray(val).dtype == np.float32 else np.float64
     lirp = np.zeros(n + 1, np.int32); uirp = np.zeros(n + 1, np.int32)
     lja = np.zeros(max(nz, 1), np.int32); uja = np.zeros(max(nz, 1), np.int32)
-    lval = np.zeros(max(nz, 1)); uval = np.zeros(max(nz, 1)); d = np.zeros(n)
-    rc = lib().orc_dilu0_fact(n, np.ascontiguousarray(irp, np.int32), np.ascontiguousarray(ja, np.int32),
-                              np.ascontiguousarray(val, np.float64), lirp, lja, lval, uirp, uja, uval, d)
+    lval = np.zeros(max(nz, 1), dt); uval = np.zeros(max(nz, 1), dt); d = np.zeros(n, dt)
+    fn = lib().orc_silu0_fact if dt == np.float32 else lib().orc_dilu0_fact
+    rc = fn(n, np.ascontiguousarray(irp, np.int32), np.ascontiguousarray(ja, np.int32),
+            np.ascontiguousarray(val, dt), lirp, lja, lval, uirp, uja, uval, d)
     if rc < 0:
         raise ZeroDivisionError(f"ILU(0) pivot of row {-rc} too small")
     nl, nu = lirp[n] - 1, uirp[n] - 1

@@ -30,7 +30,9 @@
  *     scipy A^T x, an exact integer case;
  *   - ILU(0) / BJAC: the defining property (L U equals A on A's pattern),
  *     exact LU on tridiagonal matrices, dense triangular solves (scipy);
- *     the library's host factorization equals it bit for bit;
+ *     the library's host factorization equals it bit for bit; the FP32
+ *     factorization by the same property at FP32 accuracy and against the
+ *     FP64 factors;
  *   - CSR triangular solve (psb_d_csr_cssv): the torture SpSV known answers
  *     (test/torture/psb_{d,s}_mvsv_tester.f90), scipy dense triangular
  *     solves for every triangle / diagonal / op, exact integer solutions.
@@ -800,56 +802,59 @@ static int orc_check_conv(orc_conv_t *s, int it, int n, const double *x, const d
    and uja/uval with room for nnz entries.  Returns 0, or -i (1-based row)
    when the pivot of row i is below 1.1e-16 in magnitude
    (psb_err_pivot_too_small_). */
-EXPORT int orc_dilu0_fact(int n, const int *irp, const int *ja, const double *val,
-                          int *lirp, int *lja, double *lval, int *uirp, int *uja, double *uval,
-                          double *d)
-{
-  int l1 = 0, l2 = 0;
-  lirp[0] = 1;
-  uirp[0] = 1;
-  for (int i = 1; i <= n; i++) {
-    d[i - 1] = 0.0;
-    /* ilu_copyin: L part (k < i), diagonal, U part (k > i) */
-    for (int j = irp[i - 1]; j < irp[i]; j++) {
-      int k = ja[j - 1];
-      if (k < i && k >= 1) { l1++; lval[l1 - 1] = val[j - 1]; lja[l1 - 1] = k; }
-      else if (k == i) d[i - 1] = val[j - 1];
-      else if (k > i && k <= n) { l2++; uval[l2 - 1] = val[j - 1]; uja[l2 - 1] = k; }
-    }
-    lirp[i] = l1 + 1;
-    uirp[i] = l2 + 1;
-    double dia = d[i - 1];
-    for (int kk = lirp[i - 1]; kk < lirp[i]; kk++) {
-      double temp = lval[kk - 1];
-      int k = lja[kk - 1];
-      lval[kk - 1] = temp * d[k - 1];
-      int low1 = kk + 1, low2 = uirp[i - 1];
-      for (int jj = uirp[k - 1]; jj < uirp[k]; jj++) {
-        int j = uja[jj - 1];
-        if (j < i) {
-          for (int ll = low1; ll < lirp[i]; ll++) {
-            int l = lja[ll - 1];
-            if (l > j) { low1 = ll; break; }
-            if (l == j) { lval[ll - 1] = lval[ll - 1] - temp * uval[jj - 1]; low1 = ll + 1; break; }
-          }
-        } else if (j == i) {
-          dia = dia - temp * uval[jj - 1];
-        } else {
-          for (int ll = low2; ll < uirp[i]; ll++) {
-            int l = uja[ll - 1];
-            if (l > j) { low2 = ll; break; }
-            if (l == j) { uval[ll - 1] = uval[ll - 1] - temp * uval[jj - 1]; low2 = ll + 1; break; }
-          }
-        }
-      }
-    }
-    if (fabs(dia) < 1.1e-16) return -i;
-    dia = 1.0 / dia;
-    d[i - 1] = dia;
-    for (int kk = uirp[i - 1]; kk < uirp[i]; kk++) uval[kk - 1] = uval[kk - 1] * dia;
-  }
-  return 0;
+#define ORC_ILU0(NAME, T, EPS)                                                  \
+EXPORT int NAME(int n, const int *irp, const int *ja, const T *val,                       \
+                int *lirp, int *lja, T *lval, int *uirp, int *uja, T *uval, T *d)         \
+{                                                                                         \
+  int l1 = 0, l2 = 0;                                                                     \
+  lirp[0] = 1;                                                                            \
+  uirp[0] = 1;                                                                            \
+  for (int i = 1; i <= n; i++) {                                                          \
+    d[i - 1] = (T)0;                                                                      \
+    /* ilu_copyin: L part (k < i), diagonal, U part (k > i) */                            \
+    for (int j = irp[i - 1]; j < irp[i]; j++) {                                           \
+      int k = ja[j - 1];                                                                  \
+      if (k < i && k >= 1) { l1++; lval[l1 - 1] = val[j - 1]; lja[l1 - 1] = k; }          \
+      else if (k == i) d[i - 1] = val[j - 1];                                             \
+      else if (k > i && k <= n) { l2++; uval[l2 - 1] = val[j - 1]; uja[l2 - 1] = k; }     \
+    }                                                                                     \
+    lirp[i] = l1 + 1;                                                                     \
+    uirp[i] = l2 + 1;                                                                     \
+    T dia = d[i - 1];                                                                     \
+    for (int kk = lirp[i - 1]; kk < lirp[i]; kk++) {                                      \
+      T temp = lval[kk - 1];                                                              \
+      int k = lja[kk - 1];                                                                \
+      lval[kk - 1] = temp * d[k - 1];                                                     \
+      int low1 = kk + 1, low2 = uirp[i - 1];                                              \
+      for (int jj = uirp[k - 1]; jj < uirp[k]; jj++) {                                    \
+        int j = uja[jj - 1];                                                              \
+        if (j < i) {                                                                      \
+          for (int ll = low1; ll < lirp[i]; ll++) {                                       \
+            int l = lja[ll - 1];                                                          \
+            if (l > j) { low1 = ll; break; }                                              \
+            if (l == j) { lval[ll - 1] = lval[ll - 1] - temp * uval[jj - 1]; low1 = ll + 1; break; }\
+          }                                                                               \
+        } else if (j == i) {                                                              \
+          dia = dia - temp * uval[jj - 1];                                                \
+        } else {                                                                          \
+          for (int ll = low2; ll < uirp[i]; ll++) {                                       \
+            int l = uja[ll - 1];                                                          \
+            if (l > j) { low2 = ll; break; }                                              \
+            if (l == j) { uval[ll - 1] = uval[ll - 1] - temp * uval[jj - 1]; low2 = ll + 1; break; }\
+          }                                                                               \
+        }                                                                                 \
+      }                                                                                   \
+    }                                                                                     \
+    if (fabs((double)dia) < (double)(EPS)) return -i;                                     \
+    dia = (T)1 / dia;                                                                     \
+    d[i - 1] = dia;                                                                       \
+    for (int kk = uirp[i - 1]; kk < uirp[i]; kk++) uval[kk - 1] = uval[kk - 1] * dia;     \
+  }                                                                                       \
+  return 0;                                                                               \
 }
+ORC_ILU0(orc_dilu0_fact, double, 1.1e-16)
+/* psb_silu0_factint (prec/impl/psb_s_ilu0_fact.f90): single precision, s_epstol */
+ORC_ILU0(orc_silu0_fact, float, 5.e-8f)
 
 /* inner_csrsv (base/serial/impl/psb_d_csr_impl.F90:890-941), no transpose,
    unit diagonal: lower (rows ascending) or upper (rows descending),

@@ -165,6 +165,7 @@ _SIGS = {
     "psb_hip_mm_write_coo": (i32, [C.c_char_p, C.c_char_p, i32, i32, i64, vp, vp, vp]),
     "psb_hip_mm_write_vec": (i32, [C.c_char_p, C.c_char_p, i64, vp]),
     "psb_hip_ilu0_fact": (i32, [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, pi32]),
+    "psb_hip_ilu0_fact_s": (i32, [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, pi32]),
     "psb_hip_ilu_alloc": (i32, [pvp, i32, vp, vp, vp, vp, vp, vp, vp]),
     "psb_hip_ilu_free": (i32, [vp]),
     "psb_hip_ilu_levels": (i32, [vp, pi32, pi32]),

@@ -8,6 +8,8 @@
 // row, duplicates merged left to right in input order.
 #include <ctype.h>
 #include <math.h>
+
+#include <cmath>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -59,6 +61,80 @@ struct Uniform {
   double u01(uint64_t i) const { return (double)(mix64(key ^ i) >> 11) * 0x1.0p-53; }
   double pm1(uint64_t i) const { return 2.0 * u01(i) - 1.0; }
 };
+
+// ---------------------------------------------------------------- ILU(0)
+// psb_dilu0_factint / psb_silu0_factint (prec/impl/psb_d_ilu0_fact.f90:330-460,
+// psb_s_ilu0_fact.f90: the same in single precision), ialg ILU_N,
+// upd 'F', no shift: IKJ elimination restricted to the pattern of A, L
+// scaled by the inverse pivots, each U row scaled by its row's inverse pivot.
+
+template <typename T>
+int ilu0_fact(int n, const int* irp, const int* ja, const T* val, int* lirp, int* lja, T* lval,
+              int* uirp, int* uja, T* uval, T* d, int* bad_row) {
+  // pivot threshold: d_epstol / s_epstol (base/modules/psb_const_mod.F90:165-166)
+  const T epstol = sizeof(T) == 8 ? T(1.1e-16) : T(5.e-8f);
+  if (n < 0 || !irp || !lirp || !uirp || !d) return PSB_HIP_INVALID;
+  if (bad_row) *bad_row = 0;
+  int nl = 0, nu = 0;
+  lirp[0] = 1;
+  uirp[0] = 1;
+  for (int i = 1; i <= n; i++) {
+    T dia = T(0);
+    for (int p = irp[i - 1]; p < irp[i]; p++) {
+      const int k = ja[p - 1];
+      if (k >= 1 && k < i) {
+        lval[nl] = val[p - 1];
+        lja[nl++] = k;
+      } else if (k == i) {
+        dia = val[p - 1];
+      } else if (k > i && k <= n) {
+        uval[nu] = val[p - 1];
+        uja[nu++] = k;
+      }
+    }
+    lirp[i] = nl + 1;
+    uirp[i] = nu + 1;
+    for (int kk = lirp[i - 1]; kk < lirp[i]; kk++) {
+      const T lik = lval[kk - 1];
+      const int k = lja[kk - 1];
+      lval[kk - 1] = lik * d[k - 1];
+      int lpos = kk + 1, upos = uirp[i - 1];
+      for (int jj = uirp[k - 1]; jj < uirp[k]; jj++) {
+        const int j = uja[jj - 1];
+        const T u = uval[jj - 1];
+        if (j == i) {
+          dia = dia - lik * u;
+        } else if (j < i) {
+          for (; lpos < lirp[i]; lpos++) {
+            if (lja[lpos - 1] > j) break;
+            if (lja[lpos - 1] == j) {
+              lval[lpos - 1] = lval[lpos - 1] - lik * u;
+              lpos++;
+              break;
+            }
+          }
+        } else {
+          for (; upos < uirp[i]; upos++) {
+            if (uja[upos - 1] > j) break;
+            if (uja[upos - 1] == j) {
+              uval[upos - 1] = uval[upos - 1] - lik * u;
+              upos++;
+              break;
+            }
+          }
+        }
+      }
+    }
+    if (std::fabs(dia) < epstol) {   // psb_err_pivot_too_small_
+      if (bad_row) *bad_row = i;
+      return PSB_HIP_INVALID;
+    }
+    dia = T(1) / dia;
+    d[i - 1] = dia;
+    for (int kk = uirp[i - 1]; kk < uirp[i]; kk++) uval[kk - 1] = uval[kk - 1] * dia;
+  }
+  return PSB_HIP_SUCCESS;
+}
 
 }  // namespace
 
@@ -333,74 +409,16 @@ void psb_hip_fill_uniform_host(uint64_t seed, uint32_t stream, int64_t n, double
 }
 
 // ---------------------------------------------------------------- ILU(0)
-// psb_dilu0_factint (prec/impl/psb_d_ilu0_fact.f90:330-460), ialg ILU_N,
-// upd 'F', no shift: IKJ elimination restricted to the pattern of A, L
-// scaled by the inverse pivots, each U row scaled by its row's inverse pivot.
-
 int psb_hip_ilu0_fact(int n, const int* irp, const int* ja, const double* val, int* lirp,
                       int* lja, double* lval, int* uirp, int* uja, double* uval, double* d,
                       int* bad_row) {
-  if (n < 0 || !irp || !lirp || !uirp || !d) return PSB_HIP_INVALID;
-  if (bad_row) *bad_row = 0;
-  int nl = 0, nu = 0;
-  lirp[0] = 1;
-  uirp[0] = 1;
-  for (int i = 1; i <= n; i++) {
-    double dia = 0.0;
-    for (int p = irp[i - 1]; p < irp[i]; p++) {
-      const int k = ja[p - 1];
-      if (k >= 1 && k < i) {
-        lval[nl] = val[p - 1];
-        lja[nl++] = k;
-      } else if (k == i) {
-        dia = val[p - 1];
-      } else if (k > i && k <= n) {
-        uval[nu] = val[p - 1];
-        uja[nu++] = k;
-      }
-    }
-    lirp[i] = nl + 1;
-    uirp[i] = nu + 1;
-    for (int kk = lirp[i - 1]; kk < lirp[i]; kk++) {
-      const double lik = lval[kk - 1];
-      const int k = lja[kk - 1];
-      lval[kk - 1] = lik * d[k - 1];
-      int lpos = kk + 1, upos = uirp[i - 1];
-      for (int jj = uirp[k - 1]; jj < uirp[k]; jj++) {
-        const int j = uja[jj - 1];
-        const double u = uval[jj - 1];
-        if (j == i) {
-          dia = dia - lik * u;
-        } else if (j < i) {
-          for (; lpos < lirp[i]; lpos++) {
-            if (lja[lpos - 1] > j) break;
-            if (lja[lpos - 1] == j) {
-              lval[lpos - 1] = lval[lpos - 1] - lik * u;
-              lpos++;
-              break;
-            }
-          }
-        } else {
-          for (; upos < uirp[i]; upos++) {
-            if (uja[upos - 1] > j) break;
-            if (uja[upos - 1] == j) {
-              uval[upos - 1] = uval[upos - 1] - lik * u;
-              upos++;
-              break;
-            }
-          }
-        }
-      }
-    }
-    if (fabs(dia) < 1.1e-16) {   // psb_err_pivot_too_small_
-      if (bad_row) *bad_row = i;
-      return PSB_HIP_INVALID;
-    }
-    dia = 1.0 / dia;
-    d[i - 1] = dia;
-    for (int kk = uirp[i - 1]; kk < uirp[i]; kk++) uval[kk - 1] = uval[kk - 1] * dia;
-  }
-  return PSB_HIP_SUCCESS;
+  return ilu0_fact<double>(n, irp, ja, val, lirp, lja, lval, uirp, uja, uval, d, bad_row);
+}
+
+int psb_hip_ilu0_fact_s(int n, const int* irp, const int* ja, const float* val, int* lirp,
+                        int* lja, float* lval, int* uirp, int* uja, float* uval, float* d,
+                        int* bad_row) {
+  return ilu0_fact<float>(n, irp, ja, val, lirp, lja, lval, uirp, uja, uval, d, bad_row);
 }
 
 // ---------------------------------------------------------------- MatrixMarket

@@ -92,12 +92,22 @@ class _hip_base_mat:
         return self._unit
 
     def inner_vect_sv(self, alpha, x, beta, y, trans="N"):
-        """Triangular solve y = alpha * op(T)^-1 x + beta * y.  Only the CSR
-        format solves on the device (psb_d_cuda_csrg_inner_vect_sv); the
-        reference's HLG and ELG types copy to the host for it, which this
-        library does not do."""
-        raise N.PsbError(N.UNSUPPORTED, f"{self.fmt}_inner_vect_sv",
-                         "triangular solves run on the CSRG format")
+        """Triangular solve y = alpha * op(T)^-1 x + beta * y for the HLG and ELG
+        formats (psb_d_cuda_hlg_inner_vect_sv copies to the host and runs the
+        CPU HLL solve): here a CSR copy of the stored entries, built on the
+        device at the first solve, solves on the device with the CPU's
+        results (the CPU HLL and CSR solves take each row's entries in the
+        same order)."""
+        if not self.is_triangle():
+            raise N.PsbError(N.INVALID, f"{self.fmt}_inner_vect_sv", "matrix is not triangular")
+        c = getattr(self, "_svcsr", None)
+        if c is None:
+            csr_type = psb_s_hip_csrg_sparse_mat if self._dtype == np.float32 \
+                else psb_d_hip_csrg_sparse_mat
+            c = csr_type().cp_from_coo(self.to_coo())
+            self._svcsr = c
+        c._triangle, c._upper, c._unit = self._triangle, self._upper, self._unit
+        c.inner_vect_sv(alpha, x, beta, y, trans=trans)
 
     def get_nrows(self):
         return self.nr
@@ -227,6 +237,10 @@ class _hip_base_mat:
         if t is not None:
             N.fn("psb_hip_tmat_free")(t)
             self._tmat = None
+        c = getattr(self, "_svcsr", None)
+        if c is not None:
+            c.free()
+            self._svcsr = None
 
     def vect_mv_t(self, alpha, x, beta, y):
         """y = alpha * A^T x + beta * y on the device, equal bit for bit to the

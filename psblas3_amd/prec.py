@@ -61,58 +61,73 @@ class psb_dprec_type:
             N.call("psb_hip_mv_axpby", yv.get_nrows(), float(alpha), xv.dev, float(beta), yv.dev)
         elif self.ptype == "BJAC":
             t = str(trans).upper()
-            if t == "N":
-                N.call("psb_hip_ilu_apply", self.ilu, float(alpha), xv.dev, float(beta), yv.dev)
-            elif t in ("T", "C"):
-                self._apply_bjac_t(xv, yv, alpha, beta)
-            else:
+            if t not in ("N", "T", "C"):
                 raise N.PsbError(N.INVALID, "psb_bjac_apply", f"trans {trans!r}")
+            if t == "N" and self.ilu is not None:
+                N.call("psb_hip_ilu_apply", self.ilu, float(alpha), xv.dev, float(beta), yv.dev)
+            else:
+                self._apply_bjac_csr(xv, yv, alpha, beta, t)
         else:
             N.call("psb_hip_mv_axybz", yv.get_nrows(), float(alpha), self.dv.dev, xv.dev,
                    float(beta), yv.dev)
         yv.set_dev()
 
-    def _apply_bjac_t(self, xv, yv, alpha, beta):
-        """psb_d_bjac_apply_vect, trans 'T' / 'C' (prec/impl/psb_d_bjacprec_impl.f90):
-        w = d .* (U^-T x) (psb_spsm scale 'L'), then y = alpha L^-T w + beta y,
-        with the factors as unit-triangular CSR matrices solved on the device
-        (built on the first transposed apply)."""
-        from .mat import psb_d_coo_sparse_mat, psb_d_hip_csrg_sparse_mat
+    def _apply_bjac_csr(self, xv, yv, alpha, beta, trans):
+        """psb_d_bjac_apply_vect / psb_s_bjac_apply_vect (prec/impl/
+        psb_d_bjacprec_impl.f90) through the factors as unit-triangular CSR
+        matrices solved on the device (built on first use): op 'N'
+        w = d .* (L^-1 x) (psb_spsm scale 'L'), y = alpha U^-1 w + beta y; op
+        'T' / 'C' w = d .* (U^-T x), y = alpha L^-T w + beta y.  FP32 factors
+        always go this way; FP64 ones for the transposed ops."""
+        from .mat import (psb_d_coo_sparse_mat, psb_d_hip_csrg_sparse_mat,
+                          psb_s_hip_csrg_sparse_mat)
+        from .vect import psb_s_hip_vect
         if self._tri is None:
             n, lirp, lja, lval, uirp, uja, uval, d = self._factors
+            single = d.dtype == np.float32
+            mtype = psb_s_hip_csrg_sparse_mat if single else psb_d_hip_csrg_sparse_mat
+            vtype = psb_s_hip_vect if single else psb_d_hip_vect
             tri = []
             for irp, ja, val, lower in ((lirp, lja, lval, True), (uirp, uja, uval, False)):
                 ia = np.repeat(np.arange(1, n + 1, dtype=np.int32), np.diff(irp))
-                m = psb_d_hip_csrg_sparse_mat().cp_from_coo(
-                    psb_d_coo_sparse_mat(n, n, ia, ja, val).fix())
+                m = mtype().cp_from_coo(psb_d_coo_sparse_mat(n, n, ia, ja, val).fix())
                 m.set_triangle()
                 m.set_lower(lower)
                 m.set_unit()
                 tri.append(m)
-            dv = psb_d_hip_vect().bld(d[:n])
-            self._tri = (tri[0], tri[1], dv, psb_d_hip_vect().all(n))
+            self._tri = (tri[0], tri[1], vtype().bld(d[:n]), vtype().all(n))
         lt, ut, dv, w = self._tri
-        ut.inner_vect_sv(1.0, xv, 0.0, w, trans="T")
+        first, second = (lt, ut) if trans == "N" else (ut, lt)
+        first.inner_vect_sv(1.0, xv, 0.0, w, trans=trans)
         w.mlt_v(dv)
-        lt.inner_vect_sv(alpha, w, beta, yv, trans="T")
+        second.inner_vect_sv(alpha, w, beta, yv, trans=trans)
 
     def _build_ilu0(self, a):
         inner = a.a if hasattr(a, "a") else a
-        if inner._dtype != np.float64:
-            raise N.PsbError(N.UNSUPPORTED, "psb_bjac_precbld", "double matrices")
+        single = inner._dtype == np.float32
+        dt = np.float32 if single else np.float64
         irp, ja, val = inner.to_csr()
+        val = np.ascontiguousarray(val, dt)
         n, nz = len(irp) - 1, max(len(val), 1)
         lirp = np.zeros(n + 1, np.int32)
         uirp = np.zeros(n + 1, np.int32)
         lja, uja = np.zeros(nz, np.int32), np.zeros(nz, np.int32)
-        lval, uval, d = np.zeros(nz), np.zeros(nz), np.zeros(max(n, 1))
+        lval, uval, d = np.zeros(nz, dt), np.zeros(nz, dt), np.zeros(max(n, 1), dt)
         bad = C.c_int(0)
-        rc = N.fn("psb_hip_ilu0_fact")(n, irp.ctypes.data, ja.ctypes.data, val.ctypes.data,
-                                       lirp.ctypes.data, lja.ctypes.data, lval.ctypes.data,
-                                       uirp.ctypes.data, uja.ctypes.data, uval.ctypes.data,
-                                       d.ctypes.data, C.byref(bad))
+        rc = N.fn("psb_hip_ilu0_fact_s" if single else "psb_hip_ilu0_fact")(
+            n, irp.ctypes.data, ja.ctypes.data, val.ctypes.data, lirp.ctypes.data,
+            lja.ctypes.data, lval.ctypes.data, uirp.ctypes.data, uja.ctypes.data,
+            uval.ctypes.data, d.ctypes.data, C.byref(bad))
         if rc != N.SUCCESS:
-            raise N.PsbError(rc, "psb_dilu0_fact", f"pivot of row {bad.value} too small")
+            raise N.PsbError(rc, "psb_silu0_fact" if single else "psb_dilu0_fact",
+                             f"pivot of row {bad.value} too small")
+        if single:   # FP32 factors: applied through the CSR solves
+            nl, nu = int(lirp[n]) - 1, int(uirp[n]) - 1
+            self._factors = (n, lirp, lja[:nl].copy(), lval[:nl].copy(), uirp,
+                             uja[:nu].copy(), uval[:nu].copy(), d)
+            self._tri = None
+            self.levels = self.segments = None
+            return self
         h = C.c_void_p()
         N.call("psb_hip_ilu_alloc", C.byref(h), n, lirp.ctypes.data, lja.ctypes.data,
                lval.ctypes.data, uirp.ctypes.data, uja.ctypes.data, uval.ctypes.data,

@@ -252,12 +252,18 @@ def test_spsv_errors(gpu, orc):
     with pytest.raises(P.PsbError):
         P.psb_spsm(1.0, c, x, 0.0, y, desc, scale="L", diag=P.vect_from(np.ones(49)))
     c.free()
-    h = P.matrix_from_coo(50, 50, np.repeat(np.arange(1, 51, dtype=np.int32), np.diff(irp)),
-                          ja, val, mold=P.psb_d_hip_hlg_sparse_mat(32))
-    h.set_triangle()
-    with pytest.raises(P.PsbError):
-        P.psb_spsm(1.0, h, x, 0.0, y, desc)
-    h.free()
+    # HLG / ELG solve through a device CSR copy of their entries: the CPU's bits
+    for mold in (P.psb_d_hip_hlg_sparse_mat(32), P.psb_d_hip_elg_sparse_mat()):
+        h = P.matrix_from_coo(50, 50, np.repeat(np.arange(1, 51, dtype=np.int32), np.diff(irp)),
+                              ja, val, mold=mold)
+        with pytest.raises(P.PsbError):            # not flagged triangular
+            P.psb_spsm(1.0, h, x, 0.0, y, desc)
+        h.set_triangle()
+        for op in ("N", "T"):
+            P.psb_spsm(1.0, h, x, 0.0, y, desc, trans=op)
+            ref = orc.csr_cssv(irp, ja, val, x.get_vect(), lower=True, unit=False, trans=op)
+            assert np.array_equal(y.get_vect(), ref), op
+        h.free()
     # a non-unit row with no entries
     irp0 = np.array([1, 2, 2, 3], np.int32)
     ja0 = np.array([1, 3], np.int32)

@@ -129,3 +129,56 @@ def test_bjac_solvers_converge(orc, method):
                               irst=2 if method == "BICGSTABL" else None)
     assert err <= 1e-10 and it < itd
     assert np.linalg.norm(x - np.linalg.solve(A, b)) <= 1e-6 * np.linalg.norm(xt)
+
+
+# ---------------------------------------------------------------- single precision
+@pytest.mark.parametrize("name", ["convdiff", "aniso", "poisson"])
+def test_ilu0_single_precision(orc, name):
+    """psb_silu0_fact: the same elimination in FP32.  Pinned by the ILU(0)
+    property at FP32 accuracy, by the FP64 factors (agreement to FP32
+    rounding) and, for the library's host code, bit for bit."""
+    (irp, ja, val), A = _csr(orc, _cases(orc)[name])
+    v32 = val.astype(np.float32)
+    f = orc.ilu0_fact(irp, ja, v32)
+    assert f.lval.dtype == np.float32 and f.d.dtype == np.float32
+    L, U = _dense_factors(f)
+    A32 = sp.csr_matrix((v32.astype(np.float64), ja - 1, irp - 1), shape=A.shape).toarray()
+    mask = A32 != 0
+    assert np.max(np.abs(L @ U - A32)[mask]) <= 1e-5 * np.max(np.abs(A32))
+    f64 = orc.ilu0_fact(irp, ja, v32.astype(np.float64))
+    assert np.allclose(f.lval, f64.lval, rtol=1e-4, atol=1e-6)
+    assert np.allclose(f.d, f64.d, rtol=1e-4)
+    n, nz = len(irp) - 1, len(val)
+    lirp, uirp = np.zeros(n + 1, np.int32), np.zeros(n + 1, np.int32)
+    lja, uja = np.zeros(nz, np.int32), np.zeros(nz, np.int32)
+    lval, uval, d = (np.zeros(nz, np.float32), np.zeros(nz, np.float32),
+                     np.zeros(n, np.float32))
+    bad = C.c_int(0)
+    rc = N.fn("psb_hip_ilu0_fact_s")(n, irp.ctypes.data, ja.ctypes.data, v32.ctypes.data,
+                                     lirp.ctypes.data, lja.ctypes.data, lval.ctypes.data,
+                                     uirp.ctypes.data, uja.ctypes.data, uval.ctypes.data,
+                                     d.ctypes.data, C.byref(bad))
+    assert rc == 0
+    nl, nu = lirp[n] - 1, uirp[n] - 1
+    assert np.array_equal(lja[:nl], f.lja) and np.array_equal(uja[:nu], f.uja)
+    for a_, b_ in ((lval[:nl], f.lval), (uval[:nu], f.uval), (d, f.d)):
+        assert np.array_equal(a_.view(np.int32), b_.view(np.int32))
+
+
+def test_ilu0_single_tridiagonal_against_fp64(orc):
+    """The (1, 2, 1) tridiagonal matrix, where ILU(0) is the exact LU: its
+    pivots are (k+1)/k, so the FP32 inverse pivots equal the FP64 ones to
+    FP32 rounding, the first (1/2) exactly."""
+    n = 40
+    ia, ja, val = [], [], []
+    for i in range(n):
+        for j, v in ((i - 1, 1.0), (i, 2.0), (i + 1, 1.0)):
+            if 0 <= j < n:
+                ia.append(i + 1); ja.append(j + 1); val.append(v)
+    (irp, jj, vv), _ = _csr(orc, orc.COO(n, n, np.array(ia, np.int32), np.array(ja, np.int32),
+                                         np.array(val)))
+    f32 = orc.ilu0_fact(irp, jj, vv.astype(np.float32))
+    f64 = orc.ilu0_fact(irp, jj, vv)
+    assert f32.d[0] == np.float32(0.5) and f64.d[0] == 0.5
+    assert np.allclose(f64.d, np.arange(1, n + 1) / np.arange(2, n + 2), rtol=1e-12)
+    assert np.allclose(f32.d, f64.d, rtol=1e-6)
