@@ -118,7 +118,6 @@ def solve_bench(P, N, L=256, eps=1e-10):
     a = P.psb_dspmat_type()
     a.m = a.n = n
     a.a = P.psb_d_hip_hlg_sparse_mat(hksz=64).cp_from_coo(coo)
-    del coo
     desc = P.psb_cdall(nl=n)
     prec = P.psb_dprec_type("DIAG").build(a, desc)
     xt = P.psb_d_hip_vect().all(n)
@@ -163,7 +162,7 @@ def solve_bench(P, N, L=256, eps=1e-10):
     out["speedup_mixed_vs_fp64"] = max(out["mixed_restart"]["speedup_vs_fp64"],
                                        out["mixed_reliable"]["speedup_vs_fp64"])
     a.free()
-    return out
+    return out, coo
 
 
 def methods_bench(P, N, L=40, eps=1e-9, itmax=1000):
@@ -310,7 +309,7 @@ def aniso_matrix(P, N, L, e):
     return m
 
 
-def batch_bench(P, N, nsys=256, eps=1e-8, seed=3):
+def batch_bench(P, N, nsys=256, eps=1e-8, seed=3, use_graph=True):
     """Batched uneven solves: nsys anisotropic 2-D Poisson systems -e u_xx - u_yy,
     side L log-uniform in [32, 512] and e log-uniform in [1e-3, 1] (host uniform
     stream 0 and 1 of the seed), b = A x_true with x_true ~ U[-1,1) (stream
@@ -354,7 +353,8 @@ def batch_bench(P, N, nsys=256, eps=1e-8, seed=3):
     for s in systems:
         n = s[5]
         total += 3 * ((8 * n + 255) // 256 * 256) + 5 * ((4 * n + 255) // 256 * 256)
-    eng = P.psb_hip_batch("CG", max_systems=nsys, arena_bytes=total + (1 << 20), check_every=8)
+    eng = P.psb_hip_batch("CG", max_systems=nsys, arena_bytes=total + (1 << 20), check_every=8,
+                          use_graph=use_graph)
     mids = [eng.register(s[0], s[1]) for s in systems]
     fps = [eng.footprint(m) for m in mids]
     assert sum(fps) <= total
@@ -406,7 +406,7 @@ def batch_bench(P, N, nsys=256, eps=1e-8, seed=3):
 
 
 def serve_bench(P, N, nmat=64, seed=4, rates=(200.0, 1000.0, 5000.0), count=160, eps=1e-10,
-                arena_mb=768):
+                arena_mb=768, use_graph=True):
     """Serving loop: nmat registered 3-D 7-point convection-diffusion matrices
     (L uniform in [24, 160], convection magnitude uniform in [0, 10]: host
     uniform streams 0 and 1 of the seed; HLG hksz 64) with Jacobi
@@ -453,7 +453,7 @@ def serve_bench(P, N, nmat=64, seed=4, rates=(200.0, 1000.0, 5000.0), count=160,
     # arena: ~13M rows of BiCGSTAB work vectors (60 bytes per row) in flight, enough to keep
     # the device busy; requests beyond it wait in the queue (admission by footprint)
     srv = P.psb_hip_server("BICGSTAB", max_systems=256, arena_bytes=arena_mb << 20,
-                           check_every=4, eps=eps, refine="restart")
+                           check_every=4, eps=eps, refine="restart", use_graph=use_graph)
     for m, prec, _ in systems:
         srv.register(m, prec)
     srv.run([(0.0, k, 999000 + k) for k in range(4)])      # warm: graph capture, first launches
@@ -470,7 +470,8 @@ def serve_bench(P, N, nmat=64, seed=4, rates=(200.0, 1000.0, 5000.0), count=160,
         if rate == 1000.0:
             # BiCGSTAB with FP64 reliable updates on the same stream, for comparison
             srv_r = P.psb_hip_server("BICGSTAB", max_systems=256, arena_bytes=arena_mb << 20,
-                                     check_every=4, eps=eps, refine="reliable", delta=1e-2)
+                                     check_every=4, eps=eps, refine="reliable", delta=1e-2,
+                                     use_graph=use_graph)
             for m, prec, _ in systems:
                 srv_r.register(m, prec)
             srv_r.run([(0.0, k, 999000 + k) for k in range(4)])
@@ -499,27 +500,145 @@ def profiled_traffic(L, variant=0):
     return int(m.group(1)) if m else None
 
 
-def cpu_baseline(seconds=12.0, L=128):
-    """Oracle HLL (hksz 32) FP64 product, OpenMP over the host cores."""
+def _timed(fn, seconds, min_reps=3):
+    """Median seconds per call of fn over about `seconds` (one warm-up call)."""
+    fn()
+    ts, t_end = [], time.perf_counter() + seconds
+    while len(ts) < min_reps or time.perf_counter() < t_end:
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    return ts[len(ts) // 2], len(ts)
+
+
+def cpu_baseline(P, N, coo_a, L, coo_poisson=None, solve_L=256, fp64_cg_iterations=None,
+                 serve_fp64_iterations=None, seconds=4.0):
+    """The reference CPU path restated by the oracle (C, OpenMP over the host
+    cores for the products and the elementwise vector operations; dot and
+    nrm2 sequential, as the reference's BLAS calls), timed on bounded
+    samples of each configuration, on matrices assembled by the library's
+    host code (whose fix is bit-exact with the oracle's):
+      - headline: HLL (hksz 32, the CPU default) and CSR FP64 products on the
+        L=200 stencil, median of repeated products; the HLL SpMM with 8
+        columns on the same matrix;
+      - BiCGSTAB + Jacobi on the L=40 convection-diffusion, b = A*ones, 1e-9;
+      - CG + Jacobi on the SPD Poisson L=256: a 10-iteration sample, time per
+        iteration (times the GPU's FP64 iteration count for a full-solve
+        estimate);
+      - 8 of the 256 anisotropic systems (every 32nd), CG + Jacobi to 1e-8
+        one after another;
+      - 4 serving requests of the 1000 req/s stream: 10 BiCGSTAB iterations
+        each, time per iteration (times the GPU's mean FP64 iteration count
+        per request for a solves/s estimate)."""
     from oracle import oracle as O
     import numpy as np
+    out = {"cores": O.num_threads(), "kind": "port"}
     t0 = time.perf_counter()
-    coo = O.fix_coo(O.gen_convdiff(L))
-    h = O.hll_from_coo(coo, 32)
-    x = O.uniform_pm1(1, coo.nc)
+    f = O.COO(coo_a.nr, coo_a.nc, coo_a.ia, coo_a.ja, coo_a.val)   # fixed on assembly
+    csr = O.csr_from_coo(f)
+    h = O.hll_from_coo(f, 32)
+    x = O.uniform_pm1(1, f.nc)
     log(f"cpu_baseline: oracle setup {time.perf_counter() - t0:.1f}s, {O.num_threads()} threads")
-    y = O.hll_spmv(h, x)
-    reps, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < seconds:
-        y = O.hll_spmv(h, x)
-        reps += 1
-    dt = (time.perf_counter() - t0) / reps
-    gbs = spmv_bytes(coo.nr, coo.nz) / dt / 1e9
-    del y, np
-    return {"value": round(gbs, 3), "unit": "GB/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"oracle HLL(hksz=32) FP64 SpMV, 7-point stencil L={L} "
-                      f"(n={coo.nr}, nnz={coo.nz}), {reps} products in {reps * dt:.1f}s, "
-                      f"ms/product={dt * 1e3:.2f}"}
+    nbytes = spmv_bytes(f.nr, f.nz)
+    t_hll, r_hll = _timed(lambda: O.hll_spmv(h, x), seconds)
+    t_csr, r_csr = _timed(lambda: O.csr_spmv(*csr, x), seconds)
+    out["value"] = round(nbytes / t_hll / 1e9, 3)
+    out["unit"] = "GB/s"
+    out["sample"] = (f"oracle HLL(hksz=32) FP64 SpMV on the 7-point stencil L={L} (n={f.nr}, "
+                     f"nnz={f.nz}): median of {r_hll} products, {t_hll * 1e3:.2f} ms each")
+    out["spmv_L200"] = {"hll32_ms": round(t_hll * 1e3, 3), "hll32_GBs": out["value"],
+                        "csr_ms": round(t_csr * 1e3, 3),
+                        "csr_GBs": round(nbytes / t_csr / 1e9, 3)}
+    X = np.stack([O.uniform_pm1(1, f.nc, stream=c) for c in range(8)], axis=1)
+    t_mm, r_mm = _timed(lambda: O.hll_spmm(h, X), 0.5, min_reps=2)
+    bmm = 12 * f.nz + 4 * f.nr + 16 * f.nr * 8
+    out["spmm_L200_k8"] = {"ms": round(t_mm * 1e3, 2), "GBs": round(bmm / t_mm / 1e9, 2)}
+    del h, csr, X
+    log(f"cpu_baseline: spmv done at {time.perf_counter() - t0:.1f}s")
+    # BiCGSTAB + Jacobi on the L=40 convection-diffusion
+    f40 = O.fix_coo(O.gen_convdiff(40))
+    c40 = O.csr_from_coo(f40)
+    b40 = O.csr_spmv(*c40, np.ones(f40.nc))
+    res = {}
+    t_bi, _ = _timed(lambda: res.update(r=O.krylov("BICGSTAB", c40, b40, eps=1e-9)), 1.0)
+    out["bicgstab_L40"] = {"ms": round(t_bi * 1e3, 2), "iterations": res["r"][1],
+                           "final_rel_resid": res["r"][2]}
+    # CG + Jacobi on the SPD Poisson L=256: a sample of iterations
+    if coo_poisson is not None:
+        fp = O.COO(coo_poisson.nr, coo_poisson.nc, coo_poisson.ia, coo_poisson.ja,
+                   coo_poisson.val)
+        cp = O.csr_from_coo(fp)
+        bp = O.csr_spmv(*cp, O.uniform_pm1(2, fp.nc))
+        k = 10
+        t1 = time.perf_counter()
+        O.krylov("CG", cp, bp, eps=1e-30, itmax=k)
+        per_it = (time.perf_counter() - t1) / k
+        ent = {"L": solve_L, "ms_per_iteration": round(per_it * 1e3, 2), "sample_iterations": k}
+        if fp64_cg_iterations:
+            ent["estimated_solve_s"] = round(per_it * fp64_cg_iterations, 2)
+            ent["estimate"] = f"ms_per_iteration x {fp64_cg_iterations} (the GPU FP64 CG count)"
+        out["cg_poisson"] = ent
+        del cp, bp
+    log(f"cpu_baseline: CG sample done at {time.perf_counter() - t0:.1f}s")
+    # 8 of the 256 anisotropic systems, one after another
+    nsys, seed = 256, 3
+    u = O.uniform_pm1(seed, nsys, stream=0)
+    ue = O.uniform_pm1(seed, nsys, stream=1)
+    Ls = np.rint(32.0 * 16.0 ** ((u + 1.0) / 2.0)).astype(int)
+    es = 10.0 ** (-3.0 + 3.0 * (ue + 1.0) / 2.0)
+    pick = list(range(0, nsys, 32))
+    systems = []
+    for kk in pick:
+        fa = O.fix_coo(O.gen_aniso2d(int(Ls[kk]), float(es[kk])))
+        ca = O.csr_from_coo(fa)
+        systems.append((ca, O.csr_spmv(*ca, O.uniform_pm1(seed, fa.nr, stream=1000 + kk))))
+    t1 = time.perf_counter()
+    its = [O.krylov("CG", ca, ba, eps=1e-8, itmax=20000)[1] for ca, ba in systems]
+    tb = time.perf_counter() - t1
+    out["batch_sample"] = {"systems": len(pick), "rows": int(sum(int(Ls[kk]) ** 2 for kk in pick)),
+                           "seconds": round(tb, 3), "solves_per_s": round(len(pick) / tb, 2),
+                           "iterations_total": int(sum(its)),
+                           "sample": "every 32nd of the 256 systems, FP64 CG + Jacobi to 1e-8, "
+                                     "one after another"}
+    del systems
+    log(f"cpu_baseline: batch sample done at {time.perf_counter() - t0:.1f}s")
+    # serving: BiCGSTAB iterations on 4 requests of the 1000 req/s stream
+    nmat = 64
+    uu = O.uniform_pm1(4, nmat, stream=0)
+    ww = O.uniform_pm1(4, nmat, stream=1)
+    Lm = np.minimum(24 + ((uu + 1.0) / 2.0 * 137).astype(int), 160)
+    cv = (ww + 1.0) / 2.0 * 10.0
+    reqs = P.poisson_arrivals(1000.0, 4, nmat, seed=4, stream=20)
+    fn = N.fn("psb_hip_gen_pde3d")
+    a3 = np.ones(3)
+    per_it, rows = [], []
+    for _, mi, stream in reqs:
+        Lr = int(Lm[mi])
+        b3 = np.full(3, float(cv[mi]) / np.sqrt(3.0))
+        nz = fn(Lr, a3.ctypes.data, b3.ctypes.data, 0.0, 1, 0, None, None, None, None)
+        ia = np.empty(nz, np.int32); ja = np.empty(nz, np.int32); val = np.empty(nz)
+        fn(Lr, a3.ctypes.data, b3.ctypes.data, 0.0, 1, 0, ia.ctypes.data, ja.ctypes.data,
+           val.ctypes.data, None)
+        pc = P.psb_d_coo_sparse_mat(Lr ** 3, Lr ** 3, ia, ja, val).fix()
+        cm = O.csr_from_coo(O.COO(pc.nr, pc.nc, pc.ia, pc.ja, pc.val))
+        bm = O.uniform_pm1(4, pc.nr, stream=stream)
+        t1 = time.perf_counter()
+        O.krylov("BICGSTAB", cm, bm, eps=1e-30, itmax=10)
+        per_it.append((time.perf_counter() - t1) / 10)
+        rows.append(Lr ** 3)
+    ent = {"requests": len(reqs), "L": [int(Lm[mi]) for _, mi, _ in reqs],
+           "ms_per_iteration": [round(t * 1e3, 2) for t in per_it],
+           "sample": "4 requests of the 1000 req/s stream, 10 FP64 BiCGSTAB + Jacobi "
+                     "iterations each"}
+    if serve_fp64_iterations:
+        est = float(np.mean(per_it)) * serve_fp64_iterations
+        ent["estimated_solves_per_s"] = round(1.0 / est, 3)
+        ent["estimate"] = (f"1 / (mean ms_per_iteration x {serve_fp64_iterations}, the GPU FP64 "
+                           "mean iterations per request)")
+    out["serve_sample"] = ent
+    log(f"cpu_baseline: serving sample done at {time.perf_counter() - t0:.1f}s")
+    return out
 
 
 def main():
@@ -540,6 +659,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--serve-arena-mb", type=int, default=768)
     ap.add_argument("--batch-systems", type=int, default=256)
+    ap.add_argument("--eager", action="store_true",
+                    help="batch and serving engines launch their kernels directly instead of "
+                         "replaying a captured graph (the mode profiled runs use)")
     ap.add_argument("--hll-variant", type=int, default=0,
                     help="HLL kernel variant (psb_hip_hll_set_variant): 0 auto, 1 one row per "
                          "lane, 2 row pairs, +4 XCD-contiguous block order")
@@ -673,7 +795,13 @@ def main():
         for fmt, dt, mk in [("CSR", "f64", lambda: P.psb_d_hip_csrg_sparse_mat()),
                             ("CSR", "f32", lambda: P.psb_s_hip_csrg_sparse_mat()),
                             ("HLL64", "f64", lambda: P.psb_d_hip_hlg_sparse_mat(64)),
-                            ("HLL64", "f32", lambda: P.psb_s_hip_hlg_sparse_mat(64))]:
+                            ("HLL64", "f32", lambda: P.psb_s_hip_hlg_sparse_mat(64)),
+                            ("HLL64sorted", "f64",
+                             lambda: P.psb_d_hip_hlg_sparse_mat(64, sort_window=0)),
+                            ("HLL64sorted", "f32",
+                             lambda: P.psb_s_hip_hlg_sparse_mat(64, sort_window=0)),
+                            ("HLL64sorted64K", "f64",
+                             lambda: P.psb_d_hip_hlg_sparse_mat(64, sort_window=65536))]:
             m = P.psb_dspmat_type()
             m.m = m.n = nb
             m.a = mk().cp_from_coo(coo_b)
@@ -703,14 +831,23 @@ def main():
         sweep["B_ELL"] = "not representable (n x max_row_len slots)"
         sweep["B_n"], sweep["B_nnz"] = nb, nnzb
 
-    solve = None if args.no_solve else solve_bench(P, N, args.solve_L)
+    solve, coo_solve = (None, None) if args.no_solve else solve_bench(P, N, args.solve_L)
     methods = None if args.no_methods else methods_bench(P, N, args.methods_L)
-    batch = None if args.no_batch else batch_bench(P, N, args.batch_systems)
+    batch = None if args.no_batch else batch_bench(P, N, args.batch_systems,
+                                                   use_graph=not args.eager)
     serve = None if args.no_serve else serve_bench(P, N, count=args.serve_requests,
-                                                   arena_mb=args.serve_arena_mb)
-    cpu = None if args.no_cpu else cpu_baseline()
-    if cpu:
-        log(f"bench: cpu baseline {cpu['value']} GB/s on {cpu['cores']} threads")
+                                                   arena_mb=args.serve_arena_mb,
+                                                   use_graph=not args.eager)
+    cpu = None
+    if not args.no_cpu:
+        fp64_its = solve["fp64"]["iterations"] if solve else None
+        serve_its = None
+        if serve:
+            serve_its = serve["rates"]["1000"]["fp64_one_at_a_time"].get("iterations_mean")
+        cpu = cpu_baseline(P, N, coo_a, args.L, coo_poisson=coo_solve, solve_L=args.solve_L,
+                           fp64_cg_iterations=fp64_its, serve_fp64_iterations=serve_its)
+        log(f"bench: cpu baseline {cpu['value']} GB/s on {cpu['cores']} threads; "
+            f"{json.dumps({k: v for k, v in cpu.items() if k not in ('sample',)})}")
 
     line = {
         "metric": "SpMV GB/s & %HBM peak; solves/s + p99 latency at 1e-10 resid, mixed vs FP64",

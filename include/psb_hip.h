@@ -212,6 +212,33 @@ int psb_hip_fix_coo_d(int nr, int nc, int64_t nz, int *ia, int *ja, double *val,
 int psb_hip_hll_offsets(int nr, int64_t nz, const int *ia, int hksz, int *irn, int *idisp,
                         int *hkoffs, int64_t *isz);
 
+/* Host copies of the dual-memory plug-ins (cuda/psb_d_cuda_hlg_mat_mod.F90:
+   40-84: host / sync / dev state; the host copy is the CPU format the GPU
+   type extends).  Products on the host copy, for a product whose x or y is
+   not a device vector (cuda/impl/psb_d_cuda_hlg_vect_mv.F90:77-108 falls
+   back to the CPU format's product there): y = alpha*A*x + beta*y with the
+   CPU products' arithmetic (ext/impl/psb_d_hll_csmv.f90:291-371,
+   ext/impl/psb_d_ell_csmv.f90, base/serial/impl/psb_d_csr_impl.F90:46-417:
+   each row's entries in stored order, products and sums separately rounded;
+   y not read when beta = 0).  HLL: position p is row perm(p) when perm is not
+   NULL.  ELL: column-major with leading dimension ld.  elem_type FLOAT or
+   DOUBLE for val, x and y (host arrays). */
+int psb_hip_host_hll_spmv(int nr, int hksz, const int *hkoffs, const int *irn, const int *ja,
+                          const void *val, const int *perm, int elem_type, double alpha,
+                          const void *x, double beta, void *y);
+int psb_hip_host_ell_spmv(int nr, int ld, const int *irn, const int *ja, const void *val,
+                          int elem_type, double alpha, const void *x, double beta, void *y);
+int psb_hip_host_csr_spmv(int nr, const int *irp, const int *ja, const void *val, int elem_type,
+                          double alpha, const void *x, double beta, void *y);
+/* psi_d_convert_hll_from_coo on the host (ext/impl/psi_d_convert_hll_from_coo.f90:
+   31-122 with psi_d_xtr_ell_from_coo's padding, :56-60): the HLL arrays ja /
+   val (hkoffs(noffs+1) slots) and idiag(nr) from a row-sorted COO, irn and
+   idisp as psb_hip_hll_offsets gives them (or, with perm, the per-position
+   counts of psb_hip_hll_sort_rows and per-row idisp). */
+int psb_hip_host_hll_fill(int nr, int hksz, const int *irn, const int *hkoffs, const int *idisp,
+                          const int *cja, const void *cval, const int *perm, int elem_type,
+                          int *ja, void *val, int *idiag);
+
 /* Generators of the benchmark operators (host, COO, 1-based, entries in the
    insertion order of test/pargen/psb_d_pde3d.F90:478-560).  Call with
    NULL arrays to get the entry count.  a[3], b[3]: diffusion and
@@ -277,6 +304,38 @@ int psb_hip_hll_get_params(void *dmat, int *hksz, int *rows, int *nzeros, int *a
 int psb_hip_hll_copy_coo(int nr, int nc, int nza, int hksz, int noffs, int isz,
                          const int *irn, const int *hoffs, const int *idisp, const int *ja,
                          const void *val, void *dmat);
+/* Row-sorted HLL (the reorder array rIdx and per-row sizes rS of the SPGPU
+   HLL kernel, cuda/spgpu/hell.h:37-101 and
+   cuda/spgpu/kernels/hell_spmv_base_template.cuh:197-214,227-252, which the
+   reference HLG type never fills: cuda/hlldev.c:202-204 passes NULL).
+   psb_hip_hll_sort_rows (host): from the row counts irn(nr) of a row-sorted
+   COO, perm(nr) = 1-based row at each position, rows sorted by descending
+   count inside consecutive windows of `window` rows (rounded up to a
+   multiple of hksz; <= 0: one window), ties in ascending row order;
+   irn_pos(nr) the count at each position; hkoffs(noffs+1) the hack offsets
+   over positions (hksz times each hack's widest count); *isz the slots.
+   psb_hip_hll_copy_coo_sorted: as psb_hip_hll_copy_coo with position p
+   holding row perm(p) (idisp per row, irn_pos per position); the matrix's
+   irn and idiag are then per position.  The product writes y(perm(p)) from
+   position p, each row summed over its own entries only (padding not read),
+   in CSR order; a hack wider than 256 slots is computed in several work
+   units whose partial sums are added in slot order (those rows then follow
+   the per-row error bound, every other row equals the unsorted product's
+   value).  Products of several columns run one column at a time.  A later
+   psb_hip_hll_write / psb_hip_hll_copy_coo returns to rows in order.
+   psb_hip_hll_read_perm: the permutation (1..nr when rows are in order).
+   psb_hip_hll_units: work units and cut hacks of the product plan. */
+int psb_hip_hll_sort_rows(int nr, int hksz, int window, const int *irn, int *perm, int *irn_pos,
+                          int *hkoffs, int64_t *isz);
+int psb_hip_hll_copy_coo_sorted(int nr, int nc, int nza, int hksz, int noffs, int isz,
+                                const int *perm, const int *irn_pos, const int *hoffs,
+                                const int *idisp, const int *ja, const void *val, void *dmat);
+int psb_hip_hll_read_perm(void *dmat, int *perm);
+int psb_hip_hll_units(void *dmat, int *nunits, int *nsplit);
+/* After psb_hip_hll_write of arrays in a row-sorted layout (hkoffs the same
+   host array written): install the permutation perm(rows) and the product's
+   work-unit plan (the to_gpu of a row-sorted matrix built on the host). */
+int psb_hip_hll_set_perm(void *dmat, const int *perm, const int *hkoffs);
 /* spmvHllDevice{Double,Float} (cuda/hlldev.c:185-207): y = alpha*A*x +
    beta*y for every column of x/y (the reference computes column 1 only),
    y not read when beta == 0.  x, y multivectors of the matrix type. */

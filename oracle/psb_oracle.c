@@ -267,12 +267,43 @@ static int orc_key_cmp(const void *pa, const void *pb)
  */
 EXPORT int64_t orc_fix_coo(int nr, int nc, int64_t nz, int *ia, int *ja, double *val, int dupl)
 {
-  (void)nr; (void)nc;
+  (void)nc;
   if (nz < 2) return nz;
   orc_key_t *key = (orc_key_t *)malloc(sizeof(orc_key_t) * nz);
   double *vs = (double *)malloc(sizeof(double) * nz);
-  for (int64_t i = 0; i < nz; i++) { key[i].r = ia[i]; key[i].c = ja[i]; key[i].pos = i; }
-  qsort(key, (size_t)nz, sizeof(orc_key_t), orc_key_cmp);
+  int rows_ok = nr > 0;
+  for (int64_t i = 0; i < nz && rows_ok; i++) rows_ok = ia[i] >= 1 && ia[i] <= nr;
+  if (rows_ok) {
+    /* the same total order (row, column, input position) as one sort of the
+       keys: a stable counting sort by row, then each row sorted alone */
+    int64_t *start = (int64_t *)calloc((size_t)nr + 1, sizeof(int64_t));
+    for (int64_t i = 0; i < nz; i++) start[ia[i]]++;
+    for (int r = 0; r < nr; r++) start[r + 1] += start[r];
+    int64_t *fill = (int64_t *)malloc(sizeof(int64_t) * ((size_t)nr + 1));
+    memcpy(fill, start, sizeof(int64_t) * ((size_t)nr + 1));
+    for (int64_t i = 0; i < nz; i++) {
+      orc_key_t *k = &key[fill[ia[i] - 1]++];
+      k->r = ia[i]; k->c = ja[i]; k->pos = i;
+    }
+    #pragma omp parallel for schedule(dynamic, 4096)
+    for (int r = 0; r < nr; r++) {
+      int64_t b = start[r], e = start[r + 1];
+      if (e - b > 32) {
+        qsort(key + b, (size_t)(e - b), sizeof(orc_key_t), orc_key_cmp);
+      } else {
+        for (int64_t i = b + 1; i < e; i++) {   /* insertion sort of a short row */
+          orc_key_t t = key[i];
+          int64_t j = i - 1;
+          while (j >= b && orc_key_cmp(&key[j], &t) > 0) { key[j + 1] = key[j]; j--; }
+          key[j + 1] = t;
+        }
+      }
+    }
+    free(start); free(fill);
+  } else {
+    for (int64_t i = 0; i < nz; i++) { key[i].r = ia[i]; key[i].c = ja[i]; key[i].pos = i; }
+    qsort(key, (size_t)nz, sizeof(orc_key_t), orc_key_cmp);
+  }
   for (int64_t i = 0; i < nz; i++) vs[i] = val[key[i].pos];
   int64_t k = 0;
   ia[0] = key[0].r; ja[0] = key[0].c; val[0] = vs[0];
@@ -602,6 +633,7 @@ EXPORT void orc_d_hll_csmm(int nr, int hksz, const int *hkoffs, const int *irn,
 #define ORC_AXPBY(NAME, T)                                                     \
 EXPORT void NAME(int m, T alpha, const T *x, T beta, T *y)                      \
 {                                                                               \
+  _Pragma("omp parallel for schedule(static) if(m > 65536)")                    \
   for (int i = 0; i < m; i++) {                                                 \
     if (alpha == (T)0) y[i] = beta == (T)0 ? (T)0 : (beta == (T)1 ? y[i] : beta * y[i]); \
     else if (beta == (T)0) y[i] = alpha == (T)1 ? x[i] : alpha * x[i];          \
@@ -616,6 +648,7 @@ ORC_AXPBY(orc_s_axpby, float)
 EXPORT void orc_d_upd_xyz(int m, double alpha, double beta, double gamma, double delta,
                           const double *x, double *y, double *z)
 {
+  #pragma omp parallel for schedule(static) if(m > 65536)
   for (int i = 0; i < m; i++) {
     double yy;
     if (alpha != 0.0 && beta != 0.0) yy = alpha * x[i] + beta * y[i];
@@ -640,6 +673,7 @@ EXPORT void orc_d_xyzw(int m, double a, double b, double c, double d, double e, 
                        const double *x, double *y, double *z, double *w)
 {
   if (a == 0.0 || b == 0.0 || c == 0.0 || d == 0.0 || e == 0.0 || f == 0.0) return;
+  #pragma omp parallel for schedule(static) if(m > 65536)
   for (int i = 0; i < m; i++) {
     y[i] = a * x[i] + b * y[i];
     z[i] = c * y[i] + d * z[i];
@@ -652,6 +686,7 @@ EXPORT void orc_d_xyzw(int m, double a, double b, double c, double d, double e, 
    evaluation order follows the reference branches. */
 EXPORT void orc_d_mlt(int n, double alpha, const double *x, const double *y, double beta, double *z)
 {
+  #pragma omp parallel for schedule(static) if(n > 65536)
   for (int i = 0; i < n; i++) {
     if (alpha == 0.0) { if (beta != 1.0) z[i] = beta * z[i]; }
     else if (alpha == 1.0) {
@@ -1275,25 +1310,26 @@ static int orc_dgcr(const orc_mat_t *a, const orc_prec_t *pc, const double *b, d
 }
 
 /* psb_drgmres_vect (krylov/psb_drgmres.f90:247-475), right DIAG
-   preconditioning, restart nl, istop 2 or 3 (the reference builds x at
-   convergence only for istop 1 and 2; here always). */
+   preconditioning, restart nl.  istop 2 (||r||/||b||, the rotated
+   least-squares residual) or 1 (normwise backward error, :387-405: at every
+   inner iteration xt = x + M^-1 V y is rebuilt from the partial
+   least-squares solve and ||b - A xt||_inf / (ani ||xt||_inf + ||b||_inf)
+   tested; ani = ||A||_inf from the caller; xt is rebuilt from zero, where the
+   reference accumulates into an uncleared xt).  At an invariant Krylov space
+   (h(i+1,i) = 0) v(i+1) is not normalised and the solve ends. */
 static int orc_drgmres(const orc_mat_t *a, const orc_prec_t *pc, const double *b, double *x,
                        double eps, int itmax, int istop, double ani, int nl,
                        int *iter, double *err, double *hist)
 {
-  (void)ani;
   int n = a->nr;
   double **v = orc_vecs(nl + 2, n), **h = orc_vecs(nl + 2, nl + 2);
   double *w = calloc(n > 0 ? n : 1, sizeof(double)), *w1 = calloc(n > 0 ? n : 1, sizeof(double));
+  double *xt = calloc(n > 0 ? n : 1, sizeof(double));
   double *c = calloc(nl + 2, sizeof(double)), *sn = calloc(nl + 2, sizeof(double));
-  double *rs = calloc(nl + 2, sizeof(double));
-  double bn2 = 0, r0n2 = 0, errnum = 0, errden = 1;
-  if (istop == 2) bn2 = orc_d_nrm2(n, b);
-  if (istop == 3) {
-    orc_d_axpby(n, 1.0, b, 0.0, v[1]);
-    orc_spmm(a, -1.0, x, 1.0, v[1]);
-    r0n2 = orc_d_nrm2(n, v[1]);
-  }
+  double *rs = calloc(nl + 2, sizeof(double)), *rst = calloc(nl + 2, sizeof(double));
+  double bn2 = 0, bni = 0, errnum = 0, errden = 1;
+  if (istop == 1) bni = orc_d_amax(n, b);
+  else bn2 = orc_d_nrm2(n, b);
   int itx = 0, done = 0;
   for (;;) { /* restart */
     orc_d_axpby(n, 1.0, b, 0.0, v[1]);
@@ -1301,8 +1337,13 @@ static int orc_drgmres(const orc_mat_t *a, const orc_prec_t *pc, const double *b
     rs[1] = orc_d_nrm2(n, v[1]);
     for (int k = 2; k < nl + 2; k++) rs[k] = 0.0;
     double scal = 1.0 / rs[1];
-    errnum = orc_d_nrm2(n, v[1]);
-    errden = istop == 2 ? bn2 : r0n2;
+    if (istop == 1) {
+      errnum = orc_d_amax(n, v[1]);
+      errden = ani * orc_d_amax(n, x) + bni;
+    } else {
+      errnum = orc_d_nrm2(n, v[1]);
+      errden = bn2;
+    }
     if (errnum <= eps * errden) break;
     for (int i = 0; i < n; i++) v[1][i] = v[1][i] * scal;
     if (itx >= itmax) break;
@@ -1316,29 +1357,47 @@ static int orc_drgmres(const orc_mat_t *a, const orc_prec_t *pc, const double *b
         orc_d_axpby(n, -h[k][i], v[k], 1.0, w);
       }
       h[i + 1][i] = orc_d_nrm2(n, w);
-      orc_d_axpby(n, 1.0 / h[i + 1][i], w, 0.0, v[i + 1]);
+      int invariant = h[i + 1][i] == 0.0;
+      if (!invariant) orc_d_axpby(n, 1.0 / h[i + 1][i], w, 0.0, v[i + 1]);
       for (int k = 2; k <= i; k++) orc_drot(&h[k - 1][i], &h[k][i], c[k - 1], sn[k - 1]);
       double rti = h[i][i], rti1 = h[i + 1][i];
       orc_drotg(&rti, &rti1, &c[i], &sn[i]);
       orc_drot(&h[i][i], &h[i + 1][i], c[i], sn[i]);
       h[i + 1][i] = 0.0;
       orc_drot(&rs[i], &rs[i + 1], c[i], sn[i]);
-      errnum = fabs(rs[i + 1]);
+      if (istop == 1) {
+        memcpy(rst, rs, sizeof(double) * (nl + 2));
+        orc_trsv_upper(i, h, rst);
+        memset(w1, 0, sizeof(double) * n);
+        for (int k = 1; k <= i; k++) orc_d_axpby(n, rst[k], v[k], 1.0, w1);
+        orc_papply(pc, w1, xt);
+        orc_d_axpby(n, 1.0, x, 1.0, xt);
+        orc_d_axpby(n, 1.0, b, 0.0, w1);
+        orc_spmm(a, -1.0, xt, 1.0, w1);
+        errnum = orc_d_amax(n, w1);
+        errden = ani * orc_d_amax(n, xt) + bni;
+      } else {
+        errnum = fabs(rs[i + 1]);
+      }
       if (hist) hist[itx] = errden > 0 ? errnum / errden : errnum;
-      if (errnum <= eps * errden) { done = 1; break; }
+      if (errnum <= eps * errden || invariant) { done = 1; break; }
     }
-    int m = done ? i : nl;
-    orc_trsv_upper(m, h, rs);
-    memset(w1, 0, sizeof(double) * n);
-    for (int k = 1; k <= m; k++) orc_d_axpby(n, rs[k], v[k], 1.0, w1);
-    orc_papply(pc, w1, w);
-    orc_d_axpby(n, 1.0, w, 1.0, x);
+    if (istop == 1) {
+      memcpy(x, xt, sizeof(double) * n);
+    } else {
+      int m = done ? i : nl;
+      orc_trsv_upper(m, h, rs);
+      memset(w1, 0, sizeof(double) * n);
+      for (int k = 1; k <= m; k++) orc_d_axpby(n, rs[k], v[k], 1.0, w1);
+      orc_papply(pc, w1, w);
+      orc_d_axpby(n, 1.0, w, 1.0, x);
+    }
     if (done) break;
   }
   *iter = itx;
   *err = errden > 0 ? errnum / errden : errnum;
   orc_free_vecs(v, nl + 2); orc_free_vecs(h, nl + 2);
-  free(w); free(w1); free(c); free(sn); free(rs);
+  free(w); free(w1); free(xt); free(c); free(sn); free(rs); free(rst);
   return 0;
 }
 

@@ -12,15 +12,15 @@ from .env import (DeviceTimer, psb_exit, psb_hip_DeviceName, psb_hip_DeviceSync,
 from .mat import (psb_d_coo_sparse_mat, psb_d_hip_csrg_sparse_mat,  # noqa: F401
                   psb_d_hip_elg_sparse_mat, psb_d_hip_hlg_sparse_mat, psb_dspmat_type,
                   psb_s_hip_csrg_sparse_mat, psb_s_hip_elg_sparse_mat, psb_s_hip_hlg_sparse_mat)
-from .vect import (psb_d_hip_vect, psb_d_vect_type, psb_i_hip_vect,  # noqa: F401
-                   psb_s_hip_vect)
+from .vect import (psb_d_base_vect, psb_d_hip_vect, psb_d_vect_type,  # noqa: F401
+                   psb_i_hip_vect, psb_s_base_vect, psb_s_hip_vect)
 from .psblas import (matrix_from_coo, psb_cdall, psb_cdasb, psb_cscnv, psb_desc_type,  # noqa: F401
                      psb_geabs, psb_geall, psb_geamax, psb_geasb, psb_geasum, psb_geaxpby,
                      psb_geaxpbyz, psb_gedot, psb_gefree, psb_gemlt, psb_genrm2, psb_gescal,
                      psb_spall, psb_spasb, psb_spfree, psb_spins, psb_spmm, psb_spsm, psb_upd_xyz,
                      psb_xyzw, vect_from)
 from .prec import psb_dprec_type, psb_precinit  # noqa: F401
-from .krylov import psb_krylov, psb_krylov_fused, psb_krylov_mixed  # noqa: F401
+from .krylov import psb_krylov, psb_krylov_fused, psb_krylov_mixed, psb_spnrmi  # noqa: F401
 from .batch import psb_hip_batch  # noqa: F401
 from .serve import (latency_summary, poisson_arrivals, psb_hip_server,  # noqa: F401
                     serve_one_at_a_time)

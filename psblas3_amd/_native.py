@@ -120,6 +120,16 @@ _SIGS = {
     "psb_hip_hll_read": (i32, [vp, vp, vp, vp, vp, vp]),
     "psb_hip_hll_get_params": (i32, [vp, pi32, pi32, pi32, pi32, pi32, pi32, pi32]),
     "psb_hip_hll_copy_coo": (i32, [i32, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp]),
+    "psb_hip_hll_sort_rows": (i32, [i32, i32, i32, vp, vp, vp, vp, pi64]),
+    "psb_hip_hll_copy_coo_sorted": (i32, [i32, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp,
+                                          vp]),
+    "psb_hip_hll_read_perm": (i32, [vp, vp]),
+    "psb_hip_hll_units": (i32, [vp, pi32, pi32]),
+    "psb_hip_hll_set_perm": (i32, [vp, vp, vp]),
+    "psb_hip_host_hll_spmv": (i32, [i32, i32, vp, vp, vp, vp, vp, i32, f64, vp, f64, vp]),
+    "psb_hip_host_ell_spmv": (i32, [i32, i32, vp, vp, vp, i32, f64, vp, f64, vp]),
+    "psb_hip_host_csr_spmv": (i32, [i32, vp, vp, vp, i32, f64, vp, f64, vp]),
+    "psb_hip_host_hll_fill": (i32, [i32, i32, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp]),
     "psb_hip_hll_spmv": (i32, [vp, f64, vp, f64, vp]),
     "psb_hip_hll_make_single": (i32, [vp]),
     "psb_hip_hll_spmv_single": (i32, [vp, f64, vp, f64, vp]),

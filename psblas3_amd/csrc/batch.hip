@@ -935,6 +935,8 @@ int psb_hip_batch_register(void* batch, void* dmat, void* diag, int* mat_id) {
   if (!B || !h || !mat_id) return PSB_HIP_INVALID;
   if (h->elem_type != PSB_HIP_TYPE_DOUBLE || h->rows < 1 || h->max_col > h->rows)
     return set_error(PSB_HIP_INVALID, __func__, "needs a square double HLL matrix");
+  if (h->perm)
+    return set_error(PSB_HIP_UNSUPPORTED, __func__, "row-sorted HLL: rows must be in order");
   if (!h->val32) {
     int rc = psb_hip_hll_make_single(h);
     if (rc) return rc;

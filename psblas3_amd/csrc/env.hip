@@ -199,8 +199,10 @@ int psb_hip_get_device_name(char* buf, int len) {
   if (rc) return rc;
   hipDeviceProp_t prop;
   PSB_HIP_CHECK(hipGetDeviceProperties(&prop, g_ctx.device));
+  // the marketing name can be empty; the architecture name (gfx950...) is not
+  const char* name = prop.name[0] ? prop.name : prop.gcnArchName;
   if (len > 0) {
-    strncpy(buf, prop.name, (size_t)len - 1);
+    strncpy(buf, name, (size_t)len - 1);
     buf[len - 1] = 0;
   }
   return PSB_HIP_SUCCESS;

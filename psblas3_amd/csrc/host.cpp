@@ -136,9 +136,179 @@ int ilu0_fact(int n, const int* irp, const int* ja, const T* val, int* lirp, int
   return PSB_HIP_SUCCESS;
 }
 
+// ---------------------------------------------------------------- host copies
+// The CPU side of the dual-memory plug-ins: the matrix formats' own products
+// (the csmv the GPU types inherit from the CPU HLL / ELL / CSR types) on the
+// host copy, used when an operand is not a device vector
+// (cuda/impl/psb_d_cuda_hlg_vect_mv.F90:77-108), and the host fills of the
+// HLL / ELL layouts.  Each row sums its own entries in stored order with
+// separately rounded products and sums (this file is compiled without
+// contraction), then y = alpha*acc + beta*y (y not read when beta = 0; y =
+// beta*y when alpha = 0), as psb_d_hll_csmv (ext/impl/psb_d_hll_csmv.f90:
+// 291-371), psb_d_ell_csmv and psb_d_csr_csmv
+// (base/serial/impl/psb_d_csr_impl.F90:46-417) compute them.
+
+template <typename T>
+inline void host_row_out(T* y, int64_t row, T alpha, T acc, T beta) {
+  if (alpha == T(0)) y[row] = beta == T(0) ? T(0) : beta * y[row];
+  else if (beta == T(0)) y[row] = alpha * acc;
+  else y[row] = alpha * acc + beta * y[row];
+}
+
+template <typename T>
+void host_hll_spmv(int nr, int hksz, const int* hkoffs, const int* irn, const int* ja,
+                   const T* val, const int* perm, T alpha, const T* x, T beta, T* y) {
+  const int noffs = (nr + hksz - 1) / hksz;
+  parallel_for(noffs, [&](int64_t lo, int64_t hi) {
+    for (int64_t h = lo; h < hi; h++) {
+      const int64_t p0 = h * hksz, p1 = std::min<int64_t>(nr, p0 + hksz);
+      for (int64_t p = p0; p < p1; p++) {
+        const int64_t base = hkoffs[h] + (p - p0);
+        T acc = T(0);
+        for (int j = 0; j < irn[p]; j++) {
+          const int64_t s = base + (int64_t)j * hksz;
+          acc = acc + val[s] * x[ja[s] - 1];
+        }
+        host_row_out(y, perm ? perm[p] - 1 : p, alpha, acc, beta);
+      }
+    }
+  });
+}
+
+template <typename T>
+void host_ell_spmv(int nr, int ld, const int* irn, const int* ja, const T* val, T alpha,
+                   const T* x, T beta, T* y) {
+  parallel_for(nr, [&](int64_t lo, int64_t hi) {
+    for (int64_t i = lo; i < hi; i++) {
+      T acc = T(0);
+      for (int j = 0; j < irn[i]; j++) {
+        const int64_t s = i + (int64_t)j * ld;
+        acc = acc + val[s] * x[ja[s] - 1];
+      }
+      host_row_out(y, i, alpha, acc, beta);
+    }
+  });
+}
+
+template <typename T>
+void host_csr_spmv(int nr, const int* irp, const int* ja, const T* val, T alpha, const T* x,
+                   T beta, T* y) {
+  parallel_for(nr, [&](int64_t lo, int64_t hi) {
+    for (int64_t i = lo; i < hi; i++) {
+      T acc = T(0);
+      for (int64_t k = irp[i] - 1; k < irp[i + 1] - 1; k++) acc = acc + val[k] * x[ja[k] - 1];
+      host_row_out(y, i, alpha, acc, beta);
+    }
+  });
+}
+
+// HLL layout on the host from a row-sorted COO: the rule of the device fill
+// (CPU padding: a row's last column, an empty row the hack's first entry,
+// positions past nr ja = nr, val = 0; position p holds row perm[p] when perm
+// is given).
+template <typename T>
+void host_hll_fill(int nr, int hksz, const int* irn, const int* hkoffs, const int* idisp,
+                   const int* cja, const T* cval, const int* perm, int* ja, T* val, int* idiag) {
+  const int noffs = (nr + hksz - 1) / hksz;
+  parallel_for(noffs, [&](int64_t lo, int64_t hi) {
+    for (int64_t h = lo; h < hi; h++) {
+      const int off = hkoffs[h];
+      const int w = (hkoffs[h + 1] - off) / hksz;
+      for (int lane = 0; lane < hksz; lane++) {
+        const int64_t i = h * hksz + lane;
+        if (i >= nr) {
+          for (int j = 0; j < w; j++) {
+            ja[off + (int64_t)j * hksz + lane] = nr;
+            val[off + (int64_t)j * hksz + lane] = T(0);
+          }
+          continue;
+        }
+        const int row = perm ? perm[i] - 1 : (int)i;
+        const int rs = irn[i];
+        const int p0 = idisp[row];
+        int dg = 0, ic = 0;
+        for (int j = 0; j < rs; j++) {
+          ic = cja[p0 + j];
+          if (ic == row + 1) dg = j + 1;
+          ja[off + (int64_t)j * hksz + lane] = ic;
+          val[off + (int64_t)j * hksz + lane] = cval[p0 + j];
+        }
+        if (rs == 0 && w > 0) {
+          const int q = idisp[perm ? perm[h * hksz] - 1 : h * hksz];
+          ic = q < idisp[nr] ? cja[q] : nr;
+        }
+        for (int j = rs; j < w; j++) {
+          ja[off + (int64_t)j * hksz + lane] = ic;
+          val[off + (int64_t)j * hksz + lane] = T(0);
+        }
+        idiag[i] = dg;
+      }
+    }
+  });
+}
+
 }  // namespace
 
 extern "C" {
+
+int psb_hip_host_hll_spmv(int nr, int hksz, const int* hkoffs, const int* irn, const int* ja,
+                          const void* val, const int* perm, int elem_type, double alpha,
+                          const void* x, double beta, void* y) {
+  if (nr < 0 || hksz < 1 || (nr > 0 && (!hkoffs || !irn || !x || !y))) return PSB_HIP_INVALID;
+  if (elem_type == PSB_HIP_TYPE_DOUBLE)
+    host_hll_spmv<double>(nr, hksz, hkoffs, irn, ja, (const double*)val, perm, alpha,
+                          (const double*)x, beta, (double*)y);
+  else if (elem_type == PSB_HIP_TYPE_FLOAT)
+    host_hll_spmv<float>(nr, hksz, hkoffs, irn, ja, (const float*)val, perm, (float)alpha,
+                         (const float*)x, (float)beta, (float*)y);
+  else
+    return PSB_HIP_UNSUPPORTED;
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_host_ell_spmv(int nr, int ld, const int* irn, const int* ja, const void* val,
+                          int elem_type, double alpha, const void* x, double beta, void* y) {
+  if (nr < 0 || ld < nr || (nr > 0 && (!irn || !x || !y))) return PSB_HIP_INVALID;
+  if (elem_type == PSB_HIP_TYPE_DOUBLE)
+    host_ell_spmv<double>(nr, ld, irn, ja, (const double*)val, alpha, (const double*)x, beta,
+                          (double*)y);
+  else if (elem_type == PSB_HIP_TYPE_FLOAT)
+    host_ell_spmv<float>(nr, ld, irn, ja, (const float*)val, (float)alpha, (const float*)x,
+                         (float)beta, (float*)y);
+  else
+    return PSB_HIP_UNSUPPORTED;
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_host_csr_spmv(int nr, const int* irp, const int* ja, const void* val, int elem_type,
+                          double alpha, const void* x, double beta, void* y) {
+  if (nr < 0 || (nr > 0 && (!irp || !x || !y))) return PSB_HIP_INVALID;
+  if (elem_type == PSB_HIP_TYPE_DOUBLE)
+    host_csr_spmv<double>(nr, irp, ja, (const double*)val, alpha, (const double*)x, beta,
+                          (double*)y);
+  else if (elem_type == PSB_HIP_TYPE_FLOAT)
+    host_csr_spmv<float>(nr, irp, ja, (const float*)val, (float)alpha, (const float*)x,
+                         (float)beta, (float*)y);
+  else
+    return PSB_HIP_UNSUPPORTED;
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_host_hll_fill(int nr, int hksz, const int* irn, const int* hkoffs, const int* idisp,
+                          const int* cja, const void* cval, const int* perm, int elem_type,
+                          int* ja, void* val, int* idiag) {
+  if (nr < 0 || hksz < 1) return PSB_HIP_INVALID;
+  if (elem_type == PSB_HIP_TYPE_DOUBLE)
+    host_hll_fill<double>(nr, hksz, irn, hkoffs, idisp, cja, (const double*)cval, perm, ja,
+                          (double*)val, idiag);
+  else if (elem_type == PSB_HIP_TYPE_FLOAT)
+    host_hll_fill<float>(nr, hksz, irn, hkoffs, idisp, cja, (const float*)cval, perm, ja,
+                         (float*)val, idiag);
+  else
+    return PSB_HIP_UNSUPPORTED;
+  return PSB_HIP_SUCCESS;
+}
+
 
 int psb_hip_fix_coo_d(int nr, int nc, int64_t nz, int* ia, int* ja, double* val, int dupl,
                       int64_t* nzout) {
@@ -254,6 +424,43 @@ int psb_hip_hll_offsets(int nr, int64_t nz, const int* ia, int hksz, int* irn, i
     int i1 = std::min(nr, i0 + hksz);
     int mx = 0;
     for (int i = i0; i < i1; i++) mx = std::max(mx, irn[i]);
+    off += (int64_t)mx * hksz;
+    if (off > 0x7fffffffLL) return PSB_HIP_UNSUPPORTED;
+    hkoffs[h + 1] = (int)off;
+  }
+  *isz = off;
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_hll_sort_rows(int nr, int hksz, int window, const int* irn, int* perm, int* irn_pos,
+                          int* hkoffs, int64_t* isz) {
+  if (hksz < 1 || nr < 0 || !irn || !perm || !irn_pos || !hkoffs || !isz) return PSB_HIP_INVALID;
+  int64_t win = window <= 0 ? (int64_t)nr : ((int64_t)window + hksz - 1) / hksz * hksz;
+  if (win < 1) win = 1;
+  for (int i = 0; i < nr; i++) {
+    if (irn[i] < 0) return PSB_HIP_INVALID;
+    perm[i] = i;
+  }
+  // descending length inside each window, ties in ascending row order
+  const int64_t nwin = ((int64_t)nr + win - 1) / win;
+  parallel_for(nwin, [&](int64_t lo, int64_t hi) {
+    for (int64_t w = lo; w < hi; w++) {
+      int* b = perm + w * win;
+      int* e = perm + std::min<int64_t>((w + 1) * win, nr);
+      std::stable_sort(b, e, [&](int a, int c) { return irn[a] > irn[c]; });
+    }
+  });
+  for (int p = 0; p < nr; p++) {
+    irn_pos[p] = irn[perm[p]];
+    perm[p] += 1;  // 1-based row of position p
+  }
+  const int noffs = (nr + hksz - 1) / hksz;
+  int64_t off = 0;
+  hkoffs[0] = 0;
+  for (int h = 0; h < noffs; h++) {
+    const int p0 = h * hksz, p1 = std::min(nr, p0 + hksz);
+    int mx = 0;
+    for (int p = p0; p < p1; p++) mx = std::max(mx, irn_pos[p]);
     off += (int64_t)mx * hksz;
     if (off > 0x7fffffffLL) return PSB_HIP_UNSUPPORTED;
     hkoffs[h + 1] = (int)off;

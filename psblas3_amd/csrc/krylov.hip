@@ -889,6 +889,9 @@ static int setup(Solver& s, int method, int fmt, void* dmat, int itmax) {
   s.stream = ctx().stream;
   if (fmt == FMT_HLL) {
     HllDev* h = (HllDev*)dmat;
+    if (h->perm)
+      return set_error(PSB_HIP_UNSUPPORTED, "krylov",
+                       "row-sorted HLL: the fused solvers take rows in order");
     s.n = h->rows;
     s.etype = h->elem_type;
   } else if (fmt == FMT_ELL) {
@@ -1148,6 +1151,8 @@ int psb_hip_krylov_mixed(int method, void* dmat, void* diag, void* bv, void* xv,
   HllDev* h = (HllDev*)dmat;
   if (h->elem_type != PSB_HIP_TYPE_DOUBLE || !h->val32)
     return set_error(PSB_HIP_INVALID, __func__, "needs a double HLL matrix with a single copy");
+  if (h->perm)
+    return set_error(PSB_HIP_UNSUPPORTED, __func__, "row-sorted HLL: rows must be in order");
   const int n = h->rows;
   MultiVec *b = (MultiVec*)bv, *x = (MultiVec*)xv, *d = (MultiVec*)diag;
   if (mv_check(b, PSB_HIP_TYPE_DOUBLE, n) || mv_check(x, PSB_HIP_TYPE_DOUBLE, n) ||

@@ -298,7 +298,21 @@ struct HllDev {
   int* idiag;           // rows
   float* val32;         // optional FP32 copy of a double matrix
   int variant;          // SpMV kernel variant chosen for this matrix (0: the global one)
+  // Row-sorted storage (psb_hip_hll_copy_coo_sorted): position p of the hack
+  // layout holds row perm[p] (1-based); irn and idiag are then per position.
+  // The product runs work units (hack, first slot, end slot, partial slot or
+  // -1): a hack wider than kHllUnitSlots is cut into several units whose
+  // per-lane partial sums a second kernel adds in unit order (splits:
+  // hack, first partial slot, unit count).  null perm: rows in order.
+  int* perm = nullptr;
+  int nunits = 0;
+  int4* units = nullptr;
+  int nsplit = 0;
+  int4* splits = nullptr;
+  void* part = nullptr;   // partial sums, (split units) x hksz entries of 8 bytes
 };
+// slots of one work unit of the row-sorted HLL product
+constexpr int kHllUnitSlots = 256;
 
 struct EllDev {
   int elem_type;

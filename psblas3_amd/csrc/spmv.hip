@@ -113,6 +113,62 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmv_pair(int nr, int hksz, int 
   if (row + 1 < nr) store_y(y, row + 1, alpha, a1, beta);
 }
 
+// Row-sorted HLL product (the permuted SPGPU kernel,
+// cuda/spgpu/kernels/hell_spmv_base_template.cuh:227-252, which loops each
+// row to its own length, :197-214): position p of hack h holds row perm[p]
+// and irn[p] entries.  Threads take hksz-lane work units (hack, first slot,
+// end slot, partial slot): lane p sums val * x over its row's slots inside
+// the unit in slot order (slot_dot: loads in groups issued before the
+// gathers), then writes y(perm[p]) = alpha*acc + beta*y, or, for a unit of a
+// hack cut into several, its partial sum into part.  Padding slots past a
+// row's length are not read.
+template <typename T, typename TV>
+__global__ __launch_bounds__(kBlock) void k_hll_spmv_units(int nr, int hksz, int nunits,
+                                                           const int4* __restrict__ units,
+                                                           const int* __restrict__ hkoffs,
+                                                           const int* __restrict__ irn,
+                                                           const int* __restrict__ perm,
+                                                           const int* __restrict__ ja,
+                                                           const TV* __restrict__ val,
+                                                           const T* __restrict__ x, T alpha,
+                                                           T beta, T* __restrict__ y,
+                                                           T* __restrict__ part) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int u = (int)(t / hksz);
+  if (u >= nunits) return;
+  const int lane = (int)(t - (int64_t)u * hksz);
+  const int4 un = units[u];
+  const int p = un.x * hksz + lane;
+  const int len = p < nr ? irn[p] : 0;
+  const int e = len < un.z ? len : un.z;
+  const int w = e > un.y ? e - un.y : 0;
+  const size_t off = (size_t)hkoffs[un.x] + (size_t)un.y * hksz + lane;
+  const T acc = slot_dot<T, TV>(ja + off, val + off, hksz, w, x);
+  if (un.w >= 0) part[(size_t)un.w * hksz + lane] = acc;
+  else if (p < nr) store_y(y, perm[p] - 1, alpha, acc, beta);
+}
+
+// The hacks cut into several units: lane p adds its units' partial sums in
+// unit (slot) order and writes y(perm[p]).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_hll_units_combine(int nr, int hksz, int nsplit,
+                                                              const int4* __restrict__ splits,
+                                                              const int* __restrict__ perm,
+                                                              const T* __restrict__ part,
+                                                              T alpha, T beta,
+                                                              T* __restrict__ y) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int s = (int)(t / hksz);
+  if (s >= nsplit) return;
+  const int lane = (int)(t - (int64_t)s * hksz);
+  const int4 sp = splits[s];
+  const int p = sp.x * hksz + lane;
+  if (p >= nr) return;
+  T acc = part[(size_t)sp.y * hksz + lane];
+  for (int k = 1; k < sp.z; k++) acc += part[(size_t)(sp.y + k) * hksz + lane];
+  store_y(y, perm[p] - 1, alpha, acc, beta);
+}
+
 // Multi-RHS HLL product (psb_d_hll_csmm, ext/impl/psb_d_hll_csmm.f90:33-235):
 // one row per lane and up to NC columns of x/y (column-major, ldx/ldy
 // elements apart) per launch.  Slots go in groups of 4 whose (ja, val) loads
@@ -365,15 +421,17 @@ __global__ __launch_bounds__(kBlock) void k_hll_scal(int rows, int hksz,
                                                      const int* __restrict__ hkoffs,
                                                      const int* __restrict__ irn,
                                                      const int* __restrict__ ja,
+                                                     const int* __restrict__ perm,
                                                      const T* __restrict__ d, int left,
                                                      T* __restrict__ val) {
-  const int r = blockIdx.x * kBlock + threadIdx.x;
+  const int r = blockIdx.x * kBlock + threadIdx.x;  // position (row when perm is null)
   if (r >= rows) return;
   const int hk = r / hksz;
   const size_t off = (size_t)hkoffs[hk] + (r - hk * hksz);
+  const T dl = left ? d[perm ? perm[r] - 1 : r] : T(0);
   for (int j = 0; j < irn[r]; j++) {
     const size_t p = off + (size_t)j * hksz;
-    val[p] = val[p] * (left ? d[r] : d[ja[p] - 1]);
+    val[p] = val[p] * (left ? dl : d[ja[p] - 1]);
   }
 }
 
@@ -455,6 +513,8 @@ __global__ __launch_bounds__(kBlock) void k_tmv_run(int rows, const int* __restr
 // psi_d_convert_hll_from_coo + psi_d_xtr_ell_from_coo, CPU padding rule:
 // one thread per slot row; rows past nr in the last hack get ja = nr, val 0;
 // an empty row is padded with the first column of its hack's entries.
+// perm (row-sorted storage, null for rows in order): position i holds row
+// perm[i] (1-based), irn is per position, idisp per row.
 template <typename TV>
 __global__ __launch_bounds__(kBlock) void k_hll_fill(int nr, int hksz, int noffs,
                                                      const int* __restrict__ irn,
@@ -462,13 +522,14 @@ __global__ __launch_bounds__(kBlock) void k_hll_fill(int nr, int hksz, int noffs
                                                      const int* __restrict__ idisp,
                                                      const int* __restrict__ cja,
                                                      const TV* __restrict__ cval,
+                                                     const int* __restrict__ perm,
                                                      int* __restrict__ ja, TV* __restrict__ val,
                                                      int* __restrict__ idiag) {
   const int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x;
   if (t >= (int64_t)noffs * hksz) return;
   const int hk = (int)(t / hksz);
   const int lane = (int)(t - (int64_t)hk * hksz);
-  const int i = hk * hksz + lane;  // 0-based row
+  const int i = hk * hksz + lane;  // 0-based position (the row when perm is null)
   const int off = hoffs[hk];
   const int w = (hoffs[hk + 1] - off) / hksz;
   if (i >= nr) {
@@ -478,17 +539,23 @@ __global__ __launch_bounds__(kBlock) void k_hll_fill(int nr, int hksz, int noffs
     }
     return;
   }
+  const int row = perm ? perm[i] - 1 : i;
   const int rs = irn[i];
-  const int p0 = idisp[i];
+  const int p0 = idisp[row];
   int dg = 0;
   int ic = 0;
   for (int j = 0; j < rs; j++) {
     ic = cja[p0 + j];
-    if (ic == i + 1) dg = j + 1;
+    if (ic == row + 1) dg = j + 1;
     ja[off + (size_t)j * hksz + lane] = ic;
     val[off + (size_t)j * hksz + lane] = cval[p0 + j];
   }
-  if (rs == 0 && w > 0) ic = cja[idisp[hk * hksz]];
+  if (rs == 0 && w > 0) {
+    // the first entry of the hack's first row (in bounds: some row of the
+    // hack has entries); a sorted layout has its longest row first
+    const int q = idisp[perm ? perm[hk * hksz] - 1 : hk * hksz];
+    ic = q < idisp[nr] ? cja[q] : nr;
+  }
   for (int j = rs; j < w; j++) {
     ja[off + (size_t)j * hksz + lane] = ic;
     val[off + (size_t)j * hksz + lane] = TV(0);
@@ -531,16 +598,18 @@ __global__ __launch_bounds__(kBlock) void k_ell_fill(int nr, int pitch, int nzm,
 }
 
 // diagonal extraction: d[i] = val(slot idiag[i]) or 0 when the row has none
+// (row-sorted storage: position i holds row perm[i])
 template <typename TV>
 __global__ __launch_bounds__(kBlock) void k_hll_diag(int nr, int hksz,
                                                      const int* __restrict__ hkoffs,
                                                      const int* __restrict__ idiag,
+                                                     const int* __restrict__ perm,
                                                      const TV* __restrict__ val, TV* __restrict__ d) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= nr) return;
   const int hk = i / hksz, lane = i - hk * hksz;
   const int s = idiag[i];
-  d[i] = s > 0 ? val[hkoffs[hk] + (size_t)(s - 1) * hksz + lane] : TV(0);
+  d[perm ? perm[i] - 1 : i] = s > 0 ? val[hkoffs[hk] + (size_t)(s - 1) * hksz + lane] : TV(0);
 }
 
 template <typename TV>
@@ -636,8 +705,31 @@ static void hll_launch_hk(const HllDev* m, bool pair, bool xcd, const TV* val, c
 // chooses (pairs for 8-byte values, single rows for 4-byte values, both in
 // XCD order); adding 4 deals the blocks to the XCDs in chunks of
 // g_hll_xchunk blocks (xcd_block_chunk) instead of launch order.
+// Row-sorted storage: the work units, then (when some hack is cut into
+// several units) the partial-sum combination.
+template <typename T, typename TV>
+static int hll_units_launch(const HllDev* m, const TV* val, const T* x, T alpha, T beta, T* y) {
+  hipStream_t s = ctx().stream;
+  if (m->nunits > 0) {
+    dim3 grid(grid_for((int64_t)m->nunits * m->hksz, kBlock));
+    hipLaunchKernelGGL((k_hll_spmv_units<T, TV>), grid, dim3(kBlock), 0, s, m->rows, m->hksz,
+                       m->nunits, (const int4*)m->units, (const int*)m->hkoffs,
+                       (const int*)m->irn, (const int*)m->perm, (const int*)m->ja, val, x, alpha,
+                       beta, y, (T*)m->part);
+  }
+  if (m->nsplit > 0) {
+    dim3 grid(grid_for((int64_t)m->nsplit * m->hksz, kBlock));
+    hipLaunchKernelGGL((k_hll_units_combine<T>), grid, dim3(kBlock), 0, s, m->rows, m->hksz,
+                       m->nsplit, (const int4*)m->splits, (const int*)m->perm,
+                       (const T*)m->part, alpha, beta, y);
+  }
+  PSB_HIP_LAUNCH_CHECK();
+  return PSB_HIP_SUCCESS;
+}
+
 template <typename T, typename TV>
 static int hll_launch(const HllDev* m, const TV* val, const T* x, T alpha, T beta, T* y) {
+  if (m->perm) return hll_units_launch<T, TV>(m, val, x, alpha, beta, y);
   const int v = m->variant != 0 ? m->variant : g_hll_variant;
   const int variant = v & 3;
   const bool xcd = (v & 4) != 0 || v == 0;
@@ -653,6 +745,15 @@ static int hll_launch(const HllDev* m, const TV* val, const T* x, T alpha, T bet
 template <typename T>
 static int hll_spmm_launch(const HllDev* m, const T* val, const T* x, int ldx, T alpha, T beta,
                            T* y, int ldy, int ncols) {
+  if (m->perm) {
+    // row-sorted storage: one product per column (the matrix is read k times)
+    for (int c = 0; c < ncols; c++) {
+      int rc = hll_units_launch<T, T>(m, val, x + (size_t)c * ldx, alpha, beta,
+                                      y + (size_t)c * ldy);
+      if (rc) return rc;
+    }
+    return PSB_HIP_SUCCESS;
+  }
   hipStream_t s = ctx().stream;
   // XCD block order, in chunks of g_hll_xchunk blocks: one contiguous range
   // per XCD measured 2-6% faster than launch order for k = 8, 16 and 32
@@ -964,6 +1065,76 @@ static int csr_build_slices(CsrDev* m, const int* irp, const int* ja, const void
   return PSB_HIP_SUCCESS;
 }
 
+// Back to rows in order (the arrays written next are in the reference's
+// layout): drop the permutation and the work-unit plan.
+static void hll_drop_sorted(HllDev* m) {
+  hipFree(m->perm);
+  hipFree(m->units);
+  hipFree(m->splits);
+  hipFree(m->part);
+  m->perm = nullptr;
+  m->units = nullptr;
+  m->splits = nullptr;
+  m->part = nullptr;
+  m->nunits = m->nsplit = 0;
+}
+
+// Row-sorted storage: copy perm (1-based row of each position, a bijection
+// the caller has checked) and build the work-unit plan from the host hack
+// offsets: hacks wider than kHllUnitSlots are cut, their units first (the
+// widest hacks' units lead the launch), then every other hack in order.
+static int hll_install_perm(HllDev* m, const int* perm, const int* hoffs) {
+  const int hksz = m->hksz, noffs = m->noffs, nr = m->rows;
+  std::vector<int4> units, splits;
+  int npart = 0;
+  for (int h = 0; h < noffs; h++) {
+    const int w = (hoffs[h + 1] - hoffs[h]) / hksz;
+    if (w <= kHllUnitSlots) continue;
+    const int nu = (w + kHllUnitSlots - 1) / kHllUnitSlots;
+    splits.push_back(make_int4(h, npart, nu, 0));
+    for (int k = 0; k < nu; k++)
+      units.push_back(make_int4(h, k * kHllUnitSlots, std::min(w, (k + 1) * kHllUnitSlots),
+                                npart + k));
+    npart += nu;
+  }
+  for (int h = 0; h < noffs; h++) {
+    const int w = (hoffs[h + 1] - hoffs[h]) / hksz;
+    if (w <= kHllUnitSlots) units.push_back(make_int4(h, 0, w, -1));
+  }
+  hipStream_t s = ctx().stream;
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = hipMalloc((void**)&m->perm, sizeof(int) * (nr > 0 ? nr : 1));
+  if (e == hipSuccess) e = hipMalloc((void**)&m->units, sizeof(int4) * (units.size() + 1));
+  if (e == hipSuccess) e = hipMalloc((void**)&m->splits, sizeof(int4) * (splits.size() + 1));
+  if (e == hipSuccess) e = hipMalloc(&m->part, (size_t)8 * hksz * (size_t)(npart > 0 ? npart : 1));
+  if (e != hipSuccess) {
+    hll_drop_sorted(m);
+    return set_error(PSB_HIP_OUTOFMEM, "hll_install_perm", hipGetErrorString(e));
+  }
+  if (nr > 0)
+    PSB_HIP_CHECK(hipMemcpyAsync(m->perm, perm, sizeof(int) * nr, hipMemcpyHostToDevice, s));
+  if (!units.empty())
+    PSB_HIP_CHECK(hipMemcpyAsync(m->units, units.data(), sizeof(int4) * units.size(),
+                                 hipMemcpyHostToDevice, s));
+  if (!splits.empty())
+    PSB_HIP_CHECK(hipMemcpyAsync(m->splits, splits.data(), sizeof(int4) * splits.size(),
+                                 hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipStreamSynchronize(s));   // the vectors above are freed on return
+  m->nunits = (int)units.size();
+  m->nsplit = (int)splits.size();
+  return PSB_HIP_SUCCESS;
+}
+
+static int check_perm(const int* perm, int nr) {
+  std::vector<char> seen((size_t)(nr > 0 ? nr : 1), 0);
+  for (int p = 0; p < nr; p++) {
+    const int r = perm[p];
+    if (r < 1 || r > nr || seen[r - 1]) return PSB_HIP_INVALID;
+    seen[r - 1] = 1;
+  }
+  return PSB_HIP_SUCCESS;
+}
+
 }  // namespace psbhip
 
 extern "C" {
@@ -1012,6 +1183,10 @@ void psb_hip_hll_free(void* dmat) {
   hipFree(m->irn);
   hipFree(m->idiag);
   hipFree(m->val32);
+  hipFree(m->perm);
+  hipFree(m->units);
+  hipFree(m->splits);
+  hipFree(m->part);
   delete m;
 }
 
@@ -1020,6 +1195,8 @@ int psb_hip_hll_write(void* dmat, const void* val, const int* ja, const int* hko
   PSB_ENSURE_INIT();
   HllDev* m = (HllDev*)dmat;
   hipStream_t s = ctx().stream;
+  PSB_HIP_CHECK(hipStreamSynchronize(s));
+  hll_drop_sorted(m);
   // the hack offsets must describe the allocation exactly
   if (hkoffs[0] != 0 || hkoffs[m->noffs] != m->allocsize)
     return set_error(PSB_HIP_INVALID, __func__, "hkoffs inconsistent with allocsize");
@@ -1098,6 +1275,8 @@ int psb_hip_hll_copy_coo(int nr, int nc, int nza, int hksz, int noffs, int isz, 
   if (nza > 0 && (mn < 1 || mx > nc)) return set_error(PSB_HIP_INVALID, __func__, "ja out of range");
   m->max_col = std::max(mx, nr);  // padding of the last hack uses ja = nr
   hipStream_t s = ctx().stream;
+  PSB_HIP_CHECK(hipStreamSynchronize(s));
+  hll_drop_sorted(m);
   size_t es = elem_size(m->elem_type);
   int *d_irn = nullptr, *d_hoffs = nullptr, *d_idisp = nullptr, *d_ja = nullptr;
   void* d_val = nullptr;
@@ -1125,17 +1304,125 @@ int psb_hip_hll_copy_coo(int nr, int nc, int nza, int hksz, int noffs, int isz, 
     dim3 grid((unsigned)((nthreads + kBlock - 1) / kBlock));
     if (m->elem_type == PSB_HIP_TYPE_DOUBLE)
       hipLaunchKernelGGL((k_hll_fill<double>), grid, dim3(kBlock), 0, s, nr, hksz, noffs, d_irn,
-                         d_hoffs, d_idisp, d_ja, (const double*)d_val, m->ja, (double*)m->val,
-                         m->idiag);
+                         d_hoffs, d_idisp, d_ja, (const double*)d_val, (const int*)nullptr,
+                         m->ja, (double*)m->val, m->idiag);
     else
       hipLaunchKernelGGL((k_hll_fill<float>), grid, dim3(kBlock), 0, s, nr, hksz, noffs, d_irn,
-                         d_hoffs, d_idisp, d_ja, (const float*)d_val, m->ja, (float*)m->val,
-                         m->idiag);
+                         d_hoffs, d_idisp, d_ja, (const float*)d_val, (const int*)nullptr,
+                         m->ja, (float*)m->val, m->idiag);
     PSB_HIP_LAUNCH_CHECK();
   }
   PSB_HIP_CHECK(hipStreamSynchronize(s));
   hipFree(d_irn); hipFree(d_hoffs); hipFree(d_idisp); hipFree(d_ja); hipFree(d_val);
   m->nzeros = nza;
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_hll_copy_coo_sorted(int nr, int nc, int nza, int hksz, int noffs, int isz,
+                                const int* perm, const int* irn_pos, const int* hoffs,
+                                const int* idisp, const int* ja, const void* val, void* dmat) {
+  PSB_ENSURE_INIT();
+  HllDev* m = (HllDev*)dmat;
+  if (!m || !perm || !irn_pos || !hoffs || !idisp) return PSB_HIP_INVALID;
+  if (m->rows != nr || m->hksz != hksz || m->noffs != noffs || m->allocsize != isz)
+    return set_error(PSB_HIP_INVALID, __func__, "matrix shape mismatch");
+  if (hoffs[0] != 0 || hoffs[noffs] != isz || idisp[0] != 0 || idisp[nr] != nza)
+    return set_error(PSB_HIP_INVALID, __func__, "offsets inconsistent");
+  if (check_perm(perm, nr))
+    return set_error(PSB_HIP_INVALID, __func__, "perm is not a permutation of 1..nr");
+  for (int p = 0; p < nr; p++)
+    if (idisp[perm[p]] - idisp[perm[p] - 1] != irn_pos[p])
+      return set_error(PSB_HIP_INVALID, __func__, "irn_pos does not match the rows' counts");
+  for (int h = 0; h < noffs; h++) {
+    int p0 = h * hksz, p1 = std::min(nr, p0 + hksz), mx = 0;
+    for (int p = p0; p < p1; p++) mx = std::max(mx, irn_pos[p]);
+    if (hoffs[h + 1] - hoffs[h] != mx * hksz)
+      return set_error(PSB_HIP_INVALID, __func__, "hack offsets do not match row counts");
+  }
+  int mn = min_index(ja, nza), mx = max_index(ja, nza);
+  if (nza > 0 && (mn < 1 || mx > nc)) return set_error(PSB_HIP_INVALID, __func__, "ja out of range");
+  hipStream_t s = ctx().stream;
+  PSB_HIP_CHECK(hipStreamSynchronize(s));
+  hll_drop_sorted(m);
+  m->max_col = std::max(mx, nr);
+  size_t es = elem_size(m->elem_type);
+  int *d_irn = nullptr, *d_hoffs = nullptr, *d_idisp = nullptr, *d_ja = nullptr;
+  void* d_val = nullptr;
+  hipError_t e = hipSuccess;
+  size_t nzb = (size_t)(nza > 0 ? nza : 1);
+  if (e == hipSuccess) e = hipMalloc(&d_irn, sizeof(int) * (nr > 0 ? nr : 1));
+  if (e == hipSuccess) e = hipMalloc(&d_hoffs, sizeof(int) * (noffs + 1));
+  if (e == hipSuccess) e = hipMalloc(&d_idisp, sizeof(int) * (nr + 1));
+  if (e == hipSuccess) e = hipMalloc(&d_ja, sizeof(int) * nzb);
+  if (e == hipSuccess) e = hipMalloc(&d_val, es * nzb);
+  if (e != hipSuccess) {
+    hipFree(d_irn); hipFree(d_hoffs); hipFree(d_idisp); hipFree(d_ja); hipFree(d_val);
+    return set_error(PSB_HIP_OUTOFMEM, __func__, hipGetErrorString(e));
+  }
+  int rc = hll_install_perm(m, perm, hoffs);
+  if (rc) {
+    hipFree(d_irn); hipFree(d_hoffs); hipFree(d_idisp); hipFree(d_ja); hipFree(d_val);
+    return rc;
+  }
+  PSB_HIP_CHECK(hipMemcpyAsync(d_irn, irn_pos, sizeof(int) * nr, hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(d_hoffs, hoffs, sizeof(int) * (noffs + 1), hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(d_idisp, idisp, sizeof(int) * (nr + 1), hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(d_ja, ja, sizeof(int) * nza, hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(d_val, val, es * nza, hipMemcpyHostToDevice, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(m->hkoffs, hoffs, sizeof(int) * (noffs + 1), hipMemcpyHostToDevice, s));
+  if (nr > 0)
+    PSB_HIP_CHECK(hipMemcpyAsync(m->irn, irn_pos, sizeof(int) * nr, hipMemcpyHostToDevice, s));
+  int64_t nthreads = (int64_t)noffs * hksz;
+  if (nthreads > 0) {
+    dim3 grid((unsigned)((nthreads + kBlock - 1) / kBlock));
+    if (m->elem_type == PSB_HIP_TYPE_DOUBLE)
+      hipLaunchKernelGGL((k_hll_fill<double>), grid, dim3(kBlock), 0, s, nr, hksz, noffs, d_irn,
+                         d_hoffs, d_idisp, d_ja, (const double*)d_val, (const int*)m->perm,
+                         m->ja, (double*)m->val, m->idiag);
+    else
+      hipLaunchKernelGGL((k_hll_fill<float>), grid, dim3(kBlock), 0, s, nr, hksz, noffs, d_irn,
+                         d_hoffs, d_idisp, d_ja, (const float*)d_val, (const int*)m->perm,
+                         m->ja, (float*)m->val, m->idiag);
+    PSB_HIP_LAUNCH_CHECK();
+  }
+  PSB_HIP_CHECK(hipStreamSynchronize(s));
+  hipFree(d_irn); hipFree(d_hoffs); hipFree(d_idisp); hipFree(d_ja); hipFree(d_val);
+  m->nzeros = nza;
+  if (m->val32) refresh_single(m);
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_hll_set_perm(void* dmat, const int* perm, const int* hkoffs) {
+  PSB_ENSURE_INIT();
+  HllDev* m = (HllDev*)dmat;
+  if (!m || !perm || !hkoffs) return PSB_HIP_INVALID;
+  if (check_perm(perm, m->rows))
+    return set_error(PSB_HIP_INVALID, __func__, "perm is not a permutation of 1..rows");
+  if (hkoffs[0] != 0 || hkoffs[m->noffs] != m->allocsize)
+    return set_error(PSB_HIP_INVALID, __func__, "hkoffs inconsistent with allocsize");
+  PSB_HIP_CHECK(hipStreamSynchronize(ctx().stream));
+  hll_drop_sorted(m);
+  return hll_install_perm(m, perm, hkoffs);
+}
+
+int psb_hip_hll_read_perm(void* dmat, int* perm) {
+  PSB_ENSURE_INIT();
+  HllDev* m = (HllDev*)dmat;
+  if (!m || !perm) return PSB_HIP_INVALID;
+  if (!m->perm) {
+    for (int p = 0; p < m->rows; p++) perm[p] = p + 1;
+    return PSB_HIP_SUCCESS;
+  }
+  if (m->rows > 0)
+    PSB_HIP_CHECK(hipMemcpy(perm, m->perm, sizeof(int) * (size_t)m->rows, hipMemcpyDeviceToHost));
+  return PSB_HIP_SUCCESS;
+}
+
+int psb_hip_hll_units(void* dmat, int* nunits, int* nsplit) {
+  HllDev* m = (HllDev*)dmat;
+  if (!m) return PSB_HIP_INVALID;
+  if (nunits) *nunits = m->nunits;
+  if (nsplit) *nsplit = m->nsplit;
   return PSB_HIP_SUCCESS;
 }
 
@@ -1197,10 +1484,12 @@ int psb_hip_hll_get_diag(void* dmat, void* dv) {
   dim3 grid(grid_for(m->rows, kBlock));
   if (m->elem_type == PSB_HIP_TYPE_DOUBLE)
     hipLaunchKernelGGL((k_hll_diag<double>), grid, dim3(kBlock), 0, ctx().stream, m->rows,
-                       m->hksz, m->hkoffs, m->idiag, (const double*)m->val, (double*)d->v);
+                       m->hksz, m->hkoffs, m->idiag, (const int*)m->perm, (const double*)m->val,
+                       (double*)d->v);
   else
     hipLaunchKernelGGL((k_hll_diag<float>), grid, dim3(kBlock), 0, ctx().stream, m->rows,
-                       m->hksz, m->hkoffs, m->idiag, (const float*)m->val, (float*)d->v);
+                       m->hksz, m->hkoffs, m->idiag, (const int*)m->perm, (const float*)m->val,
+                       (float*)d->v);
   PSB_HIP_LAUNCH_CHECK();
   return PSB_HIP_SUCCESS;
 }
@@ -1620,10 +1909,12 @@ int psb_hip_hll_scal(void* dmat, void* dv, int left) {
   dim3 grid(grid_for(m->rows, kBlock));
   if (m->elem_type == PSB_HIP_TYPE_DOUBLE)
     hipLaunchKernelGGL((k_hll_scal<double>), grid, dim3(kBlock), 0, ctx().stream, m->rows, m->hksz,
-                       m->hkoffs, m->irn, m->ja, (const double*)d->v, left, (double*)m->val);
+                       m->hkoffs, m->irn, m->ja, (const int*)m->perm, (const double*)d->v, left,
+                       (double*)m->val);
   else
     hipLaunchKernelGGL((k_hll_scal<float>), grid, dim3(kBlock), 0, ctx().stream, m->rows, m->hksz,
-                       m->hkoffs, m->irn, m->ja, (const float*)d->v, left, (float*)m->val);
+                       m->hkoffs, m->irn, m->ja, (const int*)m->perm, (const float*)d->v, left,
+                       (float*)m->val);
   refresh_single(m);
   PSB_HIP_LAUNCH_CHECK();
   return PSB_HIP_SUCCESS;

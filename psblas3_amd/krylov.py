@@ -48,16 +48,9 @@ def psb_spnrmi(a, desc_a=None):
         rows = np.repeat(np.arange(a_.nr), np.diff(irp))
         return float(np.bincount(rows, weights=np.abs(val), minlength=a_.nr).max())
     if a_.get_fmt() == "HLG":
-        hk, irn, ja, val, idiag = a_.to_host()
-        hksz = a_.hksz
-        s = np.zeros(a_.nr)
-        for h in range(len(hk) - 1):
-            w = (hk[h + 1] - hk[h]) // hksz
-            blk = np.abs(val[hk[h]:hk[h + 1]]).reshape(w, hksz)
-            r0 = h * hksz
-            r1 = min(a_.nr, r0 + hksz)
-            s[r0:r1] = blk.sum(axis=0)[:r1 - r0]
-        return float(s.max())
+        irp, ja, val = a_.to_csr()
+        rows = np.repeat(np.arange(a_.nr), np.diff(irp))
+        return float(np.bincount(rows, weights=np.abs(val), minlength=a_.nr).max())
     irn, ja, val, idiag = a_.to_host()
     v = np.abs(val).reshape(a_.nzm, a_.nr)
     return float(v.sum(axis=0).max())
@@ -463,6 +456,12 @@ def psb_dgcr_vect(a, prec, b, x, eps, desc_a, itmax=1000, itrace=0, istop=2, irs
                     psb_geaxpby(1.0, c[i], 0.0, c[i + 1], desc_a)
                     psb_geaxpby(-h[i][j], cs[i], 1.0, c[i + 1], desc_a)
                 h[j][j] = psb_genrm2(c[j], desc_a)
+                if h[j][j] == 0.0:
+                    # A z(j) lies in the span of the earlier directions: z(j)
+                    # adds nothing; the cycle ends on the j-1 before it (the
+                    # reference divides by zero here, psb_dgcr.f90:292)
+                    j -= 1
+                    break
                 psb_geaxpby(1.0 / h[j][j], c[j], 0.0, cs[j], desc_a)
                 alpha[j] = psb_gedot(cs[j], r, desc_a)
                 psb_geaxpby(-alpha[j], cs[j], 1.0, r, desc_a)
@@ -483,36 +482,50 @@ def psb_dgcr_vect(a, prec, b, x, eps, desc_a, itmax=1000, itrace=0, istop=2, irs
 
 def psb_drgmres_vect(a, prec, b, x, eps, desc_a, itmax=1000, itrace=0, istop=2, irst=10):
     """psb_drgmres_vect (krylov/psb_drgmres.f90:109-502): restarted GMRES with
-    right preconditioning, Givens rotations on the host; istop 2 or 3 (x is
-    updated at convergence for both; the reference omits it for istop 3)."""
+    right preconditioning, Givens rotations on the host.  istop 1 (normwise
+    backward error, infinity norm) or 2 (||r||/||b||), as the reference
+    accepts (:182); anything else is psb_err_invalid_istop_.  istop 1
+    rebuilds xt = x + M^-1 V y from the partial least-squares solve at every
+    inner iteration and tests ||b - A xt||_inf / (||A||_inf ||xt||_inf +
+    ||b||_inf) (:387-405); the reference accumulates the rebuild into xt
+    without clearing it (it clears w1 instead, :392), which only equals the
+    rebuild at the first inner iteration; here xt is rebuilt from zero.  At an
+    exact breakdown (h(i+1,i) = 0: the Krylov space is invariant) v(i+1) is
+    not normalised; the rotation then zeroes the residual estimate and the
+    solve ends there (the reference divides by zero and never uses the
+    result)."""
     methd = "RGMRES"
     nl = int(irst)
     if nl <= 0:
         raise N.PsbError(N.INVALID, "psb_drgmres", f"irst {irst}")
-    if istop not in (2, 3):
-        raise N.PsbError(N.UNSUPPORTED, "psb_drgmres", f"istop {istop}")
+    if istop not in (1, 2):
+        raise N.PsbError(N.INVALID, "psb_drgmres", f"invalid istop {istop}")
     v = [None] + [psb_geall(desc_a, mold=x.v) for _ in range(nl + 1)]
     w = psb_geall(desc_a, mold=x.v)
     w1 = psb_geall(desc_a, mold=x.v)
+    xt = psb_geall(desc_a, mold=x.v) if istop == 1 else None
     hist = []
     itx = 0
     errnum, errden = 0.0, 1.0
     try:
-        bn2 = r0n2 = 0.0
-        if istop == 2:
-            bn2 = psb_genrm2(b, desc_a)
+        ani = bni = bn2 = 0.0
+        if istop == 1:
+            ani = psb_spnrmi(a, desc_a)
+            bni = psb_geamax(b, desc_a)
         else:
-            psb_geaxpby(1.0, b, 0.0, v[1], desc_a)
-            psb_spmm(-1.0, a, x, 1.0, v[1], desc_a)
-            r0n2 = psb_genrm2(v[1], desc_a)
+            bn2 = psb_genrm2(b, desc_a)
         while True:  # restart
             psb_geaxpby(1.0, b, 0.0, v[1], desc_a)
             psb_spmm(-1.0, a, x, 1.0, v[1], desc_a)
             rs = [0.0] * (nl + 2)
             rs[1] = psb_genrm2(v[1], desc_a)
             scal = 1.0 / rs[1] if rs[1] != 0.0 else math.inf
-            errnum = rs[1]
-            errden = bn2 if istop == 2 else r0n2
+            if istop == 1:
+                errnum = psb_geamax(v[1], desc_a)
+                errden = ani * psb_geamax(x, desc_a) + bni
+            else:
+                errnum = rs[1]
+                errden = bn2
             if errnum <= eps * errden:
                 break
             psb_gescal(scal, v[1], desc_a)
@@ -531,31 +544,50 @@ def psb_drgmres_vect(a, prec, b, x, eps, desc_a, itmax=1000, itrace=0, istop=2, 
                     h[k][i] = psb_gedot(v[k], w, desc_a)
                     psb_geaxpby(-h[k][i], v[k], 1.0, w, desc_a)
                 h[i + 1][i] = psb_genrm2(w, desc_a)
-                psb_geaxpby(1.0 / h[i + 1][i], w, 0.0, v[i + 1], desc_a)
+                invariant = h[i + 1][i] == 0.0
+                if not invariant:
+                    psb_geaxpby(1.0 / h[i + 1][i], w, 0.0, v[i + 1], desc_a)
                 for k in range(2, i + 1):
                     h[k - 1][i], h[k][i] = _drot(h[k - 1][i], h[k][i], cc[k - 1], sn[k - 1])
                 cc[i], sn[i] = _drotg(h[i][i], h[i + 1][i])
                 h[i][i], h[i + 1][i] = _drot(h[i][i], h[i + 1][i], cc[i], sn[i])
                 h[i + 1][i] = 0.0
                 rs[i], rs[i + 1] = _drot(rs[i], rs[i + 1], cc[i], sn[i])
-                errnum = abs(rs[i + 1])
+                if istop == 1:
+                    # xt = x + M^-1 (V y), y the least-squares solution so far
+                    rst = list(rs)
+                    _trsv_upper(i, h, rst)
+                    _zero(w1, desc_a)
+                    for k in range(1, i + 1):
+                        psb_geaxpby(rst[k], v[k], 1.0, w1, desc_a)
+                    prec.apply(w1, xt, desc_a)
+                    psb_geaxpby(1.0, x, 1.0, xt, desc_a)
+                    psb_geaxpby(1.0, b, 0.0, w1, desc_a)
+                    psb_spmm(-1.0, a, xt, 1.0, w1, desc_a)
+                    errnum = psb_geamax(w1, desc_a)
+                    errden = ani * psb_geamax(xt, desc_a) + bni
+                else:
+                    errnum = abs(rs[i + 1])
                 hist.append(errnum / errden if errden > 0 else errnum)
                 if itrace > 0 and itx % itrace == 0:
                     print(f"{methd + ':':18s} {itx:9d} {errnum / errden:20.6e} {eps:20.6e}")
-                if errnum <= eps * errden:
+                if errnum <= eps * errden or invariant:
                     done = True
                     break
-            m = i if done else nl
-            _trsv_upper(m, h, rs)
-            _zero(w1, desc_a)
-            for k in range(1, m + 1):
-                psb_geaxpby(rs[k], v[k], 1.0, w1, desc_a)
-            prec.apply(w1, w, desc_a)
-            psb_geaxpby(1.0, w, 1.0, x, desc_a)
+            if istop == 1:
+                psb_geaxpby(1.0, xt, 0.0, x, desc_a)
+            else:
+                m = i if done else nl
+                _trsv_upper(m, h, rs)
+                _zero(w1, desc_a)
+                for k in range(1, m + 1):
+                    psb_geaxpby(rs[k], v[k], 1.0, w1, desc_a)
+                prec.apply(w1, w, desc_a)
+                psb_geaxpby(1.0, w, 1.0, x, desc_a)
             if done:
                 break
     finally:
-        for t in v[1:] + [w, w1]:
+        for t in v[1:] + [w, w1] + ([xt] if xt is not None else []):
             t.free()
     return itx, (errnum / errden if errden > 0 else errnum), hist
 

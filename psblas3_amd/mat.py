@@ -18,6 +18,7 @@ import numpy as np
 from . import _native as N
 
 DUPL_OVWRT, DUPL_ADD, DUPL_ERR = 0, 1, 2
+IS_HOST, IS_SYNC, IS_DEV = -1, 0, 1
 _DUPL = {"ovwrt": DUPL_OVWRT, "add": DUPL_ADD, "err": DUPL_ERR}
 
 
@@ -60,6 +61,11 @@ class _hip_base_mat:
     def __init__(self):
         self.dev = None
         self.nr = self.nc = self.nnz = 0
+        # dual-memory state (cuda/psb_d_cuda_hlg_mat_mod.F90:40-42) and the
+        # host copy of the format's arrays (None until built on the host or
+        # read back by sync)
+        self._state = IS_DEV
+        self._hc = None
         # triangle flags of psb_d_base_sparse_mat (set_triangle, set_lower,
         # set_unit: base/modules/serial/psb_base_mat_mod.f90); a triangular
         # solve reads them as the reference's inner_vect_sv does
@@ -147,37 +153,138 @@ class _hip_base_mat:
 
     def get_diag(self, d):
         """get_diag on the device into the vector d (0 where no diagonal)."""
+        if self.is_host():
+            self.sync()
         d.ensure_dev()
         N.call(self._diag_fn, self.dev, d.dev)
         d.set_dev()
 
+    def vect_mv(self, alpha, x, beta, y):
+        """psb_d_cuda_*_vect_mv (cuda/impl/psb_d_cuda_hlg_vect_mv.F90:77-108):
+        with device vectors y = alpha*A*x + beta*y on the device (a "host"
+        matrix is synced first) and y left "dev"; otherwise on the host copy
+        (a "dev" matrix is synced first) with the CPU format's product, and y
+        rebuilt from the host result."""
+        from .vect import psb_hip_base_vect
+        if isinstance(x, psb_hip_base_vect) and isinstance(y, psb_hip_base_vect):
+            self._check_vects(x, y)
+            if self.is_host():
+                self.sync()
+            self._dev_mv(alpha, x, beta, y)
+            y.set_dev()
+        else:
+            self._host_mv(alpha, x, beta, y, trans=False)
+
+    def _host_mv(self, alpha, x, beta, y, trans):
+        rx = np.asarray(x.get_vect(), dtype=self._dtype)
+        ry = np.array(y.get_vect(), dtype=self._dtype)
+        nx, ny = (self.nr, self.nc) if trans else (self.nc, self.nr)
+        if rx.shape[0] < nx or ry.shape[0] < ny:
+            raise N.PsbError(N.INVALID, f"{self.fmt}_vect_mv", "x or y too short")
+        if self.is_dev():
+            self.sync()
+        cols = 1 if rx.ndim == 1 else rx.shape[1]
+        for c in range(cols):
+            xc = np.ascontiguousarray(rx if rx.ndim == 1 else rx[:, c])
+            yc = np.ascontiguousarray(ry if ry.ndim == 1 else ry[:, c])
+            if trans:
+                self._host_spmv_t(alpha, xc, beta, yc)
+            else:
+                self._host_spmv(alpha, xc, beta, yc)
+            if ry.ndim == 1:
+                ry = yc
+            else:
+                ry[:, c] = yc
+        y.bld(ry)
+
+    def _host_spmv_t(self, alpha, x, beta, y):
+        """y = alpha*A^T x + beta*y on the host copy, in the CPU loop's order
+        (base/serial/impl/psb_d_csr_impl.F90:308-370): y = beta*y, then rows
+        ascending, each row's entries in stored order, y(ja) updated one entry
+        at a time."""
+        irp, ja, val = self._host_csr()
+        dt = self._dtype
+        a = dt(alpha)
+        b = dt(beta)
+        if b == 0:
+            y[:] = 0
+        elif b == -1:
+            y[:] = -y
+        elif b != 1:
+            y[:] = b * y
+        if a == 0:
+            return
+        rows = np.repeat(np.arange(self.nr), np.diff(irp))
+        cols = ja.astype(np.int64) - 1
+        if a == 1:
+            np.add.at(y, cols, val * x[rows])
+        elif a == -1:
+            np.subtract.at(y, cols, val * x[rows])
+        else:
+            np.add.at(y, cols, (a * val) * x[rows])
+
     # ---- state of the dual-memory plug-in (cuda/psb_d_cuda_hlg_mat_mod.F90:
-    # is_host/is_dev/is_sync, set_*, sync, to_gpu, from_gpu).  The storage
-    # lives on the device only: it is always "dev", sync and to_gpu have
-    # nothing to copy, from_gpu reads the arrays back.
+    # 40-84, 303-364): host (-1), sync (0) or dev (1).  Built on the device
+    # (cp_from_coo) the matrix is "dev" with no host copy, as the reference's
+    # GPU cp_from_coo leaves it; built on the host (cp_from_coo(on_host=True))
+    # it is "host" until sync / to_gpu writes the device copy.  sync brings
+    # the stale side up to date; device products run on the device copy
+    # (syncing a "host" matrix first), products with a host vector on the
+    # host copy (syncing a "dev" matrix first); device-side scaling leaves
+    # the matrix "dev".
     def is_host(self):
-        return False
+        return self._state == IS_HOST
+
+    def is_dev(self):
+        return self._state == IS_DEV
 
     def is_sync(self):
-        return False
+        return self._state == IS_SYNC
 
     def set_host(self):
-        pass
+        self._state = IS_HOST
 
     def set_dev(self):
-        pass
+        self._state = IS_DEV
 
     def set_sync(self):
-        pass
+        self._state = IS_SYNC
 
     def sync(self):
-        pass
+        """d_cuda_hlg_sync: host -> to_gpu, dev -> from_gpu, then sync."""
+        if self._state == IS_HOST:
+            self.to_gpu()
+        elif self._state == IS_DEV:
+            self.from_gpu()
+        self._state = IS_SYNC
 
     def to_gpu(self):
-        pass
+        """psb_d_cuda_hlg_to_gpu: write the host copy to the device."""
+        if self._hc is None:
+            raise N.PsbError(N.INVALID, f"{self.fmt}_to_gpu", "no host copy")
+        self._write_dev(self._hc)
+        self._drop_t()
+        self._state = IS_SYNC
 
     def from_gpu(self):
-        return self.to_host()
+        """psb_d_cuda_hlg_from_gpu: read the device arrays into the host copy."""
+        if self._state != IS_SYNC and self._state != IS_HOST:
+            self._hc = self._read_dev()
+            self._state = IS_SYNC
+        return self._hc
+
+    def host_copy(self):
+        """The host copy of the format's arrays (read back first when stale)."""
+        if self._state == IS_DEV:
+            self.sync()
+        return self._hc
+
+    def to_host(self):
+        """The format's arrays: the host copy when it is the current one,
+        else read from the device."""
+        if self._state == IS_HOST:
+            return self._hc
+        return self._read_dev()
 
     def to_coo(self):
         """The stored entries as a host COO (row-major sorted)."""
@@ -244,8 +351,15 @@ class _hip_base_mat:
 
     def vect_mv_t(self, alpha, x, beta, y):
         """y = alpha * A^T x + beta * y on the device, equal bit for bit to the
-        CPU's trans 'T' / 'C' product (psb_d_csr_csmv, psb_d_hll_csmv)."""
+        CPU's trans 'T' / 'C' product (psb_d_csr_csmv, psb_d_hll_csmv); with a
+        host vector operand, on the host copy (as the reference always does)."""
+        from .vect import psb_hip_base_vect
+        if not (isinstance(x, psb_hip_base_vect) and isinstance(y, psb_hip_base_vect)):
+            self._host_mv(alpha, x, beta, y, trans=True)
+            return
         self._check_vects(x, y)
+        if self.is_host():
+            self.sync()
         if getattr(self, "_tmat", None) is None:
             irp, ja, val = self.to_csr()
             h = C.c_void_p()
@@ -271,10 +385,14 @@ class _hip_base_mat:
         self._triangle, self._upper, self._unit = flags[0], flags[1], False
 
     def scals(self, d):
-        """psb_d_cuda_*_scals: every value times the scalar d."""
+        """psb_d_cuda_*_scals: every value times the scalar d (on the device
+        copy; the matrix is then "dev")."""
         self.make_nonunit()
         self._drop_t()
+        if self.is_host():
+            self.sync()
         N.call(self._pfx + "_scals", self.dev, float(d))
+        self._state = IS_DEV
 
     def scal(self, d, side="L"):
         """psb_d_cuda_*_scal: rows (side 'L') or columns ('R') times d."""
@@ -283,6 +401,9 @@ class _hip_base_mat:
         need = self.nr if left else self.nc
         self.make_nonunit()
         self._drop_t()
+        if self.is_host():
+            self.sync()
+        self._state = IS_DEV
         if isinstance(d, psb_hip_base_vect):
             if d._dtype != self._dtype or d.get_nrows() < need:
                 raise N.PsbError(N.INVALID, f"{self.fmt}_scal", "d precision or size")
@@ -303,21 +424,37 @@ class _hip_base_mat:
 class psb_d_hip_hlg_sparse_mat(_hip_base_mat):
     """HLL on the device (psb_d_cuda_hlg_sparse_mat).  hksz defaults to the
     wavefront size, 64 on gfx950, as the reference GPU type uses WarpSize
-    (cuda/impl/psb_d_cuda_cp_hlg_from_coo.F90:60)."""
+    (cuda/impl/psb_d_cuda_cp_hlg_from_coo.F90:60).
+
+    sort_window: None (the default) stores rows in order, the reference's
+    layout.  Otherwise rows are sorted by descending length inside windows of
+    sort_window rows (0 or less: the whole matrix) and the hacks are built
+    over that order (the reorder array of the SPGPU HLL kernel,
+    cuda/spgpu/kernels/hell_spmv_base_template.cuh:227-252): each hack then
+    holds rows of similar length, so a matrix with very uneven rows is not
+    padded to its longest rows.  irn / idiag / the hack layout are then per
+    position and self.perm gives the row (1-based) at each position."""
     fmt = "HLG"
     _etype = N.TYPE_DOUBLE
     _dtype = np.float64
     _diag_fn = "psb_hip_hll_get_diag"
     _pfx = "psb_hip_hll"
 
-    def __init__(self, hksz=64):
+    def __init__(self, hksz=64, sort_window=None):
         super().__init__()
         self.hksz = int(hksz)
-        self.irn = self.hkoffs = None
+        self.sort_window = None if sort_window is None else int(sort_window)
+        self.irn = self.hkoffs = self.perm = None
         self.isz = 0
 
-    def cp_from_coo(self, coo):
-        """psb_d_cuda_cp_hlg_from_coo: offsets on the host, fill on the device."""
+    def mold(self):
+        return type(self)(hksz=self.hksz, sort_window=self.sort_window)
+
+    def cp_from_coo(self, coo, on_host=False):
+        """psb_d_cuda_cp_hlg_from_coo: offsets on the host, fill on the device
+        (with the row order of sort_window when set); the matrix is then "dev".
+        on_host=True fills the host copy instead (the CPU format's
+        psi_d_convert_hll_from_coo) and leaves the matrix "host" until sync."""
         if not coo.sorted:
             coo = psb_d_coo_sparse_mat(coo.nr, coo.nc, coo.ia, coo.ja, coo.val).fix()
         self.free()
@@ -329,18 +466,69 @@ class psb_d_hip_hlg_sparse_mat(_hip_base_mat):
         isz = C.c_int64(0)
         N.call("psb_hip_hll_offsets", nr, nz, _ptr(coo.ia), self.hksz, irn.ctypes.data,
                idisp.ctypes.data, hkoffs.ctypes.data, C.byref(isz))
+        perm = None
+        if self.sort_window is not None:
+            perm = np.zeros(max(nr, 1), np.int32)
+            irn_pos = np.zeros(max(nr, 1), np.int32)
+            N.call("psb_hip_hll_sort_rows", nr, self.hksz, self.sort_window, irn.ctypes.data,
+                   perm.ctypes.data, irn_pos.ctypes.data, hkoffs.ctypes.data, C.byref(isz))
         h = C.c_void_p()
         N.call("psb_hip_hll_alloc", C.byref(h), self.hksz, nr, nz, int(isz.value), self._etype, 1)
         self.dev = h
         val = np.ascontiguousarray(coo.val, dtype=self._dtype)
-        N.call("psb_hip_hll_copy_coo", nr, coo.nc, nz, self.hksz, noffs, int(isz.value),
-               irn.ctypes.data, hkoffs.ctypes.data, idisp.ctypes.data, _ptr(coo.ja), _ptr(val),
-               self.dev)
+        if on_host:
+            cnt = irn if perm is None else irn_pos
+            hja = np.zeros(max(int(isz.value), 1), np.int32)
+            hval = np.zeros(max(int(isz.value), 1), self._dtype)
+            hdiag = np.zeros(max(nr, 1), np.int32)
+            N.call("psb_hip_host_hll_fill", nr, self.hksz, cnt.ctypes.data, hkoffs.ctypes.data,
+                   idisp.ctypes.data, _ptr(coo.ja), _ptr(val),
+                   None if perm is None else perm.ctypes.data, self._etype, hja.ctypes.data,
+                   hval.ctypes.data, hdiag.ctypes.data)
+            self._hc = (hkoffs, cnt[:nr].copy(), hja[:int(isz.value)], hval[:int(isz.value)],
+                        hdiag[:nr])
+            self._state = IS_HOST
+            self.nr, self.nc, self.nnz = nr, coo.nc, nz
+            self.irn, self.hkoffs, self.isz = cnt[:nr], hkoffs, int(isz.value)
+            self.perm = None if perm is None else perm[:nr]
+            return self
+        if perm is None:
+            N.call("psb_hip_hll_copy_coo", nr, coo.nc, nz, self.hksz, noffs, int(isz.value),
+                   irn.ctypes.data, hkoffs.ctypes.data, idisp.ctypes.data, _ptr(coo.ja), _ptr(val),
+                   self.dev)
+        else:
+            N.call("psb_hip_hll_copy_coo_sorted", nr, coo.nc, nz, self.hksz, noffs,
+                   int(isz.value), perm.ctypes.data, irn_pos.ctypes.data, hkoffs.ctypes.data,
+                   idisp.ctypes.data, _ptr(coo.ja), _ptr(val), self.dev)
+            irn = irn_pos
         self.nr, self.nc, self.nnz = nr, coo.nc, nz
         self.irn, self.hkoffs, self.isz = irn[:nr], hkoffs, int(isz.value)
+        self.perm = None if perm is None else perm[:nr]
+        self._hc, self._state = None, IS_DEV
         return self
 
-    def to_host(self):
+    def _write_dev(self, hc):
+        hk, irn, ja, val, idiag = hc
+        val = np.ascontiguousarray(val, self._dtype)
+        N.call("psb_hip_hll_write", self.dev, val.ctypes.data, ja.ctypes.data, hk.ctypes.data,
+               irn.ctypes.data, idiag.ctypes.data)
+        if self.perm is not None:
+            N.call("psb_hip_hll_set_perm", self.dev, self.perm.ctypes.data, hk.ctypes.data)
+
+    def _dev_mv(self, alpha, x, beta, y):
+        N.call("psb_hip_hll_spmv", self.dev, float(alpha), x.dev, float(beta), y.dev)
+
+    def _host_spmv(self, alpha, x, beta, y):
+        hk, irn, ja, val, _ = self._hc
+        N.call("psb_hip_host_hll_spmv", self.nr, self.hksz, hk.ctypes.data, irn.ctypes.data,
+               ja.ctypes.data, val.ctypes.data,
+               None if self.perm is None else self.perm.ctypes.data, self._etype,
+               float(alpha), x.ctypes.data, float(beta), y.ctypes.data)
+
+    def _host_csr(self):
+        return self._csr_of(self._hc)
+
+    def _read_dev(self):
         """Read the device arrays back: (hkoffs, irn, ja, val, idiag)."""
         val = np.empty(max(self.isz, 1), self._dtype)
         ja = np.empty(max(self.isz, 1), np.int32)
@@ -353,21 +541,31 @@ class psb_d_hip_hlg_sparse_mat(_hip_base_mat):
 
     def to_csr(self):
         """The stored entries as host CSR (irp, ja, val), 1-based, padding dropped:
-        row r's entries are the first irn(r) slots of its hack column."""
-        hk, irn, ja, val, _ = self.to_host()
+        row r's entries are the first irn slots of its hack column (of the
+        position holding row r when the rows are sorted)."""
+        return self._csr_of(self.to_host())
+
+    def _csr_of(self, arrays):
+        hk, irn, ja, val, _ = arrays
         nr, hs = self.nr, self.hksz
-        rows = np.repeat(np.arange(nr, dtype=np.int64), irn)
-        start = np.concatenate([[0], np.cumsum(irn, dtype=np.int64)[:-1]])
-        j = np.arange(len(rows), dtype=np.int64) - np.repeat(start, irn)
-        pos = hk[rows // hs].astype(np.int64) + j * hs + rows % hs
-        irp = np.concatenate([[1], 1 + np.cumsum(irn, dtype=np.int64)]).astype(np.int32)
+        if self.perm is None:
+            order = np.arange(nr, dtype=np.int64)          # position of each row
+        else:
+            order = np.empty(nr, np.int64)
+            order[self.perm.astype(np.int64) - 1] = np.arange(nr, dtype=np.int64)
+        cnt = irn[order] if nr else irn
+        posn = np.repeat(order, cnt)
+        start = np.concatenate([[0], np.cumsum(cnt, dtype=np.int64)[:-1]])
+        j = np.arange(len(posn), dtype=np.int64) - np.repeat(start, cnt)
+        pos = hk[posn // hs].astype(np.int64) + j * hs + posn % hs
+        irp = np.concatenate([[1], 1 + np.cumsum(cnt, dtype=np.int64)]).astype(np.int32)
         return irp, ja[pos].astype(np.int32), val[pos].copy()
 
-    def vect_mv(self, alpha, x, beta, y):
-        """psb_d_cuda_hlg_vect_mv: y = alpha*A*x + beta*y on the device."""
-        self._check_vects(x, y)
-        N.call("psb_hip_hll_spmv", self.dev, float(alpha), x.dev, float(beta), y.dev)
-        y.set_dev()
+    def units(self):
+        """(work units, hacks cut into several units) of a row-sorted matrix."""
+        nu, ns = C.c_int(0), C.c_int(0)
+        N.call("psb_hip_hll_units", self.dev, C.byref(nu), C.byref(ns))
+        return nu.value, ns.value
 
     def tune(self, x, y, reps=20):
         """Time the HLL kernel variants on this matrix with vectors x, y and keep
@@ -408,7 +606,11 @@ class psb_d_hip_elg_sparse_mat(_hip_base_mat):
         super().__init__()
         self.nzm = 0
 
-    def cp_from_coo(self, coo):
+    def cp_from_coo(self, coo, on_host=False):
+        """psb_d_cuda_cp_elg_from_coo: filled on the device ("dev"), or with
+        on_host=True the host copy in the CPU layout (leading dimension nr,
+        psi_d_convert_ell_from_coo + psi_d_xtr_ell_from_coo's padding: a row's
+        last column, an empty row the matrix's first column) left "host"."""
         if not coo.sorted:
             coo = psb_d_coo_sparse_mat(coo.nr, coo.nc, coo.ia, coo.ja, coo.val).fix()
         self.free()
@@ -426,13 +628,29 @@ class psb_d_hip_elg_sparse_mat(_hip_base_mat):
         N.call("psb_hip_ell_alloc", C.byref(h), nr, nzm, nz, self._etype, 1)
         self.dev = h
         val = np.ascontiguousarray(coo.val, dtype=self._dtype)
-        N.call("psb_hip_ell_copy_coo", nr, coo.nc, nz, _ptr(irn), _ptr(idisp), _ptr(coo.ja),
-               _ptr(val), self.dev)
         self.nr, self.nc, self.nnz, self.nzm = nr, coo.nc, nz, nzm
         self.irn = irn
+        if on_host:
+            rows = (coo.ia - 1).astype(np.int64)
+            j = np.arange(nz, dtype=np.int64) - idisp[rows]
+            last = np.where(irn > 0, coo.ja[np.maximum(idisp[1:] - 1, 0)] if nz else 1,
+                            coo.ja[0] if nz else 1).astype(np.int32)
+            ja2 = np.repeat(last[None, :], nzm, axis=0)            # (nzm, nr): column-major
+            va2 = np.zeros((nzm, nr), self._dtype)
+            ja2[j, rows] = coo.ja
+            va2[j, rows] = val
+            diag = np.zeros(nr, np.int32)
+            on = coo.ia == coo.ja
+            diag[rows[on]] = (j[on] + 1).astype(np.int32)
+            self._hc = (irn.copy(), ja2.ravel(), va2.ravel(), diag)
+            self._state = IS_HOST
+            return self
+        N.call("psb_hip_ell_copy_coo", nr, coo.nc, nz, _ptr(irn), _ptr(idisp), _ptr(coo.ja),
+               _ptr(val), self.dev)
+        self._hc, self._state = None, IS_DEV
         return self
 
-    def to_host(self):
+    def _read_dev(self):
         """(irn, ja, val, idiag) with ja/val of shape (nr, nzm) column-major."""
         ja = np.empty(max(self.nr * self.nzm, 1), np.int32)
         val = np.empty(max(self.nr * self.nzm, 1), self._dtype)
@@ -445,7 +663,28 @@ class psb_d_hip_elg_sparse_mat(_hip_base_mat):
 
     def to_csr(self):
         """The stored entries as host CSR (irp, ja, val), padding dropped."""
-        irn, ja, val, _ = self.to_host()
+        return self._csr_of(self.to_host())
+
+    def _host_csr(self):
+        return self._csr_of(self._hc)
+
+    def _write_dev(self, hc):
+        irn, ja, val, idiag = hc
+        val = np.ascontiguousarray(val, self._dtype)
+        N.call("psb_hip_ell_write", self.dev, val.ctypes.data, ja.ctypes.data, max(self.nr, 1),
+               irn.ctypes.data, idiag.ctypes.data)
+
+    def _dev_mv(self, alpha, x, beta, y):
+        N.call("psb_hip_ell_spmv", self.dev, float(alpha), x.dev, float(beta), y.dev)
+
+    def _host_spmv(self, alpha, x, beta, y):
+        irn, ja, val, _ = self._hc
+        N.call("psb_hip_host_ell_spmv", self.nr, max(self.nr, 1), irn.ctypes.data,
+               ja.ctypes.data, val.ctypes.data, self._etype, float(alpha), x.ctypes.data,
+               float(beta), y.ctypes.data)
+
+    def _csr_of(self, arrays):
+        irn, ja, val, _ = arrays
         rows = np.repeat(np.arange(self.nr, dtype=np.int64), irn)
         start = np.concatenate([[0], np.cumsum(irn, dtype=np.int64)[:-1]])
         j = np.arange(len(rows), dtype=np.int64) - np.repeat(start, irn)
@@ -455,11 +694,6 @@ class psb_d_hip_elg_sparse_mat(_hip_base_mat):
 
     def pitch(self):
         return N.fn("psb_hip_ell_pitch")(self.dev)
-
-    def vect_mv(self, alpha, x, beta, y):
-        self._check_vects(x, y)
-        N.call("psb_hip_ell_spmv", self.dev, float(alpha), x.dev, float(beta), y.dev)
-        y.set_dev()
 
     def free(self):
         self._drop_t()
@@ -482,8 +716,10 @@ class psb_d_hip_csrg_sparse_mat(_hip_base_mat):
     _diag_fn = "psb_hip_csr_get_diag"
     _pfx = "psb_hip_csr"
 
-    def cp_from_coo(self, coo):
-        """psb_d_cp_csr_from_coo (base/serial/impl/psb_d_csr_impl.F90:3154)."""
+    def cp_from_coo(self, coo, on_host=False):
+        """psb_d_cp_csr_from_coo (base/serial/impl/psb_d_csr_impl.F90:3154): the
+        device copy written ("dev"), or with on_host=True only the host copy
+        ("host" until sync)."""
         if not coo.sorted:
             coo = psb_d_coo_sparse_mat(coo.nr, coo.nc, coo.ia, coo.ja, coo.val).fix()
         self.free()
@@ -496,11 +732,16 @@ class psb_d_hip_csrg_sparse_mat(_hip_base_mat):
         N.call("psb_hip_csr_alloc", C.byref(h), nr, coo.nc, nz, self._etype, 1)
         self.dev = h
         val = np.ascontiguousarray(coo.val, dtype=self._dtype)
-        N.call("psb_hip_csr_write", self.dev, irp.ctypes.data, _ptr(coo.ja), _ptr(val))
         self.nr, self.nc, self.nnz = nr, coo.nc, nz
+        if on_host:
+            self._hc = (irp, np.ascontiguousarray(coo.ja, np.int32).copy(), val.copy())
+            self._state = IS_HOST
+            return self
+        N.call("psb_hip_csr_write", self.dev, irp.ctypes.data, _ptr(coo.ja), _ptr(val))
+        self._hc, self._state = None, IS_DEV
         return self
 
-    def to_host(self):
+    def _read_dev(self):
         irp = np.empty(self.nr + 1, np.int32)
         ja = np.empty(max(self.nnz, 1), np.int32)
         val = np.empty(max(self.nnz, 1), self._dtype)
@@ -510,15 +751,26 @@ class psb_d_hip_csrg_sparse_mat(_hip_base_mat):
     def to_csr(self):
         return self.to_host()
 
+    def _host_csr(self):
+        return self._hc
+
+    def _write_dev(self, hc):
+        irp, ja, val = hc
+        val = np.ascontiguousarray(val, self._dtype)
+        N.call("psb_hip_csr_write", self.dev, irp.ctypes.data, _ptr(ja), _ptr(val))
+
+    def _dev_mv(self, alpha, x, beta, y):
+        N.call("psb_hip_csr_spmv", self.dev, float(alpha), x.dev, float(beta), y.dev)
+
+    def _host_spmv(self, alpha, x, beta, y):
+        irp, ja, val = self._hc
+        N.call("psb_hip_host_csr_spmv", self.nr, irp.ctypes.data, _ptr(ja), _ptr(val),
+               self._etype, float(alpha), x.ctypes.data, float(beta), y.ctypes.data)
+
     def analysis(self):
         lanes, nlong, thr = C.c_int(), C.c_int(), C.c_int()
         N.call("psb_hip_csr_get_analysis", self.dev, C.byref(lanes), C.byref(nlong), C.byref(thr))
         return lanes.value, nlong.value, thr.value
-
-    def vect_mv(self, alpha, x, beta, y):
-        self._check_vects(x, y)
-        N.call("psb_hip_csr_spmv", self.dev, float(alpha), x.dev, float(beta), y.dev)
-        y.set_dev()
 
     def inner_vect_sv(self, alpha, x, beta, y, trans="N"):
         """psb_d_cuda_csrg_inner_vect_sv (cuda/impl/psb_d_cuda_csrg_inner_vect_sv.F90):

@@ -80,7 +80,7 @@ def psb_spasb(a, desc=None, mold=None, afmt=None, dupl="add", keep_coo=False):
     a.coo.fix(_DUPL[dupl] if isinstance(dupl, str) else dupl)
     if mold is None:
         mold = _fmt_mold(afmt or "HLG")
-    a.a = mold.mold() if not hasattr(mold, "hksz") else type(mold)(hksz=mold.hksz)
+    a.a = mold.mold()
     a.a.cp_from_coo(a.coo)
     a.a._triangle, a.a._upper, a.a._unit = a._tri
     if not keep_coo:
@@ -93,7 +93,7 @@ def psb_cscnv(a, mold):
     if a.coo is None:
         raise N.PsbError(N.INVALID, "psb_cscnv", "assemble with keep_coo=True to convert")
     old = a.a
-    a.a = mold.mold() if not hasattr(mold, "hksz") else type(mold)(hksz=mold.hksz)
+    a.a = mold.mold()
     a.a.cp_from_coo(a.coo)
     a.a._triangle, a.a._upper, a.a._unit = a._tri
     if old is not None:
@@ -170,11 +170,15 @@ def _kind(*vs):
 # ------------------------------------------------------------------ BLAS
 def psb_spmm(alpha, a, x, beta, y, desc_a=None, trans="N"):
     """psb_spmm (base/psblas/psb_dspmm.f90:57-310): y = alpha*op(A)*x + beta*y;
-    op 'T' / 'C' runs on the transpose the storage builds on the device."""
+    op 'T' / 'C' runs on the transpose the storage builds on the device (on
+    the host copy when an operand is a host vector)."""
     t = str(trans).upper()
     if t not in ("N", "T", "C"):
         raise N.PsbError(N.INVALID, "psb_spmm", f"trans {trans!r}")
-    xv, yv = _dev(x), _dev(y)
+    # the storage plug-ins: device vectors take the device product, any other
+    # vector type the matrix's host copy (cuda/impl/psb_d_cuda_hlg_vect_mv.F90:77-108)
+    xv = x.v if isinstance(x, psb_d_vect_type) else x
+    yv = y.v if isinstance(y, psb_d_vect_type) else y
     if t == "N":
         a.a.vect_mv(alpha, xv, beta, yv)
     else:                       # 'T' and 'C' are the same product for real data

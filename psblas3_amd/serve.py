@@ -33,12 +33,13 @@ class psb_hip_server:
     reliable updates for CG."""
 
     def __init__(self, method="BICGSTAB", max_systems=128, arena_bytes=4 << 30, check_every=4,
-                 eps=1e-10, itmax=20000, inner_eps=1e-4, delta=1e-4, seed=4, refine=None):
+                 eps=1e-10, itmax=20000, inner_eps=1e-4, delta=1e-4, seed=4, refine=None,
+                 use_graph=True):
         if refine is None:
             refine = "restart" if method.upper() == "BICGSTAB" else "reliable"
         self.eng = psb_hip_batch(method, max_systems=max_systems, arena_bytes=arena_bytes,
                                  check_every=check_every, delta=delta, inner_eps=inner_eps,
-                                 refine=refine)
+                                 refine=refine, use_graph=use_graph)
         self.eps, self.itmax, self.seed = eps, itmax, seed
         self.mats = []           # (engine mat id, rows, rhs staging vector, solution vector)
 
@@ -133,17 +134,24 @@ def poisson_arrivals(rate, count, nmat, seed=4, stream=0):
 
 
 def latency_summary(rec):
-    """p50 / p99 / mean latency (ms) and solves per second over finished requests."""
+    """p50 / p99 / mean latency (ms, arrival to done) and solves per second over
+    finished requests, with the latency split into queue wait (arrival to
+    admission) and service (admission to done)."""
     fin = [r for r in rec if r.get("status", 0) != 0]
     if not fin:
         return {"finished": 0, "requests": len(rec)}
     lat = np.array([r["latency"] for r in fin]) * 1e3
+    svc = np.array([r["done"] - r["admitted"] for r in fin]) * 1e3
+    wait = np.array([r["admitted"] - r["arrival"] for r in fin]) * 1e3
     span = max(r["done"] for r in fin) - min(r["arrival"] for r in rec)
+    pct = lambda a, q: round(float(np.percentile(a, q)), 3)
     return {"requests": len(rec), "finished": len(fin),
             "converged": int(sum(r["status"] == 1 for r in fin)),
-            "p50_ms": round(float(np.percentile(lat, 50)), 3),
-            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "p50_ms": pct(lat, 50),
+            "p99_ms": pct(lat, 99),
             "mean_ms": round(float(lat.mean()), 3),
+            "service_p50_ms": pct(svc, 50), "service_p99_ms": pct(svc, 99),
+            "queue_wait_p50_ms": pct(wait, 50), "queue_wait_p99_ms": pct(wait, 99),
             "solves_per_s": round(len(fin) / span, 2) if span > 0 else None,
             "iterations_mean": round(float(np.mean([r["iterations"] for r in fin])), 1),
             "max_relres": max(r["relres"] for r in fin)}

@@ -403,6 +403,72 @@ class psb_i_hip_vect(psb_hip_base_vect):
     _setscal = "psb_hip_mv_setscal_i"
 
 
+class psb_d_base_vect:
+    """Host-only vector (psb_d_base_vect_type, base/modules/serial/
+    psb_d_base_vect_mod.F90:63-150): values in host memory, always in the host
+    state.  A product of a device matrix with such a vector runs on the
+    matrix's host copy (cuda/impl/psb_d_cuda_hlg_vect_mv.F90:77-108)."""
+    _dtype = np.float64
+    fmt = "BASE"
+
+    def __init__(self):
+        self.h = None
+        self.n = 0
+        self.count = 1
+
+    def all(self, n, count=1):
+        self.h = np.zeros((n,) if count == 1 else (n, count), dtype=self._dtype, order="F")
+        self.n, self.count = n, count
+        return self
+
+    def bld(self, values):
+        a = np.asarray(values, dtype=self._dtype)
+        self.all(a.shape[0], 1 if a.ndim == 1 else a.shape[1])
+        self.h[...] = a
+        return self
+
+    def free(self):
+        self.h = None
+        self.n = 0
+
+    def mold(self):
+        return type(self)()
+
+    def is_host(self):
+        return True
+
+    def is_dev(self):
+        return False
+
+    def is_sync(self):
+        return False
+
+    def set_host(self):
+        pass
+
+    def sync(self):
+        pass
+
+    def get_vect(self):
+        return self.h.copy()
+
+    def get_nrows(self):
+        return self.n
+
+    def zero(self):
+        self.h[...] = 0
+
+    def set_scal(self, val, first=None, last=None):
+        lo = 0 if first is None else first - 1
+        hi = self.n if last is None else last
+        self.h[lo:hi] = val
+
+
+class psb_s_base_vect(psb_d_base_vect):
+    """Single-precision host-only vector (psb_s_base_vect_type)."""
+    _dtype = np.float32
+
+
 def _same_kind(*vs):
     t = type(vs[0])
     for v in vs[1:]:

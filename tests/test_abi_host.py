@@ -139,3 +139,36 @@ def test_uniform_fill_matches_oracle(orc):
     N.fn("psb_hip_fill_uniform_host")(5, 0, len(x), x.ctypes.data)
     assert np.array_equal(x, orc.uniform_pm1(5, len(x)))
     assert x.min() >= -1.0 and x.max() < 1.0 and abs(x.mean()) < 0.01
+
+
+@pytest.mark.parametrize("window,hksz", [(0, 64), (64, 64), (100, 32), (1000, 64), (-1, 7)])
+def test_hll_sort_rows_host(window, hksz):
+    """psb_hip_hll_sort_rows against a numpy restatement: inside each window
+    of ceil(window/hksz)*hksz rows (all rows for window <= 0), positions take
+    rows by descending count, ties by ascending row; perm is a bijection of
+    1..nr; hack offsets are hksz times each hack's widest count."""
+    rng = np.random.default_rng(3)
+    nr = 1000
+    irn = np.minimum(rng.zipf(2.0, nr), 300).astype(np.int32)
+    irn[::17] = 0
+    noffs = (nr + hksz - 1) // hksz
+    perm = np.zeros(nr, np.int32)
+    ipos = np.zeros(nr, np.int32)
+    hk = np.zeros(noffs + 1, np.int32)
+    isz = C.c_int64()
+    N.call("psb_hip_hll_sort_rows", nr, hksz, window, irn.ctypes.data, perm.ctypes.data,
+           ipos.ctypes.data, hk.ctypes.data, C.byref(isz))
+    win = nr if window <= 0 else -(-window // hksz) * hksz
+    exp = []
+    for w0 in range(0, nr, win):
+        idx = np.arange(w0, min(nr, w0 + win))
+        exp.extend(idx[np.lexsort((idx, -irn[idx]))])
+    exp = np.array(exp) + 1
+    assert np.array_equal(perm, exp)
+    assert np.array_equal(np.sort(perm), np.arange(1, nr + 1))
+    assert np.array_equal(ipos, irn[perm - 1])
+    widths = [ipos[h * hksz:(h + 1) * hksz].max() for h in range(noffs)]
+    assert np.array_equal(np.diff(hk), hksz * np.array(widths))
+    assert isz.value == hk[-1]
+    # sorting never pads more than rows in order
+    assert isz.value <= sum(hksz * irn[h * hksz:(h + 1) * hksz].max() for h in range(noffs))

@@ -69,7 +69,9 @@ def test_batch_cg_aniso(gpu, orc):
         assert o["status"] == 1
         assert rr <= 1.0e-8 * 1.01
         assert abs(o["relres"] - rr) <= 1e-3 * rr + 1e-15
-        assert 0.8 * ito - 2 <= o["iterations"] <= 1.6 * ito + 10
+        # FP32 CG with FP64 residual replacement follows the FP64 recurrence
+        # closely: within 30% (+5) above and 10% (+2) below the oracle's count
+        assert 0.9 * ito - 2 <= o["iterations"] <= 1.3 * ito + 5
 
 
 def test_batch_alone_vs_mix_bitwise(gpu, orc):
@@ -145,6 +147,10 @@ def test_batch_bicgstab_convdiff(gpu, orc, refine):
              f"relres {rr:.2e} (oracle FP64 BiCGSTAB {ito} its)")
         assert o["status"] == 1
         assert rr <= 1.0e-10 * 1.01
+        # FP32 BiCGSTAB needs more iterations than FP64 (rounding in the
+        # inner recurrence, restarts at each FP64 transition): at most 1.5x the
+        # oracle's FP64 count (+10), and not fewer than 0.7x
+        assert 0.7 * ito <= o["iterations"] <= 1.5 * ito + 10
 
 
 def test_batch_admission_by_footprint(gpu, orc):

@@ -363,3 +363,59 @@ def test_krylov_single_precision(gpu, orc, method, irst):
     tol = 1e-1 if method == "BICGSTABL" else 1e-3
     assert np.linalg.norm(xs - xt) <= tol * np.linalg.norm(xt)
     a.free()
+
+
+def test_rgmres_istop1_dropin(gpu, orc):
+    """RGMRES with istop 1 (normwise backward error) through psb_krylov on the
+    device against the oracle: +-2 iterations, the bound met by the returned
+    x; istop 3 is rejected as the reference rejects it (psb_drgmres.f90:182)."""
+    print("case psb_krylov RGMRES istop 1", flush=True)
+    coo = orc.gen_convdiff(12, conv=4.0 / np.sqrt(3.0))
+    csr = orc.csr_from_coo(orc.fix_coo(coo))
+    bh = orc.csr_spmv(*csr, orc.uniform_pm1(15, coo.nr))
+    a, desc, prec = _setup(coo, P.psb_d_hip_hlg_sparse_mat())
+    x = P.vect_from(np.zeros(coo.nr))
+    it, err, hist = P.psb_krylov("RGMRES", a, prec, P.vect_from(bh), x, 1e-9, desc, itmax=2000,
+                                 istop=1, irst=10)
+    ani = P.psb_spnrmi(a, desc)
+    xo, ito, erro, histo = orc.krylov("RGMRES", csr, bh, eps=1e-9, itmax=2000, istop=1,
+                                      ani=ani, irst=10)
+    print(f"  device {it} its err {err:.3e} / oracle {ito} its err {erro:.3e}", flush=True)
+    assert abs(it - ito) <= 2 and err <= 1e-9
+    k = min(len(hist), len(histo), 10)
+    assert np.allclose(hist[:k], histo[:k], rtol=1e-6)
+    xs = x.get_vect()
+    irp, ja, val = csr
+    r = bh - orc.csr_spmv(irp, ja, val, xs)
+    bwe = np.abs(r).max() / (ani * np.abs(xs).max() + np.abs(bh).max())
+    assert bwe <= 1e-9
+    with pytest.raises(P.PsbError):
+        P.psb_krylov("RGMRES", a, prec, P.vect_from(bh), P.vect_from(np.zeros(coo.nr)), 1e-9,
+                     desc, istop=3)
+    a.free()
+
+
+@pytest.mark.parametrize("method", ["RGMRES", "GCR"])
+def test_invariant_krylov_space(gpu, orc, method):
+    """A = I, b = e1: the Krylov space is invariant after one step (RGMRES:
+    h(2,1) = 0); both methods end with the exact solution, no division by
+    zero."""
+    print(f"case {method} invariant space", flush=True)
+    n = 70
+    ia = np.arange(1, n + 1, dtype=np.int32)
+    coo = P.psb_d_coo_sparse_mat(n, n, ia, ia.copy(), np.ones(n))
+    coo.fix()
+    a = P.psb_dspmat_type()
+    a.m = a.n = n
+    a.a = P.psb_d_hip_hlg_sparse_mat().cp_from_coo(coo)
+    desc = P.psb_cdall(nl=n)
+    prec = P.psb_dprec_type("DIAG").build(a, desc)
+    b = np.zeros(n)
+    b[0] = 1.0
+    for istop in ((1, 2) if method == "RGMRES" else (2,)):
+        x = P.vect_from(np.zeros(n))
+        it, err, _ = P.psb_krylov(method, a, prec, P.vect_from(b), x, 1e-12, desc, itmax=20,
+                                  istop=istop, irst=4)
+        assert it == 1 and err == 0.0, (istop, it, err)
+        assert np.array_equal(x.get_vect(), b)
+    a.free()

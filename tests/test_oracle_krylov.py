@@ -156,6 +156,78 @@ def test_rgmres_minimal_residual(orc):
         assert abs(hist[k - 1] - best) <= 1e-7 * best + 1e-14, (k, hist[k - 1], best)
 
 
+def _gmres_iterates(Ad, d, b, kmax):
+    """Unrestarted right-preconditioned GMRES iterates x_k = D Q_k y_k (x0 =
+    0), y_k the least-squares minimiser over the k-dimensional Krylov space
+    of A D, by dense linear algebra."""
+    n = len(b)
+    Q = np.zeros((n, kmax + 1))
+    Q[:, 0] = b / np.linalg.norm(b)
+    xs = []
+    for k in range(1, kmax + 1):
+        w = Ad @ Q[:, k - 1]
+        for _ in range(2):
+            w -= Q[:, :k] @ (Q[:, :k].T @ w)
+        Q[:, k] = w / np.linalg.norm(w)
+        y, *_ = np.linalg.lstsq(Ad @ Q[:, :k], b, rcond=None)
+        xs.append(d * (Q[:, :k] @ y))
+    return xs
+
+
+def test_rgmres_istop1_normwise_backward_error(orc):
+    """istop 1 (krylov/psb_drgmres.f90:387-405): the solve stops at the first
+    GMRES iterate whose normwise backward error ||b - A x||_inf /
+    (||A||_inf ||x||_inf + ||b||_inf) is <= eps, returns that iterate, and its
+    history is that error per iteration (brute-force iterates as reference)."""
+    csr, A = _csr_scipy(orc, orc.gen_convdiff(6, conv=3.0))
+    n = A.shape[0]
+    b = A @ orc.uniform_pm1(13, n)
+    d = orc.diag_build(*csr)
+    ani = float(abs(A).sum(axis=1).max())
+    eps = 1e-9
+    x, it, err, hist = orc.krylov("RGMRES", csr, b, eps=eps, itmax=80, irst=80, istop=1,
+                                  ani=ani)
+    xs = _gmres_iterates(A.toarray() @ np.diag(d), d, b, it)
+    bni = np.abs(b).max()
+    bwe = [np.abs(b - A @ xk).max() / (ani * np.abs(xk).max() + bni) for xk in xs]
+    assert bwe[-1] <= eps and all(e > eps for e in bwe[:-1]), bwe[-3:]
+    assert np.allclose(hist, bwe, rtol=1e-4, atol=1e-15)
+    assert abs(err - bwe[-1]) <= 1e-4 * bwe[-1]
+    assert np.linalg.norm(x - xs[-1]) <= 1e-8 * np.linalg.norm(xs[-1])
+    # and it is not the 2-norm ratio of istop 2
+    _, _, _, hist2 = orc.krylov("RGMRES", csr, b, eps=eps, itmax=80, irst=80, istop=2)
+    k = min(len(hist), len(hist2))
+    assert np.all(np.abs(hist[:k] - hist2[:k]) > 1e-3 * hist2[:k])
+
+
+def test_rgmres_istop1_restarted(orc):
+    """With restarts each cycle continues from the last rebuilt iterate: the
+    final x meets the normwise backward error bound."""
+    csr, A = _csr_scipy(orc, orc.gen_convdiff(8, conv=4.0))
+    n = A.shape[0]
+    b = A @ orc.uniform_pm1(14, n)
+    ani = float(abs(A).sum(axis=1).max())
+    x, it, err, _ = orc.krylov("RGMRES", csr, b, eps=1e-10, itmax=2000, irst=6, istop=1,
+                               ani=ani)
+    bwe = np.abs(b - A @ x).max() / (ani * np.abs(x).max() + np.abs(b).max())
+    assert it > 6 and err <= 1e-10 and bwe <= 1e-10
+
+
+def test_rgmres_invariant_space(orc):
+    """A = I, b = e1: h(2,1) = 0 after one step; the solve ends there with
+    the exact solution (no division by zero)."""
+    n = 5
+    irp = np.arange(1, n + 2, dtype=np.int32)
+    ja = np.arange(1, n + 1, dtype=np.int32)
+    val = np.ones(n)
+    b = np.zeros(n)
+    b[0] = 1.0
+    for istop in (1, 2):
+        x, it, err, _ = orc.krylov("RGMRES", (irp, ja, val), b, eps=0.0, itmax=20, irst=4,
+                                   istop=istop, ani=1.0)
+        assert it == 1 and err == 0.0 and np.array_equal(x, b), (istop, it, err, x)
+
+
 def test_bicgstabl_1_equals_bicgstab_unpreconditioned(orc):
     csr, A = _csr_scipy(orc, orc.gen_convdiff(10, conv=4.0 / np.sqrt(3.0)))
     b = A @ orc.uniform_pm1(7, A.shape[0])

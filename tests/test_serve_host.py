@@ -25,12 +25,15 @@ def test_poisson_arrivals_deterministic():
 
 
 def test_latency_summary():
-    rec = [{"arrival": 0.0, "done": 0.010, "latency": 0.010, "status": 1, "relres": 1e-11,
-            "iterations": 10},
-           {"arrival": 0.005, "done": 0.020, "latency": 0.015, "status": 1, "relres": 2e-11,
-            "iterations": 20},
+    rec = [{"arrival": 0.0, "admitted": 0.0, "done": 0.010, "latency": 0.010, "status": 1,
+            "relres": 1e-11, "iterations": 10},
+           {"arrival": 0.005, "admitted": 0.011, "done": 0.020, "latency": 0.015, "status": 1,
+            "relres": 2e-11, "iterations": 20},
            {"arrival": 0.006, "status": 0}]
     s = latency_summary(rec)
     assert s["finished"] == 2 and s["requests"] == 3 and s["converged"] == 2
     assert abs(s["p50_ms"] - 12.5) < 1e-9
     assert abs(s["solves_per_s"] - 100.0) < 1e-9
+    # latency = queue wait (arrival -> admitted) + service (admitted -> done)
+    assert abs(s["queue_wait_p50_ms"] - 3.0) < 1e-9
+    assert abs(s["service_p50_ms"] - 9.5) < 1e-9
