@@ -96,10 +96,18 @@ def test_sorted_conversion(gpu, orc, name, window, hksz):
         sl = hk[last] + np.arange(wl) * hksz + lane
         assert np.all(ja[sl] == f.nr) and np.all(val[sl] == 0)
     if name == "powerlaw_20000":
-        # sorting removes nearly all of the padding of the rows-in-order layout
-        assert a.a.isz <= 1.2 * f.nz or window == 64
+        # a whole-matrix sort removes nearly all of the padding of the
+        # rows-in-order layout; a window sort keeps each window's first hack
+        # as wide as its longest row (5 windows of 4096 rows here: 1.85x),
+        # which the width check above already pins
+        if window <= 0:
+            assert a.a.isz <= 1.2 * f.nz
         h = orc.hll_from_coo(f, hksz)
-        assert a.a.isz < h.hkoffs[-1]
+        if window == hksz:
+            # sorting inside one hack leaves every hack's width unchanged
+            assert a.a.isz == h.hkoffs[-1]
+        else:
+            assert a.a.isz < h.hkoffs[-1]
     a.free()
 
 
