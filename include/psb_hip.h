@@ -516,6 +516,23 @@ int psb_hip_krylov_fused(int method, int fmt, void *dmat, void *diag, void *b, v
                          double eps, int itmax, int istop, int check_every, int *iter,
                          double *err, double *hist);
 
+/* psb_hip_krylov_fused with every device buffer it needs (state, partials,
+   history, work vectors) carved from a static device array of 64 MiB that
+   each GPU holds until the process ends, and the host read-backs through a
+   pinned block made by psb_hip_init: it calls no hipMalloc / hipFree /
+   hipHostMalloc.  No counterpart in the reference: it is for callers that
+   cannot allocate (code under HIP graph capture of their own, serving loops
+   that must not allocate per request).  Same arguments, results and errors
+   as psb_hip_krylov_fused; needs psb_hip_init first (PSB_HIP_INVALID
+   otherwise) and rejects with PSB_HIP_INVALID a solve whose buffers exceed
+   the array: 4 (CG) or 8 (BiCGSTAB) vectors of n elements, 8 * (itmax + 2)
+   bytes of history and 512 KiB of state and partials, i.e. n up to about
+   2M rows for FP64 CG and 1M for FP64 BiCGSTAB.  Every call shares the
+   array: calls that overlap on one GPU (several host threads) race. */
+int psb_hip_krylov_fused_noalloc(int method, int fmt, void *dmat, void *diag, void *b, void *x,
+                                 double eps, int itmax, int istop, int check_every, int *iter,
+                                 double *err, double *hist);
+
 /* FP32-inner / FP64-outer iterative refinement on a double HLL matrix with a
    single copy (psb_hip_hll_make_single).  FP64: r = b - A x, x += d and the
    convergence norms; FP32: the matrix values, the inner Krylov vectors and

@@ -168,6 +168,8 @@ _SIGS = {
     "psb_hip_hll_set_xcd_chunk": (i32, [i32]),
     "psb_hip_csr_get_diag": (i32, [vp, vp]),
     "psb_hip_krylov_fused": (i32, [i32, i32, vp, vp, vp, vp, f64, i32, i32, i32, pi32, pf64, vp]),
+    "psb_hip_krylov_fused_noalloc": (i32, [i32, i32, vp, vp, vp, vp, f64, i32, i32, i32, pi32, pf64,
+                                           vp]),
     "psb_hip_krylov_mixed": (i32, [i32, vp, vp, vp, vp, f64, i32, f64, i32, i32, i32, pi32,
                                    pi32, pf64]),
     "psb_hip_mm_read_coo": (i32, [C.c_char_p, i64, pi32, pi32, pi64, vp, vp, vp]),

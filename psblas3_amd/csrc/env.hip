@@ -16,6 +16,14 @@ static std::string g_last_error;
 static std::mutex g_init_mutex;
 static hipEvent_t g_ev_start = nullptr, g_ev_stop = nullptr;
 
+// Workspace of psb_hip_krylov_fused_noalloc: a static device array, so that
+// entry point needs no allocation.  Each device holds its own copy until the
+// process ends.  In host code the name `scratch` is not a device address:
+// init_device looks the address up (hipGetSymbolAddress, with the device
+// current) into Context::scratch, and every kernel argument or copy uses
+// that pointer.
+static __device__ __align__(256) char scratch[kScratchBytes];
+
 Context& ctx() { return g_ctx; }
 
 int set_error(int code, const char* where, const char* what) {
@@ -44,6 +52,13 @@ static int init_device(int dev) {
   PSB_HIP_CHECK(hipMalloc(&c.red_result, sizeof(double) * 64));
   PSB_HIP_CHECK(hipMemset(c.red_counter, 0, sizeof(unsigned int) * 64));
   PSB_HIP_CHECK(hipHostMalloc(&c.host_result, sizeof(double) * 64, hipHostMallocDefault));
+  void* sp = nullptr;
+  PSB_HIP_CHECK(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+  c.scratch = (char*)sp;
+  c.scratch_bytes = kScratchBytes;
+  PSB_HIP_CHECK(hipHostMalloc(&c.scratch_host, kScratchHostBytes, hipHostMallocDefault));
+  for (hipEvent_t& e : c.scratch_ev)
+    PSB_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   PSB_HIP_CHECK(hipEventCreate(&g_ev_start));
   PSB_HIP_CHECK(hipEventCreate(&g_ev_stop));
   PSB_HIP_CHECK(hipDeviceSynchronize());
@@ -151,6 +166,8 @@ int psb_hip_exit(void) {
   hipFree(c.red_counter);
   hipFree(c.red_result);
   hipHostFree(c.host_result);
+  hipHostFree(c.scratch_host);
+  for (hipEvent_t e : c.scratch_ev) hipEventDestroy(e);
   hipEventDestroy(g_ev_start);
   hipEventDestroy(g_ev_stop);
   hipStreamDestroy(c.stream);

@@ -603,8 +603,21 @@ struct Solver {
   double* hist = nullptr;
   std::vector<void*> bufs;
   hipStream_t stream = nullptr;
+  // allocation-free mode: device buffers are carved from the library's
+  // scratch array and the host mirror and events are the context's
+  char* arena = nullptr;
+  size_t arena_left = 0;
+  bool borrowed = false;
 
   int alloc(void** p, size_t bytes) {
+    if (borrowed) {
+      const size_t b = (bytes + 255) & ~(size_t)255;
+      if (b > arena_left) return set_error(PSB_HIP_INVALID, "krylov", "scratch exhausted");
+      *p = arena;
+      arena += b;
+      arena_left -= b;
+      return PSB_HIP_SUCCESS;
+    }
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) return set_error(PSB_HIP_OUTOFMEM, "krylov", hipGetErrorString(e));
     bufs.push_back(*p);
@@ -612,12 +625,24 @@ struct Solver {
   }
   ~Solver() {
     for (void* p : bufs) (void)hipFree(p);
+    if (borrowed) return;
     if (hst) (void)hipHostFree(hst);
     if (ring) (void)hipHostFree(ring);
     for (hipEvent_t e : ev)
       if (e) (void)hipEventDestroy(e);
   }
 };
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Device bytes one fused solve takes: state, partials, history and the work
+// vectors (CG: r p q z; BiCGSTAB: r q p v f s z t).
+static size_t solve_bytes(int method, int n, int etype, int itmax) {
+  const size_t vec = align256(elem_size(etype) * (size_t)(n > 0 ? n : 1));
+  return align256(sizeof(KState)) + align256(sizeof(double) * 2 * kMaxGrid) +
+         align256(sizeof(double) * ((size_t)itmax + 2)) + (method == 0 ? 4 : 8) * vec;
+}
+static_assert(3 * sizeof(KState) <= kScratchHostBytes, "host mirror + ring in the pinned block");
 
 template <typename T>
 struct Mats {
@@ -882,7 +907,9 @@ static int solve(Solver& s, const T* b, T* x, const T* d, double eps, int itmax,
   return PSB_HIP_SUCCESS;
 }
 
-static int setup(Solver& s, int method, int fmt, void* dmat, int itmax) {
+// The matrix side of setup: format, size and element type.
+static int describe(Solver& s, int method, int fmt, void* dmat) {
+  if (!dmat) return set_error(PSB_HIP_INVALID, "krylov", "no matrix");
   s.method = method;
   s.fmt = fmt;
   s.mat = dmat;
@@ -905,16 +932,36 @@ static int setup(Solver& s, int method, int fmt, void* dmat, int itmax) {
   } else {
     return set_error(PSB_HIP_UNSUPPORTED, "krylov", "format");
   }
+  return PSB_HIP_SUCCESS;
+}
+
+// State, partials and history buffers plus the host mirror and events
+// (carved from the scratch array and borrowed from the context when
+// s.borrowed is set).
+static int setup_state(Solver& s, int itmax) {
   int rc = s.alloc((void**)&s.st, sizeof(KState));
   if (rc) return rc;
   rc = s.alloc((void**)&s.partials, sizeof(double) * 2 * kMaxGrid);
   if (rc) return rc;
   rc = s.alloc((void**)&s.hist, sizeof(double) * ((size_t)itmax + 2));
   if (rc) return rc;
+  if (s.borrowed) {
+    s.hst = (KState*)ctx().scratch_host;
+    s.ring = s.hst + 1;
+    s.ev[0] = ctx().scratch_ev[0];
+    s.ev[1] = ctx().scratch_ev[1];
+    return PSB_HIP_SUCCESS;
+  }
   PSB_HIP_CHECK(hipHostMalloc((void**)&s.hst, sizeof(KState), hipHostMallocDefault));
   PSB_HIP_CHECK(hipHostMalloc((void**)&s.ring, 2 * sizeof(KState), hipHostMallocDefault));
   for (hipEvent_t& e : s.ev) PSB_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   return PSB_HIP_SUCCESS;
+}
+
+static int setup(Solver& s, int method, int fmt, void* dmat, int itmax) {
+  int rc = describe(s, method, fmt, dmat);
+  if (rc) return rc;
+  return setup_state(s, itmax);
 }
 
 static int mv_check(const MultiVec* v, int etype, int n) {
@@ -1103,26 +1150,21 @@ static int mixed_restart(int method, HllDev* h, const double* b, double* x, cons
   return PSB_HIP_SUCCESS;
 }
 
-}  // namespace psbhip
-
-using namespace psbhip;
-
-extern "C" {
-
-int psb_hip_krylov_fused(int method, int fmt, void* dmat, void* diag, void* bv, void* xv,
-                         double eps, int itmax, int istop, int check_every, int* iter,
-                         double* err, double* hist) {
-  int rc = ensure_init();
-  if (rc) return rc;
+static int fused_args(const char* who, int istop, int itmax) {
   if (istop != 2 && istop != 3)
-    return set_error(PSB_HIP_UNSUPPORTED, __func__, "istop must be 2 or 3 in the fused solver");
-  if (itmax < 0) return set_error(PSB_HIP_INVALID, __func__, "itmax");
-  Solver s;
-  rc = setup(s, method == 0 ? 0 : 1, fmt, dmat, itmax);
-  if (rc) return rc;
+    return set_error(PSB_HIP_UNSUPPORTED, who, "istop must be 2 or 3 in the fused solver");
+  if (itmax < 0) return set_error(PSB_HIP_INVALID, who, "itmax");
+  return PSB_HIP_SUCCESS;
+}
+
+// The body of both fused entry points once the solver's buffers are set up.
+static int fused_run(const char* who, Solver& s, void* diag, void* bv, void* xv, double eps,
+                     int itmax, int istop, int check_every, int* iter, double* err,
+                     double* hist) {
+  int rc;
   MultiVec *b = (MultiVec*)bv, *x = (MultiVec*)xv, *d = (MultiVec*)diag;
   if (mv_check(b, s.etype, s.n) || mv_check(x, s.etype, s.n) || (d && mv_check(d, s.etype, s.n)))
-    return set_error(PSB_HIP_INVALID, __func__, "b/x/diag type or size");
+    return set_error(PSB_HIP_INVALID, who, "b/x/diag type or size");
   if (s.n == 0) {
     *iter = 0;
     *err = 0.0;
@@ -1141,6 +1183,52 @@ int psb_hip_krylov_fused(int method, int fmt, void* dmat, void* diag, void* bv, 
     PSB_HIP_CHECK(hipStreamSynchronize(s.stream));
   }
   return PSB_HIP_SUCCESS;
+}
+
+}  // namespace psbhip
+
+using namespace psbhip;
+
+extern "C" {
+
+int psb_hip_krylov_fused(int method, int fmt, void* dmat, void* diag, void* bv, void* xv,
+                         double eps, int itmax, int istop, int check_every, int* iter,
+                         double* err, double* hist) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if ((rc = fused_args(__func__, istop, itmax))) return rc;
+  Solver s;
+  rc = setup(s, method == 0 ? 0 : 1, fmt, dmat, itmax);
+  if (rc) return rc;
+  return fused_run(__func__, s, diag, bv, xv, eps, itmax, istop, check_every, iter, err, hist);
+}
+
+int psb_hip_krylov_fused_noalloc(int method, int fmt, void* dmat, void* diag, void* bv, void* xv,
+                                 double eps, int itmax, int istop, int check_every, int* iter,
+                                 double* err, double* hist) {
+  Context& c = ctx();
+  if (!c.initialized || !c.scratch)
+    return set_error(PSB_HIP_INVALID, __func__,
+                     "the library is not initialised (call psb_hip_init first; this entry point "
+                     "allocates nothing, initialisation included)");
+  int rc = fused_args(__func__, istop, itmax);
+  if (rc) return rc;
+  Solver s;
+  s.borrowed = true;
+  if ((rc = describe(s, method == 0 ? 0 : 1, fmt, dmat))) return rc;
+  const size_t need = solve_bytes(s.method, s.n, s.etype, itmax);
+  if (need > c.scratch_bytes) {
+    char msg[256];
+    snprintf(msg, sizeof(msg),
+             "needs %zu bytes of scratch (n = %d, itmax = %d), over the %zu-byte maximum; "
+             "use psb_hip_krylov_fused",
+             need, s.n, itmax, c.scratch_bytes);
+    return set_error(PSB_HIP_INVALID, __func__, msg);
+  }
+  s.arena = c.scratch;
+  s.arena_left = c.scratch_bytes;
+  if ((rc = setup_state(s, itmax))) return rc;
+  return fused_run(__func__, s, diag, bv, xv, eps, itmax, istop, check_every, iter, err, hist);
 }
 
 int psb_hip_krylov_mixed(int method, void* dmat, void* diag, void* bv, void* xv, double eps,

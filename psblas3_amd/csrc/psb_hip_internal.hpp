@@ -42,7 +42,17 @@ struct Context {
   unsigned int* red_counter = nullptr;  // 64 counters, kept at zero between uses
   double* red_result = nullptr;         // 64 result slots
   double* host_result = nullptr;        // pinned, 64 slots
+  // the allocation-free solver's workspace: the device address of the
+  // file-scope scratch array on this context's device (looked up at init),
+  // a pinned host block for its state read-backs and two events
+  char* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  void* scratch_host = nullptr;         // pinned, kScratchHostBytes
+  hipEvent_t scratch_ev[2] = {nullptr, nullptr};
 };
+// size of the device scratch array and of its pinned host companion
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+constexpr size_t kScratchHostBytes = 4096;
 Context& ctx();
 int ensure_init();                      // lazily initialises device 0
 // early return with the error code when the library cannot initialise

@@ -727,9 +727,12 @@ def _diag_only(prec, name):
 
 
 def psb_krylov_fused(method, a, prec, b, x, eps, desc_a=None, itmax=1000, istop=2,
-                     check_every=8, history=False):
+                     check_every=8, history=False, noalloc=False):
     """Device-resident CG / BiCGSTAB (same recurrences as psb_krylov, scalars
     kept on the device, one host poll every check_every iterations).
+    noalloc: run through psb_hip_krylov_fused_noalloc, whose buffers come
+    from the library's static 64 MiB scratch array (no allocation per call;
+    FP64 CG up to about 2M rows, BiCGSTAB about 1M).
     Returns (iter, err, history)."""
     import ctypes as C
     import numpy as np
@@ -744,7 +747,8 @@ def psb_krylov_fused(method, a, prec, b, x, eps, desc_a=None, itmax=1000, istop=
     it, err = C.c_int(0), C.c_double(0.0)
     hist = np.zeros(max(itmax, 1)) if history else None
     meth = 0 if method.upper() == "CG" else 1
-    N.call("psb_hip_krylov_fused", meth, _FMT[a.a.get_fmt()], a.a.dev, diag, b.v.dev, x.v.dev,
+    entry = "psb_hip_krylov_fused_noalloc" if noalloc else "psb_hip_krylov_fused"
+    N.call(entry, meth, _FMT[a.a.get_fmt()], a.a.dev, diag, b.v.dev, x.v.dev,
            float(eps), int(itmax), int(istop), int(check_every), C.byref(it), C.byref(err),
            hist.ctypes.data if history else None)
     x.v.set_dev()

@@ -6,6 +6,7 @@ import ctypes as C
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -172,3 +173,20 @@ def test_hll_sort_rows_host(window, hksz):
     assert isz.value == hk[-1]
     # sorting never pads more than rows in order
     assert isz.value <= sum(hksz * irn[h * hksz:(h + 1) * hksz].max() for h in range(noffs))
+
+
+def test_noalloc_solver_needs_init():
+    """The allocation-free fused solver does not initialise the library: in a
+    fresh process without psb_hip_init it returns PSB_HIP_INVALID before any
+    device work."""
+    code = (
+        "import ctypes as C\n"
+        "from psblas3_amd import _native as N\n"
+        "it, err = C.c_int(0), C.c_double(0.0)\n"
+        "rc = N.fn('psb_hip_krylov_fused_noalloc')(0, 0, None, None, None, None, 1e-9, 10, 2, 8,"
+        " C.byref(it), C.byref(err), None)\n"
+        "print(rc, N.fn('psb_hip_last_error')().decode())\n")
+    out = subprocess.check_output([sys.executable, "-c", code], cwd=ROOT,
+                                  env=dict(os.environ, HIP_VISIBLE_DEVICES="")).decode()
+    rc, msg = out.split(" ", 1)
+    assert int(rc) == N.INVALID and "psb_hip_init" in msg
