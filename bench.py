@@ -309,7 +309,7 @@ def aniso_matrix(P, N, L, e):
     return m
 
 
-def batch_bench(P, N, nsys=256, eps=1e-8, seed=3, use_graph=True):
+def batch_bench(P, N, nsys=256, eps=1e-8, seed=3, use_graph=True, baselines=True):
     """Batched uneven solves: nsys anisotropic 2-D Poisson systems -e u_xx - u_yy,
     side L log-uniform in [32, 512] and e log-uniform in [1e-3, 1] (host uniform
     stream 0 and 1 of the seed), b = A x_true with x_true ~ U[-1,1) (stream
@@ -376,6 +376,10 @@ def batch_bench(P, N, nsys=256, eps=1e-8, seed=3, use_graph=True):
                     "arena_bytes": int(sum(fps))}
     log(f"batch: engine {dt * 1e3:.1f} ms for {nsys} solves, max relres "
         f"{out['batch']['max_relres']:.2e}, max its {out['batch']['fp32_iterations_max']}")
+    if not baselines:
+        for s in systems:
+            s[0].free()
+        return out
     # one at a time
     for name in ("fp64_one_at_a_time", "mixed_one_at_a_time"):
         xs = [P.psb_d_vect_type(P.psb_d_hip_vect().all(s[5])) for s in systems]
@@ -659,6 +663,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--serve-arena-mb", type=int, default=768)
     ap.add_argument("--batch-systems", type=int, default=256)
+    ap.add_argument("--no-batch-baselines", action="store_true",
+                    help="batch section: the engine only, without the one-at-a-time solves")
     ap.add_argument("--eager", action="store_true",
                     help="batch and serving engines launch their kernels directly instead of "
                          "replaying a captured graph (the mode profiled runs use)")
@@ -834,7 +840,8 @@ def main():
     solve, coo_solve = (None, None) if args.no_solve else solve_bench(P, N, args.solve_L)
     methods = None if args.no_methods else methods_bench(P, N, args.methods_L)
     batch = None if args.no_batch else batch_bench(P, N, args.batch_systems,
-                                                   use_graph=not args.eager)
+                                                   use_graph=not args.eager,
+                                                   baselines=not args.no_batch_baselines)
     serve = None if args.no_serve else serve_bench(P, N, count=args.serve_requests,
                                                    arena_mb=args.serve_arena_mb,
                                                    use_graph=not args.eager)
