@@ -128,6 +128,33 @@ def solve_bench(P, N, L=256, eps=1e-10):
     log(f"solve: Poisson L={L} n={n} nnz={nz} set up in {time.perf_counter() - t0:.1f}s")
     xtrue = xt.get_vect()
     out = {"matrix": f"SPD Poisson 7-point L={L}", "n": n, "nnz": int(nz), "eps": eps}
+    # the standalone products on the same matrix (FP64, and FP32 through the
+    # single copy the mixed solvers use), for the per-iteration accounting
+    N.call("psb_hip_hll_make_single", a.a.dev)
+    a.a._has_single = True
+    ys = P.psb_d_hip_vect().all(n)
+    prod = {}
+    for name, fn_name in (("fp64", "psb_hip_hll_spmv"), ("fp32", "psb_hip_hll_spmv_single")):
+        if name == "fp64":
+            xs_, ys_ = xt, ys
+        else:
+            xs_ = P.psb_s_hip_vect().bld(xtrue.astype(np.float32))
+            ys_ = P.psb_s_hip_vect().all(n)
+        for _ in range(3):
+            N.call(fn_name, a.a.dev, 1.0, xs_.dev, 0.0, ys_.dev)
+        timer = P.DeviceTimer()
+        timer.start()
+        for _ in range(20):
+            N.call(fn_name, a.a.dev, 1.0, xs_.dev, 0.0, ys_.dev)
+        us = timer.stop_ms() / 20 * 1e3
+        nbytes = spmv_bytes(n, nz, single=name == "fp32")
+        prod[name] = {"us": round(us, 2), "GBs": round(nbytes / (us * 1e-6) / 1e9, 1)}
+        if name == "fp32":
+            xs_.free()
+            ys_.free()
+    ys.free()
+    out["products"] = prod
+    log(f"solve: standalone products FP64 {prod['fp64']['us']} us, FP32 {prod['fp32']['us']} us")
     # warm both paths (kernel loads, first allocations)
     x = P.psb_geall(desc)
     P.psb_krylov_fused("CG", a, prec, b, x, eps, desc, itmax=3)

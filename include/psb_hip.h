@@ -365,6 +365,30 @@ int psb_hip_hll_scal(void *dmat, void *d, int left);
    1 = one row per lane, 2 = two adjacent rows per lane (even hksz), both in
    launch order; + 4 = XCD block order.  Returns the previous value. */
 int psb_hip_hll_set_variant(int variant);
+/* Plane-tile block order of this matrix's products (single and multi-RHS;
+   no reference counterpart): for banded matrices whose couplings sit near
+   multiples of a period of period_rows rows (a 3-D stencil in natural order:
+   L*L), the row blocks are walked in tile columns of tile_blocks blocks
+   through every plane and whole tile columns are dealt to the XCDs, so the
+   rows that read the same x entries across planes run on one XCD back to
+   back.  The results are those of the default order.  period_rows = 0 or
+   tile_blocks = 0 restores the default (chunked XCD order).  Not for
+   row-sorted matrices. */
+int psb_hip_hll_set_plane(void *dmat, int period_rows, int tile_blocks);
+/* 16-bit column offsets (no reference counterpart): when a rows-in-order
+   HLL matrix is written (psb_hip_hll_write, psb_hip_hll_copy_coo) and every
+   slot's column lies within +-32767 of lane 0's column in the same hack
+   slot column shifted by the row difference (banded matrices, the
+   stencils), the library also keeps each slot's column as a 16-bit
+   residual plus one 32-bit base per hack slot column, and the products
+   (single and multi-RHS) read those instead of ja: 2 bytes of index traffic
+   per slot instead of 4, the same columns, the same results bit for bit.
+   ja stays as written (read back, transposed products, scaling).
+   psb_hip_hll_index16: 1 when the matrix has them.  psb_hip_hll_set_index16
+   (0 / 1, default 1) switches their use in products on or off for every
+   matrix; returns the previous setting. */
+int psb_hip_hll_index16(void *dmat);
+int psb_hip_hll_set_index16(int on);
 /* XCD block order of the HLL (variants 0 and + 4), ELL, multi-RHS HLL and
    row-block CSR products: logical blocks dealt to the XCDs in chunks of
    `blocks` (default 16; 0: one contiguous range per XCD, for the row-block

@@ -72,6 +72,8 @@ struct BSys {
   int n, hksz;
   const int* hkoffs;
   const int* ja;
+  const short* res16;   // 16-bit column offsets (HllDev::res16 / base16) or null
+  const int* base16;
   const float* a32;
   const double* a64;
   const float* d32;   // FP32 Jacobi 1/a_ii, or null
@@ -99,14 +101,25 @@ __device__ __forceinline__ bool b_transiting(const BState* s) {
   return s->phase != BP_RUN && s->done == 0;
 }
 
-// (A x)[r] of a system's HLL matrix with values a (FP32 or FP64)
+// (A x)[r] of a system's HLL matrix with values a (FP32 or FP64); HK > 0:
+// the hack size as a compile-time constant
+template <typename T, int HK>
+__device__ __forceinline__ T b_row_hk(const BSys& sy, const T* a, int r,
+                                      const T* __restrict__ x) {
+  const int stride = HK > 0 ? HK : sy.hksz;
+  const int hk = r / stride;
+  const int lane = r - hk * stride;
+  const int off0 = sy.hkoffs[hk];
+  const int w = (sy.hkoffs[hk + 1] - off0) / stride;
+  if (sy.res16)
+    return slot_dot16<T, T>(sy.res16 + off0 + lane, sy.base16 + off0 / stride, a + off0 + lane,
+                            stride, w, r, x);
+  return slot_dot<T, T>(sy.ja + off0 + lane, a + off0 + lane, stride, w, x);
+}
+
 template <typename T>
 __device__ __forceinline__ T b_row(const BSys& sy, const T* a, int r, const T* __restrict__ x) {
-  const int hk = r / sy.hksz;
-  const int lane = r - hk * sy.hksz;
-  const int off0 = sy.hkoffs[hk];
-  const int w = (sy.hkoffs[hk + 1] - off0) / sy.hksz;
-  return slot_dot<T, T>(sy.ja + off0 + lane, a + off0 + lane, sy.hksz, w, x);
+  return sy.hksz == 64 ? b_row_hk<T, 64>(sy, a, r, x) : b_row_hk<T, 0>(sy, a, r, x);
 }
 
 // The body runs for every tile of a system satisfying COND (st, sy, r0 =
@@ -998,6 +1011,9 @@ int psb_hip_batch_submit(void* batch, int mat_id, void* bv, void* x0v, double ep
   sy.hksz = m.h->hksz;
   sy.hkoffs = m.h->hkoffs;
   sy.ja = m.h->ja;
+  const bool i16 = hll_use_idx16(m.h);
+  sy.res16 = i16 ? m.h->res16 : nullptr;
+  sy.base16 = i16 ? m.h->base16 : nullptr;
   sy.a32 = m.h->val32;
   sy.a64 = (const double*)m.h->val;
   sy.d32 = m.d32;

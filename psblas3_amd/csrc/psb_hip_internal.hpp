@@ -296,6 +296,55 @@ inline int xcd_chunk_grid(int nlog, int chunk) {
   return ((nc + kXcds - 1) / kXcds) * kXcds * chunk;
 }
 
+// Block order of a product launch.  mode 0: launch order; 1: chunked XCD
+// order (xcd_block_chunk, xchunk blocks per chunk); 2: plane-tile order for
+// banded matrices whose couplings sit near multiples of a period of P rows
+// (a 3-D stencil in natural order: P = L^2).  The row blocks are viewed as nz
+// planes of pb blocks; tile columns of tb adjacent blocks are walked through
+// every plane (block in the tile fastest, then plane), and whole tile
+// columns are dealt to the XCDs in turn (column c to XCD c % kXcds), so the
+// blocks that read the same x rows across planes run back to back on one
+// XCD and find them in its L2.
+struct BlockOrder {
+  int mode = 0, xchunk = 0;
+  int pb = 1, tb = 1, nz = 1, ntc = 1;
+};
+
+// Logical block of this workgroup under order o; nlog (past the matrix) for
+// the workgroups the order leaves idle.
+__device__ __forceinline__ int order_block(const BlockOrder& o, int nlog) {
+  if (o.mode == 0) return (int)blockIdx.x;
+  if (o.mode == 1) return xcd_block_chunk(nlog, o.xchunk);
+  const int b = (int)blockIdx.x, j = b % kXcds, s = b / kXcds;
+  const int per_col = o.nz * o.tb;
+  const int k = s / per_col, rem = s - k * per_col;
+  const int c = k * kXcds + j;
+  const int z = rem / o.tb, sub = rem - z * o.tb;
+  const int col = c * o.tb + sub;
+  if (c >= o.ntc || col >= o.pb) return nlog;
+  const int64_t pblk = (int64_t)z * o.pb + col;
+  return pblk < nlog ? (int)pblk : nlog;
+}
+
+// Plane-tile order for nlog blocks of rows_per_block rows: planes of
+// period_rows rows rounded to whole blocks, tiles of tile_blocks blocks.
+inline BlockOrder plane_order(int nlog, int rows_per_block, int period_rows, int tile_blocks) {
+  BlockOrder o;
+  o.mode = 2;
+  o.pb = (period_rows + rows_per_block / 2) / rows_per_block;
+  if (o.pb < 1) o.pb = 1;
+  o.tb = tile_blocks < 1 ? 1 : (tile_blocks > o.pb ? o.pb : tile_blocks);
+  o.nz = (nlog + o.pb - 1) / o.pb;
+  o.ntc = (o.pb + o.tb - 1) / o.tb;
+  return o;
+}
+
+inline int order_grid(const BlockOrder& o, int nlog) {
+  if (o.mode == 0) return nlog;
+  if (o.mode == 1) return xcd_chunk_grid(nlog, o.xchunk);
+  return ((o.ntc + kXcds - 1) / kXcds) * kXcds * o.nz * o.tb;
+}
+
 // ------------------------------------------------------------------ device matrices
 struct HllDev {
   int elem_type;
@@ -320,7 +369,19 @@ struct HllDev {
   int nsplit = 0;
   int4* splits = nullptr;
   void* part = nullptr;   // partial sums, (split units) x hksz entries of 8 bytes
+  // plane-tile block order of the products (plane_order), 0: chunked XCD order
+  int plane_rows = 0;
+  int plane_tb = 0;
+  // 16-bit column offsets (rows in order, built when every slot fits): the
+  // 0-based column of slot (hack h, slot column j, lane l) is
+  // h*hksz + l + base16[hkoffs[h]/hksz + j] + res16[hkoffs[h] + j*hksz + l],
+  // i.e. ja - 1; base16 holds lane 0's offset of each slot column.  The
+  // products read these 2-byte residuals instead of the 4-byte ja.
+  short* res16 = nullptr;   // allocsize
+  int* base16 = nullptr;    // allocsize / hksz
 };
+// whether the products of m read its 16-bit column offsets (spmv.hip)
+bool hll_use_idx16(const HllDev* m);
 // slots of one work unit of the row-sorted HLL product
 constexpr int kHllUnitSlots = 256;
 
@@ -425,6 +486,94 @@ __device__ __forceinline__ T slot_dot(const int* __restrict__ pj, const TV* __re
 
 template <typename T>
 using vec2 = T __attribute__((ext_vector_type(2)));
+
+// The same groups over 16-bit column offsets (HllDev::res16 / base16): pe
+// and pv address the row's slot 0, pb the hack's slot-column bases, r is the
+// row (0-based).  Column of slot j: r + pb[j] + pe[j*stride].
+template <int G, typename T, typename TV>
+__device__ __forceinline__ T slot_group16(const short* __restrict__ pe,
+                                          const int* __restrict__ pb,
+                                          const TV* __restrict__ pv, int stride, int j, int w,
+                                          int r, const T* __restrict__ x, T acc) {
+  int c[G];
+  TV v[G];
+#pragma unroll
+  for (int k = 0; k < G; k++) {
+    const int jj = j + k < w ? j + k : w - 1;
+    c[k] = r + pb[jj] + (int)ld_stream(pe + (size_t)jj * stride);
+    v[k] = ld_stream(pv + (size_t)jj * stride);
+  }
+  T xv[G];
+#pragma unroll
+  for (int k = 0; k < G; k++) xv[k] = x[c[k]];
+#pragma unroll
+  for (int k = 0; k < G; k++) {
+    const T t = fma((T)v[k], xv[k], acc);
+    acc = j + k < w ? t : acc;
+  }
+  return acc;
+}
+
+template <typename T, typename TV>
+__device__ __forceinline__ T slot_dot16(const short* __restrict__ pe, const int* __restrict__ pb,
+                                        const TV* __restrict__ pv, int stride, int w, int r,
+                                        const T* __restrict__ x) {
+  T acc = T(0);
+  int j = 0;
+  for (; j + 8 <= w; j += 8) acc = slot_group16<8, T, TV>(pe, pb, pv, stride, j, w, r, x, acc);
+  const int rem = w - j;
+  if (rem > 4) acc = slot_group16<8, T, TV>(pe, pb, pv, stride, j, w, r, x, acc);
+  else if (rem > 0) acc = slot_group16<4, T, TV>(pe, pb, pv, stride, j, w, r, x, acc);
+  return acc;
+}
+
+// Rows r (even) and r + 1 of one hack over 16-bit offsets: one 4-byte
+// offset-pair load and one 2-element value load per slot.
+template <int G, typename T, typename TV>
+__device__ __forceinline__ void slot_group16_pair(const short* __restrict__ pe,
+                                                  const int* __restrict__ pb,
+                                                  const TV* __restrict__ pv, int stride, int j,
+                                                  int w, int r, const T* __restrict__ x, T& a0,
+                                                  T& a1) {
+  int c0[G], c1[G];
+  vec2<TV> v[G];
+#pragma unroll
+  for (int k = 0; k < G; k++) {
+    const int jj = j + k < w ? j + k : w - 1;
+    const int b = r + pb[jj];
+    const vec2<short> e = ld_stream(reinterpret_cast<const vec2<short>*>(pe + (size_t)jj * stride));
+    c0[k] = b + (int)e.x;
+    c1[k] = b + 1 + (int)e.y;
+    v[k] = ld_stream(reinterpret_cast<const vec2<TV>*>(pv + (size_t)jj * stride));
+  }
+  T x0[G], x1[G];
+#pragma unroll
+  for (int k = 0; k < G; k++) {
+    x0[k] = x[c0[k]];
+    x1[k] = x[c1[k]];
+  }
+#pragma unroll
+  for (int k = 0; k < G; k++) {
+    const T t0 = fma((T)v[k].x, x0[k], a0);
+    const T t1 = fma((T)v[k].y, x1[k], a1);
+    a0 = j + k < w ? t0 : a0;
+    a1 = j + k < w ? t1 : a1;
+  }
+}
+
+template <typename T, typename TV>
+__device__ __forceinline__ void slot_dot16_pair(const short* __restrict__ pe,
+                                                const int* __restrict__ pb,
+                                                const TV* __restrict__ pv, int stride, int w,
+                                                int r, const T* __restrict__ x, T& a0, T& a1) {
+  a0 = T(0);
+  a1 = T(0);
+  int j = 0;
+  for (; j + 8 <= w; j += 8) slot_group16_pair<8, T, TV>(pe, pb, pv, stride, j, w, r, x, a0, a1);
+  const int rem = w - j;
+  if (rem > 4) slot_group16_pair<8, T, TV>(pe, pb, pv, stride, j, w, r, x, a0, a1);
+  else if (rem > 0) slot_group16_pair<4, T, TV>(pe, pb, pv, stride, j, w, r, x, a0, a1);
+}
 
 // Two adjacent rows of one hack (same width w): one 8-byte index load and one
 // 2-element value load per slot feed both rows.  pj/pv address the even row's

@@ -65,16 +65,18 @@ inline int csr_class(int len) {
 }
 
 // ------------------------------------------------------------------ HLL SpMV
-// HK > 0: compile-time hack size (32 or 64); HK == 0: any hksz.  XCD: blocks
-// take XCD-contiguous row ranges (xcd_block); otherwise blocks in launch order.
-template <typename T, typename TV, int HK, bool XCD>
-__global__ __launch_bounds__(kBlock) void k_hll_spmv(int nr, int hksz, int nlog, int xchunk,
+// HK > 0: compile-time hack size (32 or 64); HK == 0: any hksz.  ord: the
+// block order (launch, chunked XCD or plane-tile).
+template <typename T, typename TV, int HK, bool IDX16>
+__global__ __launch_bounds__(kBlock) void k_hll_spmv(int nr, int hksz, int nlog, BlockOrder ord,
                                                      const int* __restrict__ hkoffs,
                                                      const int* __restrict__ ja,
+                                                     const short* __restrict__ res16,
+                                                     const int* __restrict__ base16,
                                                      const TV* __restrict__ val,
                                                      const T* __restrict__ x, T alpha, T beta,
                                                      T* __restrict__ y) {
-  const int lb = XCD ? xcd_block_chunk(nlog, xchunk) : (int)blockIdx.x;
+  const int lb = order_block(ord, nlog);
   if (lb >= nlog) return;
   const int row = lb * kBlock + threadIdx.x;
   const int r = row < nr ? row : nr - 1;
@@ -83,21 +85,29 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmv(int nr, int hksz, int nlog,
   const int lane = r - hk * stride;
   const int off0 = hkoffs[hk];
   const int w = (hkoffs[hk + 1] - off0) / stride;
-  T acc = slot_dot<T, TV>(ja + off0 + lane, val + off0 + lane, stride, w, x);
+  T acc;
+  if (IDX16)
+    acc = slot_dot16<T, TV>(res16 + off0 + lane, base16 + off0 / stride, val + off0 + lane,
+                            stride, w, r, x);
+  else
+    acc = slot_dot<T, TV>(ja + off0 + lane, val + off0 + lane, stride, w, x);
   if (row < nr) store_y(y, row, alpha, acc, beta);
 }
 
 // Two adjacent rows per lane (rows 2t and 2t+1 of a 2*kBlock-row tile, same
 // hack since hksz is even): 8-byte index and 2-element value loads.  Rows
 // past nr read the last hack's padding rows (the hack stores hksz rows).
-template <typename T, typename TV, int HK, bool XCD>
-__global__ __launch_bounds__(kBlock) void k_hll_spmv_pair(int nr, int hksz, int nlog, int xchunk,
+template <typename T, typename TV, int HK, bool IDX16>
+__global__ __launch_bounds__(kBlock) void k_hll_spmv_pair(int nr, int hksz, int nlog,
+                                                          BlockOrder ord,
                                                           const int* __restrict__ hkoffs,
                                                           const int* __restrict__ ja,
+                                                          const short* __restrict__ res16,
+                                                          const int* __restrict__ base16,
                                                           const TV* __restrict__ val,
                                                           const T* __restrict__ x, T alpha,
                                                           T beta, T* __restrict__ y) {
-  const int lb = XCD ? xcd_block_chunk(nlog, xchunk) : (int)blockIdx.x;
+  const int lb = order_block(ord, nlog);
   if (lb >= nlog) return;
   const int row = lb * (2 * kBlock) + 2 * (int)threadIdx.x;
   const int last = (nr - 1) & ~1;
@@ -108,7 +118,11 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmv_pair(int nr, int hksz, int 
   const int off0 = hkoffs[hk];
   const int w = (hkoffs[hk + 1] - off0) / stride;
   T a0, a1;
-  slot_dot_pair<T, TV>(ja + off0 + lane, val + off0 + lane, stride, w, x, a0, a1);
+  if (IDX16)
+    slot_dot16_pair<T, TV>(res16 + off0 + lane, base16 + off0 / stride, val + off0 + lane, stride,
+                           w, r, x, a0, a1);
+  else
+    slot_dot_pair<T, TV>(ja + off0 + lane, val + off0 + lane, stride, w, x, a0, a1);
   if (row < nr) store_y(y, row, alpha, a0, beta);
   if (row + 1 < nr) store_y(y, row + 1, alpha, a1, beta);
 }
@@ -175,15 +189,17 @@ __global__ __launch_bounds__(kBlock) void k_hll_units_combine(int nr, int hksz, 
 // issue together (slots past the row's width read its last slot, product
 // dropped, as in slot_group); each slot is loaded once and used for every
 // column, each column accumulating in slot order.
-template <typename T, int NC, bool XCD>
+template <typename T, int NC, bool IDX16>
 __global__ __launch_bounds__(kBlock) void k_hll_spmm(int nr, int hksz, int nlog,
                                                      const int* __restrict__ hkoffs,
                                                      const int* __restrict__ ja,
+                                                     const short* __restrict__ res16,
+                                                     const int* __restrict__ base16,
                                                      const T* __restrict__ val,
                                                      const T* __restrict__ x, int ldx,
                                                      T alpha, T beta, T* __restrict__ y,
-                                                     int ldy, int ncols, int xchunk) {
-  const int lb = XCD ? xcd_block_chunk(nlog, xchunk) : (int)blockIdx.x;
+                                                     int ldy, int ncols, BlockOrder ord) {
+  const int lb = order_block(ord, nlog);
   if (lb >= nlog) return;
   const int row = lb * kBlock + threadIdx.x;
   const int r = row < nr ? row : nr - 1;
@@ -192,6 +208,8 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmm(int nr, int hksz, int nlog,
   const int off0 = hkoffs[hk];
   const int w = (hkoffs[hk + 1] - off0) / hksz;
   const int* pj = ja + off0 + lane;
+  const short* pe = res16 + off0 + lane;
+  const int* pb = base16 + off0 / hksz;
   const T* pv = val + off0 + lane;
   T acc[NC];
 #pragma unroll
@@ -202,7 +220,10 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmm(int nr, int hksz, int nlog,
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       const int jj = j + k < w ? j + k : w - 1;
-      c[k] = ld_stream(pj + (size_t)jj * hksz) - 1;
+      if (IDX16)
+        c[k] = r + pb[jj] + (int)ld_stream(pe + (size_t)jj * hksz);
+      else
+        c[k] = ld_stream(pj + (size_t)jj * hksz) - 1;
       v[k] = ld_stream(pv + (size_t)jj * hksz);
     }
 #pragma unroll
@@ -509,6 +530,43 @@ __global__ __launch_bounds__(kBlock) void k_tmv_run(int rows, const int* __restr
   y[c] = r;
 }
 
+// 16-bit column offsets of a rows-in-order HLL matrix (HllDev::res16 /
+// base16), one thread per hack: each slot column's base is the midpoint of
+// its lanes' offsets (column - row), so the residuals fit in 16 bits when the
+// offsets of one slot column span at most 65535; *bad is set otherwise (the
+// matrix then keeps using ja).
+__global__ __launch_bounds__(kBlock) void k_hll_idx16(int hksz, int noffs,
+                                                      const int* __restrict__ hkoffs,
+                                                      const int* __restrict__ ja,
+                                                      short* __restrict__ res,
+                                                      int* __restrict__ base, int* bad) {
+  const int h = blockIdx.x * kBlock + threadIdx.x;
+  if (h >= noffs) return;
+  const int off0 = hkoffs[h];
+  const int w = (hkoffs[h + 1] - off0) / hksz;
+  const int r0 = h * hksz;
+  for (int j = 0; j < w; j++) {
+    const size_t q = (size_t)off0 + (size_t)j * hksz;
+    int lo = 0x7fffffff, hi = -0x7fffffff - 1;
+    for (int l = 0; l < hksz; l++) {
+      const int d = ja[q + l] - 1 - (r0 + l);
+      lo = d < lo ? d : lo;
+      hi = d > hi ? d : hi;
+    }
+    // ceiling midpoint: residuals in [-32768, 32767] for a span up to 65535
+    const int b = (int)(((int64_t)lo + (int64_t)hi + 1) >> 1);
+    bool fits = true;
+    for (int l = 0; l < hksz; l++) {
+      const int64_t d = (int64_t)ja[q + l] - 1 - (r0 + l) - b;
+      const bool ok = d >= -32768 && d <= 32767;
+      fits = fits && ok;
+      res[q + l] = (short)(ok ? d : 0);
+    }
+    if (!fits) atomicOr(bad, 1);
+    base[off0 / hksz + j] = b;
+  }
+}
+
 // ------------------------------------------------------------------ fills
 // psi_d_convert_hll_from_coo + psi_d_xtr_ell_from_coo, CPU padding rule:
 // one thread per slot row; rows past nr in the last hack get ja = nr, val 0;
@@ -680,25 +738,46 @@ static int check_vectors(const MultiVec* x, const MultiVec* y, int elem_type, in
   return PSB_HIP_SUCCESS;
 }
 
+// 16-bit column offsets in the HLL products when the matrix has them
+// (psb_hip_hll_set_index16 switches them off for comparisons)
+static int g_hll_idx16 = 1;
+bool hll_use_idx16(const HllDev* m) { return g_hll_idx16 && m->res16 && !m->perm; }
+
+// The block order of an HLL product over blocks of rows_per_block rows: the
+// matrix's plane-tile order when it has one, else chunked XCD order (xcd) or
+// launch order.
+static BlockOrder hll_order(const HllDev* m, bool xcd, int rows_per_block, int nlog) {
+  if (m->plane_rows > 0 && m->plane_tb > 0)
+    return plane_order(nlog, rows_per_block, m->plane_rows, m->plane_tb);
+  BlockOrder o;
+  if (xcd) {
+    o.mode = 1;
+    o.xchunk = g_hll_xchunk;
+  }
+  return o;
+}
+
 template <typename T, typename TV, int HK>
 static void hll_launch_hk(const HllDev* m, bool pair, bool xcd, const TV* val, const T* x,
                           T alpha, T beta, T* y) {
   hipStream_t s = ctx().stream;
-  const int nlog = grid_for(m->rows, pair ? 2 * kBlock : kBlock);
-  const int xc = g_hll_xchunk;
-  dim3 grid(xcd ? xcd_chunk_grid(nlog, xc) : nlog), blk(kBlock);
-  if (pair && xcd)
-    hipLaunchKernelGGL((k_hll_spmv_pair<T, TV, HK, true>), grid, blk, 0, s, m->rows, m->hksz, nlog,
-                       xc, m->hkoffs, m->ja, val, x, alpha, beta, y);
+  const int rpb = pair ? 2 * kBlock : kBlock;
+  const int nlog = grid_for(m->rows, rpb);
+  const BlockOrder o = hll_order(m, xcd, rpb, nlog);
+  dim3 grid(order_grid(o, nlog)), blk(kBlock);
+  const bool i16 = hll_use_idx16(m);
+  if (pair && i16)
+    hipLaunchKernelGGL((k_hll_spmv_pair<T, TV, HK, true>), grid, blk, 0, s, m->rows, m->hksz,
+                       nlog, o, m->hkoffs, m->ja, m->res16, m->base16, val, x, alpha, beta, y);
   else if (pair)
     hipLaunchKernelGGL((k_hll_spmv_pair<T, TV, HK, false>), grid, blk, 0, s, m->rows, m->hksz,
-                       nlog, 0, m->hkoffs, m->ja, val, x, alpha, beta, y);
-  else if (xcd)
-    hipLaunchKernelGGL((k_hll_spmv<T, TV, HK, true>), grid, blk, 0, s, m->rows, m->hksz, nlog,
-                       xc, m->hkoffs, m->ja, val, x, alpha, beta, y);
+                       nlog, o, m->hkoffs, m->ja, m->res16, m->base16, val, x, alpha, beta, y);
+  else if (i16)
+    hipLaunchKernelGGL((k_hll_spmv<T, TV, HK, true>), grid, blk, 0, s, m->rows, m->hksz, nlog, o,
+                       m->hkoffs, m->ja, m->res16, m->base16, val, x, alpha, beta, y);
   else
-    hipLaunchKernelGGL((k_hll_spmv<T, TV, HK, false>), grid, blk, 0, s, m->rows, m->hksz, nlog,
-                       0, m->hkoffs, m->ja, val, x, alpha, beta, y);
+    hipLaunchKernelGGL((k_hll_spmv<T, TV, HK, false>), grid, blk, 0, s, m->rows, m->hksz, nlog, o,
+                       m->hkoffs, m->ja, m->res16, m->base16, val, x, alpha, beta, y);
 }
 
 // Variant: 1 one row per lane, 2 two adjacent rows per lane (even hksz), 0
@@ -759,22 +838,24 @@ static int hll_spmm_launch(const HllDev* m, const T* val, const T* x, int ldx, T
   // per XCD measured 2-6% faster than launch order for k = 8, 16 and 32
   // columns on the L=200 stencil, chunks of 16 blocks another 2-5% (485.7 vs
   // 496.4-521.8 us for k = 8, 1782.8 vs 1873.4-1892.6 us for k = 32)
-  const bool xcd = true;
   const int nlog = grid_for(m->rows, kBlock);
-  const int xch = g_hll_xchunk;
-  dim3 grid(xcd ? xcd_chunk_grid(nlog, xch) : nlog), blk(kBlock);
+  const BlockOrder o = hll_order(m, true, kBlock, nlog);
+  const bool i16 = hll_use_idx16(m);
+  dim3 grid(order_grid(o, nlog)), blk(kBlock);
   for (int c0 = 0; c0 < ncols; c0 += 32) {
     const int nc = ncols - c0 < 32 ? ncols - c0 : 32;
     const T* xc = x + (size_t)c0 * ldx;
     T* yc = y + (size_t)c0 * ldy;
-#define PSB_SPMM_CASE(NC)                                                                      \
-  do {                                                                                         \
-    if (xcd)                                                                                   \
-      hipLaunchKernelGGL((k_hll_spmm<T, NC, true>), grid, blk, 0, s, m->rows, m->hksz, nlog,   \
-                         m->hkoffs, m->ja, val, xc, ldx, alpha, beta, yc, ldy, nc, xch);       \
-    else                                                                                       \
-      hipLaunchKernelGGL((k_hll_spmm<T, NC, false>), grid, blk, 0, s, m->rows, m->hksz, nlog,  \
-                         m->hkoffs, m->ja, val, xc, ldx, alpha, beta, yc, ldy, nc, 0);         \
+#define PSB_SPMM_CASE(NC)                                                                     \
+  do {                                                                                        \
+    if (i16)                                                                                  \
+      hipLaunchKernelGGL((k_hll_spmm<T, NC, true>), grid, blk, 0, s, m->rows, m->hksz, nlog,  \
+                         m->hkoffs, m->ja, m->res16, m->base16, val, xc, ldx, alpha, beta, yc, \
+                         ldy, nc, o);                                                         \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_hll_spmm<T, NC, false>), grid, blk, 0, s, m->rows, m->hksz, nlog, \
+                         m->hkoffs, m->ja, m->res16, m->base16, val, xc, ldx, alpha, beta, yc, \
+                         ldy, nc, o);                                                         \
   } while (0)
     if (nc > 16) PSB_SPMM_CASE(32);
     else if (nc > 8) PSB_SPMM_CASE(16);
@@ -1067,7 +1148,15 @@ static int csr_build_slices(CsrDev* m, const int* irp, const int* ja, const void
 
 // Back to rows in order (the arrays written next are in the reference's
 // layout): drop the permutation and the work-unit plan.
+static void hll_drop_idx16(HllDev* m) {
+  hipFree(m->res16);
+  hipFree(m->base16);
+  m->res16 = nullptr;
+  m->base16 = nullptr;
+}
+
 static void hll_drop_sorted(HllDev* m) {
+  hll_drop_idx16(m);   // ja is about to change
   hipFree(m->perm);
   hipFree(m->units);
   hipFree(m->splits);
@@ -1077,6 +1166,35 @@ static void hll_drop_sorted(HllDev* m) {
   m->splits = nullptr;
   m->part = nullptr;
   m->nunits = m->nsplit = 0;
+}
+
+// Build the 16-bit column offsets of a rows-in-order matrix; when a slot's
+// offset does not fit (or memory is short) the matrix keeps only ja.
+static int hll_build_idx16(HllDev* m) {
+  hll_drop_idx16(m);
+  if (m->perm || m->allocsize <= 0 || m->rows <= 0) return PSB_HIP_SUCCESS;
+  hipStream_t s = ctx().stream;
+  int* bad = nullptr;
+  hipError_t e = hipMalloc((void**)&m->res16, sizeof(short) * (size_t)m->allocsize);
+  if (e == hipSuccess)
+    e = hipMalloc((void**)&m->base16, sizeof(int) * (size_t)(m->allocsize / m->hksz + 1));
+  if (e == hipSuccess) e = hipMalloc((void**)&bad, sizeof(int));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    hipFree(bad);
+    hll_drop_idx16(m);
+    return PSB_HIP_SUCCESS;
+  }
+  int hbad = 0;
+  PSB_HIP_CHECK(hipMemsetAsync(bad, 0, sizeof(int), s));
+  hipLaunchKernelGGL(k_hll_idx16, dim3((unsigned)grid_for(m->noffs, kBlock)), dim3(kBlock), 0, s,
+                     m->hksz, m->noffs, m->hkoffs, m->ja, m->res16, m->base16, bad);
+  PSB_HIP_LAUNCH_CHECK();
+  PSB_HIP_CHECK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  PSB_HIP_CHECK(hipStreamSynchronize(s));
+  hipFree(bad);
+  if (hbad) hll_drop_idx16(m);
+  return PSB_HIP_SUCCESS;
 }
 
 // Row-sorted storage: copy perm (1-based row of each position, a bijection
@@ -1187,6 +1305,8 @@ void psb_hip_hll_free(void* dmat) {
   hipFree(m->units);
   hipFree(m->splits);
   hipFree(m->part);
+  hipFree(m->res16);
+  hipFree(m->base16);
   delete m;
 }
 
@@ -1216,7 +1336,7 @@ int psb_hip_hll_write(void* dmat, const void* val, const int* ja, const int* hko
     PSB_HIP_CHECK(hipMemcpyAsync(m->idiag, idiag, (size_t)m->rows * 4, hipMemcpyHostToDevice, s));
   }
   PSB_HIP_CHECK(hipStreamSynchronize(s));
-  return PSB_HIP_SUCCESS;
+  return hll_build_idx16(m);
 }
 
 int psb_hip_hll_read(void* dmat, void* val, int* ja, int* hkoffs, int* irn, int* idiag) {
@@ -1315,7 +1435,7 @@ int psb_hip_hll_copy_coo(int nr, int nc, int nza, int hksz, int noffs, int isz, 
   PSB_HIP_CHECK(hipStreamSynchronize(s));
   hipFree(d_irn); hipFree(d_hoffs); hipFree(d_idisp); hipFree(d_ja); hipFree(d_val);
   m->nzeros = nza;
-  return PSB_HIP_SUCCESS;
+  return hll_build_idx16(m);
 }
 
 int psb_hip_hll_copy_coo_sorted(int nr, int nc, int nza, int hksz, int noffs, int isz,
@@ -1529,6 +1649,28 @@ int psb_hip_hll_tune(void* dmat, void* xv, void* yv, int reps, int* best, float*
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return rc;
+}
+
+int psb_hip_hll_set_index16(int on) {
+  const int old = g_hll_idx16;
+  if (on == 0 || on == 1) g_hll_idx16 = on;
+  return old;
+}
+
+int psb_hip_hll_index16(void* dmat) {
+  const HllDev* m = (const HllDev*)dmat;
+  return m && m->res16 ? 1 : 0;
+}
+
+int psb_hip_hll_set_plane(void* dmat, int period_rows, int tile_blocks) {
+  HllDev* m = (HllDev*)dmat;
+  if (!m || period_rows < 0 || tile_blocks < 0)
+    return set_error(PSB_HIP_INVALID, __func__, "matrix, period or tile");
+  if (m->perm && period_rows > 0)
+    return set_error(PSB_HIP_UNSUPPORTED, __func__, "row-sorted HLL");
+  m->plane_rows = tile_blocks > 0 ? period_rows : 0;
+  m->plane_tb = period_rows > 0 ? tile_blocks : 0;
+  return PSB_HIP_SUCCESS;
 }
 
 int psb_hip_hll_set_variant(int variant) {
