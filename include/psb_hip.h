@@ -336,6 +336,20 @@ int psb_hip_hll_units(void *dmat, int *nunits, int *nsplit);
    host array written): install the permutation perm(rows) and the product's
    work-unit plan (the to_gpu of a row-sorted matrix built on the host). */
 int psb_hip_hll_set_perm(void *dmat, const int *perm, const int *hkoffs);
+/* Column slices of row-sorted matrices (no reference counterpart): when
+   psb_hip_hll_copy_coo_sorted writes a matrix whose x is over 12 MiB and
+   whose entries mostly lie more than cols/16 from the diagonal (scattered
+   columns), the entries are also stored split into consecutive column
+   ranges of about 8 MiB (FP64) or 5.6 MB (FP32) of x, each a row-sorted
+   HLL matrix of its own, and
+   the product runs them one after another accumulating into y, so each
+   slice's x range stays in L2 (the rule of the CSR slices,
+   psb_hip_csr_set_slices).  Rows then follow the per-row error bound.
+   psb_hip_hll_set_slices: -1 automatic (default), 0 or 1 none, >= 2 that
+   many, for matrices written afterwards; returns the previous value.
+   psb_hip_hll_slices: the matrix's slice count (0 when unsliced). */
+int psb_hip_hll_set_slices(int slices);
+int psb_hip_hll_slices(void *dmat);
 /* spmvHllDevice{Double,Float} (cuda/hlldev.c:185-207): y = alpha*A*x +
    beta*y for every column of x/y (the reference computes column 1 only),
    y not read when beta == 0.  x, y multivectors of the matrix type. */

@@ -135,6 +135,8 @@ _SIGS = {
     "psb_hip_hll_spmv_single": (i32, [vp, f64, vp, f64, vp]),
     "psb_hip_hll_index16": (i32, [vp]),
     "psb_hip_hll_set_index16": (i32, [i32]),
+    "psb_hip_hll_set_slices": (i32, [i32]),
+    "psb_hip_hll_slices": (i32, [vp]),
     "psb_hip_hll_set_plane": (i32, [vp, i32, i32]),
     "psb_hip_hll_set_variant": (i32, [i32]),
     "psb_hip_hll_tune": (i32, [vp, vp, vp, i32, pi32, C.POINTER(f32)]),

@@ -379,6 +379,11 @@ struct HllDev {
   // products read these 2-byte residuals instead of the 4-byte ja.
   short* res16 = nullptr;   // allocsize
   int* base16 = nullptr;    // allocsize / hksz
+  // row-sorted matrices with scattered columns and a large x: the same rows
+  // restricted to consecutive column ranges, each a row-sorted HLL matrix of
+  // its own; the product runs them one after another accumulating into y
+  int nslices = 0;
+  HllDev** slices = nullptr;
 };
 // whether the products of m read its 16-bit column offsets (spmv.hip)
 bool hll_use_idx16(const HllDev* m);

@@ -46,6 +46,15 @@ static int g_csr_variant = 0;
 // 916 us against 1041 us unsliced: a slice's x range stays in the XCDs' L2
 // while its rows run, so fewer of the random gathers miss.
 static int g_csr_slices = -1;
+// Column slices of row-sorted HLL matrices (psb_hip_hll_copy_coo_sorted):
+// the CSR rule for when to slice, slices of about kHllSliceBytes of x (FP64)
+// or kHllSliceBytes32 (FP32); -1 automatic, 0 or 1 none, >= 2 that many.  On
+// the n = 4M power-law matrix (x 32 MB FP64) 3, 4, 5, 6 and 8 slices
+// measured 894, 866-869, 908, 1003 and 1241 us against 1008 us unsliced;
+// FP32 (x 16 MB) 2, 3 and 4 slices 621, 571 and 633 us against 785 us.
+static int g_hll_slices = -1;
+constexpr size_t kHllSliceBytes = (size_t)8 << 20;
+constexpr size_t kHllSliceBytes32 = 5600000;
 constexpr size_t kCsrSliceMinX = (size_t)12 << 20;
 constexpr size_t kCsrSliceBytes = 6400000;
 constexpr int kCsrMaxSlices = 16;
@@ -806,9 +815,29 @@ static int hll_units_launch(const HllDev* m, const TV* val, const T* x, T alpha,
   return PSB_HIP_SUCCESS;
 }
 
+// Row-sorted storage with column slices: y = alpha A_0 x + beta y, then
+// y = alpha A_s x + y slice after slice (the matrix's own values only; the
+// FP32 copy of a double matrix runs unsliced).  Each slice is sorted by its
+// own row lengths: slices sharing the whole matrix's row order and adding
+// their sums by position measured 5-10% slower (their rows' lengths in a
+// slice vary inside a hack, so the lanes of a wave diverge).
+template <typename T, typename TV>
+static int hll_units_sliced(const HllDev* m, const TV* val, const T* x, T alpha, T beta, T* y) {
+  if (m->nslices > 0 && (const void*)val == m->val) {
+    for (int s = 0; s < m->nslices; s++) {
+      const HllDev* c = m->slices[s];
+      const int rc =
+          hll_units_launch<T, TV>(c, (const TV*)c->val, x, alpha, s == 0 ? beta : T(1), y);
+      if (rc) return rc;
+    }
+    return PSB_HIP_SUCCESS;
+  }
+  return hll_units_launch<T, TV>(m, val, x, alpha, beta, y);
+}
+
 template <typename T, typename TV>
 static int hll_launch(const HllDev* m, const TV* val, const T* x, T alpha, T beta, T* y) {
-  if (m->perm) return hll_units_launch<T, TV>(m, val, x, alpha, beta, y);
+  if (m->perm) return hll_units_sliced<T, TV>(m, val, x, alpha, beta, y);
   const int v = m->variant != 0 ? m->variant : g_hll_variant;
   const int variant = v & 3;
   const bool xcd = (v & 4) != 0 || v == 0;
@@ -827,7 +856,7 @@ static int hll_spmm_launch(const HllDev* m, const T* val, const T* x, int ldx, T
   if (m->perm) {
     // row-sorted storage: one product per column (the matrix is read k times)
     for (int c = 0; c < ncols; c++) {
-      int rc = hll_units_launch<T, T>(m, val, x + (size_t)c * ldx, alpha, beta,
+      int rc = hll_units_sliced<T, T>(m, val, x + (size_t)c * ldx, alpha, beta,
                                       y + (size_t)c * ldy);
       if (rc) return rc;
     }
@@ -1155,8 +1184,17 @@ static void hll_drop_idx16(HllDev* m) {
   m->base16 = nullptr;
 }
 
+static void hll_drop_slices(HllDev* m) {
+  if (!m->slices) return;
+  for (int s = 0; s < m->nslices; s++) psb_hip_hll_free(m->slices[s]);
+  delete[] m->slices;
+  m->slices = nullptr;
+  m->nslices = 0;
+}
+
 static void hll_drop_sorted(HllDev* m) {
   hll_drop_idx16(m);   // ja is about to change
+  hll_drop_slices(m);
   hipFree(m->perm);
   hipFree(m->units);
   hipFree(m->splits);
@@ -1307,6 +1345,7 @@ void psb_hip_hll_free(void* dmat) {
   hipFree(m->part);
   hipFree(m->res16);
   hipFree(m->base16);
+  hll_drop_slices(m);
   delete m;
 }
 
@@ -1438,11 +1477,26 @@ int psb_hip_hll_copy_coo(int nr, int nc, int nza, int hksz, int noffs, int isz, 
   return hll_build_idx16(m);
 }
 
+static int hll_copy_sorted_impl(int nr, int nc, int nza, int hksz, int noffs, int isz,
+                                const int* perm, const int* irn_pos, const int* hoffs,
+                                const int* idisp, const int* ja, const void* val, HllDev* m);
+static int hll_build_slices(HllDev* m, int nc, const int* idisp, const int* ja, const void* val);
+
 int psb_hip_hll_copy_coo_sorted(int nr, int nc, int nza, int hksz, int noffs, int isz,
                                 const int* perm, const int* irn_pos, const int* hoffs,
                                 const int* idisp, const int* ja, const void* val, void* dmat) {
   PSB_ENSURE_INIT();
   HllDev* m = (HllDev*)dmat;
+  if (!m || !perm || !irn_pos || !hoffs || !idisp) return PSB_HIP_INVALID;
+  int rc = hll_copy_sorted_impl(nr, nc, nza, hksz, noffs, isz, perm, irn_pos, hoffs, idisp, ja,
+                                val, m);
+  if (rc) return rc;
+  return hll_build_slices(m, nc, idisp, ja, val);
+}
+
+static int hll_copy_sorted_impl(int nr, int nc, int nza, int hksz, int noffs, int isz,
+                                const int* perm, const int* irn_pos, const int* hoffs,
+                                const int* idisp, const int* ja, const void* val, HllDev* m) {
   if (!m || !perm || !irn_pos || !hoffs || !idisp) return PSB_HIP_INVALID;
   if (m->rows != nr || m->hksz != hksz || m->noffs != noffs || m->allocsize != isz)
     return set_error(PSB_HIP_INVALID, __func__, "matrix shape mismatch");
@@ -1509,6 +1563,76 @@ int psb_hip_hll_copy_coo_sorted(int nr, int nc, int nza, int hksz, int noffs, in
   hipFree(d_irn); hipFree(d_hoffs); hipFree(d_idisp); hipFree(d_ja); hipFree(d_val);
   m->nzeros = nza;
   if (m->val32) refresh_single(m);
+  return PSB_HIP_SUCCESS;
+}
+
+// The rule of csr_slice_count on the host arrays of a row-sorted copy: x
+// over kCsrSliceMinX bytes and most sampled entries farther than cols/16 from
+// the diagonal; slices of about kCsrSliceBytes of x.
+static int hll_slice_count(const HllDev* m, int nc, const int* idisp, const int* ja) {
+  if (g_hll_slices >= 2) return std::min(g_hll_slices, std::max(nc, 1));
+  if (g_hll_slices >= 0 || m->nzeros == 0 || m->rows == 0) return 0;
+  const size_t xb = (size_t)nc * elem_size(m->elem_type);
+  if (xb <= kCsrSliceMinX) return 0;
+  const int step = std::max(1, m->rows / 65536);
+  const int64_t dist = nc / 16;
+  int64_t far = 0, tot = 0;
+  for (int i = 0; i < m->rows; i += step)
+    for (int k = idisp[i]; k < idisp[i + 1]; k++) {
+      const int64_t d = (int64_t)ja[k] - 1 - i;
+      far += (d > dist || d < -dist) ? 1 : 0;
+      tot++;
+    }
+  if (far * 2 <= tot) return 0;
+  const size_t sb = elem_size(m->elem_type) == 8 ? kHllSliceBytes : kHllSliceBytes32;
+  return (int)std::min<size_t>(kCsrMaxSlices, (xb + sb - 1) / sb);
+}
+
+// Slice s: the entries of columns s*w+1 .. (s+1)*w of every row (row order
+// and CSR order kept), sorted by its own row lengths over the whole matrix.
+static int hll_build_slices(HllDev* m, int nc, const int* idisp, const int* ja, const void* val) {
+  const int S = hll_slice_count(m, nc, idisp, ja);
+  if (S < 2) return PSB_HIP_SUCCESS;
+  const size_t es = elem_size(m->elem_type);
+  const int nr = m->rows, hksz = m->hksz, noffs = m->noffs;
+  const int w = (nc + S - 1) / S;
+  m->slices = new HllDev*[S]();
+  m->nslices = S;
+  std::vector<int> sidisp(nr + 1), cnt(nr > 0 ? nr : 1), perm(nr > 0 ? nr : 1),
+      cnt_pos(nr > 0 ? nr : 1), hoffs(noffs + 1), sja;
+  std::vector<char> sval;
+  for (int s = 0; s < S; s++) {
+    const int lo = s * w + 1, hi = std::min(nc, (s + 1) * w);
+    sja.clear();
+    sval.clear();
+    sidisp[0] = 0;
+    for (int i = 0; i < nr; i++) {
+      for (int k = idisp[i]; k < idisp[i + 1]; k++)
+        if (ja[k] >= lo && ja[k] <= hi) {
+          sja.push_back(ja[k]);
+          const char* v = (const char*)val + (size_t)k * es;
+          sval.insert(sval.end(), v, v + es);
+        }
+      sidisp[i + 1] = (int)sja.size();
+      cnt[i] = sidisp[i + 1] - sidisp[i];
+    }
+    int64_t isz = 0;
+    int rc = psb_hip_hll_sort_rows(nr, hksz, 0, cnt.data(), perm.data(), cnt_pos.data(),
+                                   hoffs.data(), &isz);
+    if (!rc && isz > 0x7fffffffLL) rc = set_error(PSB_HIP_UNSUPPORTED, __func__, "slice size");
+    void* sub = nullptr;
+    if (!rc) rc = psb_hip_hll_alloc(&sub, hksz, nr, (int)sja.size(), (int)isz, m->elem_type, 1);
+    if (!rc) {
+      m->slices[s] = (HllDev*)sub;
+      rc = hll_copy_sorted_impl(nr, nc, (int)sja.size(), hksz, noffs, (int)isz, perm.data(),
+                                cnt_pos.data(), hoffs.data(), sidisp.data(), sja.data(),
+                                sval.data(), (HllDev*)sub);
+    }
+    if (rc) {
+      hll_drop_slices(m);
+      return rc;
+    }
+  }
   return PSB_HIP_SUCCESS;
 }
 
@@ -1649,6 +1773,17 @@ int psb_hip_hll_tune(void* dmat, void* xv, void* yv, int reps, int* best, float*
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return rc;
+}
+
+int psb_hip_hll_set_slices(int slices) {
+  const int old = g_hll_slices;
+  if (slices >= -1 && slices <= kCsrMaxSlices) g_hll_slices = slices;
+  return old;
+}
+
+int psb_hip_hll_slices(void* dmat) {
+  const HllDev* m = (const HllDev*)dmat;
+  return m ? m->nslices : -1;
 }
 
 int psb_hip_hll_set_index16(int on) {

@@ -150,6 +150,8 @@ def test_fullsize_powerlaw_n4M(gpu, orc, powerlaw4m):
         m = _dev_matrix(f, mold)
         if "sorted" in name:
             assert m.a.isz <= 1.2 * f.nz
+            # x is 32 MB with scattered columns: the automatic column slices
+            assert N.fn("psb_hip_hll_slices")(m.a.dev) >= 2
         yv = P.vect_from(np.zeros(f.nr))
         P.psb_spmm(1.0, m, xv, 0.0, yv)
         _check_bound(yv.get_vect(), yref, absax, nnz_row, np.finfo(np.float64).eps)

@@ -232,3 +232,53 @@ def test_sorted_diag_scal_and_solve(gpu, orc):
     assert its[0] == its[1]
     srt.free()
     ref.free()
+
+
+@pytest.mark.parametrize("slices", [2, 3, 5])
+@pytest.mark.parametrize("single", [False, True])
+def test_sorted_column_slices(gpu, orc, slices, single):
+    """Row-sorted HLL with column slices (forced; automatic only for x over
+    12 MiB with scattered columns): each slice is a row-sorted matrix of its
+    own over a column range, the product adds the slices' sums; every row
+    meets the per-row bound against the oracle, for alpha / beta and
+    several columns."""
+    from psblas3_amd import _native as N
+    _say("case sorted slices", slices, "fp32" if single else "fp64")
+    coo = orc.gen_powerlaw(20000, cap=1000, seed=5)
+    f = orc.fix_coo(coo)
+    old = N.fn("psb_hip_hll_set_slices")(slices)
+    try:
+        mold = P.psb_s_hip_hlg_sparse_mat(64, sort_window=0) if single \
+            else P.psb_d_hip_hlg_sparse_mat(64, sort_window=0)
+        a = _mat(coo, mold)
+    finally:
+        N.fn("psb_hip_hll_set_slices")(old)
+    assert N.fn("psb_hip_hll_slices")(a.a.dev) == slices
+    dt = np.float32 if single else np.float64
+    eps = np.finfo(dt).eps * (2.0 if single else 1.0)
+    vm = P.psb_s_hip_vect() if single else P.psb_d_hip_vect()
+    x = orc.uniform_pm1(8, f.nc)
+    y0 = orc.uniform_pm1(9, f.nr)
+    irp, ja, val = orc.csr_from_coo(f)
+    for alpha, beta in ((1.0, 0.0), (-1.5, 0.25)):
+        xv = P.vect_from(x.astype(dt), mold=vm)
+        yv = P.vect_from(y0.astype(dt), mold=vm)
+        P.psb_spmm(alpha, a, xv, beta, yv)
+        yref = orc.csr_spmv(irp, ja, val, x, alpha, beta, y0.copy())
+        bound = _row_bound(f, x, alpha, beta, y0, eps)
+        assert np.all(np.abs(yv.get_vect().astype(np.float64) - yref) <= bound)
+    if not single:
+        # three columns at once: each column as the single product
+        X = np.stack([orc.uniform_pm1(30 + c, f.nc) for c in range(3)], axis=1)
+        xk = P.psb_d_hip_vect().bld(X)
+        yk = P.psb_d_hip_vect().all(f.nr, 3)
+        P.psb_spmm(1.0, a, xk, 0.0, yk)
+        Y = yk.get_vect()
+        for c in range(3):
+            ys = P.vect_from(np.zeros(f.nr))
+            P.psb_spmm(1.0, a, P.vect_from(np.ascontiguousarray(X[:, c])), 0.0, ys)
+            assert np.array_equal(Y[:, c], ys.get_vect())
+    # the stored (unsliced) arrays are unchanged: read back, they are the oracle's CSR
+    irp2, ja2, val2 = a.a.to_csr()
+    assert np.array_equal(irp2, irp) and np.array_equal(ja2, ja)
+    a.free()
