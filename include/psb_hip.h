@@ -397,11 +397,15 @@ int psb_hip_hll_set_plane(void *dmat, int period_rows, int tile_blocks);
    residual plus one 32-bit base per hack slot column, and the products
    (single and multi-RHS) read those instead of ja: 2 bytes of index traffic
    per slot instead of 4, the same columns, the same results bit for bit.
-   ja stays as written (read back, transposed products, scaling).
-   psb_hip_hll_index16: 1 when the matrix has them.  psb_hip_hll_set_index16
+   ja stays as written (read back, transposed products, scaling).  ELL
+   matrices (psb_hip_ell_write, psb_hip_ell_copy_coo) get the same, with a
+   base per slot column of each group of 64 rows.
+   psb_hip_hll_index16: 1 when the matrix has them (psb_hip_ell_index16 for
+   ELL).  psb_hip_hll_set_index16
    (0 / 1, default 1) switches their use in products on or off for every
    matrix; returns the previous setting. */
 int psb_hip_hll_index16(void *dmat);
+int psb_hip_ell_index16(void *dmat);
 int psb_hip_hll_set_index16(int on);
 /* XCD block order of the HLL (variants 0 and + 4), ELL, multi-RHS HLL and
    row-block CSR products: logical blocks dealt to the XCDs in chunks of

@@ -133,6 +133,7 @@ _SIGS = {
     "psb_hip_hll_spmv": (i32, [vp, f64, vp, f64, vp]),
     "psb_hip_hll_make_single": (i32, [vp]),
     "psb_hip_hll_spmv_single": (i32, [vp, f64, vp, f64, vp]),
+    "psb_hip_ell_index16": (i32, [vp]),
     "psb_hip_hll_index16": (i32, [vp]),
     "psb_hip_hll_set_index16": (i32, [i32]),
     "psb_hip_hll_set_slices": (i32, [i32]),

@@ -101,12 +101,20 @@ struct EllView {
   int nr, pitch, nzm;
   const int* ja;
   const T* val;
+  const short* res16;   // 16-bit column offsets (EllDev::res16 / base16) or null
+  const int* base16;
   template <int R>
   __device__ __forceinline__ void rows(int row0, const T* __restrict__ x, T (&ax)[R]) const {
     const int last = R == 1 ? nr - 1 : ((nr - 1) & ~1);
     const int r = row0 < last ? row0 : last;
-    if constexpr (R == 1) ax[0] = slot_dot<T, T>(ja + r, val + r, pitch, nzm, x);
-    else slot_dot_pair<T, T>(ja + r, val + r, pitch, nzm, x, ax[0], ax[1]);
+    if (res16) {
+      const int* pb = base16 + (size_t)(r >> 6) * nzm;
+      if constexpr (R == 1) ax[0] = slot_dot16<T, T>(res16 + r, pb, val + r, pitch, nzm, r, x);
+      else slot_dot16_pair<T, T>(res16 + r, pb, val + r, pitch, nzm, r, x, ax[0], ax[1]);
+    } else {
+      if constexpr (R == 1) ax[0] = slot_dot<T, T>(ja + r, val + r, pitch, nzm, x);
+      else slot_dot_pair<T, T>(ja + r, val + r, pitch, nzm, x, ax[0], ax[1]);
+    }
   }
 };
 
@@ -691,6 +699,9 @@ static Mats<T> make_views(Solver& s) {
     m.ell.nzm = e->nzm;
     m.ell.ja = e->ja;
     m.ell.val = (const T*)e->val;
+    const bool i16 = ell_use_idx16(e);
+    m.ell.res16 = i16 ? e->res16 : nullptr;
+    m.ell.base16 = i16 ? e->base16 : nullptr;
     s.rpl = sizeof(T) == 8 && e->pitch % 2 == 0 ? 2 : 1;
   } else {
     s.rpl = 1;
@@ -746,7 +757,7 @@ static int restart(Solver& s, const Mats<T>& m, const T* b, const T* x, T* r, T*
     if (rc) return rc;
     rc = psb_hip_csr_spmv(s.mat, -1.0, &mx, 1.0, &mr);
     if (rc) return rc;
-    EllView<T> zero{n, n, 0, nullptr, nullptr};
+    EllView<T> zero{n, n, 0, nullptr, nullptr, nullptr, nullptr};
     hipLaunchKernelGGL((k_restart<T, EllView<T>, 1>), dim3(g), blk, 0, s.stream, zero,
                        (const T*)r, x, r, q, d, z, s.st, s.partials, bicg);
   }

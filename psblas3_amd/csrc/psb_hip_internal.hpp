@@ -398,7 +398,14 @@ struct EllDev {
   int* ja;
   int* irn;
   int* idiag;
+  // 16-bit column offsets, as HllDev's over groups of 64 rows: the 0-based
+  // column of slot j of row r is r + base16[(r/64)*nzm + j] + res16[j*pitch + r]
+  // (rows past `rows` are never read and hold 0)
+  short* res16 = nullptr;   // pitch * nzm
+  int* base16 = nullptr;    // ceil(rows/64) * nzm
 };
+// whether the products of an ELL matrix read its 16-bit column offsets
+bool ell_use_idx16(const EllDev* m);
 
 struct CsrDev {
   int elem_type;

@@ -258,9 +258,11 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmm(int nr, int hksz, int nlog,
 }
 
 // ------------------------------------------------------------------ ELL SpMV
-template <typename T, typename TV>
+template <typename T, typename TV, bool IDX16>
 __global__ __launch_bounds__(kBlock) void k_ell_spmv(int nr, int pitch, int nzm, int nlog,
                                                      int xchunk, const int* __restrict__ ja,
+                                                     const short* __restrict__ res16,
+                                                     const int* __restrict__ base16,
                                                      const TV* __restrict__ val,
                                                      const T* __restrict__ x, T alpha, T beta,
                                                      T* __restrict__ y) {
@@ -268,8 +270,45 @@ __global__ __launch_bounds__(kBlock) void k_ell_spmv(int nr, int pitch, int nzm,
   if (lb >= nlog) return;
   const int row = lb * kBlock + threadIdx.x;
   const int r = row < nr ? row : nr - 1;
-  T acc = slot_dot<T, TV>(ja + r, val + r, pitch, nzm, x);
+  T acc;
+  if (IDX16)
+    acc = slot_dot16<T, TV>(res16 + r, base16 + (size_t)(r >> 6) * nzm, val + r, pitch, nzm, r, x);
+  else
+    acc = slot_dot<T, TV>(ja + r, val + r, pitch, nzm, x);
   if (row < nr) store_y(y, row, alpha, acc, beta);
+}
+
+// 16-bit column offsets of an ELL matrix (EllDev::res16 / base16), one thread
+// per group of 64 rows: each slot column's base is the ceiling midpoint of
+// the group's offsets (column - row) over rows < nr; *bad is set when some
+// residual does not fit.
+__global__ __launch_bounds__(kBlock) void k_ell_idx16(int nr, int pitch, int nzm,
+                                                      const int* __restrict__ ja,
+                                                      short* __restrict__ res,
+                                                      int* __restrict__ base, int* bad) {
+  const int g = blockIdx.x * kBlock + threadIdx.x;
+  if (g * 64 >= nr) return;
+  const int r0 = g * 64, r1 = r0 + 64 < nr ? r0 + 64 : nr;
+  const int p1 = r0 + 64 < pitch ? r0 + 64 : pitch;
+  for (int j = 0; j < nzm; j++) {
+    const size_t q = (size_t)j * pitch;
+    int lo = 0x7fffffff, hi = -0x7fffffff - 1;
+    for (int r = r0; r < r1; r++) {
+      const int d = ja[q + r] - 1 - r;
+      lo = d < lo ? d : lo;
+      hi = d > hi ? d : hi;
+    }
+    const int b = (int)(((int64_t)lo + (int64_t)hi + 1) >> 1);
+    bool fits = true;
+    for (int r = r0; r < p1; r++) {
+      const int64_t d = r < r1 ? (int64_t)ja[q + r] - 1 - r - b : 0;
+      const bool ok = d >= -32768 && d <= 32767;
+      fits = fits && ok;
+      res[q + r] = (short)(ok ? d : 0);
+    }
+    if (!fits) atomicOr(bad, 1);
+    base[(size_t)g * nzm + j] = b;
+  }
 }
 
 // ------------------------------------------------------------------ CSR SpMV
@@ -751,6 +790,43 @@ static int check_vectors(const MultiVec* x, const MultiVec* y, int elem_type, in
 // (psb_hip_hll_set_index16 switches them off for comparisons)
 static int g_hll_idx16 = 1;
 bool hll_use_idx16(const HllDev* m) { return g_hll_idx16 && m->res16 && !m->perm; }
+bool ell_use_idx16(const EllDev* m) { return g_hll_idx16 && m->res16; }
+
+static void ell_drop_idx16(EllDev* m) {
+  hipFree(m->res16);
+  hipFree(m->base16);
+  m->res16 = nullptr;
+  m->base16 = nullptr;
+}
+
+// Build the 16-bit column offsets of an ELL matrix (kept only when every
+// residual fits and memory allows).
+static int ell_build_idx16(EllDev* m) {
+  ell_drop_idx16(m);
+  if (m->rows <= 0 || m->nzm <= 0) return PSB_HIP_SUCCESS;
+  hipStream_t s = ctx().stream;
+  const int ng = (m->rows + 63) / 64;
+  int* bad = nullptr;
+  hipError_t e = hipMalloc((void**)&m->res16, sizeof(short) * (size_t)m->pitch * m->nzm);
+  if (e == hipSuccess) e = hipMalloc((void**)&m->base16, sizeof(int) * (size_t)ng * m->nzm);
+  if (e == hipSuccess) e = hipMalloc((void**)&bad, sizeof(int));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    hipFree(bad);
+    ell_drop_idx16(m);
+    return PSB_HIP_SUCCESS;
+  }
+  int hbad = 0;
+  PSB_HIP_CHECK(hipMemsetAsync(bad, 0, sizeof(int), s));
+  hipLaunchKernelGGL(k_ell_idx16, dim3((unsigned)grid_for(ng, kBlock)), dim3(kBlock), 0, s, m->rows,
+                     m->pitch, m->nzm, m->ja, m->res16, m->base16, bad);
+  PSB_HIP_LAUNCH_CHECK();
+  PSB_HIP_CHECK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  PSB_HIP_CHECK(hipStreamSynchronize(s));
+  hipFree(bad);
+  if (hbad) ell_drop_idx16(m);
+  return PSB_HIP_SUCCESS;
+}
 
 // The block order of an HLL product over blocks of rows_per_block rows: the
 // matrix's plane-tile order when it has one, else chunked XCD order (xcd) or
@@ -1792,6 +1868,11 @@ int psb_hip_hll_set_index16(int on) {
   return old;
 }
 
+int psb_hip_ell_index16(void* dmat) {
+  const EllDev* m = (const EllDev*)dmat;
+  return m && m->res16 ? 1 : 0;
+}
+
 int psb_hip_hll_index16(void* dmat) {
   const HllDev* m = (const HllDev*)dmat;
   return m && m->res16 ? 1 : 0;
@@ -1854,6 +1935,7 @@ void psb_hip_ell_free(void* dmat) {
   hipFree(m->ja);
   hipFree(m->irn);
   hipFree(m->idiag);
+  ell_drop_idx16(m);
   delete m;
 }
 
@@ -1894,7 +1976,7 @@ int psb_hip_ell_write(void* dmat, const void* val, const int* ja, int ldv, const
     PSB_HIP_CHECK(hipMemcpyAsync(m->idiag, idiag, (size_t)m->rows * 4, hipMemcpyHostToDevice, s));
   }
   PSB_HIP_CHECK(hipStreamSynchronize(s));
-  return PSB_HIP_SUCCESS;
+  return ell_build_idx16(m);
 }
 
 int psb_hip_ell_read(void* dmat, void* val, int* ja, int ldv, int* irn, int* idiag) {
@@ -1971,7 +2053,7 @@ int psb_hip_ell_copy_coo(int nr, int nc, int nza, const int* irn, const int* idi
   PSB_HIP_CHECK(hipStreamSynchronize(s));
   hipFree(d_irn); hipFree(d_idisp); hipFree(d_ja); hipFree(d_val);
   m->nnz = nza;
-  return PSB_HIP_SUCCESS;
+  return ell_build_idx16(m);
 }
 
 int psb_hip_ell_get_diag(void* dmat, void* dv) {
@@ -2006,18 +2088,22 @@ int psb_hip_ell_spmv(void* dmat, double alpha, void* xv, double beta, void* yv) 
   int nlog = grid_for(m->rows, kBlock);
   const int xc = g_hll_xchunk;
   dim3 grid(xcd_chunk_grid(nlog, xc));
+  const bool i16 = ell_use_idx16(m);
   // every column of x/y (cuda/elldev.c:195-215 loops count_)
   for (int c = 0; c < x->count; c++) {
-    if (m->elem_type == PSB_HIP_TYPE_DOUBLE)
-      hipLaunchKernelGGL((k_ell_spmv<double, double>), grid, dim3(kBlock), 0, s, m->rows, m->pitch,
-                         m->nzm, nlog, xc, m->ja, (const double*)m->val,
-                         (const double*)x->v + (size_t)c * x->pitch, alpha, beta,
-                         (double*)y->v + (size_t)c * y->pitch);
-    else
-      hipLaunchKernelGGL((k_ell_spmv<float, float>), grid, dim3(kBlock), 0, s, m->rows, m->pitch,
-                         m->nzm, nlog, xc, m->ja, (const float*)m->val,
-                         (const float*)x->v + (size_t)c * x->pitch, (float)alpha, (float)beta,
-                         (float*)y->v + (size_t)c * y->pitch);
+#define PSB_ELL_CASE(T, I16)                                                                    \
+  hipLaunchKernelGGL((k_ell_spmv<T, T, I16>), grid, dim3(kBlock), 0, s, m->rows, m->pitch,      \
+                     m->nzm, nlog, xc, m->ja, m->res16, m->base16, (const T*)m->val,            \
+                     (const T*)x->v + (size_t)c * x->pitch, (T)alpha, (T)beta,                 \
+                     (T*)y->v + (size_t)c * y->pitch)
+    if (m->elem_type == PSB_HIP_TYPE_DOUBLE) {
+      if (i16) PSB_ELL_CASE(double, true);
+      else PSB_ELL_CASE(double, false);
+    } else {
+      if (i16) PSB_ELL_CASE(float, true);
+      else PSB_ELL_CASE(float, false);
+    }
+#undef PSB_ELL_CASE
     PSB_HIP_LAUNCH_CHECK();
   }
   return PSB_HIP_SUCCESS;

@@ -148,3 +148,65 @@ def test_plane_order_bitexact(gpu, orc, tile):
         outs.append((yv.get_vect(), yk.get_vect()))
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
     m.free()
+
+
+def _ell_fits(orc, f):
+    """The ELL encoding rule on the oracle's ELL arrays (ldv = nr): every slot
+    column of every group of 64 rows spans at most 65535 columns."""
+    e = orc.ell_from_coo(f)
+    nr, nzm = f.nr, e.nzm
+    ja = e.ja.astype(np.int64).reshape(nzm, -1)[:, :nr]      # column-major, pitch = nr
+    d = ja - 1 - np.arange(nr)[None, :]
+    for g in range(0, nr, 64):
+        blk = d[:, g:g + 64]
+        if ((blk.max(1) - blk.min(1)) > 65535).any():
+            return False
+    return True
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("single", [False, True])
+def test_index16_ell_bitexact(gpu, orc, name, single):
+    _say("case index16 ell", name, "fp32" if single else "fp64")
+    gen, fits = CASES[name]
+    coo = gen(orc)
+    f = orc.fix_coo(coo)
+    assert _ell_fits(orc, f) == fits
+    mold = P.psb_s_hip_elg_sparse_mat() if single else P.psb_d_hip_elg_sparse_mat()
+    m = _mat(coo, mold)
+    assert N.fn("psb_hip_ell_index16")(m.a.dev) == int(fits)
+    dt = np.float32 if single else np.float64
+    vm = P.psb_s_hip_vect() if single else P.psb_d_hip_vect()
+    x = orc.uniform_pm1(3, coo.nc).astype(dt)
+    y0 = orc.uniform_pm1(4, coo.nr).astype(dt)
+    xv = P.vect_from(x, mold=vm)
+    for alpha, beta in ((1.0, 0.0), (-2.0, 0.5)):
+        off, on = _both(m, xv, vm, alpha, beta, y0)
+        assert np.array_equal(off.view(np.uint8), on.view(np.uint8))
+    m.free()
+
+
+@pytest.mark.parametrize("fmt", ["HLG", "ELG"])
+@pytest.mark.parametrize("method", ["CG", "BICGSTAB"])
+def test_index16_fused_solvers_bitexact(gpu, orc, fmt, method):
+    """The fused solvers' products read the same offsets: iteration counts,
+    histories and solutions equal those over ja bit for bit."""
+    _say("case index16 fused", fmt, method)
+    coo = orc.gen_poisson3d(20) if method == "CG" else orc.gen_convdiff(20)
+    mold = P.psb_d_hip_hlg_sparse_mat(64) if fmt == "HLG" else P.psb_d_hip_elg_sparse_mat()
+    a = _mat(coo, mold)
+    prec = P.psb_dprec_type("DIAG").build(a, P.psb_cdall(nl=coo.nr))
+    b = orc.uniform_pm1(6, coo.nr)
+    res = []
+    for on in (0, 1):
+        old = N.fn("psb_hip_hll_set_index16")(on)
+        try:
+            x = P.vect_from(np.zeros(coo.nr))
+            it, err, hist = P.psb_krylov_fused(method, a, prec, P.vect_from(b), x, 1e-9,
+                                               history=True)
+            res.append((it, err, hist, x.get_vect()))
+        finally:
+            N.fn("psb_hip_hll_set_index16")(old)
+    assert res[0][0] == res[1][0] and res[0][1] == res[1][1] and res[0][2] == res[1][2]
+    assert np.array_equal(res[0][3], res[1][3])
+    a.free()
