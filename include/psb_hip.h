@@ -409,9 +409,13 @@ int psb_hip_ell_index16(void *dmat);
 int psb_hip_hll_set_index16(int on);
 /* XCD block order of the HLL (variants 0 and + 4), ELL, multi-RHS HLL and
    row-block CSR products: logical blocks dealt to the XCDs in chunks of
-   `blocks` (default 16; 0: one contiguous range per XCD, for the row-block
-   CSR kernel launch order).  No counterpart in the reference.  Returns the
-   previous value. */
+   `blocks` (0: one contiguous range per XCD, for the row-block CSR kernel
+   launch order).  -1, the default: per matrix, a chunk of about an eighth
+   of the matrix's bandwidth (max |column - row|, recorded when an HLL or
+   ELL matrix is written) when that is 4 to 64 blocks, so an XCD's
+   successive chunks are about one band apart (the z-neighbours of a 3-D
+   stencil); 16 otherwise and for CSR.  No counterpart in the reference.
+   Returns the previous value. */
 int psb_hip_hll_set_xcd_chunk(int blocks);
 /* Autotuner (no counterpart in the reference): times y = A*x with each HLL
    kernel variant (1, 2, 5, 6 above; reps launches each, HIP events), stores

@@ -73,10 +73,18 @@ struct HllView {
     const int last = R == 1 ? nr - 1 : ((nr - 1) & ~1);
     const int r = row0 < last ? row0 : last;
     const int stride = HK > 0 ? HK : hksz;
-    const int hk = r / stride;
+    int hk = r / stride;
+    // one row per lane and hack size = wavefront size: the wavefront is one
+    // hack (tiles are whole hacks), so its offsets come through scalar loads
+    if (R == 1 && HK == kWave) hk = __builtin_amdgcn_readfirstlane(hk);
     const int lane = r - hk * stride;
-    const int off0 = hkoffs[hk];
-    const int w = (hkoffs[hk + 1] - off0) / stride;
+    int off0, w;
+    if constexpr (R == 2) {
+      hll_pair_hack<HK>(hkoffs, nr, hk, stride, off0, w);
+    } else {
+      off0 = hkoffs[hk];
+      w = (hkoffs[hk + 1] - off0) / stride;
+    }
     if (res16) {
       const short* pe = res16 + off0 + lane;
       const int* pb = base16 + off0 / stride;

@@ -379,6 +379,7 @@ struct HllDev {
   // products read these 2-byte residuals instead of the 4-byte ja.
   short* res16 = nullptr;   // allocsize
   int* base16 = nullptr;    // allocsize / hksz
+  int band = 0;             // max |column - row| over the stored slots (rows in order)
   // row-sorted matrices with scattered columns and a large x: the same rows
   // restricted to consecutive column ranges, each a row-sorted HLL matrix of
   // its own; the product runs them one after another accumulating into y
@@ -403,7 +404,11 @@ struct EllDev {
   // (rows past `rows` are never read and hold 0)
   short* res16 = nullptr;   // pitch * nzm
   int* base16 = nullptr;    // ceil(rows/64) * nzm
+  int band = 0;             // max |column - row| over the stored slots of rows < rows
 };
+// XCD chunk (blocks) of a product over blocks of rows_per_block rows of a
+// matrix of bandwidth band (spmv.hip)
+int xcd_chunk_for(int band, int rows_per_block);
 // whether the products of an ELL matrix read its 16-bit column offsets
 bool ell_use_idx16(const EllDev* m);
 
@@ -498,6 +503,27 @@ __device__ __forceinline__ T slot_dot(const int* __restrict__ pj, const TV* __re
 
 template <typename T>
 using vec2 = T __attribute__((ext_vector_type(2)));
+
+// Offset and width of hack hk for a product with two adjacent rows per lane.
+// With the hack size equal to the wavefront size a wavefront covers two whole
+// hacks (the first lane's and the next; rows past nr clamp into the last
+// hack), so the three offsets come through scalar loads.
+template <int HK>
+__device__ __forceinline__ void hll_pair_hack(const int* __restrict__ hkoffs, int nr, int hk,
+                                              int stride, int& off0, int& w) {
+  if (HK == kWave) {
+    const int hka = __builtin_amdgcn_readfirstlane(hk);
+    const int noffs = (nr + HK - 1) / HK;
+    const int oa = hkoffs[hka], ob = hkoffs[hka + 1];
+    const int oc = hka + 2 <= noffs ? hkoffs[hka + 2] : ob;
+    const bool second = hk != hka;
+    off0 = second ? ob : oa;
+    w = ((second ? oc : ob) - off0) / stride;
+  } else {
+    off0 = hkoffs[hk];
+    w = (hkoffs[hk + 1] - off0) / stride;
+  }
+}
 
 // The same groups over 16-bit column offsets (HllDev::res16 / base16): pe
 // and pv address the row's slot 0, pb the hack's slot-column bases, r is the

@@ -27,13 +27,15 @@
 namespace psbhip {
 
 static int g_hll_variant = 0;
-// XCD order: chunks of 16 blocks dealt to the XCDs in turn (0: one contiguous
-// range per XCD).  On the 7-point stencils at L = 200 and 256 chunks of 4-64
-// blocks measured 1-2% (FP64 pairs) and 2-5% (FP32) faster than launch order
-// for HLL, contiguous ranges 3-5% slower; for ELL the chunks beat contiguous
-// ranges by 9% (FP64) and 6% (FP32), for the row-block CSR kernel launch
-// order by 1-2%.
-static int g_hll_xchunk = 16;
+// XCD order: chunks of blocks dealt to the XCDs in turn (0: one contiguous
+// range per XCD; -1, the default: per matrix, xcd_chunk_for, and 16 blocks
+// for the row-block CSR kernel).  On the 7-point stencils at L = 200 and 256
+// chunks of 4-64 blocks measured 1-2% (FP64 pairs) and 2-5% (FP32) faster
+// than launch order for HLL, contiguous ranges 3-5% slower; for ELL the
+// chunks beat contiguous ranges by 9% (FP64) and 6% (FP32), for the
+// row-block CSR kernel launch order by 1-2%.
+static int g_hll_xchunk = -1;
+constexpr int kXchunkDefault = 16;
 // 0 auto (the row-block kernel, on the column slices when the matrix has
 // them), 1 uniform lanes per row, 2 length-binned, 3 row-block on the whole
 // matrix, 4 = 0
@@ -90,7 +92,10 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmv(int nr, int hksz, int nlog,
   const int row = lb * kBlock + threadIdx.x;
   const int r = row < nr ? row : nr - 1;
   const int stride = HK > 0 ? HK : hksz;
-  const int hk = r / stride;
+  int hk = r / stride;
+  // hack size = wavefront size: the wavefront is one hack (rows past nr
+  // clamp into the last one), so its offsets come through scalar loads
+  if (HK == kWave) hk = __builtin_amdgcn_readfirstlane(hk);
   const int lane = r - hk * stride;
   const int off0 = hkoffs[hk];
   const int w = (hkoffs[hk + 1] - off0) / stride;
@@ -124,8 +129,8 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmv_pair(int nr, int hksz, int 
   const int stride = HK > 0 ? HK : hksz;
   const int hk = r / stride;
   const int lane = r - hk * stride;
-  const int off0 = hkoffs[hk];
-  const int w = (hkoffs[hk + 1] - off0) / stride;
+  int off0, w;
+  hll_pair_hack<HK>(hkoffs, nr, hk, stride, off0, w);
   T a0, a1;
   if (IDX16)
     slot_dot16_pair<T, TV>(res16 + off0 + lane, base16 + off0 / stride, val + off0 + lane, stride,
@@ -286,6 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_ell_idx16(int nr, int pitch, int nzm
                                                       const int* __restrict__ ja,
                                                       short* __restrict__ res,
                                                       int* __restrict__ base, int* bad) {
+  // bad[0]: some residual does not fit; bad[1]: max |column - row| (band)
   const int g = blockIdx.x * kBlock + threadIdx.x;
   if (g * 64 >= nr) return;
   const int r0 = g * 64, r1 = r0 + 64 < nr ? r0 + 64 : nr;
@@ -297,6 +303,10 @@ __global__ __launch_bounds__(kBlock) void k_ell_idx16(int nr, int pitch, int nzm
       const int d = ja[q + r] - 1 - r;
       lo = d < lo ? d : lo;
       hi = d > hi ? d : hi;
+    }
+    {
+      const int64_t bw = -(int64_t)lo > (int64_t)hi ? -(int64_t)lo : (int64_t)hi;
+      atomicMax(bad + 1, bw > 0x7fffffff ? 0x7fffffff : (int)bw);
     }
     const int b = (int)(((int64_t)lo + (int64_t)hi + 1) >> 1);
     bool fits = true;
@@ -588,6 +598,7 @@ __global__ __launch_bounds__(kBlock) void k_hll_idx16(int hksz, int noffs,
                                                       const int* __restrict__ ja,
                                                       short* __restrict__ res,
                                                       int* __restrict__ base, int* bad) {
+  // bad[0]: some residual does not fit; bad[1]: max |column - row| (band)
   const int h = blockIdx.x * kBlock + threadIdx.x;
   if (h >= noffs) return;
   const int off0 = hkoffs[h];
@@ -600,6 +611,10 @@ __global__ __launch_bounds__(kBlock) void k_hll_idx16(int hksz, int noffs,
       const int d = ja[q + l] - 1 - (r0 + l);
       lo = d < lo ? d : lo;
       hi = d > hi ? d : hi;
+    }
+    {
+      const int64_t bw = -(int64_t)lo > (int64_t)hi ? -(int64_t)lo : (int64_t)hi;
+      atomicMax(bad + 1, bw > 0x7fffffff ? 0x7fffffff : (int)bw);
     }
     // ceiling midpoint: residuals in [-32768, 32767] for a span up to 65535
     const int b = (int)(((int64_t)lo + (int64_t)hi + 1) >> 1);
@@ -790,6 +805,22 @@ static int check_vectors(const MultiVec* x, const MultiVec* y, int elem_type, in
 // (psb_hip_hll_set_index16 switches them off for comparisons)
 static int g_hll_idx16 = 1;
 bool hll_use_idx16(const HllDev* m) { return g_hll_idx16 && m->res16 && !m->perm; }
+
+// Chunked XCD order whose chunk spans about an eighth of the band: the chunks
+// an XCD takes in turn (c, c + 8, ...) are then about one band apart, so for a
+// 3-D stencil (band L^2) the chunk before holds the z-1 neighbours of this
+// one's rows in the same L2.  L=200 stencil: FP64 pairs (10 blocks) 116.5 us,
+// FP32 and SpMM k=8 (20 blocks) 63.9 and 479.8 us, against 116.5-119.5, 64.6
+// and 490.9-514.6 us with 16-block chunks.  Bands under 32 blocks (chunks
+// below 4) and non-banded matrices keep kXchunkDefault; an explicit
+// psb_hip_hll_set_xcd_chunk overrides.
+int xcd_chunk_for(int band, int rows_per_block) {
+  if (g_hll_xchunk >= 0) return g_hll_xchunk;
+  const int64_t per = (int64_t)rows_per_block * kXcds;
+  const int64_t c = ((int64_t)band + per / 2) / per;
+  if (band <= 0 || c < 4 || c > 64) return kXchunkDefault;
+  return (int)c;
+}
 bool ell_use_idx16(const EllDev* m) { return g_hll_idx16 && m->res16; }
 
 static void ell_drop_idx16(EllDev* m) {
@@ -797,6 +828,7 @@ static void ell_drop_idx16(EllDev* m) {
   hipFree(m->base16);
   m->res16 = nullptr;
   m->base16 = nullptr;
+  m->band = 0;
 }
 
 // Build the 16-bit column offsets of an ELL matrix (kept only when every
@@ -809,22 +841,23 @@ static int ell_build_idx16(EllDev* m) {
   int* bad = nullptr;
   hipError_t e = hipMalloc((void**)&m->res16, sizeof(short) * (size_t)m->pitch * m->nzm);
   if (e == hipSuccess) e = hipMalloc((void**)&m->base16, sizeof(int) * (size_t)ng * m->nzm);
-  if (e == hipSuccess) e = hipMalloc((void**)&bad, sizeof(int));
+  if (e == hipSuccess) e = hipMalloc((void**)&bad, 2 * sizeof(int));
   if (e != hipSuccess) {
     (void)hipGetLastError();
     hipFree(bad);
     ell_drop_idx16(m);
     return PSB_HIP_SUCCESS;
   }
-  int hbad = 0;
-  PSB_HIP_CHECK(hipMemsetAsync(bad, 0, sizeof(int), s));
+  int hbad[2] = {0, 0};
+  PSB_HIP_CHECK(hipMemsetAsync(bad, 0, 2 * sizeof(int), s));
   hipLaunchKernelGGL(k_ell_idx16, dim3((unsigned)grid_for(ng, kBlock)), dim3(kBlock), 0, s, m->rows,
                      m->pitch, m->nzm, m->ja, m->res16, m->base16, bad);
   PSB_HIP_LAUNCH_CHECK();
-  PSB_HIP_CHECK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(hbad, bad, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
   PSB_HIP_CHECK(hipStreamSynchronize(s));
   hipFree(bad);
-  if (hbad) ell_drop_idx16(m);
+  if (hbad[0]) ell_drop_idx16(m);
+  m->band = hbad[1];
   return PSB_HIP_SUCCESS;
 }
 
@@ -837,7 +870,7 @@ static BlockOrder hll_order(const HllDev* m, bool xcd, int rows_per_block, int n
   BlockOrder o;
   if (xcd) {
     o.mode = 1;
-    o.xchunk = g_hll_xchunk;
+    o.xchunk = xcd_chunk_for(m->band, rows_per_block);
   }
   return o;
 }
@@ -1015,7 +1048,7 @@ static int csr_launch_rowblock(CsrDev* m, const T* x, T alpha, T beta, T* y) {
   hipStream_t s = ctx().stream;
   const T* val = (const T*)m->val;
   if (m->nsblk > 0) {
-    const int xc = g_hll_xchunk;
+    const int xc = g_hll_xchunk >= 0 ? g_hll_xchunk : kXchunkDefault;
     const int g = xc > 0 ? xcd_chunk_grid(m->nsblk, xc) : m->nsblk;
     hipLaunchKernelGGL((k_csr_rowblock<T>), dim3(g), dim3(kBlock), 0, s, m->nsblk, xc, m->sblk,
                        m->irp, m->ja, val, x, alpha, beta, y);
@@ -1258,6 +1291,7 @@ static void hll_drop_idx16(HllDev* m) {
   hipFree(m->base16);
   m->res16 = nullptr;
   m->base16 = nullptr;
+  m->band = 0;
 }
 
 static void hll_drop_slices(HllDev* m) {
@@ -1292,22 +1326,23 @@ static int hll_build_idx16(HllDev* m) {
   hipError_t e = hipMalloc((void**)&m->res16, sizeof(short) * (size_t)m->allocsize);
   if (e == hipSuccess)
     e = hipMalloc((void**)&m->base16, sizeof(int) * (size_t)(m->allocsize / m->hksz + 1));
-  if (e == hipSuccess) e = hipMalloc((void**)&bad, sizeof(int));
+  if (e == hipSuccess) e = hipMalloc((void**)&bad, 2 * sizeof(int));
   if (e != hipSuccess) {
     (void)hipGetLastError();
     hipFree(bad);
     hll_drop_idx16(m);
     return PSB_HIP_SUCCESS;
   }
-  int hbad = 0;
-  PSB_HIP_CHECK(hipMemsetAsync(bad, 0, sizeof(int), s));
+  int hbad[2] = {0, 0};
+  PSB_HIP_CHECK(hipMemsetAsync(bad, 0, 2 * sizeof(int), s));
   hipLaunchKernelGGL(k_hll_idx16, dim3((unsigned)grid_for(m->noffs, kBlock)), dim3(kBlock), 0, s,
                      m->hksz, m->noffs, m->hkoffs, m->ja, m->res16, m->base16, bad);
   PSB_HIP_LAUNCH_CHECK();
-  PSB_HIP_CHECK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  PSB_HIP_CHECK(hipMemcpyAsync(hbad, bad, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
   PSB_HIP_CHECK(hipStreamSynchronize(s));
   hipFree(bad);
-  if (hbad) hll_drop_idx16(m);
+  if (hbad[0]) hll_drop_idx16(m);
+  m->band = hbad[1];
   return PSB_HIP_SUCCESS;
 }
 
@@ -2086,7 +2121,7 @@ int psb_hip_ell_spmv(void* dmat, double alpha, void* xv, double beta, void* yv) 
   if (m->rows == 0) return PSB_HIP_SUCCESS;
   hipStream_t s = ctx().stream;
   int nlog = grid_for(m->rows, kBlock);
-  const int xc = g_hll_xchunk;
+  const int xc = xcd_chunk_for(m->band, kBlock);
   dim3 grid(xcd_chunk_grid(nlog, xc));
   const bool i16 = ell_use_idx16(m);
   // every column of x/y (cuda/elldev.c:195-215 loops count_)
@@ -2196,7 +2231,7 @@ int psb_hip_csr_get_diag(void* dmat, void* dv) {
 
 int psb_hip_hll_set_xcd_chunk(int blocks) {
   const int old = g_hll_xchunk;
-  if (blocks >= 0) g_hll_xchunk = blocks;
+  if (blocks >= -1) g_hll_xchunk = blocks;
   return old;
 }
 

@@ -373,7 +373,7 @@ def test_xcd_block_order_is_a_permutation(gpu, orc, fmt):
     outs = []
     old = N.fn("psb_hip_hll_set_xcd_chunk")(16)
     try:
-        for chunk in (16, 0, 1, 4, 64):
+        for chunk in (16, 0, 1, 4, 64, -1):
             N.fn("psb_hip_hll_set_xcd_chunk")(chunk)
             yv = P.vect_from(np.zeros(coo.nr))
             P.psb_spmm(1.0, a, P.vect_from(x), 0.0, yv)

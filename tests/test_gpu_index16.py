@@ -210,3 +210,35 @@ def test_index16_fused_solvers_bitexact(gpu, orc, fmt, method):
     assert res[0][0] == res[1][0] and res[0][1] == res[1][1] and res[0][2] == res[1][2]
     assert np.array_equal(res[0][3], res[1][3])
     a.free()
+
+
+@pytest.mark.parametrize("single", [False, True])
+def test_band_chunk_order_bitexact(gpu, orc, single):
+    """The automatic XCD chunk (about an eighth of the bandwidth: L = 128,
+    band 16384, chunks of 4 pair blocks / 8 single blocks) only reorders the
+    blocks: products equal those in 16-block chunks bit for bit."""
+    _say("case band chunk", "fp32" if single else "fp64")
+    coo = orc.gen_convdiff(128)
+    mold = P.psb_s_hip_hlg_sparse_mat(64) if single else P.psb_d_hip_hlg_sparse_mat(64)
+    m = _mat(coo, mold)
+    dt = np.float32 if single else np.float64
+    vm = P.psb_s_hip_vect() if single else P.psb_d_hip_vect()
+    x = orc.uniform_pm1(7, coo.nc).astype(dt)
+    outs = []
+    for chunk in (16, -1):
+        old = N.fn("psb_hip_hll_set_xcd_chunk")(chunk)
+        try:
+            yv = P.vect_from(np.zeros(coo.nr, dt), mold=vm)
+            P.psb_spmm(1.0, m, P.vect_from(x, mold=vm), 0.0, yv)
+            res = [yv.get_vect()]
+            if not single:
+                X = np.stack([orc.uniform_pm1(40 + c, coo.nc) for c in range(4)], axis=1)
+                yk = P.psb_d_hip_vect().all(coo.nr, 4)
+                P.psb_spmm(1.0, m, P.psb_d_hip_vect().bld(X), 0.0, yk)
+                res.append(yk.get_vect())
+            outs.append(res)
+        finally:
+            N.fn("psb_hip_hll_set_xcd_chunk")(old)
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    m.free()
