@@ -915,6 +915,15 @@ def main():
         "batch": batch,
         "serve": serve,
     }
+    if methods and cpu and "BICGSTAB_DIAG" in methods.get("runs", {}):
+        # configuration 1 (pde3d-style convection-diffusion L=40, BiCGSTAB +
+        # DIAG to 1e-9): the device drop-in's iterations beside the oracle's
+        methods["config1_bicgstab_iterations"] = {
+            "device_dropin": methods["runs"]["BICGSTAB_DIAG"]["iterations"],
+            "oracle": cpu["bicgstab_L40"]["iterations"],
+            "note": "the oracle's count moves over 78-92 (pde3d) / 96-118 (this matrix) when b "
+                    "is perturbed by a few ulps; the device count is checked against that "
+                    "spread (tests/test_gpu_krylov.py)"}
     print(json.dumps(line), flush=True)
 
 
