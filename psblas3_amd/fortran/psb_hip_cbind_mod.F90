@@ -122,7 +122,40 @@ module psb_hip_cbind_mod
       integer(c_int) :: res
     end function psb_hip_hll_offsets
 
+    ! host copy of the HLL format (psi_d_convert_hll_from_coo) and its product
+    function psb_hip_host_hll_fill(nr, hksz, irn, hkoffs, idisp, cja, cval, perm, elem_type, &
+         & ja, val, idiag) result(res) bind(c, name='psb_hip_host_hll_fill')
+      import :: c_int, c_ptr
+      integer(c_int), value :: nr, hksz, elem_type
+      integer(c_int) :: irn(*), hkoffs(*), idisp(*), cja(*), ja(*), idiag(*)
+      type(c_ptr), value :: cval, perm, val
+      integer(c_int) :: res
+    end function psb_hip_host_hll_fill
+    function psb_hip_host_hll_spmv(nr, hksz, hkoffs, irn, ja, val, perm, elem_type, alpha, x, &
+         & beta, y) result(res) bind(c, name='psb_hip_host_hll_spmv')
+      import :: c_int, c_ptr, c_double
+      integer(c_int), value :: nr, hksz, elem_type
+      integer(c_int) :: hkoffs(*), irn(*), ja(*)
+      type(c_ptr), value :: val, perm, x, y
+      real(c_double), value :: alpha, beta
+      integer(c_int) :: res
+    end function psb_hip_host_hll_spmv
+
     ! ---------------------------------------------------------------- HLL (HLG)
+    function psb_hip_hll_write(dmat, val, ja, hkoffs, irn, idiag) result(res) &
+         & bind(c, name='psb_hip_hll_write')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: dmat, val
+      integer(c_int) :: ja(*), hkoffs(*), irn(*), idiag(*)
+      integer(c_int) :: res
+    end function psb_hip_hll_write
+    function psb_hip_hll_read(dmat, val, ja, hkoffs, irn, idiag) result(res) &
+         & bind(c, name='psb_hip_hll_read')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: dmat, val
+      integer(c_int) :: ja(*), hkoffs(*), irn(*), idiag(*)
+      integer(c_int) :: res
+    end function psb_hip_hll_read
     function psb_hip_hll_alloc(dmat, hksz, rows, nzeros, allocsize, elem_type, first_index) &
          & result(res) bind(c, name='psb_hip_hll_alloc')
       import :: c_int, c_ptr

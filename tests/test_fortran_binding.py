@@ -1,7 +1,10 @@
 """The Fortran side of the boundary: psb_hip_cbind_mod (iso_c_binding
-interfaces to include/psb_hip.h) compiles with flang, and a Fortran driver
-issuing the spasb(HLG) / spmm / gedot / krylov(CG) sequence through it runs
-on the GPU."""
+interfaces to include/psb_hip.h) and psb_hip_plugin_mod (the dual-memory
+plug-in types psb_d_hip_hlg_sparse_mat and psb_d_hip_vect over it) compile
+with flang; a Fortran driver issuing the spasb(HLG) / spmm / gedot /
+krylov(CG) sequence through the bindings, and one exercising the plug-in
+types' host / sync / dev states, host and device products and CG over the
+vector type, run on the GPU."""
 import os
 import shutil
 import subprocess
@@ -11,11 +14,13 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 FDIR = os.path.join(ROOT, "psblas3_amd", "fortran")
 DRIVER = os.path.join(ROOT, "psblas3_amd", "lib", "fortran", "hip_pde3d_driver")
+PLUGIN_DRIVER = os.path.join(ROOT, "psblas3_amd", "lib", "fortran", "hip_plugin_driver")
 
 
 @pytest.mark.skipif(shutil.which("flang") is None, reason="flang not installed")
 def test_binding_module_compiles(tmp_path):
-    for src in ("psb_hip_cbind_mod.F90", "hip_pde3d_driver.f90"):
+    for src in ("psb_hip_cbind_mod.F90", "psb_hip_plugin_mod.F90", "hip_pde3d_driver.f90",
+                "hip_plugin_driver.f90"):
         subprocess.check_call(["flang", "-fsyntax-only", "-module-dir", str(tmp_path),
                                os.path.join(FDIR, src)], cwd=tmp_path)
 
@@ -26,5 +31,15 @@ def test_fortran_driver_runs():
         pytest.skip("Fortran driver not built (flang missing at build time)")
     print("case fortran driver", flush=True)
     out = subprocess.run([DRIVER], capture_output=True, text=True, timeout=120)
+    print(out.stdout, out.stderr)
+    assert out.returncode == 0 and "PASS" in out.stdout
+
+
+@pytest.mark.gpu
+def test_fortran_plugin_types_run():
+    if not os.path.exists(PLUGIN_DRIVER):
+        pytest.skip("Fortran plug-in driver not built (flang missing at build time)")
+    print("case fortran plug-in driver", flush=True)
+    out = subprocess.run([PLUGIN_DRIVER], capture_output=True, text=True, timeout=120)
     print(out.stdout, out.stderr)
     assert out.returncode == 0 and "PASS" in out.stdout
