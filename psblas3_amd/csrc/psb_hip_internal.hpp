@@ -675,7 +675,10 @@ template <typename T>
 __device__ __forceinline__ void store_y(T* __restrict__ y, int row, T alpha, T acc, T beta) {
   T r = alpha * acc;
   if (beta != T(0)) r = fma(beta, y[row], r);
-  y[row] = r;
+  // y is written once per product: a non-temporal store leaves the L2 to x
+  // (L=200 stencil: FP32 HLL 66.2 -> 63.3 us, ELL FP64 133 -> 130.5 us, FP64
+  // pairs 120.5 -> 119.8 us)
+  __builtin_nontemporal_store(r, y + row);
 }
 
 }  // namespace psbhip
