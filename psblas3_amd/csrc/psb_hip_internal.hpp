@@ -452,6 +452,12 @@ struct TransDev {
   int* cnt;             // entries of each row
   int* src;             // the row i of A of each slot (0 in padding)
   void* val;            // each slot's value (0 in padding)
+  // 16-bit source offsets, as HllDev's (built when every hack slot column's
+  // real entries span at most 65535 rows of A): source row of slot (hack h,
+  // slot column j, lane l) = h*64 + l + base16[hkoffs[h]/64 + j] + res16[slot],
+  // clamped to [0, cols - 1] (padding slots hold 0; their product is dropped)
+  short* res16 = nullptr;
+  int* base16 = nullptr;
 };
 
 // frees the prepared triangular solve of a CSR matrix (trsv.hip)

@@ -543,10 +543,13 @@ __global__ __launch_bounds__(kBlock) void k_ell_scal(int rows, int pitch, int nz
 // (padding: source 0, value 0) are loaded and their update dropped.
 constexpr int kTHack = 64;
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_tmv_run(int rows, const int* __restrict__ hkoffs,
+template <typename T, bool IDX16>
+__global__ __launch_bounds__(kBlock) void k_tmv_run(int rows, int cols,
+                                                    const int* __restrict__ hkoffs,
                                                     const int* __restrict__ cnt,
                                                     const int* __restrict__ src,
+                                                    const short* __restrict__ res16,
+                                                    const int* __restrict__ base16,
                                                     const T* __restrict__ val,
                                                     const T* __restrict__ x, T alpha, T beta,
                                                     T* __restrict__ y) {
@@ -562,9 +565,11 @@ __global__ __launch_bounds__(kBlock) void k_tmv_run(int rows, const int* __restr
   else if (beta == T(1)) r = y[c];
   else if (beta == T(-1)) r = -y[c];
   else r = beta * y[c];
-  const int h = c / kTHack, lane = c - h * kTHack;
-  const int off = hkoffs[h] + lane;
-  const int w = (hkoffs[h + 1] - hkoffs[h]) / kTHack;
+  // a wavefront is one hack (64 columns of A): its offsets through scalar loads
+  const int h = __builtin_amdgcn_readfirstlane(c / kTHack), lane = c - h * kTHack;
+  const int h0 = hkoffs[h];
+  const int off = h0 + lane;
+  const int w = (hkoffs[h + 1] - h0) / kTHack;
   const int n = cnt[c];
   for (int j0 = 0; j0 < w; j0 += 4) {
     int s[4];
@@ -572,7 +577,12 @@ __global__ __launch_bounds__(kBlock) void k_tmv_run(int rows, const int* __restr
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       const int j = j0 + k < w ? j0 + k : w - 1;
-      s[k] = ld_stream(src + off + (size_t)j * kTHack);
+      if (IDX16) {
+        const int sr = c + base16[h0 / kTHack + j] + (int)ld_stream(res16 + off + (size_t)j * kTHack);
+        s[k] = sr < 0 ? 0 : (sr >= cols ? cols - 1 : sr);
+      } else {
+        s[k] = ld_stream(src + off + (size_t)j * kTHack);
+      }
       v[k] = ld_stream(val + off + (size_t)j * kTHack);
     }
 #pragma unroll
@@ -585,7 +595,7 @@ __global__ __launch_bounds__(kBlock) void k_tmv_run(int rows, const int* __restr
       r = j0 + k < n ? t : r;
     }
   }
-  y[c] = r;
+  __builtin_nontemporal_store(r, y + c);
 }
 
 // 16-bit column offsets of a rows-in-order HLL matrix (HllDev::res16 /
@@ -2398,6 +2408,32 @@ int psb_hip_tmat_alloc(void** tmat, int nr, int nc, const int* irp, const int* j
       src[q] = i;
       memcpy(tval.data() + q * es, (const char*)val + (size_t)k * es, es);
     }
+  // 16-bit source offsets over each hack slot column's real entries
+  std::vector<short> res(slots > 0 ? slots : 1, 0);
+  std::vector<int> base(slots / kTHack + 1, 0);
+  bool fits = true;
+  for (int h = 0; h < nh && fits; h++) {
+    const int w = (hk[h + 1] - hk[h]) / kTHack;
+    const int c0 = h * kTHack, c1 = std::min(nc, c0 + kTHack);
+    for (int j = 0; j < w && fits; j++) {
+      const size_t q = (size_t)hk[h] + (size_t)j * kTHack;
+      int64_t lo = INT64_MAX, hi = INT64_MIN;
+      for (int c = c0; c < c1; c++)
+        if (j < cnt[c]) {
+          const int64_t d = (int64_t)src[q + (c - c0)] - c;
+          lo = std::min(lo, d);
+          hi = std::max(hi, d);
+        }
+      if (hi - lo > 65535) {
+        fits = false;
+        break;
+      }
+      const int64_t b = (lo + hi + 1) >> 1;
+      base[(size_t)hk[h] / kTHack + j] = (int)b;
+      for (int c = c0; c < c1; c++)
+        if (j < cnt[c]) res[q + (c - c0)] = (short)((int64_t)src[q + (c - c0)] - c - b);
+    }
+  }
   TransDev* t = new TransDev();
   t->elem_type = elem_type;
   t->rows = nc;
@@ -2414,6 +2450,14 @@ int psb_hip_tmat_alloc(void** tmat, int nr, int nc, const int* irp, const int* j
   if (e == hipSuccess)
     e = hipMemcpy(t->src, src.data(), sizeof(int) * src.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(t->val, tval.data(), tval.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess && fits) {
+    e = hipMalloc(&t->res16, sizeof(short) * res.size());
+    if (e == hipSuccess) e = hipMalloc(&t->base16, sizeof(int) * base.size());
+    if (e == hipSuccess)
+      e = hipMemcpy(t->res16, res.data(), sizeof(short) * res.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(t->base16, base.data(), sizeof(int) * base.size(), hipMemcpyHostToDevice);
+  }
   if (e != hipSuccess) {
     psb_hip_tmat_free(t);
     return set_error(e == hipErrorOutOfMemory ? PSB_HIP_OUTOFMEM : PSB_HIP_RUNTIME, __func__,
@@ -2431,6 +2475,8 @@ void psb_hip_tmat_free(void* tmat) {
   hipFree(t->cnt);
   hipFree(t->src);
   hipFree(t->val);
+  hipFree(t->res16);
+  hipFree(t->base16);
   delete t;
 }
 
@@ -2443,17 +2489,21 @@ int psb_hip_tmat_spmv(void* tmat, double alpha, void* xv, double beta, void* yv)
   if (rc) return set_error(rc, __func__, "x/y type, size or aliasing");
   if (t->rows == 0) return PSB_HIP_SUCCESS;
   dim3 grid(grid_for(t->rows, kBlock));
+  const bool i16 = g_hll_idx16 && t->res16;
   for (int c = 0; c < x->count; c++) {
-    if (t->elem_type == PSB_HIP_TYPE_DOUBLE)
-      hipLaunchKernelGGL((k_tmv_run<double>), grid, dim3(kBlock), 0, ctx().stream, t->rows,
-                         t->hkoffs, t->cnt, t->src, (const double*)t->val,
-                         (const double*)x->v + (size_t)c * x->pitch, alpha, beta,
-                         (double*)y->v + (size_t)c * y->pitch);
-    else
-      hipLaunchKernelGGL((k_tmv_run<float>), grid, dim3(kBlock), 0, ctx().stream, t->rows,
-                         t->hkoffs, t->cnt, t->src, (const float*)t->val,
-                         (const float*)x->v + (size_t)c * x->pitch, (float)alpha, (float)beta,
-                         (float*)y->v + (size_t)c * y->pitch);
+#define PSB_TMV_CASE(T, I16)                                                                     \
+  hipLaunchKernelGGL((k_tmv_run<T, I16>), grid, dim3(kBlock), 0, ctx().stream, t->rows, t->cols, \
+                     t->hkoffs, t->cnt, t->src, t->res16, t->base16, (const T*)t->val,           \
+                     (const T*)x->v + (size_t)c * x->pitch, (T)alpha, (T)beta,                  \
+                     (T*)y->v + (size_t)c * y->pitch)
+    if (t->elem_type == PSB_HIP_TYPE_DOUBLE) {
+      if (i16) PSB_TMV_CASE(double, true);
+      else PSB_TMV_CASE(double, false);
+    } else {
+      if (i16) PSB_TMV_CASE(float, true);
+      else PSB_TMV_CASE(float, false);
+    }
+#undef PSB_TMV_CASE
   }
   PSB_HIP_LAUNCH_CHECK();
   return PSB_HIP_SUCCESS;

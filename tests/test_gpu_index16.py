@@ -242,3 +242,37 @@ def test_band_chunk_order_bitexact(gpu, orc, single):
     for a, b in zip(outs[0], outs[1]):
         assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
     m.free()
+
+
+@pytest.mark.parametrize("name", ["convdiff_L23", "aniso_L77", "powerlaw_20000", "powerlaw_100000"])
+@pytest.mark.parametrize("single", [False, True])
+def test_index16_transposed_bitexact(gpu, orc, name, single):
+    """The transposed product (psb_spmm trans 'T') reads 16-bit source offsets
+    when the transpose's slot columns allow them: the same bits as over the
+    32-bit source rows, and the oracle's column-oriented loop bit for bit."""
+    _say("case index16 transposed", name, "fp32" if single else "fp64")
+    gen, _ = CASES[name]
+    coo = gen(orc)
+    f = orc.fix_coo(coo)
+    mold = P.psb_s_hip_hlg_sparse_mat(64) if single else P.psb_d_hip_hlg_sparse_mat(64)
+    m = _mat(coo, mold)
+    dt = np.float32 if single else np.float64
+    vm = P.psb_s_hip_vect() if single else P.psb_d_hip_vect()
+    x = orc.uniform_pm1(11, coo.nr).astype(dt)
+    y0 = orc.uniform_pm1(12, coo.nc).astype(dt)
+    outs = []
+    for on in (0, 1):
+        old = N.fn("psb_hip_hll_set_index16")(on)
+        try:
+            yv = P.vect_from(y0.copy(), mold=vm)
+            P.psb_spmm(-1.5, m, P.vect_from(x, mold=vm), 0.5, yv, trans="T")
+            outs.append(yv.get_vect())
+        finally:
+            N.fn("psb_hip_hll_set_index16")(old)
+        m.a._drop_t()      # rebuild the transpose under the next setting
+    assert np.array_equal(outs[0].view(np.uint8), outs[1].view(np.uint8))
+    if not single:
+        irp, ja, val = orc.csr_from_coo(f)
+        yref = orc.csr_spmv_t(irp, ja, val, x, -1.5, 0.5, y0.copy(), nc=f.nc)
+        assert np.array_equal(outs[1].view(np.uint8), yref.view(np.uint8))
+    m.free()
