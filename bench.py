@@ -758,7 +758,7 @@ def main():
             m.m = m.n = n
             m.a = mk().cp_from_coo(coo_a)
             single = dt == "f32"
-            t = time_spmv(P, m, xs if single else x, ys if single else y, max(args.steps // 2, 10), 3)
+            t = time_spmv(P, m, xs if single else x, ys if single else y, max(args.steps // 2, 10), 10)
             b = spmv_bytes(n, nnz, single)
             sweep[f"{mat}_{fmt}_{dt}"] = {"us": round(t * 1e3, 2), "GBs": round(b / (t * 1e-3) / 1e9, 1),
                                           "GFLOPs": round(2 * nnz / (t * 1e-3) / 1e9, 2)}
@@ -770,7 +770,7 @@ def main():
         m.a = P.psb_d_hip_csrg_sparse_mat().cp_from_coo(coo_a)
         for var, tag in ((1, "uniform"), (2, "binned"), (3, "rowblock")):
             old = N.fn("psb_hip_csr_set_variant")(var)
-            t = time_spmv(P, m, x, y, max(args.steps // 2, 10), 3)
+            t = time_spmv(P, m, x, y, max(args.steps // 2, 10), 10)
             N.fn("psb_hip_csr_set_variant")(old)
             sweep[f"A_CSR_f64_{tag}"] = {"us": round(t * 1e3, 2),
                                          "GBs": round(spmv_bytes(n, nnz) / (t * 1e-3) / 1e9, 1)}
@@ -789,7 +789,7 @@ def main():
             N.call("psb_hip_mv_fill_uniform", xk.dev, 1, 0)
             xk.set_dev()
             yk = P.psb_d_hip_vect().all(n, k)
-            t = time_spmv(P, head, xk, yk, max(args.steps // 4, 5), 2)
+            t = time_spmv(P, head, xk, yk, max(args.steps // 4, 5), 5)
             b = 12 * nnz + 4 * n + 16 * n * k
             ent = {"us": round(t * 1e3, 2), "GBs": round(b / (t * 1e-3) / 1e9, 1),
                    "GFLOPs": round(2 * nnz * k / (t * 1e-3) / 1e9, 2), "bytes": b,
@@ -839,7 +839,7 @@ def main():
             m.m = m.n = nb
             m.a = mk().cp_from_coo(coo_b)
             single = dt == "f32"
-            t = time_spmv(P, m, xbs if single else xb, ybs if single else yb, max(args.steps // 4, 5), 2)
+            t = time_spmv(P, m, xbs if single else xb, ybs if single else yb, max(args.steps // 4, 5), 5)
             b = spmv_bytes(nb, nnzb, single)
             ent = {"us": round(t * 1e3, 2), "GBs": round(b / (t * 1e-3) / 1e9, 1),
                    "GFLOPs": round(2 * nnzb / (t * 1e-3) / 1e9, 2)}
@@ -854,7 +854,7 @@ def main():
         m.a = P.psb_d_hip_csrg_sparse_mat().cp_from_coo(coo_b)
         for var, tag in ((1, "uniform"), (2, "binned"), (3, "rowblock")):
             old = N.fn("psb_hip_csr_set_variant")(var)
-            t = time_spmv(P, m, xb, yb, max(args.steps // 4, 5), 2)
+            t = time_spmv(P, m, xb, yb, max(args.steps // 4, 5), 5)
             N.fn("psb_hip_csr_set_variant")(old)
             b = spmv_bytes(nb, nnzb)
             sweep[f"B_CSR_f64_{tag}"] = {"us": round(t * 1e3, 2),
