@@ -107,7 +107,11 @@ template <typename T, int HK>
 __device__ __forceinline__ T b_row_hk(const BSys& sy, const T* a, int r,
                                       const T* __restrict__ x) {
   const int stride = HK > 0 ? HK : sy.hksz;
-  const int hk = r / stride;
+  int hk = r / stride;
+  // tiles start on 64-row boundaries and rows past n clamp to n - 1: with the
+  // hack size equal to the wavefront size a wavefront is one hack, so its
+  // offsets come through scalar loads
+  if (HK == kWave) hk = __builtin_amdgcn_readfirstlane(hk);
   const int lane = r - hk * stride;
   const int off0 = sy.hkoffs[hk];
   const int w = (sy.hkoffs[hk + 1] - off0) / stride;
