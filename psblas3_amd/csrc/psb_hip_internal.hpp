@@ -571,6 +571,23 @@ __device__ __forceinline__ T slot_dot16(const short* __restrict__ pe, const int*
   return acc;
 }
 
+// x entries of columns c0 and c1 for the two rows of a lane: one 16-byte
+// load when c1 == c0 + 1 and c0 is even (x 16-byte aligned: xal), else two
+// 8-byte loads.  Banded matrices take the single load on every slot whose
+// column offset is even.
+template <typename T>
+__device__ __forceinline__ void gather_pair(const T* __restrict__ x, bool xal, int c0, int c1,
+                                            T& x0, T& x1) {
+  if (xal && c1 == c0 + 1 && (c0 & 1) == 0) {
+    const vec2<T> t = *reinterpret_cast<const vec2<T>*>(x + c0);
+    x0 = t.x;
+    x1 = t.y;
+  } else {
+    x0 = x[c0];
+    x1 = x[c1];
+  }
+}
+
 // Rows r (even) and r + 1 of one hack over 16-bit offsets: one 4-byte
 // offset-pair load and one 2-element value load per slot.
 template <int G, typename T, typename TV>
@@ -591,11 +608,9 @@ __device__ __forceinline__ void slot_group16_pair(const short* __restrict__ pe,
     v[k] = ld_stream(reinterpret_cast<const vec2<TV>*>(pv + (size_t)jj * stride));
   }
   T x0[G], x1[G];
+  const bool xal = (reinterpret_cast<uintptr_t>(x) & (2 * sizeof(T) - 1)) == 0;
 #pragma unroll
-  for (int k = 0; k < G; k++) {
-    x0[k] = x[c0[k]];
-    x1[k] = x[c1[k]];
-  }
+  for (int k = 0; k < G; k++) gather_pair(x, xal, c0[k], c1[k], x0[k], x1[k]);
 #pragma unroll
   for (int k = 0; k < G; k++) {
     const T t0 = fma((T)v[k].x, x0[k], a0);
@@ -635,11 +650,9 @@ __device__ __forceinline__ void slot_group_pair(const int* __restrict__ pj,
     v[k] = ld_stream(reinterpret_cast<const vec2<TV>*>(pv + (size_t)jj * stride));
   }
   T x0[G], x1[G];
+  const bool xal = (reinterpret_cast<uintptr_t>(x) & (2 * sizeof(T) - 1)) == 0;
 #pragma unroll
-  for (int k = 0; k < G; k++) {
-    x0[k] = x[c[k].x - 1];
-    x1[k] = x[c[k].y - 1];
-  }
+  for (int k = 0; k < G; k++) gather_pair(x, xal, c[k].x - 1, c[k].y - 1, x0[k], x1[k]);
 #pragma unroll
   for (int k = 0; k < G; k++) {
     const T t0 = fma((T)v[k].x, x0[k], a0);

@@ -214,10 +214,12 @@ __global__ __launch_bounds__(kBlock) void k_hll_units_combine(int nr, int hksz, 
 
 // Multi-RHS HLL product (psb_d_hll_csmm, ext/impl/psb_d_hll_csmm.f90:33-235):
 // one row per lane and up to NC columns of x/y (column-major, ldx/ldy
-// elements apart) per launch.  Slots go in groups of 4 whose (ja, val) loads
-// issue together (slots past the row's width read its last slot, product
-// dropped, as in slot_group); each slot is loaded once and used for every
-// column, each column accumulating in slot order.
+// elements apart) per launch.  Slots go in groups of S (S * NC = 32 x
+// gathers per group, S <= 4) whose (ja, val) loads issue together, one group
+// ahead of its use; every x gather of a group issues before its first fma.
+// Slots past the row's width read its last slot, product dropped, as in
+// slot_group; each slot is loaded once and used for every column, each
+// column accumulating in slot order.
 template <typename T, int NC, bool IDX16>
 __global__ __launch_bounds__(kBlock) void k_hll_spmm(int nr, int hksz, int nlog,
                                                      const int* __restrict__ hkoffs,
@@ -228,6 +230,7 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmm(int nr, int hksz, int nlog,
                                                      const T* __restrict__ x, int ldx,
                                                      T alpha, T beta, T* __restrict__ y,
                                                      int ldy, int ncols, BlockOrder ord) {
+  constexpr int S = NC >= 32 ? 1 : NC >= 16 ? 2 : NC >= 8 ? 4 : 4;
   const int lb = order_block(ord, nlog);
   if (lb >= nlog) return;
   const int row = lb * kBlock + threadIdx.x;
@@ -240,14 +243,9 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmm(int nr, int hksz, int nlog,
   const short* pe = res16 + off0 + lane;
   const int* pb = base16 + off0 / hksz;
   const T* pv = val + off0 + lane;
-  T acc[NC];
+  auto load_group = [&](int j, int (&c)[S], T (&v)[S]) {
 #pragma unroll
-  for (int q = 0; q < NC; q++) acc[q] = T(0);
-  for (int j = 0; j < w; j += 4) {
-    int c[4];
-    T v[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < S; k++) {
       const int jj = j + k < w ? j + k : w - 1;
       if (IDX16)
         c[k] = r + pb[jj] + (int)ld_stream(pe + (size_t)jj * hksz);
@@ -255,19 +253,41 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmm(int nr, int hksz, int nlog,
         c[k] = ld_stream(pj + (size_t)jj * hksz) - 1;
       v[k] = ld_stream(pv + (size_t)jj * hksz);
     }
+  };
+  T acc[NC];
+#pragma unroll
+  for (int q = 0; q < NC; q++) acc[q] = T(0);
+  int c[S];
+  T v[S];
+  if (w > 0) load_group(0, c, v);
+  for (int j = 0; j < w; j += S) {
+    int cn[S];
+    T vn[S];
+#pragma unroll
+    for (int k = 0; k < S; k++) {
+      cn[k] = c[k];
+      vn[k] = v[k];
+    }
+    if (j + S < w) load_group(j + S, cn, vn);
+    T xv[NC][S];
 #pragma unroll
     for (int q = 0; q < NC; q++) {
-      if (q < ncols) {
-        const T* xq = x + (size_t)q * ldx;
-        T xv[4];
+      const T* xq = x + (size_t)q * ldx;
 #pragma unroll
-        for (int k = 0; k < 4; k++) xv[k] = xq[c[k]];
+      for (int k = 0; k < S; k++) xv[q][k] = q < ncols ? xq[c[k]] : T(0);
+    }
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-          const T t = fma(v[k], xv[k], acc[q]);
-          acc[q] = j + k < w ? t : acc[q];
-        }
+    for (int q = 0; q < NC; q++) {
+#pragma unroll
+      for (int k = 0; k < S; k++) {
+        const T t = fma(v[k], xv[q][k], acc[q]);
+        acc[q] = j + k < w ? t : acc[q];
       }
+    }
+#pragma unroll
+    for (int k = 0; k < S; k++) {
+      c[k] = cn[k];
+      v[k] = vn[k];
     }
   }
   if (row < nr) {
