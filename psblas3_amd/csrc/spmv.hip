@@ -27,6 +27,8 @@
 namespace psbhip {
 
 static int g_hll_variant = 0;
+// multi-RHS HLL products over two rows per lane (k_hll_spmm_pair)
+static bool g_spmm_pair = true;
 // XCD order: chunks of blocks dealt to the XCDs in turn (0: one contiguous
 // range per XCD; -1, the default: per matrix, xcd_chunk_for, and 16 blocks
 // for the row-block CSR kernel).  On the 7-point stencils at L = 200 and 256
@@ -294,6 +296,122 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmm(int nr, int hksz, int nlog,
 #pragma unroll
     for (int q = 0; q < NC; q++)
       if (q < ncols) store_y(y + (size_t)q * ldy, row, alpha, acc[q], beta);
+  }
+}
+
+// The multi-RHS product over two adjacent rows per lane (rows 2t and 2t+1
+// of a 2*kBlock-row tile, same hack since hksz is even), up to NC <= 8
+// columns per launch: one offset-pair and one value-pair load per slot feed
+// both rows, and each column's x entries of the two rows come in one 16-byte
+// load where their columns are adjacent and even (gather_pair).  Per row and
+// column the sum runs in slot order as in k_hll_spmm: the same bits.
+template <typename T, int NC, bool IDX16>
+__global__ __launch_bounds__(kBlock) void k_hll_spmm_pair(int nr, int hksz, int nlog,
+                                                          const int* __restrict__ hkoffs,
+                                                          const int* __restrict__ ja,
+                                                          const short* __restrict__ res16,
+                                                          const int* __restrict__ base16,
+                                                          const T* __restrict__ val,
+                                                          const T* __restrict__ x, int ldx,
+                                                          T alpha, T beta, T* __restrict__ y,
+                                                          int ldy, int ncols, BlockOrder ord) {
+  static_assert(NC <= 8, "two rows per lane: up to 8 columns");
+  constexpr int S = NC >= 8 ? 2 : 4;
+  const int lb = order_block(ord, nlog);
+  if (lb >= nlog) return;
+  const int row = lb * (2 * kBlock) + 2 * (int)threadIdx.x;
+  const int last = (nr - 1) & ~1;
+  const int r = row < last ? row : last;
+  const int hk = r / hksz;
+  const int lane = r - hk * hksz;
+  const int off0 = hkoffs[hk];
+  const int w = (hkoffs[hk + 1] - off0) / hksz;
+  const int* pj = ja + off0 + lane;
+  const short* pe = res16 + off0 + lane;
+  const int* pb = base16 + off0 / hksz;
+  const T* pv = val + off0 + lane;
+  const bool xal = (reinterpret_cast<uintptr_t>(x) & (2 * sizeof(T) - 1)) == 0 && (ldx & 1) == 0;
+  auto load_group = [&](int j, int (&c0)[S], int (&c1)[S], vec2<T> (&v)[S]) {
+#pragma unroll
+    for (int k = 0; k < S; k++) {
+      const int jj = j + k < w ? j + k : w - 1;
+      if (IDX16) {
+        const int b = r + pb[jj];
+        const vec2<short> e =
+            ld_stream(reinterpret_cast<const vec2<short>*>(pe + (size_t)jj * hksz));
+        c0[k] = b + (int)e.x;
+        c1[k] = b + 1 + (int)e.y;
+      } else {
+        const vec2<int> c = ld_stream(reinterpret_cast<const vec2<int>*>(pj + (size_t)jj * hksz));
+        c0[k] = c.x - 1;
+        c1[k] = c.y - 1;
+      }
+      v[k] = ld_stream(reinterpret_cast<const vec2<T>*>(pv + (size_t)jj * hksz));
+    }
+  };
+  T a0[NC], a1[NC];
+#pragma unroll
+  for (int q = 0; q < NC; q++) a0[q] = a1[q] = T(0);
+  int c0[S], c1[S];
+  vec2<T> v[S];
+  if (w > 0) load_group(0, c0, c1, v);
+  for (int j = 0; j < w; j += S) {
+    int n0[S], n1[S];
+    vec2<T> nv[S];
+#pragma unroll
+    for (int k = 0; k < S; k++) {
+      n0[k] = c0[k];
+      n1[k] = c1[k];
+      nv[k] = v[k];
+    }
+    if (j + S < w) load_group(j + S, n0, n1, nv);
+    T x0[NC][S], x1[NC][S];
+#pragma unroll
+    for (int q = 0; q < NC; q++) {
+      const T* xq = x + (size_t)q * ldx;
+#pragma unroll
+      for (int k = 0; k < S; k++) {
+        if (q < ncols) gather_pair(xq, xal, c0[k], c1[k], x0[q][k], x1[q][k]);
+        else x0[q][k] = x1[q][k] = T(0);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NC; q++) {
+#pragma unroll
+      for (int k = 0; k < S; k++) {
+        const T t0 = fma(v[k].x, x0[q][k], a0[q]);
+        const T t1 = fma(v[k].y, x1[q][k], a1[q]);
+        a0[q] = j + k < w ? t0 : a0[q];
+        a1[q] = j + k < w ? t1 : a1[q];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < S; k++) {
+      c0[k] = n0[k];
+      c1[k] = n1[k];
+      v[k] = nv[k];
+    }
+  }
+  const bool yal = (reinterpret_cast<uintptr_t>(y) & (2 * sizeof(T) - 1)) == 0 && (ldy & 1) == 0;
+#pragma unroll
+  for (int q = 0; q < NC; q++) {
+    if (q >= ncols) continue;
+    T* yq = y + (size_t)q * ldy;
+    if (row + 1 < nr && yal) {
+      T r0 = alpha * a0[q], r1 = alpha * a1[q];
+      if (beta != T(0)) {
+        const vec2<T> yo = *reinterpret_cast<const vec2<T>*>(yq + row);
+        r0 = fma(beta, yo.x, r0);
+        r1 = fma(beta, yo.y, r1);
+      }
+      vec2<T> yn;
+      yn.x = r0;
+      yn.y = r1;
+      __builtin_nontemporal_store(yn, reinterpret_cast<vec2<T>*>(yq + row));
+    } else {
+      if (row < nr) store_y(yq, row, alpha, a0[q], beta);
+      if (row + 1 < nr) store_y(yq, row + 1, alpha, a1[q], beta);
+    }
   }
 }
 
@@ -1017,6 +1135,38 @@ static int hll_spmm_launch(const HllDev* m, const T* val, const T* x, int ldx, T
     return PSB_HIP_SUCCESS;
   }
   hipStream_t s = ctx().stream;
+  // two rows per lane up to 8 columns (L=200 stencil: k = 4 270 -> 229 us,
+  // k = 8 476 -> 433 us); from 16 columns on its 2 x NC accumulators cut the
+  // occupancy to 3 waves per SIMD and one row per lane is faster (k = 16 862
+  // against 1004 us, k = 32 1592 against 2002 us)
+  if (g_spmm_pair && m->hksz % 2 == 0 && ncols > 1 && ncols <= 8) {
+    const int nlog = grid_for(m->rows, 2 * kBlock);
+    const BlockOrder o = hll_order(m, true, 2 * kBlock, nlog);
+    const bool i16 = hll_use_idx16(m);
+    dim3 grid(order_grid(o, nlog)), blk(kBlock);
+    {
+      const int nc = ncols;
+      const T* xc = x;
+      T* yc = y;
+#define PSB_SPMMP_CASE(NC)                                                                    \
+  do {                                                                                        \
+    if (i16)                                                                                  \
+      hipLaunchKernelGGL((k_hll_spmm_pair<T, NC, true>), grid, blk, 0, s, m->rows, m->hksz,   \
+                         nlog, m->hkoffs, m->ja, m->res16, m->base16, val, xc, ldx, alpha,    \
+                         beta, yc, ldy, nc, o);                                               \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_hll_spmm_pair<T, NC, false>), grid, blk, 0, s, m->rows, m->hksz,  \
+                         nlog, m->hkoffs, m->ja, m->res16, m->base16, val, xc, ldx, alpha,    \
+                         beta, yc, ldy, nc, o);                                               \
+  } while (0)
+      if (nc > 4) PSB_SPMMP_CASE(8);
+      else if (nc > 2) PSB_SPMMP_CASE(4);
+      else PSB_SPMMP_CASE(2);
+#undef PSB_SPMMP_CASE
+      PSB_HIP_LAUNCH_CHECK();
+    }
+    return PSB_HIP_SUCCESS;
+  }
   // XCD block order, in chunks of g_hll_xchunk blocks: one contiguous range
   // per XCD measured 2-6% faster than launch order for k = 8, 16 and 32
   // columns on the L=200 stencil, chunks of 16 blocks another 2-5% (485.7 vs

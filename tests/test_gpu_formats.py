@@ -177,6 +177,36 @@ def test_spmv_multicolumn(gpu, orc, k, kind):
     a.free()
 
 
+@pytest.mark.parametrize("k", [2, 3, 5, 8, 9, 16, 33])
+@pytest.mark.parametrize("hk,kind,idx16", [(64, "stencil", 1), (64, "stencil", 0),
+                                           (32, "stencil", 1), (64, "powerlaw", 1)])
+def test_spmm_columns_match_spmv(gpu, orc, k, hk, kind, idx16):
+    """Every column of the multi-RHS product (two rows per lane up to 8
+    columns, one row per lane beyond) equals the single-vector product of that
+    column bit for bit: both sum each row's slots in slot order from zero and
+    apply alpha and beta the same way.  n is odd (the last lane of the pair
+    kernel has one row)."""
+    from psblas3_amd import _native as N
+    _say(f"case spmm columns vs spmv k={k} hk={hk} {kind} idx16={idx16}")
+    coo = orc.gen_convdiff(13) if kind == "stencil" else orc.gen_powerlaw(3001, cap=90, seed=5)
+    assert coo.nr % 2 == 1
+    old = N.fn("psb_hip_hll_set_index16")(idx16)
+    try:
+        a = _mat(coo, P.psb_d_hip_hlg_sparse_mat(hk))
+        X = np.stack([orc.uniform_pm1(40 + s, coo.nc) for s in range(k)], axis=1)
+        Y0 = np.stack([orc.uniform_pm1(80 + s, coo.nr) for s in range(k)], axis=1)
+        yv = P.vect_from(Y0)
+        P.psb_spmm(2.0, a, P.vect_from(X), -1.0, yv)
+        Y = yv.get_vect()
+        for col in range(k):
+            y1 = P.vect_from(Y0[:, col].copy())
+            P.psb_spmm(2.0, a, P.vect_from(X[:, col].copy()), -1.0, y1)
+            assert np.array_equal(Y[:, col], y1.get_vect()), col
+        a.free()
+    finally:
+        N.fn("psb_hip_hll_set_index16")(old)
+
+
 def test_hll_variants_agree(gpu, orc):
     _say("case hll variants")
     from psblas3_amd import _native as N
