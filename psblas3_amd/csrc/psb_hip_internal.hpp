@@ -572,13 +572,16 @@ __device__ __forceinline__ T slot_dot16(const short* __restrict__ pe, const int*
 }
 
 // x entries of columns c0 and c1 for the two rows of a lane: one 16-byte
-// load when c1 == c0 + 1 and c0 is even (x 16-byte aligned: xal), else two
-// 8-byte loads.  Banded matrices take the single load on every slot whose
-// column offset is even.
+// load when, on every active lane of the wavefront, c1 == c0 + 1 and c0 is
+// even (x 16-byte aligned: xal), else two 8-byte loads.  Banded matrices
+// take the single load on every slot whose column offset is even.  The
+// choice is per wavefront (a per-lane choice runs both paths in wavefronts
+// where only some lanes qualify).  Used over 16-bit offsets, i.e. on banded
+// matrices.
 template <typename T>
 __device__ __forceinline__ void gather_pair(const T* __restrict__ x, bool xal, int c0, int c1,
                                             T& x0, T& x1) {
-  if (xal && c1 == c0 + 1 && (c0 & 1) == 0) {
+  if (__all(xal && c1 == c0 + 1 && (c0 & 1) == 0)) {
     const vec2<T> t = *reinterpret_cast<const vec2<T>*>(x + c0);
     x0 = t.x;
     x1 = t.y;
@@ -649,10 +652,15 @@ __device__ __forceinline__ void slot_group_pair(const int* __restrict__ pj,
     c[k] = ld_stream(reinterpret_cast<const vec2<int>*>(pj + (size_t)jj * stride));
     v[k] = ld_stream(reinterpret_cast<const vec2<TV>*>(pv + (size_t)jj * stride));
   }
+  // plain gathers: the 32-bit index path serves matrices with scattered
+  // columns, where gather_pair's test rarely passes and only costs time
+  // (rows-in-order HLL of a power-law matrix 8.9 -> 11.1 ms with it)
   T x0[G], x1[G];
-  const bool xal = (reinterpret_cast<uintptr_t>(x) & (2 * sizeof(T) - 1)) == 0;
 #pragma unroll
-  for (int k = 0; k < G; k++) gather_pair(x, xal, c[k].x - 1, c[k].y - 1, x0[k], x1[k]);
+  for (int k = 0; k < G; k++) {
+    x0[k] = x[c[k].x - 1];
+    x1[k] = x[c[k].y - 1];
+  }
 #pragma unroll
   for (int k = 0; k < G; k++) {
     const T t0 = fma((T)v[k].x, x0[k], a0);

@@ -302,8 +302,9 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmm(int nr, int hksz, int nlog,
 // The multi-RHS product over two adjacent rows per lane (rows 2t and 2t+1
 // of a 2*kBlock-row tile, same hack since hksz is even), up to NC <= 8
 // columns per launch: one offset-pair and one value-pair load per slot feed
-// both rows, and each column's x entries of the two rows come in one 16-byte
-// load where their columns are adjacent and even (gather_pair).  Per row and
+// both rows, and over 16-bit offsets (banded matrices) each column's x
+// entries of the two rows come in one 16-byte load where their columns are
+// adjacent and even (gather_pair).  Per row and
 // column the sum runs in slot order as in k_hll_spmm: the same bits.
 template <typename T, int NC, bool IDX16>
 __global__ __launch_bounds__(kBlock) void k_hll_spmm_pair(int nr, int hksz, int nlog,
@@ -371,8 +372,14 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmm_pair(int nr, int hksz, int 
       const T* xq = x + (size_t)q * ldx;
 #pragma unroll
       for (int k = 0; k < S; k++) {
-        if (q < ncols) gather_pair(xq, xal, c0[k], c1[k], x0[q][k], x1[q][k]);
-        else x0[q][k] = x1[q][k] = T(0);
+        if (q >= ncols) {
+          x0[q][k] = x1[q][k] = T(0);
+        } else if (IDX16) {
+          gather_pair(xq, xal, c0[k], c1[k], x0[q][k], x1[q][k]);
+        } else {
+          x0[q][k] = xq[c0[k]];
+          x1[q][k] = xq[c1[k]];
+        }
       }
     }
 #pragma unroll
