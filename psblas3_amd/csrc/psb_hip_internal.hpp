@@ -470,10 +470,13 @@ void release_csr_sv(CsrDev* m);
 // the group is unconditional and all of them issue before the first gather.
 // (Masking each load instead makes the compiler wait for each index load in
 // turn.)  Requires j < w.
+// With xd / fd given (r the row, 0-based), the x entry gathered at column r
+// -- the row's diagonal -- is kept in *xd and *fd set, for kernels that need
+// x(r) beside the row's product (the fused CG's p.q) without reading x again.
 template <int G, typename T, typename TV>
 __device__ __forceinline__ T slot_group(const int* __restrict__ pj, const TV* __restrict__ pv,
                                         int stride, int j, int w, const T* __restrict__ x,
-                                        T acc) {
+                                        T acc, int r = -1, T* xd = nullptr, int* fd = nullptr) {
   int c[G];
   TV v[G];
 #pragma unroll
@@ -489,6 +492,10 @@ __device__ __forceinline__ T slot_group(const int* __restrict__ pj, const TV* __
   for (int k = 0; k < G; k++) {
     const T t = fma((T)v[k], xv[k], acc);
     acc = j + k < w ? t : acc;
+    if (xd && c[k] - 1 == r) {
+      *xd = xv[k];
+      *fd = 1;
+    }
   }
   return acc;
 }
@@ -497,13 +504,14 @@ __device__ __forceinline__ T slot_group(const int* __restrict__ pj, const TV* __
 // remainder in one group of 8 or 4.
 template <typename T, typename TV>
 __device__ __forceinline__ T slot_dot(const int* __restrict__ pj, const TV* __restrict__ pv,
-                                      int stride, int w, const T* __restrict__ x) {
+                                      int stride, int w, const T* __restrict__ x, int r = -1,
+                                      T* xd = nullptr, int* fd = nullptr) {
   T acc = T(0);
   int j = 0;
-  for (; j + 8 <= w; j += 8) acc = slot_group<8, T, TV>(pj, pv, stride, j, w, x, acc);
+  for (; j + 8 <= w; j += 8) acc = slot_group<8, T, TV>(pj, pv, stride, j, w, x, acc, r, xd, fd);
   const int rem = w - j;
-  if (rem > 4) acc = slot_group<8, T, TV>(pj, pv, stride, j, w, x, acc);
-  else if (rem > 0) acc = slot_group<4, T, TV>(pj, pv, stride, j, w, x, acc);
+  if (rem > 4) acc = slot_group<8, T, TV>(pj, pv, stride, j, w, x, acc, r, xd, fd);
+  else if (rem > 0) acc = slot_group<4, T, TV>(pj, pv, stride, j, w, x, acc, r, xd, fd);
   return acc;
 }
 
@@ -538,7 +546,8 @@ template <int G, typename T, typename TV>
 __device__ __forceinline__ T slot_group16(const short* __restrict__ pe,
                                           const int* __restrict__ pb,
                                           const TV* __restrict__ pv, int stride, int j, int w,
-                                          int r, const T* __restrict__ x, T acc) {
+                                          int r, const T* __restrict__ x, T acc,
+                                          T* xd = nullptr, int* fd = nullptr) {
   int c[G];
   TV v[G];
 #pragma unroll
@@ -554,6 +563,10 @@ __device__ __forceinline__ T slot_group16(const short* __restrict__ pe,
   for (int k = 0; k < G; k++) {
     const T t = fma((T)v[k], xv[k], acc);
     acc = j + k < w ? t : acc;
+    if (xd && c[k] == r) {
+      *xd = xv[k];
+      *fd = 1;
+    }
   }
   return acc;
 }
@@ -561,13 +574,15 @@ __device__ __forceinline__ T slot_group16(const short* __restrict__ pe,
 template <typename T, typename TV>
 __device__ __forceinline__ T slot_dot16(const short* __restrict__ pe, const int* __restrict__ pb,
                                         const TV* __restrict__ pv, int stride, int w, int r,
-                                        const T* __restrict__ x) {
+                                        const T* __restrict__ x, T* xd = nullptr,
+                                        int* fd = nullptr) {
   T acc = T(0);
   int j = 0;
-  for (; j + 8 <= w; j += 8) acc = slot_group16<8, T, TV>(pe, pb, pv, stride, j, w, r, x, acc);
+  for (; j + 8 <= w; j += 8)
+    acc = slot_group16<8, T, TV>(pe, pb, pv, stride, j, w, r, x, acc, xd, fd);
   const int rem = w - j;
-  if (rem > 4) acc = slot_group16<8, T, TV>(pe, pb, pv, stride, j, w, r, x, acc);
-  else if (rem > 0) acc = slot_group16<4, T, TV>(pe, pb, pv, stride, j, w, r, x, acc);
+  if (rem > 4) acc = slot_group16<8, T, TV>(pe, pb, pv, stride, j, w, r, x, acc, xd, fd);
+  else if (rem > 0) acc = slot_group16<4, T, TV>(pe, pb, pv, stride, j, w, r, x, acc, xd, fd);
   return acc;
 }
 
@@ -592,13 +607,14 @@ __device__ __forceinline__ void gather_pair(const T* __restrict__ x, bool xal, i
 }
 
 // Rows r (even) and r + 1 of one hack over 16-bit offsets: one 4-byte
-// offset-pair load and one 2-element value load per slot.
+// offset-pair load and one 2-element value load per slot.  xd / fd as in
+// slot_group, two entries (rows r and r + 1).
 template <int G, typename T, typename TV>
 __device__ __forceinline__ void slot_group16_pair(const short* __restrict__ pe,
                                                   const int* __restrict__ pb,
                                                   const TV* __restrict__ pv, int stride, int j,
                                                   int w, int r, const T* __restrict__ x, T& a0,
-                                                  T& a1) {
+                                                  T& a1, T* xd = nullptr, int* fd = nullptr) {
   int c0[G], c1[G];
   vec2<TV> v[G];
 #pragma unroll
@@ -620,6 +636,16 @@ __device__ __forceinline__ void slot_group16_pair(const short* __restrict__ pe,
     const T t1 = fma((T)v[k].y, x1[k], a1);
     a0 = j + k < w ? t0 : a0;
     a1 = j + k < w ? t1 : a1;
+    if (xd) {
+      if (c0[k] == r) {
+        xd[0] = x0[k];
+        fd[0] = 1;
+      }
+      if (c1[k] == r + 1) {
+        xd[1] = x1[k];
+        fd[1] = 1;
+      }
+    }
   }
 }
 
@@ -627,14 +653,16 @@ template <typename T, typename TV>
 __device__ __forceinline__ void slot_dot16_pair(const short* __restrict__ pe,
                                                 const int* __restrict__ pb,
                                                 const TV* __restrict__ pv, int stride, int w,
-                                                int r, const T* __restrict__ x, T& a0, T& a1) {
+                                                int r, const T* __restrict__ x, T& a0, T& a1,
+                                                T* xd = nullptr, int* fd = nullptr) {
   a0 = T(0);
   a1 = T(0);
   int j = 0;
-  for (; j + 8 <= w; j += 8) slot_group16_pair<8, T, TV>(pe, pb, pv, stride, j, w, r, x, a0, a1);
+  for (; j + 8 <= w; j += 8)
+    slot_group16_pair<8, T, TV>(pe, pb, pv, stride, j, w, r, x, a0, a1, xd, fd);
   const int rem = w - j;
-  if (rem > 4) slot_group16_pair<8, T, TV>(pe, pb, pv, stride, j, w, r, x, a0, a1);
-  else if (rem > 0) slot_group16_pair<4, T, TV>(pe, pb, pv, stride, j, w, r, x, a0, a1);
+  if (rem > 4) slot_group16_pair<8, T, TV>(pe, pb, pv, stride, j, w, r, x, a0, a1, xd, fd);
+  else if (rem > 0) slot_group16_pair<4, T, TV>(pe, pb, pv, stride, j, w, r, x, a0, a1, xd, fd);
 }
 
 // Two adjacent rows of one hack (same width w): one 8-byte index load and one
@@ -643,7 +671,8 @@ __device__ __forceinline__ void slot_dot16_pair(const short* __restrict__ pe,
 template <int G, typename T, typename TV>
 __device__ __forceinline__ void slot_group_pair(const int* __restrict__ pj,
                                                 const TV* __restrict__ pv, int stride, int j,
-                                                int w, const T* __restrict__ x, T& a0, T& a1) {
+                                                int w, const T* __restrict__ x, T& a0, T& a1,
+                                                int r = -1, T* xd = nullptr, int* fd = nullptr) {
   vec2<int> c[G];
   vec2<TV> v[G];
 #pragma unroll
@@ -667,20 +696,31 @@ __device__ __forceinline__ void slot_group_pair(const int* __restrict__ pj,
     const T t1 = fma((T)v[k].y, x1[k], a1);
     a0 = j + k < w ? t0 : a0;
     a1 = j + k < w ? t1 : a1;
+    if (xd) {
+      if (c[k].x - 1 == r) {
+        xd[0] = x0[k];
+        fd[0] = 1;
+      }
+      if (c[k].y - 1 == r + 1) {
+        xd[1] = x1[k];
+        fd[1] = 1;
+      }
+    }
   }
 }
 
 template <typename T, typename TV>
 __device__ __forceinline__ void slot_dot_pair(const int* __restrict__ pj,
                                               const TV* __restrict__ pv, int stride, int w,
-                                              const T* __restrict__ x, T& a0, T& a1) {
+                                              const T* __restrict__ x, T& a0, T& a1, int r = -1,
+                                              T* xd = nullptr, int* fd = nullptr) {
   a0 = T(0);
   a1 = T(0);
   int j = 0;
-  for (; j + 8 <= w; j += 8) slot_group_pair<8, T, TV>(pj, pv, stride, j, w, x, a0, a1);
+  for (; j + 8 <= w; j += 8) slot_group_pair<8, T, TV>(pj, pv, stride, j, w, x, a0, a1, r, xd, fd);
   const int rem = w - j;
-  if (rem > 4) slot_group_pair<8, T, TV>(pj, pv, stride, j, w, x, a0, a1);
-  else if (rem > 0) slot_group_pair<4, T, TV>(pj, pv, stride, j, w, x, a0, a1);
+  if (rem > 4) slot_group_pair<8, T, TV>(pj, pv, stride, j, w, x, a0, a1, r, xd, fd);
+  else if (rem > 0) slot_group_pair<4, T, TV>(pj, pv, stride, j, w, x, a0, a1, r, xd, fd);
 }
 
 // 16-byte vectors of T for the streaming vector kernels
