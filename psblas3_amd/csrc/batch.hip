@@ -129,8 +129,10 @@ __device__ __forceinline__ T b_row(const BSys& sy, const T* a, int r, const T* _
 // The body runs for every tile of a system satisfying COND (st, sy, r0 =
 // the tile's first row, t_ = tile index in scope).  COND is uniform over the
 // block.  A tile is kTileRows rows: elementwise kernels give each of their
-// kBT threads 4 consecutive rows (16-byte accesses), product kernels run
-// kTileRows threads, one row each.  The descriptor loads are kept short: the
+// kBT threads 4 consecutive rows (16-byte accesses), product kernels give
+// each thread 4 rows kBT apart (a wavefront walks 4 whole hacks; 1024-thread
+// blocks of one row per thread measured 6% slower on the 256-system CG
+// batch).  The descriptor loads are kept short: the
 // block's first tile entry is read together with the tile count (the table
 // has at least gridDim.x entries), a tile's state and system descriptor
 // together, and the next tile's entry before the body runs.
@@ -148,30 +150,6 @@ __device__ __forceinline__ T b_row(const BSys& sy, const T* a, int r, const T* _
       __VA_ARGS__                                                        \
     }                                                                    \
   }
-
-// Product kernels run one row per thread of a tile.
-constexpr int kBTP = kTileRows;
-
-__device__ __forceinline__ void tile_partials_p(double a, double b, double* partials, int t) {
-  __shared__ double lds[2][kBTP / kWave];
-  const double wa = wave_sum(a), wb = wave_sum(b);
-  const int wid = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    lds[0][wid] = wa;
-    lds[1][wid] = wb;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double sa = 0.0, sb = 0.0;
-    for (int k = 0; k < kBTP / kWave; k++) {
-      sa += lds[0][k];
-      sb += lds[1][k];
-    }
-    partials[2 * t] = sa;
-    partials[2 * t + 1] = sb;
-  }
-  __syncthreads();   // the next tile reuses lds
-}
 
 __device__ __forceinline__ void tile_partials(double a, double b, double* partials, int t) {
   __shared__ double lds[2][kBT / kWave];
@@ -251,17 +229,28 @@ _Pragma("unroll")
   })
 }
 
-__global__ __launch_bounds__(kBTP) void kb_cg_q(BTables tb) {
+// Product kernels: 4 rows per thread (rows r0 + k*kBT + thread, k = 0..3,
+// each wavefront walking 4 whole hacks of the tile), so a block's descriptor
+// loads and reduction serve the whole tile with kBT threads and 4 times as
+// many blocks fit a CU as with one row per thread.
+__global__ __launch_bounds__(kBT) void kb_cg_q(BTables tb) {
   PSB_BATCH_TILES(tb, b_iterating(st), {
-    const int i = r0 + (int)threadIdx.x;
-    const bool in = i < sy.n;
-    const float ax = b_row<float>(sy, sy.a32, in ? i : sy.n - 1, sy.p);
-    double pq = 0.0;
-    if (in) {
-      sy.q[i] = ax;
-      pq = (double)sy.p[i] * (double)ax;
+    float ax[4];
+_Pragma("unroll")
+    for (int k = 0; k < 4; k++) {
+      const int i = r0 + k * kBT + (int)threadIdx.x;
+      ax[k] = b_row<float>(sy, sy.a32, i < sy.n ? i : sy.n - 1, sy.p);
     }
-    tile_partials_p(pq, 0.0, tb.partials, t_);
+    double pq = 0.0;
+_Pragma("unroll")
+    for (int k = 0; k < 4; k++) {
+      const int i = r0 + k * kBT + (int)threadIdx.x;
+      if (i < sy.n) {
+        sy.q[i] = ax[k];
+        pq += (double)sy.p[i] * (double)ax[k];
+      }
+    }
+    tile_partials(pq, 0.0, tb.partials, t_);
   })
 }
 
@@ -323,17 +312,24 @@ _Pragma("unroll")
   })
 }
 
-__global__ __launch_bounds__(kBTP) void kb_bi_v(BTables tb) {
+__global__ __launch_bounds__(kBT) void kb_bi_v(BTables tb) {
   PSB_BATCH_TILES(tb, b_iterating(st), {
-    const int i = r0 + (int)threadIdx.x;
-    const bool in = i < sy.n;
-    const float ax = b_row<float>(sy, sy.a32, in ? i : sy.n - 1, sy.f);
-    double qv = 0.0;
-    if (in) {
-      sy.v[i] = ax;
-      qv = (double)sy.q[i] * (double)ax;
+    float ax[4];
+_Pragma("unroll")
+    for (int k = 0; k < 4; k++) {
+      const int i = r0 + k * kBT + (int)threadIdx.x;
+      ax[k] = b_row<float>(sy, sy.a32, i < sy.n ? i : sy.n - 1, sy.f);
     }
-    tile_partials_p(qv, 0.0, tb.partials, t_);
+    double qv = 0.0;
+_Pragma("unroll")
+    for (int k = 0; k < 4; k++) {
+      const int i = r0 + k * kBT + (int)threadIdx.x;
+      if (i < sy.n) {
+        sy.v[i] = ax[k];
+        qv += (double)sy.q[i] * (double)ax[k];
+      }
+    }
+    tile_partials(qv, 0.0, tb.partials, t_);
   })
 }
 
@@ -360,18 +356,25 @@ _Pragma("unroll")
   })
 }
 
-__global__ __launch_bounds__(kBTP) void kb_bi_t(BTables tb) {
+__global__ __launch_bounds__(kBT) void kb_bi_t(BTables tb) {
   PSB_BATCH_TILES(tb, b_iterating(st), {
-    const int i = r0 + (int)threadIdx.x;
-    const bool in = i < sy.n;
-    const float ax = b_row<float>(sy, sy.a32, in ? i : sy.n - 1, sy.z);
-    double tt = 0.0, ts = 0.0;
-    if (in) {
-      sy.t[i] = ax;
-      tt = (double)ax * (double)ax;
-      ts = (double)ax * (double)sy.s[i];
+    float ax[4];
+_Pragma("unroll")
+    for (int k = 0; k < 4; k++) {
+      const int i = r0 + k * kBT + (int)threadIdx.x;
+      ax[k] = b_row<float>(sy, sy.a32, i < sy.n ? i : sy.n - 1, sy.z);
     }
-    tile_partials_p(tt, ts, tb.partials, t_);
+    double tt = 0.0, ts = 0.0;
+_Pragma("unroll")
+    for (int k = 0; k < 4; k++) {
+      const int i = r0 + k * kBT + (int)threadIdx.x;
+      if (i < sy.n) {
+        sy.t[i] = ax[k];
+        tt += (double)ax[k] * (double)ax[k];
+        ts += (double)ax[k] * (double)sy.s[i];
+      }
+    }
+    tile_partials(tt, ts, tb.partials, t_);
   })
 }
 
@@ -428,20 +431,27 @@ _Pragma("unroll")
 }
 
 // r64 = b - A64 x64 with partials r64.r64 and b.b (FP64)
-__global__ __launch_bounds__(kBTP) void kb_res(BTables tb) {
+__global__ __launch_bounds__(kBT) void kb_res(BTables tb) {
   PSB_BATCH_TILES(tb, b_transiting(st), {
-    const int i = r0 + (int)threadIdx.x;
-    const bool in = i < sy.n;
-    const double ax = b_row<double>(sy, sy.a64, in ? i : sy.n - 1, sy.x);
-    double rr = 0.0, bb = 0.0;
-    if (in) {
-      const double bi = sy.b[i];
-      const double ri = bi - ax;
-      sy.r64[i] = ri;
-      rr = ri * ri;
-      bb = bi * bi;
+    double ax[4];
+_Pragma("unroll")
+    for (int k = 0; k < 4; k++) {
+      const int i = r0 + k * kBT + (int)threadIdx.x;
+      ax[k] = b_row<double>(sy, sy.a64, i < sy.n ? i : sy.n - 1, sy.x);
     }
-    tile_partials_p(rr, bb, tb.partials, t_);
+    double rr = 0.0, bb = 0.0;
+_Pragma("unroll")
+    for (int k = 0; k < 4; k++) {
+      const int i = r0 + k * kBT + (int)threadIdx.x;
+      if (i < sy.n) {
+        const double bi = sy.b[i];
+        const double ri = bi - ax[k];
+        sy.r64[i] = ri;
+        rr += ri * ri;
+        bb += bi * bi;
+      }
+    }
+    tile_partials(rr, bb, tb.partials, t_);
   })
 }
 
@@ -762,21 +772,21 @@ static void arena_give(Batch& B, size_t off, size_t bytes) {
 // The kernels of one step (check_every iterations + the transition pass).
 static void enqueue_step(Batch& B) {
   const BTables tb = tables(B);
-  dim3 g(kBGrid), blk(kBT), blkp(kBTP), gs(B.nslots);
+  dim3 g(kBGrid), blk(kBT), gs(B.nslots);
   hipStream_t s = B.stream;
   for (int k = 0; k < B.check_every; k++) {
     if (B.method == 0) {
       hipLaunchKernelGGL(kb_cg_p, g, blk, 0, s, tb);
-      hipLaunchKernelGGL(kb_cg_q, g, blkp, 0, s, tb);
+      hipLaunchKernelGGL(kb_cg_q, g, blk, 0, s, tb);
       hipLaunchKernelGGL(kb_fin<BF_CG_Q>, gs, blk, 0, s, tb);
       hipLaunchKernelGGL(kb_cg_x, g, blk, 0, s, tb);
       hipLaunchKernelGGL(kb_fin<BF_CG_X>, gs, blk, 0, s, tb);
     } else {
       hipLaunchKernelGGL(kb_bi_a, g, blk, 0, s, tb);
-      hipLaunchKernelGGL(kb_bi_v, g, blkp, 0, s, tb);
+      hipLaunchKernelGGL(kb_bi_v, g, blk, 0, s, tb);
       hipLaunchKernelGGL(kb_fin<BF_BI_V>, gs, blk, 0, s, tb);
       hipLaunchKernelGGL(kb_bi_s, g, blk, 0, s, tb);
-      hipLaunchKernelGGL(kb_bi_t, g, blkp, 0, s, tb);
+      hipLaunchKernelGGL(kb_bi_t, g, blk, 0, s, tb);
       hipLaunchKernelGGL(kb_fin<BF_BI_T>, gs, blk, 0, s, tb);
       hipLaunchKernelGGL(kb_bi_x, g, blk, 0, s, tb);
       if (B.method == 1) hipLaunchKernelGGL(kb_fin<BF_BI_X>, gs, blk, 0, s, tb);
@@ -784,7 +794,7 @@ static void enqueue_step(Batch& B) {
     }
   }
   hipLaunchKernelGGL(kb_accum, g, blk, 0, s, tb, B.method == 1 ? 0 : 1);
-  hipLaunchKernelGGL(kb_res, g, blkp, 0, s, tb);
+  hipLaunchKernelGGL(kb_res, g, blk, 0, s, tb);
   hipLaunchKernelGGL(kb_fin<BF_RES>, gs, blk, 0, s, tb);
   if (B.method == 0) {
     hipLaunchKernelGGL(kb_cg_reset, g, blk, 0, s, tb);
