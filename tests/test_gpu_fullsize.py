@@ -103,6 +103,28 @@ def test_fullsize_stencil_L200(gpu, orc, stencil200):
         mat.free()
 
 
+def test_fullsize_spmm_L200(gpu, orc, stencil200):
+    """Multi-RHS products at the benchmark size, k = 8 (two rows per lane) and
+    k = 12 (one row per lane): every column equals the single-vector product
+    of that column bit for bit (both sum each row in slot order from zero),
+    and the single-vector product is held to the oracle's per-row bound in
+    test_fullsize_stencil_L200."""
+    f, _ = stencil200
+    hll = _dev_matrix(f, P.psb_d_hip_hlg_sparse_mat(64))
+    y1 = P.vect_from(np.zeros(f.nr))
+    for k in (8, 12):
+        _say(f"[fullsize] L=200 HLL64 SpMM k={k}")
+        X = np.stack([orc.uniform_pm1(100 + s, f.nc) for s in range(k)], axis=1)
+        yv = P.vect_from(np.zeros((f.nr, k)))
+        P.psb_spmm(1.0, hll, P.vect_from(X), 0.0, yv)
+        Y = yv.get_vect()
+        for col in range(k):
+            P.psb_spmm(1.0, hll, P.vect_from(X[:, col].copy()), 0.0, y1)
+            assert np.array_equal(Y[:, col], y1.get_vect()), (k, col)
+        del X, Y, yv
+    hll.free()
+
+
 @pytest.fixture(scope="module")
 def powerlaw4m(orc):
     _say("[fullsize] assembling the n=4M power-law matrix")
