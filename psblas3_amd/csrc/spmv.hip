@@ -302,8 +302,8 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmm(int nr, int hksz, int nlog,
 }
 
 // The multi-RHS product over two adjacent rows per lane (rows 2t and 2t+1
-// of a 2*kBlock-row tile, same hack since hksz is even), up to NC <= 8
-// columns per launch: one offset-pair and one value-pair load per slot feed
+// of a 2*kBlock-row tile, same hack since hksz is even), the columns in
+// groups of NC <= 8: one offset-pair and one value-pair load per slot feed
 // both rows, and over 16-bit offsets (banded matrices) each column's x
 // entries of the two rows come in one 16-byte load where their columns are
 // adjacent and even (gather_pair).  Per row and
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmm_pair(int nr, int hksz, int 
                                                           const T* __restrict__ x, int ldx,
                                                           T alpha, T beta, T* __restrict__ y,
                                                           int ldy, int ncols, BlockOrder ord) {
-  static_assert(NC <= 8, "two rows per lane: up to 8 columns");
+  static_assert(NC <= 8, "two rows per lane: up to 8 columns per group");
   constexpr int S = NC >= 8 ? 2 : 4;
   const int lb = order_block(ord, nlog);
   if (lb >= nlog) return;
@@ -352,74 +352,81 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmm_pair(int nr, int hksz, int 
       v[k] = ld_stream(reinterpret_cast<const vec2<T>*>(pv + (size_t)jj * hksz));
     }
   };
-  T a0[NC], a1[NC];
+  const bool yal = (reinterpret_cast<uintptr_t>(y) & (2 * sizeof(T) - 1)) == 0 && (ldy & 1) == 0;
+  // columns in groups of NC, one after another: the group's slot loads come
+  // again from the caches, and the group's pages are the ones in use
+  for (int cg = 0; cg < ncols; cg += NC) {
+    const int nc = ncols - cg < NC ? ncols - cg : NC;
+    const T* xg = x + (size_t)cg * ldx;
+    T* yg = y + (size_t)cg * ldy;
+    T a0[NC], a1[NC];
 #pragma unroll
-  for (int q = 0; q < NC; q++) a0[q] = a1[q] = T(0);
-  int c0[S], c1[S];
-  vec2<T> v[S];
-  if (w > 0) load_group(0, c0, c1, v);
-  for (int j = 0; j < w; j += S) {
-    int n0[S], n1[S];
-    vec2<T> nv[S];
-#pragma unroll
-    for (int k = 0; k < S; k++) {
-      n0[k] = c0[k];
-      n1[k] = c1[k];
-      nv[k] = v[k];
-    }
-    if (j + S < w) load_group(j + S, n0, n1, nv);
-    T x0[NC][S], x1[NC][S];
-#pragma unroll
-    for (int q = 0; q < NC; q++) {
-      const T* xq = x + (size_t)q * ldx;
+    for (int q = 0; q < NC; q++) a0[q] = a1[q] = T(0);
+    int c0[S], c1[S];
+    vec2<T> v[S];
+    if (w > 0) load_group(0, c0, c1, v);
+    for (int j = 0; j < w; j += S) {
+      int n0[S], n1[S];
+      vec2<T> nv[S];
 #pragma unroll
       for (int k = 0; k < S; k++) {
-        if (q >= ncols) {
-          x0[q][k] = x1[q][k] = T(0);
-        } else if (IDX16) {
-          gather_pair(xq, xal, c0[k], c1[k], x0[q][k], x1[q][k]);
-        } else {
-          x0[q][k] = xq[c0[k]];
-          x1[q][k] = xq[c1[k]];
+        n0[k] = c0[k];
+        n1[k] = c1[k];
+        nv[k] = v[k];
+      }
+      if (j + S < w) load_group(j + S, n0, n1, nv);
+      T x0[NC][S], x1[NC][S];
+#pragma unroll
+      for (int q = 0; q < NC; q++) {
+        const T* xq = xg + (size_t)q * ldx;
+#pragma unroll
+        for (int k = 0; k < S; k++) {
+          if (q >= nc) {
+            x0[q][k] = x1[q][k] = T(0);
+          } else if (IDX16) {
+            gather_pair(xq, xal, c0[k], c1[k], x0[q][k], x1[q][k]);
+          } else {
+            x0[q][k] = xq[c0[k]];
+            x1[q][k] = xq[c1[k]];
+          }
         }
       }
+#pragma unroll
+      for (int q = 0; q < NC; q++) {
+#pragma unroll
+        for (int k = 0; k < S; k++) {
+          const T t0 = fma(v[k].x, x0[q][k], a0[q]);
+          const T t1 = fma(v[k].y, x1[q][k], a1[q]);
+          a0[q] = j + k < w ? t0 : a0[q];
+          a1[q] = j + k < w ? t1 : a1[q];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < S; k++) {
+        c0[k] = n0[k];
+        c1[k] = n1[k];
+        v[k] = nv[k];
+      }
     }
 #pragma unroll
     for (int q = 0; q < NC; q++) {
-#pragma unroll
-      for (int k = 0; k < S; k++) {
-        const T t0 = fma(v[k].x, x0[q][k], a0[q]);
-        const T t1 = fma(v[k].y, x1[q][k], a1[q]);
-        a0[q] = j + k < w ? t0 : a0[q];
-        a1[q] = j + k < w ? t1 : a1[q];
+      if (q >= nc) continue;
+      T* yq = yg + (size_t)q * ldy;
+      if (row + 1 < nr && yal) {
+        T r0 = alpha * a0[q], r1 = alpha * a1[q];
+        if (beta != T(0)) {
+          const vec2<T> yo = *reinterpret_cast<const vec2<T>*>(yq + row);
+          r0 = fma(beta, yo.x, r0);
+          r1 = fma(beta, yo.y, r1);
+        }
+        vec2<T> yn;
+        yn.x = r0;
+        yn.y = r1;
+        __builtin_nontemporal_store(yn, reinterpret_cast<vec2<T>*>(yq + row));
+      } else {
+        if (row < nr) store_y(yq, row, alpha, a0[q], beta);
+        if (row + 1 < nr) store_y(yq, row + 1, alpha, a1[q], beta);
       }
-    }
-#pragma unroll
-    for (int k = 0; k < S; k++) {
-      c0[k] = n0[k];
-      c1[k] = n1[k];
-      v[k] = nv[k];
-    }
-  }
-  const bool yal = (reinterpret_cast<uintptr_t>(y) & (2 * sizeof(T) - 1)) == 0 && (ldy & 1) == 0;
-#pragma unroll
-  for (int q = 0; q < NC; q++) {
-    if (q >= ncols) continue;
-    T* yq = y + (size_t)q * ldy;
-    if (row + 1 < nr && yal) {
-      T r0 = alpha * a0[q], r1 = alpha * a1[q];
-      if (beta != T(0)) {
-        const vec2<T> yo = *reinterpret_cast<const vec2<T>*>(yq + row);
-        r0 = fma(beta, yo.x, r0);
-        r1 = fma(beta, yo.y, r1);
-      }
-      vec2<T> yn;
-      yn.x = r0;
-      yn.y = r1;
-      __builtin_nontemporal_store(yn, reinterpret_cast<vec2<T>*>(yq + row));
-    } else {
-      if (row < nr) store_y(yq, row, alpha, a0[q], beta);
-      if (row + 1 < nr) store_y(yq, row + 1, alpha, a1[q], beta);
     }
   }
 }
@@ -1187,11 +1194,12 @@ static int hll_spmm_launch(const HllDev* m, const T* val, const T* x, int ldx, T
     return PSB_HIP_SUCCESS;
   }
   hipStream_t s = ctx().stream;
-  // two rows per lane up to 8 columns (L=200 stencil: k = 4 270 -> 229 us,
-  // k = 8 476 -> 433 us); from 16 columns on its 2 x NC accumulators cut the
-  // occupancy to 3 waves per SIMD and one row per lane is faster (k = 16 862
-  // against 1004 us, k = 32 1592 against 2002 us)
-  if (g_spmm_pair && m->hksz % 2 == 0 && ncols > 1 && ncols <= 8) {
+  // two rows per lane, columns in groups of up to 8 inside the launch, up to
+  // 16 columns (L=200 stencil against one row per lane: k = 4 270 -> 229 us,
+  // k = 8 476 -> 433 us, k = 16 862 -> 812 us; k = 32 1605 against 1595 us,
+  // so wider blocks stay one row per lane; 16 columns per group cut the
+  // occupancy to 3 waves per SIMD: k = 16 1004 us)
+  if (g_spmm_pair && m->hksz % 2 == 0 && ncols > 1 && ncols <= 16) {
     const int nlog = grid_for(m->rows, 2 * kBlock);
     const BlockOrder o = hll_order(m, true, 2 * kBlock, nlog);
     const bool i16 = hll_use_idx16(m);
