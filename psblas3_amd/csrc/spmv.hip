@@ -31,6 +31,8 @@ static int g_hll_variant = 0;
 static bool g_spmm_pair = true;
 // FP64 ELL products over two rows per lane (k_ell_spmv_pair)
 static bool g_ell_pair = true;
+// FP64 transposed products over two rows per lane (k_tmv_pair)
+static bool g_tmv_pair = true;
 // XCD order: chunks of blocks dealt to the XCDs in turn (0: one contiguous
 // range per XCD; -1, the default: per matrix, xcd_chunk_for, and 16 blocks
 // for the row-block CSR kernel).  On the 7-point stencils at L = 200 and 256
@@ -808,6 +810,90 @@ __global__ __launch_bounds__(kBlock) void k_tmv_run(int rows, int cols,
     }
   }
   __builtin_nontemporal_store(r, y + c);
+}
+
+// The transposed product over 16-bit source offsets with two adjacent rows
+// of A^T per lane (rows c and c+1, c even, one hack; a wavefront covers two
+// hacks, their offsets through scalar loads): one offset-pair and one
+// value-pair load per slot, the two rows' x entries in one 16-byte load
+// where their source rows are adjacent and even, one 16-byte store.  Each
+// row runs k_tmv_run's update sequence over its own count (a row past the
+// last has count 0): the same bits.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_tmv_pair(int rows, int cols,
+                                                     const int* __restrict__ hkoffs,
+                                                     const int* __restrict__ cnt,
+                                                     const short* __restrict__ res16,
+                                                     const int* __restrict__ base16,
+                                                     const T* __restrict__ val,
+                                                     const T* __restrict__ x, T alpha, T beta,
+                                                     T* __restrict__ y) {
+#pragma clang fp contract(off)   // separate multiply and add, as the CPU loop
+  const int c = 2 * (int)(blockIdx.x * kBlock + threadIdx.x);
+  if (c >= rows) return;
+  const bool two = c + 1 < rows;
+  if (alpha == T(0)) {
+    y[c] = beta == T(0) ? T(0) : beta * y[c];
+    if (two) y[c + 1] = beta == T(0) ? T(0) : beta * y[c + 1];
+    return;
+  }
+  auto scaled = [&](int i) -> T {
+    if (beta == T(0)) return T(0);
+    if (beta == T(1)) return y[i];
+    if (beta == T(-1)) return -y[i];
+    return beta * y[i];
+  };
+  T r0 = scaled(c), r1 = two ? scaled(c + 1) : T(0);
+  int off0, w;
+  hll_pair_hack<kTHack>(hkoffs, rows, c / kTHack, kTHack, off0, w);
+  const int lane = c % kTHack;
+  const int n0 = cnt[c], n1 = two ? cnt[c + 1] : 0;
+  const short* pe = res16 + off0 + lane;
+  const int* pb = base16 + off0 / kTHack;
+  const T* pv = val + off0 + lane;
+  const bool xal = (reinterpret_cast<uintptr_t>(x) & (2 * sizeof(T) - 1)) == 0;
+  for (int j0 = 0; j0 < w; j0 += 4) {
+    int s0[4], s1[4];
+    vec2<T> v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int j = j0 + k < w ? j0 + k : w - 1;
+      const int b = c + pb[j];
+      const vec2<short> e = ld_stream(reinterpret_cast<const vec2<short>*>(pe + (size_t)j * kTHack));
+      const int a0 = b + (int)e.x, a1 = b + 1 + (int)e.y;
+      s0[k] = a0 < 0 ? 0 : (a0 >= cols ? cols - 1 : a0);
+      s1[k] = a1 < 0 ? 0 : (a1 >= cols ? cols - 1 : a1);
+      v[k] = ld_stream(reinterpret_cast<const vec2<T>*>(pv + (size_t)j * kTHack));
+    }
+    T x0[4], x1[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) gather_pair(x, xal, s0[k], s1[k], x0[k], x1[k]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      T t0, t1;
+      if (alpha == T(1)) {
+        t0 = r0 + v[k].x * x0[k];
+        t1 = r1 + v[k].y * x1[k];
+      } else if (alpha == T(-1)) {
+        t0 = r0 - v[k].x * x0[k];
+        t1 = r1 - v[k].y * x1[k];
+      } else {
+        t0 = r0 + alpha * v[k].x * x0[k];
+        t1 = r1 + alpha * v[k].y * x1[k];
+      }
+      r0 = j0 + k < n0 ? t0 : r0;
+      r1 = j0 + k < n1 ? t1 : r1;
+    }
+  }
+  if (two && (reinterpret_cast<uintptr_t>(y) & (2 * sizeof(T) - 1)) == 0) {
+    vec2<T> yn;
+    yn.x = r0;
+    yn.y = r1;
+    __builtin_nontemporal_store(yn, reinterpret_cast<vec2<T>*>(y + c));
+  } else {
+    __builtin_nontemporal_store(r0, y + c);
+    if (two) __builtin_nontemporal_store(r1, y + c + 1);
+  }
 }
 
 // 16-bit column offsets of a rows-in-order HLL matrix (HllDev::res16 /
@@ -2742,6 +2828,19 @@ int psb_hip_tmat_spmv(void* tmat, double alpha, void* xv, double beta, void* yv)
   if (t->rows == 0) return PSB_HIP_SUCCESS;
   dim3 grid(grid_for(t->rows, kBlock));
   const bool i16 = g_hll_idx16 && t->res16;
+  // FP64 over 16-bit source offsets: two rows per lane (L=200 stencil:
+  // 143-145 -> 127-128 us)
+  const bool pair = g_tmv_pair && i16 && t->elem_type == PSB_HIP_TYPE_DOUBLE;
+  if (pair) {
+    dim3 gp(grid_for((t->rows + 1) / 2, kBlock));
+    for (int c = 0; c < x->count; c++)
+      hipLaunchKernelGGL((k_tmv_pair<double>), gp, dim3(kBlock), 0, ctx().stream, t->rows,
+                         t->cols, t->hkoffs, t->cnt, t->res16, t->base16, (const double*)t->val,
+                         (const double*)x->v + (size_t)c * x->pitch, alpha, beta,
+                         (double*)y->v + (size_t)c * y->pitch);
+    PSB_HIP_LAUNCH_CHECK();
+    return PSB_HIP_SUCCESS;
+  }
   for (int c = 0; c < x->count; c++) {
 #define PSB_TMV_CASE(T, I16)                                                                     \
   hipLaunchKernelGGL((k_tmv_run<T, I16>), grid, dim3(kBlock), 0, ctx().stream, t->rows, t->cols, \
