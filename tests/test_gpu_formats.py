@@ -181,9 +181,9 @@ def test_spmv_multicolumn(gpu, orc, k, kind):
 @pytest.mark.parametrize("hk,kind,idx16", [(64, "stencil", 1), (64, "stencil", 0),
                                            (32, "stencil", 1), (64, "powerlaw", 1)])
 def test_spmm_columns_match_spmv(gpu, orc, k, hk, kind, idx16):
-    """Every column of the multi-RHS product (two rows per lane up to 8
-    columns, one row per lane beyond) equals the single-vector product of that
-    column bit for bit: both sum each row's slots in slot order from zero and
+    """Every column of the multi-RHS product (two rows per lane up to 16
+    columns, in groups of 8; one row per lane beyond) equals the single-vector
+    product of that column bit for bit: both sum each row's slots in slot order from zero and
     apply alpha and beta the same way.  n is odd (the last lane of the pair
     kernel has one row)."""
     from psblas3_amd import _native as N
@@ -440,15 +440,17 @@ def _torture_t_cases():
 @pytest.mark.parametrize("single", [False, True], ids=["d", "s"])
 @pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (-1.0, 1.0), (2.5, -0.5), (0.0, 2.0),
                                         (1.0, -1.0)])
-def test_transposed_product_bitexact(gpu, orc, fmt, single, alpha, beta):
+@pytest.mark.parametrize("ncols", [5200, 5201])
+def test_transposed_product_bitexact(gpu, orc, fmt, single, alpha, beta, ncols):
     """psb_spmm with trans 'T' / 'C': the transpose built on the device from
     the stored entries, each y(c) updated in the CPU loop's order, equal to
     psb_d_csr_csmv's transposed branch bit for bit (rectangular matrix, empty
-    rows and columns)."""
-    _say(f"case transposed product {fmt} single={single} alpha={alpha} beta={beta}")
+    rows and columns; an odd column count leaves the FP64 two-rows-per-lane
+    kernel's last lane one row)."""
+    _say(f"case transposed product {fmt} single={single} alpha={alpha} beta={beta} nc={ncols}")
     base = orc.gen_powerlaw(5000, cap=400, seed=12)
     keep = (base.ja % 53) != 0                       # some empty columns
-    coo = orc.COO(5000, 5200, base.ia[keep], base.ja[keep], base.val[keep])
+    coo = orc.COO(5000, ncols, base.ia[keep], base.ja[keep], base.val[keep])
     f = orc.fix_coo(coo)
     irp, ja, val = orc.csr_from_coo(f)
     dt = np.float32 if single else np.float64
