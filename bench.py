@@ -700,8 +700,9 @@ def main():
                          "lane, 2 row pairs, +4 XCD-contiguous block order")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(the multi-GPU sharding is described in DESIGN.md)"}))
+        print(json.dumps({"error": "the product and solvers run on one GPU per process; "
+                                   "--gpus must be 1 (row-block sharding over several GPUs "
+                                   "is a design, not built)"}))
         sys.exit(2)
 
     import numpy as np
