@@ -437,7 +437,7 @@ def batch_bench(P, N, nsys=256, eps=1e-8, seed=3, use_graph=True, baselines=True
 
 
 def serve_bench(P, N, nmat=64, seed=4, rates=(200.0, 1000.0, 5000.0), count=160, eps=1e-10,
-                arena_mb=768, use_graph=True):
+                arena_mb=768, use_graph=True, low_rate=20.0, low_count=60):
     """Serving loop: nmat registered 3-D 7-point convection-diffusion matrices
     (L uniform in [24, 160], convection magnitude uniform in [0, 10]: host
     uniform streams 0 and 1 of the seed; HLG hksz 64) with Jacobi
@@ -446,7 +446,10 @@ def serve_bench(P, N, nmat=64, seed=4, rates=(200.0, 1000.0, 5000.0), count=160,
     right-hand side, served by mixed-precision BiCGSTAB with restarted
     refinement (continuous batching) and, on the same stream, one at a time by
     FP64 BiCGSTAB; at 1000 req/s also by the batched BiCGSTAB with reliable
-    updates (replacement when the residual drops by 1e-2)."""
+    updates (replacement when the residual drops by 1e-2).  The three rates
+    exceed one GPU's capacity, so their latencies are mostly queue wait; a
+    fourth run of low_count requests at low_rate, under both servers'
+    capacity, gives latencies that are service times."""
     import numpy as np
     u = np.empty(nmat)
     N.fn("psb_hip_fill_uniform_host")(seed, 0, nmat, u.ctypes.data)
@@ -513,6 +516,20 @@ def serve_bench(P, N, nmat=64, seed=4, rates=(200.0, 1000.0, 5000.0), count=160,
             f"{mixed.get('p99_ms')} ms {mixed.get('solves_per_s')} solves/s | FP64 one at a time "
             f"p50 {fp64.get('p50_ms')} ms p99 {fp64.get('p99_ms')} ms {fp64.get('solves_per_s')} "
             f"solves/s")
+    if low_rate:
+        # a rate under both servers' capacity (about 50 / 37 solves/s on these
+        # matrices): latency is service time, not queue position
+        reqs = P.poisson_arrivals(low_rate, low_count, nmat, seed=seed, stream=90)
+        mixed = P.latency_summary(srv.run(reqs, time_limit=90.0))
+        fp64 = P.latency_summary(P.serve_one_at_a_time(systems, reqs, eps=eps, seed=seed,
+                                                       time_limit=90.0))
+        rates_out[str(int(low_rate))] = {"offered_per_s": low_rate,
+                                         "arrival_span_s": round(reqs[-1][0], 4),
+                                         "under_capacity": True,
+                                         "mixed_batched": mixed, "fp64_one_at_a_time": fp64}
+        log(f"serve: {low_rate:.0f} req/s (under capacity): batched p50 {mixed.get('p50_ms')} "
+            f"ms p99 {mixed.get('p99_ms')} ms | FP64 one at a time p50 {fp64.get('p50_ms')} ms "
+            f"p99 {fp64.get('p99_ms')} ms")
     srv.free()
     out["rates"] = rates_out
     for m, _, _ in systems:

@@ -414,7 +414,9 @@ __global__ __launch_bounds__(kBT) void kb_accum(BTables tb, int unit_scale) {
     PSB_QUAD(i, nv)
     if (nv > 0) {
       float x32[4];
-      if (st->phase != BP_INIT) {
+      // a recurrence that ended on a non-finite residual norm leaves its
+      // iterate out (st->rr: the norm after its last iteration)
+      if (st->phase != BP_INIT && isfinite(st->rr)) {
         const double scale = unit_scale ? 1.0 : st->ref;
         double x[4];
         ld4(sy.x + i, nv, x);
@@ -547,22 +549,29 @@ __global__ __launch_bounds__(kBT) void kb_fin(BTables tb) {
   a = block_sum(a, lds[0]);
   b = block_sum(b, lds[1]);
   if (threadIdx.x != 0) return;
+  // A zero or non-finite denominator or scalar is a breakdown: the FP32
+  // recurrence stops before its vectors take the value, the transition
+  // keeps the last finite iterate and restarts from the FP64 residual.  (An
+  // FP32 inner BiCGSTAB on a convection-dominated request at L = 159
+  // overflowed near a breakdown; the exact-zero tests alone let the NaN run
+  // to the iteration limit.)
   if (OP == BF_CG_Q || OP == BF_BI_V) {
     st->sigma = a;
-    if (a == 0.0) {
+    const double alpha = a != 0.0 ? st->rho / a : 0.0;
+    if (a == 0.0 || !isfinite(a) || !isfinite(alpha)) {
       st->itx += 1;
       st->phase = BP_BREAK;
     } else {
-      st->alpha = st->rho / a;
+      st->alpha = alpha;
     }
   } else if (OP == BF_BI_T) {
-    if (a == 0.0) {
+    if (a == 0.0 || !isfinite(a) || !isfinite(b)) {
       st->itx += 1;
       st->phase = BP_BREAK;
       return;
     }
     st->omega = b / a;
-    if (st->omega == 0.0) {
+    if (st->omega == 0.0 || !isfinite(st->omega)) {
       st->itx += 1;
       st->phase = BP_BREAK;
     }
@@ -571,6 +580,10 @@ __global__ __launch_bounds__(kBT) void kb_fin(BTables tb) {
     st->it += 1;
     const double rn = sqrt(a);
     st->rr = rn;
+    if (!isfinite(rn)) {   // the iterate is dropped at the transition (kb_accum)
+      st->phase = BP_BREAK;
+      return;
+    }
     st->rho_old = st->rho;
     if (rn <= st->delta * st->ref || rn <= st->eps * st->bnorm || st->itx >= st->itmax) {
       st->phase = BP_TRANS;
@@ -588,17 +601,22 @@ __global__ __launch_bounds__(kBT) void kb_fin(BTables tb) {
     st->it += 1;
     const double rn = sqrt(a);
     st->rr = rn;
+    if (!isfinite(rn)) {
+      st->phase = BP_BREAK;
+      return;
+    }
     if (rn <= st->inner_tol || st->itx >= st->itmax) {
       st->phase = BP_TRANS;
       return;
     }
     st->rho_old = st->rho;
     st->rho = b;
-    if (st->rho == 0.0) {
+    const double beta = (st->rho / st->rho_old) * (st->alpha / st->omega);
+    if (st->rho == 0.0 || !isfinite(beta)) {
       st->itx += 1;
       st->phase = BP_BREAK;
     } else {
-      st->beta = (st->rho / st->rho_old) * (st->alpha / st->omega);
+      st->beta = beta;
     }
   } else if (OP == BF_BIRU_X) {
     st->itx += 1;
@@ -606,16 +624,21 @@ __global__ __launch_bounds__(kBT) void kb_fin(BTables tb) {
     const double rn = sqrt(a);
     st->rr = rn;
     st->rho_old = st->rho;
+    if (!isfinite(rn)) {
+      st->phase = BP_BREAK;
+      return;
+    }
     if (rn <= st->delta * st->ref || rn <= st->eps * st->bnorm || st->itx >= st->itmax) {
       st->phase = BP_TRANS;   // rho and beta follow from the replaced residual
       return;
     }
     st->rho = b;
-    if (st->rho == 0.0) {
+    const double beta = (st->rho / st->rho_old) * (st->alpha / st->omega);
+    if (st->rho == 0.0 || !isfinite(beta)) {
       st->itx += 1;
       st->phase = BP_BREAK;
     } else {
-      st->beta = (st->rho / st->rho_old) * (st->alpha / st->omega);
+      st->beta = beta;
     }
   } else if (OP == BF_RES) {
     st->rnorm = sqrt(a);

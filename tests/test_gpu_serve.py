@@ -97,3 +97,56 @@ def test_serve_fp64_baseline(gpu, orc):
         assert r["status"] == 1 and r["relres"] <= 1e-10
     s = P.latency_summary(rec)
     assert s["finished"] == 4 and s["p99_ms"] >= s["p50_ms"] > 0
+
+
+def test_restart_refinement_survives_nonfinite_breakdown(gpu):
+    """A request whose FP32 inner BiCGSTAB overflows near a breakdown: matrix
+    25 of the 64-matrix serving set (3-D convection-diffusion, L = 159,
+    convection magnitude from host uniform stream 1 of seed 4, about 6.34)
+    with the right-hand side of device stream 9100026.  With restarted
+    refinement its third inner cycle went non-finite and, when only exact
+    zeros counted as breakdowns, ran to the iteration limit with a NaN
+    residual.  It must converge to 1e-10; its FP64 residual is checked on
+    the device by the engine and recomputed here through the product."""
+    from psblas3_amd import _native as N
+    _say("case serve: restarted refinement through a non-finite FP32 breakdown (L=159)")
+    nmat, seed, k = 64, 4, 25
+    u = np.empty(nmat)
+    N.fn("psb_hip_fill_uniform_host")(seed, 0, nmat, u.ctypes.data)
+    w = np.empty(nmat)
+    N.fn("psb_hip_fill_uniform_host")(seed, 1, nmat, w.ctypes.data)
+    L = int(min(24 + int((u[k] + 1.0) / 2.0 * 137), 160))
+    conv = (w[k] + 1.0) / 2.0 * 10.0
+    assert L == 159 and abs(conv - 6.341) < 1e-3
+    fn = N.fn("psb_hip_gen_pde3d")
+    a3, b3 = np.ones(3), np.full(3, conv / np.sqrt(3.0))
+    nz = fn(L, a3.ctypes.data, b3.ctypes.data, 0.0, 1, 0, None, None, None, None)
+    ia = np.empty(nz, np.int32); ja = np.empty(nz, np.int32); val = np.empty(nz)
+    fn(L, a3.ctypes.data, b3.ctypes.data, 0.0, 1, 0, ia.ctypes.data, ja.ctypes.data,
+       val.ctypes.data, None)
+    coo = P.psb_d_coo_sparse_mat(L ** 3, L ** 3, ia, ja, val)
+    coo.fix()
+    m = P.psb_dspmat_type()
+    m.m = m.n = L ** 3
+    m.a = P.psb_d_hip_hlg_sparse_mat(64).cp_from_coo(coo)
+    desc = P.psb_cdall(nl=L ** 3)
+    prec = P.psb_dprec_type("DIAG").build(m, desc)
+    srv = P.psb_hip_server("BICGSTAB", max_systems=4, arena_bytes=768 << 20, check_every=4,
+                           eps=1e-10, refine="restart")
+    j = srv.register(m, prec)
+    rec = srv.run([(0.0, j, 9100026)], keep_solutions=True, time_limit=120.0)[0]
+    _say(f"  status {rec['status']} iterations {rec['iterations']} outer {rec['outer']} "
+         f"relres {rec['relres']:.3e}")
+    assert rec["status"] == 1 and rec["relres"] <= 1e-10
+    # the residual again, through the device product
+    b = P.psb_d_hip_vect().all(m.m)
+    N.call("psb_hip_mv_fill_uniform", b.dev, seed, 9100026)
+    b.set_dev()
+    bh = b.get_vect()
+    ax = P.vect_from(np.zeros(m.m))
+    P.psb_spmm(1.0, m, P.psb_d_vect_type(rec["x"]), 0.0, ax)
+    r = bh - ax.get_vect()
+    assert np.isfinite(r).all()
+    assert np.linalg.norm(r) <= 1.5e-10 * np.linalg.norm(bh)
+    srv.free()
+    m.free()
