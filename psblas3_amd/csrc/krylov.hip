@@ -299,21 +299,25 @@ __global__ __launch_bounds__(kFin) void k_fin(KState* st, const double* __restri
       st->breakdown = 1;
     }
   } else if (OP == FIN_CG_Q || OP == FIN_BI_V) {
+    // a zero or non-finite denominator or scalar is a breakdown, caught
+    // before x takes it (FP32 inner solves can overflow near a breakdown):
+    // the solve restarts from b - A x
     st->sigma = v[0];
-    if (v[0] == 0.0) {
+    const double alpha = v[0] != 0.0 ? st->rho / v[0] : 0.0;
+    if (v[0] == 0.0 || !isfinite(v[0]) || !isfinite(alpha)) {
       st->itx += 1;  // the reference counted this iteration before breaking
       st->breakdown = 1;
     } else {
-      st->alpha = st->rho / v[0];
+      st->alpha = alpha;
     }
   } else if (OP == FIN_BI_T) {
-    if (v[0] == 0.0) {
+    if (v[0] == 0.0 || !isfinite(v[0]) || !isfinite(v1)) {
       st->itx += 1;
       st->breakdown = 1;
       return;
     }
     st->omega = v1 / v[0];
-    if (st->omega == 0.0) {
+    if (st->omega == 0.0 || !isfinite(st->omega)) {
       st->itx += 1;
       st->breakdown = 1;
     }
@@ -338,11 +342,12 @@ __global__ __launch_bounds__(kFin) void k_fin(KState* st, const double* __restri
     if (!st->done) {
       st->rho_old = st->rho;
       st->rho = v1;
-      if (st->rho == 0.0) {
+      const double beta = (st->rho / st->rho_old) * (st->alpha / st->omega);
+      if (st->rho == 0.0 || !isfinite(beta)) {
         st->itx += 1;  // next iteration counted, then breaks (psb_dcgstab.f90:267-271)
         st->breakdown = 1;
       } else {
-        st->beta = (st->rho / st->rho_old) * (st->alpha / st->omega);
+        st->beta = beta;
       }
     }
   } else if (OP == FIN_RU_X) {
