@@ -43,7 +43,7 @@
 
 namespace psbhip {
 
-constexpr int kBT = kBlock;      // threads per block
+constexpr int kBT = kBlock;      // threads per block (128: serving 12% slower, batch 5%)
 constexpr int kTileRows = 4 * kBT;  // rows per tile
 constexpr int kBGrid = 2048;     // blocks of the tile kernels
 
@@ -153,8 +153,8 @@ __device__ __forceinline__ T b_row(const BSys& sy, const T* a, int r, const T* _
 
 __device__ __forceinline__ void tile_partials(double a, double b, double* partials, int t) {
   __shared__ double lds[2][kBT / kWave];
-  const double sa = block_sum(a, lds[0]);
-  const double sb = block_sum(b, lds[1]);
+  const double sa = block_sum<double, kBT>(a, lds[0]);
+  const double sb = block_sum<double, kBT>(b, lds[1]);
   if (threadIdx.x == 0) {
     partials[2 * t] = sa;
     partials[2 * t + 1] = sb;
@@ -546,8 +546,8 @@ __global__ __launch_bounds__(kBT) void kb_fin(BTables tb) {
     b += tb.partials[2 * (rg.x + k) + 1];
   }
   __shared__ double lds[2][kBT / kWave];
-  a = block_sum(a, lds[0]);
-  b = block_sum(b, lds[1]);
+  a = block_sum<double, kBT>(a, lds[0]);
+  b = block_sum<double, kBT>(b, lds[1]);
   if (threadIdx.x != 0) return;
   // A zero or non-finite denominator or scalar is a breakdown: the FP32
   // recurrence stops before its vectors take the value, the transition

@@ -175,16 +175,17 @@ __device__ __forceinline__ T group_sum(T v) {
   return v;
 }
 
-// Block-wide sum (blockDim.x == kBlock); every thread gets the result.
-template <typename T>
-__device__ __forceinline__ T block_sum(T v, T* lds /* kBlock/kWave */) {
+// Block-wide sum over a block of NT threads (blockDim.x == NT, kBlock unless
+// given); every thread gets the result.
+template <typename T, int NT = kBlock>
+__device__ __forceinline__ T block_sum(T v, T* lds /* NT/kWave */) {
   T w = wave_sum(v);
   int wid = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) lds[wid] = w;
   __syncthreads();
   T r = 0;
   #pragma unroll
-  for (int k = 0; k < kBlock / kWave; k++) r += lds[k];
+  for (int k = 0; k < NT / kWave; k++) r += lds[k];
   __syncthreads();
   return r;
 }
